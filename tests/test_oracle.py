@@ -1,0 +1,254 @@
+"""Pins the CPU oracle (oracle/refcpu.cpp) to krets' own known answers, its
+golden AC vector, closed forms and a dense long-double brute force.
+
+Known-answer values are those asserted by crates/krets-solver/tests/*.rs and
+crates/krets-parser/src/elements/voltage_source.rs; tolerances here are the
+reference's own or tighter where a closed form exists.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, read_circuit
+
+
+def sol(c, x):
+    return {k: x[i] for k, i in c.index.items()}
+
+
+@pytest.mark.parametrize("cost", [0, 1])
+def test_op_farid_najm(oracle, cost):
+    # crates/krets-solver/tests/test_op.rs:25-46
+    c = oracle.Circuit(read_circuit("circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir"))
+    x, _ = c.op(oracle.config(cost=cost))
+    s = sol(c, x)
+    exp = {"V(4)": 1.9888, "V(8)": 1.0, "V(3)": 2.00879, "V(2)": 1.80879, "V(6)": 1.98814, "V(5)": 2.0,
+           "V(1)": 1.88527, "V(7)": 3.98814, "I(V3)": 0.0, "I(V2)": -199.88e-3, "I(V1)": -198.88e-3}
+    for k, v in exp.items():
+        assert abs(s[k] - v) < 1e-3, k
+
+
+def test_op_case_1(oracle):
+    # test_op.rs:48-68 (exact integers: tightened to 1e-12)
+    c = oracle.Circuit("V1 2 1 32\nR1 1 0 2\nR2 2 3 4\nR3 2 0 8\nV2 3 0 20\n")
+    s = sol(c, c.op()[0])
+    for k, v in {"V(1)": -8.0, "V(2)": 24.0, "V(3)": 20.0, "I(V1)": -4.0, "I(V2)": 1.0}.items():
+        assert abs(s[k] - v) < 1e-12, k
+
+
+def test_op_divider_filters_basic(oracle):
+    c = oracle.Circuit(read_circuit("voltage_divider/voltage_divider.cir"))
+    s = sol(c, c.op()[0])  # test_op.rs:70-81
+    assert abs(s["V(out)"] - 2 / 3) < 1e-14 and abs(s["I(V1)"] + 1 / 3000) < 1e-15
+    for f in ("low_pass_filter/low_pass_filter.cir", "high_pass_filter/high_pass_filter.cir"):
+        c = oracle.Circuit(read_circuit(f))  # test_op.rs:83-105
+        s = sol(c, c.op()[0])
+        assert abs(s["V(in)"] - 1) < 1e-12 and abs(s["V(out)"] - 1) < 1e-12
+    c = oracle.Circuit(read_circuit("basic_001/basic_001.cir"))  # test_op.rs:107-118
+    s = sol(c, c.op()[0])
+    assert abs(s["V(1)"] - 3) < 1e-12 and abs(s["V(2)"] - 0.5) < 1e-12 and abs(s["I(V4)"] + 0.5) < 1e-12
+
+
+def test_op_series_ladders(oracle):
+    # test_op.rs:120-129; closed form V(k) = 1 - (k-1)/N, I(V1) = -1/(N*1000)
+    for N in (500, 1000):
+        c = oracle.Circuit(read_circuit(f"resistor_ladder_{N}/resistor_ladder_{N}.cir"))
+        x, _ = c.op()
+        s = sol(c, x)
+        ks = np.arange(1, N + 1)
+        v = np.array([s[f"V({k})"] for k in ks])
+        assert np.max(np.abs(v - (1 - (ks - 1) / N))) < 1e-12
+        assert abs(s["I(V1)"] * N * 1000.0 + 1.0) < 1e-11
+
+
+def test_op_diode(oracle):
+    # test_op.rs:131-140; closed form: scalar Newton on 1 = 1000*I + V, I = Is(e^{V/Vt}-1)
+    c = oracle.Circuit(read_circuit("diode_iv_curve/diode_iv_curve.cir"))
+    x, it = c.op()
+    s = sol(c, x)
+    assert abs(s["V(out)"] - 0.517) < 1e-3
+    vt = 1.380649e-23 * 300.0 / 1.602176634e-19
+    v = 0.5
+    for _ in range(100):
+        f = (1 - v) / 1000 - 1e-12 * (math.exp(v / vt) - 1)
+        df = -1 / 1000 - 1e-12 / vt * math.exp(v / vt)
+        v -= f / df
+    assert abs(s["V(out)"] - v) < 1e-5  # krets' Newton stops at reltol 1e-3
+    assert it >= 2
+
+
+def test_dc_divider_sweep(oracle):
+    # test_dc.rs:25-54 (pins the I(V1) sign convention)
+    c = oracle.Circuit(read_circuit("voltage_divider/voltage_divider.cir"))
+    x = c.dc("V1", 0.0, 1.0, 1.0)
+    assert x.shape[0] == 2
+    s0, s1 = sol(c, x[0]), sol(c, x[1])
+    assert abs(s0["V(in)"]) < 1e-15 and abs(s0["I(V1)"]) < 1e-15
+    assert abs(s1["V(out)"] - 2 / 3) < 1e-14 and abs(s1["I(V1)"] + 1 / 3000) < 1e-16
+    with pytest.raises(oracle.OracleError) as e:
+        c.dc("V9", 0, 1, 1)
+    assert e.value.code == oracle.ELEMENT_NOT_FOUND
+
+
+def test_ac_known_answers(oracle):
+    c = oracle.Circuit(read_circuit("low_pass_filter/low_pass_filter.cir"))
+    x = c.ac([1000.0])[0]  # test_ac.rs:25-46
+    assert abs(x[c.index["V(out)"]] - (2.470452e-02 - 1.55223e-01j)) < 1e-6
+    w = 2 * math.pi * 1000 * 1e3 * 1e-6  # test_ac.rs:76-105 analytic RC
+    assert abs(x[c.index["V(out)"]] - 1 / (1 + 1j * w)) < 1e-14
+    c = oracle.Circuit(read_circuit("high_pass_filter/high_pass_filter.cir"))
+    x = c.ac([1000.0])[0]  # test_ac.rs:48-74
+    assert abs(x[c.index["V(out)"]] - (7.169568e-01 - 4.50477e-01j)) < 1e-6
+    assert abs(x[c.index["I(V1)"]] - (-7.16957e-03 + 4.504772e-03j)) < 1e-8
+
+
+def test_ac_golden_parquet(oracle):
+    # circuits/low_pass_filter/ac/result.parquet: the only full-precision golden vector
+    g = json.load(open(os.path.join(GOLDEN, "low_pass_ac_result.json")))
+    cols = {k: np.array([float(v) for v in vals]) for k, vals in g["columns"].items()}
+    f = oracle.ac_frequencies(oracle.LINEAR, 10, 0.1, 1000.0)
+    assert np.array_equal(f, cols["frequency"])  # generate_frequencies is bit-exact
+    c = oracle.Circuit(read_circuit("low_pass_filter/ac/ac.cir"))
+    x = c.ac(f)
+    for name in ("I(V1)", "V(in)", "V(out)"):
+        v = x[:, c.index[name]]
+        mag = np.sqrt(v.real ** 2 + v.imag ** 2)
+        ph = np.arctan2(v.imag, v.real) * 180.0 / math.pi
+        assert np.max(np.abs(mag - cols[name + "_mag"]) / np.abs(cols[name + "_mag"])) < 1e-13
+        assert np.max(np.abs(ph - cols[name + "_phase_deg"])) < 1e-11
+
+
+def test_ac_frequencies_decade_octave(oracle):
+    f = oracle.ac_frequencies(oracle.DECADE, 10, 1.0, 1e6)
+    assert len(f) == 61 and f[0] == 1.0 and f[-1] == 1e6
+    assert np.all(np.diff(f) > 0)
+    f = oracle.ac_frequencies(oracle.OCTAVE, 1, 1.0, 8.0)
+    assert np.allclose(f, [1, 2, 4, 8])
+    assert len(oracle.ac_frequencies(oracle.LINEAR, 5, 0.0, 10.0)) == 0
+
+
+def test_transient_lowpass_lag(oracle):
+    # test_transient.rs:180-212: the source-time-lag pin (index 42 -> 0.8647)
+    c = oracle.Circuit(read_circuit("low_pass_filter/transient.cir"))
+    x, _ = c.tran(50e-6, 20e-3)
+    vo = x[:, c.index["V(out)"]]
+    assert abs(vo[0]) < 1e-12 and abs(vo[42] - 0.8647) < 1e-3 and abs(vo[-1] - 1.0) < 1e-3
+    # closed form of lagged BE: v_n = v_{n-1} + (u(t_{n-1}) - v_{n-1}) * a, a = h/(RC + h)
+    h, rc = 50e-6, 1e-3
+    a = h / (rc + h)
+    v, ref = 0.0, [0.0]
+    for n in range(1, len(vo)):
+        tp = (n - 1) * h
+        u = 0.0 if tp < 1e-6 else (min((tp - 1e-6) / 1e-6, 1.0))
+        v = (v * rc / h + u) * a
+        ref.append(v)
+    assert np.max(np.abs(vo - np.array(ref))) < 1e-13
+
+
+def test_transient_rl_dual_rc_rectifier(oracle):
+    c = oracle.Circuit(read_circuit("high_pass_filter/transient.cir"))  # test_transient.rs:214-238
+    x, _ = c.tran(10e-6, 2e-3)
+    assert abs(x[-1, c.index["V(out)"]] - 1.0) < 1e-3
+    c = oracle.Circuit(read_circuit("dual_rc_ladder/dual_rc_ladder.cir"))  # test_transient.rs:139-161
+    x, _ = c.tran(50e-6, 50e-3)
+    assert abs(x[-1, c.index["V(out)"]] - 0.989) < 1e-3
+    c = oracle.Circuit(read_circuit("rectifier/rectifier.cir"))  # test_transient.rs:163-178 (smoke)
+    x, it = c.tran(50e-6, 50e-3)
+    assert it.max() < 300 and np.all(np.isfinite(x))
+
+
+def test_sparse_lu_vs_dense_long_double(oracle):
+    c = oracle.Circuit(read_circuit("circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir"))
+    A, b = c.assemble_dc()
+    x = oracle.dense_solve(A, b)
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-13, atol=1e-15)
+    assert np.max(np.abs(x - c.op()[0])) < 1e-14
+    c = oracle.Circuit(read_circuit("high_pass_filter/high_pass_filter.cir"))
+    xop, _ = c.op()
+    A, b = c.assemble_ac(xop, 1234.5)
+    assert np.max(np.abs(oracle.dense_solve(A, b) - c.ac([1234.5])[0])) < 1e-15
+
+
+def test_errors(oracle):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.Circuit("")
+    assert e.value.code == oracle.EMPTY_NETLIST
+    c = oracle.Circuit(read_circuit("diode_iv_curve/diode_iv_curve.cir"))
+    with pytest.raises(oracle.OracleError) as e:
+        c.op(oracle.config(maxiter=1))
+    assert e.value.code == oracle.MAX_ITERATIONS
+    c = oracle.Circuit("I1 1 0 1\nR1 1 0 1\n")  # AC current source lacks (ic,ic): singular
+    with pytest.raises(oracle.OracleError) as e:
+        c.ac([1.0])
+    assert e.value.code == oracle.DECOMPOSITION
+    c = oracle.Circuit("V1 1 0 1\nR1 1 2 1\nC1 2 0 1u G2\n")  # G2 capacitor is singular in OP
+    with pytest.raises(oracle.OracleError) as e:
+        c.op()
+    assert e.value.code == oracle.DECOMPOSITION
+
+
+def test_parser_quirks(oracle):
+    c = oracle.Circuit("V1 1 0 1k\nR1 1 0 1k\n")  # V values take no SI suffix (quirk 7)
+    assert c.element(0)[4] == 1.0 and c.element(1)[4] == 1000.0
+    c = oracle.Circuit("* t\nv1 in 0 dc 0 PULSE (0 5 1u 100u 100u 5u 10u)\nrin in 0 1.5k\n.control\nfoo\n.endc\n")
+    assert c.names == ["I(V1)", "V(in)"] and c.element(1)[4] == 1500.0
+    with pytest.raises(oracle.OracleError):
+        oracle.Circuit("Q1 1 2 3 mod\n")
+
+
+def test_trap_closed_form_rc(oracle):
+    # TRAP on an RC low-pass driven by a step: v_n = v_{n-1} + (2u - 2v_{n-1}) h/(2RC + h)
+    net = "V1 in 0 PULSE(0 1 0 0 0 1 1)\nR1 in out 1k\nC1 out 0 1u\n"
+    c = oracle.Circuit(net)
+    h = 1e-5
+    x, _ = c.tran(h, 1e-3, method=oracle.TRAP)
+    vo = x[:, c.index["V(out)"]]
+    rc = 1e-3
+    v, ref = 0.0, [0.0]
+    u0 = 0.0
+    for n in range(1, len(vo)):
+        v = ((2 * rc / h - 1) * v + u0 + 1.0) / (2 * rc / h + 1)
+        u0 = 1.0
+        ref.append(v)
+    assert np.max(np.abs(vo - ref)) < 1e-14
+    xb, _ = c.tran(h, 1e-3, method=oracle.BE)
+    assert abs(xb[-1, c.index["V(out)"]] - (1 - (1 / (1 + h / rc)) ** 100)) < 1e-14
+
+
+def test_adjoint_vs_fd(oracle):
+    # discrete adjoint vs central finite differences on an RLC ladder
+    net = ("V1 in 0 PULSE(0 1 0 1n 1n 10n 1)\nR1 in m1 20\nL1 m1 n1 2n\nC1 n1 0 0.5p\n"
+           "R2 n1 m2 30\nL2 m2 n2 3n\nC2 n2 0 0.3p\n")
+    c = oracle.Circuit(net)
+    cvec = np.zeros(c.n)
+    cvec[c.index["V(n2)"]] = 1.0
+    for method in (oracle.TRAP, oracle.BE):
+        J, g = c.adjoint(1e-11, 300, cvec, method=method)
+        for e in range(c.n_elements):
+            t, _, _, _, v = c.element(e)
+            if t not in "RCL":
+                continue
+            d = 1e-6 * v
+            c.set_value(e, v + d)
+            Jp, _ = c.adjoint(1e-11, 300, cvec, method=method)
+            c.set_value(e, v - d)
+            Jm, _ = c.adjoint(1e-11, 300, cvec, method=method)
+            c.set_value(e, v)
+            fd = (Jp - Jm) / (2 * d)
+            assert abs(g[e] - fd) <= 1e-6 * abs(fd) + 1e-9 * abs(J) / v, (method, e)
+
+
+def test_lte_rectifier_accuracy(oracle):
+    net = ("V1 in 0 SIN(0 3 1k 0 0 0)\nD1 in out DMOD\nR1 out 0 5k\nC1 out 0 5u\n"
+           ".model DMOD D (is=1e-14 n=1)\n")
+    c = oracle.Circuit(net)
+    cfg = oracle.config(vabstol=1e-9, iabstol=1e-12)
+    t, x, nt, rej = c.tran_lte(5e-3, 1e-6, 1e-14, 5e-5, 1e-3, 1e-6, cfg=cfg)
+    assert abs(t[-1] - 5e-3) < 1e-15 and len(t) < 5000
+    xf, _ = c.tran(2e-7, 5e-3, method=oracle.BE, cfg=cfg)  # fine fixed-step reference
+    tf = np.arange(xf.shape[0]) * 2e-7
+    vo = np.interp(t, tf, xf[:, c.index["V(out)"]])
+    assert np.max(np.abs(x[:, c.index["V(out)"]] - vo)) < 2e-2
