@@ -1,0 +1,196 @@
+/*
+ * krets_gpu.h — C ABI of the MI355X (gfx950) MNA solve backend for krets.
+ *
+ * This library replaces one path of krets: MNA stamping into a fixed sparse
+ * pattern, sparse LU, triangular solves, and the Newton / DC-sweep / AC-sweep /
+ * transient loops around them. It keeps krets' unknown numbering
+ * (crates/krets-parser/src/parser.rs:105-125), sign conventions and status
+ * codes, so results map back 1:1 onto krets' AnalysisResult maps.
+ *
+ * All functions are synchronous, return a kg_status, and never throw or abort.
+ * A context owns its device memory and one HIP stream; it is not re-entrant
+ * (one context per host thread), like krets' `Solver::solve(&mut self, ..)`.
+ * Output buffers are caller-owned, either host memory (KG_MEM_HOST) or device
+ * memory on the context's GPU (KG_MEM_DEVICE).
+ */
+#ifndef KRETS_GPU_H
+#define KRETS_GPU_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 0..6 mirror krets_solver::Error
+ * (crates/krets-solver/src/error.rs:3-36): MatrixBuild, DecompositionFailed /
+ * MatrixDecomposition, MaximumIterationsExceeded, ElementNotFound,
+ * InvalidElementFormat, Unexpected. 7..8 mirror krets_parser errors
+ * (ParseError, EmptyNetlist). */
+typedef enum {
+  KG_OK = 0,
+  KG_MATRIX_BUILD = 1,
+  KG_DECOMPOSITION = 2,
+  KG_MAX_ITERATIONS = 3,
+  KG_ELEMENT_NOT_FOUND = 4,
+  KG_INVALID_ELEMENT_FORMAT = 5,
+  KG_UNEXPECTED = 6,
+  KG_PARSE = 7,
+  KG_EMPTY_NETLIST = 8,
+  KG_INVALID_ARGUMENT = 9,
+  KG_HIP_ERROR = 100
+} kg_status;
+
+typedef enum { KG_R = 0, KG_C = 1, KG_L = 2, KG_V = 3, KG_I = 4, KG_D = 5 } kg_element_type;
+typedef enum { KG_WAVE_NONE = 0, KG_WAVE_PULSE = 1, KG_WAVE_SIN = 2 } kg_wave_kind;
+typedef enum { KG_FP64 = 0, KG_MIXED = 1 } kg_precision;
+typedef enum { KG_BE_COMPAT = 0, KG_BE = 1, KG_TRAP = 2 } kg_method;
+typedef enum { KG_MEM_HOST = 0, KG_MEM_DEVICE = 1 } kg_mem;
+/* Work mapping: threads per system. 0 = autotuned. */
+typedef enum { KG_MAP_AUTO = 0 } kg_mapping;
+
+/* One element in krets' numbering: plus/minus are the indices of V(plus) and
+ * V(minus) (-1 for ground "0"), branch the index of I(<id>) or -1. */
+typedef struct {
+  int32_t type;
+  int32_t plus;
+  int32_t minus;
+  int32_t branch;
+  int32_t g2; /* Group-2 flag of R/C (capacitor.rs:172) */
+  int32_t reserved;
+} kg_element;
+
+/* Per (instance, element) parameter record of KG_NPARAM doubles:
+ *   R: [0] ohms          C: [0] farads        L: [0] henries
+ *   I: [0] amps          D: [0] Is, [1] N
+ *   V: [0] dc value, [1] ac magnitude, [2] waveform kind (kg_wave_kind),
+ *      [3..9] PULSE(v1 v2 td tr tf pw per) or SIN(vo va f td theta phase). */
+#define KG_NPARAM 10
+
+/* A batch of circuit instances. With shared_topology = 1 every instance uses
+ * elements[0..n_elements) and n_unknowns[0]; params is
+ * [n_instances][n_elements][KG_NPARAM]. Otherwise instance i uses
+ * elements[elem_offset[i]..elem_offset[i+1]) and n_unknowns[i], and its
+ * parameter records start at params + elem_offset[i] * KG_NPARAM. */
+typedef struct {
+  int32_t n_instances;
+  int32_t shared_topology;
+  int32_t n_elements;
+  int32_t reserved;
+  const int32_t* elem_offset;
+  const int32_t* n_unknowns;
+  const kg_element* elements;
+  const double* params;
+} kg_batch_desc;
+
+/* SolverConfig (crates/krets-solver/src/config.rs:4-36) plus mode flags.
+ * compat = 1 reproduces krets' Newton semantics exactly (empty first map,
+ * 0.5 V diode default, bookkeeping key in the convergence test, unlimited
+ * diode exponent); compat = 0 is the clean mode (zero start, pnjlim). */
+typedef struct {
+  double reltol;
+  double iabstol;
+  double vabstol;
+  int32_t maxiter;
+  int32_t compat;
+  int32_t threads_per_system; /* 0 = autotuned */
+  int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
+} kg_solver_config;
+
+typedef struct kg_ctx kg_ctx;
+typedef struct kg_circuit kg_circuit;
+
+void kg_default_config(kg_solver_config* cfg);
+
+/* Context lifetime. */
+int32_t kg_create(kg_ctx** ctx, int32_t device);
+int32_t kg_destroy(kg_ctx* ctx);
+const char* kg_last_error(const kg_ctx* ctx);
+
+/* Replaces faer's per-solve symbolic analysis (called at solver/op.rs:48-51,
+ * dc.rs:167-170, ac.rs:67-70, transient.rs:183-186) with one host symbolic
+ * pass per distinct topology: pattern, static pivot order (structural
+ * matching plus fill-reducing order), fill, level sets; then uploads the
+ * batch. */
+int32_t kg_load_batch(kg_ctx* ctx, const kg_batch_desc* batch);
+
+/* op::solve (crates/krets-solver/src/solver/op.rs:15-86) for every instance.
+ * x: [n_instances][n] doubles. status/iters: [n_instances] (may be NULL).
+ * Returns the first failing instance's status (KG_OK if all succeeded). */
+int32_t kg_op(kg_ctx* ctx, const kg_solver_config* cfg, double* x, int32_t* status, int32_t* iters,
+              int32_t mem);
+
+/* dc::solve (solver/dc.rs:14-124): sweeps element `element` (index within the
+ * instance's element list; must be V or I) over start + i*step,
+ * i < |(stop-start)/step| + 1. x: [n_instances][points][n]. */
+int32_t kg_dc_sweep(kg_ctx* ctx, const kg_solver_config* cfg, int32_t element, double start, double stop,
+                    double step, double* x, int32_t max_points, int32_t* n_points, int32_t* status,
+                    int32_t* iters, int32_t mem);
+
+/* ac::solve (solver/ac.rs:17-109) over an explicit frequency list (positive
+ * entries; krets skips f <= 0). x: [n_instances][nf][n] complex128
+ * interleaved (re, im). With KG_MIXED, the factorisation runs in complex64
+ * and FP64 iterative refinement restores accuracy; err_est[instance] (may be
+ * NULL) receives the largest normwise relative size of the last refinement
+ * correction. */
+int32_t kg_ac_sweep(kg_ctx* ctx, const kg_solver_config* cfg, const double* freqs, int32_t nf,
+                    int32_t precision, double* x, int32_t* status, double* err_est, int32_t mem);
+
+/* transient::solve (solver/transient.rs:14-114) with a fixed step h and
+ * nsteps steps (krets: nsteps = round(stop/h)). method KG_BE_COMPAT is
+ * krets' integrator with its quirks; KG_BE and KG_TRAP are clean.
+ * x: [n_instances][nsteps+1][n] (row 0 = operating point); iters:
+ * [n_instances][nsteps+1] Newton iterations (may be NULL). One persistent
+ * launch runs all steps of all instances. */
+int32_t kg_transient(kg_ctx* ctx, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
+                     int32_t precision, double* x, int32_t* status, int32_t* iters, double* err_est,
+                     int32_t mem);
+
+/* Backward Euler with local-truncation-error step control (no krets
+ * counterpart; krets is fixed-step only). Accepted points per instance are
+ * written to t: [n_instances][max_points] and x: [n_instances][max_points][n].
+ * n_points, newton_total, rejected: [n_instances]. */
+int32_t kg_transient_lte(kg_ctx* ctx, const kg_solver_config* cfg, double tstop, double h0, double hmin,
+                         double hmax, double lte_reltol, double lte_abstol, int32_t max_points, double* t,
+                         double* x, int32_t* n_points, int32_t* newton_total, int32_t* rejected,
+                         int32_t* status, int32_t mem);
+
+/* Discrete adjoint of a fixed-step linear transient (no krets counterpart).
+ * Objective J = sum_n w_n * (cvec . x_n) with trapezoid weights w over
+ * t_0..t_nsteps; cvec: [n] in krets numbering, shared by all instances.
+ * Reuses the stored LU factors for transpose solves and recomputes states
+ * from checkpoints taken every `checkpoint_every` steps.
+ * J: [n_instances]; grad: [n_instances][n_elements] = dJ/d(param[0]) of every
+ * R, C and L (0 for other elements). */
+int32_t kg_adjoint_transient(kg_ctx* ctx, const kg_solver_config* cfg, double h, int32_t nsteps,
+                             int32_t method, int32_t precision, const double* cvec, int32_t checkpoint_every,
+                             double* J, double* grad, int32_t* status, int32_t mem);
+
+/* Kernel timing (no krets counterpart): reads the accumulated duration and
+ * launch count of the analysis kernels timed since the previous call (HIP
+ * events on the context's stream), then resets them and turns timing on
+ * (enable = 1) or off. */
+int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launches, const char** last_kernel);
+
+/* Diagnostics (no krets counterpart): the symbolic plan the library would
+ * build for one topology. kind: 0 DC, 1 AC, 2 transient; ordering: 0 natural,
+ * 1 minimum degree, 2 nested dissection. Returns the plan length in int32
+ * (copying at most cap entries to buf) or -status. layout receives the plan's
+ * header (PatLayout, see krets_amd/csrc/kg_internal.h). */
+int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, int32_t kind,
+                       int32_t ordering, int32_t* buf, int32_t cap, int32_t* layout, int32_t layout_cap);
+
+/* Netlist reader for the R/L/C/V/I/D subset of krets' grammar
+ * (crates/krets-parser/src/parser.rs:27-154). */
+int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* err, int32_t errlen);
+void kg_circuit_free(kg_circuit* c);
+int32_t kg_circuit_size(const kg_circuit* c, int32_t* n_unknowns, int32_t* n_elements);
+/* Name of unknown idx, e.g. "V(out)" or "I(V1)"; NULL if out of range. */
+const char* kg_circuit_unknown_name(const kg_circuit* c, int32_t idx);
+/* Identifier of element e, e.g. "V1" (Element::identifier, elements.rs:314). */
+const char* kg_circuit_element_id(const kg_circuit* c, int32_t e);
+/* Fills elements[n_elements] and params[n_elements][KG_NPARAM]. */
+int32_t kg_circuit_tables(const kg_circuit* c, kg_element* elements, double* params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,730 @@
+// Host side of the C ABI: contexts, batch upload, the per-topology symbolic
+// plan cache, launch configuration and the analysis entry points. Mirrors
+// krets_solver::Solver's dispatch (crates/krets-solver/src/solver.rs:17-53)
+// over a whole batch of instances.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "kg_launch.h"
+#include "krets_gpu.h"
+
+namespace kg {
+std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, AnalysisKind ak, Ordering ord, PatLayout& L,
+                                   std::string& err);
+}
+
+using namespace kg;
+
+namespace {
+
+struct PatEntry {
+  PatLayout lay;
+  int32_t* d_buf = nullptr;
+  PatDev dev{};
+  std::string err;
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+constexpr size_t LDS_MAX = 160 * 1024;
+constexpr size_t LDS_SHARED_BUDGET = 64 * 1024;
+
+}  // namespace
+
+struct kg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // batch
+  int n_inst = 0;
+  bool shared = false;
+  std::vector<kg_element> el;
+  std::vector<int64_t> eoff;
+  std::vector<int32_t> nunk;
+  std::vector<double> params, ebase;
+  std::vector<int> group_of, group_first;
+  double* d_params = nullptr;
+  double* d_ebase = nullptr;
+  int64_t* d_eoff = nullptr;
+  int32_t* d_nunk = nullptr;
+  // plans
+  std::map<std::tuple<int, int, int>, int> pat_index;
+  std::vector<PatEntry> pats;
+  PatDev* d_pats = nullptr;
+  size_t d_pats_n = 0;
+  std::map<std::pair<int, int>, int32_t*> inst_pat;
+  std::map<std::string, DevBuf> ws;
+  // kernel timing (HIP events on this context's stream)
+  bool prof = false;
+  double prof_ms = 0.0;
+  int64_t prof_n = 0;
+  std::string prof_name;
+};
+
+namespace {
+
+int32_t set_err(kg_ctx* c, int32_t code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, x)                                                                           \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess)                                                                        \
+      return set_err(ctx, KG_HIP_ERROR, std::string(#x ": ") + hipGetErrorString(e_));           \
+  } while (0)
+
+void* ws_get(kg_ctx* c, const std::string& name, size_t bytes, hipError_t* e) {
+  DevBuf& b = c->ws[name];
+  *e = hipSuccess;
+  if (b.cap < bytes) {
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    *e = hipMalloc(&b.p, std::max<size_t>(bytes, 256));
+    if (*e != hipSuccess) return nullptr;
+    b.cap = std::max<size_t>(bytes, 256);
+  }
+  return b.p;
+}
+
+void free_batch(kg_ctx* c) {
+  for (auto* p : {(void*)c->d_params, (void*)c->d_ebase, (void*)c->d_eoff, (void*)c->d_nunk, (void*)c->d_pats})
+    if (p) hipFree(p);
+  c->d_params = c->d_ebase = nullptr;
+  c->d_eoff = nullptr;
+  c->d_nunk = nullptr;
+  c->d_pats = nullptr;
+  c->d_pats_n = 0;
+  for (auto& p : c->pats)
+    if (p.d_buf) hipFree(p.d_buf);
+  c->pats.clear();
+  c->pat_index.clear();
+  for (auto& kv : c->inst_pat)
+    if (kv.second) hipFree(kv.second);
+  c->inst_pat.clear();
+  c->n_inst = 0;
+}
+
+PatDev to_dev(const PatLayout& L, const int32_t* b) {
+  PatDev d{};
+  d.n = L.n;
+  d.nnz = L.nnz;
+  d.ne = L.ne;
+  d.nlev = L.nlev;
+  d.nblev = L.nblev;
+  d.ntlev = L.ntlev;
+  d.ntblev = L.ntblev;
+  d.nonlinear = L.nonlinear;
+  d.has_diode = L.has_diode;
+  d.has_dyn = L.has_dyn;
+  if (!b) return d;
+  d.row_ptr = b + L.o_row_ptr;
+  d.col = b + L.o_col;
+  d.diag = b + L.o_diag;
+  d.fprog_ptr = b + L.o_fprog_ptr;
+  d.fprog = b + L.o_fprog;
+  d.lvl_ptr = b + L.o_lvl_ptr;
+  d.lvl_rows = b + L.o_lvl_rows;
+  d.blvl_ptr = b + L.o_blvl_ptr;
+  d.blvl_rows = b + L.o_blvl_rows;
+  d.scon_ptr = b + L.o_scon_ptr;
+  d.scon = b + L.o_scon;
+  d.rcon_ptr = b + L.o_rcon_ptr;
+  d.rcon = b + L.o_rcon;
+  d.col_of = b + L.o_col_of;
+  d.cpos = b + L.o_cpos;
+  d.isbr = b + L.o_isbr;
+  d.etype = b + L.o_etype;
+  d.ep = b + L.o_ep;
+  d.em = b + L.o_em;
+  d.ec = b + L.o_ec;
+  d.ut_ptr = b + L.o_ut_ptr;
+  d.ut = b + L.o_ut;
+  d.lt_ptr = b + L.o_lt_ptr;
+  d.lt = b + L.o_lt;
+  d.tlvl_ptr = b + L.o_tlvl_ptr;
+  d.tlvl_rows = b + L.o_tlvl_rows;
+  d.tblvl_ptr = b + L.o_tblvl_ptr;
+  d.tblvl_rows = b + L.o_tblvl_rows;
+  return d;
+}
+
+// What one analysis launch needs to know about its plans.
+struct Plan {
+  int32_t* d_inst_pat = nullptr;
+  int nnz_max = 0, n_max = 0, ne_max = 0;
+  bool any_diode = false, any_nonlinear = false;
+  std::vector<int32_t> group_status;  // per group
+};
+
+int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, Plan& P) {
+  int ng = (int)c->group_first.size();
+  bool added = false;
+  P.group_status.assign(ng, KG_OK);
+  std::vector<int32_t> gpi(ng);
+  for (int g = 0; g < ng; g++) {
+    auto key = std::make_tuple(g, (int)ak, (int)ord);
+    auto it = c->pat_index.find(key);
+    int idx;
+    if (it == c->pat_index.end()) {
+      int i0 = c->group_first[g];
+      int ne = (int)(c->eoff[i0 + 1] - c->eoff[i0]);
+      PatEntry pe;
+      std::vector<int32_t> buf = build_pattern(c->el.data() + c->eoff[i0], ne, c->nunk[i0], ak, ord, pe.lay, pe.err);
+      if (pe.lay.status == KG_OK) {
+        HIPCHK(c, hipMalloc(&pe.d_buf, sizeof(int32_t) * std::max<size_t>(buf.size(), 1)));
+        HIPCHK(c, hipMemcpy(pe.d_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice));
+        pe.dev = to_dev(pe.lay, pe.d_buf);
+      } else {
+        PatLayout z{};
+        pe.dev = to_dev(z, nullptr);
+      }
+      idx = (int)c->pats.size();
+      c->pats.push_back(pe);
+      c->pat_index[key] = idx;
+      added = true;
+    } else {
+      idx = it->second;
+    }
+    gpi[g] = idx;
+    const PatEntry& pe = c->pats[idx];
+    P.group_status[g] = pe.lay.status;
+    if (pe.lay.status == KG_OK) {
+      P.nnz_max = std::max(P.nnz_max, pe.lay.nnz);
+      P.n_max = std::max(P.n_max, pe.lay.n);
+      P.ne_max = std::max(P.ne_max, pe.lay.ne);
+      P.any_diode |= pe.lay.has_diode != 0;
+      P.any_nonlinear |= pe.lay.nonlinear != 0;
+    } else {
+      c->err = pe.err;
+    }
+  }
+  if (added || !c->d_pats) {
+    std::vector<PatDev> hv;
+    for (auto& p : c->pats) hv.push_back(p.dev);
+    if (c->d_pats) hipFree(c->d_pats);
+    c->d_pats = nullptr;
+    HIPCHK(c, hipMalloc(&c->d_pats, sizeof(PatDev) * hv.size()));
+    HIPCHK(c, hipMemcpy(c->d_pats, hv.data(), sizeof(PatDev) * hv.size(), hipMemcpyHostToDevice));
+    c->d_pats_n = hv.size();
+  }
+  auto k2 = std::make_pair((int)ak, (int)ord);
+  if (!c->inst_pat.count(k2)) {
+    std::vector<int32_t> tab(c->n_inst);
+    for (int i = 0; i < c->n_inst; i++) tab[i] = gpi[c->group_of[i]];
+    int32_t* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, sizeof(int32_t) * tab.size()));
+    HIPCHK(c, hipMemcpy(d, tab.data(), sizeof(int32_t) * tab.size(), hipMemcpyHostToDevice));
+    c->inst_pat[k2] = d;
+  }
+  P.d_inst_pat = c->inst_pat[k2];
+  return KG_OK;
+}
+
+BatchDev batch_dev(kg_ctx* c, const Plan& P) {
+  BatchDev B;
+  B.pats = c->d_pats;
+  B.inst_pat = P.d_inst_pat;
+  B.params = c->d_params;
+  B.ebase = c->d_ebase;
+  B.eoff = c->d_eoff;
+  B.n_inst = c->d_nunk;
+  B.n_instances = c->n_inst;
+  return B;
+}
+
+struct Launch {
+  int tps = 64, spb = 1;
+  size_t lds = 0;
+  dim3 grid, block;
+};
+
+bool valid_tps(int t) { return t == 1 || t == 4 || t == 16 || t == 64 || t == 256; }
+
+int default_tps(const kg_ctx* c, int n_max) {
+  if (c->shared && n_max <= 8) return 1;
+  if (n_max <= 96) return 16;
+  if (n_max <= 600) return 64;
+  return 256;
+}
+
+Ordering order_for(int tps) { return tps == 1 ? ORD_MINDEG : ORD_NESTED; }
+
+int32_t configure(kg_ctx* c, int tps, Lay lay, int64_t nsys, Launch& L) {
+  size_t sb = sys_bytes(lay);
+  L.tps = tps;
+  int spb = c->shared ? std::max(1, 256 / tps) : 1;
+  while (spb > 1 && spb * sb + 8 * (size_t)spb * tps > LDS_SHARED_BUDGET) spb /= 2;
+  if (spb * sb + 8 * (size_t)spb * tps > LDS_MAX)
+    return set_err(c, KG_UNEXPECTED,
+                   "system too large for on-chip memory (" + std::to_string(sb) + " bytes per system)");
+  L.spb = spb;
+  L.lds = spb * sb + 8 * (size_t)spb * tps;
+  L.block = dim3(tps * spb);
+  L.grid = dim3((unsigned)((nsys + spb - 1) / spb));
+  return KG_OK;
+}
+
+int32_t copy_out(kg_ctx* c, void* dst, const void* src, size_t bytes, int mem) {
+  if (!dst || dst == src || bytes == 0) return KG_OK;
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, mem == KG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                           c->stream));
+  return KG_OK;
+}
+
+int32_t first_failure(const std::vector<int32_t>& st) {
+  for (int32_t s : st)
+    if (s != KG_OK) return s;
+  return KG_OK;
+}
+
+// Offsets (in doubles) of each instance's output block of `per_unknown`
+// doubles per unknown.
+std::vector<int64_t> out_offsets(const kg_ctx* c, int64_t per_unknown) {
+  std::vector<int64_t> o(c->n_inst + 1, 0);
+  for (int i = 0; i < c->n_inst; i++) o[i + 1] = o[i] + per_unknown * c->nunk[i];
+  return o;
+}
+
+int32_t upload_offsets(kg_ctx* c, const std::string& name, const std::vector<int64_t>& o, int64_t** d) {
+  hipError_t e;
+  *d = (int64_t*)ws_get(c, name, sizeof(int64_t) * o.size(), &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  HIPCHK(c, hipMemcpyAsync(*d, o.data(), sizeof(int64_t) * o.size(), hipMemcpyHostToDevice, c->stream));
+  return KG_OK;
+}
+
+// Runs `launch` on the context stream, timing it with HIP events when
+// profiling is on.
+template <class F>
+int32_t timed(kg_ctx* c, const char* name, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->prof) {
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    HIPCHK(c, hipEventRecord(a, c->stream));
+  }
+  hipError_t e = launch();
+  if (e != hipSuccess) {
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    return set_err(c, KG_HIP_ERROR, std::string(name) + ": " + hipGetErrorString(e));
+  }
+  if (c->prof) {
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+    c->prof_ms += ms;
+    c->prof_n += 1;
+    c->prof_name = name;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+  }
+  return KG_OK;
+}
+
+NewtonArgs newton_args(const kg_solver_config* cfg) {
+  return NewtonArgs{cfg->reltol, cfg->iabstol, cfg->vabstol, cfg->maxiter, cfg->compat};
+}
+
+// Runs OP (npts = 1) or a DC sweep; result stays on the device in the
+// workspace buffer "newton_out"; status in "status" (int32 per instance).
+int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start, double step, int npts,
+               double** d_out, int64_t** d_off, int32_t** d_status, int32_t** d_iters) {
+  Plan P;
+  int tps0 = cfg->threads_per_system > 0 ? cfg->threads_per_system : 0;
+  // prepare with a provisional ordering to learn sizes
+  Plan P0;
+  int32_t r = prepare(c, AK_DC, ORD_NESTED, P0);
+  if (r) return r;
+  int tps = tps0 ? tps0 : default_tps(c, P0.n_max);
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  r = prepare(c, AK_DC, order_for(tps), P);
+  if (r) return r;
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, 1, 8, 8, 0, 0};
+  Launch L;
+  r = configure(c, tps, lay, c->n_inst, L);
+  if (r) return r;
+  auto off = out_offsets(c, npts);
+  r = upload_offsets(c, "newton_off", off, d_off);
+  if (r) return r;
+  hipError_t e;
+  *d_out = (double*)ws_get(c, "newton_out", sizeof(double) * std::max<int64_t>(off.back(), 1), &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  *d_status = (int32_t*)ws_get(c, "status", sizeof(int32_t) * c->n_inst, &e);
+  *d_iters = (int32_t*)ws_get(c, "newton_iters", sizeof(int32_t) * (size_t)c->n_inst * npts, &e);
+  HIPCHK(c, hipMemsetAsync(*d_status, 0, sizeof(int32_t) * c->n_inst, c->stream));
+  BatchDev B = batch_dev(c, P);
+  NewtonArgs na = newton_args(cfg);
+  int32_t rr = timed(c, "k_newton_dc", [&] {
+    return run_newton(tps, L.grid, L.block, L.lds, c->stream, B, 0, na, sweep_e, start, step, npts, *d_out, *d_off,
+                      *d_status, *d_iters, lay);
+  });
+  if (rr) return rr;
+  // instances whose topology has no plan fail as krets' sp_lu would
+  bool any_bad = false;
+  for (auto s : P.group_status) any_bad |= s != KG_OK;
+  if (any_bad) {
+    std::vector<int32_t> st(c->n_inst);
+    HIPCHK(c, hipMemcpyAsync(st.data(), *d_status, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->n_inst; i++)
+      if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
+    HIPCHK(c, hipMemcpyAsync(*d_status, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice, c->stream));
+  }
+  return KG_OK;
+}
+
+int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem) {
+  std::vector<int32_t> st(c->n_inst);
+  HIPCHK(c, hipMemcpyAsync(st.data(), d_status, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
+  if (status_out) {
+    int32_t r = copy_out(c, status_out, d_status, sizeof(int32_t) * c->n_inst, mem);
+    if (r) return r;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int32_t f = first_failure(st);
+  if (f != KG_OK && c->err.empty()) c->err = "at least one instance failed";
+  return f;
+}
+
+}  // namespace
+
+extern "C" {
+
+void kg_default_config(kg_solver_config* cfg) {
+  cfg->reltol = 1e-3;  // config.rs:25-36
+  cfg->iabstol = 1e-12;
+  cfg->vabstol = 1e-6;
+  cfg->maxiter = 300;
+  cfg->compat = 1;
+  cfg->threads_per_system = 0;
+  cfg->refine_iters = 0;
+}
+
+int32_t kg_create(kg_ctx** out, int32_t device) {
+  if (!out) return KG_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return KG_HIP_ERROR;
+  if (device < 0 || device >= n) return KG_INVALID_ARGUMENT;
+  auto* c = new kg_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return KG_HIP_ERROR;
+  }
+  *out = c;
+  return KG_OK;
+}
+
+int32_t kg_destroy(kg_ctx* c) {
+  if (!c) return KG_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  free_batch(c);
+  for (auto& kv : c->ws)
+    if (kv.second.p) hipFree(kv.second.p);
+  hipStreamDestroy(c->stream);
+  delete c;
+  return KG_OK;
+}
+
+const char* kg_last_error(const kg_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int32_t kg_load_batch(kg_ctx* c, const kg_batch_desc* b) {
+  if (!c || !b || b->n_instances <= 0 || !b->elements || !b->params || !b->n_unknowns)
+    return set_err(c, KG_INVALID_ARGUMENT, "invalid batch descriptor");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_batch(c);
+  int ni = b->n_instances;
+  c->n_inst = ni;
+  c->shared = b->shared_topology != 0;
+  c->eoff.assign(ni + 1, 0);
+  c->nunk.assign(ni, 0);
+  if (c->shared) {
+    int ne = b->n_elements;
+    c->el.assign(b->elements, b->elements + ne);
+    for (int i = 0; i <= ni; i++) c->eoff[i] = (int64_t)i * ne;
+    for (int i = 0; i < ni; i++) c->nunk[i] = b->n_unknowns[0];
+  } else {
+    if (!b->elem_offset) return set_err(c, KG_INVALID_ARGUMENT, "elem_offset required for per-instance topologies");
+    for (int i = 0; i <= ni; i++) c->eoff[i] = b->elem_offset[i];
+    c->el.assign(b->elements, b->elements + b->elem_offset[ni]);
+    for (int i = 0; i < ni; i++) c->nunk[i] = b->n_unknowns[i];
+  }
+  int64_t nrec = c->shared ? (int64_t)ni * b->n_elements : c->eoff[ni];
+  c->params.assign(b->params, b->params + nrec * KG_NPARAM);
+  c->ebase.resize(nrec);
+  for (int i = 0; i < ni; i++) {
+    int64_t e0 = c->eoff[i], e1 = c->eoff[i + 1];
+    for (int64_t r = e0; r < e1; r++) {
+      const kg_element& e = c->el[c->shared ? r - e0 : r];
+      double v = c->params[r * KG_NPARAM];
+      c->ebase[r] = (e.type == KG_R && !e.g2) ? 1.0 / v : v;  // stampable.rs:232 g = 1.0 / value
+    }
+  }
+  if (c->shared) {
+    // element records are per instance in params/ebase; the element table is shared
+    std::vector<kg_element> rep(c->el);
+    c->el.clear();
+    for (int i = 0; i < ni; i++) c->el.insert(c->el.end(), rep.begin(), rep.end());
+    c->group_of.assign(ni, 0);
+    c->group_first.assign(1, 0);
+  } else {
+    // group identical topologies
+    std::map<std::vector<int32_t>, int> sig;
+    c->group_of.assign(ni, 0);
+    c->group_first.clear();
+    for (int i = 0; i < ni; i++) {
+      std::vector<int32_t> s{c->nunk[i]};
+      for (int64_t r = c->eoff[i]; r < c->eoff[i + 1]; r++) {
+        const kg_element& e = c->el[r];
+        s.insert(s.end(), {e.type, e.plus, e.minus, e.branch, e.g2});
+      }
+      auto it = sig.find(s);
+      if (it == sig.end()) {
+        int g = (int)c->group_first.size();
+        sig[s] = g;
+        c->group_first.push_back(i);
+        c->group_of[i] = g;
+      } else {
+        c->group_of[i] = it->second;
+      }
+    }
+  }
+  for (int i = 0; i < ni; i++)
+    for (int64_t r = c->eoff[i]; r < c->eoff[i + 1]; r++) {
+      const kg_element& e = c->el[r];
+      int n = c->nunk[i];
+      if (e.type < KG_R || e.type > KG_D || e.plus >= n || e.minus >= n || e.branch >= n)
+        return set_err(c, KG_INVALID_ELEMENT_FORMAT, "element table out of range");
+      if ((e.type == KG_V || e.type == KG_L || e.type == KG_I) && e.branch < 0)
+        return set_err(c, KG_INVALID_ELEMENT_FORMAT, "V, L and I elements need a branch unknown");
+    }
+  HIPCHK(c, hipMalloc(&c->d_params, sizeof(double) * std::max<size_t>(c->params.size(), 1)));
+  HIPCHK(c, hipMalloc(&c->d_ebase, sizeof(double) * std::max<size_t>(c->ebase.size(), 1)));
+  HIPCHK(c, hipMalloc(&c->d_eoff, sizeof(int64_t) * c->eoff.size()));
+  HIPCHK(c, hipMalloc(&c->d_nunk, sizeof(int32_t) * c->nunk.size()));
+  HIPCHK(c, hipMemcpy(c->d_params, c->params.data(), sizeof(double) * c->params.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_ebase, c->ebase.data(), sizeof(double) * c->ebase.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_eoff, c->eoff.data(), sizeof(int64_t) * c->eoff.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_nunk, c->nunk.data(), sizeof(int32_t) * c->nunk.size(), hipMemcpyHostToDevice));
+  return KG_OK;
+}
+
+int32_t kg_op(kg_ctx* c, const kg_solver_config* cfg, double* x, int32_t* status, int32_t* iters, int32_t mem) {
+  if (!c || !cfg || c->n_inst == 0) return set_err(c, KG_INVALID_ARGUMENT, "no batch loaded");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  double* d_out;
+  int64_t* d_off;
+  int32_t *d_st, *d_it;
+  int32_t r = run_dc(c, cfg, -1, 0.0, 0.0, 1, &d_out, &d_off, &d_st, &d_it);
+  if (r) return r;
+  int64_t tot = 0;
+  for (int i = 0; i < c->n_inst; i++) tot += c->nunk[i];
+  r = copy_out(c, x, d_out, sizeof(double) * tot, mem);
+  if (r) return r;
+  if (iters && (r = copy_out(c, iters, d_it, sizeof(int32_t) * c->n_inst, mem))) return r;
+  return finish_status(c, d_st, status, mem);
+}
+
+int32_t kg_dc_sweep(kg_ctx* c, const kg_solver_config* cfg, int32_t element, double start, double stop, double step,
+                    double* x, int32_t max_points, int32_t* n_points, int32_t* status, int32_t* iters, int32_t mem) {
+  if (!c || !cfg || c->n_inst == 0) return set_err(c, KG_INVALID_ARGUMENT, "no batch loaded");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  // dc.rs:23-38: the swept element must exist and be a V or I source
+  for (int i = 0; i < c->n_inst; i++) {
+    int64_t ne = c->eoff[i + 1] - c->eoff[i];
+    if (element < 0 || element >= ne) return set_err(c, KG_ELEMENT_NOT_FOUND, "swept element not found");
+    int ty = c->el[c->eoff[i] + element].type;
+    if (ty != KG_V && ty != KG_I)
+      return set_err(c, KG_INVALID_ELEMENT_FORMAT, "DC sweep element must be a voltage or current source");
+  }
+  // dc.rs:130-131
+  double nf = std::fabs((stop - start) / step);
+  if (!(nf < 1e9)) return set_err(c, KG_INVALID_ARGUMENT, "invalid sweep range");
+  int npts = (int)(int64_t)nf + 1;
+  if (n_points) *n_points = npts;
+  if (npts > max_points) return set_err(c, KG_INVALID_ARGUMENT, "output buffer too small for the sweep");
+  double* d_out;
+  int64_t* d_off;
+  int32_t *d_st, *d_it;
+  int32_t r = run_dc(c, cfg, element, start, step, npts, &d_out, &d_off, &d_st, &d_it);
+  if (r) return r;
+  int64_t tot = 0;
+  for (int i = 0; i < c->n_inst; i++) tot += (int64_t)c->nunk[i] * npts;
+  if ((r = copy_out(c, x, d_out, sizeof(double) * tot, mem))) return r;
+  if (iters && (r = copy_out(c, iters, d_it, sizeof(int32_t) * c->n_inst * (size_t)npts, mem))) return r;
+  return finish_status(c, d_st, status, mem);
+}
+
+int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs, int32_t nf, int32_t precision,
+                    double* x, int32_t* status, double* err_est, int32_t mem) {
+  if (!c || !cfg || c->n_inst == 0 || !freqs || nf <= 0) return set_err(c, KG_INVALID_ARGUMENT, "bad arguments");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  for (int k = 0; k < nf; k++)
+    if (!(freqs[k] > 0.0)) return set_err(c, KG_INVALID_ARGUMENT, "frequencies must be positive (ac.rs:40-45)");
+  // ac.rs:25: the operating point comes first
+  kg_solver_config ocfg = *cfg;
+  ocfg.threads_per_system = 0;
+  double* d_xop;
+  int64_t* d_xoff;
+  int32_t *d_st, *d_it;
+  int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
+  if (r) return r;
+  Plan P0, P;
+  if ((r = prepare(c, AK_AC, ORD_NESTED, P0))) return r;
+  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  if ((r = prepare(c, AK_AC, order_for(tps), P))) return r;
+  bool mixed = precision == KG_MIXED;
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 0, P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
+  Launch L;
+  if ((r = configure(c, tps, lay, (int64_t)c->n_inst * nf, L))) return r;
+  auto off = out_offsets(c, 2 * (int64_t)nf);
+  int64_t* d_off;
+  if ((r = upload_offsets(c, "ac_off", off, &d_off))) return r;
+  hipError_t e;
+  double* d_f = (double*)ws_get(c, "freqs", sizeof(double) * nf, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  HIPCHK(c, hipMemcpyAsync(d_f, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+  double* d_out = (mem == KG_MEM_DEVICE && x) ? x : (double*)ws_get(c, "ac_out", sizeof(double) * off.back(), &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
+  HIPCHK(c, hipMemsetAsync(d_err, 0, sizeof(double) * c->n_inst, c->stream));
+  int refine = cfg->refine_iters > 0 ? cfg->refine_iters : 2;
+  BatchDev B = batch_dev(c, P);
+  r = timed(c, mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>", [&] {
+    return run_ac(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out, d_off,
+                  d_st, d_err, refine, lay);
+  });
+  if (r) return r;
+  if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
+  if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;
+  for (int g = 0; g < (int)P.group_status.size(); g++)
+    if (P.group_status[g] != KG_OK) {
+      std::vector<int32_t> st(c->n_inst);
+      HIPCHK(c, hipMemcpy(st.data(), d_st, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost));
+      for (int i = 0; i < c->n_inst; i++)
+        if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
+      HIPCHK(c, hipMemcpy(d_st, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice));
+      break;
+    }
+  return finish_status(c, d_st, status, mem);
+}
+
+int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
+                     int32_t precision, double* x, int32_t* status, int32_t* iters, double* err_est, int32_t mem) {
+  if (!c || !cfg || c->n_inst == 0 || !(h > 0) || nsteps < 0 || method < KG_BE_COMPAT || method > KG_TRAP)
+    return set_err(c, KG_INVALID_ARGUMENT, "bad arguments");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  kg_solver_config ocfg = *cfg;
+  ocfg.threads_per_system = 0;
+  ocfg.compat = method == KG_BE_COMPAT ? 1 : 0;  // transient.rs:21 op::solve; clean mode starts from zero
+  if (method != KG_BE_COMPAT) ocfg.compat = cfg->compat && method == KG_BE_COMPAT;
+  double* d_xop;
+  int64_t* d_xoff;
+  int32_t *d_st, *d_it;
+  int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
+  if (r) return r;
+  Plan P0, P;
+  if ((r = prepare(c, AK_TRAN, ORD_NESTED, P0))) return r;
+  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  if ((r = prepare(c, AK_TRAN, order_for(tps), P))) return r;
+  bool mixed = precision == KG_MIXED;
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
+  Launch L;
+  if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
+  auto off = out_offsets(c, (int64_t)nsteps + 1);
+  int64_t* d_off;
+  if ((r = upload_offsets(c, "tran_off", off, &d_off))) return r;
+  hipError_t e;
+  double* d_out = (mem == KG_MEM_DEVICE && x) ? x : (double*)ws_get(c, "tran_out", sizeof(double) * off.back(), &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  int32_t* d_iters = (int32_t*)ws_get(c, "tran_iters", sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), &e);
+  double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  TranArgs ta{h, method == KG_TRAP ? 2.0 : 1.0, nsteps, method, cfg->refine_iters > 0 ? cfg->refine_iters : 2};
+  NewtonArgs na = newton_args(cfg);
+  na.compat = method == KG_BE_COMPAT ? 1 : 0;
+  BatchDev B = batch_dev(c, P);
+  r = timed(c, mixed ? "k_tran<float,double>" : "k_tran<double,double>", [&] {
+    return run_tran(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out, d_off,
+                    d_st, d_iters, d_err, lay);
+  });
+  if (r) return r;
+  if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
+  if (iters && (r = copy_out(c, iters, d_iters, sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), mem))) return r;
+  if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;
+  for (int g = 0; g < (int)P.group_status.size(); g++)
+    if (P.group_status[g] != KG_OK) {
+      std::vector<int32_t> st(c->n_inst);
+      HIPCHK(c, hipMemcpy(st.data(), d_st, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost));
+      for (int i = 0; i < c->n_inst; i++)
+        if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
+      HIPCHK(c, hipMemcpy(d_st, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice));
+      break;
+    }
+  return finish_status(c, d_st, status, mem);
+}
+
+int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launches, const char** last_kernel) {
+  if (!c) return KG_INVALID_ARGUMENT;
+  if (total_ms) *total_ms = c->prof_ms;
+  if (launches) *launches = c->prof_n;
+  if (last_kernel) *last_kernel = c->prof_name.c_str();
+  c->prof = enable != 0;
+  c->prof_ms = 0.0;
+  c->prof_n = 0;
+  return KG_OK;
+}
+
+int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind, int32_t ordering, int32_t* buf,
+                       int32_t cap, int32_t* layout, int32_t layout_cap) {
+  if (!el || ne <= 0 || n <= 0 || kind < 0 || kind >= AK_COUNT || ordering < 0 || ordering > ORD_NESTED)
+    return -KG_INVALID_ARGUMENT;
+  PatLayout L;
+  std::string err;
+  std::vector<int32_t> b = build_pattern(el, ne, n, (AnalysisKind)kind, (Ordering)ordering, L, err);
+  if (layout) {
+    int32_t m = std::min<int32_t>(layout_cap, (int32_t)(sizeof(PatLayout) / sizeof(int32_t)));
+    std::memcpy(layout, &L, sizeof(int32_t) * m);
+  }
+  if (L.status != KG_OK) return -L.status;
+  if (buf) std::memcpy(buf, b.data(), sizeof(int32_t) * std::min<size_t>(cap, b.size()));
+  return (int32_t)b.size();
+}
+
+int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config*, double, double, double, double, double, double, int32_t,
+                         double*, double*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t) {
+  return set_err(c, KG_UNEXPECTED, "kg_transient_lte: not available in this build");
+}
+
+int32_t kg_adjoint_transient(kg_ctx* c, const kg_solver_config*, double, int32_t, int32_t, int32_t, const double*,
+                             int32_t, double*, double*, int32_t*, int32_t) {
+  return set_err(c, KG_UNEXPECTED, "kg_adjoint_transient: not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,926 @@
+// gfx950 kernels of the MNA solve backend.
+//
+// Every system (one circuit instance at one analysis point) lives in LDS for
+// its whole life: LU values in the pattern fixed by the host symbolic pass,
+// the solution, the right-hand side and a work vector. TPS threads cooperate
+// on one system (TPS = 1: lane per system, TPS = 64: wave per system, larger:
+// workgroup per system) and SPB = blockDim/TPS systems share a block; all
+// systems of a block share one pattern, so every phase is block-uniform and
+// phases are separated by barriers. Rows are processed level by level:
+//
+//   stamp   per LU slot, sum the element contributions that land there
+//           (krets' triplets + sum_triplets, stampable.rs / solver.rs:59-70)
+//   factor  up-looking row LU without pivoting (pivots fixed on the host);
+//           the reciprocal pivot is stored in the diagonal slot
+//   solve   forward and backward substitution by dependency level
+//   refine  (mixed precision) FP64 residual from freshly gathered FP64
+//           stamps, scaled FP32 correction solve, FP64 update
+//
+// Analysis drivers: Newton for OP / DC sweep (solver/op.rs, solver/dc.rs),
+// AC sweep (solver/ac.rs), fixed-step transient as one persistent launch
+// (solver/transient.rs plus trapezoidal), LTE-controlled backward Euler and
+// the discrete adjoint.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "kg_launch.h"
+
+namespace kg {
+
+// ------------------------------------------------------------ scalar types
+template <class R>
+struct cplx {
+  R re, im;
+};
+using c64 = cplx<float>;
+using c128 = cplx<double>;
+
+template <class R>
+__device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R> operator-(cplx<R> a) { return {-a.re, -a.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class R>
+__device__ __forceinline__ cplx<R> operator*(R s, cplx<R> a) { return {s * a.re, s * a.im}; }
+template <class R>
+__device__ __forceinline__ cplx<R>& operator+=(cplx<R>& a, cplx<R> b) { a = a + b; return a; }
+template <class R>
+__device__ __forceinline__ cplx<R>& operator-=(cplx<R>& a, cplx<R> b) { a = a - b; return a; }
+
+template <class S> struct Traits;
+template <> struct Traits<double> { using R = double; static constexpr bool cx = false; };
+template <> struct Traits<float> { using R = float; static constexpr bool cx = false; };
+template <> struct Traits<c128> { using R = double; static constexpr bool cx = true; };
+template <> struct Traits<c64> { using R = float; static constexpr bool cx = true; };
+
+template <class S> __device__ __forceinline__ S mk(double re, double im);
+template <> __device__ __forceinline__ double mk<double>(double re, double) { return re; }
+template <> __device__ __forceinline__ float mk<float>(double re, double) { return (float)re; }
+template <> __device__ __forceinline__ c128 mk<c128>(double re, double im) { return {re, im}; }
+template <> __device__ __forceinline__ c64 mk<c64>(double re, double im) { return {(float)re, (float)im}; }
+
+__device__ __forceinline__ double re_of(double a) { return a; }
+__device__ __forceinline__ double re_of(float a) { return a; }
+__device__ __forceinline__ double re_of(c128 a) { return a.re; }
+__device__ __forceinline__ double re_of(c64 a) { return a.re; }
+__device__ __forceinline__ double im_of(double) { return 0.0; }
+__device__ __forceinline__ double im_of(float) { return 0.0; }
+__device__ __forceinline__ double im_of(c128 a) { return a.im; }
+__device__ __forceinline__ double im_of(c64 a) { return a.im; }
+
+template <class D, class S>
+__device__ __forceinline__ D cvt(S a) { return mk<D>(re_of(a), im_of(a)); }
+
+// modulus used for pivot checks and norms
+__device__ __forceinline__ double absv(double a) { return fabs(a); }
+__device__ __forceinline__ double absv(float a) { return fabsf(a); }
+__device__ __forceinline__ double absv(c128 a) { return sqrt(a.re * a.re + a.im * a.im); }
+__device__ __forceinline__ double absv(c64 a) { return sqrt((double)a.re * a.re + (double)a.im * a.im); }
+
+// reciprocal; complex via Smith's scaled division
+__device__ __forceinline__ double rcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ float rcp(float a) { return 1.0f / a; }
+template <class R>
+__device__ __forceinline__ cplx<R> rcp(cplx<R> b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    R r = b.im / b.re, d = b.re + b.im * r;
+    return {R(1) / d, -r / d};
+  }
+  R r = b.re / b.im, d = b.re * r + b.im;
+  return {r / d, R(-1) / d};
+}
+__device__ __forceinline__ bool finite_v(double a) { return isfinite(a); }
+__device__ __forceinline__ bool finite_v(float a) { return isfinite(a); }
+template <class R>
+__device__ __forceinline__ bool finite_v(cplx<R> a) { return isfinite(a.re) && isfinite(a.im); }
+__device__ __forceinline__ double scal(double s, double a) { return s * a; }
+__device__ __forceinline__ c128 scal(double s, c128 a) { return {s * a.re, s * a.im}; }
+
+// ------------------------------------------------------- device maths
+// constants.rs:1-4
+__device__ constexpr double KG_VT = 1.380649e-23 * 300.0 / 1.602176634e-19;
+__device__ constexpr double KG_PI = 3.14159265358979323846;
+
+// Pulse::value_at / Sinusoidal::value_at (voltage_source.rs:50-93) and
+// transient_value_at (243-252).
+__device__ double wave_value(const double* p, double t) {
+  int kind = (int)p[2];
+  const double* w = p + 3;
+  if (kind == KG_WAVE_PULSE) {
+    double v1 = w[0], v2 = w[1], td = w[2], tr = w[3], tf = w[4], pw = w[5], per = w[6];
+    if (t < td) return v1;
+    double tt = fmod(t - td, per);
+    if (tt < tr) return v1 + (v2 - v1) * (tt / tr);
+    if (tt < tr + pw) return v2;
+    if (tt < tr + pw + tf) return v2 + (v1 - v2) * ((tt - tr - pw) / tf);
+    return v1;
+  }
+  if (kind == KG_WAVE_SIN) {
+    double vo = w[0], va = w[1], f = w[2], td = w[3], th = w[4], ph = w[5];
+    if (t < td) return vo;
+    double om = 2.0 * KG_PI * f;
+    double phr = ph * (KG_PI / 180.0);
+    double tt = t - td;
+    return vo + va * exp(-th * tt) * sin(om * tt + phr);
+  }
+  return p[0];
+}
+
+// Diode companion (elements/diode.rs:24-71). compat: krets' semantics, where
+// the exponent limit is n*Vt*ln(DBL_MAX*n*Vt/Is) (infinite for any practical
+// Is) and Ieq uses the unlimited voltage. clean: SPICE3 pnjlim on the
+// junction voltage against the previous iterate, consistent linearisation.
+struct DiodeLin {
+  double g, ieq, vd;
+  bool limited;
+};
+__device__ DiodeLin diode_eval(double vd, double is, double n, bool compat, double vold) {
+  double nvt = n * KG_VT;
+  DiodeLin r;
+  r.limited = false;
+  if (compat) {
+    double vcrit = nvt * log(DBL_MAX * n * KG_VT / is);
+    double vl = fmin(fmax(vd, -vcrit), vcrit);
+    double ex = exp(vl / nvt);
+    r.g = (is / nvt) * ex;
+    double i = is * (ex - 1.0);
+    r.ieq = i - r.g * vd;
+    r.vd = vd;
+    return r;
+  }
+  double vcrit = nvt * log(nvt / (1.4142135623730951 * is));
+  double v = vd;
+  if (v > vcrit && fabs(v - vold) > nvt + nvt) {
+    if (vold > 0) {
+      double arg = 1.0 + (v - vold) / nvt;
+      v = arg > 0 ? vold + nvt * log(arg) : vcrit;
+    } else {
+      v = nvt * log(v / nvt);
+    }
+    r.limited = true;
+  }
+  double ex = exp(v / nvt);
+  r.g = is / nvt * ex;
+  r.ieq = is * (ex - 1.0) - r.g * v;
+  r.vd = v;
+  return r;
+}
+
+// Per-point stamp parameters.
+struct StampArgs {
+  int ak;            // AnalysisKind
+  double freq;       // AC frequency (krets computes 2*PI*f*value left to right)
+  double h;          // transient step
+  double alpha;      // 1 = backward Euler, 2 = trapezoidal
+  double tsrc;       // transient source time
+  int sweep_e;       // DC-swept element or -1
+  double sweep_val;  // its value at this point
+};
+
+
+template <class A>
+__device__ __forceinline__ A elem_y(int e, const PatDev& P, const double* prm, const double* eb, const StampArgs& sa,
+                                    const double* ed) {
+  int ty = P.etype[e];
+  if (ty == KG_R) return mk<A>(eb[e], 0.0);
+  if (ty == KG_C) {
+    double c = prm[e * KG_NPARAM];
+    if (sa.ak == AK_AC) return mk<A>(0.0, 2.0 * KG_PI * sa.freq * c);
+    return mk<A>(sa.alpha * (c / sa.h), 0.0);
+  }
+  if (ty == KG_D) return mk<A>(ed[e * ESTRIDE], 0.0);
+  return mk<A>(0.0, 0.0);
+}
+
+template <class A>
+__device__ __forceinline__ A elem_z(int e, const PatDev& P, const double* prm, const StampArgs& sa) {
+  int ty = P.etype[e];
+  double v = prm[e * KG_NPARAM];
+  if (ty == KG_L) {
+    if (sa.ak == AK_AC) return mk<A>(-0.0, -(2.0 * KG_PI * sa.freq * v));
+    return mk<A>(-(sa.alpha * (v / sa.h)), 0.0);
+  }
+  if (ty == KG_R) return mk<A>(-v, 0.0);
+  return mk<A>(1.0, 0.0);
+}
+
+template <class A>
+__device__ __forceinline__ A mat_contrib(int32_t code, const PatDev& P, const double* prm, const double* eb,
+                                         const StampArgs& sa, const double* ed) {
+  int e = code >> 3, k = code & 7;
+  switch (k) {
+    case CK_YP: return elem_y<A>(e, P, prm, eb, sa, ed);
+    case CK_YN: return -elem_y<A>(e, P, prm, eb, sa, ed);
+    case CK_ONEP: return mk<A>(1.0, 0.0);
+    case CK_ONEN: return mk<A>(-1.0, 0.0);
+    default: return elem_z<A>(e, P, prm, sa);
+  }
+}
+
+// xp: previous-step solution (permuted numbering) for companion models.
+template <class A>
+__device__ __forceinline__ A rhs_contrib(int32_t code, const PatDev& P, const double* prm, const StampArgs& sa,
+                                         const double* ed, const double* xp) {
+  int e = code >> 3, k = code & 7;
+  const double* pe = prm + e * KG_NPARAM;
+  switch (k) {
+    case RK_SRC: {
+      if (e == sa.sweep_e) return mk<A>(sa.sweep_val, 0.0);
+      if (P.etype[e] == KG_V) {
+        if (sa.ak == AK_AC) return mk<A>(pe[1], 0.0);
+        if (sa.ak == AK_TRAN) return mk<A>(wave_value(pe, sa.tsrc), 0.0);
+      }
+      return mk<A>(pe[0], 0.0);
+    }
+    case RK_DP: return mk<A>(-ed[e * ESTRIDE + 1], 0.0);
+    case RK_DM: return mk<A>(ed[e * ESTRIDE + 1], 0.0);
+    case RK_CP:
+    case RK_CM: {
+      int p = P.ep[e], m = P.em[e];
+      double vprev = (p >= 0 ? xp[p] : 0.0) - (m >= 0 ? xp[m] : 0.0);
+      // krets: i_eq = -(C/h) * v_prev; RHS[p] += -i_eq, RHS[m] += i_eq
+      double ieq = -(sa.alpha * (pe[0] / sa.h)) * vprev;
+      if (sa.alpha != 1.0) ieq -= ed[e * ESTRIDE];
+      return mk<A>(k == RK_CP ? -ieq : ieq, 0.0);
+    }
+    default: {  // RK_IND: -(L/h) * i_prev (krets), minus v_prev for TRAP
+      int c = P.ec[e];
+      double r = -(sa.alpha * (pe[0] / sa.h)) * xp[c];
+      if (sa.alpha != 1.0) r -= ed[e * ESTRIDE];
+      return mk<A>(r, 0.0);
+    }
+  }
+}
+
+// ------------------------------------------------------------- phases
+// All phase helpers are executed by every thread of the block; t is the
+// thread's index within its system. Work is skipped when `act` is false but
+// barriers are always reached.
+
+template <class F, class A, int TPS>
+__device__ void stamp_matrix(const PatDev& P, F* v, const double* prm, const double* eb, const StampArgs& sa,
+                             const double* ed, int t, bool act) {
+  if (act)
+    for (int s = t; s < P.nnz; s += TPS) {
+      A acc = mk<A>(0.0, 0.0);
+      for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) acc += mat_contrib<A>(P.scon[q], P, prm, eb, sa, ed);
+      v[s] = cvt<F>(acc);
+    }
+}
+
+template <class A, int TPS>
+__device__ void stamp_rhs(const PatDev& P, A* b, const double* prm, const StampArgs& sa, const double* ed,
+                          const double* xp, int t, bool act) {
+  if (act)
+    for (int i = t; i < P.n; i += TPS) {
+      A acc = mk<A>(0.0, 0.0);
+      for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) acc += rhs_contrib<A>(P.rcon[q], P, prm, sa, ed, xp);
+      b[i] = acc;
+    }
+}
+
+template <class F, int TPS>
+__device__ void factor(const PatDev& P, F* v, int t, bool act, int* stat) {
+  auto row = [&](int i) {
+    const int32_t* fp = P.fprog + P.fprog_ptr[i];
+    int nk = *fp++;
+    for (int kk = 0; kk < nk; kk++) {
+      int sik = fp[0], sdk = fp[1], nu = fp[2];
+      fp += 3;
+      F lik = v[sik] * v[sdk];
+      v[sik] = lik;
+      for (int u = 0; u < nu; u++) {
+        v[fp[0]] -= lik * v[fp[1]];
+        fp += 2;
+      }
+    }
+    F piv = v[P.diag[i]];
+    double a = absv(piv);
+    if (!(a > 0.0) || !finite_v(piv)) *stat = KG_DECOMPOSITION;
+    v[P.diag[i]] = rcp(piv);
+  };
+  if constexpr (TPS == 1) {
+    if (act)
+      for (int i = 0; i < P.n; i++) row(i);
+  } else {
+    for (int l = 0; l < P.nlev; l++) {
+      if (act)
+        for (int q = P.lvl_ptr[l] + t; q < P.lvl_ptr[l + 1]; q += TPS) row(P.lvl_rows[q]);
+      __syncthreads();
+    }
+  }
+}
+
+// Solves (LU) w = w in place.
+template <class F, int TPS>
+__device__ void lu_solve(const PatDev& P, const F* v, F* w, int t, bool act) {
+  auto fwd = [&](int i) {
+    F s = w[i];
+    for (int p = P.row_ptr[i]; p < P.diag[i]; p++) s -= v[p] * w[P.col[p]];
+    w[i] = s;
+  };
+  auto bwd = [&](int i) {
+    F s = w[i];
+    int d = P.diag[i];
+    for (int p = d + 1; p < P.row_ptr[i + 1]; p++) s -= v[p] * w[P.col[p]];
+    w[i] = s * v[d];
+  };
+  if constexpr (TPS == 1) {
+    if (act) {
+      for (int i = 0; i < P.n; i++) fwd(i);
+      for (int i = P.n - 1; i >= 0; i--) bwd(i);
+    }
+  } else {
+    for (int l = 0; l < P.nlev; l++) {
+      if (act)
+        for (int q = P.lvl_ptr[l] + t; q < P.lvl_ptr[l + 1]; q += TPS) fwd(P.lvl_rows[q]);
+      __syncthreads();
+    }
+    for (int l = 0; l < P.nblev; l++) {
+      if (act)
+        for (int q = P.blvl_ptr[l] + t; q < P.blvl_ptr[l + 1]; q += TPS) bwd(P.blvl_rows[q]);
+      __syncthreads();
+    }
+  }
+}
+
+// Solves (LU)^T w = w in place: U^T forward, then L^T backward.
+template <class F, int TPS>
+__device__ void lu_solve_t(const PatDev& P, const F* v, F* w, int t, bool act) {
+  auto fwd = [&](int i) {
+    F s = w[i];
+    for (int q = P.ut_ptr[i]; q < P.ut_ptr[i + 1]; q += 2) s -= v[P.ut[q + 1]] * w[P.ut[q]];
+    w[i] = s * v[P.diag[i]];
+  };
+  auto bwd = [&](int i) {
+    F s = w[i];
+    for (int q = P.lt_ptr[i]; q < P.lt_ptr[i + 1]; q += 2) s -= v[P.lt[q + 1]] * w[P.lt[q]];
+    w[i] = s;
+  };
+  if constexpr (TPS == 1) {
+    if (act) {
+      for (int i = 0; i < P.n; i++) fwd(i);
+      for (int i = P.n - 1; i >= 0; i--) bwd(i);
+    }
+  } else {
+    for (int l = 0; l < P.ntlev; l++) {
+      if (act)
+        for (int q = P.tlvl_ptr[l] + t; q < P.tlvl_ptr[l + 1]; q += TPS) fwd(P.tlvl_rows[q]);
+      __syncthreads();
+    }
+    for (int l = 0; l < P.ntblev; l++) {
+      if (act)
+        for (int q = P.tblvl_ptr[l] + t; q < P.tblvl_ptr[l + 1]; q += TPS) bwd(P.tblvl_rows[q]);
+      __syncthreads();
+    }
+  }
+}
+
+// Max-reduction of a non-negative double over the TPS threads of a system.
+template <int TPS>
+__device__ double sys_max(double v, double* red, int ls, int t) {
+  if constexpr (TPS == 1) {
+    return v;
+  } else {
+    red[ls * TPS + t] = v;
+    __syncthreads();
+    for (int s = TPS / 2; s > 0; s >>= 1) {
+      if (t < s) red[ls * TPS + t] = fmax(red[ls * TPS + t], red[ls * TPS + t + s]);
+      __syncthreads();
+    }
+    double r = red[ls * TPS];
+    __syncthreads();
+    return r;
+  }
+}
+
+// Full solve of A x = b given the factored v. FP64 (F == A): x := LU \ b.
+// Mixed: x := LU \ b followed by `refine` steps of FP64 iterative refinement
+// with A re-gathered in FP64. Returns the normwise size of the last
+// correction relative to x (0 for FP64).
+template <class F, class A, int TPS>
+__device__ double solve_refine(const PatDev& P, const F* v, const A* b, A* x, F* w, const double* prm,
+                               const double* eb, const StampArgs& sa, const double* ed, int refine, double* red,
+                               int ls, int t, bool act) {
+  if constexpr (std::is_same<F, A>::value) {
+    if (act)
+      for (int i = t; i < P.n; i += TPS) x[i] = b[i];
+    __syncthreads();
+    lu_solve<F, TPS>(P, v, x, t, act);
+    return 0.0;
+  } else {
+    if (act)
+      for (int i = t; i < P.n; i += TPS) w[i] = cvt<F>(b[i]);
+    __syncthreads();
+    lu_solve<F, TPS>(P, v, w, t, act);
+    if (act)
+      for (int i = t; i < P.n; i += TPS) x[i] = cvt<A>(w[i]);
+    __syncthreads();
+    double est = 0.0;
+    for (int it = 0; it < refine; it++) {
+      // r = b - A x in FP64 (A gathered in FP64)
+      double rmax = 0.0;
+      if (act)
+        for (int i = t; i < P.n; i += TPS) {
+          A r = b[i];
+          for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) {
+            int q0 = P.scon_ptr[p], q1 = P.scon_ptr[p + 1];
+            if (q0 == q1) continue;
+            A a = mk<A>(0.0, 0.0);
+            for (int q = q0; q < q1; q++) a += mat_contrib<A>(P.scon[q], P, prm, eb, sa, ed);
+            r -= a * x[P.col[p]];
+          }
+          rmax = fmax(rmax, absv(r));
+          w[i] = cvt<F>(r);  // provisional, rescaled below
+          // keep the unscaled residual in x's shadow: recompute after scaling
+        }
+      double s = sys_max<TPS>(rmax, red, ls, t);
+      // rescale: recompute residual into w with scaling 1/s (avoids FP32 range loss)
+      if (act && s > 0.0) {
+        double inv = 1.0 / s;
+        for (int i = t; i < P.n; i += TPS) {
+          A r = b[i];
+          for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) {
+            int q0 = P.scon_ptr[p], q1 = P.scon_ptr[p + 1];
+            if (q0 == q1) continue;
+            A a = mk<A>(0.0, 0.0);
+            for (int q = q0; q < q1; q++) a += mat_contrib<A>(P.scon[q], P, prm, eb, sa, ed);
+            r -= a * x[P.col[p]];
+          }
+          w[i] = cvt<F>(scal(inv, r));
+        }
+      }
+      __syncthreads();
+      if (s > 0.0) lu_solve<F, TPS>(P, v, w, t, act);
+      double dmax = 0.0, xmax = 0.0;
+      if (act && s > 0.0)
+        for (int i = t; i < P.n; i += TPS) {
+          A d = scal(s, cvt<A>(w[i]));
+          x[i] += d;
+          dmax = fmax(dmax, absv(d));
+          xmax = fmax(xmax, absv(x[i]));
+        }
+      __syncthreads();
+      double dm = sys_max<TPS>(dmax, red, ls, t);
+      double xm = sys_max<TPS>(xmax, red, ls, t);
+      est = xm > 0.0 ? dm / xm : 0.0;
+    }
+    return est;
+  }
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+template <class F, class A>
+struct SysMem {
+  F* v;
+  A* x;
+  A* b;
+  F* w;
+  double* g;   // Newton guess (FP64)
+  double* xp;  // previous solution (transient)
+  double* ed;  // element state
+  int* flags;  // [0] status [1] active [2] iterations [4] any diode limited
+};
+
+template <class F, class A>
+__device__ SysMem<F, A> carve(char* base, const Lay& L) {
+  SysMem<F, A> m;
+  char* p = base;
+  m.flags = (int*)p;
+  p += 64;
+  m.v = (F*)p;
+  p += lay_al(sizeof(F) * (size_t)L.nnz_max);
+  m.x = (A*)p;
+  p += lay_al(sizeof(A) * (size_t)L.n_max);
+  m.b = (A*)p;
+  p += lay_al(sizeof(A) * (size_t)L.n_max);
+  if (std::is_same<F, A>::value) {
+    m.w = (F*)m.x;
+  } else {
+    m.w = (F*)p;
+    p += lay_al(sizeof(F) * (size_t)L.n_max);
+  }
+  m.g = L.need_g ? (double*)p : nullptr;
+  if (L.need_g) p += lay_al(8 * (size_t)L.n_max);
+  m.xp = L.need_xp ? (double*)p : nullptr;
+  if (L.need_xp) p += lay_al(8 * (size_t)L.n_max);
+  m.ed = L.need_ed ? (double*)p : nullptr;
+  return m;
+}
+
+// Evaluates all diodes of a system at guess g (permuted numbering).
+// compat_first: krets' empty solution map on the first OP iteration.
+template <int TPS>
+__device__ void eval_diodes(const PatDev& P, const double* prm, const double* g, double* ed, bool compat,
+                            bool compat_first, int* any_limited, int t, bool act) {
+  if (!act || !P.has_diode) return;
+  for (int e = t; e < P.ne; e += TPS) {
+    if (P.etype[e] != KG_D) continue;
+    int p = P.ep[e], m = P.em[e];
+    double vp, vm;
+    if (compat) {  // diode.rs:26-36: absent key -> 0.5 / 0.0
+      vp = (compat_first || p < 0) ? 0.5 : g[p];
+      vm = (compat_first || m < 0) ? 0.0 : g[m];
+    } else {
+      vp = p >= 0 ? g[p] : 0.0;
+      vm = m >= 0 ? g[m] : 0.0;
+    }
+    const double* pe = prm + e * KG_NPARAM;
+    DiodeLin d = diode_eval(vp - vm, pe[0], pe[1], compat, ed[e * ESTRIDE + 2]);
+    ed[e * ESTRIDE] = d.g;
+    ed[e * ESTRIDE + 1] = d.ieq;
+    ed[e * ESTRIDE + 2] = d.vd;
+    if (d.limited) *any_limited = 1;
+  }
+}
+
+// Newton convergence test. compat: solver.rs:76-104 (an empty previous map
+// never converges; the bookkeeping key takes part with the voltage
+// tolerance). xnew/gold are permuted-numbering solution and guess.
+template <int TPS>
+__device__ bool newton_converged(const PatDev& P, const double* xnew, const double* gold, double reltol,
+                                 double iabstol, double vabstol, double* red, int ls, int t, bool act) {
+  double bad = 0.0;
+  if (act)
+    for (int i = t; i < P.n; i += TPS) {
+      double a = xnew[i], b = gold[i];
+      double diff = fabs(a - b), scale = fmax(fabs(a), fabs(b));
+      double atol = P.isbr[i] ? iabstol : vabstol;
+      if (!(diff <= reltol * scale + atol)) bad = 1.0;
+    }
+  return sys_max<TPS>(bad, red, ls, t) == 0.0;
+}
+
+// ----------------------------------------------------------- kernels
+// OP and DC sweep (solver/op.rs:15-86, solver/dc.rs:14-124); one system per
+// instance, sweep points and Newton iterations inside the kernel.
+// out: [instance][npts][n] (krets numbering). FP64.
+template <int TPS>
+__global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_local_e, double start, double step,
+                            int npts, double* out, const int64_t* out_off, int32_t* status, int32_t* iters,
+                            Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int inst = blockIdx.x * spb + ls;
+  const bool valid = inst < B.n_instances;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + (valid ? inst : 0)]];
+  size_t sb = sys_bytes(lay);
+  double* red = (double*)(smem + spb * sb);
+  SysMem<double, double> M = carve<double, double>(smem + ls * sb, lay);
+  const double* prm = B.params + (valid ? B.eoff[inst] : 0) * KG_NPARAM;
+  const double* eb = B.ebase + (valid ? B.eoff[inst] : 0);
+  if (t == 0) {
+    M.flags[0] = KG_OK;
+    M.flags[1] = valid ? 1 : 0;
+    M.flags[2] = 0;
+  }
+  __syncthreads();
+  StampArgs sa{AK_DC, 0.0, 1.0, 1.0, 0.0, sweep_local_e, 0.0};
+  bool have_last = false;  // dc.rs: warm start from the previous point
+  for (int pt = 0; pt < npts; pt++) {
+    sa.sweep_val = start + (double)pt * step;
+    bool act = M.flags[1] != 0;
+    bool prev_empty = na.compat ? !have_last : false;
+    if (act && !na.compat && !have_last)
+      for (int i = t; i < P.n; i += TPS) M.g[i] = 0.0;
+    if (act)
+      for (int e = t; e < P.ne; e += TPS) M.ed[e * ESTRIDE + 2] = 0.0;
+    __syncthreads();
+    if (act && !na.compat)
+      for (int e = t; e < P.ne; e += TPS)
+        if (P.etype[e] == KG_D) {
+          int p = P.ep[e], m = P.em[e];
+          M.ed[e * ESTRIDE + 2] = (p >= 0 ? M.g[p] : 0.0) - (m >= 0 ? M.g[m] : 0.0);
+        }
+    __syncthreads();
+    int it = 0;
+    bool done = !act;
+    for (it = 0; it < na.maxiter; it++) {
+      bool a2 = !done;
+      if (t == 0) M.flags[4] = 0;
+      __syncthreads();
+      eval_diodes<TPS>(P, prm, M.g, M.ed, na.compat != 0, prev_empty, &M.flags[4], t, a2);
+      __syncthreads();
+      stamp_matrix<double, double, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
+      stamp_rhs<double, TPS>(P, M.b, prm, sa, M.ed, nullptr, t, a2);
+      __syncthreads();
+      factor<double, TPS>(P, M.v, t, a2, &M.flags[0]);
+      solve_refine<double, double, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, 0, red, ls, t, a2);
+      __syncthreads();
+      bool stop;
+      if (!P.nonlinear) {
+        stop = true;
+      } else if (na.compat && prev_empty) {
+        stop = false;
+        sys_max<TPS>(0.0, red, ls, t);
+      } else {
+        bool cv = newton_converged<TPS>(P, M.x, M.g, na.reltol, na.iabstol, na.vabstol, red, ls, t, a2);
+        if (na.compat && it == 0 && pt > 0 && sweep_local_e >= 0) {
+          // the "step" key: i vs i-1 on the first iteration of a point
+          double v1 = (double)pt, v0 = (double)(pt - 1);
+          cv = cv && fabs(v1 - v0) <= na.reltol * fmax(fabs(v1), fabs(v0)) + na.vabstol;
+        }
+        if (!na.compat && M.flags[4]) cv = false;
+        stop = cv;
+      }
+      if (a2 && M.flags[0] != KG_OK) stop = true;
+      // guess <- x for the next iteration
+      if (a2 && !stop)
+        for (int i = t; i < P.n; i += TPS) M.g[i] = M.x[i];
+      prev_empty = false;
+      __syncthreads();
+      if (a2 && stop) {
+        done = true;
+        if (t == 0) M.flags[2] = it + 1;
+      }
+      int alldone = __syncthreads_and(done ? 1 : 0);
+      if (alldone) break;
+    }
+    if (act && !done && t == 0) {
+      M.flags[0] = KG_MAX_ITERATIONS;
+      M.flags[2] = na.maxiter;
+    }
+    __syncthreads();
+    if (act) {
+      double* o = out + out_off[inst] + (int64_t)pt * P.n;
+      for (int j = t; j < P.n; j += TPS) o[P.col_of[j]] = M.x[j];
+      for (int j = t; j < P.n; j += TPS) M.g[j] = M.x[j];
+      if (t == 0 && iters) iters[(int64_t)inst * npts + pt] = M.flags[2];
+    }
+    have_last = true;
+    __syncthreads();
+    if (t == 0 && M.flags[0] != KG_OK) M.flags[1] = 0;
+    __syncthreads();
+  }
+  if (valid && t == 0) status[inst] = M.flags[0];
+}
+
+// AC sweep (solver/ac.rs:17-109). One system per (instance, frequency).
+// xop: operating points [instance][n] (krets numbering) for diode small-signal
+// conductances. out: complex128 [instance][nf][n].
+template <class F, class A, int TPS>
+__global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
+                     const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status, double* err,
+                     int refine, Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int64_t sys = (int64_t)blockIdx.x * spb + ls;
+  const int64_t nsys = (int64_t)B.n_instances * nf;
+  const bool valid = sys < nsys;
+  const int inst = valid ? (int)(sys / nf) : 0;
+  const int fi = valid ? (int)(sys % nf) : 0;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+  size_t sb = sys_bytes(lay);
+  double* red = (double*)(smem + spb * sb);
+  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[inst];
+  if (t == 0) M.flags[0] = KG_OK;
+  // diode small-signal conductance at the operating point (stampable.rs:614-641)
+  if (valid && P.has_diode)
+    for (int e = t; e < P.ne; e += TPS)
+      if (P.etype[e] == KG_D) {
+        const double* x0 = xop + xop_off[inst];
+        int p = P.ep[e], m = P.em[e];
+        double vp = p >= 0 ? x0[P.col_of[p]] : 0.5, vm = m >= 0 ? x0[P.col_of[m]] : 0.0;
+        DiodeLin d = diode_eval(vp - vm, prm[e * KG_NPARAM], prm[e * KG_NPARAM + 1], true, 0.0);
+        M.ed[e * ESTRIDE] = d.g;
+      }
+  __syncthreads();
+  StampArgs sa{AK_AC, valid ? freqs[fi] : 1.0, 1.0, 1.0, 0.0, -1, 0.0};
+  stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, valid);
+  stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, nullptr, t, valid);
+  __syncthreads();
+  factor<F, TPS>(P, M.v, t, valid, &M.flags[0]);
+  double est = solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, refine, red, ls, t, valid);
+  __syncthreads();
+  if (valid) {
+    double* o = out + out_off[inst] + (int64_t)fi * P.n * 2;
+    for (int j = t; j < P.n; j += TPS) {
+      int c = P.col_of[j];
+      o[2 * c] = re_of(M.x[j]);
+      o[2 * c + 1] = im_of(M.x[j]);
+    }
+    if (t == 0) {
+      if (M.flags[0] != KG_OK) atomicMax(&status[inst], M.flags[0]);
+      if (err) atomic_max_nonneg(&err[inst], est);
+    }
+  }
+}
+
+// Fixed-step transient (solver/transient.rs:14-114 and trapezoidal), one
+// persistent launch: every instance runs all steps in-kernel. Linear
+// circuits factor once. xop: OP per instance. out: [instance][nsteps+1][n].
+template <class F, class A, int TPS>
+__global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, const double* xop,
+                       const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status,
+                       int32_t* iters, double* err, Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int inst = blockIdx.x * spb + ls;
+  const bool valid = inst < B.n_instances;
+  const int ii = valid ? inst : 0;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  size_t sb = sys_bytes(lay);
+  double* red = (double*)(smem + spb * sb);
+  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[ii];
+  const bool compat = ta.method == KG_BE_COMPAT;
+  const bool trap = ta.method == KG_TRAP;
+  const double* x0 = xop + xop_off[ii];
+  if (t == 0) {
+    M.flags[0] = valid ? status[inst] : KG_OK;
+    M.flags[1] = (valid && M.flags[0] == KG_OK) ? 1 : 0;
+  }
+  __syncthreads();
+  bool act = M.flags[1] != 0;
+  // initial state: OP solution, capacitor currents 0, inductor voltages v(x0)
+  if (act) {
+    for (int j = t; j < P.n; j += TPS) M.xp[j] = x0[P.col_of[j]];
+    double* o = out + out_off[ii];
+    for (int j = t; j < P.n; j += TPS) o[j] = x0[j];
+    if (t == 0 && iters) iters[(int64_t)ii * (ta.nsteps + 1)] = 0;
+  }
+  __syncthreads();
+  if (act)
+    for (int e = t; e < P.ne; e += TPS) {
+      int p = P.ep[e], m = P.em[e];
+      double vd = (p >= 0 ? M.xp[p] : 0.0) - (m >= 0 ? M.xp[m] : 0.0);
+      M.ed[e * ESTRIDE] = P.etype[e] == KG_L ? vd : 0.0;
+      M.ed[e * ESTRIDE + 2] = P.etype[e] == KG_D ? vd : 0.0;
+    }
+  __syncthreads();
+  StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
+  const bool linear = !P.nonlinear;
+  if (linear) {
+    stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, act);
+    __syncthreads();
+    factor<F, TPS>(P, M.v, t, act, &M.flags[0]);
+    __syncthreads();
+  }
+  double est_max = 0.0;
+  for (int s = 1; s <= ta.nsteps; s++) {
+    act = M.flags[1] != 0 && M.flags[0] == KG_OK;
+    const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
+    int used = 1;
+    if (linear) {
+      sa.tsrc = compat ? tprev : tn;  // krets evaluates sources at the guess's time
+      stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, act);
+      __syncthreads();
+      double est =
+          solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, act);
+      est_max = fmax(est_max, est);
+    } else {
+      if (act)
+        for (int i = t; i < P.n; i += TPS) M.g[i] = M.xp[i];
+      // clean Newton starts pnjlim from the junction voltages of the guess
+      if (act && !compat)
+        for (int e = t; e < P.ne; e += TPS)
+          if (P.etype[e] == KG_D) {
+            int p = P.ep[e], m = P.em[e];
+            M.ed[e * ESTRIDE + 2] = (p >= 0 ? M.xp[p] : 0.0) - (m >= 0 ? M.xp[m] : 0.0);
+          }
+      __syncthreads();
+      bool done = !act;
+      int it;
+      for (it = 0; it < na.maxiter; it++) {
+        bool a2 = !done;
+        sa.tsrc = (compat && it == 0) ? tprev : tn;
+        if (t == 0) M.flags[4] = 0;
+        __syncthreads();
+        eval_diodes<TPS>(P, prm, M.g, M.ed, compat, false, &M.flags[4], t, a2);
+        __syncthreads();
+        stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
+        stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, a2);
+        __syncthreads();
+        factor<F, TPS>(P, M.v, t, a2, &M.flags[0]);
+        double est =
+            solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, a2);
+        est_max = fmax(est_max, est);
+        __syncthreads();
+        // convergence against the guess (FP64 copies)
+        double bad = 0.0;
+        if (a2)
+          for (int i = t; i < P.n; i += TPS) {
+            double a = re_of(M.x[i]), b = M.g[i];
+            double diff = fabs(a - b), scale = fmax(fabs(a), fabs(b));
+            double atol = P.isbr[i] ? na.iabstol : na.vabstol;
+            if (!(diff <= na.reltol * scale + atol)) bad = 1.0;
+          }
+        bool cv = sys_max<TPS>(bad, red, ls, t) == 0.0;
+        if (compat && it == 0)  // transient.rs:204, time key t_n vs t_{n-1}
+          cv = cv && fabs(tn - tprev) <= na.reltol * fmax(fabs(tn), fabs(tprev)) + na.vabstol;
+        if (!compat && M.flags[4]) cv = false;
+        bool stop = cv || M.flags[0] != KG_OK;
+        if (a2 && !stop)
+          for (int i = t; i < P.n; i += TPS) M.g[i] = re_of(M.x[i]);
+        __syncthreads();
+        if (a2 && stop) {
+          done = true;
+          used = it + 1;
+        }
+        if (__syncthreads_and(done ? 1 : 0)) break;
+      }
+      if (act && !done) {
+        if (t == 0) M.flags[0] = KG_MAX_ITERATIONS;
+        used = na.maxiter;
+      }
+      __syncthreads();
+      act = act && M.flags[0] == KG_OK;
+    }
+    // trapezoidal state update, store, advance
+    if (act && trap)
+      for (int e = t; e < P.ne; e += TPS) {
+        int ty = P.etype[e];
+        if (ty != KG_C && ty != KG_L) continue;
+        int p = P.ep[e], m = P.em[e];
+        double vn = (p >= 0 ? re_of(M.x[p]) : 0.0) - (m >= 0 ? re_of(M.x[m]) : 0.0);
+        if (ty == KG_C) {
+          double vo = (p >= 0 ? M.xp[p] : 0.0) - (m >= 0 ? M.xp[m] : 0.0);
+          M.ed[e * ESTRIDE] = 2.0 * prm[e * KG_NPARAM] / ta.h * (vn - vo) - M.ed[e * ESTRIDE];
+        } else {
+          M.ed[e * ESTRIDE] = vn;
+        }
+      }
+    __syncthreads();
+    if (act) {
+      double* o = out + out_off[ii] + (int64_t)s * P.n;
+      for (int j = t; j < P.n; j += TPS) {
+        double v = re_of(M.x[j]);
+        o[P.col_of[j]] = v;
+        M.xp[j] = v;
+      }
+      if (t == 0 && iters) iters[(int64_t)ii * (ta.nsteps + 1) + s] = used;
+    }
+    __syncthreads();
+  }
+  if (valid && t == 0) {
+    status[inst] = M.flags[0];
+    if (err) err[inst] = est_max;
+  }
+}
+
+// ------------------------------------------------------- launch helpers
+template <class K, class... Args>
+hipError_t launch(K fn, dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args) {
+  hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(fn, g, b, lds, st, args...);
+  return hipGetLastError();
+}
+
+#define KG_TPS_SWITCH(tps, CALL) \
+  switch (tps) {                 \
+    case 1: { constexpr int T = 1; return CALL; }     \
+    case 4: { constexpr int T = 4; return CALL; }     \
+    case 16: { constexpr int T = 16; return CALL; }   \
+    case 64: { constexpr int T = 64; return CALL; }   \
+    case 256: { constexpr int T = 256; return CALL; } \
+    default: return hipErrorInvalidValue;             \
+  }
+
+hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
+                      int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
+                      int32_t* status, int32_t* iters, Lay lay) {
+  KG_TPS_SWITCH(tps, launch(k_newton_dc<T>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out,
+                            out_off, status, iters, lay))
+}
+
+hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                  const double* freqs, int nf, const double* xop, const int64_t* xop_off, double* out,
+                  const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay) {
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_ac<c64, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out,
+                              out_off, status, err, refine, lay))
+  }
+  KG_TPS_SWITCH(tps, launch(k_ac<c128, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out, out_off,
+                            status, err, refine, lay))
+}
+
+hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                    NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
+                    const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay) {
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_tran<float, double, T>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
+                              out_off, status, iters, err, lay))
+  }
+  KG_TPS_SWITCH(tps, launch(k_tran<double, double, T>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
+                            out_off, status, iters, err, lay))
+}
+
+}  // namespace kg

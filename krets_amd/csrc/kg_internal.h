@@ -1,0 +1,67 @@
+// Internal definitions shared by the host symbolic pass and the gfx950 kernels.
+#pragma once
+#include <cstdint>
+
+#include "krets_gpu.h"
+
+namespace kg {
+
+// Which of krets' stamp families a pattern is built for. DC covers op.rs and
+// dc.rs (capacitors filtered out, op.rs:20-24), AC covers ac.rs, TRAN covers
+// transient.rs (backward-Euler companions) and the trapezoidal additions.
+enum AnalysisKind : int32_t { AK_DC = 0, AK_AC = 1, AK_TRAN = 2, AK_COUNT = 3 };
+
+// Symmetric fill-reducing orders applied after the structural matching.
+enum Ordering : int32_t { ORD_NATURAL = 0, ORD_MINDEG = 1, ORD_NESTED = 2 };
+
+// Matrix contribution kinds gathered per LU slot. A contribution is encoded as
+// (element << 3) | kind.
+enum : int32_t {
+  CK_YP = 0,    // +y(e): 1/R, j*w*C, alpha*C/h, diode G
+  CK_YN = 1,    // -y(e)
+  CK_ONEP = 2,  // +1 (incidence, current-source diagonal, G2 capacitor AC diagonal)
+  CK_ONEN = 3,  // -1
+  CK_BR = 4     // branch diagonal: -j*w*L, -alpha*L/h, -R (G2 resistor)
+};
+
+// Right-hand-side contribution kinds gathered per permuted row.
+enum : int32_t {
+  RK_SRC = 0,  // independent source value (V dc/ac/waveform, I dc)
+  RK_DP = 1,   // -Ieq of a diode at its plus node
+  RK_DM = 2,   // +Ieq at its minus node
+  RK_CP = 3,   // +capacitor companion current at plus node
+  RK_CM = 4,   // -capacitor companion current at minus node
+  RK_IND = 5   // inductor companion on its branch row
+};
+
+// Layout of one pattern inside a flat int32 buffer. Offsets are in int32 units.
+struct PatLayout {
+  int32_t n, nnz, ne, nlev, nblev, ntlev, ntblev, nonlinear;
+  int32_t has_diode, has_dyn;  // diode present; C or L present (companions)
+  int32_t o_row_ptr, o_col, o_diag;
+  int32_t o_fprog_ptr, o_fprog;
+  int32_t o_lvl_ptr, o_lvl_rows, o_blvl_ptr, o_blvl_rows;
+  int32_t o_scon_ptr, o_scon, o_rcon_ptr, o_rcon;
+  int32_t o_col_of, o_cpos, o_isbr;
+  int32_t o_etype, o_ep, o_em, o_ec;
+  int32_t o_ut_ptr, o_ut, o_lt_ptr, o_lt;
+  int32_t o_tlvl_ptr, o_tlvl_rows, o_tblvl_ptr, o_tblvl_rows;
+  int32_t o_row_of;
+  int32_t status;  // KG_OK or KG_DECOMPOSITION (structurally singular)
+  int32_t total;   // buffer length
+};
+
+// Device view of one pattern (pointers into uploaded buffer).
+struct PatDev {
+  int32_t n, nnz, ne, nlev, nblev, ntlev, ntblev, nonlinear, has_diode, has_dyn;
+  const int32_t *row_ptr, *col, *diag;
+  const int32_t *fprog_ptr, *fprog;
+  const int32_t *lvl_ptr, *lvl_rows, *blvl_ptr, *blvl_rows;
+  const int32_t *scon_ptr, *scon, *rcon_ptr, *rcon;
+  const int32_t *col_of, *cpos, *isbr;
+  const int32_t *etype, *ep, *em, *ec;
+  const int32_t *ut_ptr, *ut, *lt_ptr, *lt;
+  const int32_t *tlvl_ptr, *tlvl_rows, *tblvl_ptr, *tblvl_rows;
+};
+
+}  // namespace kg

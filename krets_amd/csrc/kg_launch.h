@@ -1,0 +1,69 @@
+// Launch interface between the host API (api.cpp) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "kg_internal.h"
+
+namespace kg {
+
+// Element state stride in LDS (doubles per element): [0] diode G or
+// companion state (capacitor current / inductor voltage), [1] diode Ieq,
+// [2] diode junction voltage of the previous iterate.
+constexpr int ESTRIDE = 3;
+
+// Per-instance views.
+struct BatchDev {
+  const PatDev* pats;      // pattern table
+  const int32_t* inst_pat; // instance -> pattern index (per analysis kind table)
+  const double* params;    // [elements][KG_NPARAM]
+  const double* ebase;     // [elements]: 1/R for resistors, else param[0]
+  const int64_t* eoff;     // instance -> first element record
+  const int32_t* n_inst;   // instance -> unknown count
+  int32_t n_instances;
+};
+
+// LDS carve-up of one system. The same layout is computed on the host
+// (sys_bytes) to size the dynamic LDS of a launch.
+struct Lay {
+  int32_t nnz_max, n_max, ne_max;
+  int32_t need_g, need_xp, need_ed;
+  int32_t fsz, asz, mixed, pad;
+};
+
+__host__ __device__ inline size_t lay_al(size_t b) { return (b + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t sys_bytes(const Lay& L) {
+  size_t s = 64 + lay_al((size_t)L.fsz * L.nnz_max) + 2 * lay_al((size_t)L.asz * L.n_max);
+  if (L.mixed) s += lay_al((size_t)L.fsz * L.n_max);
+  if (L.need_g) s += lay_al(8 * (size_t)L.n_max);
+  if (L.need_xp) s += lay_al(8 * (size_t)L.n_max);
+  if (L.need_ed) s += lay_al(8 * (size_t)ESTRIDE * L.ne_max);
+  return s;
+}
+
+struct NewtonArgs {
+  double reltol, iabstol, vabstol;
+  int maxiter, compat;
+};
+
+struct TranArgs {
+  double h, alpha;
+  int nsteps, method;  // kg_method
+  int refine;
+};
+
+
+hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
+                      int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
+                      int32_t* status, int32_t* iters, Lay lay);
+hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                  const double* freqs, int nf, const double* xop, const int64_t* xop_off, double* out,
+                  const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay);
+hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                    NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
+                    const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay);
+
+}  // namespace kg

@@ -1,0 +1,455 @@
+// Netlist reader for the R/L/C/V/I/D subset of krets' grammar, producing the
+// element tables of the C ABI in krets' unknown numbering.
+//
+// Follows crates/krets-parser/src/parser.rs:27-154 (line handling, .control
+// and .subckt skipping, .model binding, index map at 105-125), the element
+// parsers in elements/{resistor,capacitor,inductor,current_source,diode,
+// voltage_source}.rs, utils.rs:32-78 (SI suffixes, value tokens) and
+// models.rs:40-95. Elements are matched in krets' `alt` order
+// (elements.rs:208-229) and, like krets, a line is not required to be
+// consumed entirely.
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "krets_gpu.h"
+
+namespace {
+
+struct Rec {
+  char letter;
+  std::string name, plus, minus, model;
+  double p[KG_NPARAM] = {0};
+  bool g2 = false;
+};
+
+struct Scan {
+  const std::string* s;
+  size_t i = 0;
+  bool end() const { return i >= s->size(); }
+  char at() const { return end() ? '\0' : (*s)[i]; }
+};
+
+bool blank(char c) { return c == ' ' || c == '\t'; }
+
+bool sp1(Scan& c) {
+  size_t j = c.i;
+  while (j < c.s->size() && blank((*c.s)[j])) j++;
+  if (j == c.i) return false;
+  c.i = j;
+  return true;
+}
+void sp0(Scan& c) {
+  while (!c.end() && blank(c.at())) c.i++;
+}
+bool word(Scan& c, const char* w, bool nocase) {
+  size_t n = std::strlen(w);
+  if (c.i + n > c.s->size()) return false;
+  for (size_t k = 0; k < n; k++) {
+    char a = (*c.s)[c.i + k], b = w[k];
+    if (nocase ? std::tolower((unsigned char)a) != std::tolower((unsigned char)b) : a != b) return false;
+  }
+  c.i += n;
+  return true;
+}
+bool ident(Scan& c, std::string& out) {
+  size_t j = c.i;
+  while (j < c.s->size() && (std::isalnum((unsigned char)(*c.s)[j]) || (*c.s)[j] == '_')) j++;
+  if (j == c.i) return false;
+  out = c.s->substr(c.i, j - c.i);
+  c.i = j;
+  return true;
+}
+
+// Rust f64::from_str: [+-](inf|infinity|nan|digits[.digits][e[+-]digits]|.digits[...])
+bool f64_text(const std::string& t, double& v) {
+  std::string u;
+  for (char ch : t) u.push_back((char)std::tolower((unsigned char)ch));
+  size_t k = (!u.empty() && (u[0] == '+' || u[0] == '-')) ? 1 : 0;
+  std::string r = u.substr(k);
+  if (r == "inf" || r == "infinity" || r == "nan") {
+    v = std::strtod(u.c_str(), nullptr);
+    return true;
+  }
+  size_t a = k, ndig = 0;
+  while (a < u.size() && std::isdigit((unsigned char)u[a])) a++, ndig++;
+  if (a < u.size() && u[a] == '.') {
+    a++;
+    while (a < u.size() && std::isdigit((unsigned char)u[a])) a++, ndig++;
+  }
+  if (!ndig) return false;
+  if (a < u.size() && u[a] == 'e') {
+    a++;
+    if (a < u.size() && (u[a] == '+' || u[a] == '-')) a++;
+    size_t e0 = a;
+    while (a < u.size() && std::isdigit((unsigned char)u[a])) a++;
+    if (a == e0) return false;
+  }
+  if (a != u.size()) return false;
+  v = std::strtod(u.c_str(), nullptr);
+  return true;
+}
+
+// utils.rs:32-64 (parse_value) applied to a value token (utils.rs:72-78)
+bool si_value(Scan& c, double& v) {
+  size_t j = c.i;
+  while (j < c.s->size() && !std::strchr(" \t\r\n()", (*c.s)[j])) j++;
+  if (j == c.i) return false;
+  std::string tok;
+  for (size_t k = c.i; k < j; k++) tok.push_back((char)std::toupper((unsigned char)(*c.s)[k]));
+  double mul = 1.0;
+  if (tok.size() >= 3 && tok.compare(tok.size() - 3, 3, "MEG") == 0) {
+    mul = 1e6;
+    tok.resize(tok.size() - 3);
+  } else if (!tok.empty()) {
+    static const char* suf = "FPNUMKGT";
+    static const double mult[] = {1e-15, 1e-12, 1e-9, 1e-6, 1e-3, 1e3, 1e9, 1e12};
+    const char* q = std::strchr(suf, tok.back());
+    if (q && *q) {
+      mul = mult[q - suf];
+      tok.pop_back();
+    }
+  }
+  double base;
+  if (!f64_text(tok, base)) return false;
+  v = base * mul;
+  c.i = j;
+  return true;
+}
+
+// nom::number::complete::double (used for V source dc/ac values).
+// 1 = parsed, 0 = no match, -1 = exponent without digits (hard error).
+int nom_float(Scan& c, double& v) {
+  const std::string& s = *c.s;
+  size_t j = c.i;
+  if (j < s.size() && (s[j] == '+' || s[j] == '-')) j++;
+  size_t d = j;
+  bool num = false;
+  while (j < s.size() && std::isdigit((unsigned char)s[j])) j++;
+  if (j > d) {
+    num = true;
+    if (j < s.size() && s[j] == '.') {
+      j++;
+      while (j < s.size() && std::isdigit((unsigned char)s[j])) j++;
+    }
+  } else if (j < s.size() && s[j] == '.' && j + 1 < s.size() && std::isdigit((unsigned char)s[j + 1])) {
+    num = true;
+    j++;
+    while (j < s.size() && std::isdigit((unsigned char)s[j])) j++;
+  }
+  if (num) {
+    if (j < s.size() && (s[j] == 'e' || s[j] == 'E')) {
+      size_t q = j + 1;
+      if (q < s.size() && (s[q] == '+' || s[q] == '-')) q++;
+      size_t e0 = q;
+      while (q < s.size() && std::isdigit((unsigned char)s[q])) q++;
+      if (q == e0) return -1;
+      j = q;
+    }
+    v = std::strtod(s.substr(c.i, j - c.i).c_str(), nullptr);
+    c.i = j;
+    return 1;
+  }
+  for (const char* w : {"nan", "infinity", "inf"}) {
+    Scan t = c;
+    if (word(t, w, true)) {
+      v = std::strtod(w, nullptr);
+      c.i = t.i;
+      return 1;
+    }
+  }
+  return 0;
+}
+
+bool head(Scan& c, char letter, Rec& r) {
+  char w[2] = {letter, 0};
+  r = Rec();
+  r.letter = letter;
+  return word(c, w, true) && ident(c, r.name) && sp1(c) && ident(c, r.plus) && sp1(c) && ident(c, r.minus);
+}
+
+bool parse_two_terminal(const std::string& line, char letter, Rec& r) {
+  Scan c{&line};
+  if (!head(c, letter, r) || !sp1(c) || !si_value(c, r.p[0])) return false;
+  if (letter == 'C') {
+    Scan d = c;
+    if (sp1(d) && word(d, "G2", true)) r.g2 = true;
+  }
+  return true;
+}
+
+bool parse_diode(const std::string& line, Rec& r) {
+  Scan c{&line};
+  if (!head(c, 'D', r)) return false;
+  Scan d = c;
+  std::string m;
+  r.model = (sp1(d) && ident(d, m)) ? m : "default";
+  return true;
+}
+
+bool parse_wave(Scan& c, const char* tag, int nv, double* out) {
+  Scan d = c;
+  if (!word(d, tag, true)) return false;
+  sp0(d);
+  if (!word(d, "(", false)) return false;
+  for (int k = 0; k < nv; k++) {
+    if (k == 0) sp0(d);
+    else if (!sp1(d)) return false;
+    if (!si_value(d, out[k])) return false;
+  }
+  sp0(d);
+  if (!word(d, ")", false)) return false;
+  c = d;
+  return true;
+}
+
+// voltage_source.rs:187-235: optional implicit dc, then dc/ac/pulse/sin blocks
+int parse_vsrc(const std::string& line, Rec& r) {
+  Scan c{&line};
+  if (!head(c, 'V', r)) return 0;
+  {
+    Scan d = c;
+    double v;
+    if (sp1(d)) {
+      int k = nom_float(d, v);
+      if (k < 0) return -1;
+      if (k > 0) {
+        r.p[0] = v;
+        c = d;
+      }
+    }
+  }
+  for (;;) {
+    Scan d = c;
+    if (!sp1(d)) break;
+    double v, w[7];
+    Scan a = d;
+    if (word(a, "dc", true) && sp1(a)) {
+      int k = nom_float(a, v);
+      if (k < 0) return -1;
+      if (k > 0) { r.p[0] = v; c = a; continue; }
+    }
+    a = d;
+    if (word(a, "ac", true) && sp1(a)) {
+      int k = nom_float(a, v);
+      if (k < 0) return -1;
+      if (k > 0) { r.p[1] = v; c = a; continue; }
+    }
+    a = d;
+    if (parse_wave(a, "pulse", 7, w)) {
+      r.p[2] = KG_WAVE_PULSE;
+      for (int k = 0; k < 7; k++) r.p[3 + k] = w[k];
+      c = a;
+      continue;
+    }
+    a = d;
+    if (parse_wave(a, "sin", 6, w)) {
+      r.p[2] = KG_WAVE_SIN;
+      for (int k = 0; k < 7; k++) r.p[3 + k] = k < 6 ? w[k] : 0.0;
+      c = a;
+      continue;
+    }
+    break;
+  }
+  return 1;
+}
+
+struct Model {
+  bool diode;
+  double is = 1e-12, n = 1.0;
+};
+
+bool parse_model(std::string line, std::string& name, Model& m) {
+  line = line.substr(0, line.find('%'));
+  size_t a = line.find_first_not_of(" \t\r\n"), b = line.find_last_not_of(" \t\r\n");
+  line = a == std::string::npos ? std::string() : line.substr(a, b - a + 1);
+  Scan c{&line};
+  if (!word(c, ".model", true) || !sp1(c) || !ident(c, name) || !sp1(c)) return false;
+  if (word(c, "NMOS", false) || word(c, "PMOS", false)) m.diode = false;
+  else if (word(c, "D", false)) m.diode = true;
+  else return false;
+  sp0(c);
+  word(c, "(", false);
+  for (;;) {
+    Scan d = c;
+    std::string k;
+    double v;
+    sp0(d);
+    if (!ident(d, k)) break;
+    sp0(d);
+    if (!word(d, "=", false)) break;
+    sp0(d);
+    if (!si_value(d, v)) break;
+    for (auto& ch : k) ch = (char)std::tolower((unsigned char)ch);
+    if (k == "is") m.is = v;
+    else if (k == "n") m.n = v;
+    c = d;
+  }
+  return true;
+}
+
+std::string low(const std::string& s) {
+  std::string r(s);
+  for (auto& ch : r) ch = (char)std::tolower((unsigned char)ch);
+  return r;
+}
+
+}  // namespace
+
+struct kg_circuit_impl {
+  std::vector<Rec> recs;
+  std::vector<kg_element> el;
+  std::vector<std::string> names;
+  std::vector<std::string> ids;
+};
+
+extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* err, int32_t errlen) {
+  auto fail = [&](int32_t code, const std::string& msg) {
+    if (err && errlen > 0) std::snprintf(err, errlen, "%s", msg.c_str());
+    return code;
+  };
+  if (!text || !out) return fail(KG_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  std::string src(text);
+  auto* C = new kg_circuit_impl();
+  std::map<std::string, Model> models;
+  bool ctl = false, sub = false;
+  size_t pos = 0;
+  int lineno = 0;
+  while (pos < src.size()) {
+    size_t e = src.find('\n', pos);
+    if (e == std::string::npos) e = src.size();
+    std::string line = src.substr(pos, e - pos);
+    pos = e + 1;
+    lineno++;
+    size_t a = line.find_first_not_of(" \t\r\n\v\f"), b = line.find_last_not_of(" \t\r\n\v\f");
+    line = a == std::string::npos ? std::string() : line.substr(a, b - a + 1);
+    if (line.empty() || line[0] == '%' || line[0] == '*') continue;
+    std::string lo = low(line);
+    if (lo.rfind(".control", 0) == 0) { ctl = true; continue; }
+    if (lo.rfind(".endc", 0) == 0) { ctl = false; continue; }
+    if (ctl) continue;
+    if (lo.rfind(".subckt", 0) == 0) { sub = true; continue; }
+    if (lo.rfind(".ends", 0) == 0) { sub = false; continue; }
+    if (sub) continue;
+    if (lo == ".end") continue;
+    if (lo.rfind(".model", 0) == 0) {
+      std::string nm;
+      Model m;
+      if (!parse_model(line, nm, m)) {
+        delete C;
+        return fail(KG_PARSE, "line " + std::to_string(lineno) + ": invalid .model card");
+      }
+      models[nm] = m;
+      continue;
+    }
+    Rec r;
+    int vs = 0;
+    bool ok = parse_two_terminal(line, 'R', r) || parse_two_terminal(line, 'C', r) ||
+              parse_two_terminal(line, 'L', r) || (vs = parse_vsrc(line, r)) == 1;
+    if (!ok && vs == 0) ok = parse_two_terminal(line, 'I', r) || parse_diode(line, r);
+    if (!ok) {
+      char f = (char)std::toupper((unsigned char)line[0]);
+      delete C;
+      if (f == 'Q' || f == 'M' || f == 'X')
+        return fail(KG_INVALID_ELEMENT_FORMAT,
+                    "line " + std::to_string(lineno) + ": BJT/MOSFET/subcircuit elements are not supported");
+      return fail(KG_PARSE, "line " + std::to_string(lineno) + ": cannot parse '" + line + "'");
+    }
+    C->recs.push_back(r);
+  }
+  if (C->recs.empty()) {
+    delete C;
+    return fail(KG_EMPTY_NETLIST, "empty netlist");
+  }
+  // parser.rs:105-125: branch current first (group 2), then new nodes in
+  // (plus, minus) order; ground "0" never gets an index.
+  std::map<std::string, int> idx;
+  std::set<std::string> seen;
+  int counter = 0;
+  auto g2 = [](const Rec& r) {
+    return r.letter == 'V' || r.letter == 'L' || r.letter == 'I' || ((r.letter == 'C' || r.letter == 'R') && r.g2);
+  };
+  for (auto& r : C->recs) {
+    std::string id = std::string(1, r.letter) + r.name;
+    C->ids.push_back(id);
+    if (g2(r)) {
+      std::string k = "I(" + id + ")";
+      if (idx.count(k)) {
+        delete C;
+        return fail(KG_UNEXPECTED, "duplicate element identifier " + id);
+      }
+      idx[k] = counter++;
+    }
+    for (const std::string* nd : {&r.plus, &r.minus})
+      if (seen.insert(*nd).second && *nd != "0") idx["V(" + *nd + ")"] = counter++;
+  }
+  C->names.assign(counter, "");
+  for (auto& kv : idx) C->names[kv.second] = kv.first;
+  for (size_t k = 0; k < C->recs.size(); k++) {
+    Rec& r = C->recs[k];
+    if (r.letter == 'D') {  // parser.rs:131-140: a diode needs its .model
+      auto it = models.find(r.model);
+      if (it == models.end() || !it->second.diode) {
+        delete C;
+        return fail(KG_UNEXPECTED, "diode " + C->ids[k] + " refers to missing .model '" + r.model + "'");
+      }
+      r.p[0] = it->second.is;
+      r.p[1] = it->second.n;
+    }
+    kg_element e{};
+    static const char* L = "RCLVID";
+    e.type = (int32_t)(std::strchr(L, r.letter) - L);
+    auto node = [&](const std::string& nd) {
+      auto it = idx.find("V(" + nd + ")");
+      return it == idx.end() ? -1 : it->second;
+    };
+    e.plus = node(r.plus);
+    e.minus = node(r.minus);
+    auto it = idx.find("I(" + C->ids[k] + ")");
+    e.branch = it == idx.end() ? -1 : it->second;
+    e.g2 = r.g2 ? 1 : 0;
+    C->el.push_back(e);
+  }
+  *out = reinterpret_cast<kg_circuit*>(C);
+  return KG_OK;
+}
+
+extern "C" void kg_circuit_free(kg_circuit* c) { delete reinterpret_cast<kg_circuit_impl*>(c); }
+
+extern "C" int32_t kg_circuit_size(const kg_circuit* c, int32_t* n, int32_t* ne) {
+  if (!c) return KG_INVALID_ARGUMENT;
+  auto* C = reinterpret_cast<const kg_circuit_impl*>(c);
+  if (n) *n = (int32_t)C->names.size();
+  if (ne) *ne = (int32_t)C->el.size();
+  return KG_OK;
+}
+
+extern "C" const char* kg_circuit_unknown_name(const kg_circuit* c, int32_t i) {
+  auto* C = reinterpret_cast<const kg_circuit_impl*>(c);
+  if (!C || i < 0 || i >= (int32_t)C->names.size()) return nullptr;
+  return C->names[i].c_str();
+}
+
+extern "C" const char* kg_circuit_element_id(const kg_circuit* c, int32_t e) {
+  auto* C = reinterpret_cast<const kg_circuit_impl*>(c);
+  if (!C || e < 0 || e >= (int32_t)C->ids.size()) return nullptr;
+  return C->ids[e].c_str();
+}
+
+extern "C" int32_t kg_circuit_tables(const kg_circuit* c, kg_element* el, double* params) {
+  auto* C = reinterpret_cast<const kg_circuit_impl*>(c);
+  if (!C) return KG_INVALID_ARGUMENT;
+  for (size_t k = 0; k < C->el.size(); k++) {
+    if (el) el[k] = C->el[k];
+    if (params)
+      for (int q = 0; q < KG_NPARAM; q++) params[k * KG_NPARAM + q] = C->recs[k].p[q];
+  }
+  return KG_OK;
+}

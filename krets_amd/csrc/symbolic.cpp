@@ -1,0 +1,545 @@
+// Host symbolic pass: run once per distinct circuit topology and analysis
+// kind. It replaces the per-solve symbolic work that krets asks of faer
+// (SparseColMat::try_new_from_triplets + sp_lu at solver/op.rs:48-51,
+// dc.rs:167-170, ac.rs:67-70, transient.rs:183-186) with a fixed plan:
+//
+//  1. the stamp pattern of krets' element stamps (stampable.rs:202-1050) for
+//     the analysis kind, with every entry tagged by the element contribution
+//     that lands there;
+//  2. a structural row matching so every pivot is structurally non-zero
+//     (MNA branch rows of V sources, DC inductors and current sources have a
+//     zero diagonal);
+//  3. a symmetric fill-reducing order of the matched matrix;
+//  4. the symbolic LU (fill) of the permuted matrix, the row-oriented factor
+//     program, and dependency levels for the factor, both triangular solves
+//     and both transposed solves;
+//  5. per-slot and per-row gather lists for stamping on the device.
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <queue>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "kg_internal.h"
+
+namespace kg {
+
+bool validate_plan(const PatLayout& L, const std::vector<int32_t>& b, std::string& err);
+
+namespace {
+
+struct Entry {
+  int r, c;
+  int32_t code;
+};
+
+std::vector<int> order_min_degree(int n, const std::vector<std::vector<int>>& adj0) {
+  std::vector<std::set<int>> adj(n);
+  for (int v = 0; v < n; v++) adj[v].insert(adj0[v].begin(), adj0[v].end());
+  std::set<std::pair<int, int>> pq;
+  for (int v = 0; v < n; v++) pq.insert({(int)adj[v].size(), v});
+  std::vector<int> order;
+  order.reserve(n);
+  while (!pq.empty()) {
+    int v = pq.begin()->second;
+    pq.erase(pq.begin());
+    order.push_back(v);
+    std::vector<int> nb(adj[v].begin(), adj[v].end());
+    for (int u : nb) {
+      pq.erase({(int)adj[u].size(), u});
+      adj[u].erase(v);
+      for (int w : nb)
+        if (w != u) adj[u].insert(w);
+      pq.insert({(int)adj[u].size(), u});
+    }
+    adj[v].clear();
+  }
+  return order;
+}
+
+// Nested dissection by recursive BFS level-structure bisection: the middle
+// level of a level structure rooted at a pseudo-peripheral vertex is the
+// separator. On the chain-like graphs of ladder circuits this yields an
+// elimination tree of logarithmic depth.
+std::vector<int> order_nested(int n, const std::vector<std::vector<int>>& adj) {
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<int> part(n, 0), dist(n, -1);
+  int next_id = 1;
+  std::function<void(std::vector<int>&)> rec = [&](std::vector<int>& verts) {
+    if (verts.empty()) return;
+    if (verts.size() <= 2) {
+      for (int v : verts) order.push_back(v);
+      return;
+    }
+    int id = next_id++;
+    for (int v : verts) part[v] = id;
+    // split into connected components
+    std::vector<std::vector<int>> comps;
+    for (int v : verts) dist[v] = -1;
+    for (int s : verts) {
+      if (dist[s] >= 0) continue;
+      std::vector<int> comp{s};
+      dist[s] = 0;
+      for (size_t h = 0; h < comp.size(); h++)
+        for (int w : adj[comp[h]])
+          if (part[w] == id && dist[w] < 0) {
+            dist[w] = 0;
+            comp.push_back(w);
+          }
+      comps.push_back(std::move(comp));
+    }
+    if (comps.size() > 1) {
+      for (auto& c : comps) rec(c);
+      return;
+    }
+    auto bfs = [&](int s, std::vector<int>& seq) {
+      for (int v : verts) dist[v] = -1;
+      seq.assign(1, s);
+      dist[s] = 0;
+      for (size_t h = 0; h < seq.size(); h++)
+        for (int w : adj[seq[h]])
+          if (part[w] == id && dist[w] < 0) {
+            dist[w] = dist[seq[h]] + 1;
+            seq.push_back(w);
+          }
+    };
+    std::vector<int> seq;
+    int start = *std::min_element(verts.begin(), verts.end());
+    bfs(start, seq);
+    bfs(seq.back(), seq);
+    bfs(seq.back(), seq);
+    int L = dist[seq.back()];
+    if (L < 2) {
+      for (int v : verts) order.push_back(v);
+      return;
+    }
+    int mid = L / 2;
+    std::vector<int> A, B, S;
+    for (int v : seq) {
+      if (dist[v] < mid) A.push_back(v);
+      else if (dist[v] > mid) B.push_back(v);
+      else S.push_back(v);
+    }
+    rec(A);
+    rec(B);
+    std::sort(S.begin(), S.end());
+    for (int v : S) order.push_back(v);
+  };
+  std::vector<int> all(n);
+  for (int i = 0; i < n; i++) all[i] = i;
+  rec(all);
+  return order;
+}
+
+}  // namespace
+
+// Builds the plan for one topology. Returns the packed int32 buffer; layout
+// and status are written to L. err receives a message on failure.
+std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, AnalysisKind ak, Ordering ordk,
+                                   PatLayout& L, std::string& err) {
+  std::memset(&L, 0, sizeof(L));
+  L.n = n;
+  L.ne = ne;
+  L.status = KG_OK;
+  std::vector<Entry> A;
+  std::vector<Entry> R;  // rhs: c unused
+  auto add = [&](int r, int c, int e, int k) {
+    if (r >= 0 && c >= 0) A.push_back({r, c, (e << 3) | k});
+  };
+  auto rhs = [&](int r, int e, int k) {
+    if (r >= 0) R.push_back({r, 0, (e << 3) | k});
+  };
+  auto four = [&](int p, int m, int e) {
+    add(p, p, e, CK_YP);
+    add(m, m, e, CK_YP);
+    if (p >= 0 && m >= 0) {
+      add(p, m, e, CK_YN);
+      add(m, p, e, CK_YN);
+    }
+  };
+  auto incidence = [&](int p, int m, int c, int e) {  // stampable.rs:961-983
+    add(p, c, e, CK_ONEP);
+    add(c, p, e, CK_ONEP);
+    add(m, c, e, CK_ONEN);
+    add(c, m, e, CK_ONEN);
+  };
+  bool has_dyn = false, has_diode = false;
+  for (int e = 0; e < ne; e++) {
+    const kg_element& x = el[e];
+    int p = x.plus, m = x.minus, c = x.branch;
+    if (p >= n || m >= n || c >= n) {
+      err = "element index out of range";
+      L.status = KG_INVALID_ARGUMENT;
+      return {};
+    }
+    switch (x.type) {
+      case KG_R:  // stampable.rs:202-297
+        if (x.g2) {
+          incidence(p, m, c, e);
+          add(c, c, e, CK_BR);
+        } else {
+          four(p, m, e);
+        }
+        break;
+      case KG_C:  // stampable.rs:360-506
+        if (ak == AK_AC) {
+          if (!x.g2) four(p, m, e);
+          else {
+            add(c, p, e, CK_YN);
+            add(c, m, e, CK_YP);
+            add(c, c, e, CK_ONEP);
+          }
+        } else if (ak == AK_TRAN) {
+          four(p, m, e);
+          rhs(p, e, RK_CP);
+          rhs(m, e, RK_CM);
+          has_dyn = true;
+        }
+        break;
+      case KG_L:  // stampable.rs:676-817
+        incidence(p, m, c, e);
+        if (ak != AK_DC) add(c, c, e, CK_BR);
+        if (ak == AK_TRAN) {
+          rhs(c, e, RK_IND);
+          has_dyn = true;
+        }
+        break;
+      case KG_V:  // stampable.rs:960-1050
+        incidence(p, m, c, e);
+        rhs(c, e, RK_SRC);
+        break;
+      case KG_I:  // stampable.rs:508-585
+        if (ak != AK_AC) add(c, c, e, CK_ONEP);
+        add(p, c, e, CK_ONEP);
+        add(m, c, e, CK_ONEN);
+        rhs(c, e, RK_SRC);
+        break;
+      case KG_D:  // stampable.rs:587-674
+        four(p, m, e);
+        if (ak != AK_AC) {
+          rhs(p, e, RK_DP);
+          rhs(m, e, RK_DM);
+        }
+        has_diode = true;
+        break;
+      default:
+        err = "unknown element type";
+        L.status = KG_INVALID_ELEMENT_FORMAT;
+        return {};
+    }
+  }
+  L.nonlinear = has_diode ? 1 : 0;
+  L.has_diode = has_diode ? 1 : 0;
+  L.has_dyn = has_dyn ? 1 : 0;
+
+  // unique pattern, row and column adjacency (original numbering)
+  std::vector<std::vector<int>> rows(n), cols(n);
+  {
+    std::vector<std::pair<int, int>> pr;
+    for (auto& a : A) pr.push_back({a.r, a.c});
+    std::sort(pr.begin(), pr.end());
+    pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+    for (auto& q : pr) {
+      rows[q.first].push_back(q.second);
+      cols[q.second].push_back(q.first);
+    }
+  }
+  // structural matching: diagonal first, then augmenting paths (MC21 style)
+  std::vector<int> mrow(n, -1), rowm(n, -1);
+  for (int c = 0; c < n; c++)
+    if (std::binary_search(rows[c].begin(), rows[c].end(), c)) {
+      mrow[c] = c;
+      rowm[c] = c;
+    }
+  std::vector<int> seen(n, -1);
+  std::function<bool(int, int)> augment = [&](int c, int stamp) -> bool {
+    for (int r : cols[c]) {
+      if (seen[r] == stamp) continue;
+      seen[r] = stamp;
+      if (rowm[r] < 0 || augment(rowm[r], stamp)) {
+        rowm[r] = c;
+        mrow[c] = r;
+        return true;
+      }
+    }
+    return false;
+  };
+  for (int c = 0; c < n; c++) {
+    if (mrow[c] >= 0) continue;
+    if (!augment(c, c)) {
+      L.status = KG_DECOMPOSITION;
+      err = "structurally singular MNA matrix (no pivot for unknown " + std::to_string(c) + ")";
+      return {};
+    }
+  }
+  // symmetric order of the matched matrix B(c, c') = A(mrow[c], c')
+  std::vector<std::vector<int>> adj(n);
+  for (int r = 0; r < n; r++)
+    for (int c2 : rows[r]) {
+      int u = rowm[r];
+      if (u != c2) {
+        adj[u].push_back(c2);
+        adj[c2].push_back(u);
+      }
+    }
+  for (auto& a : adj) {
+    std::sort(a.begin(), a.end());
+    a.erase(std::unique(a.begin(), a.end()), a.end());
+  }
+  std::vector<int> ord;
+  if (ordk == ORD_MINDEG) ord = order_min_degree(n, adj);
+  else if (ordk == ORD_NESTED) ord = order_nested(n, adj);
+  else {
+    ord.resize(n);
+    for (int i = 0; i < n; i++) ord[i] = i;
+  }
+  std::vector<int> cpos(n), rpos(n);
+  for (int j = 0; j < n; j++) cpos[ord[j]] = j;
+  for (int i = 0; i < n; i++) rpos[mrow[ord[i]]] = i;
+
+  // symbolic LU of M(i, j) = A(mrow[ord[i]], ord[j])
+  std::vector<std::vector<int>> mrows(n);
+  for (int r = 0; r < n; r++)
+    for (int c2 : rows[r]) mrows[rpos[r]].push_back(cpos[c2]);
+  std::vector<std::vector<int>> Lp(n), Up(n);
+  std::vector<std::vector<int>> full(n);
+  for (int i = 0; i < n; i++) {
+    std::set<int> S(mrows[i].begin(), mrows[i].end());
+    for (auto it = S.begin(); it != S.end() && *it < i; ++it) {
+      int k = *it;
+      for (int j : Up[k]) S.insert(j);
+    }
+    if (!S.count(i)) {
+      L.status = KG_DECOMPOSITION;
+      err = "internal: missing diagonal after matching";
+      return {};
+    }
+    for (int j : S) {
+      if (j < i) Lp[i].push_back(j);
+      else if (j > i) Up[i].push_back(j);
+    }
+    full[i].assign(S.begin(), S.end());
+  }
+  // CSR of L+U
+  std::vector<int32_t> row_ptr(n + 1, 0), col, diag(n);
+  for (int i = 0; i < n; i++) {
+    row_ptr[i + 1] = row_ptr[i] + (int)full[i].size();
+    for (int j : full[i]) {
+      if (j == i) diag[i] = (int32_t)col.size();
+      col.push_back(j);
+    }
+  }
+  int nnz = (int)col.size();
+  L.nnz = nnz;
+  auto slot = [&](int i, int j) -> int32_t {
+    auto b = col.begin() + row_ptr[i], e = col.begin() + row_ptr[i + 1];
+    auto it = std::lower_bound(b, e, j);
+    return (it != e && *it == j) ? (int32_t)(it - col.begin()) : -1;
+  };
+  // factor program (up-looking, row oriented)
+  std::vector<int32_t> fptr(n + 1, 0), fprog;
+  for (int i = 0; i < n; i++) {
+    fprog.push_back((int32_t)Lp[i].size());
+    for (int k : Lp[i]) {
+      fprog.push_back(slot(i, k));
+      fprog.push_back(diag[k]);
+      fprog.push_back((int32_t)Up[k].size());
+      for (int j : Up[k]) {
+        fprog.push_back(slot(i, j));
+        fprog.push_back(slot(k, j));
+      }
+    }
+    fptr[i + 1] = (int32_t)fprog.size();
+  }
+  // levels
+  auto group = [&](const std::vector<int>& lev, std::vector<int32_t>& ptr, std::vector<int32_t>& rws) {
+    int nl = 0;
+    for (int v : lev) nl = std::max(nl, v + 1);
+    ptr.assign(nl + 1, 0);
+    for (int v : lev) ptr[v + 1]++;
+    for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+    rws.assign(n, 0);
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; i++) rws[fill[lev[i]]++] = i;
+    return nl;
+  };
+  std::vector<int> lev(n, 0), blev(n, 0), tlev(n, 0), tblev(n, 0);
+  for (int i = 0; i < n; i++)
+    for (int k : Lp[i]) lev[i] = std::max(lev[i], lev[k] + 1);
+  for (int i = n - 1; i >= 0; i--)
+    for (int j : Up[i]) blev[i] = std::max(blev[i], blev[j] + 1);
+  // transposed structures: column lists of U (k < i) and L (j > i)
+  std::vector<std::vector<std::pair<int, int>>> ucol(n), lcol(n);
+  for (int i = 0; i < n; i++) {
+    for (int k : Lp[i]) lcol[k].push_back({i, slot(i, k)});
+    for (int j : Up[i]) ucol[j].push_back({i, slot(i, j)});
+  }
+  for (int i = 0; i < n; i++)
+    for (auto& kv : ucol[i]) tlev[i] = std::max(tlev[i], tlev[kv.first] + 1);
+  for (int i = n - 1; i >= 0; i--)
+    for (auto& kv : lcol[i]) tblev[i] = std::max(tblev[i], tblev[kv.first] + 1);
+  std::vector<int32_t> lvl_ptr, lvl_rows, blvl_ptr, blvl_rows, tlvl_ptr, tlvl_rows, tblvl_ptr, tblvl_rows;
+  L.nlev = group(lev, lvl_ptr, lvl_rows);
+  L.nblev = group(blev, blvl_ptr, blvl_rows);
+  L.ntlev = group(tlev, tlvl_ptr, tlvl_rows);
+  L.ntblev = group(tblev, tblvl_ptr, tblvl_rows);
+  std::vector<int32_t> ut_ptr(n + 1, 0), ut, lt_ptr(n + 1, 0), lt;
+  for (int i = 0; i < n; i++) {
+    for (auto& kv : ucol[i]) { ut.push_back(kv.first); ut.push_back(kv.second); }
+    ut_ptr[i + 1] = (int32_t)ut.size();
+    for (auto& kv : lcol[i]) { lt.push_back(kv.first); lt.push_back(kv.second); }
+    lt_ptr[i + 1] = (int32_t)lt.size();
+  }
+  // gather lists
+  std::vector<std::vector<int32_t>> sc(nnz), rc(n);
+  for (auto& a : A) {
+    int s = slot(rpos[a.r], cpos[a.c]);
+    sc[s].push_back(a.code);
+  }
+  for (auto& b : R) rc[rpos[b.r]].push_back(b.code);
+  std::vector<int32_t> scon_ptr(nnz + 1, 0), scon, rcon_ptr(n + 1, 0), rcon;
+  for (int s = 0; s < nnz; s++) {
+    scon.insert(scon.end(), sc[s].begin(), sc[s].end());
+    scon_ptr[s + 1] = (int32_t)scon.size();
+  }
+  for (int i = 0; i < n; i++) {
+    rcon.insert(rcon.end(), rc[i].begin(), rc[i].end());
+    rcon_ptr[i + 1] = (int32_t)rcon.size();
+  }
+  std::vector<int32_t> col_of(ord.begin(), ord.end()), cposv(cpos.begin(), cpos.end()), isbr(n, 0);
+  std::vector<int32_t> et(ne), ep(ne), em(ne), ec(ne);
+  for (int e = 0; e < ne; e++) {
+    et[e] = el[e].type;
+    ep[e] = el[e].plus >= 0 ? cpos[el[e].plus] : -1;
+    em[e] = el[e].minus >= 0 ? cpos[el[e].minus] : -1;
+    ec[e] = el[e].branch >= 0 ? cpos[el[e].branch] : -1;
+    if (el[e].branch >= 0) isbr[cpos[el[e].branch]] = 1;
+  }
+  // pack
+  std::vector<int32_t> buf;
+  auto put = [&](const std::vector<int32_t>& v) {
+    int32_t off = (int32_t)buf.size();
+    buf.insert(buf.end(), v.begin(), v.end());
+    while (buf.size() % 4) buf.push_back(0);
+    return off;
+  };
+  L.o_row_ptr = put(row_ptr);
+  L.o_col = put(col);
+  L.o_diag = put(diag);
+  L.o_fprog_ptr = put(fptr);
+  L.o_fprog = put(fprog);
+  L.o_lvl_ptr = put(lvl_ptr);
+  L.o_lvl_rows = put(lvl_rows);
+  L.o_blvl_ptr = put(blvl_ptr);
+  L.o_blvl_rows = put(blvl_rows);
+  L.o_scon_ptr = put(scon_ptr);
+  L.o_scon = put(scon);
+  L.o_rcon_ptr = put(rcon_ptr);
+  L.o_rcon = put(rcon);
+  L.o_col_of = put(col_of);
+  L.o_cpos = put(cposv);
+  L.o_isbr = put(isbr);
+  L.o_etype = put(et);
+  L.o_ep = put(ep);
+  L.o_em = put(em);
+  L.o_ec = put(ec);
+  L.o_ut_ptr = put(ut_ptr);
+  L.o_ut = put(ut);
+  L.o_lt_ptr = put(lt_ptr);
+  L.o_lt = put(lt);
+  L.o_tlvl_ptr = put(tlvl_ptr);
+  L.o_tlvl_rows = put(tlvl_rows);
+  L.o_tblvl_ptr = put(tblvl_ptr);
+  L.o_tblvl_rows = put(tblvl_rows);
+  std::vector<int32_t> row_of(n);
+  for (int i = 0; i < n; i++) row_of[i] = mrow[ord[i]];
+  L.o_row_of = put(row_of);
+  L.total = (int32_t)buf.size();
+  if (!validate_plan(L, buf, err)) {
+    L.status = KG_UNEXPECTED;
+    return {};
+  }
+  return buf;
+}
+
+// Checks every index a kernel will dereference before the plan is uploaded.
+bool validate_plan(const PatLayout& L, const std::vector<int32_t>& b, std::string& err) {
+  const int n = L.n, nnz = L.nnz, ne = L.ne;
+  auto in = [](int64_t v, int64_t lo, int64_t hi) { return v >= lo && v < hi; };
+  auto fail = [&](const char* what) {
+    err = std::string("plan validation failed: ") + what;
+    return false;
+  };
+  const int32_t* rp = b.data() + L.o_row_ptr;
+  const int32_t* col = b.data() + L.o_col;
+  const int32_t* dg = b.data() + L.o_diag;
+  if (rp[0] != 0 || rp[n] != nnz) return fail("row_ptr");
+  for (int i = 0; i < n; i++) {
+    if (rp[i + 1] < rp[i]) return fail("row_ptr order");
+    if (!in(dg[i], rp[i], rp[i + 1]) || col[dg[i]] != i) return fail("diag");
+    for (int p = rp[i]; p < rp[i + 1]; p++)
+      if (!in(col[p], 0, n) || (p > rp[i] && col[p] <= col[p - 1])) return fail("col");
+  }
+  const int32_t* fpp = b.data() + L.o_fprog_ptr;
+  const int32_t* fp = b.data() + L.o_fprog;
+  for (int i = 0; i < n; i++) {
+    int q = fpp[i], e = fpp[i + 1];
+    int nk = fp[q++];
+    for (int k = 0; k < nk; k++) {
+      if (q + 3 > e) return fail("fprog length");
+      if (!in(fp[q], 0, nnz) || !in(fp[q + 1], 0, nnz)) return fail("fprog slot");
+      int nu = fp[q + 2];
+      q += 3;
+      for (int u = 0; u < nu; u++, q += 2)
+        if (q + 2 > e || !in(fp[q], 0, nnz) || !in(fp[q + 1], 0, nnz)) return fail("fprog update");
+    }
+    if (q != e) return fail("fprog end");
+  }
+  auto levels = [&](int o_ptr, int o_rows, int nl) {
+    const int32_t* lp = b.data() + o_ptr;
+    const int32_t* lr = b.data() + o_rows;
+    if (lp[0] != 0 || lp[nl] != n) return false;
+    std::vector<char> seen(n, 0);
+    for (int q = 0; q < n; q++) {
+      if (!in(lr[q], 0, n) || seen[lr[q]]) return false;
+      seen[lr[q]] = 1;
+    }
+    return true;
+  };
+  if (!levels(L.o_lvl_ptr, L.o_lvl_rows, L.nlev) || !levels(L.o_blvl_ptr, L.o_blvl_rows, L.nblev) ||
+      !levels(L.o_tlvl_ptr, L.o_tlvl_rows, L.ntlev) || !levels(L.o_tblvl_ptr, L.o_tblvl_rows, L.ntblev))
+    return fail("levels");
+  const int32_t* scp = b.data() + L.o_scon_ptr;
+  const int32_t* sc = b.data() + L.o_scon;
+  for (int s = 0; s <= nnz; s++)
+    if (s > 0 && scp[s] < scp[s - 1]) return fail("scon_ptr");
+  for (int q = 0; q < scp[nnz]; q++)
+    if (!in(sc[q] >> 3, 0, ne) || (sc[q] & 7) > CK_BR) return fail("scon code");
+  const int32_t* rcp = b.data() + L.o_rcon_ptr;
+  const int32_t* rc = b.data() + L.o_rcon;
+  for (int q = 0; q < rcp[n]; q++)
+    if (!in(rc[q] >> 3, 0, ne) || (rc[q] & 7) > RK_IND) return fail("rcon code");
+  for (int j = 0; j < n; j++)
+    if (!in(b[L.o_col_of + j], 0, n) || !in(b[L.o_row_of + j], 0, n) || !in(b[L.o_cpos + j], 0, n))
+      return fail("permutation");
+  for (int e = 0; e < ne; e++) {
+    if (!in(b[L.o_etype + e], KG_R, KG_D + 1)) return fail("etype");
+    for (int o : {L.o_ep, L.o_em, L.o_ec})
+      if (!in(b[o + e], -1, n)) return fail("element index");
+    int ty = b[L.o_etype + e];
+    if ((ty == KG_V || ty == KG_L || ty == KG_I) && b[L.o_ec + e] < 0) return fail("branch index");
+  }
+  for (int o_ptr : {L.o_ut_ptr, L.o_lt_ptr}) {
+    const int32_t* pp = b.data() + o_ptr;
+    const int32_t* v = b.data() + (o_ptr == L.o_ut_ptr ? L.o_ut : L.o_lt);
+    for (int i = 0; i < n; i++)
+      for (int q = pp[i]; q < pp[i + 1]; q += 2)
+        if (!in(v[q], 0, n) || !in(v[q + 1], 0, nnz)) return fail("transposed lists");
+  }
+  return true;
+}
+
+}  // namespace kg

@@ -1,0 +1,369 @@
+"""Host-side mirror of krets' solver interface over the C ABI.
+
+`Solver(circuit, config).solve(analysis)` mirrors krets_solver::Solver
+(crates/krets-solver/src/solver.rs:17-53) and returns krets' result shapes
+(AnalysisResult, crates/krets-solver/src/lib.rs:13-30): dicts keyed by
+"V(node)" / "I(elem)" plus "step", "frequency" or "time". `Context` is the
+batched interface: many instances per call, numpy (host) or torch (device)
+buffers.
+"""
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import KretsGpuError, lib, ptr
+
+
+# ------------------------------------------------------------- netlists
+class Circuit:
+    """A parsed netlist in krets' numbering (parser.rs:27-154)."""
+
+    def __init__(self, text: str):
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        st = lib().kg_parse_netlist(text.encode(), ctypes.byref(h), err, 512)
+        if st != _lib.OK:
+            raise KretsGpuError(st, err.value.decode())
+        try:
+            n, ne = ctypes.c_int32(), ctypes.c_int32()
+            lib().kg_circuit_size(h, ctypes.byref(n), ctypes.byref(ne))
+            self.n, self.n_elements = n.value, ne.value
+            self.names = [lib().kg_circuit_unknown_name(h, i).decode() for i in range(self.n)]
+            self.element_ids = [lib().kg_circuit_element_id(h, e).decode() for e in range(self.n_elements)]
+            self.elements = np.zeros(self.n_elements, _lib.ELEMENT_DTYPE)
+            self.params = np.zeros((self.n_elements, _lib.NPARAM))
+            lib().kg_circuit_tables(h, ctypes.c_void_p(self.elements.ctypes.data),
+                                    self.params.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        finally:
+            lib().kg_circuit_free(h)
+        self.index_map = {k: i for i, k in enumerate(self.names)}
+
+    @classmethod
+    def from_file(cls, path):
+        with open(path) as f:
+            return cls(f.read())
+
+    def element_index(self, identifier: str) -> int:
+        try:
+            return self.element_ids.index(identifier)
+        except ValueError:
+            raise KretsGpuError(_lib.ELEMENT_NOT_FOUND, f"Element '{identifier}' not found in the netlist")
+
+
+# ------------------------------------------------------------- config / analyses
+@dataclass
+class SolverConfig:
+    """crates/krets-solver/src/config.rs:4-36 (plus backend options)."""
+    relative_tolerance: float = 1e-3
+    current_absolute_tolerance: float = 1e-12
+    voltage_absolute_tolerance: float = 1e-6
+    maximum_iterations: int = 300
+    compat: bool = True
+    threads_per_system: int = 0
+    refine_iters: int = 0
+
+    def to_c(self):
+        c = _lib.KgConfig()
+        c.reltol = self.relative_tolerance
+        c.iabstol = self.current_absolute_tolerance
+        c.vabstol = self.voltage_absolute_tolerance
+        c.maxiter = self.maximum_iterations
+        c.compat = 1 if self.compat else 0
+        c.threads_per_system = self.threads_per_system
+        c.refine_iters = self.refine_iters
+        return c
+
+
+@dataclass
+class Decade:
+    points_per_decade: int
+
+
+@dataclass
+class Octave:
+    points_per_octave: int
+
+
+@dataclass
+class Linear:
+    total_points: int
+
+
+@dataclass
+class Op:
+    pass
+
+
+@dataclass
+class Dc:
+    element: str
+    start: float
+    stop: float
+    step_size: float
+
+
+@dataclass
+class Ac:
+    sweep: Union[Decade, Octave, Linear]
+    fstart: float
+    fstop: float
+
+    def generate_frequencies(self) -> List[float]:
+        """AcAnalysis::generate_frequencies (krets-parser/src/analyses.rs:80-158)."""
+        f: List[float] = []
+        fstart, fstop = float(self.fstart), float(self.fstop)
+        if fstart <= 0.0 or fstop <= 0.0 or fstart > fstop:
+            return f
+        s = self.sweep
+        if isinstance(s, Linear):
+            if s.total_points == 1:
+                f.append(fstart)
+            elif s.total_points > 1:
+                step = (fstop - fstart) / float(s.total_points - 1)
+                f = [fstart + float(i) * step for i in range(s.total_points)]
+        else:
+            ppd = s.points_per_decade if isinstance(s, Decade) else s.points_per_octave
+            if ppd == 0:
+                return f
+            nd = math.log10(fstop / fstart) if isinstance(s, Decade) else math.log2(fstop / fstart)
+            total = int(_round_half_away(nd * float(ppd))) + 1
+            factor = (10.0 if isinstance(s, Decade) else 2.0) ** (1.0 / float(ppd))
+            cur = fstart
+            for _ in range(total):
+                if cur > fstop * (1.0 + 1e-9):
+                    break
+                f.append(cur)
+                cur *= factor
+                if cur > fstop and (not f or f[-1] < fstop):
+                    f.append(fstop)
+                    break
+            if not f or f[-1] < fstop * (1.0 - 1e-9):
+                f.append(fstop)
+        return sorted(set(f))
+
+
+@dataclass
+class Transient:
+    time_step: float
+    stop_time: float
+
+
+def _round_half_away(x):
+    return math.floor(x + 0.5) if x >= 0 else -math.floor(-x + 0.5)
+
+
+# ------------------------------------------------------------- batched context
+class Context:
+    """One kg_ctx: a HIP stream on one GPU plus the loaded batch."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        st = lib().kg_create(ctypes.byref(h), device)
+        if st != _lib.OK:
+            raise KretsGpuError(st, "kg_create failed (no usable HIP device?)")
+        self._h = h
+        self.n_instances = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, st):
+        if st != _lib.OK:
+            raise KretsGpuError(st, lib().kg_last_error(self._h).decode())
+
+    def load(self, elements, params, n_unknowns, elem_offset=None):
+        """Shared topology: elements [ne], params [ni, ne, NPARAM], n_unknowns int.
+        Per-instance topologies: elements [sum ne], params [sum ne, NPARAM],
+        n_unknowns [ni], elem_offset [ni+1]."""
+        el = np.ascontiguousarray(elements, _lib.ELEMENT_DTYPE)
+        pr = np.ascontiguousarray(params, np.float64)
+        b = _lib.KgBatch()
+        if elem_offset is None:
+            ni = pr.shape[0]
+            nu = np.array([n_unknowns], np.int32)
+            b.n_instances, b.shared_topology, b.n_elements = ni, 1, el.shape[0]
+            self.n_unknowns = np.full(ni, int(n_unknowns), np.int64)
+        else:
+            off = np.ascontiguousarray(elem_offset, np.int32)
+            nu = np.ascontiguousarray(n_unknowns, np.int32)
+            ni = len(nu)
+            b.n_instances, b.shared_topology, b.n_elements = ni, 0, int(off[-1])
+            b.elem_offset = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            self.n_unknowns = nu.astype(np.int64)
+            self._keep_off = off
+        b.n_unknowns = nu.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        b.elements = ctypes.c_void_p(el.ctypes.data)
+        b.params = pr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self._check(lib().kg_load_batch(self._h, ctypes.byref(b)))
+        self.n_instances = ni
+        self.shared = elem_offset is None
+        self.n_elements = el.shape[0] if elem_offset is None else np.diff(elem_offset)
+
+    def _split(self, flat, per):
+        if self.shared:
+            n = int(self.n_unknowns[0])
+            return flat.reshape((self.n_instances,) + per + (n,))
+        out, o = [], 0
+        for n in self.n_unknowns:
+            k = int(np.prod(per)) * int(n)
+            out.append(flat[o:o + k].reshape(per + (int(n),)))
+            o += k
+        return out
+
+    def op(self, config=None, out=None):
+        cfg = (config or SolverConfig()).to_c()
+        tot = int(self.n_unknowns.sum())
+        x = np.zeros(tot) if out is None else out
+        st = np.zeros(self.n_instances, np.int32)
+        it = np.zeros(self.n_instances, np.int32)
+        p, mem = ptr(x)
+        r = lib().kg_op(self._h, ctypes.byref(cfg), p, ctypes.c_void_p(st.ctypes.data),
+                        ctypes.c_void_p(it.ctypes.data), mem)
+        return self._result(r, x, (), st, extra=it)
+
+    def dc_sweep(self, element, start, stop, step, config=None):
+        cfg = (config or SolverConfig()).to_c()
+        npts = int(abs((stop - start) / step)) + 1
+        x = np.zeros(int(self.n_unknowns.sum()) * npts)
+        st = np.zeros(self.n_instances, np.int32)
+        it = np.zeros(self.n_instances * npts, np.int32)
+        n = ctypes.c_int32()
+        r = lib().kg_dc_sweep(self._h, ctypes.byref(cfg), element, start, stop, step, ctypes.c_void_p(x.ctypes.data),
+                              npts, ctypes.byref(n), ctypes.c_void_p(st.ctypes.data),
+                              ctypes.c_void_p(it.ctypes.data), _lib.MEM_HOST)
+        return self._result(r, x, (npts,), st, extra=it.reshape(self.n_instances, npts))
+
+    def ac_sweep(self, freqs, precision=_lib.FP64, config=None, out=None):
+        cfg = (config or SolverConfig()).to_c()
+        f = np.ascontiguousarray(freqs, np.float64)
+        nf = len(f)
+        tot = int(self.n_unknowns.sum()) * nf * 2
+        x = np.zeros(tot) if out is None else out
+        st = np.zeros(self.n_instances, np.int32)
+        err = np.zeros(self.n_instances)
+        p, mem = ptr(x)
+        r = lib().kg_ac_sweep(self._h, ctypes.byref(cfg), f.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nf,
+                              precision, p, ctypes.c_void_p(st.ctypes.data), ctypes.c_void_p(err.ctypes.data), mem)
+        if out is not None:
+            return self._raise_or(r, st), st, err
+        xc = x.view(np.complex128)
+        return self._result(r, xc, (nf,), st, extra=err)
+
+    def transient(self, h, nsteps, method=_lib.TRAP, precision=_lib.FP64, config=None, out=None):
+        cfg = (config or SolverConfig()).to_c()
+        tot = int(self.n_unknowns.sum()) * (nsteps + 1)
+        x = np.zeros(tot) if out is None else out
+        st = np.zeros(self.n_instances, np.int32)
+        it = np.zeros(self.n_instances * (nsteps + 1), np.int32)
+        err = np.zeros(self.n_instances)
+        p, mem = ptr(x)
+        r = lib().kg_transient(self._h, ctypes.byref(cfg), h, nsteps, method, precision, p,
+                               ctypes.c_void_p(st.ctypes.data), ctypes.c_void_p(it.ctypes.data),
+                               ctypes.c_void_p(err.ctypes.data), mem)
+        if out is not None:
+            return self._raise_or(r, st), st, err
+        return self._result(r, x, (nsteps + 1,), st, extra=(it.reshape(self.n_instances, nsteps + 1), err))
+
+    def transient_lte(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points, config=None):
+        cfg = (config or SolverConfig(compat=False)).to_c()
+        ni = self.n_instances
+        t = np.zeros(ni * max_points)
+        x = np.zeros(int(self.n_unknowns.sum()) * max_points)
+        npts = np.zeros(ni, np.int32)
+        nt = np.zeros(ni, np.int32)
+        rej = np.zeros(ni, np.int32)
+        st = np.zeros(ni, np.int32)
+        r = lib().kg_transient_lte(self._h, ctypes.byref(cfg), tstop, h0, hmin, hmax, lte_reltol, lte_abstol,
+                                   max_points, ctypes.c_void_p(t.ctypes.data), ctypes.c_void_p(x.ctypes.data),
+                                   ctypes.c_void_p(npts.ctypes.data), ctypes.c_void_p(nt.ctypes.data),
+                                   ctypes.c_void_p(rej.ctypes.data), ctypes.c_void_p(st.ctypes.data), _lib.MEM_HOST)
+        xs = self._result(r, x, (max_points,), st)
+        return t.reshape(ni, max_points), xs, npts, nt, rej
+
+    def adjoint(self, h, nsteps, cvec, method=_lib.TRAP, precision=_lib.FP64, checkpoint_every=64, config=None):
+        cfg = (config or SolverConfig(compat=False)).to_c()
+        ni = self.n_instances
+        c = np.ascontiguousarray(cvec, np.float64)
+        J = np.zeros(ni)
+        ne = int(np.max(self.n_elements)) if not self.shared else int(self.n_elements)
+        g = np.zeros(ni * ne if self.shared else int(np.sum(self.n_elements)))
+        st = np.zeros(ni, np.int32)
+        r = lib().kg_adjoint_transient(self._h, ctypes.byref(cfg), h, nsteps, method, precision,
+                                       c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), checkpoint_every,
+                                       ctypes.c_void_p(J.ctypes.data), ctypes.c_void_p(g.ctypes.data),
+                                       ctypes.c_void_p(st.ctypes.data), _lib.MEM_HOST)
+        self._raise_or(r, st)
+        return J, (g.reshape(ni, ne) if self.shared else g), st
+
+    def profile(self, enable=True):
+        """Returns (total_ms, launches, last_kernel) of the analysis kernels
+        timed since the previous call, then resets and sets timing on/off."""
+        ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
+        lib().kg_profile(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(name))
+        return ms.value, n.value, (name.value or b"").decode()
+
+    def _raise_or(self, r, st):
+        if r != _lib.OK and (st == _lib.OK).all():
+            self._check(r)
+        return r
+
+    def _result(self, r, x, per, st, extra=None):
+        self._raise_or(r, st)
+        return self._split(x, per), st, extra
+
+
+# ------------------------------------------------------------- krets mirror
+class Solver:
+    """krets_solver::Solver on the GPU backend (solver.rs:17-53)."""
+
+    def __init__(self, circuit: Circuit, config: Optional[SolverConfig] = None, device: int = 0,
+                 precision: int = _lib.FP64):
+        self.circuit = circuit
+        self.config = config or SolverConfig()
+        self.precision = precision
+        self._ctx = Context(device)
+        self._ctx.load(circuit.elements, circuit.params[None], circuit.n)
+
+    def _map(self, x, extra=None):
+        d = {k: x[i] for i, k in enumerate(self.circuit.names)}
+        if extra:
+            d.update(extra)
+        return d
+
+    def _one(self, st, r):
+        if st[0] != _lib.OK:
+            raise KretsGpuError(int(st[0]), lib().kg_last_error(self._ctx._h).decode())
+
+    def solve(self, analysis):
+        ctx = self._ctx
+        if isinstance(analysis, Op):
+            xs, st, _ = ctx.op(self.config)
+            self._one(st, None)
+            return self._map(xs[0])
+        if isinstance(analysis, Dc):
+            e = self.circuit.element_index(analysis.element)
+            xs, st, _ = ctx.dc_sweep(e, analysis.start, analysis.stop, analysis.step_size, self.config)
+            self._one(st, None)
+            return [self._map(v, {"step": float(i)}) for i, v in enumerate(xs[0])]
+        if isinstance(analysis, Ac):
+            f = [v for v in analysis.generate_frequencies() if v > 0.0]
+            if not f:
+                return []
+            xs, st, _ = ctx.ac_sweep(f, self.precision, self.config)
+            self._one(st, None)
+            return [self._map(v, {"frequency": complex(fr, 0.0)}) for fr, v in zip(f, xs[0])]
+        if isinstance(analysis, Transient):
+            n = int(_round_half_away(analysis.stop_time / analysis.time_step))
+            xs, st, _ = ctx.transient(analysis.time_step, n, _lib.BE_COMPAT, self.precision, self.config)
+            self._one(st, None)
+            return [self._map(v, {"time": float(i) * analysis.time_step}) for i, v in enumerate(xs[0])]
+        raise TypeError(f"unknown analysis {analysis!r}")

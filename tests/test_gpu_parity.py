@@ -1,0 +1,223 @@
+"""GPU parity: libkrets_gpu.so (HIP, gfx950) against the CPU oracle.
+
+Every case runs the HIP kernels through the C ABI and compares with the
+oracle (krets semantics) on the same inputs. Bounds are normwise relative
+per solve, ||x - x_ref||_2 / ||x_ref||_2: 1e-12 for the series+shunt ladder
+and linear OP cases, 1e-9 elsewhere (mixed precision included); the krets
+known-answer values are asserted at krets' own tolerances as well.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import read_circuit
+
+pytestmark = pytest.mark.gpu
+
+TPS = [1, 4, 16, 64, 256]
+
+
+def say(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def nerr(x, ref):
+    x, ref = np.asarray(x), np.asarray(ref)
+    d = np.linalg.norm(ref)
+    return np.linalg.norm(x - ref) / (d if d > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def kg():
+    import krets_amd
+    return krets_amd
+
+
+def gpu_op(kg, text, tps=0, compat=True):
+    c = kg.Circuit(text)
+    s = kg.Solver(c, kg.SolverConfig(threads_per_system=tps, compat=compat))
+    return c, s.solve(kg.Op())
+
+
+OP_CASES = [
+    "circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir",
+    "voltage_divider/voltage_divider.cir",
+    "low_pass_filter/low_pass_filter.cir",
+    "high_pass_filter/high_pass_filter.cir",
+    "basic_001/basic_001.cir",
+    "resistor_ladder_500/resistor_ladder_500.cir",
+    "diode_iv_curve/diode_iv_curve.cir",
+]
+
+
+@pytest.mark.parametrize("tps", TPS)
+@pytest.mark.parametrize("path", OP_CASES)
+def test_op_vs_oracle(kg, oracle, path, tps):
+    say(f"op {path} tps={tps}")
+    text = read_circuit(path)
+    c, r = gpu_op(kg, text, tps)
+    oc = oracle.Circuit(text)
+    xr, _ = oc.op()
+    x = np.array([r[k] for k in oc.names])
+    tol = 1e-9 if "diode" in path else 1e-12
+    assert nerr(x, xr) < tol
+
+
+def test_op_krets_known_answers(kg):
+    say("op known answers")
+    _, s = gpu_op(kg, read_circuit("circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir"))
+    exp = {"V(4)": 1.9888, "V(8)": 1.0, "V(3)": 2.00879, "V(2)": 1.80879, "V(6)": 1.98814, "V(5)": 2.0,
+           "V(1)": 1.88527, "V(7)": 3.98814, "I(V3)": 0.0, "I(V2)": -199.88e-3, "I(V1)": -198.88e-3}
+    for k, v in exp.items():  # test_op.rs:25-46
+        assert abs(s[k] - v) < 1e-3
+    _, s = gpu_op(kg, "V1 2 1 32\nR1 1 0 2\nR2 2 3 4\nR3 2 0 8\nV2 3 0 20\n")  # test_op.rs:48-68
+    for k, v in {"V(1)": -8.0, "V(2)": 24.0, "V(3)": 20.0, "I(V1)": -4.0, "I(V2)": 1.0}.items():
+        assert abs(s[k] - v) < 1e-12
+    _, s = gpu_op(kg, read_circuit("diode_iv_curve/diode_iv_curve.cir"))  # test_op.rs:131-140
+    assert abs(s["V(out)"] - 0.517) < 1e-3
+
+
+@pytest.mark.parametrize("tps", [1, 64, 256])
+def test_c1_series_shunt_ladder(kg, oracle, tps):
+    from krets_amd import workloads as W
+    say(f"C1 ladder tps={tps}")
+    text = W.c1_ladder()
+    c, r = gpu_op(kg, text, tps)
+    oc = oracle.Circuit(text)
+    xr, _ = oc.op()
+    x = np.array([r[k] for k in oc.names])
+    assert nerr(x, xr) < 1e-12
+    phi = (1 + math.sqrt(5)) / 2  # I(V1) = -phi mA (infinite-ladder closed form)
+    assert abs(r["I(V1)"] / (-phi * 1e-3) - 1) < 1e-12
+    text = read_circuit("resistor_ladder_1000/resistor_ladder_1000.cir")
+    c, r = gpu_op(kg, text, tps)
+    v = np.array([r[f"V({k})"] for k in range(1, 1001)])
+    assert np.max(np.abs(v - (1 - np.arange(1000) / 1000.0))) < 1e-12
+
+
+def test_dc_sweep(kg, oracle):
+    say("dc sweep divider + diode")
+    text = read_circuit("voltage_divider/voltage_divider.cir")
+    s = kg.Solver(kg.Circuit(text))
+    res = s.solve(kg.Dc("V1", 0.0, 1.0, 0.1))
+    xr = oracle.Circuit(text).dc("V1", 0.0, 1.0, 0.1)
+    names = oracle.Circuit(text).names
+    assert len(res) == xr.shape[0] == 11
+    for p, rr in enumerate(res):
+        assert nerr([rr[k] for k in names], xr[p]) < 1e-12
+        assert rr["step"] == float(p)
+    assert abs(res[-1]["I(V1)"] + 1 / 3000) < 1e-4  # test_dc.rs:53
+    text = read_circuit("diode_iv_curve/diode_iv_curve.cir")
+    oc = oracle.Circuit(text)
+    xr = oc.dc("V1", 0.0, 2.0, 0.25)
+    res = kg.Solver(kg.Circuit(text)).solve(kg.Dc("V1", 0.0, 2.0, 0.25))
+    for p, rr in enumerate(res):
+        assert nerr([rr[k] for k in oc.names], xr[p]) < 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ac_known_answers_and_golden(kg, oracle, precision):
+    import json, os
+    from conftest import GOLDEN
+    say(f"ac golden precision={precision}")
+    s = kg.Solver(kg.Circuit(read_circuit("low_pass_filter/low_pass_filter.cir")), precision=precision)
+    r = s.solve(kg.Ac(kg.Linear(1), 1000.0, 1000.0))[0]
+    assert abs(r["V(out)"] - (2.470452e-02 - 1.55223e-01j)) < 1e-6  # test_ac.rs:25-46
+    s = kg.Solver(kg.Circuit(read_circuit("high_pass_filter/high_pass_filter.cir")), precision=precision)
+    r = s.solve(kg.Ac(kg.Linear(1), 1000.0, 1000.0))[0]
+    assert r["frequency"] == 1000.0
+    assert abs(r["V(out)"] - (7.169568e-01 - 4.50477e-01j)) < 1e-6  # test_ac.rs:48-74
+    assert abs(r["I(V1)"] - (-7.16957e-03 + 4.504772e-03j)) < 1e-8
+    g = json.load(open(os.path.join(GOLDEN, "low_pass_ac_result.json")))
+    cols = {k: np.array([float(v) for v in vals]) for k, vals in g["columns"].items()}
+    s = kg.Solver(kg.Circuit(read_circuit("low_pass_filter/ac/ac.cir")), precision=precision)
+    res = s.solve(kg.Ac(kg.Linear(10), 0.1, 1000.0))
+    assert np.array_equal([r["frequency"].real for r in res], cols["frequency"])
+    tol = 1e-12 if precision == 0 else 1e-9
+    for name in ("I(V1)", "V(in)", "V(out)"):
+        v = np.array([r[name] for r in res])
+        assert np.max(np.abs(np.abs(v) - cols[name + "_mag"]) / cols[name + "_mag"]) < tol
+        ph = np.degrees(np.arctan2(v.imag, v.real))
+        assert np.max(np.abs(ph - cols[name + "_phase_deg"])) < 1e3 * tol
+
+
+TRAN_CASES = [("low_pass_filter/transient.cir", 50e-6, 20e-3), ("high_pass_filter/transient.cir", 10e-6, 2e-3),
+              ("dual_rc_ladder/dual_rc_ladder.cir", 50e-6, 50e-3), ("rectifier/rectifier.cir", 50e-6, 50e-3)]
+
+
+@pytest.mark.parametrize("path,h,stop", TRAN_CASES)
+def test_transient_compat_vs_oracle(kg, oracle, path, h, stop):
+    say(f"transient compat {path}")
+    text = read_circuit(path)
+    oc = oracle.Circuit(text)
+    xr, itr = oc.tran(h, stop)
+    res = kg.Solver(kg.Circuit(text)).solve(kg.Transient(h, stop))
+    x = np.array([[r[k] for k in oc.names] for r in res])
+    assert x.shape == xr.shape
+    errs = [nerr(x[i], xr[i]) for i in range(len(x))]
+    assert max(errs) < 1e-9, max(errs)
+    if "low_pass" in path:  # test_transient.rs:203-206 (the source-lag pin)
+        assert abs(res[42]["V(out)"] - 0.8647) < 1e-3 and abs(res[42]["time"] - 2.1e-3) < 1e-6
+
+
+@pytest.mark.parametrize("tps", TPS)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
+    from krets_amd import workloads as W
+    say(f"batched AC precision={precision} tps={tps}")
+    b = W.c2_rc_ladder_mc(24, sections=50, seed=11)
+    f = W.c2_frequencies(200)[::10]
+    ctx = kg.Context()
+    b.load(ctx)
+    xs, st, err = ctx.ac_sweep(f, precision=precision, config=kg.SolverConfig(threads_per_system=tps))
+    assert (st == 0).all()
+    for i in (0, 7, 23):
+        oc = oracle.Circuit(b.netlists[i])
+        xr = oc.ac(f)
+        e = max(nerr(xs[i][j], xr[j]) for j in range(len(f)))
+        assert e < (1e-12 if precision == 0 else 1e-9), (i, e)
+    if precision == 1:
+        assert (err < 1e-9).all()
+
+
+@pytest.mark.parametrize("tps", [1, 16, 64, 256])
+@pytest.mark.parametrize("method,precision", [(2, 0), (2, 1), (1, 0), (0, 0)])
+def test_batched_transient_uneven_rlc(kg, oracle, method, precision, tps):
+    from krets_amd import workloads as W
+    say(f"batched transient method={method} precision={precision} tps={tps}")
+    b = W.c3_rlc_uneven(5, smin=8, smax=40, seed=13)
+    ctx = kg.Context()
+    b.load(ctx)
+    h, n = 10e-12, 300
+    xs, st, (it, err) = ctx.transient(h, n, method=method, precision=precision,
+                                      config=kg.SolverConfig(threads_per_system=tps))
+    assert (st == 0).all()
+    for i in range(5):
+        oc = oracle.Circuit(b.netlists[i])
+        xr, _ = oc.tran(h, n * h, method=method)
+        e = max(nerr(xs[i][k], xr[k]) for k in range(1, n + 1))
+        assert e < 1e-9, (i, e)
+
+
+def test_error_statuses(kg):
+    say("error statuses")
+    s = kg.Solver(kg.Circuit("I1 1 0 1\nR1 1 0 1\n"))
+    with pytest.raises(kg.KretsGpuError) as e:
+        s.solve(kg.Ac(kg.Linear(1), 1.0, 1.0))
+    assert e.value.code == 2
+    s = kg.Solver(kg.Circuit(read_circuit("diode_iv_curve/diode_iv_curve.cir")),
+                  kg.SolverConfig(maximum_iterations=1))
+    with pytest.raises(kg.KretsGpuError) as e:
+        s.solve(kg.Op())
+    assert e.value.code == 3
+    s = kg.Solver(kg.Circuit("V1 1 0 1\nR1 1 2 1\nC1 2 0 1u G2\n"))
+    with pytest.raises(kg.KretsGpuError) as e:
+        s.solve(kg.Op())
+    assert e.value.code == 2
+    s = kg.Solver(kg.Circuit(read_circuit("voltage_divider/voltage_divider.cir")))
+    with pytest.raises(kg.KretsGpuError) as e:
+        s.solve(kg.Dc("V9", 0, 1, 1))
+    assert e.value.code == 4

@@ -1,0 +1,153 @@
+"""CPU checks of the host symbolic pass (no GPU needed).
+
+For every golden netlist, analysis kind and ordering, the exported plan is
+checked three ways:
+  * its gather lists rebuild exactly krets' MNA matrix and right-hand side, as
+    assembled by the oracle (stampable.rs semantics), under the plan's row and
+    column permutation;
+  * its factor program and level-ordered solves, run in numpy, solve the
+    permuted system (compared with numpy.linalg.solve);
+  * the transposed solves solve the transposed system.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import plan_emulator as pe
+from conftest import read_circuit
+
+CIRCUITS = [
+    "circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir",
+    "voltage_divider/voltage_divider.cir",
+    "low_pass_filter/low_pass_filter.cir",
+    "high_pass_filter/high_pass_filter.cir",
+    "basic_001/basic_001.cir",
+    "diode_iv_curve/diode_iv_curve.cir",
+    "dual_rc_ladder/dual_rc_ladder.cir",
+    "rectifier/rectifier.cir",
+    "resistor_ladder_500/resistor_ladder_500.cir",
+]
+VT = 1.380649e-23 * 300.0 / 1.602176634e-19
+
+
+def ladder(sections):
+    lines = ["V1 n0 0 1"]
+    for k in range(1, sections + 1):
+        lines += [f"R{k} n{k-1} m{k} 20", f"L{k} m{k} n{k} 1n", f"C{k} n{k} 0 0.5p"]
+    return "\n".join(lines) + "\n"
+
+
+def values(c, kind, xop=None, f=0.0, h=1e-9, alpha=1.0):
+    el, pr = c.elements, c.params
+
+    def diode(e, x):
+        p, m = el[e]["plus"], el[e]["minus"]
+        vp = 0.5 if (x is None or p < 0) else x[p]
+        vm = 0.0 if (x is None or m < 0) else x[m]
+        vd = vp - vm
+        nvt = pr[e, 1] * VT
+        g = pr[e, 0] / nvt * math.exp(vd / nvt)
+        return g, pr[e, 0] * (math.exp(vd / nvt) - 1.0) - g * vd
+
+    def y(e):
+        t = el[e]["type"]
+        if t == 0:
+            return 1.0 / pr[e, 0]
+        if t == 1:
+            return 2j * math.pi * f * pr[e, 0] if kind == pe.AC else alpha * pr[e, 0] / h
+        if t == 5:
+            return diode(e, xop)[0]
+        return 0.0
+
+    def z(e):
+        t = el[e]["type"]
+        if t == 2:
+            return -2j * math.pi * f * pr[e, 0] if kind == pe.AC else -alpha * pr[e, 0] / h
+        if t == 0:
+            return -pr[e, 0]
+        return 1.0
+
+    def contrib(e, k):
+        return [y(e), -y(e), 1.0, -1.0, z(e)][k]
+
+    def rhs(e, k):
+        t = el[e]["type"]
+        if k == 0:
+            return pr[e, 1] if (kind == pe.AC and t == 3) else pr[e, 0]
+        if k in (1, 2):
+            ieq = diode(e, xop)[1]
+            return -ieq if k == 1 else ieq
+        return 0.0  # companion terms are zero for a zero previous state
+
+    return contrib, rhs
+
+
+@pytest.mark.parametrize("ordering", [pe.NATURAL, pe.MINDEG, pe.NESTED])
+@pytest.mark.parametrize("path", CIRCUITS)
+def test_plan_dc_ac_matches_oracle(oracle, path, ordering):
+    import krets_amd as k
+    text = read_circuit(path)
+    c = k.Circuit(text)
+    oc = oracle.Circuit(text)
+    assert c.names == oc.names
+    # DC at krets' first Newton iteration (empty solution map)
+    P = pe.Plan(c.elements, c.n, pe.DC, ordering)
+    if P.status:
+        with pytest.raises(oracle.OracleError):
+            oc.op()
+        return
+    contrib, rhs = values(c, pe.DC)
+    v, M = P.gather(contrib)
+    b = P.gather_rhs(rhs)
+    A, bref = oc.assemble_dc()
+    ro, co = P.arr("row_of", c.n), P.arr("col_of", c.n)
+    assert np.allclose(M, A[np.ix_(ro, co)], rtol=1e-15, atol=0)
+    assert np.allclose(b, bref[ro], rtol=1e-15, atol=0)
+    vf = P.factor(v)
+    x = P.solve(vf, b)
+    assert np.allclose(x, np.linalg.solve(M, b), rtol=1e-9, atol=1e-14 * np.abs(x).max())
+    xt = P.solve_t(vf, b)
+    assert np.allclose(xt, np.linalg.solve(M.T, b), rtol=1e-9, atol=1e-14 * np.abs(xt).max())
+    # AC at 1234.5 Hz linearised at the operating point
+    xop, _ = oc.op()
+    PA = pe.Plan(c.elements, c.n, pe.AC, ordering)
+    if PA.status:
+        with pytest.raises(oracle.OracleError):
+            oc.ac([1234.5])
+        return
+    contrib, rhs = values(c, pe.AC, xop=xop, f=1234.5)
+    v, M = PA.gather(contrib)
+    b = PA.gather_rhs(rhs)
+    A, bref = oc.assemble_ac(xop, 1234.5)
+    ro, co = PA.arr("row_of", c.n), PA.arr("col_of", c.n)
+    assert np.allclose(M, A[np.ix_(ro, co)], rtol=1e-15, atol=0)
+    assert np.allclose(b, bref[ro], rtol=1e-15, atol=0)
+    x = PA.solve(PA.factor(v), b)
+    assert np.allclose(x, np.linalg.solve(M, b), rtol=1e-9, atol=1e-14 * max(np.abs(x).max(), 1e-300))
+
+
+@pytest.mark.parametrize("ordering", [pe.MINDEG, pe.NESTED])
+@pytest.mark.parametrize("sections", [1, 8, 37])
+def test_plan_transient_ladder(oracle, sections, ordering):
+    import krets_amd as k
+    c = k.Circuit(ladder(sections))
+    P = pe.Plan(c.elements, c.n, pe.TRAN, ordering)
+    assert P.status == 0
+    contrib, _ = values(c, pe.TRAN, h=1e-11, alpha=2.0)
+    v, M = P.gather(contrib)
+    b = np.random.default_rng(sections).standard_normal(c.n)
+    vf = P.factor(v)
+    assert np.allclose(P.solve(vf, b), np.linalg.solve(M, b), rtol=1e-9, atol=1e-12)
+    assert np.allclose(P.solve_t(vf, b), np.linalg.solve(M.T, b), rtol=1e-9, atol=1e-12)
+    if ordering == pe.NESTED and sections == 37:
+        # nested dissection keeps the elimination tree shallow on a chain
+        assert P.L["nlev"] <= 2 * math.ceil(math.log2(c.n)) + 2
+
+
+def test_plan_structurally_singular():
+    import krets_amd as k
+    c = k.Circuit("I1 1 0 1\nR1 1 0 1\n")  # AC current source lacks (ic, ic)
+    assert pe.Plan(c.elements, c.n, pe.AC, pe.NESTED).status == 2
+    c = k.Circuit("V1 1 0 1\nR1 1 2 1\nC1 2 0 1u G2\n")  # G2 capacitor: empty branch row in DC
+    assert pe.Plan(c.elements, c.n, pe.DC, pe.MINDEG).status == 2
