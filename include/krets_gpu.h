@@ -172,11 +172,14 @@ int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launc
 
 /* Diagnostics (no krets counterpart): the symbolic plan the library would
  * build for one topology. kind: 0 DC, 1 AC, 2 transient; ordering: 0 natural,
- * 1 minimum degree, 2 nested dissection. Returns the plan length in int32
- * (copying at most cap entries to buf) or -status. layout receives the plan's
- * header (PatLayout, see krets_amd/csrc/kg_internal.h). */
+ * 1 minimum degree, 2 nested dissection. params (may be NULL): one instance's
+ * parameter records, used with s_rep (alpha/h or 2*pi*f) for the static pivot
+ * choice. Returns the plan length in int32 (copying at most cap entries to
+ * buf) or -status. layout receives the plan's header (PatLayout, see
+ * krets_amd/csrc/kg_internal.h). */
 int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, int32_t kind,
-                       int32_t ordering, int32_t* buf, int32_t cap, int32_t* layout, int32_t layout_cap);
+                       int32_t ordering, const double* params, double s_rep, int32_t* buf, int32_t cap,
+                       int32_t* layout, int32_t layout_cap);
 
 /* Netlist reader for the R/L/C/V/I/D subset of krets' grammar
  * (crates/krets-parser/src/parser.rs:27-154). */

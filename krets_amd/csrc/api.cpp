@@ -17,8 +17,8 @@
 #include "krets_gpu.h"
 
 namespace kg {
-std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, AnalysisKind ak, Ordering ord, PatLayout& L,
-                                   std::string& err);
+std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, AnalysisKind ak, Ordering ord,
+                                   const double* rep, double s_rep, PatLayout& L, std::string& err);
 }
 
 using namespace kg;
@@ -59,11 +59,11 @@ struct kg_ctx {
   int64_t* d_eoff = nullptr;
   int32_t* d_nunk = nullptr;
   // plans
-  std::map<std::tuple<int, int, int>, int> pat_index;
+  std::map<std::tuple<int, int, int, double>, int> pat_index;
   std::vector<PatEntry> pats;
   PatDev* d_pats = nullptr;
   size_t d_pats_n = 0;
-  std::map<std::pair<int, int>, int32_t*> inst_pat;
+  std::map<std::tuple<int, int, double>, int32_t*> inst_pat;
   std::map<std::string, DevBuf> ws;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
@@ -170,20 +170,24 @@ struct Plan {
   std::vector<int32_t> group_status;  // per group
 };
 
-int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, Plan& P) {
+// s_rep: representative scale of reactive stamps used by the static pivot
+// choice (alpha/h for transients, 2*pi*f for AC, unused for DC).
+int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P) {
+  if (ak == AK_DC) s_rep = 0.0;
   int ng = (int)c->group_first.size();
   bool added = false;
   P.group_status.assign(ng, KG_OK);
   std::vector<int32_t> gpi(ng);
   for (int g = 0; g < ng; g++) {
-    auto key = std::make_tuple(g, (int)ak, (int)ord);
+    auto key = std::make_tuple(g, (int)ak, (int)ord, s_rep);
     auto it = c->pat_index.find(key);
     int idx;
     if (it == c->pat_index.end()) {
       int i0 = c->group_first[g];
       int ne = (int)(c->eoff[i0 + 1] - c->eoff[i0]);
       PatEntry pe;
-      std::vector<int32_t> buf = build_pattern(c->el.data() + c->eoff[i0], ne, c->nunk[i0], ak, ord, pe.lay, pe.err);
+      std::vector<int32_t> buf = build_pattern(c->el.data() + c->eoff[i0], ne, c->nunk[i0], ak, ord,
+                                               c->params.data() + c->eoff[i0] * KG_NPARAM, s_rep, pe.lay, pe.err);
       if (pe.lay.status == KG_OK) {
         HIPCHK(c, hipMalloc(&pe.d_buf, sizeof(int32_t) * std::max<size_t>(buf.size(), 1)));
         HIPCHK(c, hipMemcpy(pe.d_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice));
@@ -221,7 +225,7 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, Plan& P) {
     HIPCHK(c, hipMemcpy(c->d_pats, hv.data(), sizeof(PatDev) * hv.size(), hipMemcpyHostToDevice));
     c->d_pats_n = hv.size();
   }
-  auto k2 = std::make_pair((int)ak, (int)ord);
+  auto k2 = std::make_tuple((int)ak, (int)ord, s_rep);
   if (!c->inst_pat.count(k2)) {
     std::vector<int32_t> tab(c->n_inst);
     for (int i = 0; i < c->n_inst; i++) tab[i] = gpi[c->group_of[i]];
@@ -349,11 +353,11 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   int tps0 = cfg->threads_per_system > 0 ? cfg->threads_per_system : 0;
   // prepare with a provisional ordering to learn sizes
   Plan P0;
-  int32_t r = prepare(c, AK_DC, ORD_NESTED, P0);
+  int32_t r = prepare(c, AK_DC, ORD_NESTED, 0.0, P0);
   if (r) return r;
   int tps = tps0 ? tps0 : default_tps(c, P0.n_max);
   if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  r = prepare(c, AK_DC, order_for(tps), P);
+  r = prepare(c, AK_DC, order_for(tps), 0.0, P);
   if (r) return r;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, 1, 8, 8, 0, 0};
   Launch L;
@@ -593,10 +597,13 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
   if (r) return r;
   Plan P0, P;
-  if ((r = prepare(c, AK_AC, ORD_NESTED, P0))) return r;
+  double fmin = freqs[0], fmax = freqs[0];
+  for (int k = 0; k < nf; k++) fmin = std::min(fmin, freqs[k]), fmax = std::max(fmax, freqs[k]);
+  double s_ac = 2.0 * 3.14159265358979323846 * std::sqrt(fmin * fmax);
+  if ((r = prepare(c, AK_AC, ORD_NESTED, s_ac, P0))) return r;
   int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
   if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  if ((r = prepare(c, AK_AC, order_for(tps), P))) return r;
+  if ((r = prepare(c, AK_AC, order_for(tps), s_ac, P))) return r;
   bool mixed = precision == KG_MIXED;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 0, P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
   Launch L;
@@ -649,10 +656,11 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
   if (r) return r;
   Plan P0, P;
-  if ((r = prepare(c, AK_TRAN, ORD_NESTED, P0))) return r;
+  double s_tr = (method == KG_TRAP ? 2.0 : 1.0) / h;
+  if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P0))) return r;
   int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
   if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  if ((r = prepare(c, AK_TRAN, order_for(tps), P))) return r;
+  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
   bool mixed = precision == KG_MIXED;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
   Launch L;
@@ -701,13 +709,14 @@ int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launche
   return KG_OK;
 }
 
-int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind, int32_t ordering, int32_t* buf,
-                       int32_t cap, int32_t* layout, int32_t layout_cap) {
+int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind, int32_t ordering,
+                       const double* params, double s_rep, int32_t* buf, int32_t cap, int32_t* layout,
+                       int32_t layout_cap) {
   if (!el || ne <= 0 || n <= 0 || kind < 0 || kind >= AK_COUNT || ordering < 0 || ordering > ORD_NESTED)
     return -KG_INVALID_ARGUMENT;
   PatLayout L;
   std::string err;
-  std::vector<int32_t> b = build_pattern(el, ne, n, (AnalysisKind)kind, (Ordering)ordering, L, err);
+  std::vector<int32_t> b = build_pattern(el, ne, n, (AnalysisKind)kind, (Ordering)ordering, params, s_rep, L, err);
   if (layout) {
     int32_t m = std::min<int32_t>(layout_cap, (int32_t)(sizeof(PatLayout) / sizeof(int32_t)));
     std::memcpy(layout, &L, sizeof(int32_t) * m);
@@ -717,9 +726,59 @@ int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind
   return (int32_t)b.size();
 }
 
-int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config*, double, double, double, double, double, double, int32_t,
-                         double*, double*, int32_t*, int32_t*, int32_t*, int32_t*, int32_t) {
-  return set_err(c, KG_UNEXPECTED, "kg_transient_lte: not available in this build");
+int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, double h0, double hmin, double hmax,
+                         double lte_reltol, double lte_abstol, int32_t max_points, double* t, double* x,
+                         int32_t* n_points, int32_t* newton_total, int32_t* rejected, int32_t* status, int32_t mem) {
+  if (!c || !cfg || c->n_inst == 0 || !(tstop > 0) || !(h0 > 0) || !(hmin > 0) || !(hmax > 0) || max_points < 2 ||
+      !(lte_reltol >= 0) || !(lte_abstol > 0))
+    return set_err(c, KG_INVALID_ARGUMENT, "bad arguments");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  kg_solver_config ocfg = *cfg;
+  ocfg.threads_per_system = 0;
+  ocfg.compat = 0;
+  double* d_xop;
+  int64_t* d_xoff;
+  int32_t *d_st, *d_it;
+  int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
+  if (r) return r;
+  Plan P0, P;
+  double s_tr = 1.0 / h0;
+  if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P0))) return r;
+  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 2, 1, 8, 8, 0, 0};
+  Launch L;
+  if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
+  auto off = out_offsets(c, max_points);
+  int64_t* d_off;
+  if ((r = upload_offsets(c, "lte_off", off, &d_off))) return r;
+  hipError_t e;
+  size_t ni = c->n_inst;
+  double* d_t = (double*)ws_get(c, "lte_t", sizeof(double) * ni * max_points, &e);
+  double* d_x = (double*)ws_get(c, "lte_x", sizeof(double) * off.back(), &e);
+  int32_t* d_np = (int32_t*)ws_get(c, "lte_np", sizeof(int32_t) * ni, &e);
+  int32_t* d_nt = (int32_t*)ws_get(c, "lte_nt", sizeof(int32_t) * ni, &e);
+  int32_t* d_rj = (int32_t*)ws_get(c, "lte_rj", sizeof(int32_t) * ni, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  HIPCHK(c, hipMemcpyAsync(d_nt, d_it, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(d_t, 0, sizeof(double) * ni * max_points, c->stream));
+  NewtonArgs na = newton_args(cfg);
+  na.compat = 0;
+  LteArgs la{tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points};
+  BatchDev B = batch_dev(c, P);
+  r = timed(c, "k_tran_lte", [&] {
+    return run_lte(tps, L.grid, L.block, L.lds, c->stream, B, 0, na, la, d_xop, d_xoff, d_t, d_x, d_off, d_np, d_nt,
+                   d_rj, d_st, lay);
+  });
+  if (r) return r;
+  if ((r = copy_out(c, t, d_t, sizeof(double) * ni * max_points, mem))) return r;
+  if ((r = copy_out(c, x, d_x, sizeof(double) * off.back(), mem))) return r;
+  if (n_points && (r = copy_out(c, n_points, d_np, sizeof(int32_t) * ni, mem))) return r;
+  if (newton_total && (r = copy_out(c, newton_total, d_nt, sizeof(int32_t) * ni, mem))) return r;
+  if (rejected && (r = copy_out(c, rejected, d_rj, sizeof(int32_t) * ni, mem))) return r;
+  return finish_status(c, d_st, status, mem);
 }
 
 int32_t kg_adjoint_transient(kg_ctx* c, const kg_solver_config*, double, int32_t, int32_t, int32_t, const double*,

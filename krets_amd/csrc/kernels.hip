@@ -424,7 +424,7 @@ __device__ double solve_refine(const PatDev& P, const F* v, const A* b, A* x, F*
     if (act)
       for (int i = t; i < P.n; i += TPS) x[i] = cvt<A>(w[i]);
     __syncthreads();
-    double est = 0.0;
+    double est = 0.0, dprev = 0.0;
     for (int it = 0; it < refine; it++) {
       // r = b - A x in FP64 (A gathered in FP64)
       double rmax = 0.0;
@@ -471,7 +471,11 @@ __device__ double solve_refine(const PatDev& P, const F* v, const A* b, A* x, F*
       __syncthreads();
       double dm = sys_max<TPS>(dmax, red, ls, t);
       double xm = sys_max<TPS>(xmax, red, ls, t);
-      est = xm > 0.0 ? dm / xm : 0.0;
+      // remaining error ~ contraction ratio * last correction (geometric
+      // convergence of refinement); the last correction itself bounds it
+      double rho = (it > 0 && dprev > 0.0) ? fmin(dm / dprev, 1.0) : 1.0;
+      est = xm > 0.0 ? rho * dm / xm : 0.0;
+      dprev = dm;
     }
     return est;
   }
@@ -488,7 +492,8 @@ struct SysMem {
   A* b;
   F* w;
   double* g;   // Newton guess (FP64)
-  double* xp;  // previous solution (transient)
+  double* xp;  // previous solution(s) (transient); extra vectors follow at xp + k * xstride
+  int xstride;
   double* ed;  // element state
   int* flags;  // [0] status [1] active [2] iterations [4] any diode limited
 };
@@ -514,7 +519,8 @@ __device__ SysMem<F, A> carve(char* base, const Lay& L) {
   m.g = L.need_g ? (double*)p : nullptr;
   if (L.need_g) p += lay_al(8 * (size_t)L.n_max);
   m.xp = L.need_xp ? (double*)p : nullptr;
-  if (L.need_xp) p += lay_al(8 * (size_t)L.n_max);
+  m.xstride = (int)(lay_al(8 * (size_t)L.n_max) / 8);
+  p += (size_t)L.need_xp * lay_al(8 * (size_t)L.n_max);
   m.ed = L.need_ed ? (double*)p : nullptr;
   return m;
 }
@@ -875,6 +881,198 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
   }
 }
 
+
+// Backward Euler with local-truncation-error step control (clean mode; no
+// krets counterpart). Each system advances its own time grid; the LTE of
+// every capacitor voltage and inductor current is estimated from the
+// second divided difference of the last three accepted points,
+// LTE = h^2/2 * |x''|, against reltol * max(|x_new|, |x_old|) + abstol.
+// Rejected steps shrink h by max(0.25, 0.9/sqrt(ratio)); accepted steps
+// grow it by up to 2x; a failed Newton solve divides h by 8.
+// xop: clean operating points; tout: [instance][max_points];
+// xout: krets numbering at xout_off[instance] + point * n.
+template <int TPS>
+__global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la, const double* xop,
+                           const int64_t* xop_off, double* tout, double* xout, const int64_t* xout_off, int32_t* npts,
+                           int32_t* newton_total, int32_t* rejected, int32_t* status, Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int inst = blockIdx.x * spb + ls;
+  const bool valid = inst < B.n_instances;
+  const int ii = valid ? inst : 0;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  size_t sb = sys_bytes(lay);
+  double* red = (double*)(smem + spb * sb);
+  SysMem<double, double> M = carve<double, double>(smem + ls * sb, lay);
+  double* xc = M.xp;
+  double* xpr = M.xp + M.xstride;
+  const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[ii];
+  const double* x0 = xop + xop_off[ii];
+  if (t == 0) M.flags[0] = valid ? status[inst] : KG_OK;
+  __syncthreads();
+  bool active = valid && M.flags[0] == KG_OK;
+  int cnt = 0, nt = valid ? newton_total[inst] : 0, rej = 0;
+  double tcur = 0.0, h = la.h0, tprev = 0.0;
+  bool have_prev = false;
+  if (active) {
+    for (int j = t; j < P.n; j += TPS) {
+      double v = x0[P.col_of[j]];
+      xc[j] = v;
+      xpr[j] = v;
+    }
+    double* o = xout + xout_off[ii];
+    for (int j = t; j < P.n; j += TPS) o[j] = x0[j];
+    if (t == 0) tout[(int64_t)ii * la.max_points] = 0.0;
+    cnt = 1;
+  }
+  __syncthreads();
+  StampArgs sa{AK_TRAN, 0.0, 1.0, 1.0, 0.0, -1, 0.0};
+  const int64_t guard_max = 64 * (int64_t)la.max_points + 1000;
+  for (int64_t guard = 0;; guard++) {
+    if (!__syncthreads_or(active ? 1 : 0)) break;
+    if (guard > guard_max) {
+      if (active && t == 0) M.flags[0] = KG_MAX_ITERATIONS;
+      break;
+    }
+    double tn = tcur;
+    if (active) {
+      h = fmin(h, la.hmax);
+      if (tcur + h > la.tstop) h = la.tstop - tcur;
+      tn = tcur + h;
+    }
+    sa.h = active ? h : 1.0;
+    sa.tsrc = tn;
+    // Newton solve of the backward-Euler step from xc
+    if (active) {
+      for (int i = t; i < P.n; i += TPS) M.g[i] = xc[i];
+      for (int e = t; e < P.ne; e += TPS)
+        if (P.etype[e] == KG_D) {
+          int p = P.ep[e], m = P.em[e];
+          M.ed[e * ESTRIDE + 2] = (p >= 0 ? xc[p] : 0.0) - (m >= 0 ? xc[m] : 0.0);
+        }
+    }
+    __syncthreads();
+    bool done = !active;
+    int its = 0;
+    for (int it = 0; it < na.maxiter; it++) {
+      bool a2 = !done;
+      if (t == 0) M.flags[4] = 0;
+      __syncthreads();
+      eval_diodes<TPS>(P, prm, M.g, M.ed, false, false, &M.flags[4], t, a2);
+      __syncthreads();
+      stamp_matrix<double, double, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
+      stamp_rhs<double, TPS>(P, M.b, prm, sa, M.ed, xc, t, a2);
+      __syncthreads();
+      factor<double, TPS>(P, M.v, t, a2, &M.flags[0]);
+      solve_refine<double, double, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, 0, red, ls, t, a2);
+      __syncthreads();
+      bool stop;
+      if (!P.nonlinear) {
+        stop = true;
+        sys_max<TPS>(0.0, red, ls, t);
+      } else {
+        bool cv = newton_converged<TPS>(P, M.x, M.g, na.reltol, na.iabstol, na.vabstol, red, ls, t, a2);
+        stop = cv && !M.flags[4];
+      }
+      if (M.flags[0] != KG_OK) stop = true;
+      if (a2 && !stop)
+        for (int i = t; i < P.n; i += TPS) M.g[i] = M.x[i];
+      __syncthreads();
+      if (a2 && stop) {
+        done = true;
+        its = it + 1;
+      }
+      if (__syncthreads_and(done ? 1 : 0)) break;
+    }
+    bool ok = active && done && M.flags[0] == KG_OK;
+    if (active && M.flags[0] != KG_OK) active = false;
+    if (active && !done) {  // Newton failure: retry with h / 8
+      rej++;
+      h /= 8.0;
+      if (h < la.hmin) {
+        if (t == 0) M.flags[0] = KG_MAX_ITERATIONS;
+        active = false;
+      }
+    }
+    if (ok) nt += its;
+    // LTE ratio over capacitor voltages and inductor currents
+    double ratio = 0.0;
+    if (ok && have_prev) {
+      double hp = tcur - tprev;
+      for (int e = t; e < P.ne; e += TPS) {
+        int ty = P.etype[e];
+        double a, b0, bm;
+        if (ty == KG_C) {
+          int p = P.ep[e], m = P.em[e];
+          a = (p >= 0 ? M.x[p] : 0.0) - (m >= 0 ? M.x[m] : 0.0);
+          b0 = (p >= 0 ? xc[p] : 0.0) - (m >= 0 ? xc[m] : 0.0);
+          bm = (p >= 0 ? xpr[p] : 0.0) - (m >= 0 ? xpr[m] : 0.0);
+        } else if (ty == KG_L) {
+          int c = P.ec[e];
+          a = M.x[c];
+          b0 = xc[c];
+          bm = xpr[c];
+        } else {
+          continue;
+        }
+        double dd2 = 2.0 * ((a - b0) / h - (b0 - bm) / hp) / (h + hp);
+        double lte = 0.5 * h * h * fabs(dd2);
+        double tol = la.lte_reltol * fmax(fabs(a), fabs(b0)) + la.lte_abstol;
+        ratio = fmax(ratio, lte / tol);
+      }
+    }
+    ratio = sys_max<TPS>(ratio, red, ls, t);
+    bool accept = false;
+    double hnext = h;
+    if (ok) {
+      accept = true;
+      if (have_prev) {
+        if (ratio > 1.0) {
+          rej++;
+          h *= fmax(0.25, 0.9 / sqrt(ratio));
+          accept = false;
+          if (h < la.hmin) {
+            if (t == 0) M.flags[0] = KG_MAX_ITERATIONS;
+            active = false;
+          }
+        } else {
+          hnext = ratio > 0.0 ? h * fmin(2.0, fmax(0.25, 0.9 / sqrt(ratio))) : 2.0 * h;
+        }
+      }
+      if (accept && cnt >= la.max_points) {
+        if (t == 0) M.flags[0] = KG_UNEXPECTED;
+        active = false;
+        accept = false;
+      }
+    }
+    __syncthreads();
+    if (accept) {
+      for (int i = t; i < P.n; i += TPS) {
+        xpr[i] = xc[i];
+        xc[i] = M.x[i];
+      }
+      double* o = xout + xout_off[ii] + (int64_t)cnt * P.n;
+      for (int j = t; j < P.n; j += TPS) o[P.col_of[j]] = M.x[j];
+      if (t == 0) tout[(int64_t)ii * la.max_points + cnt] = tn;
+      cnt++;
+      tprev = tcur;
+      tcur = tn;
+      h = hnext;
+      have_prev = true;
+      if (!(tcur < la.tstop * (1.0 - 1e-12))) active = false;
+    }
+    __syncthreads();
+  }
+  if (valid && t == 0) {
+    status[inst] = M.flags[0];
+    npts[inst] = cnt;
+    newton_total[inst] = nt;
+    rejected[inst] = rej;
+  }
+}
+
 // ------------------------------------------------------- launch helpers
 template <class K, class... Args>
 hipError_t launch(K fn, dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args) {
@@ -910,6 +1108,14 @@ hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st
   }
   KG_TPS_SWITCH(tps, launch(k_ac<c128, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out, out_off,
                             status, err, refine, lay))
+}
+
+hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
+                   LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
+                   const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
+                   Lay lay) {
+  KG_TPS_SWITCH(tps, launch(k_tran_lte<T>, g, b, lds, st, B, kind_slot, na, la, xop, xop_off, tout, xout, xout_off,
+                            npts, newton_total, rejected, status, lay))
 }
 
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

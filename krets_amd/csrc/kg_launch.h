@@ -29,7 +29,7 @@ struct BatchDev {
 // (sys_bytes) to size the dynamic LDS of a launch.
 struct Lay {
   int32_t nnz_max, n_max, ne_max;
-  int32_t need_g, need_xp, need_ed;
+  int32_t need_g, need_xp, need_ed;  // need_xp: number of extra FP64 n-vectors
   int32_t fsz, asz, mixed, pad;
 };
 
@@ -39,7 +39,7 @@ __host__ __device__ inline size_t sys_bytes(const Lay& L) {
   size_t s = 64 + lay_al((size_t)L.fsz * L.nnz_max) + 2 * lay_al((size_t)L.asz * L.n_max);
   if (L.mixed) s += lay_al((size_t)L.fsz * L.n_max);
   if (L.need_g) s += lay_al(8 * (size_t)L.n_max);
-  if (L.need_xp) s += lay_al(8 * (size_t)L.n_max);
+  s += (size_t)L.need_xp * lay_al(8 * (size_t)L.n_max);
   if (L.need_ed) s += lay_al(8 * (size_t)ESTRIDE * L.ne_max);
   return s;
 }
@@ -55,6 +55,12 @@ struct TranArgs {
   int refine;
 };
 
+// Backward Euler with local-truncation-error control (clean mode).
+struct LteArgs {
+  double tstop, h0, hmin, hmax, lte_reltol, lte_abstol;
+  int max_points;
+};
+
 
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
@@ -62,6 +68,10 @@ hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, Batch
 hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                   const double* freqs, int nf, const double* xop, const int64_t* xop_off, double* out,
                   const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay);
+hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
+                   LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
+                   const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
+                   Lay lay);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay);

@@ -15,6 +15,9 @@
 //     and both transposed solves;
 //  5. per-slot and per-row gather lists for stamping on the device.
 #include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
 #include <cstring>
 #include <functional>
 #include <queue>
@@ -139,7 +142,7 @@ std::vector<int> order_nested(int n, const std::vector<std::vector<int>>& adj) {
 // Builds the plan for one topology. Returns the packed int32 buffer; layout
 // and status are written to L. err receives a message on failure.
 std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, AnalysisKind ak, Ordering ordk,
-                                   PatLayout& L, std::string& err) {
+                                   const double* rep, double s_rep, PatLayout& L, std::string& err) {
   std::memset(&L, 0, sizeof(L));
   L.n = n;
   L.ne = ne;
@@ -298,9 +301,93 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   }
   std::vector<int> cpos(n), rpos(n);
   for (int j = 0; j < n; j++) cpos[ord[j]] = j;
-  for (int i = 0; i < n; i++) rpos[mrow[ord[i]]] = i;
 
-  // symbolic LU of M(i, j) = A(mrow[ord[i]], ord[j])
+  // Static pivot sequence from one representative numeric factorisation
+  // (the approach of KLU's refactorisation): columns in the fill-reducing
+  // order, rows chosen by threshold partial pivoting that keeps the matched
+  // row when it is within 0.1 of the column maximum. This removes the exact
+  // cancellations of MNA incidence entries (e.g. an inductor short in DC)
+  // that a purely structural matching cannot see.
+  std::vector<int> prow(n, -1);
+  {
+    using cd = std::complex<double>;
+    const double vt = 1.380649e-23 * 300.0 / 1.602176634e-19;  // constants.rs:1-4
+    auto coef = [&](int32_t code) -> cd {
+      int e = code >> 3, k = code & 7;
+      const kg_element& x = el[e];
+      double v = rep ? rep[e * KG_NPARAM] : 1.0;
+      auto y = [&]() -> cd {
+        if (x.type == KG_R) return cd(1.0 / v, 0.0);
+        if (x.type == KG_C) return ak == AK_AC ? cd(0.0, s_rep * v) : cd(s_rep * v, 0.0);
+        if (x.type == KG_D) {
+          double nvt = (rep ? rep[e * KG_NPARAM + 1] : 1.0) * vt;
+          return cd(v / nvt * std::exp(0.5 / nvt), 0.0);
+        }
+        return cd(0.0, 0.0);
+      };
+      switch (k) {
+        case CK_YP: return y();
+        case CK_YN: return -y();
+        case CK_ONEP: return cd(1.0, 0.0);
+        case CK_ONEN: return cd(-1.0, 0.0);
+        default:
+          if (x.type == KG_L) return ak == AK_AC ? cd(0.0, -s_rep * v) : cd(-s_rep * v, 0.0);
+          if (x.type == KG_R) return cd(-v, 0.0);
+          return cd(1.0, 0.0);
+      }
+    };
+    std::vector<std::map<int, cd>> rowv(n);  // original row -> (permuted column -> value)
+    for (auto& a : A) rowv[a.r][cpos[a.c]] += coef(a.code);
+    std::vector<std::set<int>> colr(n);  // permuted column -> rows that may hold it
+    for (int r = 0; r < n; r++)
+      for (auto& kv : rowv[r]) colr[kv.first].insert(r);
+    std::vector<char> used(n, 0);
+    for (int k = 0; k < n; k++) {
+      int pref = mrow[ord[k]];
+      double best = 0.0;
+      int arg = -1;
+      for (int r : colr[k]) {
+        if (used[r]) continue;
+        auto it = rowv[r].find(k);
+        double a = it == rowv[r].end() ? 0.0 : std::abs(it->second);
+        if (a > best) { best = a; arg = r; }
+      }
+      int p = -1;
+      if (best > 0.0) {
+        auto it = used[pref] ? rowv[pref].end() : rowv[pref].find(k);
+        double ap = it == rowv[pref].end() ? 0.0 : std::abs(it->second);
+        p = ap >= 0.1 * best ? pref : arg;
+      } else {  // numerically singular representative: keep the structural match
+        p = !used[pref] ? pref : -1;
+        for (int r : colr[k])
+          if (p < 0 && !used[r]) p = r;
+        if (p < 0) {
+          for (int r = 0; r < n && p < 0; r++)
+            if (!used[r]) p = r;
+        }
+      }
+      prow[k] = p;
+      used[p] = 1;
+      auto pit = rowv[p].find(k);
+      cd pv = pit == rowv[p].end() ? cd(0.0, 0.0) : pit->second;
+      if (std::abs(pv) == 0.0) continue;
+      for (int r : colr[k]) {
+        if (used[r]) continue;
+        auto it = rowv[r].find(k);
+        if (it == rowv[r].end()) continue;
+        cd l = it->second / pv;
+        rowv[r].erase(it);
+        for (auto& kv : rowv[p]) {
+          if (kv.first <= k) continue;
+          rowv[r][kv.first] -= l * kv.second;
+          colr[kv.first].insert(r);
+        }
+      }
+    }
+  }
+  for (int i = 0; i < n; i++) rpos[prow[i]] = i;
+
+  // symbolic LU of M(i, j) = A(prow[i], ord[j])
   std::vector<std::vector<int>> mrows(n);
   for (int r = 0; r < n; r++)
     for (int c2 : rows[r]) mrows[rpos[r]].push_back(cpos[c2]);
@@ -455,7 +542,7 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   L.o_tblvl_ptr = put(tblvl_ptr);
   L.o_tblvl_rows = put(tblvl_rows);
   std::vector<int32_t> row_of(n);
-  for (int i = 0; i < n; i++) row_of[i] = mrow[ord[i]];
+  for (int i = 0; i < n; i++) row_of[i] = prow[i];
   L.o_row_of = put(row_of);
   L.total = (int32_t)buf.size();
   if (!validate_plan(L, buf, err)) {

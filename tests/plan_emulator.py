@@ -20,18 +20,19 @@ NATURAL, MINDEG, NESTED = 0, 1, 2
 
 
 class Plan:
-    def __init__(self, elements, n, kind, ordering):
+    def __init__(self, elements, n, kind, ordering, params=None, s_rep=1.0):
         el = np.ascontiguousarray(elements, _lib.ELEMENT_DTYPE)
         lay = np.zeros(64, np.int32)
         L = _lib.lib()
-        L.kg_plan_export.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                     ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
-        size = L.kg_plan_export(el.ctypes.data, len(el), n, kind, ordering, None, 0, lay.ctypes.data, 64)
+        pr = None if params is None else np.ascontiguousarray(params, np.float64)
+        pp = None if pr is None else pr.ctypes.data
+        size = L.kg_plan_export(el.ctypes.data, len(el), n, kind, ordering, pp, s_rep, None, 0, lay.ctypes.data, 64)
         self.status = -size if size < 0 else 0
         if size < 0:
             return
         buf = np.zeros(size, np.int32)
-        L.kg_plan_export(el.ctypes.data, len(el), n, kind, ordering, buf.ctypes.data, size, lay.ctypes.data, 64)
+        L.kg_plan_export(el.ctypes.data, len(el), n, kind, ordering, pp, s_rep, buf.ctypes.data, size,
+                         lay.ctypes.data, 64)
         self.L = {k: int(lay[i]) for i, k in enumerate(FIELDS)}
         self.b = buf
         self.n, self.nnz, self.ne = self.L["n"], self.L["nnz"], self.L["ne"]

@@ -145,6 +145,28 @@ def test_plan_transient_ladder(oracle, sections, ordering):
         assert P.L["nlev"] <= 2 * math.ceil(math.log2(c.n)) + 2
 
 
+@pytest.mark.parametrize("kind", [pe.DC, pe.TRAN])
+@pytest.mark.parametrize("ordering", [pe.MINDEG, pe.NESTED])
+def test_plan_rlc_static_pivots(kind, ordering):
+    # inductor shorts make the DC matrix's incidence entries cancel exactly
+    # under a purely structural pivot order; the representative numeric
+    # pivoting must avoid every zero pivot
+    import krets_amd as k
+    from krets_amd import workloads as W
+    b = W.c3_rlc_uneven(6, smin=8, smax=30, seed=7)
+    for i in range(6):
+        c = k.Circuit(b.netlists[i])
+        P = pe.Plan(c.elements, c.n, kind, ordering, c.params, 2e11)
+        assert P.status == 0
+        contrib, _ = values(c, kind, h=1e-11, alpha=2.0)
+        v, M = P.gather(contrib)
+        vf = P.factor(v)
+        d = vf[P.diag]
+        assert np.all(np.isfinite(d)) and np.all(d != 0)
+        rhs = np.random.default_rng(i).standard_normal(c.n)
+        assert np.allclose(P.solve(vf, rhs), np.linalg.solve(M, rhs), rtol=1e-8, atol=1e-10)
+
+
 def test_plan_structurally_singular():
     import krets_amd as k
     c = k.Circuit("I1 1 0 1\nR1 1 0 1\n")  # AC current source lacks (ic, ic)
