@@ -163,6 +163,14 @@ int32_t kg_transient_lte(kg_ctx* ctx, const kg_solver_config* cfg, double tstop,
 int32_t kg_adjoint_transient(kg_ctx* ctx, const kg_solver_config* cfg, double h, int32_t nsteps,
                              int32_t method, int32_t precision, const double* cvec, int32_t checkpoint_every,
                              double* J, double* grad, int32_t* status, int32_t mem);
+/* The two halves of kg_adjoint_transient (no krets counterpart): the forward
+ * pass returns J and keeps the checkpoints in the context; the backward pass
+ * returns grad for the most recent forward pass. Batches with a shared
+ * topology and linear circuits (R, C, L, V, I); method KG_BE or KG_TRAP. */
+int32_t kg_adjoint_forward(kg_ctx* ctx, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
+                           int32_t precision, const double* cvec, int32_t checkpoint_every, double* J,
+                           int32_t* status, int32_t mem);
+int32_t kg_adjoint_backward(kg_ctx* ctx, double* grad, int32_t* status, int32_t mem);
 
 /* Kernel timing (no krets counterpart): reads the accumulated duration and
  * launch count of the analysis kernels timed since the previous call (HIP

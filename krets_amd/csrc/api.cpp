@@ -65,6 +65,14 @@ struct kg_ctx {
   size_t d_pats_n = 0;
   std::map<std::tuple<int, int, double>, int32_t*> inst_pat;
   std::map<std::string, DevBuf> ws;
+  // adjoint state kept between kg_adjoint_forward and kg_adjoint_backward
+  struct {
+    bool ready = false;
+    AdjArgs aa{};
+    int tps = 0, mixed = 0, ncp = 0, cpw = 0;
+    int32_t refine = 2;
+    Lay lay{};
+  } adj;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
   double prof_ms = 0.0;
@@ -159,6 +167,11 @@ PatDev to_dev(const PatLayout& L, const int32_t* b) {
   d.tlvl_rows = b + L.o_tlvl_rows;
   d.tblvl_ptr = b + L.o_tblvl_ptr;
   d.tblvl_rows = b + L.o_tblvl_rows;
+  d.row_of = b + L.o_row_of;
+  d.rpos = b + L.o_rpos;
+  d.erp = b + L.o_erp;
+  d.erm = b + L.o_erm;
+  d.erc = b + L.o_erc;
   return d;
 }
 
@@ -781,9 +794,125 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   return finish_status(c, d_st, status, mem);
 }
 
-int32_t kg_adjoint_transient(kg_ctx* c, const kg_solver_config*, double, int32_t, int32_t, int32_t, const double*,
-                             int32_t, double*, double*, int32_t*, int32_t) {
-  return set_err(c, KG_UNEXPECTED, "kg_adjoint_transient: not available in this build");
+int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
+                           int32_t precision, const double* cvec, int32_t K, double* J, int32_t* status, int32_t mem) {
+  if (!c || !cfg || c->n_inst == 0 || !(h > 0) || nsteps < 1 || K < 1 || !cvec ||
+      (method != KG_BE && method != KG_TRAP))
+    return set_err(c, KG_INVALID_ARGUMENT, "bad arguments (adjoint needs KG_BE or KG_TRAP, nsteps >= 1)");
+  if (!c->shared) return set_err(c, KG_INVALID_ARGUMENT, "adjoint: batches with a shared topology only");
+  c->err.clear();
+  c->adj.ready = false;
+  HIPCHK(c, hipSetDevice(c->device));
+  kg_solver_config ocfg = *cfg;
+  ocfg.threads_per_system = 0;
+  ocfg.compat = 0;
+  double* d_xop;
+  int64_t* d_xoff;
+  int32_t *d_st, *d_it;
+  int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
+  if (r) return r;
+  Plan P0, P;
+  double s_tr = (method == KG_TRAP ? 2.0 : 1.0) / h;
+  if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P0))) return r;
+  if (P0.any_nonlinear) return set_err(c, KG_INVALID_ARGUMENT, "adjoint: linear circuits only");
+  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
+  bool mixed = precision == KG_MIXED;
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 5, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
+  Launch L;
+  if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
+  int n = c->nunk[0];
+  int ncp = nsteps / K + 1, cpw = P.n_max + P.ne_max;
+  hipError_t e;
+  size_t ni = c->n_inst;
+  double* d_ck = (double*)ws_get(c, "adj_ckpt", sizeof(double) * ni * ncp * cpw, &e);
+  double* d_c = (double*)ws_get(c, "adj_cvec", sizeof(double) * n, &e);
+  double* d_J = (double*)ws_get(c, "adj_J", sizeof(double) * ni, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (checkpoints)");
+  HIPCHK(c, hipMemcpyAsync(d_c, cvec, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  {  // keep the operating points and their offsets for the backward pass
+    double* a_x = (double*)ws_get(c, "adj_xop", sizeof(double) * ni * n, &e);
+    int64_t* a_o = (int64_t*)ws_get(c, "adj_xoff", sizeof(int64_t) * (ni + 1), &e);
+    if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint)");
+    HIPCHK(c, hipMemcpyAsync(a_x, d_xop, sizeof(double) * ni * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(a_o, d_xoff, sizeof(int64_t) * (ni + 1), hipMemcpyDeviceToDevice, c->stream));
+  }
+  AdjArgs aa{h, method == KG_TRAP ? 2.0 : 1.0, nsteps, method, cfg->refine_iters > 0 ? cfg->refine_iters : 2, K,
+             method == KG_TRAP ? 1 : 0};
+  BatchDev B = batch_dev(c, P);
+  r = timed(c, mixed ? "k_adj_fwd<float,double>" : "k_adj_fwd<double,double>", [&] {
+    return run_adj_fwd(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, aa, d_c, d_xop, d_xoff, d_ck,
+                       (int64_t)ncp * cpw, cpw, d_J, d_st, lay);
+  });
+  if (r) return r;
+  if (J && (r = copy_out(c, J, d_J, sizeof(double) * ni, mem))) return r;
+  r = finish_status(c, d_st, status, mem);
+  c->adj.ready = true;
+  c->adj.aa = aa;
+  c->adj.tps = tps;
+  c->adj.mixed = mixed ? 1 : 0;
+  c->adj.ncp = ncp;
+  c->adj.cpw = cpw;
+  c->adj.lay = lay;
+  return r;
+}
+
+int32_t kg_adjoint_backward(kg_ctx* c, double* grad, int32_t* status, int32_t mem) {
+  if (!c || !c->adj.ready) return set_err(c, KG_INVALID_ARGUMENT, "kg_adjoint_backward: no forward pass");
+  c->err.clear();
+  HIPCHK(c, hipSetDevice(c->device));
+  const AdjArgs& aa = c->adj.aa;
+  Plan P;
+  double s_tr = aa.alpha / aa.h;
+  int32_t r;
+  if ((r = prepare(c, AK_TRAN, order_for(c->adj.tps), s_tr, P))) return r;
+  Lay lay = c->adj.lay;
+  Launch L;
+  if ((r = configure(c, c->adj.tps, lay, c->n_inst, L))) return r;
+  hipError_t e;
+  size_t ni = c->n_inst;
+  int ne = (int)(c->eoff[1] - c->eoff[0]);
+  int64_t seg_stride = (int64_t)(aa.K + 1) * P.n_max;
+  double* d_ck = (double*)ws_get(c, "adj_ckpt", 8, &e);
+  double* d_c = (double*)ws_get(c, "adj_cvec", 8, &e);
+  double* d_seg = (double*)ws_get(c, "adj_seg", sizeof(double) * ni * seg_stride, &e);
+  double* d_g = (double*)ws_get(c, "adj_grad", sizeof(double) * ni * ne, &e);
+  double* d_xb = (double*)ws_get(c, "adj_xbar0", sizeof(double) * ni * P.n_max, &e);
+  double* d_xop = (double*)ws_get(c, "adj_xop", 8, &e);
+  int64_t* d_xoff = (int64_t*)ws_get(c, "adj_xoff", 8, &e);
+  int32_t* d_st = (int32_t*)ws_get(c, "adj_status", sizeof(int32_t) * ni, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint)");
+  HIPCHK(c, hipMemsetAsync(d_st, 0, sizeof(int32_t) * ni, c->stream));
+  BatchDev B = batch_dev(c, P);
+  r = timed(c, c->adj.mixed ? "k_adj_bwd<float,double>" : "k_adj_bwd<double,double>", [&] {
+    return run_adj_bwd(c->adj.tps, c->adj.mixed, L.grid, L.block, L.lds, c->stream, B, 0, aa, d_c, d_ck,
+                       (int64_t)c->adj.ncp * c->adj.cpw, c->adj.cpw, d_seg, seg_stride, d_g, ne, d_xb, d_xoff, d_st,
+                       lay);
+  });
+  if (r) return r;
+  // operating-point part (DC plan, FP64)
+  Plan PD;
+  int tpd = default_tps(c, P.n_max);
+  if ((r = prepare(c, AK_DC, order_for(tpd), 0.0, PD))) return r;
+  Lay ld{PD.nnz_max, PD.n_max, PD.ne_max, 0, 0, 0, 8, 8, 0, 0};
+  Launch LD;
+  if ((r = configure(c, tpd, ld, c->n_inst, LD))) return r;
+  BatchDev BD = batch_dev(c, PD);
+  r = timed(c, "k_adj_op", [&] {
+    return run_adj_op(tpd, LD.grid, LD.block, LD.lds, c->stream, BD, 0, d_xb, d_xop, d_xoff, d_g, ne, d_st, ld);
+  });
+  if (r) return r;
+  if (grad && (r = copy_out(c, grad, d_g, sizeof(double) * ni * ne, mem))) return r;
+  return finish_status(c, d_st, status, mem);
+}
+
+int32_t kg_adjoint_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
+                             int32_t precision, const double* cvec, int32_t K, double* J, double* grad,
+                             int32_t* status, int32_t mem) {
+  int32_t r = kg_adjoint_forward(c, cfg, h, nsteps, method, precision, cvec, K, J, status, mem);
+  if (r != KG_OK) return r;
+  return kg_adjoint_backward(c, grad, status, mem);
 }
 
 }  // extern "C"

@@ -1073,6 +1073,362 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
   }
 }
 
+
+// ------------------------------------------------------------- adjoint
+template <int TPS>
+__device__ double sys_sum(double v, double* red, int ls, int t) {
+  if constexpr (TPS == 1) {
+    return v;
+  } else {
+    red[ls * TPS + t] = v;
+    __syncthreads();
+    for (int s = TPS / 2; s > 0; s >>= 1) {
+      if (t < s) red[ls * TPS + t] += red[ls * TPS + t + s];
+      __syncthreads();
+    }
+    double r = red[ls * TPS];
+    __syncthreads();
+    return r;
+  }
+}
+
+// Forward pass of the discrete adjoint (linear circuits, clean BE or
+// trapezoidal): the transient of k_tran plus the objective
+// J = sum_s w_s c.x_s (trapezoid weights) and a checkpoint of the state
+// (solution and companion states) every K steps.
+// ckpt: [instance][ncp][cpw] with cpw >= n + ne; cvec in krets numbering.
+template <class F, class A, int TPS>
+__global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* xop,
+                          const int64_t* xop_off, double* ckpt, int64_t ck_stride, int cpw, double* Jout,
+                          int32_t* status, Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int inst = blockIdx.x * spb + ls;
+  const bool valid = inst < B.n_instances;
+  const int ii = valid ? inst : 0;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  size_t sb = sys_bytes(lay);
+  double* red = (double*)(smem + spb * sb);
+  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[ii];
+  const double* x0 = xop + xop_off[ii];
+  double* ck = ckpt + (int64_t)ii * ck_stride;
+  if (t == 0) M.flags[0] = valid ? status[inst] : KG_OK;
+  __syncthreads();
+  bool act = valid && M.flags[0] == KG_OK;
+  const bool trap = aa.trap != 0;
+  StampArgs sa{AK_TRAN, 0.0, aa.h, aa.alpha, 0.0, -1, 0.0};
+  if (act) {
+    for (int j = t; j < P.n; j += TPS) M.xp[j] = x0[P.col_of[j]];
+    for (int e = t; e < P.ne; e += TPS) {
+      int p = P.ep[e], m = P.em[e];
+      double vd = (p >= 0 ? x0[P.col_of[p]] : 0.0) - (m >= 0 ? x0[P.col_of[m]] : 0.0);
+      M.ed[e * ESTRIDE] = (trap && P.etype[e] == KG_L) ? vd : 0.0;
+    }
+  }
+  __syncthreads();
+  stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, act);
+  __syncthreads();
+  factor<F, TPS>(P, M.v, t, act, &M.flags[0]);
+  __syncthreads();
+  double jpart = 0.0;
+  for (int s = 0; s <= aa.nsteps; s++) {
+    act = valid && M.flags[0] == KG_OK;
+    if (s > 0) {
+      sa.tsrc = (double)s * aa.h;
+      stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, act);
+      __syncthreads();
+      solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, aa.refine, red, ls, t, act);
+      __syncthreads();
+      if (act && trap)
+        for (int e = t; e < P.ne; e += TPS) {
+          int ty = P.etype[e];
+          if (ty != KG_C && ty != KG_L) continue;
+          int p = P.ep[e], m = P.em[e];
+          double vn = (p >= 0 ? re_of(M.x[p]) : 0.0) - (m >= 0 ? re_of(M.x[m]) : 0.0);
+          if (ty == KG_C) {
+            double vo = (p >= 0 ? M.xp[p] : 0.0) - (m >= 0 ? M.xp[m] : 0.0);
+            M.ed[e * ESTRIDE] = 2.0 * prm[e * KG_NPARAM] / aa.h * (vn - vo) - M.ed[e * ESTRIDE];
+          } else {
+            M.ed[e * ESTRIDE] = vn;
+          }
+        }
+      __syncthreads();
+      if (act)
+        for (int j = t; j < P.n; j += TPS) M.xp[j] = re_of(M.x[j]);
+      __syncthreads();
+    }
+    double w = (s == 0 || s == aa.nsteps) ? 0.5 * aa.h : aa.h;
+    if (act)
+      for (int j = t; j < P.n; j += TPS) jpart += w * cvec[P.col_of[j]] * M.xp[j];
+    if (act && s % aa.K == 0) {
+      double* c = ck + (int64_t)(s / aa.K) * cpw;
+      for (int j = t; j < P.n; j += TPS) c[j] = M.xp[j];
+      for (int e = t; e < P.ne; e += TPS) c[P.n + e] = M.ed[e * ESTRIDE];
+    }
+  }
+  double J = sys_sum<TPS>(jpart, red, ls, t);
+  if (valid && t == 0) {
+    Jout[inst] = J;
+    status[inst] = M.flags[0];
+  }
+}
+
+// Backward pass: per checkpoint segment (last to first) recompute the
+// segment's solutions, then run the discrete adjoint of each step backwards:
+// lambda = A^-T xbar with the stored LU (transposed solves), gradient terms of
+// every R, C and L, and the adjoint states of the trapezoidal companions.
+// xbar0 receives dJ/dx_0 (krets numbering) for the operating-point adjoint.
+template <class F, class A, int TPS>
+__global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* ckpt,
+                          int64_t ck_stride, int cpw, double* seg, int64_t seg_stride, double* grad,
+                          int64_t grad_stride, double* xbar0, const int64_t* xop_off, int32_t* status, Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int inst = blockIdx.x * spb + ls;
+  const bool valid = inst < B.n_instances;
+  const int ii = valid ? inst : 0;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  size_t sb = sys_bytes(lay);
+  double* red = (double*)(smem + spb * sb);
+  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  double* xbar = M.xp + M.xstride;
+  double* xb0 = M.xp + 2 * M.xstride;
+  double* lam = M.xp + 3 * M.xstride;
+  double* rr = M.xp + 4 * M.xstride;
+  const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[ii];
+  const double* ck = ckpt + (int64_t)ii * ck_stride;
+  double* sg = seg + (int64_t)ii * seg_stride;
+  if (t == 0) M.flags[0] = valid ? status[inst] : KG_OK;
+  __syncthreads();
+  bool act = valid && M.flags[0] == KG_OK;
+  const bool trap = aa.trap != 0;
+  const double al = aa.alpha, h = aa.h;
+  StampArgs sa{AK_TRAN, 0.0, h, al, 0.0, -1, 0.0};
+  stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, act);
+  __syncthreads();
+  factor<F, TPS>(P, M.v, t, act, &M.flags[0]);
+  if (act) {
+    for (int j = t; j < P.n; j += TPS) {
+      xbar[j] = 0.5 * h * cvec[P.col_of[j]];
+      xb0[j] = 0.0;
+    }
+    for (int e = t; e < P.ne; e += TPS) {
+      M.ed[e * ESTRIDE + 1] = 0.0;
+      M.ed[e * ESTRIDE + 2] = 0.0;
+    }
+  }
+  __syncthreads();
+  const int nseg = (aa.nsteps + aa.K - 1) / aa.K;
+  for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+    const int s0 = sgi * aa.K, s1 = min(s0 + aa.K, aa.nsteps);
+    // restore the checkpoint and recompute the segment
+    if (act) {
+      const double* c = ck + (int64_t)sgi * cpw;
+      for (int j = t; j < P.n; j += TPS) {
+        M.xp[j] = c[j];
+        sg[j] = c[j];
+      }
+      for (int e = t; e < P.ne; e += TPS) M.ed[e * ESTRIDE] = c[P.n + e];
+    }
+    __syncthreads();
+    for (int s = s0 + 1; s <= s1; s++) {
+      sa.tsrc = (double)s * h;
+      stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, act);
+      __syncthreads();
+      solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, aa.refine, red, ls, t, act);
+      __syncthreads();
+      if (act && trap)
+        for (int e = t; e < P.ne; e += TPS) {
+          int ty = P.etype[e];
+          if (ty != KG_C && ty != KG_L) continue;
+          int p = P.ep[e], m = P.em[e];
+          double vn = (p >= 0 ? re_of(M.x[p]) : 0.0) - (m >= 0 ? re_of(M.x[m]) : 0.0);
+          if (ty == KG_C) {
+            double vo = (p >= 0 ? M.xp[p] : 0.0) - (m >= 0 ? M.xp[m] : 0.0);
+            M.ed[e * ESTRIDE] = 2.0 * prm[e * KG_NPARAM] / h * (vn - vo) - M.ed[e * ESTRIDE];
+          } else {
+            M.ed[e * ESTRIDE] = vn;
+          }
+        }
+      __syncthreads();
+      if (act)
+        for (int j = t; j < P.n; j += TPS) {
+          double v = re_of(M.x[j]);
+          M.xp[j] = v;
+          sg[(int64_t)(s - s0) * P.n + j] = v;
+        }
+      __syncthreads();
+    }
+    // adjoint steps s1-1 .. s0
+    for (int s = s1 - 1; s >= s0; s--) {
+      const double* x1 = sg + (int64_t)(s + 1 - s0) * P.n;
+      const double* xo = sg + (int64_t)(s - s0) * P.n;
+      if (act && trap)  // ic_{s+1} and vl_{s+1} depend on x_{s+1} (and x_s, ic_s)
+        for (int e = t; e < P.ne; e += TPS) {
+          int ty = P.etype[e], p = P.ep[e], m = P.em[e];
+          double sb1 = M.ed[e * ESTRIDE + 1];
+          if (ty == KG_C && sb1 != 0.0) {
+            double a = 2.0 * prm[e * KG_NPARAM] / h * sb1;
+            if (p >= 0) { atomicAdd(&xbar[p], a); atomicAdd(&xb0[p], -a); }
+            if (m >= 0) { atomicAdd(&xbar[m], -a); atomicAdd(&xb0[m], a); }
+            double d1 = (p >= 0 ? x1[p] : 0.0) - (m >= 0 ? x1[m] : 0.0);
+            double d0 = (p >= 0 ? xo[p] : 0.0) - (m >= 0 ? xo[m] : 0.0);
+            M.ed[e * ESTRIDE + 2] += sb1 * 2.0 / h * (d1 - d0);
+          } else if (ty == KG_L && sb1 != 0.0) {
+            if (p >= 0) atomicAdd(&xbar[p], sb1);
+            if (m >= 0) atomicAdd(&xbar[m], -sb1);
+          }
+        }
+      __syncthreads();
+      // lambda = A^-T xbar (rows of lambda are equations)
+      if constexpr (std::is_same<F, A>::value) {
+        if (act)
+          for (int j = t; j < P.n; j += TPS) M.x[j] = xbar[j];
+        __syncthreads();
+        lu_solve_t<F, TPS>(P, M.v, M.x, t, act);
+        if (act)
+          for (int j = t; j < P.n; j += TPS) lam[j] = M.x[j];
+        __syncthreads();
+      } else {
+        if (act)
+          for (int j = t; j < P.n; j += TPS) M.w[j] = cvt<F>(xbar[j]);
+        __syncthreads();
+        lu_solve_t<F, TPS>(P, M.v, M.w, t, act);
+        if (act)
+          for (int j = t; j < P.n; j += TPS) lam[j] = re_of(M.w[j]);
+        __syncthreads();
+        for (int it = 0; it < aa.refine; it++) {  // FP64 refinement of the transposed solve
+          if (act)
+            for (int j = t; j < P.n; j += TPS) rr[j] = xbar[j];
+          __syncthreads();
+          if (act)
+            for (int i = t; i < P.n; i += TPS)
+              for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) {
+                int q0 = P.scon_ptr[p], q1 = P.scon_ptr[p + 1];
+                if (q0 == q1) continue;
+                double a = 0.0;
+                for (int q = q0; q < q1; q++) a += re_of(mat_contrib<A>(P.scon[q], P, prm, eb, sa, M.ed));
+                atomicAdd(&rr[P.col[p]], -a * lam[i]);
+              }
+          __syncthreads();
+          double rmax = 0.0;
+          if (act)
+            for (int j = t; j < P.n; j += TPS) rmax = fmax(rmax, fabs(rr[j]));
+          double sc = sys_max<TPS>(rmax, red, ls, t);
+          if (act && sc > 0.0)
+            for (int j = t; j < P.n; j += TPS) M.w[j] = cvt<F>(rr[j] / sc);
+          __syncthreads();
+          if (sc > 0.0) lu_solve_t<F, TPS>(P, M.v, M.w, t, act);
+          if (act && sc > 0.0)
+            for (int j = t; j < P.n; j += TPS) lam[j] += sc * re_of(M.w[j]);
+          __syncthreads();
+        }
+      }
+      // element terms of step s -> s+1
+      if (act)
+        for (int e = t; e < P.ne; e += TPS) {
+          int ty = P.etype[e];
+          if (ty != KG_R && ty != KG_C && ty != KG_L) continue;
+          int p = P.ep[e], m = P.em[e];
+          double v = prm[e * KG_NPARAM];
+          if (ty == KG_L) {
+            int c = P.ec[e], rc = P.erc[e];
+            double lb = lam[rc];
+            atomicAdd(&xb0[c], -al * (v / h) * lb);
+            M.ed[e * ESTRIDE + 1] = trap ? -lb : 0.0;
+            M.ed[e * ESTRIDE + 2] += lb * (-al / h) * xo[c] + al / h * lb * x1[c];
+            continue;
+          }
+          double el = (P.erp[e] >= 0 ? lam[P.erp[e]] : 0.0) - (P.erm[e] >= 0 ? lam[P.erm[e]] : 0.0);
+          double d1 = (p >= 0 ? x1[p] : 0.0) - (m >= 0 ? x1[m] : 0.0);
+          if (ty == KG_R) {
+            M.ed[e * ESTRIDE + 2] += el * d1 / (v * v);
+          } else {
+            double d0 = (p >= 0 ? xo[p] : 0.0) - (m >= 0 ? xo[m] : 0.0);
+            double a = al * (v / h) * el;
+            if (p >= 0) atomicAdd(&xb0[p], a);
+            if (m >= 0) atomicAdd(&xb0[m], -a);
+            M.ed[e * ESTRIDE + 1] = trap ? (-M.ed[e * ESTRIDE + 1] + el) : 0.0;
+            M.ed[e * ESTRIDE + 2] += el * al / h * d0 - al / h * el * d1;
+          }
+        }
+      __syncthreads();
+      const double w = (s == 0) ? 0.5 * h : h;
+      if (act)
+        for (int j = t; j < P.n; j += TPS) {
+          xbar[j] = xb0[j] + w * cvec[P.col_of[j]];
+          xb0[j] = 0.0;
+        }
+      __syncthreads();
+    }
+  }
+  // vl_0 = v(x_0): its adjoint flows into x_0
+  if (act && trap)
+    for (int e = t; e < P.ne; e += TPS) {
+      if (P.etype[e] != KG_L) continue;
+      double sb1 = M.ed[e * ESTRIDE + 1];
+      int p = P.ep[e], m = P.em[e];
+      if (p >= 0) atomicAdd(&xbar[p], sb1);
+      if (m >= 0) atomicAdd(&xbar[m], -sb1);
+    }
+  __syncthreads();
+  if (act) {
+    double* xo = xbar0 + xop_off[ii];
+    for (int j = t; j < P.n; j += TPS) xo[P.col_of[j]] = xbar[j];
+    double* g = grad + (int64_t)ii * grad_stride;
+    for (int e = t; e < P.ne; e += TPS) g[e] = M.ed[e * ESTRIDE + 2];
+  }
+  if (valid && t == 0) status[inst] = M.flags[0];
+}
+
+// Operating-point part of the adjoint: x_0 solves the DC system A_dc x_0 =
+// b_dc, so dJ/dR gains (e^T lam0)(e^T x_0)/R^2 with lam0 = A_dc^-T xbar0.
+// FP64; DC plan; xbar0 and xop in krets numbering.
+template <int TPS>
+__global__ void k_adj_op(BatchDev B, int kind_slot, const double* xbar0, const double* xop, const int64_t* xop_off,
+                         double* grad, int64_t grad_stride, int32_t* status, Lay lay) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const int inst = blockIdx.x * spb + ls;
+  const bool valid = inst < B.n_instances;
+  const int ii = valid ? inst : 0;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  size_t sb = sys_bytes(lay);
+  SysMem<double, double> M = carve<double, double>(smem + ls * sb, lay);
+  const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[ii];
+  if (t == 0) M.flags[0] = valid ? status[inst] : KG_OK;
+  __syncthreads();
+  bool act = valid && M.flags[0] == KG_OK;
+  StampArgs sa{AK_DC, 0.0, 1.0, 1.0, 0.0, -1, 0.0};
+  stamp_matrix<double, double, TPS>(P, M.v, prm, eb, sa, M.ed, t, act);
+  if (act)
+    for (int j = t; j < P.n; j += TPS) M.x[j] = xbar0[xop_off[ii] + P.col_of[j]];
+  __syncthreads();
+  factor<double, TPS>(P, M.v, t, act, &M.flags[0]);
+  lu_solve_t<double, TPS>(P, M.v, M.x, t, act);
+  if (act) {
+    const double* x0 = xop + xop_off[ii];
+    double* g = grad + (int64_t)ii * grad_stride;
+    for (int e = t; e < P.ne; e += TPS) {
+      if (P.etype[e] != KG_R) continue;
+      int p = P.ep[e], m = P.em[e];
+      double el = (P.erp[e] >= 0 ? M.x[P.erp[e]] : 0.0) - (P.erm[e] >= 0 ? M.x[P.erm[e]] : 0.0);
+      double ex = (p >= 0 ? x0[P.col_of[p]] : 0.0) - (m >= 0 ? x0[P.col_of[m]] : 0.0);
+      double v = prm[e * KG_NPARAM];
+      g[e] += el * ex / (v * v);
+    }
+  }
+  __syncthreads();
+  if (valid && t == 0) status[inst] = M.flags[0];
+}
+
 // ------------------------------------------------------- launch helpers
 template <class K, class... Args>
 hipError_t launch(K fn, dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args) {
@@ -1116,6 +1472,36 @@ hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev
                    Lay lay) {
   KG_TPS_SWITCH(tps, launch(k_tran_lte<T>, g, b, lds, st, B, kind_slot, na, la, xop, xop_off, tout, xout, xout_off,
                             npts, newton_total, rejected, status, lay))
+}
+
+hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                       AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
+                       int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay) {
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_adj_fwd<float, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, xop, xop_off, ckpt,
+                              ck_stride, cpw, J, status, lay))
+  }
+  KG_TPS_SWITCH(tps, launch(k_adj_fwd<double, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, xop, xop_off, ckpt,
+                            ck_stride, cpw, J, status, lay))
+}
+
+hipError_t run_adj_bwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                       AdjArgs aa, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
+                       int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
+                       const int64_t* xop_off, int32_t* status, Lay lay) {
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_adj_bwd<float, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt, ck_stride, cpw,
+                              seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay))
+  }
+  KG_TPS_SWITCH(tps, launch(k_adj_bwd<double, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt, ck_stride, cpw,
+                            seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay))
+}
+
+hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                      const double* xbar0, const double* xop, const int64_t* xop_off, double* grad,
+                      int64_t grad_stride, int32_t* status, Lay lay) {
+  KG_TPS_SWITCH(tps, launch(k_adj_op<T>, g, b, lds, st, B, kind_slot, xbar0, xop, xop_off, grad, grad_stride, status,
+                            lay))
 }
 
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

@@ -46,7 +46,7 @@ struct PatLayout {
   int32_t o_etype, o_ep, o_em, o_ec;
   int32_t o_ut_ptr, o_ut, o_lt_ptr, o_lt;
   int32_t o_tlvl_ptr, o_tlvl_rows, o_tblvl_ptr, o_tblvl_rows;
-  int32_t o_row_of;
+  int32_t o_row_of, o_rpos, o_erp, o_erm, o_erc;
   int32_t status;  // KG_OK or KG_DECOMPOSITION (structurally singular)
   int32_t total;   // buffer length
 };
@@ -62,6 +62,9 @@ struct PatDev {
   const int32_t *etype, *ep, *em, *ec;
   const int32_t *ut_ptr, *ut, *lt_ptr, *lt;
   const int32_t *tlvl_ptr, *tlvl_rows, *tblvl_ptr, *tblvl_rows;
+  // equation (row) positions: rpos[original row] and, per element, the
+  // permuted rows of its plus / minus KCL equations and branch equation
+  const int32_t *row_of, *rpos, *erp, *erm, *erc;
 };
 
 }  // namespace kg

@@ -55,6 +55,12 @@ struct TranArgs {
   int refine;
 };
 
+// Discrete adjoint of a fixed-step linear transient.
+struct AdjArgs {
+  double h, alpha;
+  int nsteps, method, refine, K, trap;
+};
+
 // Backward Euler with local-truncation-error control (clean mode).
 struct LteArgs {
   double tstop, h0, hmin, hmax, lte_reltol, lte_abstol;
@@ -72,6 +78,16 @@ hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev
                    LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
                    Lay lay);
+hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                       AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
+                       int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay);
+hipError_t run_adj_bwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                       AdjArgs aa, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
+                       int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
+                       const int64_t* xop_off, int32_t* status, Lay lay);
+hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
+                      const double* xbar0, const double* xop, const int64_t* xop_off, double* grad,
+                      int64_t grad_stride, int32_t* status, Lay lay);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay);

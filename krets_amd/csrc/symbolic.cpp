@@ -544,6 +544,16 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   std::vector<int32_t> row_of(n);
   for (int i = 0; i < n; i++) row_of[i] = prow[i];
   L.o_row_of = put(row_of);
+  std::vector<int32_t> rposv(rpos.begin(), rpos.end()), erp(ne), erm(ne), erc(ne);
+  for (int e = 0; e < ne; e++) {
+    erp[e] = el[e].plus >= 0 ? rpos[el[e].plus] : -1;
+    erm[e] = el[e].minus >= 0 ? rpos[el[e].minus] : -1;
+    erc[e] = el[e].branch >= 0 ? rpos[el[e].branch] : -1;
+  }
+  L.o_rpos = put(rposv);
+  L.o_erp = put(erp);
+  L.o_erm = put(erm);
+  L.o_erc = put(erc);
   L.total = (int32_t)buf.size();
   if (!validate_plan(L, buf, err)) {
     L.status = KG_UNEXPECTED;
@@ -610,11 +620,12 @@ bool validate_plan(const PatLayout& L, const std::vector<int32_t>& b, std::strin
   for (int q = 0; q < rcp[n]; q++)
     if (!in(rc[q] >> 3, 0, ne) || (rc[q] & 7) > RK_IND) return fail("rcon code");
   for (int j = 0; j < n; j++)
-    if (!in(b[L.o_col_of + j], 0, n) || !in(b[L.o_row_of + j], 0, n) || !in(b[L.o_cpos + j], 0, n))
+    if (!in(b[L.o_col_of + j], 0, n) || !in(b[L.o_row_of + j], 0, n) || !in(b[L.o_cpos + j], 0, n) ||
+        !in(b[L.o_rpos + j], 0, n))
       return fail("permutation");
   for (int e = 0; e < ne; e++) {
     if (!in(b[L.o_etype + e], KG_R, KG_D + 1)) return fail("etype");
-    for (int o : {L.o_ep, L.o_em, L.o_ec})
+    for (int o : {L.o_ep, L.o_em, L.o_ec, L.o_erp, L.o_erm, L.o_erc})
       if (!in(b[o + e], -1, n)) return fail("element index");
     int ty = b[L.o_etype + e];
     if ((ty == KG_V || ty == KG_L || ty == KG_I) && b[L.o_ec + e] < 0) return fail("branch index");

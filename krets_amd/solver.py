@@ -286,23 +286,47 @@ class Context:
                                    max_points, ctypes.c_void_p(t.ctypes.data), ctypes.c_void_p(x.ctypes.data),
                                    ctypes.c_void_p(npts.ctypes.data), ctypes.c_void_p(nt.ctypes.data),
                                    ctypes.c_void_p(rej.ctypes.data), ctypes.c_void_p(st.ctypes.data), _lib.MEM_HOST)
-        xs = self._result(r, x, (max_points,), st)
+        xs, _, _ = self._result(r, x, (max_points,), st)
         return t.reshape(ni, max_points), xs, npts, nt, rej
+
+    def adjoint_forward(self, h, nsteps, cvec, method=_lib.TRAP, precision=_lib.FP64, checkpoint_every=64,
+                        config=None):
+        """Objective J = sum_s w_s c.x_s of a fixed-step linear transient;
+        keeps the checkpoints for adjoint_backward. Returns (J, status)."""
+        cfg = (config or SolverConfig(compat=False)).to_c()
+        c = np.ascontiguousarray(cvec, np.float64)
+        J = np.zeros(self.n_instances)
+        st = np.zeros(self.n_instances, np.int32)
+        r = lib().kg_adjoint_forward(self._h, ctypes.byref(cfg), h, nsteps, method, precision,
+                                     c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), checkpoint_every,
+                                     ctypes.c_void_p(J.ctypes.data), ctypes.c_void_p(st.ctypes.data), _lib.MEM_HOST)
+        self._raise_or(r, st)
+        return J, st
+
+    def adjoint_backward(self):
+        """dJ/d(value) of every element for the last adjoint_forward."""
+        ne = int(self.n_elements)
+        g = np.zeros(self.n_instances * ne)
+        st = np.zeros(self.n_instances, np.int32)
+        r = lib().kg_adjoint_backward(self._h, ctypes.c_void_p(g.ctypes.data), ctypes.c_void_p(st.ctypes.data),
+                                      _lib.MEM_HOST)
+        self._raise_or(r, st)
+        return g.reshape(self.n_instances, ne), st
 
     def adjoint(self, h, nsteps, cvec, method=_lib.TRAP, precision=_lib.FP64, checkpoint_every=64, config=None):
         cfg = (config or SolverConfig(compat=False)).to_c()
         ni = self.n_instances
         c = np.ascontiguousarray(cvec, np.float64)
         J = np.zeros(ni)
-        ne = int(np.max(self.n_elements)) if not self.shared else int(self.n_elements)
-        g = np.zeros(ni * ne if self.shared else int(np.sum(self.n_elements)))
+        ne = int(self.n_elements)
+        g = np.zeros(ni * ne)
         st = np.zeros(ni, np.int32)
         r = lib().kg_adjoint_transient(self._h, ctypes.byref(cfg), h, nsteps, method, precision,
                                        c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), checkpoint_every,
                                        ctypes.c_void_p(J.ctypes.data), ctypes.c_void_p(g.ctypes.data),
                                        ctypes.c_void_p(st.ctypes.data), _lib.MEM_HOST)
         self._raise_or(r, st)
-        return J, (g.reshape(ni, ne) if self.shared else g), st
+        return J, g.reshape(ni, ne), st
 
     def profile(self, enable=True):
         """Returns (total_ms, launches, last_kernel) of the analysis kernels

@@ -14,7 +14,7 @@ FIELDS = ["n", "nnz", "ne", "nlev", "nblev", "ntlev", "ntblev", "nonlinear", "ha
           "o_row_ptr", "o_col", "o_diag", "o_fprog_ptr", "o_fprog", "o_lvl_ptr", "o_lvl_rows", "o_blvl_ptr",
           "o_blvl_rows", "o_scon_ptr", "o_scon", "o_rcon_ptr", "o_rcon", "o_col_of", "o_cpos", "o_isbr",
           "o_etype", "o_ep", "o_em", "o_ec", "o_ut_ptr", "o_ut", "o_lt_ptr", "o_lt", "o_tlvl_ptr", "o_tlvl_rows",
-          "o_tblvl_ptr", "o_tblvl_rows", "o_row_of", "status", "total"]
+          "o_tblvl_ptr", "o_tblvl_rows", "o_row_of", "o_rpos", "o_erp", "o_erm", "o_erc", "status", "total"]
 DC, AC, TRAN = 0, 1, 2
 NATURAL, MINDEG, NESTED = 0, 1, 2
 
