@@ -1,0 +1,236 @@
+"""Measurement modes of bench.py for the other workloads (--config c1|c3|c4|c5).
+
+Each function returns one JSON-able dict: throughput of the batched GPU paths,
+the plain FP64 GPU path where it applies, errors against the CPU oracle on a
+sample, the dominant kernel's average duration and its HBM roofline share.
+"""
+import time
+
+import numpy as np
+import torch
+
+import krets_amd as kg
+from krets_amd import workloads as W
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _oracle():
+    try:
+        from oracle import refcpu
+        refcpu.lib()
+        return refcpu
+    except Exception:
+        return None
+
+
+def _nerr(x, r):
+    d = np.linalg.norm(r)
+    return float(np.linalg.norm(np.asarray(x) - r) / (d if d > 0 else 1.0))
+
+
+def _timed(ctx, fn, reps):
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = fn()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    kms, nl, name = ctx.profile(False)
+    return dt, kms / max(nl, 1), name, res
+
+
+def c1(args):
+    """Plain FP64 DC operating point of the 1000-section series+shunt ladder."""
+    text = W.c1_ladder()
+    c = kg.Circuit(text)
+    ctx = kg.Context()
+    ctx.load(c.elements, c.params[None], c.n)
+    ctx.op()
+    dt, kms, kname, (xs, st, it) = _timed(ctx, lambda: ctx.op(), 20)
+    x = xs[0]
+    out = {"config": "c1 series+shunt ladder OP", "n": c.n, "solves_per_s": 1.0 / dt, "ms_per_solve": 1e3 * dt,
+           "kernel": kname, "avg_kernel_ms": kms}
+    phi = (1 + 5 ** 0.5) / 2
+    out["I(V1)_vs_closed_form_rel"] = abs(x[c.index_map["I(V1)"]] / (-phi * 1e-3) - 1)
+    R = _oracle()
+    if R:
+        oc = R.Circuit(text)
+        xr, _ = oc.op()
+        out["normwise_err_vs_oracle"] = _nerr(x, xr)
+        cfg = R.config(cost=R.COST_COMPAT)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 5.0:
+            oc.op(cfg)
+            k += 1
+        out["cpu_baseline"] = {"value": k / (time.perf_counter() - t0), "unit": "solves/s", "cores": 1,
+                               "kind": "port", "sample": f"{k} OP solves, krets-cost restatement"}
+    bps = 8.0 * c.n + 8.0 * len(c.elements)  # solution written + element values read
+    out["roofline"] = {"bound": "hbm", "bytes_per_solve": bps, "achieved": bps / (kms / 1e3) / 1e9,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": bps / (kms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                       "note": "single 1002-unknown system: latency-bound level-scheduled LU"}
+    return out
+
+
+def _slice(flat, offs, i, nrow, n):
+    return flat[offs[i]:offs[i] + nrow * n].view(nrow, n)
+
+
+def c3(args):
+    """Batched trapezoidal transient of 2048 uneven RLC ladders, 2000 steps."""
+    b = W.c3_rlc_uneven(2048, seed=3)
+    ctx = kg.Context()
+    b.load(ctx)
+    h, N = 10e-12, 2000
+    nu = np.asarray(b.n_unknowns, np.int64)
+    offs = np.concatenate([[0], np.cumsum(nu * (N + 1))])
+    tot = int(offs[-1])
+    dev = torch.device("cuda:0")
+    buf = torch.empty(tot, dtype=torch.float64, device=dev)
+    res = {"config": "c3 2048 uneven RLC ladders, TRAP h=10ps x 2000", "sum_unknowns": int(nu.sum())}
+    sample = [0, 7, 1500]
+    ref = {}
+    for prec, key in ((kg.FP64, "fp64"), (kg.MIXED, "mixed")):
+        ctx.transient(h, N, method=kg.TRAP, precision=prec, out=buf)  # autotune + warm
+        trials = ctx.last_autotune()
+        dt, kms, kname, (r, st, err) = _timed(ctx, lambda: ctx.transient(h, N, method=kg.TRAP, precision=prec,
+                                                                         out=buf), 1)
+        assert (st == 0).all()
+        solves = int(len(nu)) * N
+        bytes_ = 8.0 * float(nu.sum()) * (N + 1)
+        res[key] = {"solves_per_s": solves / dt, "ms": 1e3 * dt, "kernel": kname, "avg_kernel_ms": kms,
+                    "autotune": trials,
+                    "roofline": {"bound": "hbm", "achieved": bytes_ / (kms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": bytes_ / (kms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                                 "traffic_note": "compulsory: every unknown of every step written once"}}
+        got = {i: _slice(buf, offs, i, N + 1, int(nu[i])).cpu().numpy() for i in sample}
+        if prec == kg.FP64:
+            ref = got
+        else:
+            res["mixed"]["max_normwise_err_vs_fp64_gpu"] = max(
+                max(_nerr(got[i][s], ref[i][s]) for s in range(1, N + 1)) for i in sample)
+    R = _oracle()
+    if R:
+        e = 0.0
+        for i in sample[:2]:
+            xr, _ = R.Circuit(b.netlists[i]).tran(h, N * h, method=R.TRAP)
+            e = max(e, max(_nerr(ref[i][s], xr[s]) for s in range(1, N + 1)))
+        res["fp64_normwise_err_vs_oracle"] = e
+    # plain FP64 path: one instance per call
+    pc = kg.Context()
+    t = 0.0
+    for i in sample:
+        o = b.elem_offset
+        pc.load(b.elements[o[i]:o[i + 1]], b.params[o[i]:o[i + 1]], [b.n_unknowns[i]], [0, o[i + 1] - o[i]])
+        pc.transient(h, N, method=kg.TRAP, config=kg.SolverConfig(compat=False, threads_per_system=64))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pc.transient(h, N, method=kg.TRAP, config=kg.SolverConfig(compat=False, threads_per_system=64))
+        t += time.perf_counter() - t0
+    res["plain_fp64_solves_per_s"] = len(sample) * N / t
+    return res
+
+
+def c4(args):
+    """1024 half-wave rectifiers: LTE-controlled backward Euler over 5 ms."""
+    b = W.c4_rectifier(1024, seed=4)
+    ctx = kg.Context()
+    b.load(ctx)
+    cfg = kg.SolverConfig(compat=False, voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12)
+    a = (5e-3, 1e-6, 1e-14, 5e-5, 1e-3, 1e-6)
+    ctx.transient_lte(*a, 4000, cfg)
+    dt, kms, kname, (t, xs, npts, nt, rej) = _timed(ctx, lambda: ctx.transient_lte(*a, 4000, cfg), 3)
+    res = {"config": "c4 1024 rectifiers, BE + LTE over 5 ms", "s": dt, "kernel": kname, "avg_kernel_ms": kms,
+           "accepted_steps_total": int(npts.sum()), "newton_iterations_total": int(nt.sum()),
+           "rejected_total": int(rej.sum()), "newton_per_step_mean": float(nt.sum() / max(npts.sum(), 1)),
+           "newton_per_step_min_instance": float((nt / np.maximum(npts, 1)).min()),
+           "newton_per_step_max_instance": float((nt / np.maximum(npts, 1)).max()),
+           "solves_per_s": float(nt.sum()) / dt}
+    R = _oracle()
+    if R:
+        ok = 0
+        e = 0.0
+        for i in (0, 511, 1023):
+            tr, xr, ntr, rejr = R.Circuit(b.netlists[i]).tran_lte(*a, cfg=R.config(vabstol=1e-9, iabstol=1e-12))
+            same = npts[i] == len(tr) and nt[i] == ntr and rej[i] == rejr
+            ok += int(same)
+            if same:
+                e = max(e, max(_nerr(xs[i][j], xr[j]) for j in range(len(tr))))
+        res["oracle_same_step_sequence"] = f"{ok}/3"
+        res["normwise_err_vs_oracle"] = e
+    # fixed-step BE-compat parity run at h = 1 us
+    ccfg = kg.SolverConfig(voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12)
+    dt2, kms2, kname2, (xs2, st2, (it2, _)) = _timed(ctx, lambda: ctx.transient(1e-6, 5000, method=kg.BE_COMPAT,
+                                                                                config=ccfg), 1)
+    res["be_compat_h1us"] = {"s": dt2, "newton_iterations_total": int(it2.sum()),
+                             "solves_per_s": float(it2.sum()) / dt2, "all_converged": bool((st2 == 0).all())}
+    if R:
+        xr, itr = R.Circuit(b.netlists[3]).tran(1e-6, 5e-3, cfg=R.config(vabstol=1e-9, iabstol=1e-12))
+        res["be_compat_h1us"]["err_vs_oracle"] = max(_nerr(xs2[3][k], xr[k]) for k in range(xr.shape[0]))
+        res["be_compat_h1us"]["same_newton_counts"] = bool(np.array_equal(it2[3], itr))
+    return res
+
+
+def c5(args):
+    """Adjoint gradients over 256 200-section RC ladders (TRAP, h = 1 ns, 2000 steps)."""
+    b = W.c5_rc_ladder(256, seed=5)
+    ctx = kg.Context()
+    b.load(ctx)
+    h, N = 1e-9, 2000
+    n = b.n_unknowns
+    res = {"config": "c5 256 RC ladders (200 sections), adjoint, TRAP h=1ns x 2000, checkpoint 64"}
+    R = _oracle()
+    for name, node in (("spec_J_Vn200", "V(n200)"), ("companion_J_Vn1", "V(n1)")):
+        c = np.zeros(n)
+        c[b.names.index(node)] = 1.0
+        r = {}
+        for prec, key in ((kg.FP64, "fp64"), (kg.MIXED, "mixed")):
+            cfg = kg.SolverConfig(compat=False)
+            buf = torch.empty(256 * n * (N + 1), dtype=torch.float64, device="cuda:0")
+            ctx.transient(h, N, method=kg.TRAP, precision=prec, config=cfg, out=buf)  # autotune + warm
+            dtf, kf, _, _ = _timed(ctx, lambda: ctx.transient(h, N, method=kg.TRAP, precision=prec, config=cfg,
+                                                                out=buf), 1)
+            del buf
+            ctx.adjoint(h, N, c, precision=prec)
+            dta, ka, _, (J, st) = _timed(ctx, lambda: ctx.adjoint_forward(h, N, c, precision=prec), 1)
+            dtb, kb, _, (g, st2) = _timed(ctx, lambda: ctx.adjoint_backward(), 1)
+            # the backward timing covers k_adj_bwd and the operating-point kernel (kb is their mean)
+            r[key] = {"forward_transient_kernel_ms": kf, "adjoint_forward_kernel_ms": ka,
+                      "adjoint_backward_wall_s": dtb, "forward_transient_wall_s": dtf,
+                      "backward_over_forward_wall": dtb / dtf,
+                      "J_mean": float(np.mean(J)), "J_zero_count": int((J == 0).sum())}
+            if R and key == "fp64":
+                errs = []
+                for i in (0, 255):
+                    Jr, gr = R.Circuit(b.netlists[i]).adjoint(h, N, c, method=R.TRAP)
+                    if np.max(np.abs(gr)) > 0:
+                        errs.append(float(np.max(np.abs(g[i] - gr)) / np.max(np.abs(gr))))
+                r[key]["grad_err_vs_oracle_fp64"] = max(errs) if errs else None
+            if key == "mixed":
+                r[key]["grad_rel_diff_vs_fp64_gpu"] = float(np.max(np.abs(g - r["_g"])) / max(np.max(np.abs(r["_g"])), 1e-300))
+            else:
+                r["_g"] = g
+        r.pop("_g")
+        # central differences (relative step 1e-6) of the GPU FP64 forward on 4 parameters
+        if name == "companion_J_Vn1":
+            i = 3
+            fds = []
+            for e in (1, 2, 200, 399):
+                p0 = b.params.copy()
+                v = p0[i, e, 0]
+                Js = []
+                for sgn in (1, -1):
+                    p = p0.copy()
+                    p[i, e, 0] = v + sgn * 1e-6 * v
+                    ctx.load(b.elements, p, n)
+                    Jp, _ = ctx.adjoint_forward(h, N, c)
+                    Js.append(Jp[i])
+                fds.append((Js[0] - Js[1]) / (2e-6 * v))
+            b.load(ctx)
+            ctx.adjoint_forward(h, N, c)
+            g, _ = ctx.adjoint_backward()
+            r["fd_check"] = [{"param": e, "fd": fd, "adjoint": float(g[i, e])} for e, fd in zip((1, 2, 200, 399), fds)]
+        res[name] = r
+    return res

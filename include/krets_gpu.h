@@ -178,6 +178,13 @@ int32_t kg_adjoint_backward(kg_ctx* ctx, double* grad, int32_t* status, int32_t 
  * (enable = 1) or off. */
 int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launches, const char** last_kernel);
 
+/* Autotuner report (no krets counterpart): the candidate mappings (threads
+ * per system) timed by the most recent autotuning run and their kernel times
+ * in ms. Returns the number of candidates (copying at most cap of them). The
+ * winner is cached per workload in the context, and in the JSON file named by
+ * the environment variable KRETS_GPU_TUNE_FILE when it is set. */
+int32_t kg_last_autotune(const kg_ctx* ctx, int32_t* tps, float* ms, int32_t cap);
+
 /* Diagnostics (no krets counterpart): the symbolic plan the library would
  * build for one topology. kind: 0 DC, 1 AC, 2 transient; ordering: 0 natural,
  * 1 minimum degree, 2 nested dissection. params (may be NULL): one instance's

@@ -28,8 +28,7 @@ def build(verbose=False):
         obj = os.path.join(objdir, src + ".o")
         cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
                "-c", path, "-o", obj]
-        if src.endswith(".hip"):
-            cmd[1:1] = DEVICE_FLAGS
+        cmd[1:1] = DEVICE_FLAGS  # hipcc compiles .cpp as HIP too: target gfx950 only
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
     for src, p in procs:

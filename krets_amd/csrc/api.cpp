@@ -6,8 +6,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <functional>
 #include <map>
+#include <sstream>
 #include <string>
 #include <tuple>
 #include <unordered_map>
@@ -73,6 +78,10 @@ struct kg_ctx {
     int32_t refine = 2;
     Lay lay{};
   } adj;
+  // autotuner: best threads-per-system per workload key
+  std::map<std::string, int> tune;
+  std::string tune_file;
+  std::vector<std::pair<int, float>> last_trials;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
   double prof_ms = 0.0;
@@ -354,6 +363,91 @@ int32_t timed(kg_ctx* c, const char* name, F&& launch) {
   return KG_OK;
 }
 
+// ---------------------------------------------------------------- autotuner
+// Picks the work mapping (threads per system and, with it, the ordering: a
+// minimum-degree order for one thread per system, nested dissection for
+// level-scheduled groups) by timing every feasible candidate on the actual
+// workload once, then caches the winner per workload key.
+void tune_load(kg_ctx* c) {
+  const char* f = std::getenv("KRETS_GPU_TUNE_FILE");
+  if (!f || !*f) return;
+  c->tune_file = f;
+  std::ifstream in(f);
+  std::string line;
+  while (std::getline(in, line)) {  // lines: "key" : tps
+    size_t a = line.find('"'), b = line.find('"', a + 1), col = line.find(':', b);
+    if (a == std::string::npos || b == std::string::npos || col == std::string::npos) continue;
+    c->tune[line.substr(a + 1, b - a - 1)] = std::atoi(line.c_str() + col + 1);
+  }
+}
+
+void tune_save(kg_ctx* c) {
+  if (c->tune_file.empty()) return;
+  std::ofstream out(c->tune_file);
+  out << "{\n";
+  size_t k = 0;
+  for (auto& kv : c->tune) out << "  \"" << kv.first << "\": " << kv.second << (++k < c->tune.size() ? ",\n" : "\n");
+  out << "}\n";
+}
+
+std::string tune_key(const kg_ctx* c, const char* what, int precision, const Plan& P, int64_t nsys) {
+  std::ostringstream k;
+  int lg = 0;
+  while ((int64_t(1) << (lg + 1)) <= nsys) lg++;
+  k << what << "|p" << precision << "|n" << P.n_max << "|nnz" << P.nnz_max << "|ne" << P.ne_max << "|s2^" << lg
+    << (c->shared ? "|shared" : "|uneven");
+  return k.str();
+}
+
+// trial(tps, &ms) runs the workload with that mapping; returns a kg_status.
+int32_t autotune(kg_ctx* c, const std::string& key, const std::function<int32_t(int, float*)>& trial, int* best) {
+  auto it = c->tune.find(key);
+  if (it != c->tune.end()) {
+    *best = it->second;
+    return KG_OK;
+  }
+  c->last_trials.clear();
+  float best_ms = 0.f;
+  *best = 0;
+  for (int tps : {1, 4, 16, 64, 256}) {
+    float ms = 0.f;
+    int32_t r = trial(tps, &ms);
+    if (r == KG_HIP_ERROR) return r;
+    if (r != KG_OK) continue;  // infeasible mapping (on-chip memory)
+    c->last_trials.push_back({tps, ms});
+    if (*best == 0 || ms < best_ms) {
+      best_ms = ms;
+      *best = tps;
+    }
+  }
+  if (*best == 0) return set_err(c, KG_UNEXPECTED, "no feasible work mapping for this workload");
+  c->tune[key] = *best;
+  tune_save(c);
+  c->err.clear();
+  return KG_OK;
+}
+
+// Times one launch with HIP events (used by the autotuner).
+template <class F>
+int32_t time_launch(kg_ctx* c, F&& launch, float* ms) {
+  hipEvent_t a, b;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  HIPCHK(c, hipEventRecord(a, c->stream));
+  hipError_t e = launch();
+  if (e != hipSuccess) {
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return set_err(c, KG_HIP_ERROR, hipGetErrorString(e));
+  }
+  HIPCHK(c, hipEventRecord(b, c->stream));
+  HIPCHK(c, hipEventSynchronize(b));
+  HIPCHK(c, hipEventElapsedTime(ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return KG_OK;
+}
+
 NewtonArgs newton_args(const kg_solver_config* cfg) {
   return NewtonArgs{cfg->reltol, cfg->iabstol, cfg->vabstol, cfg->maxiter, cfg->compat};
 }
@@ -441,6 +535,7 @@ int32_t kg_create(kg_ctx** out, int32_t device) {
   if (device < 0 || device >= n) return KG_INVALID_ARGUMENT;
   auto* c = new kg_ctx();
   c->device = device;
+  tune_load(c);
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return KG_HIP_ERROR;
@@ -614,13 +709,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   for (int k = 0; k < nf; k++) fmin = std::min(fmin, freqs[k]), fmax = std::max(fmax, freqs[k]);
   double s_ac = 2.0 * 3.14159265358979323846 * std::sqrt(fmin * fmax);
   if ((r = prepare(c, AK_AC, ORD_NESTED, s_ac, P0))) return r;
-  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
-  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  if ((r = prepare(c, AK_AC, order_for(tps), s_ac, P))) return r;
   bool mixed = precision == KG_MIXED;
-  Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 0, P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
-  Launch L;
-  if ((r = configure(c, tps, lay, (int64_t)c->n_inst * nf, L))) return r;
+  int refine = cfg->refine_iters > 0 ? cfg->refine_iters : 2;
   auto off = out_offsets(c, 2 * (int64_t)nf);
   int64_t* d_off;
   if ((r = upload_offsets(c, "ac_off", off, &d_off))) return r;
@@ -631,8 +721,45 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   double* d_out = (mem == KG_MEM_DEVICE && x) ? x : (double*)ws_get(c, "ac_out", sizeof(double) * off.back(), &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  // one AC launch with a given mapping
+  auto setup = [&](int tps, Plan& PP, Lay& lay, Launch& L) -> int32_t {
+    int32_t rr = prepare(c, AK_AC, order_for(tps), s_ac, PP);
+    if (rr) return rr;
+    lay = Lay{PP.nnz_max, PP.n_max, PP.ne_max, 0, 0, PP.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
+    rr = configure(c, tps, lay, (int64_t)c->n_inst * nf, L);
+    if (rr) return rr;
+    if (tps * L.spb < 64 && tps < 64) return set_err(c, KG_UNEXPECTED, "mapping leaves most of a wave idle");
+    return KG_OK;
+  };
+  int tps = cfg->threads_per_system;
+  if (tps == 0) {
+    r = autotune(c, tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf), [&](int t, float* ms) -> int32_t {
+      Plan PP;
+      Lay lay;
+      Launch L;
+      int32_t rr = setup(t, PP, lay, L);
+      if (rr) return rr;
+      BatchDev B = batch_dev(c, PP);
+      return time_launch(c, [&] {
+        return run_ac(t, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out,
+                      d_off, d_st, d_err, refine, lay);
+      }, ms);
+    }, &tps);
+    if (r) return r;
+  }
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  Lay lay;
+  Launch L;
+  r = setup(tps, P, lay, L);
+  if (r && cfg->threads_per_system == 0) return r;
+  if (r) {  // an explicitly requested mapping may leave part of a wave idle
+    c->err.clear();
+    if ((r = prepare(c, AK_AC, order_for(tps), s_ac, P))) return r;
+    lay = Lay{P.nnz_max, P.n_max, P.ne_max, 0, 0, P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
+    if ((r = configure(c, tps, lay, (int64_t)c->n_inst * nf, L))) return r;
+  }
   HIPCHK(c, hipMemsetAsync(d_err, 0, sizeof(double) * c->n_inst, c->stream));
-  int refine = cfg->refine_iters > 0 ? cfg->refine_iters : 2;
   BatchDev B = batch_dev(c, P);
   r = timed(c, mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>", [&] {
     return run_ac(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out, d_off,
@@ -671,13 +798,7 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   Plan P0, P;
   double s_tr = (method == KG_TRAP ? 2.0 : 1.0) / h;
   if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P0))) return r;
-  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
-  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
   bool mixed = precision == KG_MIXED;
-  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
-  Launch L;
-  if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
   auto off = out_offsets(c, (int64_t)nsteps + 1);
   int64_t* d_off;
   if ((r = upload_offsets(c, "tran_off", off, &d_off))) return r;
@@ -686,10 +807,42 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   int32_t* d_iters = (int32_t*)ws_get(c, "tran_iters", sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), &e);
   double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
+  int32_t* d_st0 = (int32_t*)ws_get(c, "tran_status0", sizeof(int32_t) * c->n_inst, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  // the kernel overwrites the status array: keep the operating-point statuses
+  HIPCHK(c, hipMemcpyAsync(d_st0, d_st, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
   TranArgs ta{h, method == KG_TRAP ? 2.0 : 1.0, nsteps, method, cfg->refine_iters > 0 ? cfg->refine_iters : 2};
   NewtonArgs na = newton_args(cfg);
   na.compat = method == KG_BE_COMPAT ? 1 : 0;
+  auto setup = [&](int tps, Plan& PP, Lay& lay, Launch& L) -> int32_t {
+    int32_t rr = prepare(c, AK_TRAN, order_for(tps), s_tr, PP);
+    if (rr) return rr;
+    lay = Lay{PP.nnz_max, PP.n_max, PP.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
+    return configure(c, tps, lay, c->n_inst, L);
+  };
+  int tps = cfg->threads_per_system;
+  if (tps == 0) {
+    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), [&](int t, float* ms) -> int32_t {
+      Plan PP;
+      Lay lay;
+      Launch L;
+      int32_t rr = setup(t, PP, lay, L);
+      if (rr) return rr;
+      if (c->shared && t * L.spb < 64 && t < 64) return set_err(c, KG_UNEXPECTED, "sub-wave mapping");
+      BatchDev B = batch_dev(c, PP);
+      HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
+      return time_launch(c, [&] {
+        return run_tran(t, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out,
+                        d_off, d_st, d_iters, d_err, lay);
+      }, ms);
+    }, &tps);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  Lay lay;
+  Launch L;
+  if ((r = setup(tps, P, lay, L))) return r;
   BatchDev B = batch_dev(c, P);
   r = timed(c, mixed ? "k_tran<float,double>" : "k_tran<double,double>", [&] {
     return run_tran(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out, d_off,
@@ -709,6 +862,16 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
       break;
     }
   return finish_status(c, d_st, status, mem);
+}
+
+int32_t kg_last_autotune(const kg_ctx* c, int32_t* tps, float* ms, int32_t cap) {
+  if (!c) return -KG_INVALID_ARGUMENT;
+  int32_t n = (int32_t)c->last_trials.size();
+  for (int32_t k = 0; k < std::min(n, cap); k++) {
+    if (tps) tps[k] = c->last_trials[k].first;
+    if (ms) ms[k] = c->last_trials[k].second;
+  }
+  return n;
 }
 
 int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launches, const char** last_kernel) {

@@ -174,6 +174,16 @@ __device__ DiodeLin diode_eval(double vd, double is, double n, bool compat, doub
   return r;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2. Remapping gives every XCD a contiguous range of logical
+// blocks, so systems that share an instance's element data (consecutive
+// logical blocks) hit the same L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Per-point stamp parameters.
 struct StampArgs {
   int ak;            // AnalysisKind
@@ -409,7 +419,7 @@ __device__ double sys_max(double v, double* red, int ls, int t) {
 template <class F, class A, int TPS>
 __device__ double solve_refine(const PatDev& P, const F* v, const A* b, A* x, F* w, const double* prm,
                                const double* eb, const StampArgs& sa, const double* ed, int refine, double* red,
-                               int ls, int t, bool act) {
+                               int ls, int t, bool act, A* rbuf = nullptr) {
   if constexpr (std::is_same<F, A>::value) {
     if (act)
       for (int i = t; i < P.n; i += TPS) x[i] = b[i];
@@ -426,7 +436,7 @@ __device__ double solve_refine(const PatDev& P, const F* v, const A* b, A* x, F*
     __syncthreads();
     double est = 0.0, dprev = 0.0;
     for (int it = 0; it < refine; it++) {
-      // r = b - A x in FP64 (A gathered in FP64)
+      // r = b - A x in FP64 (A gathered in FP64), kept in rbuf
       double rmax = 0.0;
       if (act)
         for (int i = t; i < P.n; i += TPS) {
@@ -439,27 +449,16 @@ __device__ double solve_refine(const PatDev& P, const F* v, const A* b, A* x, F*
             r -= a * x[P.col[p]];
           }
           rmax = fmax(rmax, absv(r));
-          w[i] = cvt<F>(r);  // provisional, rescaled below
-          // keep the unscaled residual in x's shadow: recompute after scaling
+          rbuf[i] = r;
         }
       double s = sys_max<TPS>(rmax, red, ls, t);
-      // rescale: recompute residual into w with scaling 1/s (avoids FP32 range loss)
+      // scaled FP32 correction right-hand side (keeps the residual in range)
       if (act && s > 0.0) {
         double inv = 1.0 / s;
-        for (int i = t; i < P.n; i += TPS) {
-          A r = b[i];
-          for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) {
-            int q0 = P.scon_ptr[p], q1 = P.scon_ptr[p + 1];
-            if (q0 == q1) continue;
-            A a = mk<A>(0.0, 0.0);
-            for (int q = q0; q < q1; q++) a += mat_contrib<A>(P.scon[q], P, prm, eb, sa, ed);
-            r -= a * x[P.col[p]];
-          }
-          w[i] = cvt<F>(scal(inv, r));
-        }
+        for (int i = t; i < P.n; i += TPS) w[i] = cvt<F>(scal(inv, rbuf[i]));
       }
       __syncthreads();
-      if (s > 0.0) lu_solve<F, TPS>(P, v, w, t, act);
+      lu_solve<F, TPS>(P, v, w, t, act && s > 0.0);  // barriers are reached by every system
       double dmax = 0.0, xmax = 0.0;
       if (act && s > 0.0)
         for (int i = t; i < P.n; i += TPS) {
@@ -491,6 +490,7 @@ struct SysMem {
   A* x;
   A* b;
   F* w;
+  A* r;  // mixed precision: FP64 residual
   double* g;   // Newton guess (FP64)
   double* xp;  // previous solution(s) (transient); extra vectors follow at xp + k * xstride
   int xstride;
@@ -512,9 +512,12 @@ __device__ SysMem<F, A> carve(char* base, const Lay& L) {
   p += lay_al(sizeof(A) * (size_t)L.n_max);
   if (std::is_same<F, A>::value) {
     m.w = (F*)m.x;
+    m.r = nullptr;
   } else {
     m.w = (F*)p;
     p += lay_al(sizeof(F) * (size_t)L.n_max);
+    m.r = (A*)p;
+    p += lay_al(sizeof(A) * (size_t)L.n_max);
   }
   m.g = L.need_g ? (double*)p : nullptr;
   if (L.need_g) p += lay_al(8 * (size_t)L.n_max);
@@ -579,7 +582,7 @@ __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = blockIdx.x * spb + ls;
+  const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + (valid ? inst : 0)]];
   size_t sb = sys_bytes(lay);
@@ -683,7 +686,7 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int64_t sys = (int64_t)blockIdx.x * spb + ls;
+  const int64_t sys = xcd_block() * spb + ls;
   const int64_t nsys = (int64_t)B.n_instances * nf;
   const bool valid = sys < nsys;
   const int inst = valid ? (int)(sys / nf) : 0;
@@ -711,7 +714,7 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
   stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, nullptr, t, valid);
   __syncthreads();
   factor<F, TPS>(P, M.v, t, valid, &M.flags[0]);
-  double est = solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, refine, red, ls, t, valid);
+  double est = solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, refine, red, ls, t, valid, M.r);
   __syncthreads();
   if (valid) {
     double* o = out + out_off[inst] + (int64_t)fi * P.n * 2;
@@ -737,7 +740,7 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = blockIdx.x * spb + ls;
+  const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
@@ -789,7 +792,7 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
       stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, act);
       __syncthreads();
       double est =
-          solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, act);
+          solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, act, M.r);
       est_max = fmax(est_max, est);
     } else {
       if (act)
@@ -816,7 +819,7 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
         __syncthreads();
         factor<F, TPS>(P, M.v, t, a2, &M.flags[0]);
         double est =
-            solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, a2);
+            solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, a2, M.r);
         est_max = fmax(est_max, est);
         __syncthreads();
         // convergence against the guess (FP64 copies)
@@ -898,7 +901,7 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = blockIdx.x * spb + ls;
+  const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
@@ -1104,7 +1107,7 @@ __global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = blockIdx.x * spb + ls;
+  const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
@@ -1140,7 +1143,7 @@ __global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
       sa.tsrc = (double)s * aa.h;
       stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, act);
       __syncthreads();
-      solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, aa.refine, red, ls, t, act);
+      solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, aa.refine, red, ls, t, act, M.r);
       __syncthreads();
       if (act && trap)
         for (int e = t; e < P.ne; e += TPS) {
@@ -1188,7 +1191,7 @@ __global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = blockIdx.x * spb + ls;
+  const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
@@ -1240,7 +1243,7 @@ __global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
       sa.tsrc = (double)s * h;
       stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, act);
       __syncthreads();
-      solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, aa.refine, red, ls, t, act);
+      solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, aa.refine, red, ls, t, act, M.r);
       __syncthreads();
       if (act && trap)
         for (int e = t; e < P.ne; e += TPS) {
@@ -1323,7 +1326,7 @@ __global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
           if (act && sc > 0.0)
             for (int j = t; j < P.n; j += TPS) M.w[j] = cvt<F>(rr[j] / sc);
           __syncthreads();
-          if (sc > 0.0) lu_solve_t<F, TPS>(P, M.v, M.w, t, act);
+          lu_solve_t<F, TPS>(P, M.v, M.w, t, act && sc > 0.0);  // barriers are reached by every system
           if (act && sc > 0.0)
             for (int j = t; j < P.n; j += TPS) lam[j] += sc * re_of(M.w[j]);
           __syncthreads();
@@ -1395,7 +1398,7 @@ __global__ void k_adj_op(BatchDev B, int kind_slot, const double* xbar0, const d
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = blockIdx.x * spb + ls;
+  const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];

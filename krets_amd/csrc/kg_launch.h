@@ -37,7 +37,7 @@ __host__ __device__ inline size_t lay_al(size_t b) { return (b + 15) & ~size_t(1
 
 __host__ __device__ inline size_t sys_bytes(const Lay& L) {
   size_t s = 64 + lay_al((size_t)L.fsz * L.nnz_max) + 2 * lay_al((size_t)L.asz * L.n_max);
-  if (L.mixed) s += lay_al((size_t)L.fsz * L.n_max);
+  if (L.mixed) s += lay_al((size_t)L.fsz * L.n_max) + lay_al((size_t)L.asz * L.n_max);
   if (L.need_g) s += lay_al(8 * (size_t)L.n_max);
   s += (size_t)L.need_xp * lay_al(8 * (size_t)L.n_max);
   if (L.need_ed) s += lay_al(8 * (size_t)ESTRIDE * L.ne_max);

@@ -328,6 +328,14 @@ class Context:
         self._raise_or(r, st)
         return J, g.reshape(ni, ne), st
 
+    def last_autotune(self):
+        """[(threads_per_system, kernel_ms)] timed by the last autotuning run."""
+        t = np.zeros(8, np.int32)
+        m = np.zeros(8, np.float32)
+        k = lib().kg_last_autotune(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                   m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 8)
+        return [(int(t[i]), float(m[i])) for i in range(max(k, 0))]
+
     def profile(self, enable=True):
         """Returns (total_ms, launches, last_kernel) of the analysis kernels
         timed since the previous call, then resets and sets timing on/off."""
