@@ -292,7 +292,8 @@ Ordering order_for(int tps) { return tps == 1 ? ORD_MINDEG : ORD_NESTED; }
 int32_t configure(kg_ctx* c, int tps, Lay lay, int64_t nsys, Launch& L) {
   size_t sb = sys_bytes(lay);
   L.tps = tps;
-  int spb = c->shared ? std::max(1, 256 / tps) : 1;
+  // one wave per block: small batches spread over more CUs, barriers stay cheap
+  int spb = c->shared ? std::max(1, 64 / tps) : 1;
   while (spb > 1 && spb * sb + 8 * (size_t)spb * tps > LDS_SHARED_BUDGET) spb /= 2;
   if (spb * sb + 8 * (size_t)spb * tps > LDS_MAX)
     return set_err(c, KG_UNEXPECTED,
