@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default) prints the benchmark line; the others print measurement lines of the "
                          "other workloads")
+    ap.add_argument("--c3-instances", type=int, default=2048)
+    ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--probe", action="store_true",
                     help="profiling probe: one mixed and one FP64 sweep, then a 1 GiB device copy whose "
                          "counters calibrate the HBM byte counters; prints no bench line")

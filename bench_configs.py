@@ -4,6 +4,7 @@ Each function returns one JSON-able dict: throughput of the batched GPU paths,
 the plain FP64 GPU path where it applies, errors against the CPU oracle on a
 sample, the dominant kernel's average duration and its HBM roofline share.
 """
+import sys
 import time
 
 import numpy as np
@@ -13,6 +14,14 @@ import krets_amd as kg
 from krets_amd import workloads as W
 
 HBM_PEAK_GBS = 8000.0
+
+
+_T0 = time.perf_counter()
+
+
+def _say(msg):
+    """Progress line on stderr (long runs must keep printing)."""
+    print(f"[{time.perf_counter() - _T0:8.2f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def _oracle():
@@ -80,21 +89,27 @@ def _slice(flat, offs, i, nrow, n):
 
 def c3(args):
     """Batched trapezoidal transient of 2048 uneven RLC ladders, 2000 steps."""
-    b = W.c3_rlc_uneven(2048, seed=3)
+    ni = getattr(args, "c3_instances", 2048)
+    N = getattr(args, "c3_steps", 2000)
+    b = W.c3_rlc_uneven(ni, seed=3)
+    _say(f"c3: {ni} instances generated")
     ctx = kg.Context()
     b.load(ctx)
-    h, N = 10e-12, 2000
+    _say("c3: batch loaded")
+    h = 10e-12
     nu = np.asarray(b.n_unknowns, np.int64)
     offs = np.concatenate([[0], np.cumsum(nu * (N + 1))])
     tot = int(offs[-1])
     dev = torch.device("cuda:0")
     buf = torch.empty(tot, dtype=torch.float64, device=dev)
-    res = {"config": "c3 2048 uneven RLC ladders, TRAP h=10ps x 2000", "sum_unknowns": int(nu.sum())}
-    sample = [0, 7, 1500]
+    res = {"config": f"c3 {ni} uneven RLC ladders, TRAP h=10ps x {N}", "sum_unknowns": int(nu.sum())}
+    sample = [0, 7, ni - 1]
     ref = {}
     for prec, key in ((kg.FP64, "fp64"), (kg.MIXED, "mixed")):
+        _say(f"c3 {key}: first call (plans + autotune)")
         ctx.transient(h, N, method=kg.TRAP, precision=prec, out=buf)  # autotune + warm
         trials = ctx.last_autotune()
+        _say(f"c3 {key}: autotune {trials}")
         dt, kms, kname, (r, st, err) = _timed(ctx, lambda: ctx.transient(h, N, method=kg.TRAP, precision=prec,
                                                                          out=buf), 1)
         assert (st == 0).all()
@@ -118,6 +133,7 @@ def c3(args):
             xr, _ = R.Circuit(b.netlists[i]).tran(h, N * h, method=R.TRAP)
             e = max(e, max(_nerr(ref[i][s], xr[s]) for s in range(1, N + 1)))
         res["fp64_normwise_err_vs_oracle"] = e
+        _say("c3: oracle comparison done")
     # plain FP64 path: one instance per call
     pc = kg.Context()
     t = 0.0
@@ -129,6 +145,7 @@ def c3(args):
         t0 = time.perf_counter()
         pc.transient(h, N, method=kg.TRAP, config=kg.SolverConfig(compat=False, threads_per_system=64))
         t += time.perf_counter() - t0
+        _say(f"c3: plain path instance {i}")
     res["plain_fp64_solves_per_s"] = len(sample) * N / t
     return res
 

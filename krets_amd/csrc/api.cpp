@@ -181,6 +181,8 @@ PatDev to_dev(const PatLayout& L, const int32_t* b) {
   d.erp = b + L.o_erp;
   d.erm = b + L.o_erm;
   d.erc = b + L.o_erc;
+  d.base = b;
+  d.total = L.total;
   return d;
 }
 
@@ -401,7 +403,8 @@ std::string tune_key(const kg_ctx* c, const char* what, int precision, const Pla
 }
 
 // trial(tps, &ms) runs the workload with that mapping; returns a kg_status.
-int32_t autotune(kg_ctx* c, const std::string& key, const std::function<int32_t(int, float*)>& trial, int* best) {
+int32_t autotune(kg_ctx* c, const std::string& key, const std::vector<int>& cands,
+                 const std::function<int32_t(int, float*)>& trial, int* best) {
   auto it = c->tune.find(key);
   if (it != c->tune.end()) {
     *best = it->second;
@@ -410,7 +413,7 @@ int32_t autotune(kg_ctx* c, const std::string& key, const std::function<int32_t(
   c->last_trials.clear();
   float best_ms = 0.f;
   *best = 0;
-  for (int tps : {1, 4, 16, 64, 256}) {
+  for (int tps : cands) {
     float ms = 0.f;
     int32_t r = trial(tps, &ms);
     if (r == KG_HIP_ERROR) return r;
@@ -723,49 +726,74 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
-  // one AC launch with a given mapping
-  auto setup = [&](int tps, Plan& PP, Lay& lay, Launch& L) -> int32_t {
-    int32_t rr = prepare(c, AK_AC, order_for(tps), s_ac, PP);
+  // One AC launch for a mapping candidate: TPS threads per system, plus 1000
+  // for the persistent variant (shared topology only) that stages the plan
+  // and each instance's element records in LDS.
+  struct AcLaunch {
+    Plan P;
+    Lay lay;
+    Launch L;
+    bool persistent = false;
+    int tps = 0, plan_ints = 0;
+    int64_t chunk = 0;
+  };
+  auto setup = [&](int cand, AcLaunch& A) -> int32_t {
+    A.persistent = cand >= 1000;
+    A.tps = cand % 1000;
+    if (!valid_tps(A.tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+    int32_t rr = prepare(c, AK_AC, order_for(A.tps), s_ac, A.P);
     if (rr) return rr;
-    lay = Lay{PP.nnz_max, PP.n_max, PP.ne_max, 0, 0, PP.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
-    rr = configure(c, tps, lay, (int64_t)c->n_inst * nf, L);
-    if (rr) return rr;
-    if (tps * L.spb < 64 && tps < 64) return set_err(c, KG_UNEXPECTED, "mapping leaves most of a wave idle");
+    A.lay = Lay{A.P.nnz_max, A.P.n_max, A.P.ne_max, 0, 0, A.P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
+    if (!A.persistent) return configure(c, A.tps, A.lay, (int64_t)c->n_inst * nf, A.L);
+    if (!c->shared) return set_err(c, KG_UNEXPECTED, "persistent AC mapping needs a shared topology");
+    const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(0, (int)AK_AC, (int)order_for(A.tps), s_ac)]];
+    A.plan_ints = pe.lay.total;
+    size_t fixed = lay_al(4 * (size_t)A.plan_ints) + lay_al(8 * (size_t)A.P.ne_max * KG_NPARAM) +
+                   lay_al(8 * (size_t)A.P.ne_max);
+    size_t sb = sys_bytes(A.lay);
+    int spb = std::max(1, 64 / A.tps);
+    size_t lds = fixed + spb * sb + 8 * (size_t)spb * A.tps;
+    if (lds > LDS_MAX) return set_err(c, KG_UNEXPECTED, "persistent AC mapping does not fit in LDS");
+    int per_cu = std::max<int>(1, (int)std::min<size_t>(8, LDS_MAX / lds));
+    int64_t items = (int64_t)c->n_inst * ((nf + spb - 1) / spb);
+    int64_t grid = std::min<int64_t>(items, 256 * (int64_t)per_cu);
+    A.chunk = (items + grid - 1) / grid;
+    grid = (items + A.chunk - 1) / A.chunk;
+    A.L.tps = A.tps;
+    A.L.spb = spb;
+    A.L.lds = lds;
+    A.L.block = dim3(A.tps * spb);
+    A.L.grid = dim3((unsigned)grid);
     return KG_OK;
   };
-  int tps = cfg->threads_per_system;
-  if (tps == 0) {
-    r = autotune(c, tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf), [&](int t, float* ms) -> int32_t {
-      Plan PP;
-      Lay lay;
-      Launch L;
-      int32_t rr = setup(t, PP, lay, L);
+  auto run = [&](const AcLaunch& A) -> hipError_t {
+    BatchDev B = batch_dev(c, A.P);
+    if (A.persistent)
+      return run_ac_persistent(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop,
+                               d_xoff, d_out, d_off, d_st, d_err, refine, A.lay, A.plan_ints, A.chunk);
+    return run_ac(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out,
+                  d_off, d_st, d_err, refine, A.lay);
+  };
+  int cand = cfg->threads_per_system;
+  if (cand == 0) {
+    std::vector<int> cands{1, 4, 16, 64, 256};
+    if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064});
+    r = autotune(c, tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf), cands, [&](int t, float* ms) -> int32_t {
+      AcLaunch A;
+      int32_t rr = setup(t, A);
       if (rr) return rr;
-      BatchDev B = batch_dev(c, PP);
-      return time_launch(c, [&] {
-        return run_ac(t, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out,
-                      d_off, d_st, d_err, refine, lay);
-      }, ms);
-    }, &tps);
+      if (A.tps * A.L.spb < 64 && A.tps < 64) return set_err(c, KG_UNEXPECTED, "mapping leaves most of a wave idle");
+      return time_launch(c, [&] { return run(A); }, ms);
+    }, &cand);
     if (r) return r;
   }
-  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  Lay lay;
-  Launch L;
-  r = setup(tps, P, lay, L);
-  if (r && cfg->threads_per_system == 0) return r;
-  if (r) {  // an explicitly requested mapping may leave part of a wave idle
-    c->err.clear();
-    if ((r = prepare(c, AK_AC, order_for(tps), s_ac, P))) return r;
-    lay = Lay{P.nnz_max, P.n_max, P.ne_max, 0, 0, P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
-    if ((r = configure(c, tps, lay, (int64_t)c->n_inst * nf, L))) return r;
-  }
+  AcLaunch A;
+  if ((r = setup(cand, A))) return r;
+  P = A.P;
   HIPCHK(c, hipMemsetAsync(d_err, 0, sizeof(double) * c->n_inst, c->stream));
-  BatchDev B = batch_dev(c, P);
-  r = timed(c, mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>", [&] {
-    return run_ac(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out, d_off,
-                  d_st, d_err, refine, lay);
-  });
+  r = timed(c, A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
+                            : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
+            [&] { return run(A); });
   if (r) return r;
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;
@@ -823,13 +851,13 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   };
   int tps = cfg->threads_per_system;
   if (tps == 0) {
-    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), [&](int t, float* ms) -> int32_t {
+    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), {1, 4, 16, 64, 256}, [&](int t, float* ms) -> int32_t {
       Plan PP;
       Lay lay;
       Launch L;
       int32_t rr = setup(t, PP, lay, L);
       if (rr) return rr;
-      if (c->shared && t * L.spb < 64 && t < 64) return set_err(c, KG_UNEXPECTED, "sub-wave mapping");
+      if (t * L.spb < 64 && t < 64) return set_err(c, KG_UNEXPECTED, "sub-wave mapping");
       BatchDev B = batch_dev(c, PP);
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
       return time_launch(c, [&] {

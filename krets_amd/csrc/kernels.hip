@@ -1077,6 +1077,105 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
 }
 
 
+
+// Re-points every array of a plan into a copy of its buffer at `lds`.
+__device__ PatDev rebase(const PatDev& G, const int32_t* lds) {
+  PatDev P = G;
+  auto rb = [&](const int32_t* p) { return p ? lds + (p - G.base) : nullptr; };
+  P.row_ptr = rb(G.row_ptr); P.col = rb(G.col); P.diag = rb(G.diag);
+  P.fprog_ptr = rb(G.fprog_ptr); P.fprog = rb(G.fprog);
+  P.lvl_ptr = rb(G.lvl_ptr); P.lvl_rows = rb(G.lvl_rows); P.blvl_ptr = rb(G.blvl_ptr); P.blvl_rows = rb(G.blvl_rows);
+  P.scon_ptr = rb(G.scon_ptr); P.scon = rb(G.scon); P.rcon_ptr = rb(G.rcon_ptr); P.rcon = rb(G.rcon);
+  P.col_of = rb(G.col_of); P.cpos = rb(G.cpos); P.isbr = rb(G.isbr);
+  P.etype = rb(G.etype); P.ep = rb(G.ep); P.em = rb(G.em); P.ec = rb(G.ec);
+  P.ut_ptr = rb(G.ut_ptr); P.ut = rb(G.ut); P.lt_ptr = rb(G.lt_ptr); P.lt = rb(G.lt);
+  P.tlvl_ptr = rb(G.tlvl_ptr); P.tlvl_rows = rb(G.tlvl_rows); P.tblvl_ptr = rb(G.tblvl_ptr);
+  P.tblvl_rows = rb(G.tblvl_rows);
+  P.row_of = rb(G.row_of); P.rpos = rb(G.rpos); P.erp = rb(G.erp); P.erm = rb(G.erm); P.erc = rb(G.erc);
+  P.base = lds;
+  return P;
+}
+
+// Persistent AC sweep for batches with one shared topology: a grid of a few
+// blocks per CU, each walking a contiguous chunk of work items. A work item is
+// SPB consecutive frequencies of one instance. The plan is copied into LDS once
+// per block and the instance's element records whenever the instance changes,
+// so the gather and the factor program read LDS instead of issuing dependent
+// global loads.
+// LDS: [plan (plan_ints, 16-aligned)] [element records ne*KG_NPARAM + ebase ne]
+//      [SPB systems] [reduction scratch].
+template <class F, class A, int TPS>
+__global__ void k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
+                       const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status, double* err,
+                       int refine, Lay lay, int plan_ints, int64_t chunk) {
+  extern __shared__ __align__(16) char smem[];
+  const int spb = blockDim.x / TPS;
+  const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
+  const PatDev& G = B.pats[B.inst_pat[kind_slot * B.n_instances]];
+  int32_t* pl = (int32_t*)smem;
+  size_t off = lay_al(sizeof(int32_t) * (size_t)plan_ints);
+  double* iprm = (double*)(smem + off);
+  off += lay_al(sizeof(double) * (size_t)G.ne * KG_NPARAM);
+  double* ieb = (double*)(smem + off);
+  off += lay_al(sizeof(double) * (size_t)G.ne);
+  size_t sb = sys_bytes(lay);
+  char* sysbase = smem + off;
+  double* red = (double*)(sysbase + spb * sb);
+  SysMem<F, A> M = carve<F, A>(sysbase + ls * sb, lay);
+  for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
+  __syncthreads();
+  const PatDev P = rebase(G, pl);
+  const int64_t gpi = (nf + spb - 1) / spb;  // work items per instance
+  const int64_t nitems = gpi * B.n_instances;
+  const int64_t i0 = xcd_block() * chunk, i1 = min(i0 + chunk, nitems);
+  int cur = -1;
+  for (int64_t item = i0; item < i1; item++) {
+    const int inst = (int)(item / gpi);
+    const int fi = (int)((item % gpi) * spb + ls);
+    const bool valid = fi < nf;
+    if (inst != cur) {  // stage the instance's element records
+      __syncthreads();
+      const double* gp = B.params + B.eoff[inst] * KG_NPARAM;
+      const double* ge = B.ebase + B.eoff[inst];
+      for (int i = threadIdx.x; i < G.ne * KG_NPARAM; i += blockDim.x) iprm[i] = gp[i];
+      for (int i = threadIdx.x; i < G.ne; i += blockDim.x) ieb[i] = ge[i];
+      cur = inst;
+      __syncthreads();
+    }
+    if (t == 0) M.flags[0] = KG_OK;
+    if (valid && P.has_diode)
+      for (int e = t; e < P.ne; e += TPS)
+        if (P.etype[e] == KG_D) {
+          const double* x0 = xop + xop_off[inst];
+          int p = P.ep[e], m = P.em[e];
+          double vp = p >= 0 ? x0[P.col_of[p]] : 0.5, vm = m >= 0 ? x0[P.col_of[m]] : 0.0;
+          DiodeLin d = diode_eval(vp - vm, iprm[e * KG_NPARAM], iprm[e * KG_NPARAM + 1], true, 0.0);
+          M.ed[e * ESTRIDE] = d.g;
+        }
+    __syncthreads();
+    StampArgs sa{AK_AC, valid ? freqs[fi] : 1.0, 1.0, 1.0, 0.0, -1, 0.0};
+    stamp_matrix<F, A, TPS>(P, M.v, iprm, ieb, sa, M.ed, t, valid);
+    stamp_rhs<A, TPS>(P, M.b, iprm, sa, M.ed, nullptr, t, valid);
+    __syncthreads();
+    factor<F, TPS>(P, M.v, t, valid, &M.flags[0]);
+    double est =
+        solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, iprm, ieb, sa, M.ed, refine, red, ls, t, valid, M.r);
+    __syncthreads();
+    if (valid) {
+      double* o = out + out_off[inst] + (int64_t)fi * P.n * 2;
+      for (int j = t; j < P.n; j += TPS) {
+        int c = P.col_of[j];
+        o[2 * c] = re_of(M.x[j]);
+        o[2 * c + 1] = im_of(M.x[j]);
+      }
+      if (t == 0) {
+        if (M.flags[0] != KG_OK) atomicMax(&status[inst], M.flags[0]);
+        if (err && est > 0.0) atomic_max_nonneg(&err[inst], est);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------- adjoint
 template <int TPS>
 __device__ double sys_sum(double v, double* red, int ls, int t) {
@@ -1505,6 +1604,18 @@ hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, Batch
                       int64_t grad_stride, int32_t* status, Lay lay) {
   KG_TPS_SWITCH(tps, launch(k_adj_op<T>, g, b, lds, st, B, kind_slot, xbar0, xop, xop_off, grad, grad_stride, status,
                             lay))
+}
+
+hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B,
+                             int kind_slot, const double* freqs, int nf, const double* xop, const int64_t* xop_off,
+                             double* out, const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
+                             int plan_ints, int64_t chunk) {
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_ac_p<c64, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out,
+                              out_off, status, err, refine, lay, plan_ints, chunk))
+  }
+  KG_TPS_SWITCH(tps, launch(k_ac_p<c128, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out,
+                            out_off, status, err, refine, lay, plan_ints, chunk))
 }
 
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

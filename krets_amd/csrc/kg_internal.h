@@ -65,6 +65,9 @@ struct PatDev {
   // equation (row) positions: rpos[original row] and, per element, the
   // permuted rows of its plus / minus KCL equations and branch equation
   const int32_t *row_of, *rpos, *erp, *erm, *erc;
+  // the plan buffer itself (all pointers above point into it)
+  const int32_t* base;
+  int32_t total;
 };
 
 }  // namespace kg

@@ -88,6 +88,10 @@ hipError_t run_adj_bwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream
 hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                       const double* xbar0, const double* xop, const int64_t* xop_off, double* grad,
                       int64_t grad_stride, int32_t* status, Lay lay);
+hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B,
+                             int kind_slot, const double* freqs, int nf, const double* xop, const int64_t* xop_off,
+                             double* out, const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
+                             int plan_ints, int64_t chunk);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay);
