@@ -38,6 +38,14 @@ def _nerr(x, r):
     return float(np.linalg.norm(np.asarray(x) - r) / (d if d > 0 else 1.0))
 
 
+def _werr(xs, xr):
+    """Worst normwise error over the points of one trajectory, with where it occurs."""
+    errs = [_nerr(xs[j], xr[j]) for j in range(len(xr))]
+    j = int(np.argmax(errs))
+    return errs[j], {"point": j, "err": errs[j], "norm_ref": float(np.linalg.norm(xr[j])),
+                     "max_abs_diff": float(np.max(np.abs(np.asarray(xs[j]) - xr[j])))}
+
+
 def _timed(ctx, fn, reps):
     torch.cuda.synchronize()
     ctx.profile(True)
@@ -169,14 +177,19 @@ def c4(args):
     if R:
         ok = 0
         e = 0.0
+        where = []
         for i in (0, 511, 1023):
             tr, xr, ntr, rejr = R.Circuit(b.netlists[i]).tran_lte(*a, cfg=R.config(vabstol=1e-9, iabstol=1e-12))
             same = npts[i] == len(tr) and nt[i] == ntr and rej[i] == rejr
             ok += int(same)
             if same:
-                e = max(e, max(_nerr(xs[i][j], xr[j]) for j in range(len(tr))))
+                ei, wi = _werr(xs[i], xr)
+                wi["instance"] = i
+                where.append(wi)
+                e = max(e, ei)
         res["oracle_same_step_sequence"] = f"{ok}/3"
         res["normwise_err_vs_oracle"] = e
+        res["worst_points"] = where
     # fixed-step BE-compat parity run at h = 1 us
     ccfg = kg.SolverConfig(voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12)
     dt2, kms2, kname2, (xs2, st2, (it2, _)) = _timed(ctx, lambda: ctx.transient(1e-6, 5000, method=kg.BE_COMPAT,
@@ -185,7 +198,9 @@ def c4(args):
                              "solves_per_s": float(it2.sum()) / dt2, "all_converged": bool((st2 == 0).all())}
     if R:
         xr, itr = R.Circuit(b.netlists[3]).tran(1e-6, 5e-3, cfg=R.config(vabstol=1e-9, iabstol=1e-12))
-        res["be_compat_h1us"]["err_vs_oracle"] = max(_nerr(xs2[3][k], xr[k]) for k in range(xr.shape[0]))
+        e2, w2 = _werr(xs2[3], xr)
+        res["be_compat_h1us"]["err_vs_oracle"] = e2
+        res["be_compat_h1us"]["worst_point"] = w2
         res["be_compat_h1us"]["same_newton_counts"] = bool(np.array_equal(it2[3], itr))
     return res
 
