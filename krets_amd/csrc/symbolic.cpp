@@ -305,7 +305,8 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   // Static pivot sequence from one representative numeric factorisation
   // (the approach of KLU's refactorisation): columns in the fill-reducing
   // order, rows chosen by threshold partial pivoting that keeps the matched
-  // row when it is within 0.1 of the column maximum. This removes the exact
+  // row when it is within 1e-3 of the column maximum (the
+  // threshold of KLU and UMFPACK). This removes the exact
   // cancellations of MNA incidence entries (e.g. an inductor short in DC)
   // that a purely structural matching cannot see.
   std::vector<int> prow(n, -1);
@@ -356,7 +357,7 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
       if (best > 0.0) {
         auto it = used[pref] ? rowv[pref].end() : rowv[pref].find(k);
         double ap = it == rowv[pref].end() ? 0.0 : std::abs(it->second);
-        p = ap >= 0.1 * best ? pref : arg;
+        p = ap >= 1e-3 * best ? pref : arg;
       } else {  // numerically singular representative: keep the structural match
         p = !used[pref] ? pref : -1;
         for (int r : colr[k])
