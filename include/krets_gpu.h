@@ -91,7 +91,12 @@ typedef struct {
   double vabstol;
   int32_t maxiter;
   int32_t compat;
-  int32_t threads_per_system; /* 0 = autotuned */
+  /* Work mapping; 0 = autotuned. 1, 4, 16, 64, 256: threads per system of the
+   * level-scheduled kernels. AC sweeps of a shared topology also take
+   * 1000 + t (t = 1, 4, 16, 64: the plan and each instance's element records
+   * staged in LDS by a persistent grid) and 2001 (one lane per system, code
+   * specialised to the topology at run time). */
+  int32_t threads_per_system;
   int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
 } kg_solver_config;
 
@@ -195,6 +200,14 @@ int32_t kg_last_autotune(const kg_ctx* ctx, int32_t* tps, float* ms, int32_t cap
 int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, int32_t kind,
                        int32_t ordering, const double* params, double s_rep, int32_t* buf, int32_t cap,
                        int32_t* layout, int32_t layout_cap);
+
+/* Diagnostic: the HIP source of the run-time specialised lane-per-system AC
+ * kernel for one topology (minimum-degree plan, static pivots from params at
+ * reactive scale s_rep). No counterpart in krets. Returns the source length
+ * (copying at most cap bytes, NUL-terminated, to buf) or -status, with the
+ * reason the topology is not specialised in buf. */
+int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
+                         double s_rep, int32_t precision, int32_t refine_iters, char* buf, int32_t cap);
 
 /* Netlist reader for the R/L/C/V/I/D subset of krets' grammar
  * (crates/krets-parser/src/parser.rs:27-154). */

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkrets_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["kernels.hip", "api.cpp", "symbolic.cpp", "netlist.cpp"]
+SOURCES = ["kernels.hip", "api.cpp", "symbolic.cpp", "netlist.cpp", "jit.cpp"]
 # Packed FP32 instructions are disabled for all device code.
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -39,7 +39,7 @@ def build(verbose=False):
         if verbose and lines:
             print("\n".join(lines))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [HIPCC] + DEVICE_FLAGS + ["-shared", "-fPIC", "-o", tmp] + objs + ["-lhiprtc"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)

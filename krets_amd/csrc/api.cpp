@@ -18,6 +18,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "kg_jit.h"
 #include "kg_launch.h"
 #include "krets_gpu.h"
 
@@ -32,6 +33,7 @@ namespace {
 
 struct PatEntry {
   PatLayout lay;
+  std::vector<int32_t> host;  // the plan buffer (host copy)
   int32_t* d_buf = nullptr;
   PatDev dev{};
   std::string err;
@@ -216,6 +218,7 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
         HIPCHK(c, hipMalloc(&pe.d_buf, sizeof(int32_t) * std::max<size_t>(buf.size(), 1)));
         HIPCHK(c, hipMemcpy(pe.d_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice));
         pe.dev = to_dev(pe.lay, pe.d_buf);
+        pe.host = std::move(buf);
       } else {
         PatLayout z{};
         pe.dev = to_dev(z, nullptr);
@@ -729,17 +732,36 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   // One AC launch for a mapping candidate: TPS threads per system, plus 1000
   // for the persistent variant (shared topology only) that stages the plan
   // and each instance's element records in LDS.
+  // 2001: the lane-per-system kernel specialised to this topology at run
+  // time (shared topology, small systems).
   struct AcLaunch {
     Plan P;
     Lay lay;
     Launch L;
-    bool persistent = false;
+    bool persistent = false, lane = false;
     int tps = 0, plan_ints = 0;
     int64_t chunk = 0;
+    JitFn fn;
   };
+  const int64_t nsys = (int64_t)c->n_inst * nf;
   auto setup = [&](int cand, AcLaunch& A) -> int32_t {
-    A.persistent = cand >= 1000;
+    A.lane = cand == 2001;
+    A.persistent = cand >= 1000 && !A.lane;
     A.tps = cand % 1000;
+    if (A.lane) {
+      if (!c->shared) return set_err(c, KG_UNEXPECTED, "lane kernel needs a shared topology");
+      int32_t rr = prepare(c, AK_AC, ORD_MINDEG, s_ac, A.P);
+      if (rr) return rr;
+      const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(0, (int)AK_AC, (int)ORD_MINDEG, s_ac)]];
+      std::string src, why, log;
+      if (!jit_ac_source(pe.lay, pe.host.data(), mixed, refine, src, why))
+        return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
+      if (jit_load(c->device, src, "kg_ac_lane", A.fn, log) != hipSuccess)
+        return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
+      A.L.grid = dim3((unsigned)((nsys + 63) / 64));
+      A.L.block = dim3(64);
+      return KG_OK;
+    }
     if (!valid_tps(A.tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
     int32_t rr = prepare(c, AK_AC, order_for(A.tps), s_ac, A.P);
     if (rr) return rr;
@@ -767,6 +789,18 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
     return KG_OK;
   };
   auto run = [&](const AcLaunch& A) -> hipError_t {
+    if (A.lane) {
+      const double* prm = c->d_params;
+      const double* eb = c->d_ebase;
+      const double* fr = d_f;
+      int nfa = nf;
+      int64_t ns = nsys;
+      double* outp = d_out;
+      int32_t* stp = d_st;
+      double* errp = mixed ? d_err : nullptr;
+      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp};
+      return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr);
+    }
     BatchDev B = batch_dev(c, A.P);
     if (A.persistent)
       return run_ac_persistent(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop,
@@ -777,12 +811,13 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   int cand = cfg->threads_per_system;
   if (cand == 0) {
     std::vector<int> cands{1, 4, 16, 64, 256};
-    if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064});
+    if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064, 2001});
     r = autotune(c, tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf), cands, [&](int t, float* ms) -> int32_t {
       AcLaunch A;
       int32_t rr = setup(t, A);
       if (rr) return rr;
-      if (A.tps * A.L.spb < 64 && A.tps < 64) return set_err(c, KG_UNEXPECTED, "mapping leaves most of a wave idle");
+      if (!A.lane && A.tps * A.L.spb < 64 && A.tps < 64)
+        return set_err(c, KG_UNEXPECTED, "mapping leaves most of a wave idle");
       return time_launch(c, [&] { return run(A); }, ms);
     }, &cand);
     if (r) return r;
@@ -791,7 +826,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if ((r = setup(cand, A))) return r;
   P = A.P;
   HIPCHK(c, hipMemsetAsync(d_err, 0, sizeof(double) * c->n_inst, c->stream));
-  r = timed(c, A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
+  r = timed(c, A.lane ? (mixed ? "kg_ac_lane<complex64,complex128>" : "kg_ac_lane<complex128>")
+             : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
                             : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
             [&] { return run(A); });
   if (r) return r;
@@ -929,6 +965,21 @@ int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind
   if (L.status != KG_OK) return -L.status;
   if (buf) std::memcpy(buf, b.data(), sizeof(int32_t) * std::min<size_t>(cap, b.size()));
   return (int32_t)b.size();
+}
+
+int32_t kg_jit_ac_source(const kg_element* el, int32_t ne, int32_t n, const double* params, double s_rep,
+                         int32_t precision, int32_t refine_iters, char* buf, int32_t cap) {
+  if (!el || ne <= 0 || n <= 0) return -KG_INVALID_ARGUMENT;
+  PatLayout L;
+  std::string err, src, why;
+  std::vector<int32_t> b = build_pattern(el, ne, n, AK_AC, ORD_MINDEG, params, s_rep, L, err);
+  if (L.status != KG_OK) return -L.status;
+  if (!jit_ac_source(L, b.data(), precision == KG_MIXED, refine_iters > 0 ? refine_iters : 2, src, why)) {
+    if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
+    return -KG_UNEXPECTED;
+  }
+  if (buf && cap > 0) std::snprintf(buf, cap, "%s", src.c_str());
+  return (int32_t)src.size();
 }
 
 int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, double h0, double hmin, double hmax,

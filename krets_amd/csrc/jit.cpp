@@ -1,0 +1,403 @@
+// Run-time specialisation of the AC sweep to one circuit topology.
+//
+// The host symbolic pass fixes, for a shared topology, every LU slot, the
+// static pivot order and which element contributions land in each slot.
+// Instead of interpreting that plan on the device (the level-scheduled
+// kernels read it from memory), this file writes it out as straight-line
+// HIP: one lane solves one (instance, frequency) system with the matrix, the
+// factors and the solution in registers.
+//
+// Factorisation (row i in pivot order, up-looking as in the plan's factor
+// program) keeps the upper factor scaled by the reciprocal pivot, U' =
+// diag(U)^-1 U, and the forward-substitution vector scaled the same way, so
+// that neither the lower factor nor the pivots outlive their row:
+//   a_ij -= a_ik * U'_kj              (k < i in the row's pattern, j > k)
+//   y_i   = b_i - sum_k a_ik * y'_k
+//   U'_ij = a_ij / a_ii,  y'_i = y_i / a_ii
+//   x_i   = y'_i - sum_j U'_ij x_j     (back substitution, i descending)
+// This is the same LU (the same pivots, the same fill) as the plan's factor
+// program, with the multiplier division moved onto the upper factor.
+//
+// Mixed precision repeats the factorisation in complex64 on every pass (it
+// costs the same VALU work as re-reading stored factors would cost in
+// registers this kernel does not have), with FP64 residuals b - A x from the
+// FP64 stamps and FP64 accumulation of the corrections, as the generic
+// kernels do.
+#include "kg_jit.h"
+
+#include <hip/hiprtc.h>
+
+#include <functional>
+#include <map>
+#include <mutex>
+#include <set>
+#include <sstream>
+#include <utility>
+#include <vector>
+
+namespace kg {
+namespace {
+
+struct View {
+  int n = 0, nnz = 0, ne = 0;
+  const int32_t *row_ptr, *col, *diag, *fprog_ptr, *fprog, *scon_ptr, *scon, *rcon_ptr, *rcon, *col_of, *etype;
+};
+
+View view(const PatLayout& L, const int32_t* b) {
+  View v;
+  v.n = L.n;
+  v.nnz = L.nnz;
+  v.ne = L.ne;
+  v.row_ptr = b + L.o_row_ptr;
+  v.col = b + L.o_col;
+  v.diag = b + L.o_diag;
+  v.fprog_ptr = b + L.o_fprog_ptr;
+  v.fprog = b + L.o_fprog;
+  v.scon_ptr = b + L.o_scon_ptr;
+  v.scon = b + L.o_scon;
+  v.rcon_ptr = b + L.o_rcon_ptr;
+  v.rcon = b + L.o_rcon;
+  v.col_of = b + L.o_col_of;
+  v.etype = b + L.o_etype;
+  return v;
+}
+
+const char* kPrelude = R"(typedef long long i64;
+struct __attribute__((aligned(16))) Z { double re, im; };
+struct __attribute__((aligned(8))) W { float re, im; };
+__device__ __forceinline__ Z operator+(Z a, Z b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Z operator-(Z a, Z b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Z operator*(Z a, Z b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ W operator-(W a, W b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ W operator*(W a, W b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ Z cz(Z a) { return a; }
+__device__ __forceinline__ Z cz(W a) { return {(double)a.re, (double)a.im}; }
+__device__ __forceinline__ W cw(Z a) { return {(float)a.re, (float)a.im}; }
+__device__ __forceinline__ Z scal(double s, Z a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ double m2(Z a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ bool fin(Z a) { return __builtin_isfinite(a.re) && __builtin_isfinite(a.im); }
+// reciprocal of a pivot, conj(b) / |b|^2, without branches: an FP64 pivot is
+// first scaled by the power of two of its larger component (exact), so
+// |b|^2 cannot leave the FP64 range; an FP32 pivot's |b|^2 always fits FP64
+__device__ __forceinline__ Z rcpF(Z b) {
+  const int k = __builtin_amdgcn_frexp_exp(__builtin_fmax(__builtin_fabs(b.re), __builtin_fabs(b.im)));
+  const double re = __builtin_amdgcn_ldexp(b.re, -k), im = __builtin_amdgcn_ldexp(b.im, -k);
+  const double q = 1.0 / (re * re + im * im);
+  return {__builtin_amdgcn_ldexp(re * q, -k), __builtin_amdgcn_ldexp(-im * q, -k)};
+}
+__device__ __forceinline__ W rcpF(W b) {
+  const double re = b.re, im = b.im, q = 1.0 / (re * re + im * im);
+  return {(float)(re * q), (float)(-im * q)};
+}
+// block order that keeps consecutive systems (one instance) on one XCD
+__device__ __forceinline__ i64 xcd_block() {
+  const i64 nb = gridDim.x, b = blockIdx.x;
+  const i64 q = nb / 8, r = nb % 8, x = b % 8, k = b / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+)";
+
+// Emits straight-line code. Element values are loaded once per pass (a pass
+// is a brace scope), LOOKAHEAD rows before the row that first uses them, and
+// an empty asm statement with a memory clobber after each row's loads keeps
+// the compiler from hoisting every load of the kernel to its start (which
+// would hold all element values in registers at once).
+constexpr int LOOKAHEAD = 8;
+
+struct Gen {
+  const View& P;
+  std::ostringstream out;         // kernel text
+  std::ostringstream* cur = &out; // where statements go
+  std::vector<std::string> loads; // loads first needed by the row being generated
+  std::set<int> have_g, have_p;
+  std::string why;
+  bool track_bm = false;          // mixed precision: |b|_inf^2 for the correction scale
+  std::function<void(std::ostringstream&, int)> on_result;
+  explicit Gen(const View& v) : P(v) {}
+
+  std::string g(int e) {  // 1/R (or the raw value for other types), per instance
+    std::string nm = "g" + std::to_string(e);
+    if (have_g.insert(e).second) loads.push_back("    const double " + nm + " = EB[" + std::to_string(e) + "];\n");
+    return nm;
+  }
+  std::string p(int e, int k) {  // params[e][k]
+    std::string nm = "p" + std::to_string(e) + "_" + std::to_string(k);
+    if (have_p.insert(e * KG_NPARAM + k).second)
+      loads.push_back("    const double " + nm + " = PR[" + std::to_string(e * KG_NPARAM + k) + "];\n");
+    return nm;
+  }
+  // FP64 value of LU slot s as (re, im) expressions; both empty when the slot
+  // receives no contribution. Follows the stamp kinds of the plan's gather
+  // lists (AC stamps, stampable.rs): +-y (1/R, j w C), +-1, branch diagonal.
+  bool slot(int s, std::string& re, std::string& im) {
+    std::vector<std::string> R, I;
+    for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) {
+      int code = P.scon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
+      switch (k) {
+        case CK_YP:
+        case CK_YN: {
+          const char* sg = k == CK_YN ? "-" : "";
+          if (ty == KG_R) R.push_back(sg + g(e));
+          else if (ty == KG_C) I.push_back(std::string(sg) + "(w * " + p(e, 0) + ")");
+          else if (ty == KG_D) { why = "diode small-signal stamps are not generated"; return false; }
+          break;
+        }
+        case CK_ONEP: R.push_back("1.0"); break;
+        case CK_ONEN: R.push_back("-1.0"); break;
+        case CK_BR:
+          if (ty == KG_L) I.push_back("-(w * " + p(e, 0) + ")");
+          else if (ty == KG_R) R.push_back("-" + p(e, 0));
+          else R.push_back("1.0");
+          break;
+        default: why = "unknown stamp kind"; return false;
+      }
+    }
+    auto join = [](const std::vector<std::string>& v) {
+      std::string s;
+      for (size_t i = 0; i < v.size(); i++) s += (i ? " + " : "") + v[i];
+      return s;
+    };
+    re = join(R);
+    im = join(I);
+    return true;
+  }
+  bool has_slot(int s) const { return P.scon_ptr[s + 1] > P.scon_ptr[s]; }
+  // FP64 right-hand side of row i (AC excitation, ac.rs: V sources carry their
+  // AC magnitude, current sources their DC value); empty when zero.
+  bool rhs(int i, std::string& re) {
+    std::vector<std::string> R;
+    for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) {
+      int code = P.rcon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
+      if (k != RK_SRC) { why = "only source excitations are generated"; return false; }
+      R.push_back(ty == KG_V ? p(e, 1) : p(e, 0));
+    }
+    re.clear();
+    for (size_t t = 0; t < R.size(); t++) re += (t ? " + " : "") + R[t];
+    return true;
+  }
+  static std::string zexpr(const std::string& re, const std::string& im) {
+    return "Z{" + (re.empty() ? std::string("0.0") : re) + ", " + (im.empty() ? std::string("0.0") : im) + "}";
+  }
+  static std::string conv(const std::string& F, const std::string& zval) {
+    return F == "Z" ? zval : "cw(" + zval + ")";
+  }
+
+  // One factorisation + forward + backward pass in type F ("Z" or "W"),
+  // appended to `out`. mode 0: right-hand side b; mode 1: scaled FP64
+  // residual b - A x (x in x<i>). Row i's back-substitution result t<i> is
+  // handed to on_result.
+  bool pass(const std::string& F, int mode) {
+    const int n = P.n;
+    have_g.clear();
+    have_p.clear();
+    std::vector<std::string> code(n);
+    std::vector<std::vector<std::string>> rload(n);
+    std::vector<char> yz(n, 1);  // y'_i structurally zero
+    for (int i = 0; i < n; i++) {
+      std::ostringstream o;
+      loads.clear();
+      // gather row i
+      for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
+        std::string re, im;
+        if (!slot(s, re, im)) return false;
+        if (mode == 1) {
+          if (has_slot(s)) o << "    const Z a" << s << " = " << zexpr(re, im) << ";\n";
+          o << "    " << F << " v" << s << " = "
+            << (has_slot(s) ? conv(F, "a" + std::to_string(s)) : F + "{0, 0}") << ";\n";
+        } else {
+          o << "    " << F << " v" << s << " = " << conv(F, zexpr(re, im)) << ";\n";
+        }
+      }
+      // right-hand side of row i
+      bool ynz = false;
+      std::string b;
+      if (!rhs(i, b)) return false;
+      if (mode == 0) {
+        if (!b.empty()) {
+          o << "    " << F << " y" << i << " = " << conv(F, "Z{" + b + ", 0.0}") << ";\n";
+          if (track_bm) o << "    bm2 = fmax(bm2, m2(Z{" << b << ", 0.0}));\n";
+          ynz = true;
+        }
+      } else {
+        o << "    Z r" << i << " = " << (b.empty() ? std::string("Z{0.0, 0.0}") : "Z{" + b + ", 0.0}") << ";\n";
+        for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++)
+          if (has_slot(s)) o << "    r" << i << " = r" << i << " - a" << s << " * x" << P.col[s] << ";\n";
+        o << "    " << F << " y" << i << " = " << conv(F, "scal(inv_s, r" + std::to_string(i) + ")") << ";\n";
+        ynz = true;
+      }
+      // factor row i: the plan's factor program for this row
+      const int32_t* fp = P.fprog + P.fprog_ptr[i];
+      int nk = *fp++;
+      for (int kk = 0; kk < nk; kk++) {
+        int sik = fp[0], k = P.col[sik], nu = fp[2];
+        fp += 3;
+        for (int u = 0; u < nu; u++, fp += 2) {
+          int dst = fp[0], src = fp[1];
+          if (!(src > P.diag[k] && src < P.row_ptr[k + 1])) { why = "factor program reads outside U"; return false; }
+          o << "    v" << dst << " = v" << dst << " - v" << sik << " * u" << src << ";\n";
+        }
+        if (!yz[k]) {
+          if (!ynz) o << "    " << F << " y" << i << " = " << F << "{0, 0} - v" << sik << " * y" << k << "s;\n";
+          else o << "    y" << i << " = y" << i << " - v" << sik << " * y" << k << "s;\n";
+          ynz = true;
+        }
+      }
+      // pivot: decomposition check as the generic kernels (|piv| > 0, finite)
+      int d = P.diag[i];
+      o << "    { const Z pz = cz(v" << d << "); bad |= !(m2(pz) > 0.0) || !fin(pz); }\n";
+      o << "    const " << F << " rd" << i << " = rcpF(v" << d << ");\n";
+      for (int s = d + 1; s < P.row_ptr[i + 1]; s++)
+        o << "    const " << F << " u" << s << " = v" << s << " * rd" << i << ";\n";
+      if (ynz) {
+        o << "    const " << F << " y" << i << "s = y" << i << " * rd" << i << ";\n";
+        yz[i] = 0;
+      }
+      code[i] = o.str();
+      rload[i] = loads;
+    }
+    // rows with their loads issued LOOKAHEAD rows early
+    std::ostringstream& O = out;
+    auto issue = [&](int r) {
+      if (r >= n || rload[r].empty()) return;
+      for (auto& l : rload[r]) O << l;
+      O << "    asm volatile(\"\" ::: \"memory\");\n";
+    };
+    for (int r = 0; r < LOOKAHEAD; r++) issue(r);
+    for (int i = 0; i < n; i++) {
+      issue(i + LOOKAHEAD);
+      O << code[i];
+      O << "    __builtin_amdgcn_sched_barrier(0);\n";
+    }
+    // back substitution
+    for (int i = n - 1; i >= 0; i--) {
+      std::string acc = yz[i] ? F + "{0, 0}" : "y" + std::to_string(i) + "s";
+      O << "    " << F << " t" << i << " = " << acc << ";\n";
+      for (int s = P.diag[i] + 1; s < P.row_ptr[i + 1]; s++)
+        O << "    t" << i << " = t" << i << " - u" << s << " * t" << P.col[s] << ";\n";
+      on_result(O, i);
+      if (i % 4 == 0) O << "    __builtin_amdgcn_sched_barrier(0);\n";
+    }
+    return true;
+  }
+};
+
+}  // namespace
+
+bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, int refine, std::string& src,
+                   std::string& why) {
+  View P = view(L, buf);
+  const int n = P.n;
+  if (L.status != KG_OK) { why = "structurally singular pattern"; return false; }
+  if (L.has_diode) { why = "diode small-signal stamps are not generated"; return false; }
+  if (refine < 0 || refine > 4) { why = "refinement count outside 0..4"; return false; }
+  int nu = 0;
+  for (int i = 0; i < n; i++) nu += P.row_ptr[i + 1] - P.diag[i] - 1;
+  // registers per lane held across a pass: U' and y' (and the FP64 solution
+  // for mixed precision); the register file holds 512 per lane at one wave
+  // per SIMD, and the pass needs a few dozen more for its temporaries
+  int regs = mixed ? 2 * (nu + n) + 4 * n : 4 * (nu + n);
+  if (regs > 440) { why = "per-lane state exceeds the register budget"; return false; }
+  Gen G(P);
+  std::ostringstream& o = G.out;
+  o << kPrelude;
+  o << "extern \"C\" __global__ void __launch_bounds__(64) kg_ac_lane(const double* __restrict__ prm,\n"
+       "    const double* __restrict__ ebase, const double* __restrict__ freqs, int nf, i64 nsys,\n"
+       "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err) {\n"
+       "  const i64 sys = xcd_block() * 64 + threadIdx.x;\n"
+       "  if (sys >= nsys) return;\n"
+       "  const int inst = (int)(sys / nf), fi = (int)(sys - (i64)inst * nf);\n"
+    << "  const double* EB = ebase + (i64)inst * " << P.ne << ";\n"
+    << "  const double* PR = prm + (i64)inst * " << (int64_t)P.ne * KG_NPARAM << ";\n"
+    << "  const double w = (2.0 * 3.14159265358979323846) * freqs[fi];\n"
+    << "  Z* o = reinterpret_cast<Z*>(out) + sys * " << n << ";\n"
+    << "  bool bad = false;\n";
+  if (!mixed) {
+    G.on_result = [&](std::ostringstream& O, int i) { O << "    o[" << P.col_of[i] << "] = t" << i << ";\n"; };
+    o << "  {\n";
+    if (!G.pass("Z", 0)) { why = G.why; return false; }
+    o << "  }\n";
+  } else {
+    for (int i = 0; i < n; i++) o << "  Z x" << i << ";\n";
+    o << "  double est = 0.0, dprev = 0.0, bm2 = 0.0;\n";
+    // pass 0: complex64 solve of b; x = its FP64 copy
+    G.on_result = [&](std::ostringstream& O, int i) { O << "    x" << i << " = cz(t" << i << ");\n"; };
+    G.track_bm = true;
+    o << "  {\n";
+    if (!G.pass("W", 0)) { why = G.why; return false; }
+    o << "  }\n";
+    G.track_bm = false;
+    // refinement: corrections scaled by |b|_inf stay in FP32 range
+    o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0;\n";
+    for (int it = 0; it < refine; it++) {
+      G.on_result = [&](std::ostringstream& O, int i) {
+        O << "    { const Z d = scal(s, cz(t" << i << ")); x" << i << " = x" << i << " + d; dm2 = fmax(dm2, m2(d));"
+          << " xm2 = fmax(xm2, m2(x" << i << ")); }\n";
+      };
+      // no branch: with s == 0 (b == 0, x == 0) the correction is zero
+      o << "  {\n    double dm2 = 0.0, xm2 = 0.0;\n";
+      if (!G.pass("W", 1)) { why = G.why; return false; }
+      o << "    const double dm = sqrt(dm2), xm = sqrt(xm2);\n"
+        << "    const double rho = (" << it << " > 0 && dprev > 0.0) ? fmin(dm / dprev, 1.0) : 1.0;\n"
+        << "    est = xm > 0.0 ? rho * dm / xm : 0.0;\n"
+        << "    dprev = dm;\n  }\n";
+    }
+    for (int i = 0; i < n; i++) o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";
+    o << "  if (err != nullptr && est > 0.0)\n"
+         "    atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(est));\n";
+  }
+  o << "  if (bad) atomicMax(&status[inst], " << (int)KG_DECOMPOSITION << ");\n}\n";
+  src = o.str();
+  return true;
+}
+
+namespace {
+std::mutex g_mu;
+std::map<std::pair<int, std::string>, std::pair<hipModule_t, JitFn>> g_cache;
+}  // namespace
+
+hipError_t jit_load(int device, const std::string& src, const char* name, JitFn& out, std::string& log) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_pair(device, src + "\n//" + name);
+  auto it = g_cache.find(key);
+  if (it != g_cache.end()) {
+    out = it->second.second;
+    return hipSuccess;
+  }
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return e;
+  std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "kg_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    log = "hiprtcCreateProgram failed";
+    return hipErrorInvalidValue;
+  }
+  // packed FP32 instructions stay disabled, as for the prebuilt kernels
+  const char* opts[] = {arch.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops"};
+  hiprtcResult rc = hiprtcCompileProgram(prog, 7, opts);
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  log.assign(ls, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &log[0]);
+  if (rc != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return hipErrorInvalidValue;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  std::vector<char> code(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  hipModule_t mod;
+  if ((e = hipModuleLoadData(&mod, code.data())) != hipSuccess) return e;
+  JitFn f;
+  if ((e = hipModuleGetFunction(&f.fn, mod, name)) != hipSuccess) return e;
+  hipFuncGetAttribute(&f.vgprs, HIP_FUNC_ATTRIBUTE_NUM_REGS, f.fn);
+  hipFuncGetAttribute(&f.scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f.fn);
+  g_cache[key] = {mod, f};
+  out = f;
+  return hipSuccess;
+}
+
+}  // namespace kg

@@ -1,0 +1,35 @@
+// Pattern-specialised kernels compiled at run time (hiprtc, gfx950).
+//
+// For a shared topology whose LU fits in one lane's registers, the host
+// symbolic plan is turned into straight-line HIP source: every LU slot, every
+// solution entry and every element-value load has a fixed register or a fixed
+// offset, so one lane solves one system with no plan metadata traffic, no
+// LDS and no barriers. The generated source is compiled once per
+// (device, source text) and cached for the life of the process.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "kg_internal.h"
+
+namespace kg {
+
+struct JitFn {
+  hipFunction_t fn = nullptr;
+  int vgprs = 0;         // registers per lane reported by the loader
+  int scratch = 0;       // private (spill) bytes per lane
+};
+
+// Source of the lane-per-system AC sweep kernel `kg_ac_lane` for one pattern
+// (AC kind, shared by every instance). Returns false, with the reason in
+// `why`, when the pattern is outside what the generator handles (diodes,
+// patterns whose per-lane state would not fit in registers).
+bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, int refine, std::string& src,
+                   std::string& why);
+
+// Compiles `src` for the device's architecture and loads kernel `name`;
+// cached per (device, source).
+hipError_t jit_load(int device, const std::string& src, const char* name, JitFn& out, std::string& log);
+
+}  // namespace kg

@@ -163,7 +163,8 @@ def test_transient_compat_vs_oracle(kg, oracle, path, h, stop):
         assert abs(res[42]["V(out)"] - 0.8647) < 1e-3 and abs(res[42]["time"] - 2.1e-3) < 1e-6
 
 
-@pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064])  # 1000 + t: persistent LDS-staged variant
+# 1000 + t: persistent LDS-staged variant; 2001: lane per system, run-time specialised
+@pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064, 2001])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
     from krets_amd import workloads as W
@@ -181,6 +182,47 @@ def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
         assert e < (1e-12 if precision == 0 else 1e-9), (i, e)
     if precision == 1:
         assert (err < 1e-8).all()  # a-posteriori estimate of the remaining refinement error
+
+
+@pytest.mark.parametrize("refine", [1, 2, 3])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lane_ac_golden_and_rlc(kg, oracle, precision, refine):
+    """Run-time specialised lane kernel: the krets AC golden vector and a shared
+    RLC ladder batch (inductor branch rows) vs the oracle."""
+    import json, os
+    from conftest import GOLDEN
+    from krets_amd import workloads as W
+    say(f"lane AC precision={precision} refine={refine}")
+    cfg = kg.SolverConfig(threads_per_system=2001, refine_iters=refine)
+    g = json.load(open(os.path.join(GOLDEN, "low_pass_ac_result.json")))
+    cols = {k: np.array([float(v) for v in vals]) for k, vals in g["columns"].items()}
+    c = kg.Circuit(read_circuit("low_pass_filter/ac/ac.cir"))
+    ctx = kg.Context()
+    ctx.load(c.elements, c.params[None], c.n)
+    f = np.array(cols["frequency"])
+    xs, st, _ = ctx.ac_sweep(f, precision=precision, config=cfg)
+    assert (st == 0).all()
+    tol = 1e-12 if precision == 0 else 1e-9
+    for name in ("I(V1)", "V(in)", "V(out)"):
+        v = xs[0][:, c.names.index(name)]
+        assert np.max(np.abs(np.abs(v) - cols[name + "_mag"]) / cols[name + "_mag"]) < tol
+    rng = np.random.default_rng(5)
+    nets = []
+    for _ in range(6):
+        lines = ["V1 n0 0 0 AC 1"]
+        for k in range(1, 9):
+            lines += [f"R{k} n{k-1} m{k} {rng.uniform(10, 100)!r}", f"L{k} m{k} n{k} {rng.uniform(1e-9, 1e-8)!r}",
+                      f"C{k} n{k} 0 {rng.uniform(1e-13, 1e-12)!r}"]
+        nets.append("\n".join(lines) + "\n")
+    cs = [kg.Circuit(t) for t in nets]
+    ctx.load(cs[0].elements, np.stack([x.params for x in cs]), cs[0].n)
+    f = np.geomspace(1e6, 1e10, 37)
+    xs, st, err = ctx.ac_sweep(f, precision=precision, config=cfg)
+    assert (st == 0).all()
+    for i in range(6):
+        xr = oracle.Circuit(nets[i]).ac(f)
+        e = max(nerr(xs[i][j], xr[j]) for j in range(len(f)))
+        assert e < (1e-12 if precision == 0 else (1e-9 if refine >= 2 else 1e-6)), (i, e)
 
 
 @pytest.mark.parametrize("tps", [1, 16, 64, 256])

@@ -1,0 +1,184 @@
+"""CPU checks of the run-time specialised lane-per-system AC kernel.
+
+The generated HIP source (kg_jit_ac_source) is compiled twice without a GPU:
+  * for gfx950 with hipcc (device only), checking that it builds and, for the
+    FP64 kernel of the 50-section RC ladder, that it needs no scratch memory
+    (the whole system lives in registers);
+  * for the host with g++, through a small header that maps the handful of
+    HIP names it uses onto host C++, and run lane by lane over a batch: its
+    results are compared with the oracle's AC solves, so the specialised
+    factorisation (scaled upper factor, fused forward substitution) is
+    checked before any GPU run.
+"""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import read_circuit
+
+HIPCC = "/opt/rocm/bin/hipcc"
+
+HOST_SHIM = r"""
+#include <cmath>
+#include <cstring>
+#define __device__
+#define __forceinline__ inline
+#define __global__
+#define __launch_bounds__(x)
+struct kg_dim3 { unsigned x; };
+static kg_dim3 threadIdx, blockIdx, gridDim;
+static inline int kg_frexp_exp(double a) { int e = 0; std::frexp(a, &e); return e; }
+#define __builtin_amdgcn_frexp_exp(a) kg_frexp_exp(a)
+#define __builtin_amdgcn_ldexp(a, e) std::ldexp((a), (e))
+#define __builtin_amdgcn_sched_barrier(x) ((void)0)
+static inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
+static inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
+  unsigned long long o = *p; if (v > o) *p = v; return o; }
+static inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
+using std::fmax; using std::sqrt;
+"""
+
+HOST_DRIVER = r"""
+extern "C" void run_host(const double* prm, const double* eb, const double* fr, int nf, long long nsys,
+                         double* out, int* st, double* err) {
+  gridDim.x = (unsigned)((nsys + 63) / 64);
+  for (unsigned b = 0; b < gridDim.x; b++)
+    for (unsigned t = 0; t < 64; t++) {
+      blockIdx.x = b;
+      threadIdx.x = t;
+      kg_ac_lane(prm, eb, fr, nf, nsys, out, st, err);
+    }
+}
+"""
+
+
+def source(c, precision, refine=2, s_rep=2 * np.pi * 1e3):
+    from krets_amd._lib import lib
+    buf = ctypes.create_string_buffer(1 << 23)
+    el = np.ascontiguousarray(c.elements)
+    pr = np.ascontiguousarray(c.params)
+    r = lib().kg_jit_ac_source(el.ctypes.data, len(el), c.n, pr.ctypes.data, s_rep, precision, refine, buf, len(buf))
+    return r, buf.value.decode()
+
+
+def ebase(elements, params):
+    e = params[..., 0].copy()
+    isr = (elements["type"] == 0) & (elements["g2"] == 0)
+    e[..., isr] = 1.0 / e[..., isr]
+    return e
+
+
+def host_run(src, params, elements, freqs, n):
+    d = tempfile.mkdtemp()
+    try:
+        cpp = os.path.join(d, "k.cpp")
+        so = os.path.join(d, "k.so")
+        open(cpp, "w").write(HOST_SHIM + src + HOST_DRIVER)
+        subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", cpp, "-o", so],
+                       check=True, capture_output=True)
+        L = ctypes.CDLL(so)
+        ni, nf = params.shape[0], len(freqs)
+        prm = np.ascontiguousarray(params, np.float64)
+        eb = np.ascontiguousarray(ebase(elements, params), np.float64)
+        fr = np.ascontiguousarray(freqs, np.float64)
+        out = np.zeros(ni * nf * n * 2)
+        st = np.zeros(ni, np.int32)
+        err = np.zeros(ni)
+        P = ctypes.c_void_p
+        L.run_host.argtypes = [P, P, P, ctypes.c_int, ctypes.c_longlong, P, P, P]
+        L.run_host(prm.ctypes.data, eb.ctypes.data, fr.ctypes.data, nf, ni * nf, out.ctypes.data, st.ctypes.data,
+                   err.ctypes.data)
+        x = out.reshape(ni, nf, n, 2)
+        return x[..., 0] + 1j * x[..., 1], st, err
+    finally:
+        shutil.rmtree(d)
+
+
+def nerr(x, r):
+    return np.linalg.norm(x - r) / np.linalg.norm(r)
+
+
+@pytest.mark.parametrize("precision,refine", [(0, 2), (1, 1), (1, 2), (1, 3)])
+def test_lane_source_on_host_vs_oracle(oracle, precision, refine):
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    b = W.c2_rc_ladder_mc(3, sections=50, seed=21)
+    c = kg.Circuit(b.netlists[0])
+    r, src = source(c, precision, refine)
+    assert r > 0, src
+    f = W.c2_frequencies(200)[::25]
+    x, st, err = host_run(src, b.params, c.elements, f, c.n)
+    assert (st == 0).all()
+    for i in range(3):
+        xr = oracle.Circuit(b.netlists[i]).ac(f)
+        e = max(nerr(x[i, j], xr[j]) for j in range(len(f)))
+        assert e < (1e-12 if precision == 0 or refine >= 2 else 1e-7), (i, e)
+    if precision == 1:  # estimate: the last correction, scaled by the contraction seen
+        assert (err > 0).all() and (err < (1e-4 if refine == 1 else 1e-8)).all()
+
+
+def test_lane_source_golden_ac_on_host():
+    import json
+    import krets_amd as kg
+    from conftest import GOLDEN
+    g = json.load(open(os.path.join(GOLDEN, "low_pass_ac_result.json")))
+    cols = {k: np.array([float(v) for v in vals]) for k, vals in g["columns"].items()}
+    c = kg.Circuit(read_circuit("low_pass_filter/ac/ac.cir"))
+    r, src = source(c, 0)
+    assert r > 0, src
+    x, st, _ = host_run(src, c.params[None], c.elements, cols["frequency"], c.n)
+    assert st[0] == 0
+    for name in ("I(V1)", "V(in)", "V(out)"):
+        v = x[0][:, c.names.index(name)]
+        assert np.max(np.abs(np.abs(v) - cols[name + "_mag"]) / cols[name + "_mag"]) < 1e-12
+
+
+def test_lane_source_singular_pivot_flagged():
+    # node a is reached only through zero-valued capacitors: its row and
+    # column are exactly zero, so its pivot is zero (Decomposition)
+    import krets_amd as kg
+    c = kg.Circuit("V1 in 0 0 AC 1\nR1 in 0 1000\nC1 in a 0\nC2 a 0 0\n")
+    r, src = source(c, 0)
+    assert r > 0, src
+    _, st, _ = host_run(src, c.params[None], c.elements, np.array([1e3]), c.n)
+    assert st[0] == 2  # KG_DECOMPOSITION
+
+
+def test_lane_source_declines_diodes_and_large_systems():
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    c = kg.Circuit(read_circuit("rectifier/rectifier.cir"))
+    r, why = source(c, 0)
+    assert r < 0 and "diode" in why
+    b = W.c2_rc_ladder_mc(1, sections=200, seed=1)
+    r, why = source(kg.Circuit(b.netlists[0]), 0)
+    assert r < 0 and "register" in why
+
+
+@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lane_source_builds_for_gfx950(precision):
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    b = W.c2_rc_ladder_mc(1, sections=50, seed=2)
+    r, src = source(kg.Circuit(b.netlists[0]), precision)
+    assert r > 0, src
+    d = tempfile.mkdtemp()
+    try:
+        p = os.path.join(d, "k.hip")
+        open(p, "w").write("#include <hip/hip_runtime.h>\n" + src)
+        out = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                              "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", p, "-o", p + ".o",
+                              "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
+        assert out.returncode == 0, out.stderr[-2000:]
+        scratch = int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1))
+        if precision == 0:
+            assert scratch == 0
+    finally:
+        shutil.rmtree(d)
