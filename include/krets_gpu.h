@@ -94,8 +94,9 @@ typedef struct {
   /* Work mapping; 0 = autotuned. 1, 4, 16, 64, 256: threads per system of the
    * level-scheduled kernels. AC sweeps of a shared topology also take
    * 1000 + t (t = 1, 4, 16, 64: the plan and each instance's element records
-   * staged in LDS by a persistent grid) and 2001 (one lane per system, code
-   * specialised to the topology at run time). */
+   * staged in LDS by a persistent grid), 2001 (one lane per system, code
+   * specialised to the topology at run time) and, for KG_MIXED, 2002 (as
+   * 2001 with the FP64 solution kept in the output between passes). */
   int32_t threads_per_system;
   int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
 } kg_solver_config;
@@ -203,7 +204,9 @@ int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n
 
 /* Diagnostic: the HIP source of the run-time specialised lane-per-system AC
  * kernel for one topology (minimum-degree plan, static pivots from params at
- * reactive scale s_rep). No counterpart in krets. Returns the source length
+ * reactive scale s_rep; refine_iters < 0 selects the mixed precision form
+ * that keeps the FP64 solution in memory between passes; otherwise the
+ * refinement count is a launch argument). No counterpart in krets. Returns the source length
  * (copying at most cap bytes, NUL-terminated, to buf) or -status, with the
  * reason the topology is not specialised in buf. */
 int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,

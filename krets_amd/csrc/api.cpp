@@ -733,7 +733,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   // for the persistent variant (shared topology only) that stages the plan
   // and each instance's element records in LDS.
   // 2001: the lane-per-system kernel specialised to this topology at run
-  // time (shared topology, small systems).
+  // time (shared topology, small systems); 2002: its mixed precision form
+  // with the FP64 solution kept in the output array between passes.
   struct AcLaunch {
     Plan P;
     Lay lay;
@@ -745,7 +746,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   };
   const int64_t nsys = (int64_t)c->n_inst * nf;
   auto setup = [&](int cand, AcLaunch& A) -> int32_t {
-    A.lane = cand == 2001;
+    A.lane = cand == 2001 || cand == 2002;
     A.persistent = cand >= 1000 && !A.lane;
     A.tps = cand % 1000;
     if (A.lane) {
@@ -754,7 +755,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       if (rr) return rr;
       const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(0, (int)AK_AC, (int)ORD_MINDEG, s_ac)]];
       std::string src, why, log;
-      if (!jit_ac_source(pe.lay, pe.host.data(), mixed, refine, src, why))
+      if (cand == 2002 && !mixed) return set_err(c, KG_UNEXPECTED, "2002 is the mixed precision lane kernel");
+      if (!jit_ac_source(pe.lay, pe.host.data(), mixed, cand == 2002, src, why))
         return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
       if (jit_load(c->device, src, "kg_ac_lane", A.fn, log) != hipSuccess)
         return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
@@ -798,7 +800,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       double* outp = d_out;
       int32_t* stp = d_st;
       double* errp = mixed ? d_err : nullptr;
-      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp};
+      int rf = refine;
+      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp, &rf};
       return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr);
     }
     BatchDev B = batch_dev(c, A.P);
@@ -812,6 +815,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if (cand == 0) {
     std::vector<int> cands{1, 4, 16, 64, 256};
     if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064, 2001});
+    if (c->shared && mixed) cands.push_back(2002);
     r = autotune(c, tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf), cands, [&](int t, float* ms) -> int32_t {
       AcLaunch A;
       int32_t rr = setup(t, A);
@@ -974,7 +978,8 @@ int32_t kg_jit_ac_source(const kg_element* el, int32_t ne, int32_t n, const doub
   std::string err, src, why;
   std::vector<int32_t> b = build_pattern(el, ne, n, AK_AC, ORD_MINDEG, params, s_rep, L, err);
   if (L.status != KG_OK) return -L.status;
-  if (!jit_ac_source(L, b.data(), precision == KG_MIXED, refine_iters > 0 ? refine_iters : 2, src, why)) {
+  // refine_iters < 0 selects the mixed precision form that keeps x in memory
+  if (!jit_ac_source(L, b.data(), precision == KG_MIXED, refine_iters < 0, src, why)) {
     if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
     return -KG_UNEXPECTED;
   }

@@ -98,24 +98,33 @@ __device__ __forceinline__ i64 xcd_block() {
 )";
 
 // Emits straight-line code. Element values are loaded once per pass (a pass
-// is a brace scope), LOOKAHEAD rows before the row that first uses them, and
+// is a brace scope), `lookahead` rows before the row that first uses them, and
 // an empty asm statement with a memory clobber after each row's loads keeps
 // the compiler from hoisting every load of the kernel to its start (which
 // would hold all element values in registers at once).
-constexpr int LOOKAHEAD = 8;
+
+// refinement passes load x as well as element values: a shorter distance
+// keeps the values in flight within the register budget
+int LOOKAHEAD_REFINE = 4;
 
 struct Gen {
   const View& P;
   std::ostringstream out;         // kernel text
   std::ostringstream* cur = &out; // where statements go
   std::vector<std::string> loads; // loads first needed by the row being generated
-  std::set<int> have_g, have_p;
+  std::set<int> have_g, have_p, have_x;
   std::string why;
   bool track_bm = false;          // mixed precision: |b|_inf^2 for the correction scale
+  bool xmem = false;              // the FP64 solution lives in the output array between passes
+  int lookahead = 8;              // rows between a load and the row that first uses it
   std::function<void(std::ostringstream&, int)> on_result;
   explicit Gen(const View& v) : P(v) {}
 
-  std::string g(int e) {  // 1/R (or the raw value for other types), per instance
+  // Element values come from the per-instance value table (1/R for plain
+  // resistors, the element value otherwise: C, L, G2 resistors, current
+  // sources), whose small offsets fit the loads' immediate field; only the
+  // AC magnitude of voltage sources is read from the parameter records.
+  std::string g(int e) {
     std::string nm = "g" + std::to_string(e);
     if (have_g.insert(e).second) loads.push_back("    const double " + nm + " = EB[" + std::to_string(e) + "];\n");
     return nm;
@@ -138,15 +147,15 @@ struct Gen {
         case CK_YN: {
           const char* sg = k == CK_YN ? "-" : "";
           if (ty == KG_R) R.push_back(sg + g(e));
-          else if (ty == KG_C) I.push_back(std::string(sg) + "(w * " + p(e, 0) + ")");
+          else if (ty == KG_C) I.push_back(std::string(sg) + "(w * " + g(e) + ")");
           else if (ty == KG_D) { why = "diode small-signal stamps are not generated"; return false; }
           break;
         }
         case CK_ONEP: R.push_back("1.0"); break;
         case CK_ONEN: R.push_back("-1.0"); break;
         case CK_BR:
-          if (ty == KG_L) I.push_back("-(w * " + p(e, 0) + ")");
-          else if (ty == KG_R) R.push_back("-" + p(e, 0));
+          if (ty == KG_L) I.push_back("-(w * " + g(e) + ")");
+          else if (ty == KG_R) R.push_back("-" + g(e));
           else R.push_back("1.0");
           break;
         default: why = "unknown stamp kind"; return false;
@@ -169,7 +178,7 @@ struct Gen {
     for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) {
       int code = P.rcon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
       if (k != RK_SRC) { why = "only source excitations are generated"; return false; }
-      R.push_back(ty == KG_V ? p(e, 1) : p(e, 0));
+      R.push_back(ty == KG_V ? p(e, 1) : g(e));
     }
     re.clear();
     for (size_t t = 0; t < R.size(); t++) re += (t ? " + " : "") + R[t];
@@ -182,14 +191,23 @@ struct Gen {
     return F == "Z" ? zval : "cw(" + zval + ")";
   }
 
+  // x_j for a residual: a register, or (xmem) a load from the output array
+  std::string xref(int j) {
+    if (!xmem) return "x" + std::to_string(j);
+    std::string nm = "xl" + std::to_string(j);
+    if (have_x.insert(j).second) loads.push_back("    const Z " + nm + " = o[" + std::to_string(P.col_of[j]) + "];\n");
+    return nm;
+  }
+
   // One factorisation + forward + backward pass in type F ("Z" or "W"),
   // appended to `out`. mode 0: right-hand side b; mode 1: scaled FP64
-  // residual b - A x (x in x<i>). Row i's back-substitution result t<i> is
-  // handed to on_result.
+  // residual b - A x. Row i's back-substitution result t<i> is handed to
+  // on_result; with xmem the previous x_i is loaded ahead as xb<i>.
   bool pass(const std::string& F, int mode) {
     const int n = P.n;
     have_g.clear();
     have_p.clear();
+    have_x.clear();
     std::vector<std::string> code(n);
     std::vector<std::vector<std::string>> rload(n);
     std::vector<char> yz(n, 1);  // y'_i structurally zero
@@ -221,7 +239,7 @@ struct Gen {
       } else {
         o << "    Z r" << i << " = " << (b.empty() ? std::string("Z{0.0, 0.0}") : "Z{" + b + ", 0.0}") << ";\n";
         for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++)
-          if (has_slot(s)) o << "    r" << i << " = r" << i << " - a" << s << " * x" << P.col[s] << ";\n";
+          if (has_slot(s)) o << "    r" << i << " = r" << i << " - a" << s << " * " << xref(P.col[s]) << ";\n";
         o << "    " << F << " y" << i << " = " << conv(F, "scal(inv_s, r" + std::to_string(i) + ")") << ";\n";
         ynz = true;
       }
@@ -255,21 +273,30 @@ struct Gen {
       code[i] = o.str();
       rload[i] = loads;
     }
-    // rows with their loads issued LOOKAHEAD rows early
+    // rows with their loads issued `lookahead` rows early
     std::ostringstream& O = out;
+    // nothing stored by the previous pass may be forwarded into registers
+    if (xmem) O << "    asm volatile(\"\" ::: \"memory\");\n";
     auto issue = [&](int r) {
       if (r >= n || rload[r].empty()) return;
       for (auto& l : rload[r]) O << l;
       O << "    asm volatile(\"\" ::: \"memory\");\n";
     };
-    for (int r = 0; r < LOOKAHEAD; r++) issue(r);
+    for (int r = 0; r < lookahead; r++) issue(r);
     for (int i = 0; i < n; i++) {
-      issue(i + LOOKAHEAD);
+      issue(i + lookahead);
       O << code[i];
       O << "    __builtin_amdgcn_sched_barrier(0);\n";
     }
     // back substitution
+    const bool bl = xmem && mode == 1;
+    auto bissue = [&](int r) {
+      if (!bl || r < 0) return;
+      O << "    const Z xb" << r << " = o[" << P.col_of[r] << "];\n    asm volatile(\"\" ::: \"memory\");\n";
+    };
+    for (int r = n - 1; r >= n - lookahead; r--) bissue(r);
     for (int i = n - 1; i >= 0; i--) {
+      bissue(i - lookahead);
       std::string acc = yz[i] ? F + "{0, 0}" : "y" + std::to_string(i) + "s";
       O << "    " << F << " t" << i << " = " << acc << ";\n";
       for (int s = P.diag[i] + 1; s < P.row_ptr[i + 1]; s++)
@@ -283,26 +310,26 @@ struct Gen {
 
 }  // namespace
 
-bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, int refine, std::string& src,
+bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem, std::string& src,
                    std::string& why) {
   View P = view(L, buf);
   const int n = P.n;
   if (L.status != KG_OK) { why = "structurally singular pattern"; return false; }
   if (L.has_diode) { why = "diode small-signal stamps are not generated"; return false; }
-  if (refine < 0 || refine > 4) { why = "refinement count outside 0..4"; return false; }
   int nu = 0;
   for (int i = 0; i < n; i++) nu += P.row_ptr[i + 1] - P.diag[i] - 1;
   // registers per lane held across a pass: U' and y' (and the FP64 solution
   // for mixed precision); the register file holds 512 per lane at one wave
   // per SIMD, and the pass needs a few dozen more for its temporaries
-  int regs = mixed ? 2 * (nu + n) + 4 * n : 4 * (nu + n);
+  int regs = !mixed ? 4 * (nu + n) : 2 * (nu + n) + (xmem ? 0 : 4 * n);
   if (regs > 440) { why = "per-lane state exceeds the register budget"; return false; }
   Gen G(P);
+  G.xmem = mixed && xmem;
   std::ostringstream& o = G.out;
   o << kPrelude;
   o << "extern \"C\" __global__ void __launch_bounds__(64) kg_ac_lane(const double* __restrict__ prm,\n"
        "    const double* __restrict__ ebase, const double* __restrict__ freqs, int nf, i64 nsys,\n"
-       "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err) {\n"
+       "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err, int refine) {\n"
        "  const i64 sys = xcd_block() * 64 + threadIdx.x;\n"
        "  if (sys >= nsys) return;\n"
        "  const int inst = (int)(sys / nf), fi = (int)(sys - (i64)inst * nf);\n"
@@ -317,10 +344,17 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, int refin
     if (!G.pass("Z", 0)) { why = G.why; return false; }
     o << "  }\n";
   } else {
-    for (int i = 0; i < n; i++) o << "  Z x" << i << ";\n";
-    o << "  double est = 0.0, dprev = 0.0, bm2 = 0.0;\n";
+    if (!G.xmem)
+      for (int i = 0; i < n; i++) o << "  Z x" << i << ";\n";
+    o << "  double est = 0.0, dprev = 0.0, bm2 = 0.0, xm2 = 0.0;\n";
     // pass 0: complex64 solve of b; x = its FP64 copy
-    G.on_result = [&](std::ostringstream& O, int i) { O << "    x" << i << " = cz(t" << i << ");\n"; };
+    // |x|_inf is taken from this first solution (the refinement changes it by
+    // the FP32 solve error only); tracking it through the refinement passes
+    // makes the compiler keep far more values live
+    G.on_result = [&](std::ostringstream& O, int i) {
+      if (G.xmem) O << "    { const Z xi = cz(t" << i << "); o[" << P.col_of[i] << "] = xi; xm2 = fmax(xm2, m2(xi)); }\n";
+      else O << "    x" << i << " = cz(t" << i << "); xm2 = fmax(xm2, m2(x" << i << "));\n";
+    };
     G.track_bm = true;
     o << "  {\n";
     if (!G.pass("W", 0)) { why = G.why; return false; }
@@ -328,20 +362,27 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, int refin
     G.track_bm = false;
     // refinement: corrections scaled by |b|_inf stay in FP32 range
     o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0;\n";
-    for (int it = 0; it < refine; it++) {
-      G.on_result = [&](std::ostringstream& O, int i) {
-        O << "    { const Z d = scal(s, cz(t" << i << ")); x" << i << " = x" << i << " + d; dm2 = fmax(dm2, m2(d));"
-          << " xm2 = fmax(xm2, m2(x" << i << ")); }\n";
-      };
-      // no branch: with s == 0 (b == 0, x == 0) the correction is zero
-      o << "  {\n    double dm2 = 0.0, xm2 = 0.0;\n";
-      if (!G.pass("W", 1)) { why = G.why; return false; }
-      o << "    const double dm = sqrt(dm2), xm = sqrt(xm2);\n"
-        << "    const double rho = (" << it << " > 0 && dprev > 0.0) ? fmin(dm / dprev, 1.0) : 1.0;\n"
-        << "    est = xm > 0.0 ? rho * dm / xm : 0.0;\n"
-        << "    dprev = dm;\n  }\n";
-    }
-    for (int i = 0; i < n; i++) o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";
+    // one refinement pass inside a run-time loop: the pass body is compiled
+    // once, so its registers are allocated once (an unrolled sequence of
+    // passes lets live ranges of neighbouring passes overlap)
+    G.on_result = [&](std::ostringstream& O, int i) {
+      if (G.xmem)
+        O << "    { const Z d = scal(s, cz(t" << i << ")); o[" << P.col_of[i] << "] = xb" << i
+          << " + d; dm2 = fmax(dm2, m2(d)); }\n";
+      else
+        O << "    { const Z d = scal(s, cz(t" << i << ")); x" << i << " = x" << i << " + d; dm2 = fmax(dm2, m2(d)); }\n";
+    };
+    o << "  if (s > 0.0)\n"
+         "#pragma clang loop unroll(disable)\n"
+         "  for (int it = 0; it < refine; it++) {\n    double dm2 = 0.0;\n";
+    G.lookahead = LOOKAHEAD_REFINE;
+    if (!G.pass("W", 1)) { why = G.why; return false; }
+    o << "    const double dm = sqrt(dm2), xm = sqrt(xm2);\n"
+      << "    const double rho = (it > 0 && dprev > 0.0) ? fmin(dm / dprev, 1.0) : 1.0;\n"
+      << "    est = xm > 0.0 ? rho * dm / xm : 0.0;\n"
+      << "    dprev = dm;\n  }\n";
+    if (!G.xmem)
+      for (int i = 0; i < n; i++) o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";
     o << "  if (err != nullptr && est > 0.0)\n"
          "    atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(est));\n";
   }

@@ -22,10 +22,13 @@ struct JitFn {
 };
 
 // Source of the lane-per-system AC sweep kernel `kg_ac_lane` for one pattern
-// (AC kind, shared by every instance). Returns false, with the reason in
-// `why`, when the pattern is outside what the generator handles (diodes,
-// patterns whose per-lane state would not fit in registers).
-bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, int refine, std::string& src,
+// (AC kind, shared by every instance); the refinement count of the mixed
+// precision kernel is a launch argument. xmem (mixed precision only) keeps
+// the FP64 solution in the output array between refinement passes instead
+// of registers. Returns false, with the reason in `why`, when the pattern is
+// outside what the generator handles (diodes, patterns whose per-lane state
+// would not fit in registers).
+bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem, std::string& src,
                    std::string& why);
 
 // Compiles `src` for the device's architecture and loads kernel `name`;

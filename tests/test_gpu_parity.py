@@ -163,8 +163,9 @@ def test_transient_compat_vs_oracle(kg, oracle, path, h, stop):
         assert abs(res[42]["V(out)"] - 0.8647) < 1e-3 and abs(res[42]["time"] - 2.1e-3) < 1e-6
 
 
-# 1000 + t: persistent LDS-staged variant; 2001: lane per system, run-time specialised
-@pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064, 2001])
+# 1000 + t: persistent LDS-staged variant; 2001: lane per system, run-time
+# specialised; 2002: as 2001 with x kept in memory between refinement passes
+@pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064, 2001, 2002])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
     from krets_amd import workloads as W
@@ -173,6 +174,10 @@ def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
     f = W.c2_frequencies(200)[::10]
     ctx = kg.Context()
     b.load(ctx)
+    if tps == 2002 and precision == 0:
+        with pytest.raises(kg.KretsGpuError):
+            ctx.ac_sweep(f, precision=precision, config=kg.SolverConfig(threads_per_system=tps))
+        return
     xs, st, err = ctx.ac_sweep(f, precision=precision, config=kg.SolverConfig(threads_per_system=tps))
     assert (st == 0).all()
     for i in (0, 7, 23):
@@ -185,15 +190,15 @@ def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
 
 
 @pytest.mark.parametrize("refine", [1, 2, 3])
-@pytest.mark.parametrize("precision", [0, 1])
-def test_lane_ac_golden_and_rlc(kg, oracle, precision, refine):
+@pytest.mark.parametrize("precision,lane", [(0, 2001), (1, 2001), (1, 2002)])
+def test_lane_ac_golden_and_rlc(kg, oracle, precision, lane, refine):
     """Run-time specialised lane kernel: the krets AC golden vector and a shared
     RLC ladder batch (inductor branch rows) vs the oracle."""
     import json, os
     from conftest import GOLDEN
     from krets_amd import workloads as W
-    say(f"lane AC precision={precision} refine={refine}")
-    cfg = kg.SolverConfig(threads_per_system=2001, refine_iters=refine)
+    say(f"lane AC precision={precision} lane={lane} refine={refine}")
+    cfg = kg.SolverConfig(threads_per_system=lane, refine_iters=refine)
     g = json.load(open(os.path.join(GOLDEN, "low_pass_ac_result.json")))
     cols = {k: np.array([float(v) for v in vals]) for k, vals in g["columns"].items()}
     c = kg.Circuit(read_circuit("low_pass_filter/ac/ac.cir"))
