@@ -71,7 +71,11 @@ struct kg_ctx {
   PatDev* d_pats = nullptr;
   size_t d_pats_n = 0;
   std::map<std::tuple<int, int, double>, int32_t*> inst_pat;
+  // run-time specialised kernels per (pattern, mixed, x in memory)
+  std::map<std::tuple<int, int, int>, JitFn> lane_fn;
   std::map<std::string, DevBuf> ws;
+  // last host contents copied into a workspace buffer (skips re-uploads)
+  std::map<std::string, std::pair<void*, std::vector<char>>> uploaded;
   // adjoint state kept between kg_adjoint_forward and kg_adjoint_backward
   struct {
     bool ready = false;
@@ -109,6 +113,7 @@ void* ws_get(kg_ctx* c, const std::string& name, size_t bytes, hipError_t* e) {
   DevBuf& b = c->ws[name];
   *e = hipSuccess;
   if (b.cap < bytes) {
+    c->uploaded.erase(name);  // a new allocation holds nothing uploaded
     if (b.p) hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
@@ -134,6 +139,7 @@ void free_batch(kg_ctx* c) {
   for (auto& kv : c->inst_pat)
     if (kv.second) hipFree(kv.second);
   c->inst_pat.clear();
+  c->lane_fn.clear();
   c->n_inst = 0;
 }
 
@@ -331,12 +337,22 @@ std::vector<int64_t> out_offsets(const kg_ctx* c, int64_t per_unknown) {
   return o;
 }
 
-int32_t upload_offsets(kg_ctx* c, const std::string& name, const std::vector<int64_t>& o, int64_t** d) {
+// Copies host data into workspace buffer `name` unless the buffer already
+// holds exactly these bytes from an earlier call.
+int32_t upload(kg_ctx* c, const std::string& name, const void* src, size_t bytes, void** d) {
   hipError_t e;
-  *d = (int64_t*)ws_get(c, name, sizeof(int64_t) * o.size(), &e);
+  *d = ws_get(c, name, bytes, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
-  HIPCHK(c, hipMemcpyAsync(*d, o.data(), sizeof(int64_t) * o.size(), hipMemcpyHostToDevice, c->stream));
+  auto& u = c->uploaded[name];
+  if (u.first == *d && u.second.size() == bytes && std::memcmp(u.second.data(), src, bytes) == 0) return KG_OK;
+  HIPCHK(c, hipMemcpyAsync(*d, src, bytes, hipMemcpyHostToDevice, c->stream));
+  u.first = *d;
+  u.second.assign((const char*)src, (const char*)src + bytes);
   return KG_OK;
+}
+
+int32_t upload_offsets(kg_ctx* c, const std::string& name, const std::vector<int64_t>& o, int64_t** d) {
+  return upload(c, name, o.data(), sizeof(int64_t) * o.size(), (void**)d);
 }
 
 // Runs `launch` on the context stream, timing it with HIP events when
@@ -722,9 +738,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   int64_t* d_off;
   if ((r = upload_offsets(c, "ac_off", off, &d_off))) return r;
   hipError_t e;
-  double* d_f = (double*)ws_get(c, "freqs", sizeof(double) * nf, &e);
-  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
-  HIPCHK(c, hipMemcpyAsync(d_f, freqs, sizeof(double) * nf, hipMemcpyHostToDevice, c->stream));
+  double* d_f = nullptr;
+  if ((r = upload(c, "freqs", freqs, sizeof(double) * nf, (void**)&d_f))) return r;
   double* d_out = (mem == KG_MEM_DEVICE && x) ? x : (double*)ws_get(c, "ac_out", sizeof(double) * off.back(), &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
@@ -753,13 +768,21 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       if (!c->shared) return set_err(c, KG_UNEXPECTED, "lane kernel needs a shared topology");
       int32_t rr = prepare(c, AK_AC, ORD_MINDEG, s_ac, A.P);
       if (rr) return rr;
-      const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(0, (int)AK_AC, (int)ORD_MINDEG, s_ac)]];
-      std::string src, why, log;
       if (cand == 2002 && !mixed) return set_err(c, KG_UNEXPECTED, "2002 is the mixed precision lane kernel");
-      if (!jit_ac_source(pe.lay, pe.host.data(), mixed, cand == 2002, src, why))
-        return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
-      if (jit_load(c->device, src, "kg_ac_lane", A.fn, log) != hipSuccess)
-        return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
+      int pidx = c->pat_index[std::make_tuple(0, (int)AK_AC, (int)ORD_MINDEG, s_ac)];
+      auto key = std::make_tuple(pidx, (int)mixed, (int)(cand == 2002));
+      auto it = c->lane_fn.find(key);
+      if (it == c->lane_fn.end()) {
+        const PatEntry& pe = c->pats[pidx];
+        std::string src, why, log;
+        if (!jit_ac_source(pe.lay, pe.host.data(), mixed, cand == 2002, src, why))
+          return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
+        if (jit_load(c->device, src, "kg_ac_lane", A.fn, log) != hipSuccess)
+          return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
+        c->lane_fn[key] = A.fn;
+      } else {
+        A.fn = it->second;
+      }
       A.L.grid = dim3((unsigned)((nsys + 63) / 64));
       A.L.block = dim3(64);
       return KG_OK;
