@@ -96,7 +96,8 @@ typedef struct {
    * 1000 + t (t = 1, 4, 16, 64: the plan and each instance's element records
    * staged in LDS by a persistent grid), 2001 (one lane per system, code
    * specialised to the topology at run time) and, for KG_MIXED, 2002 (as
-   * 2001 with the FP64 solution kept in the output between passes). */
+   * 2001 with the solution split into a complex64 first solution in LDS and
+   * a complex64 sum of corrections in registers). */
   int32_t threads_per_system;
   int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
 } kg_solver_config;
@@ -178,11 +179,12 @@ int32_t kg_adjoint_forward(kg_ctx* ctx, const kg_solver_config* cfg, double h, i
                            int32_t* status, int32_t mem);
 int32_t kg_adjoint_backward(kg_ctx* ctx, double* grad, int32_t* status, int32_t mem);
 
-/* Kernel timing (no krets counterpart): reads the accumulated duration and
- * launch count of the analysis kernels timed since the previous call (HIP
- * events on the context's stream), then resets them and turns timing on
- * (enable = 1) or off. */
-int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launches, const char** last_kernel);
+/* Kernel timing (no krets counterpart): HIP events on the context's stream
+ * time every analysis kernel launched while timing is on, per kernel. Reports
+ * the dominant kernel since the previous call (the one with the largest
+ * accumulated time): its name, accumulated ms and launch count; then resets
+ * and turns timing on (enable = 1) or off. */
+int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launches, const char** kernel);
 
 /* Autotuner report (no krets counterpart): the candidate mappings (threads
  * per system) timed by the most recent autotuning run and their kernel times
@@ -205,8 +207,9 @@ int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n
 /* Diagnostic: the HIP source of the run-time specialised lane-per-system AC
  * kernel for one topology (minimum-degree plan, static pivots from params at
  * reactive scale s_rep; refine_iters < 0 selects the mixed precision form
- * that keeps the FP64 solution in memory between passes; otherwise the
- * refinement count is a launch argument). No counterpart in krets. Returns the source length
+ * with the solution split into a complex64 first solution in LDS and a
+ * complex64 sum of corrections in registers; the refinement count is a
+ * launch argument). No counterpart in krets. Returns the source length
  * (copying at most cap bytes, NUL-terminated, to buf) or -status, with the
  * reason the topology is not specialised in buf. */
 int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,

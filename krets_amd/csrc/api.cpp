@@ -90,9 +90,8 @@ struct kg_ctx {
   std::vector<std::pair<int, float>> last_trials;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
-  double prof_ms = 0.0;
-  int64_t prof_n = 0;
-  std::string prof_name;
+  std::map<std::string, std::pair<double, int64_t>> prof_k;  // kernel -> (ms, launches)
+  std::string prof_name;  // kernel reported by the last kg_profile call
 };
 
 namespace {
@@ -376,9 +375,9 @@ int32_t timed(kg_ctx* c, const char* name, F&& launch) {
     HIPCHK(c, hipEventSynchronize(b));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, a, b));
-    c->prof_ms += ms;
-    c->prof_n += 1;
-    c->prof_name = name;
+    auto& k = c->prof_k[name];
+    k.first += ms;
+    k.second += 1;
     hipEventDestroy(a);
     hipEventDestroy(b);
   }
@@ -749,7 +748,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   // and each instance's element records in LDS.
   // 2001: the lane-per-system kernel specialised to this topology at run
   // time (shared topology, small systems); 2002: its mixed precision form
-  // with the FP64 solution kept in the output array between passes.
+  // with the solution split into a complex64 first solution in LDS and a
+  // complex64 sum of corrections in registers.
   struct AcLaunch {
     Plan P;
     Lay lay;
@@ -785,6 +785,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       }
       A.L.grid = dim3((unsigned)((nsys + 63) / 64));
       A.L.block = dim3(64);
+      A.L.lds = cand == 2002 ? 64 * sizeof(float) * 2 * (size_t)c->nunk[0] : 0;  // x0 per lane (complex64)
       return KG_OK;
     }
     if (!valid_tps(A.tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
@@ -825,7 +826,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       double* errp = mixed ? d_err : nullptr;
       int rf = refine;
       void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp, &rf};
-      return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr);
+      return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, (unsigned)A.L.lds, c->stream, args, nullptr);
     }
     BatchDev B = batch_dev(c, A.P);
     if (A.persistent)
@@ -966,14 +967,23 @@ int32_t kg_last_autotune(const kg_ctx* c, int32_t* tps, float* ms, int32_t cap) 
   return n;
 }
 
-int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launches, const char** last_kernel) {
+int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launches, const char** kernel) {
   if (!c) return KG_INVALID_ARGUMENT;
-  if (total_ms) *total_ms = c->prof_ms;
-  if (launches) *launches = c->prof_n;
-  if (last_kernel) *last_kernel = c->prof_name.c_str();
+  // the dominant kernel: largest accumulated time
+  double ms = 0.0;
+  int64_t n = 0;
+  c->prof_name.clear();
+  for (auto& kv : c->prof_k)
+    if (kv.second.first > ms) {
+      ms = kv.second.first;
+      n = kv.second.second;
+      c->prof_name = kv.first;
+    }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = n;
+  if (kernel) *kernel = c->prof_name.c_str();
   c->prof = enable != 0;
-  c->prof_ms = 0.0;
-  c->prof_n = 0;
+  c->prof_k.clear();
   return KG_OK;
 }
 
@@ -1001,7 +1011,7 @@ int32_t kg_jit_ac_source(const kg_element* el, int32_t ne, int32_t n, const doub
   std::string err, src, why;
   std::vector<int32_t> b = build_pattern(el, ne, n, AK_AC, ORD_MINDEG, params, s_rep, L, err);
   if (L.status != KG_OK) return -L.status;
-  // refine_iters < 0 selects the mixed precision form that keeps x in memory
+  // refine_iters < 0 selects the mixed precision form with the split solution
   if (!jit_ac_source(L, b.data(), precision == KG_MIXED, refine_iters < 0, src, why)) {
     if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
     return -KG_UNEXPECTED;

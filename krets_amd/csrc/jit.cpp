@@ -115,7 +115,9 @@ struct Gen {
   std::set<int> have_g, have_p, have_x;
   std::string why;
   bool track_bm = false;          // mixed precision: |b|_inf^2 for the correction scale
-  bool xmem = false;              // the FP64 solution lives in the output array between passes
+  // mixed precision with the solution split as x = x0 + dx: x0 (the complex64
+  // first solution) in LDS, dx (complex64 sum of the corrections) in registers
+  bool xsplit = false;
   int lookahead = 8;              // rows between a load and the row that first uses it
   std::function<void(std::ostringstream&, int)> on_result;
   explicit Gen(const View& v) : P(v) {}
@@ -191,18 +193,19 @@ struct Gen {
     return F == "Z" ? zval : "cw(" + zval + ")";
   }
 
-  // x_j for a residual: a register, or (xmem) a load from the output array
+  // FP64 x_j for a residual: a register, or (xsplit) x0_j read from LDS ahead
+  // of its row plus dx_j
   std::string xref(int j) {
-    if (!xmem) return "x" + std::to_string(j);
+    if (!xsplit) return "x" + std::to_string(j);
     std::string nm = "xl" + std::to_string(j);
-    if (have_x.insert(j).second) loads.push_back("    const Z " + nm + " = o[" + std::to_string(P.col_of[j]) + "];\n");
-    return nm;
+    if (have_x.insert(j).second) loads.push_back("    const W " + nm + " = xs[" + std::to_string(64 * j) + " + lane];\n");
+    return "(cz(" + nm + ") + cz(dx" + std::to_string(j) + "))";
   }
 
   // One factorisation + forward + backward pass in type F ("Z" or "W"),
   // appended to `out`. mode 0: right-hand side b; mode 1: scaled FP64
   // residual b - A x. Row i's back-substitution result t<i> is handed to
-  // on_result; with xmem the previous x_i is loaded ahead as xb<i>.
+  // on_result.
   bool pass(const std::string& F, int mode) {
     const int n = P.n;
     have_g.clear();
@@ -275,8 +278,6 @@ struct Gen {
     }
     // rows with their loads issued `lookahead` rows early
     std::ostringstream& O = out;
-    // nothing stored by the previous pass may be forwarded into registers
-    if (xmem) O << "    asm volatile(\"\" ::: \"memory\");\n";
     auto issue = [&](int r) {
       if (r >= n || rload[r].empty()) return;
       for (auto& l : rload[r]) O << l;
@@ -289,14 +290,7 @@ struct Gen {
       O << "    __builtin_amdgcn_sched_barrier(0);\n";
     }
     // back substitution
-    const bool bl = xmem && mode == 1;
-    auto bissue = [&](int r) {
-      if (!bl || r < 0) return;
-      O << "    const Z xb" << r << " = o[" << P.col_of[r] << "];\n    asm volatile(\"\" ::: \"memory\");\n";
-    };
-    for (int r = n - 1; r >= n - lookahead; r--) bissue(r);
     for (int i = n - 1; i >= 0; i--) {
-      bissue(i - lookahead);
       std::string acc = yz[i] ? F + "{0, 0}" : "y" + std::to_string(i) + "s";
       O << "    " << F << " t" << i << " = " << acc << ";\n";
       for (int s = P.diag[i] + 1; s < P.row_ptr[i + 1]; s++)
@@ -310,7 +304,7 @@ struct Gen {
 
 }  // namespace
 
-bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem, std::string& src,
+bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xsplit, std::string& src,
                    std::string& why) {
   View P = view(L, buf);
   const int n = P.n;
@@ -321,10 +315,12 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem
   // registers per lane held across a pass: U' and y' (and the FP64 solution
   // for mixed precision); the register file holds 512 per lane at one wave
   // per SIMD, and the pass needs a few dozen more for its temporaries
-  int regs = !mixed ? 4 * (nu + n) : 2 * (nu + n) + (xmem ? 0 : 4 * n);
+  // (xsplit: dx in registers, x0 in 512 * n bytes of LDS per 64-lane block)
+  int regs = !mixed ? 4 * (nu + n) : 2 * (nu + n) + (xsplit ? 2 * n : 4 * n);
+  if (mixed && xsplit && n > 78) { why = "x0 would not fit in LDS at four waves per CU"; return false; }
   if (regs > 440) { why = "per-lane state exceeds the register budget"; return false; }
   Gen G(P);
-  G.xmem = mixed && xmem;
+  G.xsplit = mixed && xsplit;
   std::ostringstream& o = G.out;
   o << kPrelude;
   o << "extern \"C\" __global__ void __launch_bounds__(64) kg_ac_lane(const double* __restrict__ prm,\n"
@@ -344,15 +340,19 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem
     if (!G.pass("Z", 0)) { why = G.why; return false; }
     o << "  }\n";
   } else {
-    if (!G.xmem)
+    if (G.xsplit) {
+      o << "  extern __shared__ W xs[];\n  const int lane = threadIdx.x;\n";
+      for (int i = 0; i < n; i++) o << "  W dx" << i << " = W{0.0f, 0.0f};\n";
+    } else {
       for (int i = 0; i < n; i++) o << "  Z x" << i << ";\n";
-    o << "  double est = 0.0, dprev = 0.0, bm2 = 0.0, xm2 = 0.0;\n";
-    // pass 0: complex64 solve of b; x = its FP64 copy
+    }
+    o << "  double est = 0.0, d2prev = 0.0, d2last = 0.0, bm2 = 0.0, xm2 = 0.0;\n";
+    // pass 0: complex64 solve of b; x = its FP64 copy (xsplit: x0 = it, dx = 0)
     // |x|_inf is taken from this first solution (the refinement changes it by
     // the FP32 solve error only); tracking it through the refinement passes
     // makes the compiler keep far more values live
     G.on_result = [&](std::ostringstream& O, int i) {
-      if (G.xmem) O << "    { const Z xi = cz(t" << i << "); o[" << P.col_of[i] << "] = xi; xm2 = fmax(xm2, m2(xi)); }\n";
+      if (G.xsplit) O << "    xs[" << 64 * i << " + lane] = t" << i << "; xm2 = fmax(xm2, m2(cz(t" << i << ")));\n";
       else O << "    x" << i << " = cz(t" << i << "); xm2 = fmax(xm2, m2(x" << i << "));\n";
     };
     G.track_bm = true;
@@ -361,14 +361,16 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem
     o << "  }\n";
     G.track_bm = false;
     // refinement: corrections scaled by |b|_inf stay in FP32 range
-    o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0;\n";
+    o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0;\n  const float sf = (float)s;\n";
     // one refinement pass inside a run-time loop: the pass body is compiled
     // once, so its registers are allocated once (an unrolled sequence of
     // passes lets live ranges of neighbouring passes overlap)
     G.on_result = [&](std::ostringstream& O, int i) {
-      if (G.xmem)
-        O << "    { const Z d = scal(s, cz(t" << i << ")); o[" << P.col_of[i] << "] = xb" << i
-          << " + d; dm2 = fmax(dm2, m2(d)); }\n";
+      // (xsplit: the correction is formed and accumulated in complex64, the
+      // precision dx is kept in; FP64 here costs registers, not accuracy)
+      if (G.xsplit)
+        O << "    { const W d = W{sf * t" << i << ".re, sf * t" << i << ".im}; dx" << i << " = W{dx" << i << ".re + d.re, dx"
+          << i << ".im + d.im}; dm2 = fmax(dm2, m2(cz(d))); }\n";
       else
         O << "    { const Z d = scal(s, cz(t" << i << ")); x" << i << " = x" << i << " + d; dm2 = fmax(dm2, m2(d)); }\n";
     };
@@ -377,12 +379,17 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem
          "  for (int it = 0; it < refine; it++) {\n    double dm2 = 0.0;\n";
     G.lookahead = LOOKAHEAD_REFINE;
     if (!G.pass("W", 1)) { why = G.why; return false; }
-    o << "    const double dm = sqrt(dm2), xm = sqrt(xm2);\n"
-      << "    const double rho = (it > 0 && dprev > 0.0) ? fmin(dm / dprev, 1.0) : 1.0;\n"
-      << "    est = xm > 0.0 ? rho * dm / xm : 0.0;\n"
-      << "    dprev = dm;\n  }\n";
-    if (!G.xmem)
-      for (int i = 0; i < n; i++) o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";
+    // only the squared correction sizes of the last two passes are carried;
+    // the estimate is formed after the loop (square roots and divisions in
+    // the loop body make the compiler keep far more values live)
+    o << "    d2prev = d2last;\n    d2last = dm2;\n  }\n"
+      << "  {\n    const double dm = sqrt(d2last), dp = sqrt(d2prev), xm = sqrt(xm2);\n"
+      << "    const double rho = (refine > 1 && dp > 0.0) ? fmin(dm / dp, 1.0) : 1.0;\n"
+      << "    est = xm > 0.0 ? rho * dm / xm : 0.0;\n  }\n";
+    for (int i = 0; i < n; i++) {
+      if (G.xsplit) o << "  o[" << P.col_of[i] << "] = cz(xs[" << 64 * i << " + lane]) + cz(dx" << i << ");\n";
+      else o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";
+    }
     o << "  if (err != nullptr && est > 0.0)\n"
          "    atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(est));\n";
   }

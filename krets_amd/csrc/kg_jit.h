@@ -23,12 +23,13 @@ struct JitFn {
 
 // Source of the lane-per-system AC sweep kernel `kg_ac_lane` for one pattern
 // (AC kind, shared by every instance); the refinement count of the mixed
-// precision kernel is a launch argument. xmem (mixed precision only) keeps
-// the FP64 solution in the output array between refinement passes instead
-// of registers. Returns false, with the reason in `why`, when the pattern is
+// precision kernel is a launch argument. xsplit (mixed precision only)
+// keeps the solution as x0 + dx: the complex64 first solution x0 in LDS
+// (512 * n bytes per 64-lane block, launch argument) and the complex64 sum
+// of the corrections dx in registers, instead of an FP64 x in registers. Returns false, with the reason in `why`, when the pattern is
 // outside what the generator handles (diodes, patterns whose per-lane state
 // would not fit in registers).
-bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xmem, std::string& src,
+bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xsplit, std::string& src,
                    std::string& why);
 
 // Compiles `src` for the device's architecture and loads kernel `name`;

@@ -337,8 +337,9 @@ class Context:
         return [(int(t[i]), float(m[i])) for i in range(max(k, 0))]
 
     def profile(self, enable=True):
-        """Returns (total_ms, launches, last_kernel) of the analysis kernels
-        timed since the previous call, then resets and sets timing on/off."""
+        """Returns (total_ms, launches, kernel) of the dominant analysis kernel
+        (largest accumulated time) since the previous call, then resets and
+        sets timing on/off."""
         ms, n, name = ctypes.c_double(), ctypes.c_int64(), ctypes.c_char_p()
         lib().kg_profile(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(name))
         return ms.value, n.value, (name.value or b"").decode()

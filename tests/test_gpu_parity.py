@@ -164,7 +164,7 @@ def test_transient_compat_vs_oracle(kg, oracle, path, h, stop):
 
 
 # 1000 + t: persistent LDS-staged variant; 2001: lane per system, run-time
-# specialised; 2002: as 2001 with x kept in memory between refinement passes
+# specialised; 2002: as 2001 with the solution split into x0 (LDS) + dx
 @pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064, 2001, 2002])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batched_ac_rc_ladder(kg, oracle, precision, tps):

@@ -59,7 +59,7 @@ extern "C" void run_host(const double* prm, const double* eb, const double* fr, 
 
 
 def source(c, precision, refine=2, s_rep=2 * np.pi * 1e3):
-    """Generated source; refine < 0 selects the x-in-memory mixed form."""
+    """Generated source; refine < 0 selects the split-solution mixed form."""
     from krets_amd._lib import lib
     buf = ctypes.create_string_buffer(1 << 23)
     el = np.ascontiguousarray(c.elements)
@@ -80,6 +80,9 @@ def host_run(src, params, elements, freqs, n, refine=2):
     try:
         cpp = os.path.join(d, "k.cpp")
         so = os.path.join(d, "k.so")
+        # the LDS array of the split-solution form: one host array (lanes run
+        # one after another and index it by lane)
+        src = src.replace("extern __shared__ W xs[];", "static W xs[64 * 128];")
         open(cpp, "w").write(HOST_SHIM + src + HOST_DRIVER)
         subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", cpp, "-o", so],
                        check=True, capture_output=True)
@@ -105,16 +108,16 @@ def nerr(x, r):
     return np.linalg.norm(x - r) / np.linalg.norm(r)
 
 
-@pytest.mark.parametrize("xmem", [False, True])
+@pytest.mark.parametrize("xsplit", [False, True])
 @pytest.mark.parametrize("precision,refine", [(0, 2), (1, 1), (1, 2), (1, 3)])
-def test_lane_source_on_host_vs_oracle(oracle, precision, refine, xmem):
+def test_lane_source_on_host_vs_oracle(oracle, precision, refine, xsplit):
     import krets_amd as kg
     from krets_amd import workloads as W
-    if xmem and precision == 0:
+    if xsplit and precision == 0:
         pytest.skip("the FP64 kernel keeps no solution between passes")
     b = W.c2_rc_ladder_mc(3, sections=50, seed=21)
     c = kg.Circuit(b.netlists[0])
-    r, src = source(c, precision, -1 if xmem else refine)
+    r, src = source(c, precision, -1 if xsplit else refine)
     assert r > 0, src
     f = W.c2_frequencies(200)[::25]
     x, st, err = host_run(src, b.params, c.elements, f, c.n, refine)
