@@ -24,6 +24,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summaries (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what
+    the counter reads for a device copy of known size); None if not profiled."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        for line in open(path):
+            m = re.match(r"traffic_bytes_per_launch\[(.+)\] = (\S+)", line.strip())
+            if m and m.group(1) == kernel:
+                return float(m.group(2))
+    return None
+
+
 def ac_bytes_per_solve(n, n_elements, nf):
     # compulsory traffic: the complex128 solution vector written per solve,
     # plus the instance's element values read once and amortised over nf
@@ -204,7 +218,7 @@ def main():
         "autotune_ms_by_threads_per_system": {"fp64": tuning, "mixed": tuning_mixed},
         "max_normwise_err_vs_oracle": err_oracle,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname_m,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname_m), "kernel": kname_m,
                      "avg_kernel_ms": 1000.0 * avg_kernel_s, "bytes_per_solve": bps},
         "fp64_kernel": {"kernel": kname_d, "avg_kernel_ms": kms_d / max(nl_d, 1)},
         "cpu_baseline": cpu,

@@ -777,7 +777,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
         std::string src, why, log;
         if (!jit_ac_source(pe.lay, pe.host.data(), mixed, cand == 2002, src, why))
           return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
-        if (jit_load(c->device, src, "kg_ac_lane", A.fn, log) != hipSuccess)
+        if (jit_load(c->device, src, jit_ac_name(mixed, cand == 2002), A.fn, log) != hipSuccess)
           return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
         c->lane_fn[key] = A.fn;
       } else {
@@ -854,7 +854,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if ((r = setup(cand, A))) return r;
   P = A.P;
   HIPCHK(c, hipMemsetAsync(d_err, 0, sizeof(double) * c->n_inst, c->stream));
-  r = timed(c, A.lane ? (mixed ? "kg_ac_lane<complex64,complex128>" : "kg_ac_lane<complex128>")
+  r = timed(c, A.lane ? jit_ac_name(mixed, cand == 2002)
              : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
                             : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
             [&] { return run(A); });

@@ -304,6 +304,10 @@ struct Gen {
 
 }  // namespace
 
+const char* jit_ac_name(bool mixed, bool xsplit) {
+  return !mixed ? "kg_ac_lane_c128" : xsplit ? "kg_ac_lane_c64_split" : "kg_ac_lane_c64";
+}
+
 bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xsplit, std::string& src,
                    std::string& why) {
   View P = view(L, buf);
@@ -323,7 +327,8 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
   G.xsplit = mixed && xsplit;
   std::ostringstream& o = G.out;
   o << kPrelude;
-  o << "extern \"C\" __global__ void __launch_bounds__(64) kg_ac_lane(const double* __restrict__ prm,\n"
+  o << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_ac_name(mixed, xsplit)
+    << "(const double* __restrict__ prm,\n"
        "    const double* __restrict__ ebase, const double* __restrict__ freqs, int nf, i64 nsys,\n"
        "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err, int refine) {\n"
        "  const i64 sys = xcd_block() * 64 + threadIdx.x;\n"

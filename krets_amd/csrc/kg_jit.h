@@ -21,7 +21,7 @@ struct JitFn {
   int scratch = 0;       // private (spill) bytes per lane
 };
 
-// Source of the lane-per-system AC sweep kernel `kg_ac_lane` for one pattern
+// Source of the lane-per-system AC sweep kernel (jit_ac_name) for one pattern
 // (AC kind, shared by every instance); the refinement count of the mixed
 // precision kernel is a launch argument. xsplit (mixed precision only)
 // keeps the solution as x0 + dx: the complex64 first solution x0 in LDS
@@ -31,6 +31,11 @@ struct JitFn {
 // would not fit in registers).
 bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xsplit, std::string& src,
                    std::string& why);
+
+// Kernel symbol of the generated AC kernel: kg_ac_lane_c128 (FP64),
+// kg_ac_lane_c64 (mixed, FP64 solution in registers), kg_ac_lane_c64_split
+// (mixed, split solution).
+const char* jit_ac_name(bool mixed, bool xsplit);
 
 // Compiles `src` for the device's architecture and loads kernel `name`;
 // cached per (device, source).

@@ -52,7 +52,7 @@ extern "C" void run_host(const double* prm, const double* eb, const double* fr, 
     for (unsigned t = 0; t < 64; t++) {
       blockIdx.x = b;
       threadIdx.x = t;
-      kg_ac_lane(prm, eb, fr, nf, nsys, out, st, err, refine);
+      KERNEL(prm, eb, fr, nf, nsys, out, st, err, refine);
     }
 }
 """
@@ -83,7 +83,8 @@ def host_run(src, params, elements, freqs, n, refine=2):
         # the LDS array of the split-solution form: one host array (lanes run
         # one after another and index it by lane)
         src = src.replace("extern __shared__ W xs[];", "static W xs[64 * 128];")
-        open(cpp, "w").write(HOST_SHIM + src + HOST_DRIVER)
+        name = re.search(r"void __launch_bounds__\(64\) (\w+)\(", src).group(1)
+        open(cpp, "w").write(HOST_SHIM + src + HOST_DRIVER.replace("KERNEL", name))
         subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", cpp, "-o", so],
                        check=True, capture_output=True)
         L = ctypes.CDLL(so)
