@@ -113,7 +113,7 @@ int32_t kg_destroy(kg_ctx* ctx);
 const char* kg_last_error(const kg_ctx* ctx);
 
 /* Replaces faer's per-solve symbolic analysis (called at solver/op.rs:48-51,
- * dc.rs:167-170, ac.rs:67-70, transient.rs:183-186) with one host symbolic
+ * dc.rs:81-84, ac.rs:67-70, transient.rs:74-77) with one host symbolic
  * pass per distinct topology: pattern, static pivot order (structural
  * matching plus fill-reducing order), fill, level sets; then uploads the
  * batch. */

@@ -52,6 +52,7 @@ constexpr size_t LDS_SHARED_BUDGET = 64 * 1024;
 struct kg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  std::vector<hipStream_t> aux;  // extra streams (default priority) for concurrent buckets
   std::string err;
   // batch
   int n_inst = 0;
@@ -522,6 +523,110 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   return KG_OK;
 }
 
+// Uneven transient batches: every block of one launch reserves LDS for the
+// largest instance, so a launch over 8..512-section ladders runs one block
+// per CU. Instances are split into size classes by how many of their blocks
+// fit one CU's LDS; each class gets its own launch, LDS footprint and
+// autotuned mapping, on its own stream so that classes overlap.
+int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bool mixed, double s_tr,
+                         int precision, const double* d_xop, const int64_t* d_xoff, double* d_out,
+                         const int64_t* d_off, int32_t* d_st, const int32_t* d_st0, int32_t* d_iters,
+                         double* d_err) {
+  Plan P;
+  int32_t r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P);
+  if (r) return r;
+  const size_t red = 8 * 256;
+  std::map<int, std::vector<int32_t>> cls;  // blocks per CU -> instances
+  std::map<int, Lay> clay;
+  for (int i = 0; i < c->n_inst; i++) {
+    const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(c->group_of[i], (int)AK_TRAN, (int)ORD_NESTED, s_tr)]];
+    if (pe.lay.status != KG_OK) continue;  // structurally singular: status set by the caller
+    Lay l{pe.lay.nnz, pe.lay.n, pe.lay.ne, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
+    int per_cu = (int)std::min<size_t>(8, LDS_MAX / (sys_bytes(l) + red));
+    if (per_cu < 1) return set_err(c, KG_UNEXPECTED, "system too large for on-chip memory");
+    cls[per_cu].push_back(i);
+    Lay& b = clay.emplace(per_cu, l).first->second;
+    b.nnz_max = std::max(b.nnz_max, l.nnz_max);
+    b.n_max = std::max(b.n_max, l.n_max);
+    b.ne_max = std::max(b.ne_max, l.ne_max);
+  }
+  struct Bucket {
+    Lay lay;
+    Launch L;
+    int tps = 0;
+    int32_t* list = nullptr;
+    int32_t n = 0;
+  };
+  std::vector<Bucket> bk;
+  for (auto& kv : cls) {
+    Bucket b;
+    b.lay = clay[kv.first];
+    b.n = (int32_t)kv.second.size();
+    hipError_t e;
+    b.list = (int32_t*)ws_get(c, "tran_bucket" + std::to_string(kv.first), sizeof(int32_t) * kv.second.size(), &e);
+    if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+    HIPCHK(c, hipMemcpyAsync(b.list, kv.second.data(), sizeof(int32_t) * kv.second.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    bk.push_back(b);
+  }
+  std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.lay.nnz_max > b.lay.nnz_max; });
+  BatchDev B0 = batch_dev(c, P);
+  auto launch = [&](const Bucket& b, int tps, hipStream_t st) -> hipError_t {
+    Launch L;
+    if (configure(c, tps, b.lay, b.n, L) != KG_OK) return hipErrorInvalidConfiguration;
+    BatchDev B = B0;
+    B.inst_list = b.list;
+    B.n_list = b.n;
+    return run_tran(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, st, B, 0, na, ta, d_xop, d_xoff, d_out, d_off, d_st,
+                    d_iters, d_err, b.lay);
+  };
+  // mapping per size class, timed on that class alone
+  for (auto& b : bk) {
+    Plan Pb = P;
+    Pb.n_max = b.lay.n_max;
+    Pb.nnz_max = b.lay.nnz_max;
+    Pb.ne_max = b.lay.ne_max;
+    r = autotune(c, tune_key(c, "tran", precision, Pb, b.n), {64, 256}, [&](int t, float* ms) -> int32_t {
+      Launch L;
+      int32_t rr = configure(c, t, b.lay, b.n, L);
+      if (rr) return rr;
+      HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
+      return time_launch(c, [&] { return launch(b, t, c->stream); }, ms);
+    }, &b.tps);
+    if (r) return r;
+  }
+  HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
+  while (c->aux.size() + 1 < bk.size()) {
+    hipStream_t s;
+    HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    c->aux.push_back(s);
+  }
+  return timed(c, mixed ? "k_tran<float,double>" : "k_tran<double,double>", [&]() -> hipError_t {
+    hipEvent_t fork;
+    hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    std::vector<hipEvent_t> done;
+    if ((e = hipEventRecord(fork, c->stream)) != hipSuccess) return e;
+    for (size_t k = 0; k < bk.size(); k++) {
+      hipStream_t st = k == 0 ? c->stream : c->aux[k - 1];
+      if (k > 0 && (e = hipStreamWaitEvent(st, fork, 0)) != hipSuccess) return e;
+      if ((e = launch(bk[k], bk[k].tps, st)) != hipSuccess) return e;
+      if (k > 0) {
+        hipEvent_t d;
+        if ((e = hipEventCreateWithFlags(&d, hipEventDisableTiming)) != hipSuccess) return e;
+        if ((e = hipEventRecord(d, st)) != hipSuccess) return e;
+        done.push_back(d);
+      }
+    }
+    for (hipEvent_t d : done) {
+      if ((e = hipStreamWaitEvent(c->stream, d, 0)) != hipSuccess) return e;
+      hipEventDestroy(d);
+    }
+    hipEventDestroy(fork);
+    return hipSuccess;
+  });
+}
+
 int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem) {
   std::vector<int32_t> st(c->n_inst);
   HIPCHK(c, hipMemcpyAsync(st.data(), d_status, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
@@ -573,6 +678,7 @@ int32_t kg_destroy(kg_ctx* c) {
   free_batch(c);
   for (auto& kv : c->ws)
     if (kv.second.p) hipFree(kv.second.p);
+  for (hipStream_t s : c->aux) hipStreamDestroy(s);
   hipStreamDestroy(c->stream);
   delete c;
   return KG_OK;
@@ -913,6 +1019,11 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
     lay = Lay{PP.nnz_max, PP.n_max, PP.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
     return configure(c, tps, lay, c->n_inst, L);
   };
+  if (!c->shared && cfg->threads_per_system == 0 && c->n_inst >= 64) {
+    r = run_tran_buckets(c, na, ta, mixed, s_tr, precision, d_xop, d_xoff, d_out, d_off, d_st, d_st0, d_iters, d_err);
+    if (r) return r;
+    P = P0;
+  } else {
   int tps = cfg->threads_per_system;
   if (tps == 0) {
     r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), {1, 4, 16, 64, 256}, [&](int t, float* ms) -> int32_t {
@@ -942,6 +1053,7 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
                     d_st, d_iters, d_err, lay);
   });
   if (r) return r;
+  }
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if (iters && (r = copy_out(c, iters, d_iters, sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), mem))) return r;
   if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;

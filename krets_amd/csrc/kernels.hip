@@ -740,9 +740,10 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = (int)(xcd_block() * spb + ls);
-  const bool valid = inst < B.n_instances;
-  const int ii = valid ? inst : 0;
+  const int slot = (int)(xcd_block() * spb + ls);
+  const bool valid = slot < (B.inst_list ? B.n_list : B.n_instances);
+  const int inst = valid ? (B.inst_list ? B.inst_list[slot] : slot) : 0;
+  const int ii = inst;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
   size_t sb = sys_bytes(lay);
   double* red = (double*)(smem + spb * sb);

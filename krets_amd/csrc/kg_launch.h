@@ -23,6 +23,10 @@ struct BatchDev {
   const int64_t* eoff;     // instance -> first element record
   const int32_t* n_inst;   // instance -> unknown count
   int32_t n_instances;
+  // optional subset of instances for this launch (size-class buckets of an
+  // uneven batch): slot k of the grid runs instance inst_list[k]
+  const int32_t* inst_list = nullptr;
+  int32_t n_list = 0;
 };
 
 // LDS carve-up of one system. The same layout is computed on the host

@@ -1,7 +1,7 @@
 // Host symbolic pass: run once per distinct circuit topology and analysis
 // kind. It replaces the per-solve symbolic work that krets asks of faer
 // (SparseColMat::try_new_from_triplets + sp_lu at solver/op.rs:48-51,
-// dc.rs:167-170, ac.rs:67-70, transient.rs:183-186) with a fixed plan:
+// dc.rs:81-84, ac.rs:67-70, transient.rs:74-77) with a fixed plan:
 //
 //  1. the stamp pattern of krets' element stamps (stampable.rs:202-1050) for
 //     the analysis kind, with every entry tagged by the element contribution

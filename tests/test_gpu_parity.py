@@ -249,6 +249,31 @@ def test_batched_transient_uneven_rlc(kg, oracle, method, precision, tps):
         assert e < 1e-9, (i, e)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_bucketed_transient_uneven_rlc(kg, oracle, precision):
+    """Autotuned uneven batch of 96 ladders, 8..300 sections: instances run in
+    size-class launches on concurrent streams; a sample of each class vs the
+    oracle, and the whole batch vs the single-launch mapping."""
+    from krets_amd import workloads as W
+    say(f"bucketed transient precision={precision}")
+    b = W.c3_rlc_uneven(96, smin=8, smax=300, seed=17)
+    ctx = kg.Context()
+    b.load(ctx)
+    h, n = 10e-12, 120
+    xs, st, (it, err) = ctx.transient(h, n, method=kg.TRAP, precision=precision)
+    assert (st == 0).all()
+    ys, st2, _ = ctx.transient(h, n, method=kg.TRAP, precision=precision,
+                               config=kg.SolverConfig(threads_per_system=256))
+    assert (st2 == 0).all()
+    for i in range(96):
+        assert max(nerr(xs[i][k], ys[i][k]) for k in range(1, n + 1)) < 1e-12, i
+    order = np.argsort(b.n_unknowns)
+    for i in (order[0], order[40], order[-1]):
+        xr, _ = oracle.Circuit(b.netlists[i]).tran(h, n * h, method=oracle.TRAP)
+        e = max(nerr(xs[i][k], xr[k]) for k in range(1, n + 1))
+        assert e < 1e-9, (i, e)
+
+
 def test_error_statuses(kg):
     say("error statuses")
     s = kg.Solver(kg.Circuit("I1 1 0 1\nR1 1 0 1\n"))
