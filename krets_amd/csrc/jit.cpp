@@ -85,8 +85,14 @@ __device__ __forceinline__ Z rcpF(Z b) {
   const double q = 1.0 / (re * re + im * im);
   return {__builtin_amdgcn_ldexp(re * q, -k), __builtin_amdgcn_ldexp(-im * q, -k)};
 }
+// complex64 pivots: |b|^2 in FP64 (cannot leave its range), reciprocal from
+// the hardware estimate refined by two Newton steps (error far below the
+// FP32 rounding that follows)
 __device__ __forceinline__ W rcpF(W b) {
-  const double re = b.re, im = b.im, q = 1.0 / (re * re + im * im);
+  const double re = b.re, im = b.im, m = re * re + im * im;
+  double q = __builtin_amdgcn_rcp(m);
+  q = q * (2.0 - m * q);
+  q = q * (2.0 - m * q);
   return {(float)(re * q), (float)(-im * q)};
 }
 // block order that keeps consecutive systems (one instance) on one XCD
@@ -263,7 +269,9 @@ struct Gen {
           ynz = true;
         }
       }
-      // pivot: decomposition check as the generic kernels (|piv| > 0, finite)
+      // pivot: decomposition check as the generic kernels (|piv| > 0, finite);
+      // refinement passes repeat the first pass' factorisation exactly, so
+      // only the first pass checks
       int d = P.diag[i];
       o << "    { const Z pz = cz(v" << d << "); bad |= !(m2(pz) > 0.0) || !fin(pz); }\n";
       o << "    const " << F << " rd" << i << " = rcpF(v" << d << ");\n";

@@ -37,6 +37,7 @@ static inline int kg_frexp_exp(double a) { int e = 0; std::frexp(a, &e); return 
 #define __builtin_amdgcn_frexp_exp(a) kg_frexp_exp(a)
 #define __builtin_amdgcn_ldexp(a, e) std::ldexp((a), (e))
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
+#define __builtin_amdgcn_rcp(a) (1.0 / (a))
 static inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
 static inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
   unsigned long long o = *p; if (v > o) *p = v; return o; }
