@@ -97,7 +97,9 @@ typedef struct {
    * staged in LDS by a persistent grid), 2001 (one lane per system, code
    * specialised to the topology at run time) and, for KG_MIXED, 2002 (as
    * 2001 with the solution split into a complex64 first solution in LDS and
-   * a complex64 sum of corrections in registers). */
+   * a complex64 sum of corrections in registers). Fixed-step transients also
+   * take 1064 and 1256 (64 or 256 threads per system with the plan copied
+   * into LDS once per launch). */
   int32_t threads_per_system;
   int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
 } kg_solver_config;

@@ -198,6 +198,7 @@ PatDev to_dev(const PatLayout& L, const int32_t* b) {
 struct Plan {
   int32_t* d_inst_pat = nullptr;
   int nnz_max = 0, n_max = 0, ne_max = 0;
+  int total_max = 0;  // longest plan buffer (int32) among the instances' patterns
   bool any_diode = false, any_nonlinear = false;
   std::vector<int32_t> group_status;  // per group
 };
@@ -243,6 +244,7 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
       P.nnz_max = std::max(P.nnz_max, pe.lay.nnz);
       P.n_max = std::max(P.n_max, pe.lay.n);
       P.ne_max = std::max(P.ne_max, pe.lay.ne);
+      P.total_max = std::max(P.total_max, pe.lay.total);
       P.any_diode |= pe.lay.has_diode != 0;
       P.any_nonlinear |= pe.lay.nonlinear != 0;
     } else {
@@ -300,17 +302,18 @@ int default_tps(const kg_ctx* c, int n_max) {
 
 Ordering order_for(int tps) { return tps == 1 ? ORD_MINDEG : ORD_NESTED; }
 
-int32_t configure(kg_ctx* c, int tps, Lay lay, int64_t nsys, Launch& L) {
+// extra: bytes at the start of the block's LDS (a staged plan), once per block.
+int32_t configure(kg_ctx* c, int tps, Lay lay, int64_t nsys, Launch& L, size_t extra = 0) {
   size_t sb = sys_bytes(lay);
   L.tps = tps;
   // one wave per block: small batches spread over more CUs, barriers stay cheap
   int spb = c->shared ? std::max(1, 64 / tps) : 1;
-  while (spb > 1 && spb * sb + 8 * (size_t)spb * tps > LDS_SHARED_BUDGET) spb /= 2;
-  if (spb * sb + 8 * (size_t)spb * tps > LDS_MAX)
+  while (spb > 1 && extra + spb * sb + 8 * (size_t)spb * tps > LDS_SHARED_BUDGET) spb /= 2;
+  if (extra + spb * sb + 8 * (size_t)spb * tps > LDS_MAX)
     return set_err(c, KG_UNEXPECTED,
                    "system too large for on-chip memory (" + std::to_string(sb) + " bytes per system)");
   L.spb = spb;
-  L.lds = spb * sb + 8 * (size_t)spb * tps;
+  L.lds = extra + spb * sb + 8 * (size_t)spb * tps;
   L.block = dim3(tps * spb);
   L.grid = dim3((unsigned)((nsys + spb - 1) / spb));
   return KG_OK;
@@ -538,6 +541,7 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   const size_t red = 8 * 256;
   std::map<int, std::vector<int32_t>> cls;  // blocks per CU -> instances
   std::map<int, Lay> clay;
+  std::map<int, int> cplan;  // longest plan per class
   for (int i = 0; i < c->n_inst; i++) {
     const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(c->group_of[i], (int)AK_TRAN, (int)ORD_NESTED, s_tr)]];
     if (pe.lay.status != KG_OK) continue;  // structurally singular: status set by the caller
@@ -549,19 +553,22 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     b.nnz_max = std::max(b.nnz_max, l.nnz_max);
     b.n_max = std::max(b.n_max, l.n_max);
     b.ne_max = std::max(b.ne_max, l.ne_max);
+    cplan[per_cu] = std::max(cplan[per_cu], pe.lay.total);
   }
   struct Bucket {
     Lay lay;
     Launch L;
-    int tps = 0;
+    int tps = 0;  // 1000 + t: plan staged in LDS
     int32_t* list = nullptr;
     int32_t n = 0;
+    int plan_ints = 0;
   };
   std::vector<Bucket> bk;
   for (auto& kv : cls) {
     Bucket b;
     b.lay = clay[kv.first];
     b.n = (int32_t)kv.second.size();
+    b.plan_ints = cplan[kv.first];
     hipError_t e;
     b.list = (int32_t*)ws_get(c, "tran_bucket" + std::to_string(kv.first), sizeof(int32_t) * kv.second.size(), &e);
     if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
@@ -571,14 +578,16 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   }
   std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.lay.nnz_max > b.lay.nnz_max; });
   BatchDev B0 = batch_dev(c, P);
-  auto launch = [&](const Bucket& b, int tps, hipStream_t st) -> hipError_t {
+  auto launch = [&](const Bucket& b, int cand, hipStream_t st) -> hipError_t {
     Launch L;
-    if (configure(c, tps, b.lay, b.n, L) != KG_OK) return hipErrorInvalidConfiguration;
+    int tps = cand % 1000, pi = cand >= 1000 ? b.plan_ints : 0;
+    if (configure(c, tps, b.lay, b.n, L, pi ? lay_al(4 * (size_t)pi) : 0) != KG_OK)
+      return hipErrorInvalidConfiguration;
     BatchDev B = B0;
     B.inst_list = b.list;
     B.n_list = b.n;
     return run_tran(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, st, B, 0, na, ta, d_xop, d_xoff, d_out, d_off, d_st,
-                    d_iters, d_err, b.lay);
+                    d_iters, d_err, b.lay, pi);
   };
   // mapping per size class, timed on that class alone
   for (auto& b : bk) {
@@ -586,9 +595,9 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     Pb.n_max = b.lay.n_max;
     Pb.nnz_max = b.lay.nnz_max;
     Pb.ne_max = b.lay.ne_max;
-    r = autotune(c, tune_key(c, "tran", precision, Pb, b.n), {64, 256}, [&](int t, float* ms) -> int32_t {
+    r = autotune(c, tune_key(c, "tran", precision, Pb, b.n), {64, 256, 1064, 1256}, [&](int t, float* ms) -> int32_t {
       Launch L;
-      int32_t rr = configure(c, t, b.lay, b.n, L);
+      int32_t rr = configure(c, t % 1000, b.lay, b.n, L, t >= 1000 ? lay_al(4 * (size_t)b.plan_ints) : 0);
       if (rr) return rr;
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
       return time_launch(c, [&] { return launch(b, t, c->stream); }, ms);
@@ -1013,44 +1022,52 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   TranArgs ta{h, method == KG_TRAP ? 2.0 : 1.0, nsteps, method, cfg->refine_iters > 0 ? cfg->refine_iters : 2};
   NewtonArgs na = newton_args(cfg);
   na.compat = method == KG_BE_COMPAT ? 1 : 0;
-  auto setup = [&](int tps, Plan& PP, Lay& lay, Launch& L) -> int32_t {
+  // cand: threads per system, plus 1000 for the plan staged in LDS (64, 256)
+  auto setup = [&](int cand, Plan& PP, Lay& lay, Launch& L, int& pi) -> int32_t {
+    int tps = cand % 1000;
     int32_t rr = prepare(c, AK_TRAN, order_for(tps), s_tr, PP);
     if (rr) return rr;
     lay = Lay{PP.nnz_max, PP.n_max, PP.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
-    return configure(c, tps, lay, c->n_inst, L);
+    pi = cand >= 1000 ? PP.total_max : 0;
+    if (pi && tps != 64 && tps != 256) return set_err(c, KG_INVALID_ARGUMENT, "staged plans need 64 or 256 threads");
+    return configure(c, tps, lay, c->n_inst, L, pi ? lay_al(4 * (size_t)pi) : 0);
   };
   if (!c->shared && cfg->threads_per_system == 0 && c->n_inst >= 64) {
     r = run_tran_buckets(c, na, ta, mixed, s_tr, precision, d_xop, d_xoff, d_out, d_off, d_st, d_st0, d_iters, d_err);
     if (r) return r;
     P = P0;
   } else {
-  int tps = cfg->threads_per_system;
-  if (tps == 0) {
-    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), {1, 4, 16, 64, 256}, [&](int t, float* ms) -> int32_t {
+  int cand = cfg->threads_per_system;
+  if (cand == 0) {
+    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), {1, 4, 16, 64, 256, 1064, 1256},
+                 [&](int t, float* ms) -> int32_t {
       Plan PP;
       Lay lay;
       Launch L;
-      int32_t rr = setup(t, PP, lay, L);
+      int pi = 0;
+      int32_t rr = setup(t, PP, lay, L, pi);
       if (rr) return rr;
-      if (t * L.spb < 64 && t < 64) return set_err(c, KG_UNEXPECTED, "sub-wave mapping");
+      if ((t % 1000) * L.spb < 64 && (t % 1000) < 64) return set_err(c, KG_UNEXPECTED, "sub-wave mapping");
       BatchDev B = batch_dev(c, PP);
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
       return time_launch(c, [&] {
-        return run_tran(t, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out,
-                        d_off, d_st, d_iters, d_err, lay);
+        return run_tran(t % 1000, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff,
+                        d_out, d_off, d_st, d_iters, d_err, lay, pi);
       }, ms);
-    }, &tps);
+    }, &cand);
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
   }
-  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  if (!valid_tps(cand % 1000) || (cand >= 1000 && cand != 1064 && cand != 1256))
+    return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64, 256, 1064 or 1256");
   Lay lay;
   Launch L;
-  if ((r = setup(tps, P, lay, L))) return r;
+  int pi = 0;
+  if ((r = setup(cand, P, lay, L, pi))) return r;
   BatchDev B = batch_dev(c, P);
   r = timed(c, mixed ? "k_tran<float,double>" : "k_tran<double,double>", [&] {
-    return run_tran(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out, d_off,
-                    d_st, d_iters, d_err, lay);
+    return run_tran(cand % 1000, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out,
+                    d_off, d_st, d_iters, d_err, lay, pi);
   });
   if (r) return r;
   }

@@ -730,24 +730,63 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
   }
 }
 
+// The plan with every pointer moved into a copy of its buffer at `lds`.
+__device__ PatDev rebase(const PatDev& G, const int32_t* lds) {
+  PatDev P = G;
+  auto rb = [&](const int32_t* p) { return p ? lds + (p - G.base) : nullptr; };
+  P.row_ptr = rb(G.row_ptr); P.col = rb(G.col); P.diag = rb(G.diag);
+  P.fprog_ptr = rb(G.fprog_ptr); P.fprog = rb(G.fprog);
+  P.lvl_ptr = rb(G.lvl_ptr); P.lvl_rows = rb(G.lvl_rows); P.blvl_ptr = rb(G.blvl_ptr); P.blvl_rows = rb(G.blvl_rows);
+  P.scon_ptr = rb(G.scon_ptr); P.scon = rb(G.scon); P.rcon_ptr = rb(G.rcon_ptr); P.rcon = rb(G.rcon);
+  P.col_of = rb(G.col_of); P.cpos = rb(G.cpos); P.isbr = rb(G.isbr);
+  P.etype = rb(G.etype); P.ep = rb(G.ep); P.em = rb(G.em); P.ec = rb(G.ec);
+  P.ut_ptr = rb(G.ut_ptr); P.ut = rb(G.ut); P.lt_ptr = rb(G.lt_ptr); P.lt = rb(G.lt);
+  P.tlvl_ptr = rb(G.tlvl_ptr); P.tlvl_rows = rb(G.tlvl_rows); P.tblvl_ptr = rb(G.tblvl_ptr);
+  P.tblvl_rows = rb(G.tblvl_rows);
+  P.row_of = rb(G.row_of); P.rpos = rb(G.rpos); P.erp = rb(G.erp); P.erm = rb(G.erm); P.erc = rb(G.erc);
+  P.base = lds;
+  return P;
+}
+
 // Fixed-step transient (solver/transient.rs:14-114 and trapezoidal), one
 // persistent launch: every instance runs all steps in-kernel. Linear
 // circuits factor once. xop: OP per instance. out: [instance][nsteps+1][n].
-template <class F, class A, int TPS>
+// plan_ints > 0: the block's plan (all systems of a block share one) is
+// copied into the first lay_al(4 * plan_ints) bytes of LDS once, so the
+// thousands of steps read their row structure and level lists from LDS
+// instead of issuing dependent L2 loads every step.
+template <class F, class A, int TPS, bool STAGED>
 __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, const double* xop,
                        const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status,
-                       int32_t* iters, double* err, Lay lay) {
+                       int32_t* iters, double* err, Lay lay, int plan_ints) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int slot = (int)(xcd_block() * spb + ls);
-  const bool valid = slot < (B.inst_list ? B.n_list : B.n_instances);
+  const int64_t slot0 = xcd_block() * spb;
+  const int slot = (int)(slot0 + ls);
+  const int nslot = B.inst_list ? B.n_list : B.n_instances;
+  const bool valid = slot < nslot;
   const int inst = valid ? (B.inst_list ? B.inst_list[slot] : slot) : 0;
   const int ii = inst;
-  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  char* sys_lds = smem;
+  const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  PatDev Ps;
+  if constexpr (STAGED) {
+    const int first = slot0 < nslot ? (B.inst_list ? B.inst_list[slot0] : (int)slot0) : 0;
+    const PatDev& G = B.pats[B.inst_pat[kind_slot * B.n_instances + first]];
+    int32_t* pl = (int32_t*)smem;
+    for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
+    __syncthreads();
+    Ps = rebase(G, pl);
+    sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
+  }
+  const PatDev& P = [&]() -> const PatDev& {
+    if constexpr (STAGED) return Ps;
+    else return Pg;
+  }();
   size_t sb = sys_bytes(lay);
-  double* red = (double*)(smem + spb * sb);
-  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  double* red = (double*)(sys_lds + spb * sb);
+  SysMem<F, A> M = carve<F, A>(sys_lds + ls * sb, lay);
   const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
   const double* eb = B.ebase + B.eoff[ii];
   const bool compat = ta.method == KG_BE_COMPAT;
@@ -1080,22 +1119,6 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
 
 
 // Re-points every array of a plan into a copy of its buffer at `lds`.
-__device__ PatDev rebase(const PatDev& G, const int32_t* lds) {
-  PatDev P = G;
-  auto rb = [&](const int32_t* p) { return p ? lds + (p - G.base) : nullptr; };
-  P.row_ptr = rb(G.row_ptr); P.col = rb(G.col); P.diag = rb(G.diag);
-  P.fprog_ptr = rb(G.fprog_ptr); P.fprog = rb(G.fprog);
-  P.lvl_ptr = rb(G.lvl_ptr); P.lvl_rows = rb(G.lvl_rows); P.blvl_ptr = rb(G.blvl_ptr); P.blvl_rows = rb(G.blvl_rows);
-  P.scon_ptr = rb(G.scon_ptr); P.scon = rb(G.scon); P.rcon_ptr = rb(G.rcon_ptr); P.rcon = rb(G.rcon);
-  P.col_of = rb(G.col_of); P.cpos = rb(G.cpos); P.isbr = rb(G.isbr);
-  P.etype = rb(G.etype); P.ep = rb(G.ep); P.em = rb(G.em); P.ec = rb(G.ec);
-  P.ut_ptr = rb(G.ut_ptr); P.ut = rb(G.ut); P.lt_ptr = rb(G.lt_ptr); P.lt = rb(G.lt);
-  P.tlvl_ptr = rb(G.tlvl_ptr); P.tlvl_rows = rb(G.tlvl_rows); P.tblvl_ptr = rb(G.tblvl_ptr);
-  P.tblvl_rows = rb(G.tblvl_rows);
-  P.row_of = rb(G.row_of); P.rpos = rb(G.rpos); P.erp = rb(G.erp); P.erm = rb(G.erm); P.erc = rb(G.erc);
-  P.base = lds;
-  return P;
-}
 
 // Persistent AC sweep for batches with one shared topology: a grid of a few
 // blocks per CU, each walking a contiguous chunk of work items. A work item is
@@ -1621,13 +1644,28 @@ hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hip
 
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
-                    const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay) {
-  if (mixed) {
-    KG_TPS_SWITCH(tps, launch(k_tran<float, double, T>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
-                              out_off, status, iters, err, lay))
+                    const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay, int plan_ints) {
+  if (plan_ints > 0) {  // staged plan: level-scheduled mappings only
+    if (tps != 64 && tps != 256) return hipErrorInvalidValue;
+    if (mixed) {
+      if (tps == 64)
+        return launch(k_tran<float, double, 64, true>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out, out_off,
+                      status, iters, err, lay, plan_ints);
+      return launch(k_tran<float, double, 256, true>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out, out_off,
+                    status, iters, err, lay, plan_ints);
+    }
+    if (tps == 64)
+      return launch(k_tran<double, double, 64, true>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out, out_off,
+                    status, iters, err, lay, plan_ints);
+    return launch(k_tran<double, double, 256, true>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out, out_off,
+                  status, iters, err, lay, plan_ints);
   }
-  KG_TPS_SWITCH(tps, launch(k_tran<double, double, T>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
-                            out_off, status, iters, err, lay))
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_tran<float, double, T, false>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
+                              out_off, status, iters, err, lay, 0))
+  }
+  KG_TPS_SWITCH(tps, launch(k_tran<double, double, T, false>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
+                            out_off, status, iters, err, lay, 0))
 }
 
 }  // namespace kg

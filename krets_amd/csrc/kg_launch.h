@@ -98,6 +98,7 @@ hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hip
                              int plan_ints, int64_t chunk);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
-                    const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay);
+                    const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay,
+                    int plan_ints = 0);
 
 }  // namespace kg

@@ -230,7 +230,7 @@ def test_lane_ac_golden_and_rlc(kg, oracle, precision, lane, refine):
         assert e < (1e-12 if precision == 0 else (1e-9 if refine >= 2 else 1e-6)), (i, e)
 
 
-@pytest.mark.parametrize("tps", [1, 16, 64, 256])
+@pytest.mark.parametrize("tps", [1, 16, 64, 256, 1064, 1256])  # 1000 + t: plan staged in LDS
 @pytest.mark.parametrize("method,precision", [(2, 0), (2, 1), (1, 0), (0, 0)])
 def test_batched_transient_uneven_rlc(kg, oracle, method, precision, tps):
     from krets_amd import workloads as W
