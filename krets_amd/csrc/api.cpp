@@ -1173,7 +1173,13 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 2, 1, 8, 8, 0, 0};
   Launch L;
-  if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
+  // stage the plan in LDS when it fits (the whole step loop then reads it there)
+  int pi = P.total_max;
+  if (configure(c, tps, lay, c->n_inst, L, lay_al(4 * (size_t)pi)) != KG_OK) {
+    pi = 0;
+    if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
+  }
+  c->err.clear();
   auto off = out_offsets(c, max_points);
   int64_t* d_off;
   if ((r = upload_offsets(c, "lte_off", off, &d_off))) return r;
@@ -1193,7 +1199,7 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   BatchDev B = batch_dev(c, P);
   r = timed(c, "k_tran_lte", [&] {
     return run_lte(tps, L.grid, L.block, L.lds, c->stream, B, 0, na, la, d_xop, d_xoff, d_t, d_x, d_off, d_np, d_nt,
-                   d_rj, d_st, lay);
+                   d_rj, d_st, lay, pi);
   });
   if (r) return r;
   if ((r = copy_out(c, t, d_t, sizeof(double) * ni * max_points, mem))) return r;
@@ -1231,7 +1237,11 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
   bool mixed = precision == KG_MIXED;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 5, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
   Launch L;
-  if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
+  // stage the plan in LDS for the whole step loop when it fits
+  int pi = (tps == 64 || tps == 256) ? P.total_max : 0;
+  if (pi && configure(c, tps, lay, c->n_inst, L, lay_al(4 * (size_t)pi)) != KG_OK) pi = 0;
+  if (!pi && (r = configure(c, tps, lay, c->n_inst, L))) return r;
+  c->err.clear();
   int n = c->nunk[0];
   int ncp = nsteps / K + 1, cpw = P.n_max + P.ne_max;
   hipError_t e;
@@ -1253,7 +1263,7 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
   BatchDev B = batch_dev(c, P);
   r = timed(c, mixed ? "k_adj_fwd<float,double>" : "k_adj_fwd<double,double>", [&] {
     return run_adj_fwd(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, aa, d_c, d_xop, d_xoff, d_ck,
-                       (int64_t)ncp * cpw, cpw, d_J, d_st, lay);
+                       (int64_t)ncp * cpw, cpw, d_J, d_st, lay, pi);
   });
   if (r) return r;
   if (J && (r = copy_out(c, J, d_J, sizeof(double) * ni, mem))) return r;
@@ -1279,7 +1289,10 @@ int32_t kg_adjoint_backward(kg_ctx* c, double* grad, int32_t* status, int32_t me
   if ((r = prepare(c, AK_TRAN, order_for(c->adj.tps), s_tr, P))) return r;
   Lay lay = c->adj.lay;
   Launch L;
-  if ((r = configure(c, c->adj.tps, lay, c->n_inst, L))) return r;
+  int pi = (c->adj.tps == 64 || c->adj.tps == 256) ? P.total_max : 0;
+  if (pi && configure(c, c->adj.tps, lay, c->n_inst, L, lay_al(4 * (size_t)pi)) != KG_OK) pi = 0;
+  if (!pi && (r = configure(c, c->adj.tps, lay, c->n_inst, L))) return r;
+  c->err.clear();
   hipError_t e;
   size_t ni = c->n_inst;
   int ne = (int)(c->eoff[1] - c->eoff[0]);
@@ -1298,7 +1311,7 @@ int32_t kg_adjoint_backward(kg_ctx* c, double* grad, int32_t* status, int32_t me
   r = timed(c, c->adj.mixed ? "k_adj_bwd<float,double>" : "k_adj_bwd<double,double>", [&] {
     return run_adj_bwd(c->adj.tps, c->adj.mixed, L.grid, L.block, L.lds, c->stream, B, 0, aa, d_c, d_ck,
                        (int64_t)c->adj.ncp * c->adj.cpw, c->adj.cpw, d_seg, seg_stride, d_g, ne, d_xb, d_xoff, d_st,
-                       lay);
+                       lay, pi);
   });
   if (r) return r;
   // operating-point part (DC plan, FP64)

@@ -934,20 +934,39 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
 // grow it by up to 2x; a failed Newton solve divides h by 8.
 // xop: clean operating points; tout: [instance][max_points];
 // xout: krets numbering at xout_off[instance] + point * n.
-template <int TPS>
+// STAGED: every system of a block shares one plan (shared topology, or one
+// system per block); it is copied into the first lay_al(4 * plan_ints) bytes
+// of LDS once.
+template <int TPS, bool STAGED>
 __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la, const double* xop,
                            const int64_t* xop_off, double* tout, double* xout, const int64_t* xout_off, int32_t* npts,
-                           int32_t* newton_total, int32_t* rejected, int32_t* status, Lay lay) {
+                           int32_t* newton_total, int32_t* rejected, int32_t* status, Lay lay, int plan_ints) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = (int)(xcd_block() * spb + ls);
+  const int64_t slot0 = xcd_block() * spb;
+  const int inst = (int)(slot0 + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
-  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  char* sys_lds = smem;
+  const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  PatDev Ps;
+  if constexpr (STAGED) {
+    const int first = slot0 < B.n_instances ? (int)slot0 : 0;
+    const PatDev& G = B.pats[B.inst_pat[kind_slot * B.n_instances + first]];
+    int32_t* pl = (int32_t*)smem;
+    for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
+    __syncthreads();
+    Ps = rebase(G, pl);
+    sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
+  }
+  const PatDev& P = [&]() -> const PatDev& {
+    if constexpr (STAGED) return Ps;
+    else return Pg;
+  }();
   size_t sb = sys_bytes(lay);
-  double* red = (double*)(smem + spb * sb);
-  SysMem<double, double> M = carve<double, double>(smem + ls * sb, lay);
+  double* red = (double*)(sys_lds + spb * sb);
+  SysMem<double, double> M = carve<double, double>(sys_lds + ls * sb, lay);
   double* xc = M.xp;
   double* xpr = M.xp + M.xstride;
   const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
@@ -1223,20 +1242,34 @@ __device__ double sys_sum(double v, double* red, int ls, int t) {
 // J = sum_s w_s c.x_s (trapezoid weights) and a checkpoint of the state
 // (solution and companion states) every K steps.
 // ckpt: [instance][ncp][cpw] with cpw >= n + ne; cvec in krets numbering.
-template <class F, class A, int TPS>
+template <class F, class A, int TPS, bool STAGED>
 __global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* xop,
                           const int64_t* xop_off, double* ckpt, int64_t ck_stride, int cpw, double* Jout,
-                          int32_t* status, Lay lay) {
+                          int32_t* status, Lay lay, int plan_ints) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
   const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
-  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  char* sys_lds = smem;
+  const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  PatDev Ps;
+  if constexpr (STAGED) {  // shared topology: one plan for the whole grid
+    const PatDev& G = B.pats[B.inst_pat[kind_slot * B.n_instances]];
+    int32_t* pl = (int32_t*)smem;
+    for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
+    __syncthreads();
+    Ps = rebase(G, pl);
+    sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
+  }
+  const PatDev& P = [&]() -> const PatDev& {
+    if constexpr (STAGED) return Ps;
+    else return Pg;
+  }();
   size_t sb = sys_bytes(lay);
-  double* red = (double*)(smem + spb * sb);
-  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  double* red = (double*)(sys_lds + spb * sb);
+  SysMem<F, A> M = carve<F, A>(sys_lds + ls * sb, lay);
   const double* prm = B.params + B.eoff[ii] * KG_NPARAM;
   const double* eb = B.ebase + B.eoff[ii];
   const double* x0 = xop + xop_off[ii];
@@ -1307,20 +1340,35 @@ __global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
 // lambda = A^-T xbar with the stored LU (transposed solves), gradient terms of
 // every R, C and L, and the adjoint states of the trapezoidal companions.
 // xbar0 receives dJ/dx_0 (krets numbering) for the operating-point adjoint.
-template <class F, class A, int TPS>
+template <class F, class A, int TPS, bool STAGED>
 __global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* ckpt,
                           int64_t ck_stride, int cpw, double* seg, int64_t seg_stride, double* grad,
-                          int64_t grad_stride, double* xbar0, const int64_t* xop_off, int32_t* status, Lay lay) {
+                          int64_t grad_stride, double* xbar0, const int64_t* xop_off, int32_t* status, Lay lay,
+                          int plan_ints) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
   const int inst = (int)(xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
-  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  char* sys_lds = smem;
+  const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
+  PatDev Ps;
+  if constexpr (STAGED) {  // shared topology: one plan for the whole grid
+    const PatDev& G = B.pats[B.inst_pat[kind_slot * B.n_instances]];
+    int32_t* pl = (int32_t*)smem;
+    for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
+    __syncthreads();
+    Ps = rebase(G, pl);
+    sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
+  }
+  const PatDev& P = [&]() -> const PatDev& {
+    if constexpr (STAGED) return Ps;
+    else return Pg;
+  }();
   size_t sb = sys_bytes(lay);
-  double* red = (double*)(smem + spb * sb);
-  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  double* red = (double*)(sys_lds + spb * sb);
+  SysMem<F, A> M = carve<F, A>(sys_lds + ls * sb, lay);
   double* xbar = M.xp + M.xstride;
   double* xb0 = M.xp + 2 * M.xstride;
   double* lam = M.xp + 3 * M.xstride;
@@ -1595,32 +1643,62 @@ hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st
 hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                    LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
-                   Lay lay) {
-  KG_TPS_SWITCH(tps, launch(k_tran_lte<T>, g, b, lds, st, B, kind_slot, na, la, xop, xop_off, tout, xout, xout_off,
-                            npts, newton_total, rejected, status, lay))
+                   Lay lay, int plan_ints) {
+  if (plan_ints > 0) {
+    KG_TPS_SWITCH(tps, launch(k_tran_lte<T, true>, g, b, lds, st, B, kind_slot, na, la, xop, xop_off, tout, xout,
+                              xout_off, npts, newton_total, rejected, status, lay, plan_ints))
+  }
+  KG_TPS_SWITCH(tps, launch(k_tran_lte<T, false>, g, b, lds, st, B, kind_slot, na, la, xop, xop_off, tout, xout,
+                            xout_off, npts, newton_total, rejected, status, lay, 0))
 }
+
+// plan_ints > 0 (64 or 256 threads per system): the plan is staged in LDS.
+#define KG_STAGED_SWITCH(tps, CALL)                         \
+  switch (tps) {                                            \
+    case 64: { constexpr int T = 64; return CALL; }         \
+    case 256: { constexpr int T = 256; return CALL; }       \
+    default: return hipErrorInvalidValue;                   \
+  }
 
 hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
-                       int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay) {
-  if (mixed) {
-    KG_TPS_SWITCH(tps, launch(k_adj_fwd<float, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, xop, xop_off, ckpt,
-                              ck_stride, cpw, J, status, lay))
+                       int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay, int plan_ints) {
+  if (plan_ints > 0) {
+    if (mixed) {
+      KG_STAGED_SWITCH(tps, launch(k_adj_fwd<float, double, T, true>, g, b, lds, st, B, kind_slot, aa, cvec, xop,
+                                   xop_off, ckpt, ck_stride, cpw, J, status, lay, plan_ints))
+    }
+    KG_STAGED_SWITCH(tps, launch(k_adj_fwd<double, double, T, true>, g, b, lds, st, B, kind_slot, aa, cvec, xop,
+                                 xop_off, ckpt, ck_stride, cpw, J, status, lay, plan_ints))
   }
-  KG_TPS_SWITCH(tps, launch(k_adj_fwd<double, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, xop, xop_off, ckpt,
-                            ck_stride, cpw, J, status, lay))
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_adj_fwd<float, double, T, false>, g, b, lds, st, B, kind_slot, aa, cvec, xop, xop_off,
+                              ckpt, ck_stride, cpw, J, status, lay, 0))
+  }
+  KG_TPS_SWITCH(tps, launch(k_adj_fwd<double, double, T, false>, g, b, lds, st, B, kind_slot, aa, cvec, xop, xop_off,
+                            ckpt, ck_stride, cpw, J, status, lay, 0))
 }
 
 hipError_t run_adj_bwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
                        int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
-                       const int64_t* xop_off, int32_t* status, Lay lay) {
-  if (mixed) {
-    KG_TPS_SWITCH(tps, launch(k_adj_bwd<float, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt, ck_stride, cpw,
-                              seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay))
+                       const int64_t* xop_off, int32_t* status, Lay lay, int plan_ints) {
+  if (plan_ints > 0) {
+    if (mixed) {
+      KG_STAGED_SWITCH(tps, launch(k_adj_bwd<float, double, T, true>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt,
+                                   ck_stride, cpw, seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay,
+                                   plan_ints))
+    }
+    KG_STAGED_SWITCH(tps, launch(k_adj_bwd<double, double, T, true>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt,
+                                 ck_stride, cpw, seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay,
+                                 plan_ints))
   }
-  KG_TPS_SWITCH(tps, launch(k_adj_bwd<double, double, T>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt, ck_stride, cpw,
-                            seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay))
+  if (mixed) {
+    KG_TPS_SWITCH(tps, launch(k_adj_bwd<float, double, T, false>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt,
+                              ck_stride, cpw, seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay, 0))
+  }
+  KG_TPS_SWITCH(tps, launch(k_adj_bwd<double, double, T, false>, g, b, lds, st, B, kind_slot, aa, cvec, ckpt,
+                            ck_stride, cpw, seg, seg_stride, grad, grad_stride, xbar0, xop_off, status, lay, 0))
 }
 
 hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

@@ -81,14 +81,14 @@ hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st
 hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                    LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
-                   Lay lay);
+                   Lay lay, int plan_ints = 0);
 hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
-                       int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay);
+                       int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay, int plan_ints = 0);
 hipError_t run_adj_bwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
                        int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
-                       const int64_t* xop_off, int32_t* status, Lay lay);
+                       const int64_t* xop_off, int32_t* status, Lay lay, int plan_ints = 0);
 hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                       const double* xbar0, const double* xop, const int64_t* xop_off, double* grad,
                       int64_t grad_stride, int32_t* status, Lay lay);
