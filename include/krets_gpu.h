@@ -181,6 +181,12 @@ int32_t kg_adjoint_forward(kg_ctx* ctx, const kg_solver_config* cfg, double h, i
                            int32_t* status, int32_t mem);
 int32_t kg_adjoint_backward(kg_ctx* ctx, double* grad, int32_t* status, int32_t mem);
 
+/* Graph replay (no krets counterpart; on by default): a call whose work
+ * mapping is already known runs its launch sequence as a captured hipGraph,
+ * captured once per distinct argument set and replayed afterwards. Calls made
+ * while kernel timing is on run eagerly. Applies to kg_ac_sweep. */
+int32_t kg_use_graphs(kg_ctx* ctx, int32_t enable);
+
 /* Kernel timing (no krets counterpart): HIP events on the context's stream
  * time every analysis kernel launched while timing is on, per kernel. Reports
  * the dominant kernel since the previous call (the one with the largest

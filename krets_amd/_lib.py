@@ -86,6 +86,7 @@ def lib():
         L.kg_last_autotune.argtypes = [P, PI, ctypes.POINTER(ctypes.c_float), I32]
         L.kg_profile.argtypes = [P, I32, PD, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_char_p)]
         L.kg_plan_export.argtypes = [P, I32, I32, I32, I32, P, D, P, I32, P, I32]
+        L.kg_use_graphs.argtypes = [P, I32]
         L.kg_jit_ac_source.argtypes = [P, I32, I32, P, D, I32, I32, ctypes.c_char_p, I32]
         L.kg_parse_netlist.argtypes = [C, ctypes.POINTER(P), ctypes.c_char_p, I32]
         L.kg_circuit_free.argtypes = [P]

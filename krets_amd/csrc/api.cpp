@@ -89,6 +89,9 @@ struct kg_ctx {
   std::map<std::string, int> tune;
   std::string tune_file;
   std::vector<std::pair<int, float>> last_trials;
+  // captured launch sequences of repeated calls (AC sweeps): key -> graph
+  bool graphs = true;
+  std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gexec;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
   std::map<std::string, std::pair<double, int64_t>> prof_k;  // kernel -> (ms, launches)
@@ -140,6 +143,11 @@ void free_batch(kg_ctx* c) {
     if (kv.second) hipFree(kv.second);
   c->inst_pat.clear();
   c->lane_fn.clear();
+  for (auto& kv : c->gexec) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
+  c->gexec.clear();
   c->n_inst = 0;
 }
 
@@ -480,8 +488,15 @@ NewtonArgs newton_args(const kg_solver_config* cfg) {
 
 // Runs OP (npts = 1) or a DC sweep; result stays on the device in the
 // workspace buffer "newton_out"; status in "status" (int32 per instance).
+// A launch that can run eagerly (timed with events) or be captured.
+struct Deferred {
+  const char* name;  // nullptr: not timed
+  std::function<hipError_t()> launch;
+};
+
 int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start, double step, int npts,
-               double** d_out, int64_t** d_off, int32_t** d_status, int32_t** d_iters) {
+               double** d_out, int64_t** d_off, int32_t** d_status, int32_t** d_iters,
+               std::vector<Deferred>* deferred = nullptr) {
   Plan P;
   int tps0 = cfg->threads_per_system > 0 ? cfg->threads_per_system : 0;
   // prepare with a provisional ordering to learn sizes
@@ -504,17 +519,32 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   *d_status = (int32_t*)ws_get(c, "status", sizeof(int32_t) * c->n_inst, &e);
   *d_iters = (int32_t*)ws_get(c, "newton_iters", sizeof(int32_t) * (size_t)c->n_inst * npts, &e);
-  HIPCHK(c, hipMemsetAsync(*d_status, 0, sizeof(int32_t) * c->n_inst, c->stream));
   BatchDev B = batch_dev(c, P);
   NewtonArgs na = newton_args(cfg);
-  int32_t rr = timed(c, "k_newton_dc", [&] {
-    return run_newton(tps, L.grid, L.block, L.lds, c->stream, B, 0, na, sweep_e, start, step, npts, *d_out, *d_off,
-                      *d_status, *d_iters, lay);
-  });
-  if (rr) return rr;
+  int32_t* st = *d_status;
+  int32_t* its = *d_iters;
+  double* o = *d_out;
+  int64_t* oo = *d_off;
+  hipStream_t strm = c->stream;
+  int ni = c->n_inst;
+  std::vector<Deferred> steps{
+      {nullptr, [=] { return run_fill_i32(st, KG_OK, ni, strm); }},
+      {"k_newton_dc", [=] {
+         return run_newton(tps, L.grid, L.block, L.lds, strm, B, 0, na, sweep_e, start, step, npts, o, oo, st, its,
+                           lay);
+       }}};
   // instances whose topology has no plan fail as krets' sp_lu would
   bool any_bad = false;
   for (auto s : P.group_status) any_bad |= s != KG_OK;
+  if (deferred && !any_bad) {
+    deferred->insert(deferred->end(), steps.begin(), steps.end());
+    return KG_OK;
+  }
+  for (auto& d : steps) {
+    int32_t rr = d.name ? timed(c, d.name, d.launch) : KG_OK;
+    if (!d.name) HIPCHK(c, d.launch());
+    if (rr) return rr;
+  }
   if (any_bad) {
     std::vector<int32_t> st(c->n_inst);
     HIPCHK(c, hipMemcpyAsync(st.data(), *d_status, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
@@ -833,19 +863,26 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   HIPCHK(c, hipSetDevice(c->device));
   for (int k = 0; k < nf; k++)
     if (!(freqs[k] > 0.0)) return set_err(c, KG_INVALID_ARGUMENT, "frequencies must be positive (ac.rs:40-45)");
+  Plan P0, P;
+  double fmin = freqs[0], fmax = freqs[0];
+  for (int k = 0; k < nf; k++) fmin = std::min(fmin, freqs[k]), fmax = std::max(fmax, freqs[k]);
+  double s_ac = 2.0 * 3.14159265358979323846 * std::sqrt(fmin * fmax);
+  int32_t r = prepare(c, AK_AC, ORD_NESTED, s_ac, P0);
+  if (r) return r;
+  const std::string tkey = tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf);
+  // A call whose work mapping is already known runs as one captured graph:
+  // operating point (fill, Newton kernel), fill, sweep kernel.
+  bool graph = c->graphs && !c->prof && (cfg->threads_per_system != 0 || c->tune.count(tkey));
+  std::vector<Deferred> steps;
   // ac.rs:25: the operating point comes first
   kg_solver_config ocfg = *cfg;
   ocfg.threads_per_system = 0;
   double* d_xop;
   int64_t* d_xoff;
   int32_t *d_st, *d_it;
-  int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
+  r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it, graph ? &steps : nullptr);
   if (r) return r;
-  Plan P0, P;
-  double fmin = freqs[0], fmax = freqs[0];
-  for (int k = 0; k < nf; k++) fmin = std::min(fmin, freqs[k]), fmax = std::max(fmax, freqs[k]);
-  double s_ac = 2.0 * 3.14159265358979323846 * std::sqrt(fmin * fmax);
-  if ((r = prepare(c, AK_AC, ORD_NESTED, s_ac, P0))) return r;
+  if (steps.empty()) graph = false;  // the operating point ran eagerly (structurally singular groups)
   bool mixed = precision == KG_MIXED;
   int refine = cfg->refine_iters > 0 ? cfg->refine_iters : 2;
   auto off = out_offsets(c, 2 * (int64_t)nf);
@@ -955,7 +992,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
     std::vector<int> cands{1, 4, 16, 64, 256};
     if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064, 2001});
     if (c->shared && mixed) cands.push_back(2002);
-    r = autotune(c, tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf), cands, [&](int t, float* ms) -> int32_t {
+    r = autotune(c, tkey, cands, [&](int t, float* ms) -> int32_t {
       AcLaunch A;
       int32_t rr = setup(t, A);
       if (rr) return rr;
@@ -968,12 +1005,44 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   AcLaunch A;
   if ((r = setup(cand, A))) return r;
   P = A.P;
-  HIPCHK(c, hipMemsetAsync(d_err, 0, sizeof(double) * c->n_inst, c->stream));
-  r = timed(c, A.lane ? jit_ac_name(mixed, cand == 2002)
-             : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
-                            : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
-            [&] { return run(A); });
-  if (r) return r;
+  int ni = c->n_inst;
+  steps.push_back({nullptr, [&] { return run_fill_f64(d_err, 0.0, ni, c->stream); }});
+  steps.push_back({A.lane ? jit_ac_name(mixed, cand == 2002)
+                   : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
+                                  : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
+                   [&] { return run(A); }});
+  if (graph) {
+    std::ostringstream k;
+    k << "ac|" << cand << "|" << refine << "|" << precision << "|" << nf << "|" << ocfg.reltol << "|" << ocfg.iabstol
+      << "|" << ocfg.vabstol << "|" << ocfg.maxiter << "|" << ocfg.compat << "|" << (void*)d_out << "|" << (void*)d_f
+      << "|" << (void*)d_xop << "|" << (void*)d_st << "|" << (void*)d_err << "|" << (void*)c->d_pats << "|"
+      << (void*)d_off << "|" << (void*)d_xoff << "|" << (void*)A.fn.fn;
+    auto it = c->gexec.find(k.str());
+    if (it == c->gexec.end()) {
+      hipGraph_t g = nullptr;
+      hipGraphExec_t ge = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e2 = hipSuccess;
+      for (auto& d : steps)
+        if ((e2 = d.launch()) != hipSuccess) break;
+      hipError_t e3 = hipStreamEndCapture(c->stream, &g);
+      if (e2 != hipSuccess || e3 != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return set_err(c, KG_HIP_ERROR, std::string("graph capture: ") + hipGetErrorString(e2 ? e2 : e3));
+      }
+      HIPCHK(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      it = c->gexec.emplace(k.str(), std::make_pair(g, ge)).first;
+    }
+    HIPCHK(c, hipGraphLaunch(it->second.second, c->stream));
+  } else {
+    for (auto& d : steps) {
+      if (d.name) {
+        if ((r = timed(c, d.name, d.launch))) return r;
+      } else {
+        HIPCHK(c, d.launch());
+      }
+    }
+  }
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;
   for (int g = 0; g < (int)P.group_status.size(); g++)
@@ -1094,6 +1163,12 @@ int32_t kg_last_autotune(const kg_ctx* c, int32_t* tps, float* ms, int32_t cap) 
     if (ms) ms[k] = c->last_trials[k].second;
   }
   return n;
+}
+
+int32_t kg_use_graphs(kg_ctx* c, int32_t enable) {
+  if (!c) return KG_INVALID_ARGUMENT;
+  c->graphs = enable != 0;
+  return KG_OK;
 }
 
 int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launches, const char** kernel) {

@@ -1652,6 +1652,27 @@ hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev
                             xout_off, npts, newton_total, rejected, status, lay, 0))
 }
 
+// Fill kernels: used instead of memset nodes so that a launch sequence can be
+// captured into a hipGraph and replayed.
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double* p, double v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+hipError_t run_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+hipError_t run_fill_f64(double* p, double v, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
 // plan_ints > 0 (64 or 256 threads per system): the plan is staged in LDS.
 #define KG_STAGED_SWITCH(tps, CALL)                         \
   switch (tps) {                                            \

@@ -82,6 +82,8 @@ hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev
                    LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
                    Lay lay, int plan_ints = 0);
+hipError_t run_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+hipError_t run_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
                        int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay, int plan_ints = 0);

@@ -336,6 +336,10 @@ class Context:
                                    m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 8)
         return [(int(t[i]), float(m[i])) for i in range(max(k, 0))]
 
+    def use_graphs(self, enable=True):
+        """Replay repeated AC sweeps as captured hipGraphs (default on)."""
+        self._check(lib().kg_use_graphs(self._h, 1 if enable else 0))
+
     def profile(self, enable=True):
         """Returns (total_ms, launches, kernel) of the dominant analysis kernel
         (largest accumulated time) since the previous call, then resets and

@@ -274,6 +274,48 @@ def test_bucketed_transient_uneven_rlc(kg, oracle, precision):
         assert e < 1e-9, (i, e)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ac_graph_replay(kg, oracle, precision):
+    """Repeated sweeps replay one captured graph (operating point, fills,
+    sweep kernel). Every replay, with eager work and synchronisations in
+    between, must equal the eager result bit for bit; new parameters and new
+    output buffers must be honoured."""
+    import torch
+    from krets_amd import workloads as W
+    say(f"AC graph replay precision={precision}")
+    b = W.c2_rc_ladder_mc(40, sections=50, seed=23)
+    f = W.c2_frequencies(200)[::4]
+    ctx = kg.Context()
+    b.load(ctx)
+    ctx.use_graphs(False)
+    ref, st, err_ref = ctx.ac_sweep(f, precision=precision)
+    assert (st == 0).all()
+    ctx.use_graphs(True)
+    dev = torch.device("cuda:0")
+    out = torch.empty(40 * len(f) * b.n_unknowns * 2, dtype=torch.float64, device=dev)
+    for k in range(5):
+        out.fill_(float("nan"))
+        _, st, err = ctx.ac_sweep(f, precision=precision, out=out)
+        assert (st == 0).all()
+        x = out.view(40, len(f), b.n_unknowns, 2).cpu().numpy()
+        for i in (0, 17, 39):
+            assert np.array_equal(x[i, ..., 0] + 1j * x[i, ..., 1], ref[i]), (k, i)
+        assert np.array_equal(err, err_ref)
+        ctx.op()  # eager work on the same stream between replays
+        torch.cuda.synchronize()
+    # new parameter values: the replayed graph reads them from device memory
+    p2 = b.params.copy()
+    p2[:, :, 0] *= 1.01
+    ctx.load(b.elements, p2, b.n_unknowns)
+    x2, st, _ = ctx.ac_sweep(f, precision=precision)
+    x3, st3, _ = ctx.ac_sweep(f, precision=precision)
+    assert (st == 0).all() and (st3 == 0).all()
+    assert np.array_equal(x2[5], x3[5])
+    xr = oracle.Circuit(W.c2_netlist(p2[5, 1::2, 0], p2[5, 2::2, 0])).ac(f)  # elements: V1, R1, C1, R2, ...
+    e = max(nerr(x3[5][j], xr[j]) for j in range(len(f)))
+    assert e < (1e-12 if precision == 0 else 1e-9), e
+
+
 def test_error_statuses(kg):
     say("error statuses")
     s = kg.Solver(kg.Circuit("I1 1 0 1\nR1 1 0 1\n"))
