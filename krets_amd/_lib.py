@@ -59,11 +59,27 @@ class KretsGpuError(RuntimeError):
 _LIB = None
 
 
+def _torch_first():
+    """PyTorch wheels bundle their own HIP and HSA runtimes next to the system
+    ones that libkrets_gpu.so links. When PyTorch is loaded and its runtime
+    started before this library is loaded, the library's HIP runtime binds to
+    that same HSA runtime and device buffers pass freely between the two (the
+    out= arguments). In the other order the runtime started second finds no
+    GPU. So PyTorch, when importable, is loaded and started first."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if getattr(torch.version, "hip", None) and torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m krets_amd.build`")
+        _torch_first()
         L = ctypes.CDLL(LIB_PATH)
         P, I32, D, C = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_char_p
         PI = ctypes.POINTER(ctypes.c_int32)

@@ -157,32 +157,10 @@ def _round_half_away(x):
 
 
 # ------------------------------------------------------------- batched context
-def _torch_runtime_first():
-    """PyTorch wheels bundle their own HIP runtime next to the system one that
-    libkrets_gpu.so links; both then live in one process. Device buffers are
-    shared between them (out= arguments), but the bundled runtime finds no GPU
-    if it starts after the system one, so it is started first whenever
-    PyTorch is importable."""
-    global _TORCH_READY
-    if _TORCH_READY:
-        return
-    _TORCH_READY = True
-    try:
-        import torch
-    except ImportError:
-        return
-    if getattr(torch.version, "hip", None) and torch.cuda.is_available():
-        torch.cuda.init()
-
-
-_TORCH_READY = False
-
-
 class Context:
     """One kg_ctx: a HIP stream on one GPU plus the loaded batch."""
 
     def __init__(self, device: int = 0):
-        _torch_runtime_first()
         h = ctypes.c_void_p()
         st = lib().kg_create(ctypes.byref(h), device)
         if st != _lib.OK:
