@@ -82,7 +82,12 @@ __device__ __forceinline__ bool fin(Z a) { return __builtin_isfinite(a.re) && __
 __device__ __forceinline__ Z rcpF(Z b) {
   const int k = __builtin_amdgcn_frexp_exp(__builtin_fmax(__builtin_fabs(b.re), __builtin_fabs(b.im)));
   const double re = __builtin_amdgcn_ldexp(b.re, -k), im = __builtin_amdgcn_ldexp(b.im, -k);
-  const double q = 1.0 / (re * re + im * im);
+  // m in [0.25, 2): hardware reciprocal estimate and two Newton steps (a
+  // shorter dependent chain than a correctly rounded division; ~1 ulp)
+  const double m = re * re + im * im;
+  double q = __builtin_amdgcn_rcp(m);
+  q = q * (2.0 - m * q);
+  q = q * (2.0 - m * q);
   return {__builtin_amdgcn_ldexp(re * q, -k), __builtin_amdgcn_ldexp(-im * q, -k)};
 }
 // complex64 pivots: |b|^2 in FP64 (cannot leave its range), reciprocal from
