@@ -97,7 +97,8 @@ typedef struct {
    * staged in LDS by a persistent grid), 2001 (one lane per system, code
    * specialised to the topology at run time) and, for KG_MIXED, 2002 (as
    * 2001 with the solution split into a complex64 first solution in LDS and
-   * a complex64 sum of corrections in registers). Fixed-step transients also
+   * a complex64 sum of corrections in registers); 2011 / 2012 are 2001 / 2002
+   * on a balanced elimination order. Fixed-step transients also
    * take 1064 and 1256 (64 or 256 threads per system with the plan copied
    * into LDS once per launch). */
   int32_t threads_per_system;
@@ -203,7 +204,8 @@ int32_t kg_last_autotune(const kg_ctx* ctx, int32_t* tps, float* ms, int32_t cap
 
 /* Diagnostics (no krets counterpart): the symbolic plan the library would
  * build for one topology. kind: 0 DC, 1 AC, 2 transient; ordering: 0 natural,
- * 1 minimum degree, 2 nested dissection. params (may be NULL): one instance's
+ * 1 minimum degree, 2 nested dissection, 3 balanced (minimum fill, ties to
+ * the shallowest elimination-tree position). params (may be NULL): one instance's
  * parameter records, used with s_rep (alpha/h or 2*pi*f) for the static pivot
  * choice. Returns the plan length in int32 (copying at most cap entries to
  * buf) or -status. layout receives the plan's header (PatLayout, see

@@ -901,7 +901,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   // 2001: the lane-per-system kernel specialised to this topology at run
   // time (shared topology, small systems); 2002: its mixed precision form
   // with the solution split into a complex64 first solution in LDS and a
-  // complex64 sum of corrections in registers.
+  // complex64 sum of corrections in registers. 2011 / 2012: the same on the
+  // balanced order (independent elimination chains, more live registers).
   struct AcLaunch {
     Plan P;
     Lay lay;
@@ -913,23 +914,25 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   };
   const int64_t nsys = (int64_t)c->n_inst * nf;
   auto setup = [&](int cand, AcLaunch& A) -> int32_t {
-    A.lane = cand == 2001 || cand == 2002;
+    A.lane = cand == 2001 || cand == 2002 || cand == 2011 || cand == 2012;
     A.persistent = cand >= 1000 && !A.lane;
     A.tps = cand % 1000;
     if (A.lane) {
+      const bool split = cand % 10 == 2;
+      const Ordering ord = cand >= 2010 ? ORD_BALANCED : ORD_MINDEG;
       if (!c->shared) return set_err(c, KG_UNEXPECTED, "lane kernel needs a shared topology");
-      int32_t rr = prepare(c, AK_AC, ORD_MINDEG, s_ac, A.P);
+      int32_t rr = prepare(c, AK_AC, ord, s_ac, A.P);
       if (rr) return rr;
-      if (cand == 2002 && !mixed) return set_err(c, KG_UNEXPECTED, "2002 is the mixed precision lane kernel");
-      int pidx = c->pat_index[std::make_tuple(0, (int)AK_AC, (int)ORD_MINDEG, s_ac)];
-      auto key = std::make_tuple(pidx, (int)mixed, (int)(cand == 2002));
+      if (split && !mixed) return set_err(c, KG_UNEXPECTED, "2002 / 2012 are mixed precision lane kernels");
+      int pidx = c->pat_index[std::make_tuple(0, (int)AK_AC, (int)ord, s_ac)];
+      auto key = std::make_tuple(pidx, (int)mixed, (int)split);
       auto it = c->lane_fn.find(key);
       if (it == c->lane_fn.end()) {
         const PatEntry& pe = c->pats[pidx];
         std::string src, why, log;
-        if (!jit_ac_source(pe.lay, pe.host.data(), mixed, cand == 2002, src, why))
+        if (!jit_ac_source(pe.lay, pe.host.data(), mixed, split, src, why))
           return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
-        if (jit_load(c->device, src, jit_ac_name(mixed, cand == 2002), A.fn, log) != hipSuccess)
+        if (jit_load(c->device, src, jit_ac_name(mixed, split), A.fn, log) != hipSuccess)
           return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
         c->lane_fn[key] = A.fn;
       } else {
@@ -937,7 +940,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       }
       A.L.grid = dim3((unsigned)((nsys + 63) / 64));
       A.L.block = dim3(64);
-      A.L.lds = cand == 2002 ? 64 * sizeof(float) * 2 * (size_t)c->nunk[0] : 0;  // x0 per lane (complex64)
+      A.L.lds = split ? 64 * sizeof(float) * 2 * (size_t)c->nunk[0] : 0;  // x0 per lane (complex64)
       return KG_OK;
     }
     if (!valid_tps(A.tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
@@ -990,8 +993,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   int cand = cfg->threads_per_system;
   if (cand == 0) {
     std::vector<int> cands{1, 4, 16, 64, 256};
-    if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064, 2001});
-    if (c->shared && mixed) cands.push_back(2002);
+    if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064, 2001, 2011});
+    if (c->shared && mixed) cands.insert(cands.end(), {2002, 2012});
     r = autotune(c, tkey, cands, [&](int t, float* ms) -> int32_t {
       AcLaunch A;
       int32_t rr = setup(t, A);
@@ -1007,7 +1010,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   P = A.P;
   int ni = c->n_inst;
   steps.push_back({nullptr, [&] { return run_fill_f64(d_err, 0.0, ni, c->stream); }});
-  steps.push_back({A.lane ? jit_ac_name(mixed, cand == 2002)
+  steps.push_back({A.lane ? jit_ac_name(mixed, cand % 10 == 2)
                    : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
                                   : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
                    [&] { return run(A); }});
@@ -1194,7 +1197,7 @@ int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launche
 int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind, int32_t ordering,
                        const double* params, double s_rep, int32_t* buf, int32_t cap, int32_t* layout,
                        int32_t layout_cap) {
-  if (!el || ne <= 0 || n <= 0 || kind < 0 || kind >= AK_COUNT || ordering < 0 || ordering > ORD_NESTED)
+  if (!el || ne <= 0 || n <= 0 || kind < 0 || kind >= AK_COUNT || ordering < 0 || ordering > ORD_BALANCED)
     return -KG_INVALID_ARGUMENT;
   PatLayout L;
   std::string err;

@@ -12,7 +12,9 @@ namespace kg {
 enum AnalysisKind : int32_t { AK_DC = 0, AK_AC = 1, AK_TRAN = 2, AK_COUNT = 3 };
 
 // Symmetric fill-reducing orders applied after the structural matching.
-enum Ordering : int32_t { ORD_NATURAL = 0, ORD_MINDEG = 1, ORD_NESTED = 2 };
+// ORD_BALANCED: minimum fill with ties to the shallowest elimination-tree
+// position (independent elimination chains for one thread per system).
+enum Ordering : int32_t { ORD_NATURAL = 0, ORD_MINDEG = 1, ORD_NESTED = 2, ORD_BALANCED = 3 };
 
 // Matrix contribution kinds gathered per LU slot. A contribution is encoded as
 // (element << 3) | kind.

@@ -22,6 +22,7 @@
 #include <functional>
 #include <queue>
 #include <set>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,7 @@ struct Entry {
   int32_t code;
 };
 
+// Minimum degree (ties to the lowest index).
 std::vector<int> order_min_degree(int n, const std::vector<std::vector<int>>& adj0) {
   std::vector<std::set<int>> adj(n);
   for (int v = 0; v < n; v++) adj[v].insert(adj0[v].begin(), adj0[v].end());
@@ -56,6 +58,57 @@ std::vector<int> order_min_degree(int n, const std::vector<std::vector<int>>& ad
       for (int w : nb)
         if (w != u) adj[u].insert(w);
       pq.insert({(int)adj[u].size(), u});
+    }
+    adj[v].clear();
+  }
+  return order;
+}
+
+// Minimum fill (deficiency: neighbour pairs not yet adjacent); ties go to the
+// vertex lowest in the elimination tree built so far (depth = 1 + deepest
+// eliminated neighbour), then to minimum degree. The depth rule keeps the
+// tree balanced: a chain is eliminated from both ends at once (two
+// independent dependency chains instead of one, with no fill).
+std::vector<int> order_balanced(int n, const std::vector<std::vector<int>>& adj0) {
+  std::vector<std::set<int>> adj(n);
+  for (int v = 0; v < n; v++) adj[v].insert(adj0[v].begin(), adj0[v].end());
+  auto deficiency = [&](int v) {
+    int d = 0;
+    for (auto a = adj[v].begin(); a != adj[v].end(); ++a)
+      for (auto b = std::next(a); b != adj[v].end(); ++b)
+        if (!adj[*a].count(*b)) d++;
+    return d;
+  };
+  std::vector<int> depth(n, 0), def(n, 0);
+  std::set<std::tuple<int, int, int, int>> pq;  // (fill, depth, degree, vertex)
+  auto key = [&](int v) { return std::make_tuple(def[v], depth[v], (int)adj[v].size(), v); };
+  for (int v = 0; v < n; v++) {
+    def[v] = deficiency(v);
+    pq.insert(key(v));
+  }
+  std::vector<char> done(n, 0);
+  std::vector<int> order;
+  order.reserve(n);
+  while (!pq.empty()) {
+    int v = std::get<3>(*pq.begin());
+    pq.erase(pq.begin());
+    order.push_back(v);
+    done[v] = 1;
+    std::vector<int> nb(adj[v].begin(), adj[v].end());
+    std::set<int> touched(nb.begin(), nb.end());
+    for (int u : nb) {
+      pq.erase(key(u));
+      adj[u].erase(v);
+      for (int w : nb)
+        if (w != u) adj[u].insert(w);
+      depth[u] = std::max(depth[u], depth[v] + 1);
+    }
+    for (int u : nb) touched.insert(adj[u].begin(), adj[u].end());
+    for (int u : touched) {
+      if (done[u]) continue;
+      pq.erase(key(u));
+      def[u] = deficiency(u);
+      pq.insert(key(u));
     }
     adj[v].clear();
   }
@@ -294,6 +347,7 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   }
   std::vector<int> ord;
   if (ordk == ORD_MINDEG) ord = order_min_degree(n, adj);
+  else if (ordk == ORD_BALANCED) ord = order_balanced(n, adj);
   else if (ordk == ORD_NESTED) ord = order_nested(n, adj);
   else {
     ord.resize(n);

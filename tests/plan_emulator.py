@@ -16,7 +16,7 @@ FIELDS = ["n", "nnz", "ne", "nlev", "nblev", "ntlev", "ntblev", "nonlinear", "ha
           "o_etype", "o_ep", "o_em", "o_ec", "o_ut_ptr", "o_ut", "o_lt_ptr", "o_lt", "o_tlvl_ptr", "o_tlvl_rows",
           "o_tblvl_ptr", "o_tblvl_rows", "o_row_of", "o_rpos", "o_erp", "o_erm", "o_erc", "status", "total"]
 DC, AC, TRAN = 0, 1, 2
-NATURAL, MINDEG, NESTED = 0, 1, 2
+NATURAL, MINDEG, NESTED, BALANCED = 0, 1, 2, 3
 
 
 class Plan:

@@ -164,8 +164,9 @@ def test_transient_compat_vs_oracle(kg, oracle, path, h, stop):
 
 
 # 1000 + t: persistent LDS-staged variant; 2001: lane per system, run-time
-# specialised; 2002: as 2001 with the solution split into x0 (LDS) + dx
-@pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064, 2001, 2002])
+# specialised; 2002: as 2001 with the solution split into x0 (LDS) + dx;
+# 2011 / 2012: both on the balanced elimination order
+@pytest.mark.parametrize("tps", TPS + [1004, 1016, 1064, 2001, 2002, 2011, 2012])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
     from krets_amd import workloads as W
@@ -174,7 +175,7 @@ def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
     f = W.c2_frequencies(200)[::10]
     ctx = kg.Context()
     b.load(ctx)
-    if tps == 2002 and precision == 0:
+    if tps in (2002, 2012) and precision == 0:
         with pytest.raises(kg.KretsGpuError):
             ctx.ac_sweep(f, precision=precision, config=kg.SolverConfig(threads_per_system=tps))
         return
@@ -190,7 +191,7 @@ def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
 
 
 @pytest.mark.parametrize("refine", [1, 2, 3])
-@pytest.mark.parametrize("precision,lane", [(0, 2001), (1, 2001), (1, 2002)])
+@pytest.mark.parametrize("precision,lane", [(0, 2001), (1, 2001), (1, 2002), (0, 2011), (1, 2012)])
 def test_lane_ac_golden_and_rlc(kg, oracle, precision, lane, refine):
     """Run-time specialised lane kernel: the krets AC golden vector and a shared
     RLC ladder batch (inductor branch rows) vs the oracle."""

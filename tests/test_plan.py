@@ -83,7 +83,7 @@ def values(c, kind, xop=None, f=0.0, h=1e-9, alpha=1.0):
     return contrib, rhs
 
 
-@pytest.mark.parametrize("ordering", [pe.NATURAL, pe.MINDEG, pe.NESTED])
+@pytest.mark.parametrize("ordering", [pe.NATURAL, pe.MINDEG, pe.NESTED, pe.BALANCED])
 @pytest.mark.parametrize("path", CIRCUITS)
 def test_plan_dc_ac_matches_oracle(oracle, path, ordering):
     import krets_amd as k
@@ -127,7 +127,7 @@ def test_plan_dc_ac_matches_oracle(oracle, path, ordering):
     assert np.allclose(x, np.linalg.solve(M, b), rtol=1e-9, atol=1e-14 * max(np.abs(x).max(), 1e-300))
 
 
-@pytest.mark.parametrize("ordering", [pe.MINDEG, pe.NESTED])
+@pytest.mark.parametrize("ordering", [pe.MINDEG, pe.NESTED, pe.BALANCED])
 @pytest.mark.parametrize("sections", [1, 8, 37])
 def test_plan_transient_ladder(oracle, sections, ordering):
     import krets_amd as k
@@ -143,6 +143,10 @@ def test_plan_transient_ladder(oracle, sections, ordering):
     if ordering == pe.NESTED and sections == 37:
         # nested dissection keeps the elimination tree shallow on a chain
         assert P.L["nlev"] <= 2 * math.ceil(math.log2(c.n)) + 2
+    if ordering == pe.BALANCED and sections == 37:
+        # a chain is eliminated from both ends: about half the levels, no fill
+        Pm = pe.Plan(c.elements, c.n, pe.TRAN, pe.MINDEG)
+        assert P.L["nnz"] == Pm.L["nnz"] and P.L["nlev"] <= Pm.L["nlev"] // 2 + 2
 
 
 @pytest.mark.parametrize("kind", [pe.DC, pe.TRAN])
