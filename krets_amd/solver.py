@@ -330,11 +330,12 @@ class Context:
 
     def last_autotune(self):
         """[(threads_per_system, kernel_ms)] timed by the last autotuning run."""
-        t = np.zeros(8, np.int32)
-        m = np.zeros(8, np.float32)
+        cap = 64
+        t = np.zeros(cap, np.int32)
+        m = np.zeros(cap, np.float32)
         k = lib().kg_last_autotune(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                   m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 8)
-        return [(int(t[i]), float(m[i])) for i in range(max(k, 0))]
+                                   m.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), cap)
+        return [(int(t[i]), float(m[i])) for i in range(min(max(k, 0), cap))]
 
     def use_graphs(self, enable=True):
         """Replay repeated AC sweeps as captured hipGraphs (default on)."""
