@@ -237,6 +237,32 @@ const char* kg_circuit_element_id(const kg_circuit* c, int32_t e);
 /* Fills elements[n_elements] and params[n_elements][KG_NPARAM]. */
 int32_t kg_circuit_tables(const kg_circuit* c, kg_element* elements, double* params);
 
+/* Analysis spec (krets.toml) in the shape of krets_parser::analyses::
+ * AnalysisSpec (crates/krets-parser/src/analyses.rs:8-76): one analysis,
+ * selected by kind; fields of the other kinds stay zero. */
+typedef enum { KG_SPEC_OP = 0, KG_SPEC_DC = 1, KG_SPEC_AC = 2, KG_SPEC_TRANSIENT = 3 } kg_spec_kind;
+typedef enum { KG_SWEEP_DECADE = 0, KG_SWEEP_OCTAVE = 1, KG_SWEEP_LINEAR = 2 } kg_sweep_kind;
+typedef struct {
+  int32_t kind;              /* kg_spec_kind */
+  char circuit_path[1024];   /* as written; krets-cli resolves it against the spec's folder */
+  char element[64];          /* DC: swept source (DcAnalysis::element) */
+  double start, stop, step_size;
+  int32_t sweep;             /* AC: kg_sweep_kind (AcSweep's "variation" tag) */
+  int32_t reserved;
+  int64_t points;            /* AC: points_per_decade / points_per_octave / total_points (u32) */
+  double fstart, fstop;
+  double time_step, stop_time;
+} kg_spec;
+
+/* Reads a krets.toml text, as AnalysisSpec::from_file deserialises it
+ * (crates/krets-parser/src/analyses.rs:19-23). Returns KG_OK or KG_PARSE with
+ * a message in err. */
+int32_t kg_parse_spec(const char* toml_text, kg_spec* out, char* err, int32_t errlen);
+/* Reads a bare Analysis document ("[transient]\ntime_step = ..."), as
+ * toml::from_str::<Analysis> does in krets' own parser tests
+ * (crates/krets-parser/src/analyses.rs:170-186); circuit_path stays empty. */
+int32_t kg_parse_analysis(const char* toml_text, kg_spec* out, char* err, int32_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,8 @@ FP64, MIXED = 0, 1
 BE_COMPAT, BE, TRAP = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 NPARAM = 10
+SPEC_OP, SPEC_DC, SPEC_AC, SPEC_TRANSIENT = 0, 1, 2, 3
+SWEEP_DECADE, SWEEP_OCTAVE, SWEEP_LINEAR = 0, 1, 2
 
 STATUS_NAMES = {0: "Ok", 1: "MatrixBuild", 2: "DecompositionFailed", 3: "MaximumIterationsExceeded",
                 4: "ElementNotFound", 5: "InvalidElementFormat", 6: "Unexpected", 7: "ParseError",
@@ -46,6 +48,14 @@ class KgConfig(ctypes.Structure):
     _fields_ = [("reltol", ctypes.c_double), ("iabstol", ctypes.c_double), ("vabstol", ctypes.c_double),
                 ("maxiter", ctypes.c_int32), ("compat", ctypes.c_int32), ("threads_per_system", ctypes.c_int32),
                 ("refine_iters", ctypes.c_int32)]
+
+
+class KgSpec(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("circuit_path", ctypes.c_char * 1024), ("element", ctypes.c_char * 64),
+                ("start", ctypes.c_double), ("stop", ctypes.c_double), ("step_size", ctypes.c_double),
+                ("sweep", ctypes.c_int32), ("reserved", ctypes.c_int32), ("points", ctypes.c_int64),
+                ("fstart", ctypes.c_double), ("fstop", ctypes.c_double), ("time_step", ctypes.c_double),
+                ("stop_time", ctypes.c_double)]
 
 
 class KretsGpuError(RuntimeError):
@@ -112,6 +122,8 @@ def lib():
         L.kg_circuit_element_id.restype = C
         L.kg_circuit_element_id.argtypes = [P, I32]
         L.kg_circuit_tables.argtypes = [P, P, PD]
+        L.kg_parse_spec.argtypes = [C, ctypes.POINTER(KgSpec), ctypes.c_char_p, I32]
+        L.kg_parse_analysis.argtypes = [C, ctypes.POINTER(KgSpec), ctypes.c_char_p, I32]
         _LIB = L
     return _LIB
 

@@ -4,7 +4,9 @@ Run once in a container that holds the reference checkout (default path
 /root/reference, or $KRETS_REFERENCE). It copies the small netlists that
 krets' own solver tests load (crates/krets-solver/tests/*.rs) and converts the
 one full-precision golden vector, circuits/low_pass_filter/ac/result.parquet,
-to JSON with every float written in round-trip form. Nothing here runs krets.
+to JSON with every float written in round-trip form, plus the column layout
+of circuits/inverter/dc/result.parquet (a MOSFET circuit, so only its schema
+and step column serve as a DC-writer fixture). Nothing here runs krets.
 """
 import json
 import os
@@ -24,6 +26,8 @@ NETLISTS = [
     "high_pass_filter/high_pass_filter.cir",
     "high_pass_filter/transient.cir",
     "high_pass_filter/transient/krets.toml",
+    "high_pass_filter/transient/transient.cir",
+    "inverter/dc/krets.toml",
     "basic_001/basic_001.cir",
     "diode_iv_curve/diode_iv_curve.cir",
     "dual_rc_ladder/dual_rc_ladder.cir",
@@ -49,6 +53,14 @@ def main():
                                 "fstop": 1000.0},
                    "columns": {k: [repr(float(v)) for v in vals] for k, vals in t.items()}},
                   f, indent=1)
+    pf = pq.ParquetFile(os.path.join(REF, "circuits/inverter/dc/result.parquet"))
+    t = pf.read().to_pydict()
+    with open(os.path.join(HERE, "inverter_dc_schema.json"), "w") as f:
+        json.dump({"source": "circuits/inverter/dc/result.parquet",
+                   "columns": pf.schema_arrow.names,
+                   "types": [str(x) for x in pf.schema_arrow.types],
+                   "compression": pf.metadata.row_group(0).column(0).compression,
+                   "step": t["step"]}, f, indent=1)
 
 
 if __name__ == "__main__":
