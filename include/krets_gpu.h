@@ -100,7 +100,10 @@ typedef struct {
    * a complex64 sum of corrections in registers); 2011 / 2012 are 2001 / 2002
    * on a balanced elimination order. Fixed-step transients also
    * take 1064 and 1256 (64 or 256 threads per system with the plan copied
-   * into LDS once per launch). */
+   * into LDS once per launch) and, for FP64 on circuits without diodes,
+   * 3064 / 3128 / 3256 / 3512 (one system per block of 64..512 threads with
+   * the whole step loop in LDS: compact 16-bit plan, factors, solutions and
+   * companion states). */
   int32_t threads_per_system;
   int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
 } kg_solver_config;
@@ -203,7 +206,9 @@ int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launc
 int32_t kg_last_autotune(const kg_ctx* ctx, int32_t* tps, float* ms, int32_t cap);
 
 /* Diagnostics (no krets counterpart): the symbolic plan the library would
- * build for one topology. kind: 0 DC, 1 AC, 2 transient; ordering: 0 natural,
+ * build for one topology. kind: 0 DC, 1 AC, 2 transient, 3 the compact 16-bit
+ * plan of the LDS-resident linear transient kernel (uint16 words, two per
+ * int32; -KG_UNEXPECTED if the topology does not qualify); ordering: 0 natural,
  * 1 minimum degree, 2 nested dissection, 3 balanced (minimum fill, ties to
  * the shallowest elimination-tree position). params (may be NULL): one instance's
  * parameter records, used with s_rep (alpha/h or 2*pi*f) for the static pivot

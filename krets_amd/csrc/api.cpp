@@ -25,6 +25,7 @@
 namespace kg {
 std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, AnalysisKind ak, Ordering ord,
                                    const double* rep, double s_rep, PatLayout& L, std::string& err);
+bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vector<uint16_t>& out);
 }
 
 using namespace kg;
@@ -35,6 +36,8 @@ struct PatEntry {
   PatLayout lay;
   std::vector<int32_t> host;  // the plan buffer (host copy)
   int32_t* d_buf = nullptr;
+  std::vector<uint16_t> lin;  // compact linear-transient plan (empty if none)
+  uint16_t* d_lin = nullptr;
   PatDev dev{};
   std::string err;
 };
@@ -135,8 +138,10 @@ void free_batch(kg_ctx* c) {
   c->d_nunk = nullptr;
   c->d_pats = nullptr;
   c->d_pats_n = 0;
-  for (auto& p : c->pats)
+  for (auto& p : c->pats) {
     if (p.d_buf) hipFree(p.d_buf);
+    if (p.d_lin) hipFree(p.d_lin);
+  }
   c->pats.clear();
   c->pat_index.clear();
   for (auto& kv : c->inst_pat)
@@ -199,6 +204,8 @@ PatDev to_dev(const PatLayout& L, const int32_t* b) {
   d.erc = b + L.o_erc;
   d.base = b;
   d.total = L.total;
+  d.lin = nullptr;
+  d.lin_words = 0;
   return d;
 }
 
@@ -233,6 +240,12 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
         HIPCHK(c, hipMalloc(&pe.d_buf, sizeof(int32_t) * std::max<size_t>(buf.size(), 1)));
         HIPCHK(c, hipMemcpy(pe.d_buf, buf.data(), sizeof(int32_t) * buf.size(), hipMemcpyHostToDevice));
         pe.dev = to_dev(pe.lay, pe.d_buf);
+        if (ak == AK_TRAN && build_lin_plan(pe.lay, buf, pe.lin)) {
+          HIPCHK(c, hipMalloc(&pe.d_lin, sizeof(uint16_t) * pe.lin.size()));
+          HIPCHK(c, hipMemcpy(pe.d_lin, pe.lin.data(), sizeof(uint16_t) * pe.lin.size(), hipMemcpyHostToDevice));
+          pe.dev.lin = pe.d_lin;
+          pe.dev.lin_words = (int32_t)pe.lin.size();
+        }
         pe.host = std::move(buf);
       } else {
         PatLayout z{};
@@ -556,6 +569,35 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   return KG_OK;
 }
 
+// LDS bytes the linear-transient kernel (k_tran_lin) needs for instance i,
+// 0 if its topology has no compact plan; instances without a valid plan
+// return 1 (they are never integrated, their status is already an error).
+size_t lin_need_one(kg_ctx* c, double s_tr, int i) {
+  auto it = c->pat_index.find(std::make_tuple(c->group_of[i], (int)AK_TRAN, (int)ORD_NESTED, s_tr));
+  if (it == c->pat_index.end()) return 0;
+  const PatEntry& pe = c->pats[it->second];
+  if (pe.lay.status != KG_OK) return 1;
+  if (pe.lin.empty()) return 0;
+  LinHdr h;
+  std::memcpy(&h, pe.lin.data(), sizeof(h));
+  return lin_bytes(h);
+}
+
+// Largest need over all instances, 0 if any instance has no compact plan.
+size_t lin_need_all(kg_ctx* c, double s_tr) {
+  size_t m = 0;
+  for (int i = 0; i < c->n_inst; i++) {
+    size_t b = lin_need_one(c, s_tr, i);
+    if (b == 0) return 0;
+    m = std::max(m, b);
+  }
+  return m;
+}
+
+// Work-mapping codes of k_tran_lin: 3000 + threads per block (one system per block).
+const std::vector<int> LIN_CANDS = {3064, 3128, 3256, 3512};
+bool is_lin_cand(int t) { return t == 3064 || t == 3128 || t == 3256 || t == 3512; }
+
 // Uneven transient batches: every block of one launch reserves LDS for the
 // largest instance, so a launch over 8..512-section ladders runs one block
 // per CU. Instances are split into size classes by how many of their blocks
@@ -569,29 +611,35 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   int32_t r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P);
   if (r) return r;
   const size_t red = 8 * 256;
+  // FP64 linear batches are classed by the LDS footprint of k_tran_lin
+  const bool lin_ok = !mixed && lin_need_all(c, s_tr) > 0;
   std::map<int, std::vector<int32_t>> cls;  // blocks per CU -> instances
   std::map<int, Lay> clay;
-  std::map<int, int> cplan;  // longest plan per class
+  std::map<int, int> cplan;      // longest plan per class
+  std::map<int, size_t> clin;    // largest k_tran_lin footprint per class
   for (int i = 0; i < c->n_inst; i++) {
     const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(c->group_of[i], (int)AK_TRAN, (int)ORD_NESTED, s_tr)]];
     if (pe.lay.status != KG_OK) continue;  // structurally singular: status set by the caller
     Lay l{pe.lay.nnz, pe.lay.n, pe.lay.ne, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
-    int per_cu = (int)std::min<size_t>(8, LDS_MAX / (sys_bytes(l) + red));
-    if (per_cu < 1) return set_err(c, KG_UNEXPECTED, "system too large for on-chip memory");
+    if (sys_bytes(l) + red > LDS_MAX) return set_err(c, KG_UNEXPECTED, "system too large for on-chip memory");
+    const size_t lb = lin_ok ? lin_need_one(c, s_tr, i) : 0;
+    int per_cu = (int)std::min<size_t>(8, LDS_MAX / (lin_ok ? lb : sys_bytes(l) + red));
     cls[per_cu].push_back(i);
     Lay& b = clay.emplace(per_cu, l).first->second;
     b.nnz_max = std::max(b.nnz_max, l.nnz_max);
     b.n_max = std::max(b.n_max, l.n_max);
     b.ne_max = std::max(b.ne_max, l.ne_max);
     cplan[per_cu] = std::max(cplan[per_cu], pe.lay.total);
+    clin[per_cu] = std::max(clin[per_cu], lb);
   }
   struct Bucket {
     Lay lay;
     Launch L;
-    int tps = 0;  // 1000 + t: plan staged in LDS
+    int tps = 0;  // 1000 + t: plan staged in LDS; 3000 + t: k_tran_lin
     int32_t* list = nullptr;
     int32_t n = 0;
     int plan_ints = 0;
+    size_t lin = 0;
   };
   std::vector<Bucket> bk;
   for (auto& kv : cls) {
@@ -599,6 +647,7 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     b.lay = clay[kv.first];
     b.n = (int32_t)kv.second.size();
     b.plan_ints = cplan[kv.first];
+    b.lin = clin[kv.first];
     hipError_t e;
     b.list = (int32_t*)ws_get(c, "tran_bucket" + std::to_string(kv.first), sizeof(int32_t) * kv.second.size(), &e);
     if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
@@ -609,6 +658,13 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.lay.nnz_max > b.lay.nnz_max; });
   BatchDev B0 = batch_dev(c, P);
   auto launch = [&](const Bucket& b, int cand, hipStream_t st) -> hipError_t {
+    if (is_lin_cand(cand)) {
+      BatchDev B = B0;
+      B.inst_list = b.list;
+      B.n_list = b.n;
+      return run_tran_lin(cand - 3000, dim3((unsigned)b.n), b.lin, st, B, 0, ta, d_xop, d_xoff, d_out, d_off, d_st,
+                          d_iters, d_err);
+    }
     Launch L;
     int tps = cand % 1000, pi = cand >= 1000 ? b.plan_ints : 0;
     if (configure(c, tps, b.lay, b.n, L, pi ? lay_al(4 * (size_t)pi) : 0) != KG_OK)
@@ -625,10 +681,15 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     Pb.n_max = b.lay.n_max;
     Pb.nnz_max = b.lay.nnz_max;
     Pb.ne_max = b.lay.ne_max;
-    r = autotune(c, tune_key(c, "tran", precision, Pb, b.n), {64, 256, 1064, 1256}, [&](int t, float* ms) -> int32_t {
-      Launch L;
-      int32_t rr = configure(c, t % 1000, b.lay, b.n, L, t >= 1000 ? lay_al(4 * (size_t)b.plan_ints) : 0);
-      if (rr) return rr;
+    std::vector<int> cands = {64, 256, 1064, 1256};
+    if (lin_ok && b.lin <= LDS_MAX) cands.insert(cands.end(), LIN_CANDS.begin(), LIN_CANDS.end());
+    r = autotune(c, tune_key(c, lin_ok ? "tranl" : "tran", precision, Pb, b.n), cands,
+                 [&](int t, float* ms) -> int32_t {
+      if (!is_lin_cand(t)) {
+        Launch L;
+        int32_t rr = configure(c, t % 1000, b.lay, b.n, L, t >= 1000 ? lay_al(4 * (size_t)b.plan_ints) : 0);
+        if (rr) return rr;
+      }
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
       return time_launch(c, [&] { return launch(b, t, c->stream); }, ms);
     }, &b.tps);
@@ -640,7 +701,10 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     c->aux.push_back(s);
   }
-  return timed(c, mixed ? "k_tran<float,double>" : "k_tran<double,double>", [&]() -> hipError_t {
+  bool all_lin = true;
+  for (auto& b : bk) all_lin = all_lin && is_lin_cand(b.tps);
+  const char* kname = mixed ? "k_tran<float,double>" : all_lin ? "k_tran_lin" : "k_tran<double,double>";
+  return timed(c, kname, [&]() -> hipError_t {
     hipEvent_t fork;
     hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
     if (e != hipSuccess) return e;
@@ -1110,10 +1174,31 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
     P = P0;
   } else {
   int cand = cfg->threads_per_system;
+  // k_tran_lin (3000 + threads per block): FP64, linear topologies with a compact plan
+  size_t lin = 0;
+  auto lin_setup = [&](Plan& PP) -> int32_t {
+    int32_t rr = prepare(c, AK_TRAN, ORD_NESTED, s_tr, PP);
+    if (rr) return rr;
+    lin = mixed ? 0 : lin_need_all(c, s_tr);
+    if (lin == 0 || lin > LDS_MAX)
+      return set_err(c, KG_UNEXPECTED, "the LDS-resident kernel needs FP64 and a linear topology that fits LDS");
+    return KG_OK;
+  };
   if (cand == 0) {
-    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), {1, 4, 16, 64, 256, 1064, 1256},
-                 [&](int t, float* ms) -> int32_t {
+    std::vector<int> cands = {1, 4, 16, 64, 256, 1064, 1256};
+    cands.insert(cands.end(), LIN_CANDS.begin(), LIN_CANDS.end());
+    r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), cands, [&](int t, float* ms) -> int32_t {
       Plan PP;
+      if (is_lin_cand(t)) {
+        int32_t rr = lin_setup(PP);
+        if (rr) return rr;
+        BatchDev B = batch_dev(c, PP);
+        HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
+        return time_launch(c, [&] {
+          return run_tran_lin(t - 3000, dim3((unsigned)c->n_inst), lin, c->stream, B, 0, ta, d_xop, d_xoff, d_out,
+                              d_off, d_st, d_iters, d_err);
+        }, ms);
+      }
       Lay lay;
       Launch L;
       int pi = 0;
@@ -1130,8 +1215,18 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
   }
+  if (is_lin_cand(cand)) {
+    if ((r = lin_setup(P))) return r == KG_UNEXPECTED ? set_err(c, KG_INVALID_ARGUMENT, c->err) : r;
+    BatchDev B = batch_dev(c, P);
+    r = timed(c, "k_tran_lin", [&] {
+      return run_tran_lin(cand - 3000, dim3((unsigned)c->n_inst), lin, c->stream, B, 0, ta, d_xop, d_xoff, d_out,
+                          d_off, d_st, d_iters, d_err);
+    });
+    if (r) return r;
+  } else {
   if (!valid_tps(cand % 1000) || (cand >= 1000 && cand != 1064 && cand != 1256))
-    return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64, 256, 1064 or 1256");
+    return set_err(c, KG_INVALID_ARGUMENT,
+                   "threads_per_system must be 1, 4, 16, 64, 256, 1064, 1256, 3064, 3128, 3256 or 3512");
   Lay lay;
   Launch L;
   int pi = 0;
@@ -1142,6 +1237,7 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
                     d_off, d_st, d_iters, d_err, lay, pi);
   });
   if (r) return r;
+  }
   }
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if (iters && (r = copy_out(c, iters, d_iters, sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), mem))) return r;
@@ -1197,16 +1293,25 @@ int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launche
 int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind, int32_t ordering,
                        const double* params, double s_rep, int32_t* buf, int32_t cap, int32_t* layout,
                        int32_t layout_cap) {
-  if (!el || ne <= 0 || n <= 0 || kind < 0 || kind >= AK_COUNT || ordering < 0 || ordering > ORD_BALANCED)
+  // kind 3: the compact plan of the LDS-resident linear transient kernel
+  const bool lin = kind == 3;
+  if (!el || ne <= 0 || n <= 0 || kind < 0 || kind > 3 || ordering < 0 || ordering > ORD_BALANCED)
     return -KG_INVALID_ARGUMENT;
   PatLayout L;
   std::string err;
-  std::vector<int32_t> b = build_pattern(el, ne, n, (AnalysisKind)kind, (Ordering)ordering, params, s_rep, L, err);
+  std::vector<int32_t> b =
+      build_pattern(el, ne, n, lin ? AK_TRAN : (AnalysisKind)kind, (Ordering)ordering, params, s_rep, L, err);
   if (layout) {
     int32_t m = std::min<int32_t>(layout_cap, (int32_t)(sizeof(PatLayout) / sizeof(int32_t)));
     std::memcpy(layout, &L, sizeof(int32_t) * m);
   }
   if (L.status != KG_OK) return -L.status;
+  if (lin) {
+    std::vector<uint16_t> w;
+    if (!build_lin_plan(L, b, w)) return -KG_UNEXPECTED;
+    if (buf) std::memcpy(buf, w.data(), sizeof(uint16_t) * std::min<size_t>(2 * (size_t)cap, w.size()));
+    return (int32_t)(w.size() / 2);
+  }
   if (buf) std::memcpy(buf, b.data(), sizeof(int32_t) * std::min<size_t>(cap, b.size()));
   return (int32_t)b.size();
 }

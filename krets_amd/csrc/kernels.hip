@@ -925,6 +925,184 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
 }
 
 
+// Linear fixed-step transient with the whole step loop in LDS, one system per
+// block (TPS threads). LDS holds the compact 16-bit plan (LinHdr), the LU
+// factors, two solution buffers used alternately as x_n and x_{n-1}, and the
+// coefficient and state of every capacitor and inductor companion; source
+// parameters are copied in once. The constant matrix is stamped and factored
+// once from the full plan. Each step then assembles the right-hand side,
+// solves by level-scheduled substitution and updates the companion states,
+// touching global memory only to store its solution row (coalesced, krets
+// numbering). Same operations in the same order as k_tran on a linear
+// circuit (solver/transient.rs:14-114 semantics, trapezoidal additions).
+template <int TPS>
+__global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, TranArgs ta, const double* xop,
+                                                  const int64_t* xop_off, double* out, const int64_t* out_off,
+                                                  int32_t* status, int32_t* iters, double* err) {
+  extern __shared__ __align__(16) char smem[];
+  const int t = threadIdx.x;
+  const int nslot = B.inst_list ? B.n_list : B.n_instances;
+  const int64_t slot = xcd_block();
+  if (slot >= nslot) return;
+  const int inst = B.inst_list ? B.inst_list[slot] : (int)slot;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+  if (status[inst] != KG_OK) {  // failed operating point or no plan: nothing to integrate
+    if (t == 0 && err) err[inst] = 0.0;
+    return;
+  }
+  if (P.lin == nullptr) {  // the host launches only topologies with a compact plan
+    if (t == 0) status[inst] = KG_UNEXPECTED;
+    return;
+  }
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(P.lin);
+    uint4* dst = reinterpret_cast<uint4*>(smem);
+    for (int i = t; i < P.lin_words / 8; i += TPS) dst[i] = src[i];
+  }
+  __syncthreads();
+  const LinHdr* H = reinterpret_cast<const LinHdr*>(smem);
+  const uint16_t* lw = reinterpret_cast<const uint16_t*>(smem);
+  const int n = H->n, nnz = H->nnz, nlev = H->nlev, nblev = H->nblev, ndyn = H->ndyn, nsrc = H->nsrc;
+  const uint16_t* lvl_ptr = lw + H->o_lvl_ptr;
+  const uint16_t* lvl_rows = lw + H->o_lvl_rows;
+  const uint16_t* blvl_ptr = lw + H->o_blvl_ptr;
+  const uint16_t* blvl_rows = lw + H->o_blvl_rows;
+  const uint16_t* row_ptr = lw + H->o_row_ptr;
+  const uint16_t* diag = lw + H->o_diag;
+  const uint16_t* col = lw + H->o_col;
+  const uint16_t* cpos = lw + H->o_cpos;
+  const uint16_t* rcon_ptr = lw + H->o_rcon_ptr;
+  const uint16_t* rcon = lw + H->o_rcon;
+  const uint16_t* dyn = lw + H->o_dyn;
+  const uint16_t* src = lw + H->o_src;
+  const int lev0 = H->lev0_free ? 1 : 0;
+  char* q = smem + lay_al(2 * (size_t)P.lin_words);
+  double* v = reinterpret_cast<double*>(q);
+  q += 8 * (size_t)nnz;
+  double* xc = reinterpret_cast<double*>(q);  // x_n
+  q += 8 * (size_t)n;
+  double* xq = reinterpret_cast<double*>(q);  // x_{n-1}
+  q += 8 * (size_t)n;
+  double* ed = reinterpret_cast<double*>(q);  // capacitor current / inductor voltage
+  q += 8 * (size_t)ndyn;
+  double* cf = reinterpret_cast<double*>(q);  // alpha * value / h
+  q += 8 * (size_t)ndyn;
+  double* k2 = reinterpret_cast<double*>(q);  // 2 * C / h (trapezoidal state update)
+  q += 8 * (size_t)ndyn;
+  double* sp = reinterpret_cast<double*>(q);  // source parameter records
+  q += 8 * (size_t)KG_NPARAM * nsrc;
+  int* flags = reinterpret_cast<int*>(q);
+  if (t == 0) flags[0] = KG_OK;
+  __syncthreads();
+  const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
+  const double* eb = B.ebase + B.eoff[inst];
+  StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
+  stamp_matrix<double, double, TPS>(P, v, prm, eb, sa, ed, t, true);
+  __syncthreads();
+  factor<double, TPS>(P, v, t, true, &flags[0]);
+  for (int d = t; d < ndyn; d += TPS) {
+    const double val = prm[(int)dyn[d * LIN_DYN + 1] * KG_NPARAM];
+    cf[d] = ta.alpha * (val / ta.h);
+    k2[d] = 2.0 * val / ta.h;
+  }
+  for (int i = t; i < nsrc * KG_NPARAM; i += TPS)
+    sp[i] = prm[(int)src[(i / KG_NPARAM) * LIN_SRC + 1] * KG_NPARAM + i % KG_NPARAM];
+  const double* x0 = xop + xop_off[inst];
+  double* o = out + out_off[inst];
+  for (int j = t; j < n; j += TPS) {
+    const double xv = x0[j];
+    xq[cpos[j]] = xv;
+    o[j] = xv;
+  }
+  if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1)] = 0;
+  __syncthreads();
+  if (flags[0] != KG_OK) {
+    if (t == 0) {
+      status[inst] = flags[0];
+      if (err) err[inst] = 0.0;
+    }
+    return;
+  }
+  auto vdiff = [&](const double* x, const uint16_t* r) {
+    return (r[2] != LIN_NONE ? x[r[2]] : 0.0) - (r[3] != LIN_NONE ? x[r[3]] : 0.0);
+  };
+  for (int d = t; d < ndyn; d += TPS) {
+    const uint16_t* r = dyn + d * LIN_DYN;
+    ed[d] = r[0] == KG_L ? vdiff(xq, r) : 0.0;
+  }
+  __syncthreads();
+  const bool compat = ta.method == KG_BE_COMPAT, trap = ta.method == KG_TRAP;
+  const bool with_state = ta.alpha != 1.0;
+  for (int s = 1; s <= ta.nsteps; s++) {
+    const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
+    const double tsrc = compat ? tprev : tn;  // krets evaluates sources at the guess's time
+    // right-hand side (permuted rows)
+    for (int i = t; i < n; i += TPS) {
+      double acc = 0.0;
+      for (int k = rcon_ptr[i]; k < rcon_ptr[i + 1]; k++) {
+        const int code = rcon[k], idx = code >> 2;
+        double val;
+        if ((code & 3) == LK_SRC) {
+          const double* pe = sp + idx * KG_NPARAM;
+          val = src[idx * LIN_SRC] == KG_V ? wave_value(pe, tsrc) : pe[0];
+        } else if ((code & 3) == LK_IND) {
+          const uint16_t* r = dyn + idx * LIN_DYN;
+          double rr = -cf[idx] * xq[r[4]];
+          if (with_state) rr -= ed[idx];
+          val = rr;
+        } else {
+          double ieq = -cf[idx] * vdiff(xq, dyn + idx * LIN_DYN);
+          if (with_state) ieq -= ed[idx];
+          val = (code & 3) == LK_CP ? -ieq : ieq;
+        }
+        acc += val;
+      }
+      xc[i] = acc;
+    }
+    __syncthreads();
+    // L y = b (level 0 rows, when free of lower entries, are already done)
+    for (int l = lev0; l < nlev; l++) {
+      for (int k = lvl_ptr[l] + t; k < lvl_ptr[l + 1]; k += TPS) {
+        const int i = lvl_rows[k];
+        double sum = xc[i];
+        const int d = diag[i];
+        for (int p = row_ptr[i]; p < d; p++) sum -= v[p] * xc[col[p]];
+        xc[i] = sum;
+      }
+      __syncthreads();
+    }
+    // U x = y (diagonal stored as its reciprocal)
+    for (int l = 0; l < nblev; l++) {
+      for (int k = blvl_ptr[l] + t; k < blvl_ptr[l + 1]; k += TPS) {
+        const int i = blvl_rows[k];
+        double sum = xc[i];
+        const int d = diag[i], e = row_ptr[i + 1];
+        for (int p = d + 1; p < e; p++) sum -= v[p] * xc[col[p]];
+        xc[i] = sum * v[d];
+      }
+      __syncthreads();
+    }
+    // trapezoidal companion states, then the solution row in krets numbering
+    if (trap)
+      for (int d = t; d < ndyn; d += TPS) {
+        const uint16_t* r = dyn + d * LIN_DYN;
+        const double vn = vdiff(xc, r);
+        ed[d] = r[0] == KG_C ? k2[d] * (vn - vdiff(xq, r)) - ed[d] : vn;
+      }
+    double* os = o + (int64_t)s * n;
+    for (int j = t; j < n; j += TPS) os[j] = xc[cpos[j]];
+    if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1) + s] = 1;
+    __syncthreads();
+    double* sw = xc;
+    xc = xq;
+    xq = sw;
+  }
+  if (t == 0) {
+    status[inst] = flags[0];
+    if (err) err[inst] = 0.0;
+  }
+}
+
 // Backward Euler with local-truncation-error step control (clean mode; no
 // krets counterpart). Each system advances its own time grid; the LTE of
 // every capacitor voltage and inductor current is estimated from the
@@ -1765,6 +1943,23 @@ hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t 
   }
   KG_TPS_SWITCH(tps, launch(k_tran<double, double, T, false>, g, b, lds, st, B, kind_slot, na, ta, xop, xop_off, out,
                             out_off, status, iters, err, lay, 0))
+}
+
+hipError_t run_tran_lin(int tps, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                        const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
+                        int32_t* status, int32_t* iters, double* err) {
+  const dim3 b(tps);
+  switch (tps) {
+    case 64: return launch(k_tran_lin<64>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
+                           iters, err);
+    case 128: return launch(k_tran_lin<128>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
+                            iters, err);
+    case 256: return launch(k_tran_lin<256>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
+                            iters, err);
+    case 512: return launch(k_tran_lin<512>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
+                            iters, err);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace kg

@@ -48,6 +48,12 @@ __host__ __device__ inline size_t sys_bytes(const Lay& L) {
   return s;
 }
 
+// LDS bytes of one system of the linear transient kernel.
+__host__ __device__ inline size_t lin_bytes(const LinHdr& h) {
+  return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 3 * (size_t)h.ndyn) +
+         8 * (size_t)KG_NPARAM * h.nsrc + 64;
+}
+
 struct NewtonArgs {
   double reltol, iabstol, vabstol;
   int maxiter, compat;
@@ -98,6 +104,11 @@ hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hip
                              int kind_slot, const double* freqs, int nf, const double* xop, const int64_t* xop_off,
                              double* out, const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
                              int plan_ints, int64_t chunk);
+// Linear fixed-step transient from LDS (k_tran_lin), one system per block of
+// tps threads (64, 128, 256, 512); lds >= lin_bytes of every launched system.
+hipError_t run_tran_lin(int tps, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                        const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
+                        int32_t* status, int32_t* iters, double* err);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay,

@@ -617,6 +617,94 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   return buf;
 }
 
+// Compact plan for the LDS-resident linear transient kernel (see LinHdr).
+// Returns false, leaving `out` empty, when the topology does not qualify:
+// diodes present, or indices that do not fit 16 bits.
+bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vector<uint16_t>& out) {
+  out.clear();
+  const int n = L.n, nnz = L.nnz, ne = L.ne;
+  if (L.status != KG_OK || L.has_diode || L.nonlinear || n >= 0xFFFF || nnz >= 0xFFFF || ne >= 0xFFFF) return false;
+  const int32_t* et = b.data() + L.o_etype;
+  const int32_t* ep = b.data() + L.o_ep;
+  const int32_t* em = b.data() + L.o_em;
+  const int32_t* ec = b.data() + L.o_ec;
+  const int32_t* rcp = b.data() + L.o_rcon_ptr;
+  const int32_t* rc = b.data() + L.o_rcon;
+  // dyn (C, L) and src (V, I) indices per element, in element order
+  std::vector<int32_t> dyn_of(ne, -1), src_of(ne, -1);
+  std::vector<uint16_t> dyn, src;
+  auto u16 = [](int32_t v) { return v < 0 ? LIN_NONE : (uint16_t)v; };
+  for (int e = 0; e < ne; e++) {
+    if (et[e] == KG_C || et[e] == KG_L) {
+      dyn_of[e] = (int32_t)(dyn.size() / LIN_DYN);
+      dyn.insert(dyn.end(), {(uint16_t)et[e], (uint16_t)e, u16(ep[e]), u16(em[e]), u16(ec[e]), 0});
+    } else if (et[e] == KG_V || et[e] == KG_I) {
+      src_of[e] = (int32_t)(src.size() / LIN_SRC);
+      src.insert(src.end(), {(uint16_t)et[e], (uint16_t)e});
+    }
+  }
+  const int ndyn = (int)dyn.size() / LIN_DYN, nsrc = (int)src.size() / LIN_SRC;
+  if (ndyn >= (1 << 14) || nsrc >= (1 << 14) || rcp[n] >= 0xFFFF) return false;
+  std::vector<uint16_t> rcon;
+  for (int q = 0; q < rcp[n]; q++) {
+    int e = rc[q] >> 3, k = rc[q] & 7;
+    int code;
+    if (k == RK_SRC && src_of[e] >= 0) code = (src_of[e] << 2) | LK_SRC;
+    else if (k == RK_CP && dyn_of[e] >= 0) code = (dyn_of[e] << 2) | LK_CP;
+    else if (k == RK_CM && dyn_of[e] >= 0) code = (dyn_of[e] << 2) | LK_CM;
+    else if (k == RK_IND && dyn_of[e] >= 0) code = (dyn_of[e] << 2) | LK_IND;
+    else return false;  // diode companions never reach here (has_diode)
+    rcon.push_back((uint16_t)code);
+  }
+  LinHdr h{};
+  h.n = n;
+  h.nnz = nnz;
+  h.nlev = L.nlev;
+  h.nblev = L.nblev;
+  h.ndyn = ndyn;
+  h.ncon = (int32_t)rcon.size();
+  h.nsrc = nsrc;
+  const size_t hw = sizeof(LinHdr) / 2;
+  std::vector<uint16_t> w(hw, 0);
+  auto put = [&](const uint16_t* p, size_t k) {
+    while (w.size() % 2) w.push_back(0);  // 4-byte aligned arrays
+    int32_t off = (int32_t)w.size();
+    w.insert(w.end(), p, p + k);
+    return off;
+  };
+  auto put32 = [&](const int32_t* p, size_t k) {
+    std::vector<uint16_t> t(k);
+    for (size_t i = 0; i < k; i++) t[i] = (uint16_t)p[i];
+    return put(t.data(), k);
+  };
+  h.o_lvl_ptr = put32(b.data() + L.o_lvl_ptr, L.nlev + 1);
+  h.o_lvl_rows = put32(b.data() + L.o_lvl_rows, n);
+  h.o_blvl_ptr = put32(b.data() + L.o_blvl_ptr, L.nblev + 1);
+  h.o_blvl_rows = put32(b.data() + L.o_blvl_rows, n);
+  h.o_row_ptr = put32(b.data() + L.o_row_ptr, n + 1);
+  h.o_diag = put32(b.data() + L.o_diag, n);
+  h.o_col = put32(b.data() + L.o_col, nnz);
+  h.o_cpos = put32(b.data() + L.o_cpos, n);
+  h.o_rcon_ptr = put32(rcp, n + 1);
+  h.o_rcon = put(rcon.data(), rcon.size());
+  h.o_dyn = put(dyn.data(), dyn.size());
+  h.o_src = put(src.data(), src.size());
+  while (w.size() % 8) w.push_back(0);  // 16-byte multiple
+  h.words = (int32_t)w.size();
+  // level-0 forward rows have no strictly lower entries: their forward step
+  // is the right-hand side itself
+  const int32_t* lp = b.data() + L.o_lvl_ptr;
+  const int32_t* lr = b.data() + L.o_lvl_rows;
+  const int32_t* rp = b.data() + L.o_row_ptr;
+  const int32_t* dg = b.data() + L.o_diag;
+  h.lev0_free = 1;
+  for (int q = lp[0]; q < (L.nlev > 0 ? lp[1] : 0); q++)
+    if (dg[lr[q]] != rp[lr[q]]) h.lev0_free = 0;
+  std::memcpy(w.data(), &h, sizeof(LinHdr));
+  out.swap(w);
+  return true;
+}
+
 // Checks every index a kernel will dereference before the plan is uploaded.
 bool validate_plan(const PatLayout& L, const std::vector<int32_t>& b, std::string& err) {
   const int n = L.n, nnz = L.nnz, ne = L.ne;

@@ -174,7 +174,12 @@ class Context:
             self._h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals may already be gone; the
+        # process then releases the device memory itself
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def _check(self, st):
         if st != _lib.OK:

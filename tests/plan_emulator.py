@@ -5,6 +5,7 @@ solve steps the gfx950 kernels run, in numpy, so a plan can be checked on a
 machine without a GPU.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -130,3 +131,152 @@ class Plan:
             for i in reversed(lev):
                 w[i] -= sum(v[lt[q + 1]] * w[lt[q]] for q in range(ltp[i], ltp[i + 1], 2))
         return w
+
+
+# ------------------------------------------------------------- compact plan
+LIN_HDR = ["n", "nnz", "nlev", "nblev", "ndyn", "ncon", "nsrc", "words", "o_lvl_ptr", "o_lvl_rows", "o_blvl_ptr",
+           "o_blvl_rows", "o_row_ptr", "o_diag", "o_col", "o_cpos", "o_rcon_ptr", "o_rcon", "o_dyn", "o_src",
+           "lev0_free"]
+LIN_HDR_WORDS = 48  # sizeof(LinHdr) / 2
+LIN_DYN, LIN_SRC, LIN_NONE = 6, 2, 0xFFFF
+LK_SRC, LK_CP, LK_CM, LK_IND = 0, 1, 2, 3
+KG_C, KG_L, KG_V = 1, 2, 3
+LIN = 3  # kg_plan_export kind of the compact plan
+
+
+class LinPlan:
+    """The compact 16-bit plan of the LDS-resident linear transient kernel
+    (kg_plan_export kind 3), split into its arrays."""
+
+    def __init__(self, elements, n, params, s_rep):
+        el = np.ascontiguousarray(elements, _lib.ELEMENT_DTYPE)
+        pr = np.ascontiguousarray(params, np.float64)
+        L = _lib.lib()
+        size = L.kg_plan_export(el.ctypes.data, len(el), n, LIN, NESTED, pr.ctypes.data, s_rep, None, 0, None, 0)
+        self.status = -size if size < 0 else 0
+        if size < 0:
+            return
+        buf = np.zeros(size, np.int32)
+        L.kg_plan_export(el.ctypes.data, len(el), n, LIN, NESTED, pr.ctypes.data, s_rep, buf.ctypes.data, size,
+                         None, 0)
+        self.w = buf.view(np.uint16).astype(np.int64)
+        self.h = {k: int(v) for k, v in zip(LIN_HDR, buf[:len(LIN_HDR)])}
+        h = self.h
+        assert h["words"] == len(self.w) and h["words"] % 8 == 0
+
+        def a(name, k):
+            o = h["o_" + name]
+            assert LIN_HDR_WORDS <= o and o + k <= h["words"], name
+            return self.w[o:o + k]
+        n, nnz = h["n"], h["nnz"]
+        self.lvl_ptr, self.lvl_rows = a("lvl_ptr", h["nlev"] + 1), a("lvl_rows", n)
+        self.blvl_ptr, self.blvl_rows = a("blvl_ptr", h["nblev"] + 1), a("blvl_rows", n)
+        self.row_ptr, self.diag, self.col = a("row_ptr", n + 1), a("diag", n), a("col", nnz)
+        self.cpos, self.rcon_ptr = a("cpos", n), a("rcon_ptr", n + 1)
+        self.rcon = a("rcon", h["ncon"])
+        self.dyn = a("dyn", LIN_DYN * h["ndyn"]).reshape(-1, LIN_DYN)
+        self.src = a("src", LIN_SRC * h["nsrc"]).reshape(-1, LIN_SRC)
+
+    def check_indices(self, ne):
+        """Every index the kernel dereferences, against the buffer it indexes."""
+        h, n, nnz = self.h, self.h["n"], self.h["nnz"]
+        for ptr, rows, nl in ((self.lvl_ptr, self.lvl_rows, h["nlev"]), (self.blvl_ptr, self.blvl_rows, h["nblev"])):
+            assert ptr[0] == 0 and ptr[nl] == n and np.all(np.diff(ptr) >= 0)
+            assert sorted(rows.tolist()) == list(range(n))
+        assert self.row_ptr[0] == 0 and self.row_ptr[n] == nnz and np.all(np.diff(self.row_ptr) >= 0)
+        assert np.all((self.diag >= self.row_ptr[:-1]) & (self.diag < self.row_ptr[1:]))
+        assert np.all(self.col < n) and sorted(self.cpos.tolist()) == list(range(n))
+        assert self.rcon_ptr[0] == 0 and self.rcon_ptr[n] == h["ncon"] and np.all(np.diff(self.rcon_ptr) >= 0)
+        for code in self.rcon:
+            k, idx = int(code) & 3, int(code) >> 2
+            assert idx < (h["nsrc"] if k == LK_SRC else h["ndyn"])
+            if k == LK_IND:
+                assert self.dyn[idx, 0] == KG_L
+            elif k != LK_SRC:
+                assert self.dyn[idx, 0] == KG_C
+        for r in self.dyn:
+            assert r[0] in (KG_C, KG_L) and r[1] < ne
+            assert all(v == LIN_NONE or v < n for v in r[2:5])
+            assert r[0] != KG_L or r[4] < n
+        for r in self.src:
+            assert r[0] in (KG_V, 4) and r[1] < ne
+        if h["lev0_free"]:
+            for i in self.lvl_rows[self.lvl_ptr[0]:self.lvl_ptr[1]]:
+                assert self.row_ptr[i] == self.diag[i]
+
+
+def wave_value(p, t):
+    """The device's PULSE / SIN evaluation (voltage_source.rs:50-93)."""
+    kind, w = int(p[2]), p[3:]
+    if kind == 1:
+        v1, v2, td, tr, tf, pw, per = w[:7]
+        if t < td:
+            return v1
+        tt = math.fmod(t - td, per)
+        if tt < tr:
+            return v1 + (v2 - v1) * (tt / tr)
+        if tt < tr + pw:
+            return v2
+        if tt < tr + pw + tf:
+            return v2 + (v1 - v2) * ((tt - tr - pw) / tf)
+        return v1
+    if kind == 2:
+        vo, va, f, td, th, ph = w[:6]
+        if t < td:
+            return vo
+        return vo + va * math.exp(-th * (t - td)) * math.sin(2.0 * math.pi * f * (t - td) + ph * (math.pi / 180.0))
+    return p[0]
+
+
+def tran_lin(lp, vf, params, x0, h, nsteps, method):
+    """The step loop of the LDS-resident linear transient kernel on the CPU:
+    vf are the factored values (reciprocal diagonal), x0 the operating point
+    in krets numbering. Returns [nsteps + 1][n] in krets numbering."""
+    n = lp.h["n"]
+    alpha = 2.0 if method == 2 else 1.0
+    compat, trap = method == 0, method == 2
+    vals = np.array([params[int(r[1]), 0] for r in lp.dyn])
+    cf = alpha * (vals / h)
+    k2 = 2.0 * vals / h
+    sp = [params[int(r[1])] for r in lp.src]
+    xq = np.zeros(n)
+    xq[lp.cpos] = x0
+    out = np.zeros((nsteps + 1, n))
+    out[0] = x0
+
+    def vd(x, r):
+        return (x[r[2]] if r[2] != LIN_NONE else 0.0) - (x[r[3]] if r[3] != LIN_NONE else 0.0)
+    ed = np.array([vd(xq, r) if r[0] == KG_L else 0.0 for r in lp.dyn])
+    lev0 = 1 if lp.h["lev0_free"] else 0
+    for s in range(1, nsteps + 1):
+        tsrc = (s - 1) * h if compat else s * h
+        xc = np.zeros(n)
+        for i in range(n):
+            acc = 0.0
+            for code in lp.rcon[lp.rcon_ptr[i]:lp.rcon_ptr[i + 1]]:
+                k, idx = int(code) & 3, int(code) >> 2
+                if k == LK_SRC:
+                    acc += wave_value(sp[idx], tsrc) if lp.src[idx, 0] == KG_V else sp[idx][0]
+                elif k == LK_IND:
+                    rr = -cf[idx] * xq[lp.dyn[idx, 4]]
+                    acc += rr - ed[idx] if alpha != 1.0 else rr
+                else:
+                    ieq = -cf[idx] * vd(xq, lp.dyn[idx])
+                    if alpha != 1.0:
+                        ieq -= ed[idx]
+                    acc += -ieq if k == LK_CP else ieq
+            xc[i] = acc
+        for l in range(lev0, lp.h["nlev"]):
+            for i in lp.lvl_rows[lp.lvl_ptr[l]:lp.lvl_ptr[l + 1]]:
+                xc[i] -= sum(vf[p] * xc[lp.col[p]] for p in range(lp.row_ptr[i], lp.diag[i]))
+        for l in range(lp.h["nblev"]):
+            for i in lp.blvl_rows[lp.blvl_ptr[l]:lp.blvl_ptr[l + 1]]:
+                d = lp.diag[i]
+                xc[i] = (xc[i] - sum(vf[p] * xc[lp.col[p]] for p in range(d + 1, lp.row_ptr[i + 1]))) * vf[d]
+        if trap:
+            for j, r in enumerate(lp.dyn):
+                vn = vd(xc, r)
+                ed[j] = k2[j] * (vn - vd(xq, r)) - ed[j] if r[0] == KG_C else vn
+        out[s] = xc[lp.cpos]
+        xq = xc
+    return out

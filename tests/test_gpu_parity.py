@@ -250,6 +250,41 @@ def test_batched_transient_uneven_rlc(kg, oracle, method, precision, tps):
         assert e < 1e-9, (i, e)
 
 
+@pytest.mark.parametrize("tps", [3064, 3128, 3256, 3512])  # 3000 + t: whole step loop in LDS
+@pytest.mark.parametrize("method", [2, 1, 0])
+def test_lds_resident_transient_uneven_rlc(kg, oracle, method, tps):
+    from krets_amd import workloads as W
+    say(f"LDS-resident transient method={method} tps={tps}")
+    b = W.c3_rlc_uneven(5, smin=8, smax=40, seed=13)
+    ctx = kg.Context()
+    b.load(ctx)
+    h, n = 10e-12, 300
+    xs, st, (it, err) = ctx.transient(h, n, method=method, config=kg.SolverConfig(threads_per_system=tps))
+    assert (st == 0).all() and (it[:, 1:] == 1).all() and (it[:, 0] == 0).all() and (err == 0).all()
+    for i in range(5):
+        xr, _ = oracle.Circuit(b.netlists[i]).tran(h, n * h, method=method)
+        e = max(nerr(xs[i][k], xr[k]) for k in range(n + 1))
+        assert e < 1e-12, (i, e)
+    with pytest.raises(kg.KretsGpuError):  # FP64 only
+        ctx.transient(h, n, method=method, precision=kg.MIXED, config=kg.SolverConfig(threads_per_system=tps))
+
+
+def test_lds_resident_transient_largest_ladder(kg, oracle):
+    """512 RLC sections (1538 unknowns): the compact plan, factors and state
+    use 158.7 KB of the CU's 160 KB of LDS."""
+    say("LDS-resident transient, 512-section ladder")
+    from krets_amd import workloads as W
+    b = W.c3_rlc_uneven(2, smin=512, smax=512, seed=19)
+    ctx = kg.Context()
+    b.load(ctx)
+    h, n = 10e-12, 60
+    xs, st, _ = ctx.transient(h, n, method=kg.TRAP, config=kg.SolverConfig(threads_per_system=3512))
+    assert (st == 0).all()
+    for i in range(2):
+        xr, _ = oracle.Circuit(b.netlists[i]).tran(h, n * h, method=oracle.TRAP)
+        assert max(nerr(xs[i][k], xr[k]) for k in range(n + 1)) < 1e-12
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_bucketed_transient_uneven_rlc(kg, oracle, precision):
     """Autotuned uneven batch of 96 ladders, 8..300 sections: instances run in

@@ -177,3 +177,56 @@ def test_plan_structurally_singular():
     assert pe.Plan(c.elements, c.n, pe.AC, pe.NESTED).status == 2
     c = k.Circuit("V1 1 0 1\nR1 1 2 1\nC1 2 0 1u G2\n")  # G2 capacitor: empty branch row in DC
     assert pe.Plan(c.elements, c.n, pe.DC, pe.MINDEG).status == 2
+
+
+LIN_CASES = ["low_pass_filter/transient.cir", "high_pass_filter/transient.cir", "dual_rc_ladder/dual_rc_ladder.cir"]
+
+
+def lin_bytes(h):
+    al = lambda b: (b + 15) & ~15
+    return al(2 * h["words"]) + 8 * (h["nnz"] + 2 * h["n"] + 3 * h["ndyn"]) + 8 * 10 * h["nsrc"] + 64
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("case", LIN_CASES + ["ladder5", "ladder23", "c3"])
+def test_lin_plan_step_loop_vs_oracle(oracle, case, method):
+    """The compact plan of the LDS-resident transient kernel and its step
+    loop, run on the CPU, against the oracle's fixed-step transient."""
+    import krets_amd as k
+    from krets_amd import workloads as W
+    if case.startswith("ladder"):
+        text, h, N = ladder(int(case[6:])), 1e-11, 25
+    elif case == "c3":
+        b = W.c3_rlc_uneven(3, smin=8, smax=40, seed=11)
+        text, h, N = b.netlists[2], 10e-12, 25
+    else:
+        text, h, N = read_circuit(case), 50e-6, 25
+    c = k.Circuit(text)
+    alpha = 2.0 if method == 2 else 1.0
+    P = pe.Plan(c.elements, c.n, pe.TRAN, pe.NESTED, c.params, alpha / h)
+    lp = pe.LinPlan(c.elements, c.n, c.params, alpha / h)
+    assert P.status == 0 and lp.status == 0
+    assert lp.h["n"] == c.n and lp.h["nnz"] == P.nnz
+    lp.check_indices(len(c.elements))
+    contrib, _ = values(c, pe.TRAN, h=h, alpha=alpha)
+    v, _ = P.gather(contrib)
+    vf = P.factor(v).real
+    xr, _ = oracle.Circuit(text).tran(h, N * h, method=method)
+    x = pe.tran_lin(lp, vf, c.params, xr[0], h, N, method)
+    for s in range(N + 1):
+        d = np.linalg.norm(xr[s])
+        assert np.linalg.norm(x[s] - xr[s]) <= 1e-12 * max(d, 1e-300) + 1e-300, s
+
+
+def test_lin_plan_sizes():
+    """The largest ladder of the uneven transient batch (512 RLC sections)
+    fits one CU's 160 KB of LDS in the compact form; diode circuits have no
+    compact plan."""
+    import krets_amd as k
+    c = k.Circuit(ladder(512))
+    lp = pe.LinPlan(c.elements, c.n, c.params, 2e11)
+    assert lp.status == 0
+    lp.check_indices(len(c.elements))
+    assert lin_bytes(lp.h) <= 160 * 1024, lin_bytes(lp.h)
+    c = k.Circuit(read_circuit("rectifier/rectifier.cir"))
+    assert pe.LinPlan(c.elements, c.n, c.params, 1e5).status == 6  # KG_UNEXPECTED: not linear
