@@ -101,7 +101,7 @@ typedef struct {
    * on a balanced elimination order. Fixed-step transients also
    * take 1064 and 1256 (64 or 256 threads per system with the plan copied
    * into LDS once per launch) and, for FP64 on circuits without diodes,
-   * 3064 / 3128 / 3256 / 3512 (one system per block of 64..512 threads with
+   * 3000 + t, t = 64, 128, 256, 512, 1024 (one system per block of t threads with
    * the whole step loop in LDS: compact 16-bit plan, factors, solutions and
    * companion states). */
   int32_t threads_per_system;

@@ -583,6 +583,29 @@ size_t lin_need_one(kg_ctx* c, double s_tr, int i) {
   return lin_bytes(h);
 }
 
+// Most independent sources of any instance's compact plan.
+int lin_nsrc_max(kg_ctx* c, double s_tr) {
+  int m = 0;
+  for (int i = 0; i < c->n_inst; i++) {
+    auto it = c->pat_index.find(std::make_tuple(c->group_of[i], (int)AK_TRAN, (int)ORD_NESTED, s_tr));
+    if (it == c->pat_index.end() || c->pats[it->second].lin.empty()) continue;
+    LinHdr h;
+    std::memcpy(&h, c->pats[it->second].lin.data(), sizeof(h));
+    m = std::max(m, h.nsrc);
+  }
+  return m;
+}
+
+// Source-value table of k_tran_lin (n_instances * nsrc_max * (nsteps + 1)).
+double* lin_srcv(kg_ctx* c, double s_tr, int nsteps, int* nsrc_max, int32_t* r) {
+  *nsrc_max = lin_nsrc_max(c, s_tr);
+  hipError_t e;
+  double* p = (double*)ws_get(c, "tran_srcv",
+                              sizeof(double) * std::max<size_t>(1, (size_t)c->n_inst * *nsrc_max * (nsteps + 1)), &e);
+  *r = e == hipSuccess ? KG_OK : set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  return p;
+}
+
 // Largest need over all instances, 0 if any instance has no compact plan.
 size_t lin_need_all(kg_ctx* c, double s_tr) {
   size_t m = 0;
@@ -595,8 +618,14 @@ size_t lin_need_all(kg_ctx* c, double s_tr) {
 }
 
 // Work-mapping codes of k_tran_lin: 3000 + threads per block (one system per block).
-const std::vector<int> LIN_CANDS = {3064, 3128, 3256, 3512};
-bool is_lin_cand(int t) { return t == 3064 || t == 3128 || t == 3256 || t == 3512; }
+const std::vector<int> LIN_CANDS = {3064, 3128, 3256, 3512, 4024};
+bool is_lin_cand(int t) { return t == 3064 || t == 3128 || t == 3256 || t == 3512 || t == 4024; }
+// Rows per thread of k_tran_lin for systems of up to n_max unknowns; 0 if none fits.
+int lin_rows(int tps, int n_max) {
+  for (int r : {1, 2, 4})
+    if (n_max <= r * tps && (tps < 1024 || r <= 2)) return r;
+  return 0;
+}
 
 // Uneven transient batches: every block of one launch reserves LDS for the
 // largest instance, so a launch over 8..512-section ladders runs one block
@@ -657,13 +686,21 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   }
   std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.lay.nnz_max > b.lay.nnz_max; });
   BatchDev B0 = batch_dev(c, P);
+  int nsrc_max = 0;
+  double* srcv = nullptr;
+  if (lin_ok) {
+    srcv = lin_srcv(c, s_tr, ta.nsteps, &nsrc_max, &r);
+    if (r) return r;
+  }
   auto launch = [&](const Bucket& b, int cand, hipStream_t st) -> hipError_t {
     if (is_lin_cand(cand)) {
       BatchDev B = B0;
       B.inst_list = b.list;
       B.n_list = b.n;
-      return run_tran_lin(cand - 3000, dim3((unsigned)b.n), b.lin, st, B, 0, ta, d_xop, d_xoff, d_out, d_off, d_st,
-                          d_iters, d_err);
+      const int rows = lin_rows(cand - 3000, b.lay.n_max);
+      if (rows == 0) return hipErrorInvalidConfiguration;
+      return run_tran_lin(cand - 3000, rows, dim3((unsigned)b.n), b.lin, st, B, 0, ta, d_xop, d_xoff, d_out, d_off,
+                          d_st, d_iters, d_err, srcv, nsrc_max);
     }
     Launch L;
     int tps = cand % 1000, pi = cand >= 1000 ? b.plan_ints : 0;
@@ -689,6 +726,8 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
         Launch L;
         int32_t rr = configure(c, t % 1000, b.lay, b.n, L, t >= 1000 ? lay_al(4 * (size_t)b.plan_ints) : 0);
         if (rr) return rr;
+      } else if (lin_rows(t - 3000, b.lay.n_max) == 0) {
+        return set_err(c, KG_UNEXPECTED, "more than 4 rows per thread");
       }
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
       return time_launch(c, [&] { return launch(b, t, c->stream); }, ms);
@@ -1184,6 +1223,16 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
       return set_err(c, KG_UNEXPECTED, "the LDS-resident kernel needs FP64 and a linear topology that fits LDS");
     return KG_OK;
   };
+  auto lin_launch = [&](int cand, const Plan& PP, hipStream_t st) -> hipError_t {
+    const int rows = lin_rows(cand - 3000, PP.n_max);
+    if (rows == 0) return hipErrorInvalidConfiguration;
+    int nsrc_max = 0;
+    int32_t rr = KG_OK;
+    double* srcv = lin_srcv(c, s_tr, nsteps, &nsrc_max, &rr);
+    if (rr) return hipErrorOutOfMemory;
+    return run_tran_lin(cand - 3000, rows, dim3((unsigned)c->n_inst), lin, st, batch_dev(c, PP), 0, ta, d_xop,
+                        d_xoff, d_out, d_off, d_st, d_iters, d_err, srcv, nsrc_max);
+  };
   if (cand == 0) {
     std::vector<int> cands = {1, 4, 16, 64, 256, 1064, 1256};
     cands.insert(cands.end(), LIN_CANDS.begin(), LIN_CANDS.end());
@@ -1192,12 +1241,9 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
       if (is_lin_cand(t)) {
         int32_t rr = lin_setup(PP);
         if (rr) return rr;
-        BatchDev B = batch_dev(c, PP);
+        if (lin_rows(t - 3000, PP.n_max) == 0) return set_err(c, KG_UNEXPECTED, "more than 4 rows per thread");
         HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
-        return time_launch(c, [&] {
-          return run_tran_lin(t - 3000, dim3((unsigned)c->n_inst), lin, c->stream, B, 0, ta, d_xop, d_xoff, d_out,
-                              d_off, d_st, d_iters, d_err);
-        }, ms);
+        return time_launch(c, [&] { return lin_launch(t, PP, c->stream); }, ms);
       }
       Lay lay;
       Launch L;
@@ -1217,16 +1263,14 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   }
   if (is_lin_cand(cand)) {
     if ((r = lin_setup(P))) return r == KG_UNEXPECTED ? set_err(c, KG_INVALID_ARGUMENT, c->err) : r;
-    BatchDev B = batch_dev(c, P);
-    r = timed(c, "k_tran_lin", [&] {
-      return run_tran_lin(cand - 3000, dim3((unsigned)c->n_inst), lin, c->stream, B, 0, ta, d_xop, d_xoff, d_out,
-                          d_off, d_st, d_iters, d_err);
-    });
+    if (lin_rows(cand - 3000, P.n_max) == 0)
+      return set_err(c, KG_INVALID_ARGUMENT, "system too large for this block size (more than 4 rows per thread)");
+    r = timed(c, "k_tran_lin", [&] { return lin_launch(cand, P, c->stream); });
     if (r) return r;
   } else {
   if (!valid_tps(cand % 1000) || (cand >= 1000 && cand != 1064 && cand != 1256))
     return set_err(c, KG_INVALID_ARGUMENT,
-                   "threads_per_system must be 1, 4, 16, 64, 256, 1064, 1256, 3064, 3128, 3256 or 3512");
+                   "threads_per_system must be 1, 4, 16, 64, 256, 1064, 1256, 3064, 3128, 3256, 3512 or 4024");
   Lay lay;
   Launch L;
   int pi = 0;

@@ -926,19 +926,36 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
 
 
 // Linear fixed-step transient with the whole step loop in LDS, one system per
-// block (TPS threads). LDS holds the compact 16-bit plan (LinHdr), the LU
-// factors, two solution buffers used alternately as x_n and x_{n-1}, and the
-// coefficient and state of every capacitor and inductor companion; source
-// parameters are copied in once. The constant matrix is stamped and factored
-// once from the full plan. Each step then assembles the right-hand side,
-// solves by level-scheduled substitution and updates the companion states,
-// touching global memory only to store its solution row (coalesced, krets
-// numbering). Same operations in the same order as k_tran on a linear
-// circuit (solver/transient.rs:14-114 semantics, trapezoidal additions).
-template <int TPS>
+// block of TPS threads, every thread owning at most R rows (n <= R * TPS).
+// LDS holds the compact 16-bit plan (LinHdr), the LU factors, two solution
+// buffers used alternately as x_n and x_{n-1}, and the coefficient and state
+// of every capacitor and inductor companion; source parameters are copied in
+// once. Each array ends in a zero slot (index n, nnz or ndyn) that padded
+// work items point at, so that a thread's loads are unconditional. Source
+// values of every step are tabulated up front in srcv (nsrc_max * (nsteps + 1)
+// doubles per instance), so that no waveform code sits in the step loop; the
+// thread that owns a source row loads each value one step ahead.
+//
+// The constant matrix is stamped and factored once from the full plan. Then
+// every thread caches in registers what the plan says about its share of the
+// step: its right-hand-side rows and their first LIN_KC contributions, its
+// positions in the forward and backward level orders with their rows' first
+// LIN_KE column indices, its companion elements and its output columns. A
+// step is then: right-hand side, level-scheduled L and U substitutions, and
+// the companion-state update with the store of the solution row (coalesced,
+// krets numbering); within a phase a thread's LDS reads are independent, one
+// round trip. Entries beyond the cached ones are read from the plan in LDS.
+// Same operations in the same order as k_tran on a linear circuit
+// (solver/transient.rs:14-114 semantics, trapezoidal additions).
+
+constexpr int LIN_KE = 4;  // cached strictly-triangular entries per row
+constexpr int LIN_KC = 2;  // cached right-hand-side contributions per row
+
+template <int TPS, int R>
 __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, TranArgs ta, const double* xop,
                                                   const int64_t* xop_off, double* out, const int64_t* out_off,
-                                                  int32_t* status, int32_t* iters, double* err) {
+                                                  int32_t* status, int32_t* iters, double* err, double* srcv,
+                                                  int nsrc_max) {
   extern __shared__ __align__(16) char smem[];
   const int t = threadIdx.x;
   const int nslot = B.inst_list ? B.n_list : B.n_instances;
@@ -950,7 +967,7 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
     if (t == 0 && err) err[inst] = 0.0;
     return;
   }
-  if (P.lin == nullptr) {  // the host launches only topologies with a compact plan
+  if (P.lin == nullptr || P.n > R * TPS || (srcv == nullptr && nsrc_max > 0)) {  // launched only where it fits
     if (t == 0) status[inst] = KG_UNEXPECTED;
     return;
   }
@@ -964,11 +981,9 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
   const uint16_t* lw = reinterpret_cast<const uint16_t*>(smem);
   const int n = H->n, nnz = H->nnz, nlev = H->nlev, nblev = H->nblev, ndyn = H->ndyn, nsrc = H->nsrc;
   const uint16_t* lvl_ptr = lw + H->o_lvl_ptr;
-  const uint16_t* lvl_rows = lw + H->o_lvl_rows;
+  const ushort4* fdesc = reinterpret_cast<const ushort4*>(lw + H->o_fdesc);
   const uint16_t* blvl_ptr = lw + H->o_blvl_ptr;
-  const uint16_t* blvl_rows = lw + H->o_blvl_rows;
-  const uint16_t* row_ptr = lw + H->o_row_ptr;
-  const uint16_t* diag = lw + H->o_diag;
+  const ushort4* bdesc = reinterpret_cast<const ushort4*>(lw + H->o_bdesc);
   const uint16_t* col = lw + H->o_col;
   const uint16_t* cpos = lw + H->o_cpos;
   const uint16_t* rcon_ptr = lw + H->o_rcon_ptr;
@@ -977,22 +992,25 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
   const uint16_t* src = lw + H->o_src;
   const int lev0 = H->lev0_free ? 1 : 0;
   char* q = smem + lay_al(2 * (size_t)P.lin_words);
-  double* v = reinterpret_cast<double*>(q);
-  q += 8 * (size_t)nnz;
-  double* xc = reinterpret_cast<double*>(q);  // x_n
-  q += 8 * (size_t)n;
-  double* xq = reinterpret_cast<double*>(q);  // x_{n-1}
-  q += 8 * (size_t)n;
-  double* ed = reinterpret_cast<double*>(q);  // capacitor current / inductor voltage
-  q += 8 * (size_t)ndyn;
-  double* cf = reinterpret_cast<double*>(q);  // alpha * value / h
-  q += 8 * (size_t)ndyn;
-  double* k2 = reinterpret_cast<double*>(q);  // 2 * C / h (trapezoidal state update)
-  q += 8 * (size_t)ndyn;
+  double* v = reinterpret_cast<double*>(q);  // factors, then a zero slot
+  q += 8 * ((size_t)nnz + 1);
+  double* xc = reinterpret_cast<double*>(q);  // x_n, then a zero slot
+  q += 8 * ((size_t)n + 1);
+  double* xq = reinterpret_cast<double*>(q);  // x_{n-1}, then a zero slot
+  q += 8 * ((size_t)n + 1);
+  double* ed = reinterpret_cast<double*>(q);  // capacitor current / inductor voltage, then a zero slot
+  q += 8 * ((size_t)ndyn + 1);
+  double* cf = reinterpret_cast<double*>(q);  // alpha * value / h (= 2 C / h for the trapezoidal update), zero slot
+  q += 8 * ((size_t)ndyn + 1);
   double* sp = reinterpret_cast<double*>(q);  // source parameter records
   q += 8 * (size_t)KG_NPARAM * nsrc;
   int* flags = reinterpret_cast<int*>(q);
-  if (t == 0) flags[0] = KG_OK;
+  if (t == 0) {
+    flags[0] = KG_OK;
+    v[nnz] = 0.0;
+    xc[n] = xq[n] = 0.0;
+    ed[ndyn] = cf[ndyn] = 0.0;
+  }
   __syncthreads();
   const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
   const double* eb = B.ebase + B.eoff[inst];
@@ -1001,9 +1019,8 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
   __syncthreads();
   factor<double, TPS>(P, v, t, true, &flags[0]);
   for (int d = t; d < ndyn; d += TPS) {
-    const double val = prm[(int)dyn[d * LIN_DYN + 1] * KG_NPARAM];
+    const double val = prm[(int)(dyn[d * LIN_DYN] & 0x3FFF) * KG_NPARAM];
     cf[d] = ta.alpha * (val / ta.h);
-    k2[d] = 2.0 * val / ta.h;
   }
   for (int i = t; i < nsrc * KG_NPARAM; i += TPS)
     sp[i] = prm[(int)src[(i / KG_NPARAM) * LIN_SRC + 1] * KG_NPARAM + i % KG_NPARAM];
@@ -1023,74 +1040,207 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
     }
     return;
   }
-  auto vdiff = [&](const double* x, const uint16_t* r) {
-    return (r[2] != LIN_NONE ? x[r[2]] : 0.0) - (r[3] != LIN_NONE ? x[r[3]] : 0.0);
-  };
-  for (int d = t; d < ndyn; d += TPS) {
+  const uint32_t NN = (uint32_t)n;  // the zero solution slot stands for ground
+  auto node = [&](uint16_t c) -> uint32_t { return c == LIN_NONE ? NN : (uint32_t)c; };
+  for (int d = t; d < ndyn; d += TPS) {  // inductors start at v(x0), capacitors at 0 A
     const uint16_t* r = dyn + d * LIN_DYN;
-    ed[d] = r[0] == KG_L ? vdiff(xq, r) : 0.0;
+    ed[d] = (r[0] >> 14) == 2 ? xq[node(r[1])] - xq[node(r[2])] : 0.0;
   }
-  __syncthreads();
+  // source values of steps 1..nsteps (the step loop reads them from here)
+  const int nst1 = ta.nsteps + 1;
   const bool compat = ta.method == KG_BE_COMPAT, trap = ta.method == KG_TRAP;
-  const bool with_state = ta.alpha != 1.0;
-  for (int s = 1; s <= ta.nsteps; s++) {
+  double* tab = srcv ? srcv + (int64_t)inst * nsrc_max * nst1 : nullptr;
+  for (int i = t; i < nsrc * nst1; i += TPS) {
+    const int sidx = i / nst1, s = i - sidx * nst1;
     const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
     const double tsrc = compat ? tprev : tn;  // krets evaluates sources at the guess's time
-    // right-hand side (permuted rows)
-    for (int i = t; i < n; i += TPS) {
-      double acc = 0.0;
-      for (int k = rcon_ptr[i]; k < rcon_ptr[i + 1]; k++) {
-        const int code = rcon[k], idx = code >> 2;
-        double val;
-        if ((code & 3) == LK_SRC) {
-          const double* pe = sp + idx * KG_NPARAM;
-          val = src[idx * LIN_SRC] == KG_V ? wave_value(pe, tsrc) : pe[0];
-        } else if ((code & 3) == LK_IND) {
-          const uint16_t* r = dyn + idx * LIN_DYN;
-          double rr = -cf[idx] * xq[r[4]];
-          if (with_state) rr -= ed[idx];
-          val = rr;
-        } else {
-          double ieq = -cf[idx] * vdiff(xq, dyn + idx * LIN_DYN);
-          if (with_state) ieq -= ed[idx];
-          val = (code & 3) == LK_CP ? -ieq : ieq;
-        }
-        acc += val;
+    const double* pe = sp + sidx * KG_NPARAM;
+    tab[i] = s == 0 ? 0.0 : (src[sidx * LIN_SRC] == KG_V ? wave_value(pe, tsrc) : pe[0]);
+  }
+  // ---- per-thread caches (registers)
+  // right-hand side: row t + r*TPS; contribution = xq slots a | b << 16,
+  // cf/ed slot d | flags << 16 (bit 0: negate, bit 1: source, source index << 2)
+  int rhs_n[R];
+  uint32_t c_ab[R][LIN_KC], c_df[R][LIN_KC];
+  // forward / backward positions t + r*TPS: level, row | first << 16, count | diag << 16, columns (pairs)
+  int f_lev[R], b_lev[R];
+  uint32_t f_ip[R], f_nd[R], f_col[R][LIN_KE / 2], b_ip[R], b_nd[R], b_col[R][LIN_KE / 2];
+  // companion elements t + r*TPS: plus | minus << 16, kind; output column slot
+  uint32_t u_pm[R];
+  int u_kind[R], o_slot[R];
+  auto contrib = [&](int code, uint32_t& ab, uint32_t& df) {
+    const int idx = code >> 2, k = code & 3;
+    if (k == LK_SRC) {
+      ab = NN | (NN << 16);
+      df = (uint32_t)ndyn | ((2u | ((uint32_t)idx << 2)) << 16);
+    } else if (k == LK_IND) {
+      ab = node(dyn[idx * LIN_DYN + 3]) | (NN << 16);
+      df = (uint32_t)idx;
+    } else {
+      ab = node(dyn[idx * LIN_DYN + 1]) | (node(dyn[idx * LIN_DYN + 2]) << 16);
+      df = (uint32_t)idx | ((k == LK_CP ? 1u : 0u) << 16);
+    }
+  };
+  auto level_of = [&](const uint16_t* ptr, int nl, int g) {
+    int l = 0;
+    while (l + 1 < nl && (int)ptr[l + 1] <= g) l++;
+    return l;
+  };
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int g = t + r * TPS;
+    const bool ok = g < n;
+    // right-hand side row g
+    const int k0 = ok ? rcon_ptr[g] : 0, k1 = ok ? rcon_ptr[g + 1] : 0;
+    rhs_n[r] = ok ? k1 - k0 : -1;
+#pragma unroll
+    for (int c = 0; c < LIN_KC; c++) {
+      c_ab[r][c] = NN | (NN << 16);
+      c_df[r][c] = (uint32_t)ndyn;
+      if (k0 + c < k1) contrib(rcon[k0 + c], c_ab[r][c], c_df[r][c]);
+    }
+    // forward position g
+    f_lev[r] = b_lev[r] = -1;
+    if (ok) {
+      ushort4 fd = fdesc[g], bd = bdesc[g];
+      f_lev[r] = level_of(lvl_ptr, nlev, g);
+      f_ip[r] = (uint32_t)fd.x | ((uint32_t)fd.y << 16);
+      f_nd[r] = (uint32_t)(fd.z - fd.y) | ((uint32_t)fd.w << 16);
+      b_lev[r] = level_of(blvl_ptr, nblev, g);
+      b_ip[r] = (uint32_t)bd.x | ((uint32_t)bd.y << 16);
+      b_nd[r] = (uint32_t)(bd.z - bd.y) | ((uint32_t)bd.w << 16);
+#pragma unroll
+      for (int k = 0; k < LIN_KE; k += 2) {
+        const uint32_t c0 = k < fd.z - fd.y ? col[fd.y + k] : NN, c1 = k + 1 < fd.z - fd.y ? col[fd.y + k + 1] : NN;
+        f_col[r][k / 2] = c0 | (c1 << 16);
+        const uint32_t e0 = k < bd.z - bd.y ? col[bd.y + k] : NN, e1 = k + 1 < bd.z - bd.y ? col[bd.y + k + 1] : NN;
+        b_col[r][k / 2] = e0 | (e1 << 16);
       }
+    }
+    // companion element g, output column g
+    u_kind[r] = g < ndyn ? (int)(dyn[g * LIN_DYN] >> 14) : 0;
+    u_pm[r] = g < ndyn ? node(dyn[g * LIN_DYN + 1]) | (node(dyn[g * LIN_DYN + 2]) << 16) : (NN | (NN << 16));
+    o_slot[r] = ok ? (int)cpos[g] : (int)NN;
+  }
+  __syncthreads();  // also publishes the source table within the block
+  const bool with_state = ta.alpha != 1.0;
+  // the first cached source contribution of this thread is prefetched
+  int pre_rc = -1, pre_row = 0;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int c = 0; c < LIN_KC; c++)
+      if (pre_rc < 0 && ((c_df[r][c] >> 16) & 2u)) {
+        pre_rc = r * LIN_KC + c;
+        pre_row = (int)(c_df[r][c] >> 18) * nst1;
+      }
+  double pre = (pre_rc >= 0 && ta.nsteps >= 1) ? tab[pre_row + 1] : 0.0;
+  // one right-hand-side contribution (generic form, from the plan in LDS)
+  auto rhs_one = [&](int code, int s) -> double {
+    uint32_t ab, df;
+    contrib(code, ab, df);
+    if ((df >> 16) & 2u) return tab[(int)(df >> 18) * nst1 + s];
+    double ieq = -cf[df & 0xFFFF] * (xq[ab & 0xFFFF] - xq[ab >> 16]);
+    if (with_state) ieq -= ed[df & 0xFFFF];
+    return (df >> 16) & 1u ? -ieq : ieq;
+  };
+  for (int s = 1; s <= ta.nsteps; s++) {
+    // right-hand side (permuted rows)
+    const double pre_now = pre;
+    if (pre_rc >= 0 && s < ta.nsteps) pre = tab[pre_row + s + 1];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      if (rhs_n[r] < 0) continue;
+      const int i = t + r * TPS;
+      double xa[LIN_KC], xb[LIN_KC], cv[LIN_KC], ev[LIN_KC];
+#pragma unroll
+      for (int c = 0; c < LIN_KC; c++) {
+        xa[c] = xq[c_ab[r][c] & 0xFFFF];
+        xb[c] = xq[c_ab[r][c] >> 16];
+        cv[c] = cf[c_df[r][c] & 0xFFFF];
+        ev[c] = ed[c_df[r][c] & 0xFFFF];
+      }
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < LIN_KC; c++) {
+        const uint32_t fl = c_df[r][c] >> 16;
+        double val;
+        if (fl & 2u) {
+          val = r * LIN_KC + c == pre_rc ? pre_now : tab[(int)(fl >> 2) * nst1 + s];
+        } else {
+          double ieq = -cv[c] * (xa[c] - xb[c]);
+          if (with_state) ieq -= ev[c];
+          val = fl & 1u ? -ieq : ieq;
+        }
+        acc += val;  // padded contributions add -0.0: no change
+      }
+      if (rhs_n[r] > LIN_KC)
+        for (int k = rcon_ptr[i] + LIN_KC; k < rcon_ptr[i + 1]; k++) acc += rhs_one(rcon[k], s);
       xc[i] = acc;
     }
     __syncthreads();
     // L y = b (level 0 rows, when free of lower entries, are already done)
     for (int l = lev0; l < nlev; l++) {
-      for (int k = lvl_ptr[l] + t; k < lvl_ptr[l + 1]; k += TPS) {
-        const int i = lvl_rows[k];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (f_lev[r] != l) continue;
+        const int i = (int)(f_ip[r] & 0xFFFF), p0 = (int)(f_ip[r] >> 16), cnt = (int)(f_nd[r] & 0xFFFF);
         double sum = xc[i];
-        const int d = diag[i];
-        for (int p = row_ptr[i]; p < d; p++) sum -= v[p] * xc[col[p]];
+        double vv[LIN_KE], xx[LIN_KE];
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) {
+          vv[k] = v[k < cnt ? p0 + k : nnz];
+          xx[k] = xc[k & 1 ? f_col[r][k / 2] >> 16 : f_col[r][k / 2] & 0xFFFF];
+        }
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];  // padded entries subtract 0 * 0
+        for (int p = p0 + LIN_KE; p < p0 + cnt; p++) sum -= v[p] * xc[col[p]];
         xc[i] = sum;
       }
       __syncthreads();
     }
     // U x = y (diagonal stored as its reciprocal)
     for (int l = 0; l < nblev; l++) {
-      for (int k = blvl_ptr[l] + t; k < blvl_ptr[l + 1]; k += TPS) {
-        const int i = blvl_rows[k];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (b_lev[r] != l) continue;
+        const int i = (int)(b_ip[r] & 0xFFFF), p0 = (int)(b_ip[r] >> 16), cnt = (int)(b_nd[r] & 0xFFFF);
+        const int d = (int)(b_nd[r] >> 16);
         double sum = xc[i];
-        const int d = diag[i], e = row_ptr[i + 1];
-        for (int p = d + 1; p < e; p++) sum -= v[p] * xc[col[p]];
-        xc[i] = sum * v[d];
+        const double vd = v[d];
+        double vv[LIN_KE], xx[LIN_KE];
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) {
+          vv[k] = v[k < cnt ? p0 + k : nnz];
+          xx[k] = xc[k & 1 ? b_col[r][k / 2] >> 16 : b_col[r][k / 2] & 0xFFFF];
+        }
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+        for (int p = p0 + LIN_KE; p < p0 + cnt; p++) sum -= v[p] * xc[col[p]];
+        xc[i] = sum * vd;
       }
       __syncthreads();
     }
     // trapezoidal companion states, then the solution row in krets numbering
-    if (trap)
-      for (int d = t; d < ndyn; d += TPS) {
-        const uint16_t* r = dyn + d * LIN_DYN;
-        const double vn = vdiff(xc, r);
-        ed[d] = r[0] == KG_C ? k2[d] * (vn - vdiff(xq, r)) - ed[d] : vn;
+    if (trap) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int d = t + r * TPS;
+        if (d >= ndyn) continue;
+        const uint32_t a = u_pm[r] & 0xFFFF, b = u_pm[r] >> 16;
+        const double vn = xc[a] - xc[b], vo = xq[a] - xq[b], c = cf[d], e = ed[d];
+        ed[d] = u_kind[r] == 1 ? c * (vn - vo) - e : vn;
       }
+      for (int d = t + R * TPS; d < ndyn; d += TPS) {  // companions beyond the cached ones
+        const uint16_t* r = dyn + d * LIN_DYN;
+        const double vn = xc[node(r[1])] - xc[node(r[2])];
+        ed[d] = (r[0] >> 14) == 1 ? cf[d] * (vn - (xq[node(r[1])] - xq[node(r[2])])) - ed[d] : vn;
+      }
+    }
     double* os = o + (int64_t)s * n;
-    for (int j = t; j < n; j += TPS) os[j] = xc[cpos[j]];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (t + r * TPS < n) os[t + r * TPS] = xc[o_slot[r]];
     if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1) + s] = 1;
     __syncthreads();
     double* sw = xc;
@@ -1945,21 +2095,21 @@ hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t 
                             out_off, status, iters, err, lay, 0))
 }
 
-hipError_t run_tran_lin(int tps, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                         const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
-                        int32_t* status, int32_t* iters, double* err) {
+                        int32_t* status, int32_t* iters, double* err, double* srcv, int nsrc_max) {
   const dim3 b(tps);
-  switch (tps) {
-    case 64: return launch(k_tran_lin<64>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
-                           iters, err);
-    case 128: return launch(k_tran_lin<128>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
-                            iters, err);
-    case 256: return launch(k_tran_lin<256>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
-                            iters, err);
-    case 512: return launch(k_tran_lin<512>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status,
-                            iters, err);
-    default: return hipErrorInvalidValue;
-  }
+#define KG_LIN(T, RR)                                                                                          \
+  if (tps == T && r == RR)                                                                                   \
+    return launch(k_tran_lin<T, RR>, g, b, lds, st, B, kind_slot, ta, xop, xop_off, out, out_off, status, iters, \
+                  err, srcv, nsrc_max);
+  KG_LIN(64, 1) KG_LIN(64, 2) KG_LIN(64, 4)
+  KG_LIN(128, 1) KG_LIN(128, 2) KG_LIN(128, 4)
+  KG_LIN(256, 1) KG_LIN(256, 2) KG_LIN(256, 4)
+  KG_LIN(512, 1) KG_LIN(512, 2) KG_LIN(512, 4)
+  KG_LIN(1024, 1) KG_LIN(1024, 2)
+#undef KG_LIN
+  return hipErrorInvalidValue;
 }
 
 }  // namespace kg

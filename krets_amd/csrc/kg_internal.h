@@ -54,23 +54,27 @@ struct PatLayout {
 };
 
 // Compact plan of a linear transient (no diodes), built once per topology so
-// that a system's whole step loop runs from LDS: 16-bit row/column indices,
-// level lists, right-hand-side gather lists and the element records the
-// companion models need. The header is followed by uint16 arrays at the
-// given word offsets; the buffer is padded to 16 bytes.
-//   dyn entry (LIN_DYN words): kind (KG_C / KG_L), element, plus column,
-//     minus column, branch column (LIN_NONE for ground / none), 0
+// that a system's whole step loop runs from LDS: 16-bit indices, one packed
+// row descriptor per row and triangular solve (in level order), right-hand
+// side gather lists and the element records the companion models need. The
+// header is followed by uint16 arrays at the given word offsets; the buffer
+// is padded to 16 bytes.
+//   fdesc / bdesc (4 words per row, level order): row, first entry, last
+//     entry + 1, diagonal entry (forward: strictly lower entries; backward:
+//     strictly upper entries)
+//   dyn entry (LIN_DYN words): (kind << 14) | element (kind 1: C, 2: L),
+//     plus column, minus column, branch column (LIN_NONE for ground / none)
 //   src entry (LIN_SRC words): kind (KG_V / KG_I), element
 //   rcon code: (index << 2) | LK_* (index into src or dyn)
-constexpr int LIN_DYN = 6, LIN_SRC = 2;
+constexpr int LIN_DYN = 4, LIN_SRC = 2;
 constexpr uint16_t LIN_NONE = 0xFFFF;
 enum LinCode { LK_SRC = 0, LK_CP = 1, LK_CM = 2, LK_IND = 3 };
 struct LinHdr {
   int32_t n, nnz, nlev, nblev, ndyn, ncon, nsrc, words;
-  int32_t o_lvl_ptr, o_lvl_rows, o_blvl_ptr, o_blvl_rows, o_row_ptr, o_diag, o_col, o_cpos;
+  int32_t o_lvl_ptr, o_fdesc, o_blvl_ptr, o_bdesc, o_col, o_cpos;
   int32_t o_rcon_ptr, o_rcon, o_dyn, o_src;
   int32_t lev0_free;  // the first forward level has no lower-triangle entries
-  int32_t pad[3];
+  int32_t pad[5];
 };
 
 // Device view of one pattern (pointers into uploaded buffer).

@@ -50,7 +50,7 @@ __host__ __device__ inline size_t sys_bytes(const Lay& L) {
 
 // LDS bytes of one system of the linear transient kernel.
 __host__ __device__ inline size_t lin_bytes(const LinHdr& h) {
-  return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 3 * (size_t)h.ndyn) +
+  return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 2 * (size_t)h.ndyn + 5) +
          8 * (size_t)KG_NPARAM * h.nsrc + 64;
 }
 
@@ -105,10 +105,13 @@ hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hip
                              double* out, const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
                              int plan_ints, int64_t chunk);
 // Linear fixed-step transient from LDS (k_tran_lin), one system per block of
-// tps threads (64, 128, 256, 512); lds >= lin_bytes of every launched system.
-hipError_t run_tran_lin(int tps, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+// tps threads (64, 128, 256, 512, 1024), each owning up to r rows (1, 2, 4;
+// 1024 threads: 1, 2); every launched system needs n <= r * tps and
+// lds >= its lin_bytes. srcv: workspace of n_instances * nsrc_max *
+// (nsteps + 1) doubles for the tabulated source values.
+hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                         const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
-                        int32_t* status, int32_t* iters, double* err);
+                        int32_t* status, int32_t* iters, double* err, double* srcv, int nsrc_max);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay,

@@ -623,7 +623,8 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
 bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vector<uint16_t>& out) {
   out.clear();
   const int n = L.n, nnz = L.nnz, ne = L.ne;
-  if (L.status != KG_OK || L.has_diode || L.nonlinear || n >= 0xFFFF || nnz >= 0xFFFF || ne >= 0xFFFF) return false;
+  if (L.status != KG_OK || L.has_diode || L.nonlinear || n >= 0xFFFF || nnz >= 0xFFFF || ne >= (1 << 14))
+    return false;
   const int32_t* et = b.data() + L.o_etype;
   const int32_t* ep = b.data() + L.o_ep;
   const int32_t* em = b.data() + L.o_em;
@@ -637,7 +638,8 @@ bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vect
   for (int e = 0; e < ne; e++) {
     if (et[e] == KG_C || et[e] == KG_L) {
       dyn_of[e] = (int32_t)(dyn.size() / LIN_DYN);
-      dyn.insert(dyn.end(), {(uint16_t)et[e], (uint16_t)e, u16(ep[e]), u16(em[e]), u16(ec[e]), 0});
+      const int kind = et[e] == KG_C ? 1 : 2;
+      dyn.insert(dyn.end(), {(uint16_t)((kind << 14) | e), u16(ep[e]), u16(em[e]), u16(ec[e])});
     } else if (et[e] == KG_V || et[e] == KG_I) {
       src_of[e] = (int32_t)(src.size() / LIN_SRC);
       src.insert(src.end(), {(uint16_t)et[e], (uint16_t)e});
@@ -677,12 +679,29 @@ bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vect
     for (size_t i = 0; i < k; i++) t[i] = (uint16_t)p[i];
     return put(t.data(), k);
   };
+  const int32_t* rp = b.data() + L.o_row_ptr;
+  const int32_t* dg = b.data() + L.o_diag;
+  auto descs = [&](int o_ptr, int o_rows, int nl, bool fwd) {
+    std::vector<uint16_t> d;
+    const int32_t* lp = b.data() + o_ptr;
+    const int32_t* lr = b.data() + o_rows;
+    for (int q = 0; q < lp[nl]; q++) {
+      const int i = lr[q];
+      if (fwd) d.insert(d.end(), {(uint16_t)i, (uint16_t)rp[i], (uint16_t)dg[i], (uint16_t)dg[i]});
+      else d.insert(d.end(), {(uint16_t)i, (uint16_t)(dg[i] + 1), (uint16_t)rp[i + 1], (uint16_t)dg[i]});
+    }
+    return d;
+  };
+  auto put8 = [&](const std::vector<uint16_t>& d) {  // 8-byte aligned descriptors
+    while (w.size() % 4) w.push_back(0);
+    int32_t off = (int32_t)w.size();
+    w.insert(w.end(), d.begin(), d.end());
+    return off;
+  };
   h.o_lvl_ptr = put32(b.data() + L.o_lvl_ptr, L.nlev + 1);
-  h.o_lvl_rows = put32(b.data() + L.o_lvl_rows, n);
+  h.o_fdesc = put8(descs(L.o_lvl_ptr, L.o_lvl_rows, L.nlev, true));
   h.o_blvl_ptr = put32(b.data() + L.o_blvl_ptr, L.nblev + 1);
-  h.o_blvl_rows = put32(b.data() + L.o_blvl_rows, n);
-  h.o_row_ptr = put32(b.data() + L.o_row_ptr, n + 1);
-  h.o_diag = put32(b.data() + L.o_diag, n);
+  h.o_bdesc = put8(descs(L.o_blvl_ptr, L.o_blvl_rows, L.nblev, false));
   h.o_col = put32(b.data() + L.o_col, nnz);
   h.o_cpos = put32(b.data() + L.o_cpos, n);
   h.o_rcon_ptr = put32(rcp, n + 1);
@@ -695,8 +714,6 @@ bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vect
   // is the right-hand side itself
   const int32_t* lp = b.data() + L.o_lvl_ptr;
   const int32_t* lr = b.data() + L.o_lvl_rows;
-  const int32_t* rp = b.data() + L.o_row_ptr;
-  const int32_t* dg = b.data() + L.o_diag;
   h.lev0_free = 1;
   for (int q = lp[0]; q < (L.nlev > 0 ? lp[1] : 0); q++)
     if (dg[lr[q]] != rp[lr[q]]) h.lev0_free = 0;

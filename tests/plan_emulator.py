@@ -134,13 +134,12 @@ class Plan:
 
 
 # ------------------------------------------------------------- compact plan
-LIN_HDR = ["n", "nnz", "nlev", "nblev", "ndyn", "ncon", "nsrc", "words", "o_lvl_ptr", "o_lvl_rows", "o_blvl_ptr",
-           "o_blvl_rows", "o_row_ptr", "o_diag", "o_col", "o_cpos", "o_rcon_ptr", "o_rcon", "o_dyn", "o_src",
-           "lev0_free"]
+LIN_HDR = ["n", "nnz", "nlev", "nblev", "ndyn", "ncon", "nsrc", "words", "o_lvl_ptr", "o_fdesc", "o_blvl_ptr",
+           "o_bdesc", "o_col", "o_cpos", "o_rcon_ptr", "o_rcon", "o_dyn", "o_src", "lev0_free"]
 LIN_HDR_WORDS = 48  # sizeof(LinHdr) / 2
-LIN_DYN, LIN_SRC, LIN_NONE = 6, 2, 0xFFFF
+LIN_DYN, LIN_SRC, LIN_NONE = 4, 2, 0xFFFF
 LK_SRC, LK_CP, LK_CM, LK_IND = 0, 1, 2, 3
-KG_C, KG_L, KG_V = 1, 2, 3
+DYN_C, DYN_L, KG_V, KG_I = 1, 2, 3, 4
 LIN = 3  # kg_plan_export kind of the compact plan
 
 
@@ -164,45 +163,53 @@ class LinPlan:
         h = self.h
         assert h["words"] == len(self.w) and h["words"] % 8 == 0
 
-        def a(name, k):
+        def a(name, k, align=2):
             o = h["o_" + name]
-            assert LIN_HDR_WORDS <= o and o + k <= h["words"], name
+            assert LIN_HDR_WORDS <= o and o + k <= h["words"] and o % align == 0, name
             return self.w[o:o + k]
         n, nnz = h["n"], h["nnz"]
-        self.lvl_ptr, self.lvl_rows = a("lvl_ptr", h["nlev"] + 1), a("lvl_rows", n)
-        self.blvl_ptr, self.blvl_rows = a("blvl_ptr", h["nblev"] + 1), a("blvl_rows", n)
-        self.row_ptr, self.diag, self.col = a("row_ptr", n + 1), a("diag", n), a("col", nnz)
-        self.cpos, self.rcon_ptr = a("cpos", n), a("rcon_ptr", n + 1)
-        self.rcon = a("rcon", h["ncon"])
+        self.lvl_ptr, self.blvl_ptr = a("lvl_ptr", h["nlev"] + 1), a("blvl_ptr", h["nblev"] + 1)
+        self.fdesc = a("fdesc", 4 * n, 4).reshape(-1, 4)
+        self.bdesc = a("bdesc", 4 * n, 4).reshape(-1, 4)
+        self.col, self.cpos = a("col", nnz), a("cpos", n)
+        self.rcon_ptr, self.rcon = a("rcon_ptr", n + 1), a("rcon", h["ncon"])
         self.dyn = a("dyn", LIN_DYN * h["ndyn"]).reshape(-1, LIN_DYN)
         self.src = a("src", LIN_SRC * h["nsrc"]).reshape(-1, LIN_SRC)
+        self.dkind = self.dyn[:, 0] >> 14
+        self.delem = self.dyn[:, 0] & 0x3FFF
 
-    def check_indices(self, ne):
-        """Every index the kernel dereferences, against the buffer it indexes."""
+    def check_indices(self, ne, full=None):
+        """Every index the kernel dereferences, against the buffer it indexes;
+        with the full plan, the descriptors against its rows."""
         h, n, nnz = self.h, self.h["n"], self.h["nnz"]
-        for ptr, rows, nl in ((self.lvl_ptr, self.lvl_rows, h["nlev"]), (self.blvl_ptr, self.blvl_rows, h["nblev"])):
+        for ptr, desc, nl in ((self.lvl_ptr, self.fdesc, h["nlev"]), (self.blvl_ptr, self.bdesc, h["nblev"])):
             assert ptr[0] == 0 and ptr[nl] == n and np.all(np.diff(ptr) >= 0)
-            assert sorted(rows.tolist()) == list(range(n))
-        assert self.row_ptr[0] == 0 and self.row_ptr[n] == nnz and np.all(np.diff(self.row_ptr) >= 0)
-        assert np.all((self.diag >= self.row_ptr[:-1]) & (self.diag < self.row_ptr[1:]))
+            assert sorted(desc[:, 0].tolist()) == list(range(n))
+            assert np.all(desc[:, 1] <= desc[:, 2]) and np.all(desc[:, 2] <= nnz) and np.all(desc[:, 3] < nnz)
+        if full is not None:
+            rp, dg = full.row_ptr, full.diag
+            for i, p0, p1, d in self.fdesc:
+                assert (p0, p1, d) == (rp[i], dg[i], dg[i])
+            for i, p0, p1, d in self.bdesc:
+                assert (p0, p1, d) == (dg[i] + 1, rp[i + 1], dg[i])
         assert np.all(self.col < n) and sorted(self.cpos.tolist()) == list(range(n))
         assert self.rcon_ptr[0] == 0 and self.rcon_ptr[n] == h["ncon"] and np.all(np.diff(self.rcon_ptr) >= 0)
         for code in self.rcon:
             k, idx = int(code) & 3, int(code) >> 2
             assert idx < (h["nsrc"] if k == LK_SRC else h["ndyn"])
             if k == LK_IND:
-                assert self.dyn[idx, 0] == KG_L
+                assert self.dkind[idx] == DYN_L
             elif k != LK_SRC:
-                assert self.dyn[idx, 0] == KG_C
-        for r in self.dyn:
-            assert r[0] in (KG_C, KG_L) and r[1] < ne
-            assert all(v == LIN_NONE or v < n for v in r[2:5])
-            assert r[0] != KG_L or r[4] < n
+                assert self.dkind[idx] == DYN_C
+        for j, r in enumerate(self.dyn):
+            assert self.dkind[j] in (DYN_C, DYN_L) and self.delem[j] < ne
+            assert all(v == LIN_NONE or v < n for v in r[1:4])
+            assert self.dkind[j] != DYN_L or r[3] < n
         for r in self.src:
-            assert r[0] in (KG_V, 4) and r[1] < ne
+            assert r[0] in (KG_V, KG_I) and r[1] < ne
         if h["lev0_free"]:
-            for i in self.lvl_rows[self.lvl_ptr[0]:self.lvl_ptr[1]]:
-                assert self.row_ptr[i] == self.diag[i]
+            for i, p0, p1, _ in self.fdesc[self.lvl_ptr[0]:self.lvl_ptr[1]]:
+                assert p0 == p1
 
 
 def wave_value(p, t):
@@ -235,9 +242,8 @@ def tran_lin(lp, vf, params, x0, h, nsteps, method):
     n = lp.h["n"]
     alpha = 2.0 if method == 2 else 1.0
     compat, trap = method == 0, method == 2
-    vals = np.array([params[int(r[1]), 0] for r in lp.dyn])
+    vals = np.array([params[int(e), 0] for e in lp.delem])
     cf = alpha * (vals / h)
-    k2 = 2.0 * vals / h
     sp = [params[int(r[1])] for r in lp.src]
     xq = np.zeros(n)
     xq[lp.cpos] = x0
@@ -245,8 +251,8 @@ def tran_lin(lp, vf, params, x0, h, nsteps, method):
     out[0] = x0
 
     def vd(x, r):
-        return (x[r[2]] if r[2] != LIN_NONE else 0.0) - (x[r[3]] if r[3] != LIN_NONE else 0.0)
-    ed = np.array([vd(xq, r) if r[0] == KG_L else 0.0 for r in lp.dyn])
+        return (x[r[1]] if r[1] != LIN_NONE else 0.0) - (x[r[2]] if r[2] != LIN_NONE else 0.0)
+    ed = np.array([vd(xq, r) if k == DYN_L else 0.0 for r, k in zip(lp.dyn, lp.dkind)])
     lev0 = 1 if lp.h["lev0_free"] else 0
     for s in range(1, nsteps + 1):
         tsrc = (s - 1) * h if compat else s * h
@@ -258,7 +264,7 @@ def tran_lin(lp, vf, params, x0, h, nsteps, method):
                 if k == LK_SRC:
                     acc += wave_value(sp[idx], tsrc) if lp.src[idx, 0] == KG_V else sp[idx][0]
                 elif k == LK_IND:
-                    rr = -cf[idx] * xq[lp.dyn[idx, 4]]
+                    rr = -cf[idx] * xq[lp.dyn[idx, 3]]
                     acc += rr - ed[idx] if alpha != 1.0 else rr
                 else:
                     ieq = -cf[idx] * vd(xq, lp.dyn[idx])
@@ -267,16 +273,15 @@ def tran_lin(lp, vf, params, x0, h, nsteps, method):
                     acc += -ieq if k == LK_CP else ieq
             xc[i] = acc
         for l in range(lev0, lp.h["nlev"]):
-            for i in lp.lvl_rows[lp.lvl_ptr[l]:lp.lvl_ptr[l + 1]]:
-                xc[i] -= sum(vf[p] * xc[lp.col[p]] for p in range(lp.row_ptr[i], lp.diag[i]))
+            for i, p0, p1, _ in lp.fdesc[lp.lvl_ptr[l]:lp.lvl_ptr[l + 1]]:
+                xc[i] -= sum(vf[p] * xc[lp.col[p]] for p in range(p0, p1))
         for l in range(lp.h["nblev"]):
-            for i in lp.blvl_rows[lp.blvl_ptr[l]:lp.blvl_ptr[l + 1]]:
-                d = lp.diag[i]
-                xc[i] = (xc[i] - sum(vf[p] * xc[lp.col[p]] for p in range(d + 1, lp.row_ptr[i + 1]))) * vf[d]
+            for i, p0, p1, d in lp.bdesc[lp.blvl_ptr[l]:lp.blvl_ptr[l + 1]]:
+                xc[i] = (xc[i] - sum(vf[p] * xc[lp.col[p]] for p in range(p0, p1))) * vf[d]
         if trap:
             for j, r in enumerate(lp.dyn):
                 vn = vd(xc, r)
-                ed[j] = k2[j] * (vn - vd(xq, r)) - ed[j] if r[0] == KG_C else vn
+                ed[j] = cf[j] * (vn - vd(xq, r)) - ed[j] if lp.dkind[j] == DYN_C else vn
         out[s] = xc[lp.cpos]
         xq = xc
     return out

@@ -250,7 +250,7 @@ def test_batched_transient_uneven_rlc(kg, oracle, method, precision, tps):
         assert e < 1e-9, (i, e)
 
 
-@pytest.mark.parametrize("tps", [3064, 3128, 3256, 3512])  # 3000 + t: whole step loop in LDS
+@pytest.mark.parametrize("tps", [3064, 3128, 3256, 3512, 4024])  # 3000 + t: whole step loop in LDS
 @pytest.mark.parametrize("method", [2, 1, 0])
 def test_lds_resident_transient_uneven_rlc(kg, oracle, method, tps):
     from krets_amd import workloads as W

@@ -184,7 +184,7 @@ LIN_CASES = ["low_pass_filter/transient.cir", "high_pass_filter/transient.cir", 
 
 def lin_bytes(h):
     al = lambda b: (b + 15) & ~15
-    return al(2 * h["words"]) + 8 * (h["nnz"] + 2 * h["n"] + 3 * h["ndyn"]) + 8 * 10 * h["nsrc"] + 64
+    return al(2 * h["words"]) + 8 * (h["nnz"] + 2 * h["n"] + 2 * h["ndyn"] + 5) + 8 * 10 * h["nsrc"] + 64
 
 
 @pytest.mark.parametrize("method", [0, 1, 2])
@@ -207,7 +207,7 @@ def test_lin_plan_step_loop_vs_oracle(oracle, case, method):
     lp = pe.LinPlan(c.elements, c.n, c.params, alpha / h)
     assert P.status == 0 and lp.status == 0
     assert lp.h["n"] == c.n and lp.h["nnz"] == P.nnz
-    lp.check_indices(len(c.elements))
+    lp.check_indices(len(c.elements), P)
     contrib, _ = values(c, pe.TRAN, h=h, alpha=alpha)
     v, _ = P.gather(contrib)
     vf = P.factor(v).real
