@@ -97,7 +97,10 @@ def main():
         print(json.dumps({"probe": "done", "copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}))
         return
 
-    refine = {"iters": 1}
+    # refinement: 0 is the library default (at most two passes; a wave stops
+    # after the first once every lane's predicted error is below 1e-11),
+    # > 0 a fixed number of passes
+    refine = {"iters": 0}
 
     def step(prec):
         cfg = kg.SolverConfig(refine_iters=refine["iters"]) if prec == kg.MIXED else None
@@ -135,9 +138,10 @@ def main():
         num = torch.linalg.vector_norm((xm - xd).reshape(ni * nf, -1), dim=1)
         return float((num / den).max())
 
-    # fewest refinement steps that keep every solve within 1e-9 of FP64
+    # the default refinement, or else the fewest fixed passes that keep every
+    # solve within 1e-9 of FP64
     tuning = ctx.last_autotune()
-    for it in (1, 2, 3):
+    for it in (0, 1, 2, 3):
         refine["iters"] = it
         step(kg.MIXED)
         if mixed_err() <= 1e-9:
@@ -219,7 +223,8 @@ def main():
         "max_normwise_err_mixed_vs_fp64_gpu": err_mixed_vs_fp64,
         "max_componentwise_err_mixed_vs_fp64_gpu": err_comp,
         "max_refinement_estimate": float(est_m),
-        "refinement_steps": refine["iters"],
+        "refinement_steps": refine["iters"] if refine["iters"] > 0 else
+        "adaptive: 1 or 2 passes per wave (stop once 200*(|dx|/|x|)^2 <= 1e-11 on every lane)",
         "autotune_ms_by_threads_per_system": {"fp64": tuning, "mixed": tuning_mixed},
         "max_normwise_err_vs_oracle": err_oracle,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

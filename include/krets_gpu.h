@@ -179,7 +179,11 @@ int32_t kg_adjoint_transient(kg_ctx* ctx, const kg_solver_config* cfg, double h,
 /* The two halves of kg_adjoint_transient (no krets counterpart): the forward
  * pass returns J and keeps the checkpoints in the context; the backward pass
  * returns grad for the most recent forward pass. Batches with a shared
- * topology and linear circuits (R, C, L, V, I); method KG_BE or KG_TRAP. */
+ * topology and linear circuits (R, C, L, V, I); method KG_BE or KG_TRAP.
+ * threads_per_system: 1..256 (level-scheduled kernels), 3064 / 3256 / 3512
+ * (FP64, the step loops in LDS, one system per block of 64 / 256 / 512
+ * threads), 0 = the fastest forward pass among them, timed once per
+ * workload. */
 int32_t kg_adjoint_forward(kg_ctx* ctx, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
                            int32_t precision, const double* cvec, int32_t checkpoint_every, double* J,
                            int32_t* status, int32_t mem);

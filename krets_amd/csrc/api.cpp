@@ -606,6 +606,26 @@ double* lin_srcv(kg_ctx* c, double s_tr, int nsteps, int* nsrc_max, int32_t* r) 
   return p;
 }
 
+// LDS bytes of the LDS-resident adjoint for the (shared) topology; 0 if it has
+// no compact plan.
+size_t lin_adj_need(kg_ctx* c, double s_tr) {
+  auto it = c->pat_index.find(std::make_tuple(c->group_of[0], (int)AK_TRAN, (int)ORD_NESTED, s_tr));
+  if (it == c->pat_index.end() || c->pats[it->second].lin.empty()) return 0;
+  LinHdr h;
+  std::memcpy(&h, c->pats[it->second].lin.data(), sizeof(h));
+  return lin_adj_bytes(h);
+}
+
+// Source-value table of the adjoint kernels (n_instances * nsrc_max * (nsteps + 1)).
+double* adj_srcv(kg_ctx* c, double s_tr, int nsteps, int* nsrc_max, int32_t* r) {
+  *nsrc_max = lin_nsrc_max(c, s_tr);
+  hipError_t e;
+  double* p = (double*)ws_get(c, "adj_srcv",
+                              sizeof(double) * std::max<size_t>(1, (size_t)c->n_inst * *nsrc_max * (nsteps + 1)), &e);
+  *r = e == hipSuccess ? KG_OK : set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  return p;
+}
+
 // Largest need over all instances, 0 if any instance has no compact plan.
 size_t lin_need_all(kg_ctx* c, double s_tr) {
   size_t m = 0;
@@ -620,6 +640,15 @@ size_t lin_need_all(kg_ctx* c, double s_tr) {
 // Work-mapping codes of k_tran_lin: 3000 + threads per block (one system per block).
 const std::vector<int> LIN_CANDS = {3064, 3128, 3256, 3512, 4024};
 bool is_lin_cand(int t) { return t == 3064 || t == 3128 || t == 3256 || t == 3512 || t == 4024; }
+// Rows per thread of the LDS-resident adjoint kernels (64, 256, 512 threads;
+// 512 threads at most 2 rows); 0 if none fits.
+int adj_lin_rows(int tps, int n_max) {
+  if (tps != 64 && tps != 256 && tps != 512) return 0;
+  for (int r : {1, 2, 4})
+    if (n_max <= r * tps && (tps < 512 || r <= 2)) return r;
+  return 0;
+}
+
 // Rows per thread of k_tran_lin for systems of up to n_max unknowns; 0 if none fits.
 int lin_rows(int tps, int n_max) {
   for (int r : {1, 2, 4})
@@ -766,6 +795,68 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     }
     hipEventDestroy(fork);
     return hipSuccess;
+  });
+}
+
+int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem);
+int32_t run_adj_op_term(kg_ctx* c, const Plan& P, double* d_xb, double* d_g, int ne, int32_t* d_st);
+
+// Backward pass with the LDS-resident adjoint kernel (FP64), then the
+// operating-point term.
+int32_t adjoint_backward_lin(kg_ctx* c, double* grad, int32_t* status, int32_t mem) {
+  const AdjArgs& aa = c->adj.aa;
+  const double s_tr = aa.alpha / aa.h;
+  Plan P;
+  int32_t r;
+  if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P))) return r;
+  const int tps = c->adj.tps - 3000, rows = adj_lin_rows(tps, P.n_max);
+  const size_t ladj = lin_adj_need(c, s_tr);
+  if (rows == 0 || ladj == 0 || ladj > LDS_MAX) return set_err(c, KG_UNEXPECTED, "adjoint mapping no longer fits");
+  hipError_t e;
+  size_t ni = c->n_inst;
+  int ne = (int)(c->eoff[1] - c->eoff[0]);
+  int64_t seg_stride = (int64_t)(aa.K + 1) * P.n_max;
+  double* d_ck = (double*)ws_get(c, "adj_ckpt", 8, &e);
+  double* d_c = (double*)ws_get(c, "adj_cvec", 8, &e);
+  double* d_seg = (double*)ws_get(c, "adj_seg", sizeof(double) * ni * seg_stride, &e);
+  double* d_g = (double*)ws_get(c, "adj_grad", sizeof(double) * ni * ne, &e);
+  double* d_xb = (double*)ws_get(c, "adj_xbar0", sizeof(double) * ni * P.n_max, &e);
+  int64_t* d_xoff = (int64_t*)ws_get(c, "adj_xoff", 8, &e);
+  int32_t* d_st = (int32_t*)ws_get(c, "adj_status", sizeof(int32_t) * ni, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint)");
+  int nsrc_max = 0;
+  double* srcv = adj_srcv(c, s_tr, aa.nsteps, &nsrc_max, &r);
+  if (r) return r;
+  HIPCHK(c, run_fill_i32(d_st, KG_OK, (int64_t)ni, c->stream));
+  TranArgs ta{aa.h, aa.alpha, aa.nsteps, aa.method, 0};
+  BatchDev B = batch_dev(c, P);
+  r = timed(c, "k_adj_lin_bwd", [&] {
+    return run_adj_lin_bwd(tps, rows, dim3((unsigned)ni), ladj, c->stream, B, 0, ta, aa.K, d_c, d_ck,
+                           (int64_t)c->adj.ncp * c->adj.cpw, c->adj.cpw, d_seg, seg_stride, d_g, ne, d_xb, d_xoff,
+                           d_st, srcv, nsrc_max);
+  });
+  if (r) return r;
+  if ((r = run_adj_op_term(c, P, d_xb, d_g, ne, d_st))) return r;
+  if (grad && (r = copy_out(c, grad, d_g, sizeof(double) * ni * ne, mem))) return r;
+  return finish_status(c, d_st, status, mem);
+}
+
+// Operating-point part of the adjoint (DC plan, FP64): grad += dJ/dx_0 terms.
+int32_t run_adj_op_term(kg_ctx* c, const Plan& P, double* d_xb, double* d_g, int ne, int32_t* d_st) {
+  hipError_t e;
+  double* d_xop = (double*)ws_get(c, "adj_xop", 8, &e);
+  int64_t* d_xoff = (int64_t*)ws_get(c, "adj_xoff", 8, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint)");
+  Plan PD;
+  int tpd = default_tps(c, P.n_max);
+  int32_t r;
+  if ((r = prepare(c, AK_DC, order_for(tpd), 0.0, PD))) return r;
+  Lay ld{PD.nnz_max, PD.n_max, PD.ne_max, 0, 0, 0, 8, 8, 0, 0};
+  Launch LD;
+  if ((r = configure(c, tpd, ld, c->n_inst, LD))) return r;
+  BatchDev BD = batch_dev(c, PD);
+  return timed(c, "k_adj_op", [&] {
+    return run_adj_op(tpd, LD.grid, LD.block, LD.lds, c->stream, BD, 0, d_xb, d_xop, d_xoff, d_g, ne, d_st, ld);
   });
 }
 
@@ -1083,7 +1174,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       int32_t* stp = d_st;
       double* errp = mixed ? d_err : nullptr;
       int rf = refine;
-      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp, &rf};
+      double stop = cfg->refine_iters > 0 ? 0.0 : KG_REFINE_STOP;  // explicit counts run every pass
+      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp, &rf, &stop};
       return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, (unsigned)A.L.lds, c->stream, args, nullptr);
     }
     BatchDev B = batch_dev(c, A.P);
@@ -1119,7 +1211,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
                    [&] { return run(A); }});
   if (graph) {
     std::ostringstream k;
-    k << "ac|" << cand << "|" << refine << "|" << precision << "|" << nf << "|" << ocfg.reltol << "|" << ocfg.iabstol
+    k << "ac|" << cand << "|" << refine << "|" << (cfg->refine_iters > 0 ? "fixed" : "adaptive") << "|" << precision
+      << "|" << nf << "|" << ocfg.reltol << "|" << ocfg.iabstol
       << "|" << ocfg.vabstol << "|" << ocfg.maxiter << "|" << ocfg.compat << "|" << (void*)d_out << "|" << (void*)d_f
       << "|" << (void*)d_xop << "|" << (void*)d_st << "|" << (void*)d_err << "|" << (void*)c->d_pats << "|"
       << (void*)d_off << "|" << (void*)d_xoff << "|" << (void*)A.fn.fn;
@@ -1458,21 +1551,11 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
   double s_tr = (method == KG_TRAP ? 2.0 : 1.0) / h;
   if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P0))) return r;
   if (P0.any_nonlinear) return set_err(c, KG_INVALID_ARGUMENT, "adjoint: linear circuits only");
-  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
-  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
   bool mixed = precision == KG_MIXED;
-  Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 5, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
-  Launch L;
-  // stage the plan in LDS for the whole step loop when it fits
-  int pi = (tps == 64 || tps == 256) ? P.total_max : 0;
-  if (pi && configure(c, tps, lay, c->n_inst, L, lay_al(4 * (size_t)pi)) != KG_OK) pi = 0;
-  if (!pi && (r = configure(c, tps, lay, c->n_inst, L))) return r;
-  c->err.clear();
   int n = c->nunk[0];
-  int ncp = nsteps / K + 1, cpw = P.n_max + P.ne_max;
-  hipError_t e;
   size_t ni = c->n_inst;
+  int ncp = nsteps / K + 1, cpw = P0.n_max + P0.ne_max;
+  hipError_t e;
   double* d_ck = (double*)ws_get(c, "adj_ckpt", sizeof(double) * ni * ncp * cpw, &e);
   double* d_c = (double*)ws_get(c, "adj_cvec", sizeof(double) * n, &e);
   double* d_J = (double*)ws_get(c, "adj_J", sizeof(double) * ni, &e);
@@ -1487,6 +1570,75 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
   }
   AdjArgs aa{h, method == KG_TRAP ? 2.0 : 1.0, nsteps, method, cfg->refine_iters > 0 ? cfg->refine_iters : 2, K,
              method == KG_TRAP ? 1 : 0};
+  // LDS-resident FP64 adjoint (k_adj_lin_fwd / _bwd): mapping 3000 + t
+  const size_t ladj = mixed ? 0 : lin_adj_need(c, s_tr);
+  auto lin_feasible = [&](int cand) {
+    return ladj > 0 && ladj <= LDS_MAX && adj_lin_rows(cand - 3000, P0.n_max) > 0;
+  };
+  int32_t* d_st0 = nullptr;
+  auto run_lin_fwd = [&](int cand) -> hipError_t {
+    int nsrc_max = 0;
+    int32_t rr = KG_OK;
+    double* srcv = adj_srcv(c, s_tr, nsteps, &nsrc_max, &rr);
+    if (rr) return hipErrorOutOfMemory;
+    TranArgs ta{h, aa.alpha, nsteps, method, 0};
+    return run_adj_lin_fwd(cand - 3000, adj_lin_rows(cand - 3000, P0.n_max), dim3((unsigned)ni), ladj, c->stream,
+                           batch_dev(c, P0), 0, ta, K, d_c, d_xop, d_xoff, d_ck, (int64_t)ncp * cpw, cpw, d_J, d_st,
+                           srcv, nsrc_max);
+  };
+  int tps = cfg->threads_per_system;
+  if (tps == 0 && ladj > 0) {  // autotune the forward pass over the feasible mappings
+    d_st0 = (int32_t*)ws_get(c, "adj_status0", sizeof(int32_t) * ni, &e);
+    if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint)");
+    HIPCHK(c, hipMemcpyAsync(d_st0, d_st, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
+    const int tdef = default_tps(c, P0.n_max);
+    int32_t rr = autotune(c, tune_key(c, "adjf", precision, P0, c->n_inst), {tdef, 3064, 3256, 3512},
+                          [&](int cand, float* ms) -> int32_t {
+      if (is_lin_cand(cand) && !lin_feasible(cand)) return set_err(c, KG_UNEXPECTED, "infeasible");
+      HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
+      if (is_lin_cand(cand)) return time_launch(c, [&] { return run_lin_fwd(cand); }, ms);
+      Plan PP;
+      int32_t r2 = prepare(c, AK_TRAN, order_for(cand), s_tr, PP);
+      if (r2) return r2;
+      Lay ly{PP.nnz_max, PP.n_max, PP.ne_max, 0, 5, 1, 8, 8, 0, 0};
+      Launch LL;
+      int pi2 = (cand == 64 || cand == 256) ? PP.total_max : 0;
+      if (pi2 && configure(c, cand, ly, c->n_inst, LL, lay_al(4 * (size_t)pi2)) != KG_OK) pi2 = 0;
+      if (!pi2 && (r2 = configure(c, cand, ly, c->n_inst, LL))) return r2;
+      BatchDev BB = batch_dev(c, PP);
+      return time_launch(c, [&] {
+        return run_adj_fwd(cand, 0, LL.grid, LL.block, LL.lds, c->stream, BB, 0, aa, d_c, d_xop, d_xoff, d_ck,
+                           (int64_t)ncp * cpw, cpw, d_J, d_st, ly, pi2);
+      }, ms);
+    }, &tps);
+    if (rr) return rr;
+    HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (tps == 0) tps = default_tps(c, P0.n_max);
+  if (is_lin_cand(tps)) {
+    if (!lin_feasible(tps))
+      return set_err(c, KG_INVALID_ARGUMENT, "adjoint: the LDS-resident mapping needs FP64 and a fitting system");
+    r = timed(c, "k_adj_lin_fwd", [&] { return run_lin_fwd(tps); });
+    if (r) return r;
+    if (J && (r = copy_out(c, J, d_J, sizeof(double) * ni, mem))) return r;
+    r = finish_status(c, d_st, status, mem);
+    c->adj.ready = true;
+    c->adj.aa = aa;
+    c->adj.tps = tps;
+    c->adj.mixed = 0;
+    c->adj.ncp = ncp;
+    c->adj.cpw = cpw;
+    return r;
+  }
+  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
+  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 0, 5, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
+  Launch L;
+  // stage the plan in LDS for the whole step loop when it fits
+  int pi = (tps == 64 || tps == 256) ? P.total_max : 0;
+  if (pi && configure(c, tps, lay, c->n_inst, L, lay_al(4 * (size_t)pi)) != KG_OK) pi = 0;
+  if (!pi && (r = configure(c, tps, lay, c->n_inst, L))) return r;
+  c->err.clear();
   BatchDev B = batch_dev(c, P);
   r = timed(c, mixed ? "k_adj_fwd<float,double>" : "k_adj_fwd<double,double>", [&] {
     return run_adj_fwd(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, aa, d_c, d_xop, d_xoff, d_ck,
@@ -1513,6 +1665,7 @@ int32_t kg_adjoint_backward(kg_ctx* c, double* grad, int32_t* status, int32_t me
   Plan P;
   double s_tr = aa.alpha / aa.h;
   int32_t r;
+  if (is_lin_cand(c->adj.tps)) return adjoint_backward_lin(c, grad, status, mem);
   if ((r = prepare(c, AK_TRAN, order_for(c->adj.tps), s_tr, P))) return r;
   Lay lay = c->adj.lay;
   Launch L;

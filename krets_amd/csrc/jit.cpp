@@ -343,14 +343,23 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
   o << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_ac_name(mixed, xsplit)
     << "(const double* __restrict__ prm,\n"
        "    const double* __restrict__ ebase, const double* __restrict__ freqs, int nf, i64 nsys,\n"
-       "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err, int refine) {\n"
+       "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err, int refine,\n"
+       "    double stop) {\n"
        "  const i64 sys = xcd_block() * 64 + threadIdx.x;\n"
-       "  if (sys >= nsys) return;\n"
-       "  const int inst = (int)(sys / nf), fi = (int)(sys - (i64)inst * nf);\n"
-    << "  const double* EB = ebase + (i64)inst * " << P.ne << ";\n"
+       "  if (sys >= nsys) return;\n";
+  if (!mixed) {  // consecutive frequencies of one instance per wave
+    o << "  const int inst = (int)(sys / nf), fi = (int)(sys - (i64)inst * nf);\n";
+  } else {
+    // 64 instances at one frequency per wave: refinement converges alike
+    // across the wave, so the adaptive stop applies to whole waves
+    o << "  const i64 ni = nsys / nf, ib = sys / (64 * (i64)nf), rem = sys - ib * 64 * (i64)nf;\n"
+         "  const int nbi = (int)(ni - ib * 64 < 64 ? ni - ib * 64 : 64);\n"
+         "  const int fi = (int)(rem / nbi), inst = (int)(ib * 64 + rem % nbi);\n";
+  }
+  o << "  const double* EB = ebase + (i64)inst * " << P.ne << ";\n"
     << "  const double* PR = prm + (i64)inst * " << (int64_t)P.ne * KG_NPARAM << ";\n"
     << "  const double w = (2.0 * 3.14159265358979323846) * freqs[fi];\n"
-    << "  Z* o = reinterpret_cast<Z*>(out) + sys * " << n << ";\n"
+    << "  Z* o = reinterpret_cast<Z*>(out) + ((i64)inst * nf + fi) * " << n << ";\n"
     << "  bool bad = false;\n";
   if (!mixed) {
     G.on_result = [&](std::ostringstream& O, int i) { O << "    o[" << P.col_of[i] << "] = t" << i << ";\n"; };
@@ -364,7 +373,7 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     } else {
       for (int i = 0; i < n; i++) o << "  Z x" << i << ";\n";
     }
-    o << "  double est = 0.0, d2prev = 0.0, d2last = 0.0, bm2 = 0.0, xm2 = 0.0;\n";
+    o << "  double est = 0.0, d2prev = 0.0, d2last = 0.0, bm2 = 0.0, xm2 = 0.0;\n  int npass = 0;\n";
     // pass 0: complex64 solve of b; x = its FP64 copy (xsplit: x0 = it, dx = 0)
     // |x|_inf is taken from this first solution (the refinement changes it by
     // the FP32 solve error only); tracking it through the refinement passes
@@ -380,9 +389,9 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     G.track_bm = false;
     // refinement: corrections scaled by |b|_inf stay in FP32 range
     o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0;\n  const float sf = (float)s;\n";
-    // one refinement pass inside a run-time loop: the pass body is compiled
-    // once, so its registers are allocated once (an unrolled sequence of
-    // passes lets live ranges of neighbouring passes overlap)
+    // refinement passes after the first run inside a run-time loop: the pass
+    // body is compiled once, so its registers are allocated once (an unrolled
+    // sequence of passes lets live ranges of neighbouring passes overlap)
     G.on_result = [&](std::ostringstream& O, int i) {
       // (xsplit: the correction is formed and accumulated in complex64, the
       // precision dx is kept in; FP64 here costs registers, not accuracy)
@@ -392,18 +401,30 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
       else
         O << "    { const Z d = scal(s, cz(t" << i << ")); x" << i << " = x" << i << " + d; dm2 = fmax(dm2, m2(d)); }\n";
     };
-    o << "  if (s > 0.0)\n"
-         "#pragma clang loop unroll(disable)\n"
-         "  for (int it = 0; it < refine; it++) {\n    double dm2 = 0.0;\n";
+    // the first pass is peeled off the loop: a wave-uniform branch around the
+    // remaining passes (adaptive stop) leaves the loop itself unchanged,
+    // while an exit inside it makes the compiler spill
+    o << "  if (s > 0.0 && refine > 0) {\n  {\n    double dm2 = 0.0;\n";
     G.lookahead = LOOKAHEAD_REFINE;
     if (!G.pass("W", 1)) { why = G.why; return false; }
     // only the squared correction sizes of the last two passes are carried;
     // the estimate is formed after the loop (square roots and divisions in
     // the loop body make the compiler keep far more values live)
-    o << "    d2prev = d2last;\n    d2last = dm2;\n  }\n"
+    o << "    d2prev = d2last;\n    d2last = dm2;\n    npass = 1;\n  }\n";
+    // adaptive stop (stop > 0): a wave leaves after the first pass once every
+    // lane's predicted remaining error, the squared relative correction times
+    // KG_REFINE_SAFETY (refinement contracts by about the relative size of
+    // its correction), is at most `stop`
+    o << "  if (refine > 1 && !(stop > 0.0 && __builtin_amdgcn_ballot_w64(" << KG_REFINE_SAFETY
+      << " * d2last <= stop * xm2) == __builtin_amdgcn_read_exec()))\n"
+         "#pragma clang loop unroll(disable)\n"
+         "  for (int it = 1; it < refine; it++) {\n    double dm2 = 0.0;\n";
+    if (!G.pass("W", 1)) { why = G.why; return false; }
+    o << "    d2prev = d2last;\n    d2last = dm2;\n    npass = it + 1;\n  }\n  }\n"
       << "  {\n    const double dm = sqrt(d2last), dp = sqrt(d2prev), xm = sqrt(xm2);\n"
-      << "    const double rho = (refine > 1 && dp > 0.0) ? fmin(dm / dp, 1.0) : 1.0;\n"
-      << "    est = xm > 0.0 ? rho * dm / xm : 0.0;\n  }\n";
+      << "    const double rho = (npass > 1 && dp > 0.0) ? fmin(dm / dp, 1.0) : 1.0;\n"
+      << "    est = xm > 0.0 ? (npass == 1 && stop > 0.0 ? " << KG_REFINE_SAFETY
+      << " * d2last / xm2 : rho * dm / xm) : 0.0;\n  }\n";
     for (int i = 0; i < n; i++) {
       if (G.xsplit) o << "  o[" << P.col_of[i] << "] = cz(xs[" << 64 * i << " + lane]) + cz(dx" << i << ");\n";
       else o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";

@@ -925,150 +925,134 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
 }
 
 
-// Linear fixed-step transient with the whole step loop in LDS, one system per
-// block of TPS threads, every thread owning at most R rows (n <= R * TPS).
-// LDS holds the compact 16-bit plan (LinHdr), the LU factors, two solution
-// buffers used alternately as x_n and x_{n-1}, and the coefficient and state
-// of every capacitor and inductor companion; source parameters are copied in
-// once. Each array ends in a zero slot (index n, nnz or ndyn) that padded
-// work items point at, so that a thread's loads are unconditional. Source
-// values of every step are tabulated up front in srcv (nsrc_max * (nsteps + 1)
-// doubles per instance), so that no waveform code sits in the step loop; the
-// thread that owns a source row loads each value one step ahead.
+// ------------------------------------------------------------ LDS-resident linear step loops
+// One system per block of TPS threads, every thread owning at most R rows
+// (n <= R * TPS). LDS holds the compact 16-bit plan (LinHdr), the LU factors,
+// two solution buffers used alternately as x_n and x_{n-1}, and the
+// coefficient and state of every capacitor and inductor companion; source
+// parameters are copied in once. Each array ends in a zero slot (index n, nnz
+// or ndyn) that padded work items point at, so that a thread's loads are
+// unconditional. Source values of every step are tabulated up front in a
+// global table (nsrc_max * (nsteps + 1) doubles per instance), so that no
+// waveform code sits in a step loop; the thread that owns a source row loads
+// each value one step ahead.
 //
 // The constant matrix is stamped and factored once from the full plan. Then
-// every thread caches in registers what the plan says about its share of the
+// every thread caches in registers what the plan says about its share of a
 // step: its right-hand-side rows and their first LIN_KC contributions, its
 // positions in the forward and backward level orders with their rows' first
-// LIN_KE column indices, its companion elements and its output columns. A
-// step is then: right-hand side, level-scheduled L and U substitutions, and
-// the companion-state update with the store of the solution row (coalesced,
-// krets numbering); within a phase a thread's LDS reads are independent, one
-// round trip. Entries beyond the cached ones are read from the plan in LDS.
-// Same operations in the same order as k_tran on a linear circuit
+// LIN_KE column indices, its companion elements and its columns in krets
+// numbering. Within a phase a thread's LDS reads are independent: one round
+// trip. Entries beyond the cached ones are read from the plan in LDS. The
+// arithmetic is k_tran's on a linear circuit, operation for operation
 // (solver/transient.rs:14-114 semantics, trapezoidal additions).
-
 constexpr int LIN_KE = 4;  // cached strictly-triangular entries per row
 constexpr int LIN_KC = 2;  // cached right-hand-side contributions per row
 
 template <int TPS, int R>
-__global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, TranArgs ta, const double* xop,
-                                                  const int64_t* xop_off, double* out, const int64_t* out_off,
-                                                  int32_t* status, int32_t* iters, double* err, double* srcv,
-                                                  int nsrc_max) {
-  extern __shared__ __align__(16) char smem[];
-  const int t = threadIdx.x;
-  const int nslot = B.inst_list ? B.n_list : B.n_instances;
-  const int64_t slot = xcd_block();
-  if (slot >= nslot) return;
-  const int inst = B.inst_list ? B.inst_list[slot] : (int)slot;
-  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
-  if (status[inst] != KG_OK) {  // failed operating point or no plan: nothing to integrate
-    if (t == 0 && err) err[inst] = 0.0;
-    return;
-  }
-  if (P.lin == nullptr || P.n > R * TPS || (srcv == nullptr && nsrc_max > 0)) {  // launched only where it fits
-    if (t == 0) status[inst] = KG_UNEXPECTED;
-    return;
-  }
-  {
-    const uint4* src = reinterpret_cast<const uint4*>(P.lin);
-    uint4* dst = reinterpret_cast<uint4*>(smem);
-    for (int i = t; i < P.lin_words / 8; i += TPS) dst[i] = src[i];
-  }
-  __syncthreads();
-  const LinHdr* H = reinterpret_cast<const LinHdr*>(smem);
-  const uint16_t* lw = reinterpret_cast<const uint16_t*>(smem);
-  const int n = H->n, nnz = H->nnz, nlev = H->nlev, nblev = H->nblev, ndyn = H->ndyn, nsrc = H->nsrc;
-  const uint16_t* lvl_ptr = lw + H->o_lvl_ptr;
-  const ushort4* fdesc = reinterpret_cast<const ushort4*>(lw + H->o_fdesc);
-  const uint16_t* blvl_ptr = lw + H->o_blvl_ptr;
-  const ushort4* bdesc = reinterpret_cast<const ushort4*>(lw + H->o_bdesc);
-  const uint16_t* col = lw + H->o_col;
-  const uint16_t* cpos = lw + H->o_cpos;
-  const uint16_t* rcon_ptr = lw + H->o_rcon_ptr;
-  const uint16_t* rcon = lw + H->o_rcon;
-  const uint16_t* dyn = lw + H->o_dyn;
-  const uint16_t* src = lw + H->o_src;
-  const int lev0 = H->lev0_free ? 1 : 0;
-  char* q = smem + lay_al(2 * (size_t)P.lin_words);
-  double* v = reinterpret_cast<double*>(q);  // factors, then a zero slot
-  q += 8 * ((size_t)nnz + 1);
-  double* xc = reinterpret_cast<double*>(q);  // x_n, then a zero slot
-  q += 8 * ((size_t)n + 1);
-  double* xq = reinterpret_cast<double*>(q);  // x_{n-1}, then a zero slot
-  q += 8 * ((size_t)n + 1);
-  double* ed = reinterpret_cast<double*>(q);  // capacitor current / inductor voltage, then a zero slot
-  q += 8 * ((size_t)ndyn + 1);
-  double* cf = reinterpret_cast<double*>(q);  // alpha * value / h (= 2 C / h for the trapezoidal update), zero slot
-  q += 8 * ((size_t)ndyn + 1);
-  double* sp = reinterpret_cast<double*>(q);  // source parameter records
-  q += 8 * (size_t)KG_NPARAM * nsrc;
-  int* flags = reinterpret_cast<int*>(q);
-  if (t == 0) {
-    flags[0] = KG_OK;
-    v[nnz] = 0.0;
-    xc[n] = xq[n] = 0.0;
-    ed[ndyn] = cf[ndyn] = 0.0;
-  }
-  __syncthreads();
-  const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
-  const double* eb = B.ebase + B.eoff[inst];
-  StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
-  stamp_matrix<double, double, TPS>(P, v, prm, eb, sa, ed, t, true);
-  __syncthreads();
-  factor<double, TPS>(P, v, t, true, &flags[0]);
-  for (int d = t; d < ndyn; d += TPS) {
-    const double val = prm[(int)(dyn[d * LIN_DYN] & 0x3FFF) * KG_NPARAM];
-    cf[d] = ta.alpha * (val / ta.h);
-  }
-  for (int i = t; i < nsrc * KG_NPARAM; i += TPS)
-    sp[i] = prm[(int)src[(i / KG_NPARAM) * LIN_SRC + 1] * KG_NPARAM + i % KG_NPARAM];
-  const double* x0 = xop + xop_off[inst];
-  double* o = out + out_off[inst];
-  for (int j = t; j < n; j += TPS) {
-    const double xv = x0[j];
-    xq[cpos[j]] = xv;
-    o[j] = xv;
-  }
-  if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1)] = 0;
-  __syncthreads();
-  if (flags[0] != KG_OK) {
-    if (t == 0) {
-      status[inst] = flags[0];
-      if (err) err[inst] = 0.0;
-    }
-    return;
-  }
-  const uint32_t NN = (uint32_t)n;  // the zero solution slot stands for ground
-  auto node = [&](uint16_t c) -> uint32_t { return c == LIN_NONE ? NN : (uint32_t)c; };
-  for (int d = t; d < ndyn; d += TPS) {  // inductors start at v(x0), capacitors at 0 A
-    const uint16_t* r = dyn + d * LIN_DYN;
-    ed[d] = (r[0] >> 14) == 2 ? xq[node(r[1])] - xq[node(r[2])] : 0.0;
-  }
-  // source values of steps 1..nsteps (the step loop reads them from here)
-  const int nst1 = ta.nsteps + 1;
-  const bool compat = ta.method == KG_BE_COMPAT, trap = ta.method == KG_TRAP;
-  double* tab = srcv ? srcv + (int64_t)inst * nsrc_max * nst1 : nullptr;
-  for (int i = t; i < nsrc * nst1; i += TPS) {
-    const int sidx = i / nst1, s = i - sidx * nst1;
-    const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
-    const double tsrc = compat ? tprev : tn;  // krets evaluates sources at the guess's time
-    const double* pe = sp + sidx * KG_NPARAM;
-    tab[i] = s == 0 ? 0.0 : (src[sidx * LIN_SRC] == KG_V ? wave_value(pe, tsrc) : pe[0]);
-  }
-  // ---- per-thread caches (registers)
-  // right-hand side: row t + r*TPS; contribution = xq slots a | b << 16,
-  // cf/ed slot d | flags << 16 (bit 0: negate, bit 1: source, source index << 2)
+struct LinSys {
+  // plan views (LDS)
+  const uint16_t *lw, *lvl_ptr, *blvl_ptr, *col, *cpos, *rcon_ptr, *rcon, *dyn, *src;
+  const ushort4 *fdesc, *bdesc;
+  int n, nnz, nlev, nblev, ndyn, nsrc, lev0, nsteps, nst1, t;
+  // values (LDS), source table (global)
+  double *v, *xc, *xq, *ed, *cf, *sp;
+  double* tab;
+  bool compat, trap, with_state;
+  // register caches: right-hand side of row t + r*TPS (contribution = xq
+  // slots a | b << 16, cf/ed slot d | flags << 16: bit 0 negate, bit 1
+  // source, source index << 2); forward / backward positions t + r*TPS
+  // (level, row | first << 16, count | diagonal << 16, column pairs);
+  // companion element t + r*TPS (plus | minus << 16, kind); krets column
+  // t + r*TPS: its permuted slot
   int rhs_n[R];
   uint32_t c_ab[R][LIN_KC], c_df[R][LIN_KC];
-  // forward / backward positions t + r*TPS: level, row | first << 16, count | diag << 16, columns (pairs)
   int f_lev[R], b_lev[R];
   uint32_t f_ip[R], f_nd[R], f_col[R][LIN_KE / 2], b_ip[R], b_nd[R], b_col[R][LIN_KE / 2];
-  // companion elements t + r*TPS: plus | minus << 16, kind; output column slot
   uint32_t u_pm[R];
   int u_kind[R], o_slot[R];
-  auto contrib = [&](int code, uint32_t& ab, uint32_t& df) {
+  int pre_rc, pre_row;
+  double pre;
+
+  __device__ uint32_t node(uint16_t c) const { return c == LIN_NONE ? (uint32_t)n : (uint32_t)c; }
+
+  // Plan copy (the first `words` words) and LDS carve-up; returns the first
+  // free LDS byte. Every thread calls it; ends with a barrier.
+  __device__ char* setup(const PatDev& P, int words, char* smem, const TranArgs& ta, double* srcv, int nsrc_max,
+                         int inst) {
+    t = threadIdx.x;
+    {
+      const uint4* s4 = reinterpret_cast<const uint4*>(P.lin);
+      uint4* d4 = reinterpret_cast<uint4*>(smem);
+      for (int i = t; i < words / 8; i += TPS) d4[i] = s4[i];
+    }
+    __syncthreads();
+    const LinHdr* H = reinterpret_cast<const LinHdr*>(smem);
+    lw = reinterpret_cast<const uint16_t*>(smem);
+    n = H->n, nnz = H->nnz, nlev = H->nlev, nblev = H->nblev, ndyn = H->ndyn, nsrc = H->nsrc;
+    lvl_ptr = lw + H->o_lvl_ptr;
+    fdesc = reinterpret_cast<const ushort4*>(lw + H->o_fdesc);
+    blvl_ptr = lw + H->o_blvl_ptr;
+    bdesc = reinterpret_cast<const ushort4*>(lw + H->o_bdesc);
+    col = lw + H->o_col;
+    cpos = lw + H->o_cpos;
+    rcon_ptr = lw + H->o_rcon_ptr;
+    rcon = lw + H->o_rcon;
+    dyn = lw + H->o_dyn;
+    src = lw + H->o_src;
+    lev0 = H->lev0_free ? 1 : 0;
+    char* q = smem + lay_al(2 * (size_t)words);
+    v = reinterpret_cast<double*>(q);
+    q += 8 * ((size_t)nnz + 1);
+    xc = reinterpret_cast<double*>(q);
+    q += 8 * ((size_t)n + 1);
+    xq = reinterpret_cast<double*>(q);
+    q += 8 * ((size_t)n + 1);
+    ed = reinterpret_cast<double*>(q);
+    q += 8 * ((size_t)ndyn + 1);
+    cf = reinterpret_cast<double*>(q);
+    q += 8 * ((size_t)ndyn + 1);
+    sp = reinterpret_cast<double*>(q);
+    q += 8 * (size_t)KG_NPARAM * nsrc;
+    nsteps = ta.nsteps;
+    nst1 = ta.nsteps + 1;
+    compat = ta.method == KG_BE_COMPAT;
+    trap = ta.method == KG_TRAP;
+    with_state = ta.alpha != 1.0;
+    tab = srcv ? srcv + (int64_t)inst * nsrc_max * nst1 : nullptr;
+    if (t == 0) {
+      v[nnz] = 0.0;
+      xc[n] = xq[n] = 0.0;
+      ed[ndyn] = cf[ndyn] = 0.0;
+    }
+    return q;
+  }
+
+  // Stamps and factors the constant matrix (full plan P), fills the companion
+  // coefficients, source records and the source table. flag: LDS status word.
+  __device__ void factorise(const PatDev& P, const double* prm, const double* eb, const TranArgs& ta, int* flag) {
+    StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
+    stamp_matrix<double, double, TPS>(P, v, prm, eb, sa, ed, t, true);
+    __syncthreads();
+    factor<double, TPS>(P, v, t, true, flag);
+    for (int d = t; d < ndyn; d += TPS) {
+      const double val = prm[(int)(dyn[d * LIN_DYN] & 0x3FFF) * KG_NPARAM];
+      cf[d] = ta.alpha * (val / ta.h);
+    }
+    for (int i = t; i < nsrc * KG_NPARAM; i += TPS)
+      sp[i] = prm[(int)src[(i / KG_NPARAM) * LIN_SRC + 1] * KG_NPARAM + i % KG_NPARAM];
+    __syncthreads();
+    for (int i = t; i < nsrc * nst1; i += TPS) {
+      const int sidx = i / nst1, s = i - sidx * nst1;
+      const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
+      const double tsrc = compat ? tprev : tn;  // krets evaluates sources at the guess's time
+      const double* pe = sp + sidx * KG_NPARAM;
+      tab[i] = s == 0 ? 0.0 : (src[sidx * LIN_SRC] == KG_V ? wave_value(pe, tsrc) : pe[0]);
+    }
+  }
+
+  __device__ void contrib(int code, uint32_t& ab, uint32_t& df) const {
+    const uint32_t NN = (uint32_t)n;
     const int idx = code >> 2, k = code & 3;
     if (k == LK_SRC) {
       ab = NN | (NN << 16);
@@ -1080,74 +1064,85 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
       ab = node(dyn[idx * LIN_DYN + 1]) | (node(dyn[idx * LIN_DYN + 2]) << 16);
       df = (uint32_t)idx | ((k == LK_CP ? 1u : 0u) << 16);
     }
-  };
-  auto level_of = [&](const uint16_t* ptr, int nl, int g) {
-    int l = 0;
-    while (l + 1 < nl && (int)ptr[l + 1] <= g) l++;
-    return l;
-  };
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int g = t + r * TPS;
-    const bool ok = g < n;
-    // right-hand side row g
-    const int k0 = ok ? rcon_ptr[g] : 0, k1 = ok ? rcon_ptr[g + 1] : 0;
-    rhs_n[r] = ok ? k1 - k0 : -1;
-#pragma unroll
-    for (int c = 0; c < LIN_KC; c++) {
-      c_ab[r][c] = NN | (NN << 16);
-      c_df[r][c] = (uint32_t)ndyn;
-      if (k0 + c < k1) contrib(rcon[k0 + c], c_ab[r][c], c_df[r][c]);
-    }
-    // forward position g
-    f_lev[r] = b_lev[r] = -1;
-    if (ok) {
-      ushort4 fd = fdesc[g], bd = bdesc[g];
-      f_lev[r] = level_of(lvl_ptr, nlev, g);
-      f_ip[r] = (uint32_t)fd.x | ((uint32_t)fd.y << 16);
-      f_nd[r] = (uint32_t)(fd.z - fd.y) | ((uint32_t)fd.w << 16);
-      b_lev[r] = level_of(blvl_ptr, nblev, g);
-      b_ip[r] = (uint32_t)bd.x | ((uint32_t)bd.y << 16);
-      b_nd[r] = (uint32_t)(bd.z - bd.y) | ((uint32_t)bd.w << 16);
-#pragma unroll
-      for (int k = 0; k < LIN_KE; k += 2) {
-        const uint32_t c0 = k < fd.z - fd.y ? col[fd.y + k] : NN, c1 = k + 1 < fd.z - fd.y ? col[fd.y + k + 1] : NN;
-        f_col[r][k / 2] = c0 | (c1 << 16);
-        const uint32_t e0 = k < bd.z - bd.y ? col[bd.y + k] : NN, e1 = k + 1 < bd.z - bd.y ? col[bd.y + k + 1] : NN;
-        b_col[r][k / 2] = e0 | (e1 << 16);
-      }
-    }
-    // companion element g, output column g
-    u_kind[r] = g < ndyn ? (int)(dyn[g * LIN_DYN] >> 14) : 0;
-    u_pm[r] = g < ndyn ? node(dyn[g * LIN_DYN + 1]) | (node(dyn[g * LIN_DYN + 2]) << 16) : (NN | (NN << 16));
-    o_slot[r] = ok ? (int)cpos[g] : (int)NN;
   }
-  __syncthreads();  // also publishes the source table within the block
-  const bool with_state = ta.alpha != 1.0;
-  // the first cached source contribution of this thread is prefetched
-  int pre_rc = -1, pre_row = 0;
+
+  // Fills the register caches; the caller puts a barrier before the first
+  // step (the source table must be complete).
+  __device__ void cache() {
+    const uint32_t NN = (uint32_t)n;
+    auto level_of = [&](const uint16_t* ptr, int nl, int g) {
+      int l = 0;
+      while (l + 1 < nl && (int)ptr[l + 1] <= g) l++;
+      return l;
+    };
 #pragma unroll
-  for (int r = 0; r < R; r++)
+    for (int r = 0; r < R; r++) {
+      const int g = t + r * TPS;
+      const bool ok = g < n;
+      const int k0 = ok ? rcon_ptr[g] : 0, k1 = ok ? rcon_ptr[g + 1] : 0;
+      rhs_n[r] = ok ? k1 - k0 : -1;
 #pragma unroll
-    for (int c = 0; c < LIN_KC; c++)
-      if (pre_rc < 0 && ((c_df[r][c] >> 16) & 2u)) {
-        pre_rc = r * LIN_KC + c;
-        pre_row = (int)(c_df[r][c] >> 18) * nst1;
+      for (int c = 0; c < LIN_KC; c++) {
+        c_ab[r][c] = NN | (NN << 16);
+        c_df[r][c] = (uint32_t)ndyn;
+        if (k0 + c < k1) contrib(rcon[k0 + c], c_ab[r][c], c_df[r][c]);
       }
-  double pre = (pre_rc >= 0 && ta.nsteps >= 1) ? tab[pre_row + 1] : 0.0;
-  // one right-hand-side contribution (generic form, from the plan in LDS)
-  auto rhs_one = [&](int code, int s) -> double {
+      f_lev[r] = b_lev[r] = -1;
+      f_ip[r] = f_nd[r] = b_ip[r] = b_nd[r] = 0;
+#pragma unroll
+      for (int k = 0; k < LIN_KE / 2; k++) f_col[r][k] = b_col[r][k] = NN | (NN << 16);
+      if (ok) {
+        const ushort4 fd = fdesc[g], bd = bdesc[g];
+        f_lev[r] = level_of(lvl_ptr, nlev, g);
+        f_ip[r] = (uint32_t)fd.x | ((uint32_t)fd.y << 16);
+        f_nd[r] = (uint32_t)(fd.z - fd.y) | ((uint32_t)fd.w << 16);
+        b_lev[r] = level_of(blvl_ptr, nblev, g);
+        b_ip[r] = (uint32_t)bd.x | ((uint32_t)bd.y << 16);
+        b_nd[r] = (uint32_t)(bd.z - bd.y) | ((uint32_t)bd.w << 16);
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k += 2) {
+          const int fc = fd.z - fd.y, bc = bd.z - bd.y;
+          const uint32_t c0 = k < fc ? col[fd.y + k] : NN, c1 = k + 1 < fc ? col[fd.y + k + 1] : NN;
+          f_col[r][k / 2] = c0 | (c1 << 16);
+          const uint32_t e0 = k < bc ? col[bd.y + k] : NN, e1 = k + 1 < bc ? col[bd.y + k + 1] : NN;
+          b_col[r][k / 2] = e0 | (e1 << 16);
+        }
+      }
+      u_kind[r] = g < ndyn ? (int)(dyn[g * LIN_DYN] >> 14) : 0;
+      u_pm[r] = g < ndyn ? node(dyn[g * LIN_DYN + 1]) | (node(dyn[g * LIN_DYN + 2]) << 16) : (NN | (NN << 16));
+      o_slot[r] = ok ? (int)cpos[g] : (int)NN;
+    }
+    pre_rc = -1;
+    pre_row = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int c = 0; c < LIN_KC; c++)
+        if (pre_rc < 0 && ((c_df[r][c] >> 16) & 2u)) {
+          pre_rc = r * LIN_KC + c;
+          pre_row = (int)(c_df[r][c] >> 18) * nst1;
+        }
+    pre = 0.0;
+  }
+
+  // first prefetch of the source value of step s (after the table barrier)
+  __device__ void prefetch(int s) {
+    if (pre_rc >= 0 && s <= nsteps) pre = tab[pre_row + s];
+  }
+
+  __device__ double rhs_one(int code, int s) const {
     uint32_t ab, df;
     contrib(code, ab, df);
     if ((df >> 16) & 2u) return tab[(int)(df >> 18) * nst1 + s];
     double ieq = -cf[df & 0xFFFF] * (xq[ab & 0xFFFF] - xq[ab >> 16]);
     if (with_state) ieq -= ed[df & 0xFFFF];
     return (df >> 16) & 1u ? -ieq : ieq;
-  };
-  for (int s = 1; s <= ta.nsteps; s++) {
-    // right-hand side (permuted rows)
+  }
+
+  // Right-hand side of step s into xc (permuted rows); prefetches step s+1.
+  __device__ void rhs(int s) {
     const double pre_now = pre;
-    if (pre_rc >= 0 && s < ta.nsteps) pre = tab[pre_row + s + 1];
+    if (pre_rc >= 0 && s < nsteps) pre = tab[pre_row + s + 1];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       if (rhs_n[r] < 0) continue;
@@ -1178,8 +1173,11 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
         for (int k = rcon_ptr[i] + LIN_KC; k < rcon_ptr[i + 1]; k++) acc += rhs_one(rcon[k], s);
       xc[i] = acc;
     }
-    __syncthreads();
-    // L y = b (level 0 rows, when free of lower entries, are already done)
+  }
+
+  // L y = b, U x = y in place in xc; ends with a barrier. Level 0 rows, when
+  // free of lower entries, are the right-hand side itself.
+  __device__ void solve() {
     for (int l = lev0; l < nlev; l++) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -1199,7 +1197,6 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
       }
       __syncthreads();
     }
-    // U x = y (diagonal stored as its reciprocal)
     for (int l = 0; l < nblev; l++) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -1217,40 +1214,490 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
 #pragma unroll
         for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
         for (int p = p0 + LIN_KE; p < p0 + cnt; p++) sum -= v[p] * xc[col[p]];
-        xc[i] = sum * vd;
+        xc[i] = sum * vd;  // diagonal stored as its reciprocal
       }
       __syncthreads();
     }
-    // trapezoidal companion states, then the solution row in krets numbering
-    if (trap) {
+  }
+
+  // Trapezoidal companion states from x_n (xc) and x_{n-1} (xq).
+  __device__ void update() {
+    if (!trap) return;
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const int d = t + r * TPS;
-        if (d >= ndyn) continue;
-        const uint32_t a = u_pm[r] & 0xFFFF, b = u_pm[r] >> 16;
-        const double vn = xc[a] - xc[b], vo = xq[a] - xq[b], c = cf[d], e = ed[d];
-        ed[d] = u_kind[r] == 1 ? c * (vn - vo) - e : vn;
-      }
-      for (int d = t + R * TPS; d < ndyn; d += TPS) {  // companions beyond the cached ones
-        const uint16_t* r = dyn + d * LIN_DYN;
-        const double vn = xc[node(r[1])] - xc[node(r[2])];
-        ed[d] = (r[0] >> 14) == 1 ? cf[d] * (vn - (xq[node(r[1])] - xq[node(r[2])])) - ed[d] : vn;
-      }
+    for (int r = 0; r < R; r++) {
+      const int d = t + r * TPS;
+      if (d >= ndyn) continue;
+      const uint32_t a = u_pm[r] & 0xFFFF, b = u_pm[r] >> 16;
+      const double vn = xc[a] - xc[b], vo = xq[a] - xq[b], c = cf[d], e = ed[d];
+      ed[d] = u_kind[r] == 1 ? c * (vn - vo) - e : vn;  // 2C/h == alpha * C / h for alpha = 2
     }
-    double* os = o + (int64_t)s * n;
-#pragma unroll
-    for (int r = 0; r < R; r++)
-      if (t + r * TPS < n) os[t + r * TPS] = xc[o_slot[r]];
-    if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1) + s] = 1;
-    __syncthreads();
+    for (int d = t + R * TPS; d < ndyn; d += TPS) {  // companions beyond the cached ones
+      const uint16_t* r = dyn + d * LIN_DYN;
+      const double vn = xc[node(r[1])] - xc[node(r[2])];
+      ed[d] = (r[0] >> 14) == 1 ? cf[d] * (vn - (xq[node(r[1])] - xq[node(r[2])])) - ed[d] : vn;
+    }
+  }
+
+  __device__ void swap() {
     double* sw = xc;
     xc = xq;
     xq = sw;
+  }
+
+  // initial companion states from the state in xq: inductors at v(x0),
+  // capacitors at 0 A
+  __device__ void init_state() {
+    for (int d = t; d < ndyn; d += TPS) {
+      const uint16_t* r = dyn + d * LIN_DYN;
+      ed[d] = (r[0] >> 14) == 2 ? xq[node(r[1])] - xq[node(r[2])] : 0.0;
+    }
+  }
+};
+
+// Fixed-step linear transient (k_tran semantics) from LDS; see LinSys.
+template <int TPS, int R>
+__global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, TranArgs ta, const double* xop,
+                                                  const int64_t* xop_off, double* out, const int64_t* out_off,
+                                                  int32_t* status, int32_t* iters, double* err, double* srcv,
+                                                  int nsrc_max) {
+  extern __shared__ __align__(16) char smem[];
+  const int t = threadIdx.x;
+  const int nslot = B.inst_list ? B.n_list : B.n_instances;
+  const int64_t slot = xcd_block();
+  if (slot >= nslot) return;
+  const int inst = B.inst_list ? B.inst_list[slot] : (int)slot;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+  if (status[inst] != KG_OK) {  // failed operating point or no plan: nothing to integrate
+    if (t == 0 && err) err[inst] = 0.0;
+    return;
+  }
+  if (P.lin == nullptr || P.n > R * TPS || (srcv == nullptr && nsrc_max > 0)) {  // launched only where it fits
+    if (t == 0) status[inst] = KG_UNEXPECTED;
+    return;
+  }
+  LinSys<TPS, R> S;
+  const int words = reinterpret_cast<const LinHdr*>(P.lin)->words_fwd;
+  int* flags = reinterpret_cast<int*>(S.setup(P, words, smem, ta, srcv, nsrc_max, inst));
+  if (t == 0) flags[0] = KG_OK;
+  __syncthreads();
+  S.factorise(P, B.params + B.eoff[inst] * KG_NPARAM, B.ebase + B.eoff[inst], ta, &flags[0]);
+  const double* x0 = xop + xop_off[inst];
+  double* o = out + out_off[inst];
+  for (int j = t; j < S.n; j += TPS) {
+    const double xv = x0[j];
+    S.xq[S.cpos[j]] = xv;
+    o[j] = xv;
+  }
+  if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1)] = 0;
+  __syncthreads();
+  if (flags[0] != KG_OK) {
+    if (t == 0) {
+      status[inst] = flags[0];
+      if (err) err[inst] = 0.0;
+    }
+    return;
+  }
+  S.init_state();
+  S.cache();
+  __syncthreads();  // also publishes the source table within the block
+  S.prefetch(1);
+  for (int s = 1; s <= ta.nsteps; s++) {
+    S.rhs(s);
+    __syncthreads();
+    S.solve();
+    S.update();
+    double* os = o + (int64_t)s * S.n;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (t + r * TPS < S.n) os[t + r * TPS] = S.xc[S.o_slot[r]];
+    if (t == 0 && iters) iters[(int64_t)inst * (ta.nsteps + 1) + s] = 1;
+    __syncthreads();
+    S.swap();
   }
   if (t == 0) {
     status[inst] = flags[0];
     if (err) err[inst] = 0.0;
   }
+}
+
+// Block sum of one double per thread (buf: TPS / 64 doubles of LDS).
+template <int TPS>
+__device__ double lin_block_sum(double v, double* buf) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (TPS == 64) return v;
+  if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < TPS / 64; w++) s += buf[w];
+  __syncthreads();
+  return s;
+}
+
+// Forward pass of the discrete adjoint from LDS (linear circuits, clean BE
+// or trapezoidal; k_adj_fwd semantics): the transient, the objective
+// J = sum_s w_s c.x_s (trapezoid weights) and a checkpoint every K steps of
+// the state: the solution (permuted order) then the companion states.
+// ckpt: [instance][ncp][cpw], cpw >= n + ndyn; cvec in krets numbering.
+template <int TPS, int R>
+__global__ void __launch_bounds__(TPS) k_adj_lin_fwd(BatchDev B, int kind_slot, TranArgs ta, int K,
+                                                     const double* cvec, const double* xop, const int64_t* xop_off,
+                                                     double* ckpt, int64_t ck_stride, int cpw, double* Jout,
+                                                     int32_t* status, double* srcv, int nsrc_max) {
+  extern __shared__ __align__(16) char smem[];
+  const int t = threadIdx.x;
+  const int inst = (int)xcd_block();
+  if (inst >= B.n_instances) return;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+  if (status[inst] != KG_OK) return;
+  if (P.lin == nullptr || P.n > R * TPS || (srcv == nullptr && nsrc_max > 0)) {
+    if (t == 0) status[inst] = KG_UNEXPECTED;
+    return;
+  }
+  LinSys<TPS, R> S;
+  const LinHdr* Hg = reinterpret_cast<const LinHdr*>(P.lin);
+  char* q = S.setup(P, Hg->words, smem, ta, srcv, nsrc_max, inst);
+  q += 8 * (4 * ((size_t)S.n + 1) + (size_t)S.ndyn + 1 + (size_t)Hg->ne + 1);  // backward-only arrays
+  double* red = reinterpret_cast<double*>(q);
+  int* flags = reinterpret_cast<int*>(q + 8 * 32);
+  if (t == 0) flags[0] = KG_OK;
+  __syncthreads();
+  S.factorise(P, B.params + B.eoff[inst] * KG_NPARAM, B.ebase + B.eoff[inst], ta, &flags[0]);
+  const double* x0 = xop + xop_off[inst];
+  for (int j = t; j < S.n; j += TPS) S.xq[S.cpos[j]] = x0[j];
+  __syncthreads();
+  if (flags[0] != KG_OK) {
+    if (t == 0) status[inst] = flags[0];
+    return;
+  }
+  S.init_state();
+  S.cache();
+  double cv[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) cv[r] = t + r * TPS < S.n ? cvec[t + r * TPS] : 0.0;
+  __syncthreads();
+  S.prefetch(1);
+  double* ck = ckpt + (int64_t)inst * ck_stride;
+  double jpart = 0.0;
+  for (int s = 0; s <= ta.nsteps; s++) {
+    if (s > 0) {
+      S.rhs(s);
+      __syncthreads();
+      S.solve();
+      S.update();
+      __syncthreads();
+      S.swap();  // xq: x_s
+    }
+    const double w = (s == 0 || s == ta.nsteps) ? 0.5 * ta.h : ta.h;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (t + r * TPS < S.n) jpart += w * cv[r] * S.xq[S.o_slot[r]];
+    if (s % K == 0) {
+      double* c = ck + (int64_t)(s / K) * cpw;
+      for (int j = t; j < S.n; j += TPS) c[j] = S.xq[j];
+      for (int d = t; d < S.ndyn; d += TPS) c[S.n + d] = S.ed[d];
+    }
+  }
+  const double J = lin_block_sum<TPS>(jpart, red);
+  if (t == 0) {
+    Jout[inst] = J;
+    status[inst] = flags[0];
+  }
+}
+
+// Backward pass of the discrete adjoint from LDS (k_adj_bwd semantics): per
+// checkpoint segment, last to first, recompute the segment's solutions into
+// seg (permuted order), then run each step's adjoint backwards: the
+// companions' adjoint loads, lambda = A^-T xbar by the transposed level
+// solves, the gradient terms of every R, C and L and the companions' adjoint
+// states. The recomputed state x_{s-1} is loaded into registers one step
+// ahead and staged into LDS at the end of step s. xbar0 receives dJ/dx_0
+// (krets numbering) for the operating-point term; grad: [instance][ne].
+template <int TPS, int R>
+__global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, TranArgs ta, int K,
+                                                     const double* cvec, const double* ckpt, int64_t ck_stride,
+                                                     int cpw, double* seg, int64_t seg_stride, double* grad,
+                                                     int64_t grad_stride, double* xbar0, const int64_t* xop_off,
+                                                     int32_t* status, double* srcv, int nsrc_max) {
+  extern __shared__ __align__(16) char smem[];
+  const int t = threadIdx.x;
+  const int inst = (int)xcd_block();
+  if (inst >= B.n_instances) return;
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+  if (status[inst] != KG_OK) return;
+  if (P.lin == nullptr || P.n > R * TPS || (srcv == nullptr && nsrc_max > 0)) {
+    if (t == 0) status[inst] = KG_UNEXPECTED;
+    return;
+  }
+  LinSys<TPS, R> S;
+  const LinHdr* Hg = reinterpret_cast<const LinHdr*>(P.lin);
+  char* q = S.setup(P, Hg->words, smem, ta, srcv, nsrc_max, inst);
+  const LinHdr* H = reinterpret_cast<const LinHdr*>(smem);
+  const int n = S.n, ndyn = S.ndyn, nadj = H->nadj, ne = H->ne, ntlev = H->ntlev, ntblev = H->ntblev;
+  const uint16_t* tlvl_ptr = S.lw + H->o_tlvl_ptr;
+  const ushort4* tfdesc = reinterpret_cast<const ushort4*>(S.lw + H->o_tfdesc);
+  const uint16_t* tblvl_ptr = S.lw + H->o_tblvl_ptr;
+  const ushort4* tbdesc = reinterpret_cast<const ushort4*>(S.lw + H->o_tbdesc);
+  const uint16_t* tent = S.lw + H->o_tent;
+  const uint16_t* adj = S.lw + H->o_adj;
+  const int tl0 = H->tlev0_free ? 1 : 0;
+  double* xbar = reinterpret_cast<double*>(q);  // adjoint load of the current step
+  q += 8 * ((size_t)n + 1);
+  double* xb0 = reinterpret_cast<double*>(q);  // its part that flows to the previous step
+  q += 8 * ((size_t)n + 1);
+  double* XA = reinterpret_cast<double*>(q);  // staged recomputed states
+  q += 8 * ((size_t)n + 1);
+  double* XB = reinterpret_cast<double*>(q);
+  q += 8 * ((size_t)n + 1);
+  double* sbar = reinterpret_cast<double*>(q);  // companion adjoint states
+  q += 8 * ((size_t)ndyn + 1);
+  double* gr = reinterpret_cast<double*>(q);  // gradient per element
+  q += 8 * ((size_t)ne + 1);
+  int* flags = reinterpret_cast<int*>(q + 8 * 32);
+  if (t == 0) {
+    flags[0] = KG_OK;
+    XA[n] = XB[n] = 0.0;
+  }
+  __syncthreads();
+  const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
+  S.factorise(P, prm, B.ebase + B.eoff[inst], ta, &flags[0]);
+  __syncthreads();
+  if (flags[0] != KG_OK) {
+    if (t == 0) status[inst] = flags[0];
+    return;
+  }
+  S.cache();
+  // transposed-solve caches: position t + r*TPS of the U^T and L^T level
+  // orders (level, row | first << 16, count | diagonal << 16, entries
+  // column | slot << 16; padding points at the zero slots)
+  int tf_lev[R], tb_lev[R];
+  uint32_t tf_ip[R], tf_nd[R], tf_e[R][LIN_KE], tb_ip[R], tb_nd[R], tb_e[R][LIN_KE];
+  double cv[R];
+  {
+    auto level_of = [&](const uint16_t* ptr, int nl, int g) {
+      int l = 0;
+      while (l + 1 < nl && (int)ptr[l + 1] <= g) l++;
+      return l;
+    };
+    const uint32_t pad = (uint32_t)n | ((uint32_t)S.nnz << 16);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int g = t + r * TPS;
+      const bool ok = g < n;
+      tf_lev[r] = tb_lev[r] = -1;
+      tf_ip[r] = tf_nd[r] = tb_ip[r] = tb_nd[r] = 0;
+#pragma unroll
+      for (int k = 0; k < LIN_KE; k++) tf_e[r][k] = tb_e[r][k] = pad;
+      cv[r] = ok ? cvec[g] : 0.0;
+      if (ok) {
+        const ushort4 fd = tfdesc[g], bd = tbdesc[g];
+        tf_lev[r] = level_of(tlvl_ptr, ntlev, g);
+        tf_ip[r] = (uint32_t)fd.x | ((uint32_t)fd.y << 16);
+        tf_nd[r] = (uint32_t)(fd.z - fd.y) | ((uint32_t)fd.w << 16);
+        tb_lev[r] = level_of(tblvl_ptr, ntblev, g);
+        tb_ip[r] = (uint32_t)bd.x | ((uint32_t)bd.y << 16);
+        tb_nd[r] = (uint32_t)(bd.z - bd.y);
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) {
+          if (k < fd.z - fd.y)
+            tf_e[r][k] = (uint32_t)tent[2 * (fd.y + k)] | ((uint32_t)tent[2 * (fd.y + k) + 1] << 16);
+          if (k < bd.z - bd.y)
+            tb_e[r][k] = (uint32_t)tent[2 * (bd.y + k)] | ((uint32_t)tent[2 * (bd.y + k) + 1] << 16);
+        }
+      }
+    }
+  }
+  const double h = ta.h, al = ta.alpha;
+  const bool trap = S.trap;
+  // xbar_N = (h/2) c; no companion adjoint yet; gradients 0
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (t + r * TPS < n) {
+      xbar[S.o_slot[r]] = 0.5 * h * cv[r];
+      xb0[S.o_slot[r]] = 0.0;
+    }
+  for (int d = t; d <= ndyn; d += TPS) sbar[d] = 0.0;
+  for (int e = t; e <= ne; e += TPS) gr[e] = 0.0;
+  if (t == 0) xbar[n] = xb0[n] = 0.0;
+  __syncthreads();
+  const double* ck = ckpt + (int64_t)inst * ck_stride;
+  double* sg = seg + (int64_t)inst * seg_stride;
+  const int nseg = (ta.nsteps + K - 1) / K;
+  for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+    const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
+    // restore the checkpoint and recompute the segment (solutions of steps
+    // s0..s1 into seg, permuted order)
+    {
+      const double* c = ck + (int64_t)sgi * cpw;
+      for (int j = t; j < n; j += TPS) {
+        const double xv = c[j];
+        S.xq[j] = xv;
+        sg[j] = xv;
+      }
+      for (int d = t; d < ndyn; d += TPS) S.ed[d] = c[n + d];
+    }
+    S.prefetch(s0 + 1);
+    __syncthreads();
+    for (int s = s0 + 1; s <= s1; s++) {
+      S.rhs(s);
+      __syncthreads();
+      S.solve();
+      S.update();
+      if (s < s1) {  // x_{s1} stays in LDS
+        double* so = sg + (int64_t)(s - s0) * n;
+        for (int j = t; j < n; j += TPS) so[j] = S.xc[j];
+      }
+      __syncthreads();
+      S.swap();
+    }
+    // stage x_{s1} (LDS) and x_{s1-1} (the block's own stores: row 0 is the
+    // checkpoint); lambda lives in the recompute's free solution buffer
+    double* X1 = XA;
+    double* Xs = XB;
+    double* lam = S.xc;
+    for (int j = t; j < n; j += TPS) {
+      X1[j] = S.xq[j];
+      Xs[j] = sg[(int64_t)(s1 - 1 - s0) * n + j];
+    }
+    __syncthreads();
+    for (int s = s1 - 1; s >= s0; s--) {
+      // x_{s-1} for the next step, into registers now
+      double nx[R];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        nx[r] = (s > s0 && t + r * TPS < n) ? sg[(int64_t)(s - 1 - s0) * n + t + r * TPS] : 0.0;
+      // companions' adjoint loads of step s+1 (ic_{s+1} and vl_{s+1} depend
+      // on x_{s+1}, x_s and ic_s)
+      if (trap) {
+        for (int d = t; d < ndyn; d += TPS) {
+          const uint16_t* rd = S.dyn + d * LIN_DYN;
+          const double sb1 = sbar[d];
+          if (sb1 == 0.0) continue;
+          const int p = (int)S.node(rd[1]), m = (int)S.node(rd[2]);
+          if ((rd[0] >> 14) == 1) {
+            const double a = S.cf[d] * sb1;  // 2C/h * sb1 (alpha = 2)
+            if (p < n) { atomicAdd(&xbar[p], a); atomicAdd(&xb0[p], -a); }
+            if (m < n) { atomicAdd(&xbar[m], -a); atomicAdd(&xb0[m], a); }
+            const double d1 = X1[p] - X1[m], d0 = Xs[p] - Xs[m];
+            gr[rd[0] & 0x3FFF] += sb1 * 2.0 / h * (d1 - d0);
+          } else {
+            if (p < n) atomicAdd(&xbar[p], sb1);
+            if (m < n) atomicAdd(&xbar[m], -sb1);
+          }
+        }
+        __syncthreads();
+      }
+      // lambda = A^-T xbar: U^T forward (level 0 fused with the copy), then
+      // L^T backward (level 0 rows, when free of entries, are done)
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (tf_lev[r] < 0) continue;
+        const int i = (int)(tf_ip[r] & 0xFFFF);
+        lam[i] = tl0 && tf_lev[r] == 0 ? xbar[i] * S.v[tf_nd[r] >> 16] : xbar[i];
+      }
+      __syncthreads();
+      for (int l = tl0; l < ntlev; l++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (tf_lev[r] != l) continue;
+          const int i = (int)(tf_ip[r] & 0xFFFF), q0 = (int)(tf_ip[r] >> 16), cnt = (int)(tf_nd[r] & 0xFFFF);
+          double sum = lam[i];
+          const double vd = S.v[tf_nd[r] >> 16];
+          double vv[LIN_KE], xx[LIN_KE];
+#pragma unroll
+          for (int k = 0; k < LIN_KE; k++) {
+            vv[k] = S.v[tf_e[r][k] >> 16];
+            xx[k] = lam[tf_e[r][k] & 0xFFFF];
+          }
+#pragma unroll
+          for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+          for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
+          lam[i] = sum * vd;
+        }
+        __syncthreads();
+      }
+      for (int l = tl0; l < ntblev; l++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          if (tb_lev[r] != l) continue;
+          const int i = (int)(tb_ip[r] & 0xFFFF), q0 = (int)(tb_ip[r] >> 16), cnt = (int)(tb_nd[r] & 0xFFFF);
+          double sum = lam[i];
+          double vv[LIN_KE], xx[LIN_KE];
+#pragma unroll
+          for (int k = 0; k < LIN_KE; k++) {
+            vv[k] = S.v[tb_e[r][k] >> 16];
+            xx[k] = lam[tb_e[r][k] & 0xFFFF];
+          }
+#pragma unroll
+          for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+          for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
+          lam[i] = sum;
+        }
+        __syncthreads();
+      }
+      // element terms of step s -> s+1
+      for (int a = t; a < nadj; a += TPS) {
+        const uint16_t* ra = adj + a * LIN_ADJ;
+        const int kind = ra[0] >> 14, e = ra[0] & 0x3FFF;
+        const double v = prm[e * KG_NPARAM];
+        const int p = (int)S.node(ra[1]), m = (int)S.node(ra[2]);
+        if (kind == 2) {  // inductor: branch column c, branch row rc
+          const int c = (int)ra[5], rc = (int)ra[6], d = (int)ra[7];
+          const double lb = lam[rc];
+          atomicAdd(&xb0[c], -al * (v / h) * lb);
+          sbar[d] = trap ? -lb : 0.0;
+          gr[e] += lb * (-al / h) * Xs[c] + al / h * lb * X1[c];
+          continue;
+        }
+        const double el = lam[S.node(ra[3])] - lam[S.node(ra[4])];
+        const double d1 = X1[p] - X1[m];
+        if (kind == 0) {
+          gr[e] += el * d1 / (v * v);
+        } else {
+          const int d = (int)ra[7];
+          const double d0 = Xs[p] - Xs[m];
+          const double aa = al * (v / h) * el;
+          if (p < n) atomicAdd(&xb0[p], aa);
+          if (m < n) atomicAdd(&xb0[m], -aa);
+          sbar[d] = trap ? (-sbar[d] + el) : 0.0;
+          gr[e] += el * al / h * d0 - al / h * el * d1;
+        }
+      }
+      __syncthreads();
+      // next step's adjoint load; stage x_{s-1}
+      const double w = (s == 0) ? 0.5 * h : h;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (t + r * TPS < n) {
+          const int j = S.o_slot[r];
+          xbar[j] = xb0[j] + w * cv[r];
+          xb0[j] = 0.0;
+          X1[t + r * TPS] = nx[r];  // X1 is free now: it becomes x_{s-1}
+        }
+      __syncthreads();
+      double* sw = X1;
+      X1 = Xs;
+      Xs = sw;
+    }
+  }
+  // vl_0 = v(x_0): its adjoint flows into x_0
+  if (trap) {
+    for (int d = t; d < ndyn; d += TPS) {
+      const uint16_t* rd = S.dyn + d * LIN_DYN;
+      if ((rd[0] >> 14) != 2) continue;
+      const double sb1 = sbar[d];
+      const int p = (int)S.node(rd[1]), m = (int)S.node(rd[2]);
+      if (p < n) atomicAdd(&xbar[p], sb1);
+      if (m < n) atomicAdd(&xbar[m], -sb1);
+    }
+    __syncthreads();
+  }
+  double* xo = xbar0 + xop_off[inst];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+    if (t + r * TPS < n) xo[t + r * TPS] = xbar[S.o_slot[r]];
+  double* g = grad + (int64_t)inst * grad_stride;
+  for (int e = t; e < ne; e += TPS) g[e] = gr[e];
+  if (t == 0) status[inst] = flags[0];
 }
 
 // Backward Euler with local-truncation-error step control (clean mode; no
@@ -2108,6 +2555,37 @@ hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, Batc
   KG_LIN(256, 1) KG_LIN(256, 2) KG_LIN(256, 4)
   KG_LIN(512, 1) KG_LIN(512, 2) KG_LIN(512, 4)
   KG_LIN(1024, 1) KG_LIN(1024, 2)
+#undef KG_LIN
+  return hipErrorInvalidValue;
+}
+
+hipError_t run_adj_lin_fwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                           int K, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
+                           int64_t ck_stride, int cpw, double* J, int32_t* status, double* srcv, int nsrc_max) {
+  const dim3 b(tps);
+#define KG_LIN(T, RR)                                                                                            \
+  if (tps == T && r == RR)                                                                                     \
+    return launch(k_adj_lin_fwd<T, RR>, g, b, lds, st, B, kind_slot, ta, K, cvec, xop, xop_off, ckpt, ck_stride, \
+                  cpw, J, status, srcv, nsrc_max);
+  KG_LIN(64, 1) KG_LIN(64, 2) KG_LIN(64, 4)
+  KG_LIN(256, 1) KG_LIN(256, 2) KG_LIN(256, 4)
+  KG_LIN(512, 1) KG_LIN(512, 2) KG_LIN(512, 4)
+#undef KG_LIN
+  return hipErrorInvalidValue;
+}
+
+hipError_t run_adj_lin_bwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                           int K, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
+                           int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
+                           const int64_t* xop_off, int32_t* status, double* srcv, int nsrc_max) {
+  const dim3 b(tps);
+#define KG_LIN(T, RR)                                                                                        \
+  if (tps == T && r == RR)                                                                                 \
+    return launch(k_adj_lin_bwd<T, RR>, g, b, lds, st, B, kind_slot, ta, K, cvec, ckpt, ck_stride, cpw, seg, \
+                  seg_stride, grad, grad_stride, xbar0, xop_off, status, srcv, nsrc_max);
+  KG_LIN(64, 1) KG_LIN(64, 2) KG_LIN(64, 4)
+  KG_LIN(256, 1) KG_LIN(256, 2) KG_LIN(256, 4)
+  KG_LIN(512, 1) KG_LIN(512, 2) KG_LIN(512, 4)
 #undef KG_LIN
   return hipErrorInvalidValue;
 }

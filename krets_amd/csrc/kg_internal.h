@@ -66,7 +66,15 @@ struct PatLayout {
 //     plus column, minus column, branch column (LIN_NONE for ground / none)
 //   src entry (LIN_SRC words): kind (KG_V / KG_I), element
 //   rcon code: (index << 2) | LK_* (index into src or dyn)
-constexpr int LIN_DYN = 4, LIN_SRC = 2;
+// After the first words_fwd words (all a transient needs) follows what the
+// discrete adjoint adds:
+//   tfdesc / tbdesc (4 words per row, transposed-solve level order): row,
+//     first entry, end entry, diagonal (U^T: strictly lower entries of
+//     column i of U; L^T: of L), entries in tent as (column, LU slot) pairs
+//   adj entry (LIN_ADJ words): (kind << 14) | element (kind 0: R, 1: C,
+//     2: L), plus / minus column, plus / minus row (KCL equations), branch
+//     column, branch row, companion (dyn) index (LIN_NONE where absent)
+constexpr int LIN_DYN = 4, LIN_SRC = 2, LIN_ADJ = 8;
 constexpr uint16_t LIN_NONE = 0xFFFF;
 enum LinCode { LK_SRC = 0, LK_CP = 1, LK_CM = 2, LK_IND = 3 };
 struct LinHdr {
@@ -74,7 +82,11 @@ struct LinHdr {
   int32_t o_lvl_ptr, o_fdesc, o_blvl_ptr, o_bdesc, o_col, o_cpos;
   int32_t o_rcon_ptr, o_rcon, o_dyn, o_src;
   int32_t lev0_free;  // the first forward level has no lower-triangle entries
-  int32_t pad[5];
+  int32_t words_fwd;  // prefix a forward transient copies
+  int32_t ntlev, ntblev, o_tlvl_ptr, o_tfdesc, o_tblvl_ptr, o_tbdesc, o_tent, nadj, o_adj;
+  int32_t tlev0_free;  // the first level of each transposed solve has no entries
+  int32_t ne;          // elements (gradient slots)
+  int32_t pad;
 };
 
 // Device view of one pattern (pointers into uploaded buffer).

@@ -21,9 +21,20 @@ struct JitFn {
   int scratch = 0;       // private (spill) bytes per lane
 };
 
+// Safety factor of the mixed precision kernel's adaptive stop: the error left
+// after a refinement pass is predicted as KG_REFINE_SAFETY * (|dx| / |x|)^2
+// (max norms). On the batched RC ladders the observed ratio of the true
+// error to (|dx| / |x|)^2 stays below 150 (median 2).
+#define KG_REFINE_SAFETY 200.0
+// Default target of that prediction (the kernel's `stop` argument; 0 runs
+// every pass): two orders below the 1e-9 accuracy the mixed path promises.
+constexpr double KG_REFINE_STOP = 1e-11;
+
 // Source of the lane-per-system AC sweep kernel (jit_ac_name) for one pattern
 // (AC kind, shared by every instance); the refinement count of the mixed
-// precision kernel is a launch argument. xsplit (mixed precision only)
+// precision kernel and its adaptive stop are launch arguments (refine: the
+// most passes; stop > 0: a wave leaves once every lane's predicted error is
+// at most stop). xsplit (mixed precision only)
 // keeps the solution as x0 + dx: the complex64 first solution x0 in LDS
 // (512 * n bytes per 64-lane block, launch argument) and the complex64 sum
 // of the corrections dx in registers, instead of an FP64 x in registers. Returns false, with the reason in `why`, when the pattern is

@@ -48,10 +48,22 @@ __host__ __device__ inline size_t sys_bytes(const Lay& L) {
   return s;
 }
 
-// LDS bytes of one system of the linear transient kernel.
+// LDS bytes of one system of the linear transient kernel (k_tran_lin): the
+// plan prefix, factors, two solution buffers, companion states and
+// coefficients (each with a trailing zero slot), source records, flags.
 __host__ __device__ inline size_t lin_bytes(const LinHdr& h) {
-  return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 2 * (size_t)h.ndyn + 5) +
+  return lay_al(2 * (size_t)h.words_fwd) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 2 * (size_t)h.ndyn + 5) +
          8 * (size_t)KG_NPARAM * h.nsrc + 64;
+}
+
+// LDS bytes of one system of the linear adjoint kernels (k_adj_lin_fwd /
+// k_adj_lin_bwd): the whole plan, the transient's arrays, four more solution
+// vectors (adjoint load and its next-step part, two recomputed states), the
+// companion adjoint states, per-element gradients and a reduction buffer.
+__host__ __device__ inline size_t lin_adj_bytes(const LinHdr& h) {
+  return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 2 * (size_t)h.ndyn + 5) +
+         8 * (size_t)KG_NPARAM * h.nsrc + 8 * (4 * ((size_t)h.n + 1) + (size_t)h.ndyn + 1 + (size_t)h.ne + 1 + 32) +
+         64;
 }
 
 struct NewtonArgs {
@@ -112,6 +124,17 @@ hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hip
 hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                         const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
                         int32_t* status, int32_t* iters, double* err, double* srcv, int nsrc_max);
+// Discrete adjoint from LDS (k_adj_lin_fwd / k_adj_lin_bwd), shared topology,
+// one system per block of tps threads (64, 256, 512) owning up to r rows
+// (1, 2, 4); lds >= lin_adj_bytes; checkpoints hold the permuted solution
+// then the companion states (cpw >= n + ndyn).
+hipError_t run_adj_lin_fwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                           int K, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
+                           int64_t ck_stride, int cpw, double* J, int32_t* status, double* srcv, int nsrc_max);
+hipError_t run_adj_lin_bwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                           int K, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
+                           int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
+                           const int64_t* xop_off, int32_t* status, double* srcv, int nsrc_max);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay,

@@ -666,6 +666,7 @@ bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vect
   h.ndyn = ndyn;
   h.ncon = (int32_t)rcon.size();
   h.nsrc = nsrc;
+  h.ne = ne;
   const size_t hw = sizeof(LinHdr) / 2;
   std::vector<uint16_t> w(hw, 0);
   auto put = [&](const uint16_t* p, size_t k) {
@@ -709,6 +710,59 @@ bool build_lin_plan(const PatLayout& L, const std::vector<int32_t>& b, std::vect
   h.o_dyn = put(dyn.data(), dyn.size());
   h.o_src = put(src.data(), src.size());
   while (w.size() % 8) w.push_back(0);  // 16-byte multiple
+  h.words_fwd = (int32_t)w.size();
+  // ---- discrete adjoint: transposed solves and R / C / L element records
+  {
+    const int32_t* utp = b.data() + L.o_ut_ptr;
+    const int32_t* ut = b.data() + L.o_ut;
+    const int32_t* ltp = b.data() + L.o_lt_ptr;
+    const int32_t* lt = b.data() + L.o_lt;
+    std::vector<uint16_t> tent, tf, tb;
+    auto tdescs = [&](int o_ptr, int o_rows, int nl, const int32_t* pp, const int32_t* pv, bool upper,
+                      std::vector<uint16_t>& d) {
+      const int32_t* lp = b.data() + o_ptr;
+      const int32_t* lr = b.data() + o_rows;
+      for (int q = 0; q < lp[nl]; q++) {
+        const int i = lr[q];
+        const int q0 = (int)tent.size() / 2;
+        for (int k = pp[i]; k < pp[i + 1]; k += 2) tent.insert(tent.end(), {(uint16_t)pv[k], (uint16_t)pv[k + 1]});
+        const int q1 = (int)tent.size() / 2;
+        d.insert(d.end(), {(uint16_t)i, (uint16_t)q0, (uint16_t)q1, (uint16_t)(upper ? dg[i] : 0)});
+      }
+    };
+    tdescs(L.o_tlvl_ptr, L.o_tlvl_rows, L.ntlev, utp, ut, true, tf);
+    tdescs(L.o_tblvl_ptr, L.o_tblvl_rows, L.ntblev, ltp, lt, false, tb);
+    if (tent.size() / 2 >= 0xFFFF) return false;
+    const int32_t* erp = b.data() + L.o_erp;
+    const int32_t* erm = b.data() + L.o_erm;
+    const int32_t* erc = b.data() + L.o_erc;
+    std::vector<uint16_t> adj;
+    for (int e = 0; e < ne; e++) {
+      if (et[e] != KG_R && et[e] != KG_C && et[e] != KG_L) continue;
+      const int kind = et[e] == KG_R ? 0 : et[e] == KG_C ? 1 : 2;
+      adj.insert(adj.end(), {(uint16_t)((kind << 14) | e), u16(ep[e]), u16(em[e]), u16(erp[e]), u16(erm[e]),
+                             u16(ec[e]), u16(erc[e]), (uint16_t)(dyn_of[e] >= 0 ? dyn_of[e] : LIN_NONE)});
+    }
+    h.ntlev = L.ntlev;
+    h.ntblev = L.ntblev;
+    h.o_tlvl_ptr = put32(b.data() + L.o_tlvl_ptr, L.ntlev + 1);
+    h.o_tfdesc = put8(tf);
+    h.o_tblvl_ptr = put32(b.data() + L.o_tblvl_ptr, L.ntblev + 1);
+    h.o_tbdesc = put8(tb);
+    h.o_tent = put(tent.data(), tent.size());
+    h.nadj = (int32_t)adj.size() / LIN_ADJ;
+    h.o_adj = put(adj.data(), adj.size());
+    h.tlev0_free = 1;
+    const int32_t* tl = b.data() + L.o_tlvl_ptr;
+    const int32_t* tr = b.data() + L.o_tlvl_rows;
+    for (int q = tl[0]; q < (L.ntlev > 0 ? tl[1] : 0); q++)
+      if (utp[tr[q]] != utp[tr[q] + 1]) h.tlev0_free = 0;
+    const int32_t* tbl = b.data() + L.o_tblvl_ptr;
+    const int32_t* tbr = b.data() + L.o_tblvl_rows;
+    for (int q = tbl[0]; q < (L.ntblev > 0 ? tbl[1] : 0); q++)
+      if (ltp[tbr[q]] != ltp[tbr[q] + 1]) h.tlev0_free = 0;
+  }
+  while (w.size() % 8) w.push_back(0);
   h.words = (int32_t)w.size();
   // level-0 forward rows have no strictly lower entries: their forward step
   // is the right-hand side itself

@@ -135,9 +135,11 @@ class Plan:
 
 # ------------------------------------------------------------- compact plan
 LIN_HDR = ["n", "nnz", "nlev", "nblev", "ndyn", "ncon", "nsrc", "words", "o_lvl_ptr", "o_fdesc", "o_blvl_ptr",
-           "o_bdesc", "o_col", "o_cpos", "o_rcon_ptr", "o_rcon", "o_dyn", "o_src", "lev0_free"]
-LIN_HDR_WORDS = 48  # sizeof(LinHdr) / 2
-LIN_DYN, LIN_SRC, LIN_NONE = 4, 2, 0xFFFF
+           "o_bdesc", "o_col", "o_cpos", "o_rcon_ptr", "o_rcon", "o_dyn", "o_src", "lev0_free", "words_fwd",
+           "ntlev", "ntblev", "o_tlvl_ptr", "o_tfdesc", "o_tblvl_ptr", "o_tbdesc", "o_tent", "nadj", "o_adj",
+           "tlev0_free", "ne"]
+LIN_HDR_WORDS = 64  # sizeof(LinHdr) / 2
+LIN_DYN, LIN_SRC, LIN_ADJ, LIN_NONE = 4, 2, 8, 0xFFFF
 LK_SRC, LK_CP, LK_CM, LK_IND = 0, 1, 2, 3
 DYN_C, DYN_L, KG_V, KG_I = 1, 2, 3, 4
 LIN = 3  # kg_plan_export kind of the compact plan
@@ -177,6 +179,15 @@ class LinPlan:
         self.src = a("src", LIN_SRC * h["nsrc"]).reshape(-1, LIN_SRC)
         self.dkind = self.dyn[:, 0] >> 14
         self.delem = self.dyn[:, 0] & 0x3FFF
+        assert h["words_fwd"] <= h["words"] and h["words_fwd"] % 8 == 0
+        for name in ("lvl_ptr", "fdesc", "blvl_ptr", "bdesc", "col", "cpos", "rcon_ptr", "rcon", "dyn", "src"):
+            assert h["o_" + name] < h["words_fwd"], name  # the forward kernels copy only this prefix
+        self.tlvl_ptr, self.tblvl_ptr = a("tlvl_ptr", h["ntlev"] + 1), a("tblvl_ptr", h["ntblev"] + 1)
+        self.tfdesc = a("tfdesc", 4 * n, 4).reshape(-1, 4)
+        self.tbdesc = a("tbdesc", 4 * n, 4).reshape(-1, 4)
+        nt = int(max(self.tfdesc[:, 2].max(initial=0), self.tbdesc[:, 2].max(initial=0)))
+        self.tent = a("tent", 2 * nt).reshape(-1, 2)
+        self.adj = a("adj", LIN_ADJ * h["nadj"]).reshape(-1, LIN_ADJ)
 
     def check_indices(self, ne, full=None):
         """Every index the kernel dereferences, against the buffer it indexes;
@@ -210,6 +221,25 @@ class LinPlan:
         if h["lev0_free"]:
             for i, p0, p1, _ in self.fdesc[self.lvl_ptr[0]:self.lvl_ptr[1]]:
                 assert p0 == p1
+        # adjoint extension
+        for ptr, desc, nl in ((self.tlvl_ptr, self.tfdesc, h["ntlev"]), (self.tblvl_ptr, self.tbdesc, h["ntblev"])):
+            assert ptr[0] == 0 and ptr[nl] == n and np.all(np.diff(ptr) >= 0)
+            assert sorted(desc[:, 0].tolist()) == list(range(n))
+            assert np.all(desc[:, 1] <= desc[:, 2]) and np.all(desc[:, 2] <= len(self.tent))
+        assert np.all(self.tfdesc[:, 3] < nnz)
+        assert np.all(self.tent[:, 0] < n) and np.all(self.tent[:, 1] < nnz)
+        if h["tlev0_free"]:
+            for ptr, desc in ((self.tlvl_ptr, self.tfdesc), (self.tblvl_ptr, self.tbdesc)):
+                for _, q0, q1, _ in desc[ptr[0]:ptr[1]]:
+                    assert q0 == q1
+        for r in self.adj:
+            kind, e = r[0] >> 14, r[0] & 0x3FFF
+            assert kind in (0, 1, 2) and e < ne == h["ne"]
+            assert all(x == LIN_NONE or x < n for x in r[1:7])
+            if kind == 2:
+                assert r[5] < n and r[6] < n and r[7] < h["ndyn"] and self.dkind[r[7]] == DYN_L
+            if kind == 1:
+                assert r[7] < h["ndyn"] and self.dkind[r[7]] == DYN_C
 
 
 def wave_value(p, t):
@@ -285,3 +315,147 @@ def tran_lin(lp, vf, params, x0, h, nsteps, method):
         out[s] = xc[lp.cpos]
         xq = xc
     return out
+
+
+def _lin_setup(lp, vf, params, h, method):
+    n = lp.h["n"]
+    alpha = 2.0 if method == 2 else 1.0
+    vals = np.array([params[int(e), 0] for e in lp.delem])
+    return n, alpha, alpha * (vals / h)
+
+
+def adj_lin(lp, vf, params, x0, h, nsteps, method, cvec, K):
+    """The LDS-resident adjoint (forward with checkpoints, then the backward
+    segments) on the CPU, in the kernels' order of operations; x0 and cvec
+    in krets numbering. Returns (J, grad per element without the
+    operating-point term, xbar0 in krets numbering)."""
+    n, alpha, cf = _lin_setup(lp, vf, params, h, method)
+    trap = method == 2
+    NN = n
+
+    def node(c):
+        return NN if c == LIN_NONE else int(c)
+
+    def step(xq, ed, s):
+        x = np.zeros(n + 1)
+        tsrc = s * h
+        for i in range(n):
+            acc = 0.0
+            for code in lp.rcon[lp.rcon_ptr[i]:lp.rcon_ptr[i + 1]]:
+                k, idx = int(code) & 3, int(code) >> 2
+                if k == LK_SRC:
+                    p = params[int(lp.src[idx, 1])]
+                    acc += wave_value(p, tsrc) if lp.src[idx, 0] == KG_V else p[0]
+                elif k == LK_IND:
+                    rr = -cf[idx] * (xq[node(lp.dyn[idx, 3])] - 0.0)
+                    acc += rr - ed[idx] if alpha != 1.0 else rr
+                else:
+                    r = lp.dyn[idx]
+                    ieq = -cf[idx] * (xq[node(r[1])] - xq[node(r[2])])
+                    if alpha != 1.0:
+                        ieq -= ed[idx]
+                    acc += -ieq if k == LK_CP else ieq
+            x[i] = acc
+        for l in range(lp.h["nlev"]):
+            for i, p0, p1, _ in lp.fdesc[lp.lvl_ptr[l]:lp.lvl_ptr[l + 1]]:
+                x[i] -= sum(vf[p] * x[lp.col[p]] for p in range(p0, p1))
+        for l in range(lp.h["nblev"]):
+            for i, p0, p1, d in lp.bdesc[lp.blvl_ptr[l]:lp.blvl_ptr[l + 1]]:
+                x[i] = (x[i] - sum(vf[p] * x[lp.col[p]] for p in range(p0, p1))) * vf[d]
+        ed = ed.copy()
+        if trap:
+            for j, r in enumerate(lp.dyn):
+                vn = x[node(r[1])] - x[node(r[2])]
+                ed[j] = cf[j] * (vn - (xq[node(r[1])] - xq[node(r[2])])) - ed[j] if lp.dkind[j] == DYN_C else vn
+        return x, ed
+
+    # forward with checkpoints
+    xq = np.zeros(n + 1)
+    xq[lp.cpos] = x0
+    ed = np.array([xq[node(r[1])] - xq[node(r[2])] if k == DYN_L else 0.0 for r, k in zip(lp.dyn, lp.dkind)])
+    ck = {}
+    J = 0.0
+    for s in range(nsteps + 1):
+        if s > 0:
+            xq, ed = step(xq, ed, s)
+        w = 0.5 * h if s in (0, nsteps) else h
+        J += w * float(np.dot(cvec, xq[lp.cpos]))
+        if s % K == 0:
+            ck[s // K] = (xq.copy(), ed.copy())
+    # backward
+    ne = lp.h["ne"]
+    xbar = np.zeros(n + 1)
+    xb0 = np.zeros(n + 1)
+    xbar[lp.cpos] = 0.5 * h * cvec
+    sbar = np.zeros(lp.h["ndyn"] + 1)
+    gr = np.zeros(ne)
+    nseg = (nsteps + K - 1) // K
+    for sgi in range(nseg - 1, -1, -1):
+        s0, s1 = sgi * K, min(sgi * K + K, nsteps)
+        xs = [ck[sgi][0]]
+        e_ = ck[sgi][1]
+        for s in range(s0 + 1, s1 + 1):
+            x, e_ = step(xs[-1], e_, s)
+            xs.append(x)
+        for s in range(s1 - 1, s0 - 1, -1):
+            X1, Xs = xs[s + 1 - s0], xs[s - s0]
+            if trap:
+                for d, r in enumerate(lp.dyn):
+                    sb1 = sbar[d]
+                    if sb1 == 0.0:
+                        continue
+                    p, m = node(r[1]), node(r[2])
+                    if lp.dkind[d] == DYN_C:
+                        a_ = cf[d] * sb1
+                        xbar[p] += a_
+                        xb0[p] -= a_
+                        xbar[m] -= a_
+                        xb0[m] += a_
+                        gr[lp.delem[d]] += sb1 * 2.0 / h * ((X1[p] - X1[m]) - (Xs[p] - Xs[m]))
+                    else:
+                        xbar[p] += sb1
+                        xbar[m] -= sb1
+            lam = np.zeros(n + 1)
+            lam[:n] = xbar[:n]
+            for l in range(lp.h["ntlev"]):
+                for i, q0, q1, d in lp.tfdesc[lp.tlvl_ptr[l]:lp.tlvl_ptr[l + 1]]:
+                    sm = lam[i] - sum(vf[lp.tent[q, 1]] * lam[lp.tent[q, 0]] for q in range(q0, q1))
+                    lam[i] = sm * vf[d]
+            for l in range(lp.h["ntblev"]):
+                for i, q0, q1, _ in lp.tbdesc[lp.tblvl_ptr[l]:lp.tblvl_ptr[l + 1]]:
+                    lam[i] -= sum(vf[lp.tent[q, 1]] * lam[lp.tent[q, 0]] for q in range(q0, q1))
+            for r in lp.adj:
+                kind, e = int(r[0]) >> 14, int(r[0]) & 0x3FFF
+                v = params[e, 0]
+                p, m = node(r[1]), node(r[2])
+                if kind == 2:
+                    c, rc, d = int(r[5]), int(r[6]), int(r[7])
+                    lb = lam[rc]
+                    xb0[c] += -alpha * (v / h) * lb
+                    sbar[d] = -lb if trap else 0.0
+                    gr[e] += lb * (-alpha / h) * Xs[c] + alpha / h * lb * X1[c]
+                    continue
+                el = lam[node(r[3])] - lam[node(r[4])]
+                d1 = X1[p] - X1[m]
+                if kind == 0:
+                    gr[e] += el * d1 / (v * v)
+                else:
+                    d = int(r[7])
+                    d0 = Xs[p] - Xs[m]
+                    aa = alpha * (v / h) * el
+                    xb0[p] += aa
+                    xb0[m] -= aa
+                    sbar[d] = (-sbar[d] + el) if trap else 0.0
+                    gr[e] += el * alpha / h * d0 - alpha / h * el * d1
+            w = 0.5 * h if s == 0 else h
+            for g in range(n):
+                j = lp.cpos[g]
+                xbar[j] = xb0[j] + w * cvec[g]
+                xb0[j] = 0.0
+            xb0[n] = xbar[n] = 0.0
+    if trap:
+        for d, r in enumerate(lp.dyn):
+            if lp.dkind[d] == DYN_L:
+                xbar[node(r[1])] += sbar[d]
+                xbar[node(r[2])] -= sbar[d]
+    return J, gr, xbar[lp.cpos]

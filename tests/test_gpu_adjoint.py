@@ -39,7 +39,7 @@ def rel(g, gr):
     return np.max(np.abs(g - gr)) / np.max(np.abs(gr))
 
 
-@pytest.mark.parametrize("tps", [1, 16, 64])
+@pytest.mark.parametrize("tps", [1, 16, 64, 3064, 3256, 3512])  # 3000 + t: LDS-resident adjoint
 @pytest.mark.parametrize("method", [1, 2])
 def test_adjoint_rlc_vs_oracle(kg, oracle, method, tps):
     say(f"adjoint RLC method={method} tps={tps}")
@@ -110,3 +110,13 @@ def test_adjoint_torch_autograd(kg):
     J2, g2, _ = ctx2.adjoint(1e-9, 200, c)
     assert np.allclose(J.detach().numpy(), J2, rtol=1e-14)
     assert np.allclose(vals.grad.numpy(), g2, rtol=1e-12, atol=0)
+
+
+def test_adjoint_lds_resident_needs_fp64(kg):
+    nets, el, pr, n, names = rlc_batch(2, 4, seed=5)
+    ctx = kg.Context()
+    ctx.load(el, pr, n)
+    c = np.zeros(n)
+    c[names.index("V(n4)")] = 1.0
+    with pytest.raises(kg.KretsGpuError):
+        ctx.adjoint(1e-11, 50, c, precision=kg.MIXED, config=kg.SolverConfig(compat=False, threads_per_system=3256))

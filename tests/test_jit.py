@@ -1,8 +1,8 @@
 """CPU checks of the run-time specialised lane-per-system AC kernel.
 
 The generated HIP source (kg_jit_ac_source) is compiled twice without a GPU:
-  * for gfx950 with hipcc (device only), checking that it builds and, for the
-    FP64 kernel of the 50-section RC ladder, that it needs no scratch memory
+  * for gfx950 with hipcc (device only), checking that it builds and, for
+    every form on the 50-section RC ladder, that it needs no scratch memory
     (the whole system lives in registers);
   * for the host with g++, through a small header that maps the handful of
     HIP names it uses onto host C++, and run lane by lane over a batch: its
@@ -38,6 +38,9 @@ static inline int kg_frexp_exp(double a) { int e = 0; std::frexp(a, &e); return 
 #define __builtin_amdgcn_ldexp(a, e) std::ldexp((a), (e))
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
 #define __builtin_amdgcn_rcp(a) (1.0 / (a))
+// one lane at a time: a wave vote is the lane's own predicate
+#define __builtin_amdgcn_ballot_w64(p) ((p) ? ~0ull : 0ull)
+#define __builtin_amdgcn_read_exec() (~0ull)
 static inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
 static inline unsigned long long atomicMax(unsigned long long* p, unsigned long long v) {
   unsigned long long o = *p; if (v > o) *p = v; return o; }
@@ -47,13 +50,13 @@ using std::fmax; using std::sqrt;
 
 HOST_DRIVER = r"""
 extern "C" void run_host(const double* prm, const double* eb, const double* fr, int nf, long long nsys,
-                         double* out, int* st, double* err, int refine) {
+                         double* out, int* st, double* err, int refine, double stop) {
   gridDim.x = (unsigned)((nsys + 63) / 64);
   for (unsigned b = 0; b < gridDim.x; b++)
     for (unsigned t = 0; t < 64; t++) {
       blockIdx.x = b;
       threadIdx.x = t;
-      KERNEL(prm, eb, fr, nf, nsys, out, st, err, refine);
+      KERNEL(prm, eb, fr, nf, nsys, out, st, err, refine, stop);
     }
 }
 """
@@ -76,7 +79,7 @@ def ebase(elements, params):
     return e
 
 
-def host_run(src, params, elements, freqs, n, refine=2):
+def host_run(src, params, elements, freqs, n, refine=2, stop=0.0):
     d = tempfile.mkdtemp()
     try:
         cpp = os.path.join(d, "k.cpp")
@@ -97,9 +100,9 @@ def host_run(src, params, elements, freqs, n, refine=2):
         st = np.zeros(ni, np.int32)
         err = np.zeros(ni)
         P = ctypes.c_void_p
-        L.run_host.argtypes = [P, P, P, ctypes.c_int, ctypes.c_longlong, P, P, P, ctypes.c_int]
+        L.run_host.argtypes = [P, P, P, ctypes.c_int, ctypes.c_longlong, P, P, P, ctypes.c_int, ctypes.c_double]
         L.run_host(prm.ctypes.data, eb.ctypes.data, fr.ctypes.data, nf, ni * nf, out.ctypes.data, st.ctypes.data,
-                   err.ctypes.data, refine)
+                   err.ctypes.data, refine, stop)
         x = out.reshape(ni, nf, n, 2)
         return x[..., 0] + 1j * x[..., 1], st, err
     finally:
@@ -171,12 +174,16 @@ def test_lane_source_declines_diodes_and_large_systems():
 
 
 @pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-@pytest.mark.parametrize("precision", [0, 1])
-def test_lane_source_builds_for_gfx950(precision):
+@pytest.mark.parametrize("precision,refine", [(0, 2), (1, 2), (1, -1)])  # -1: split solution
+def test_lane_source_builds_for_gfx950(precision, refine):
+    """Every generated form builds for gfx950 and keeps the 50-section
+    ladder in registers: no scratch (spills) for FP64, the mixed form with
+    an FP64 solution, and the split-solution mixed form with its adaptive
+    refinement stop."""
     import krets_amd as kg
     from krets_amd import workloads as W
     b = W.c2_rc_ladder_mc(1, sections=50, seed=2)
-    r, src = source(kg.Circuit(b.netlists[0]), precision)
+    r, src = source(kg.Circuit(b.netlists[0]), precision, refine)
     assert r > 0, src
     d = tempfile.mkdtemp()
     try:
@@ -187,7 +194,31 @@ def test_lane_source_builds_for_gfx950(precision):
                               "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
         assert out.returncode == 0, out.stderr[-2000:]
         scratch = int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1))
-        if precision == 0:
-            assert scratch == 0
+        assert scratch == 0
     finally:
         shutil.rmtree(d)
+
+
+def test_lane_source_adaptive_stop_on_host(oracle):
+    """The mixed kernel's adaptive stop (default target 1e-11 on the
+    predicted error 200 * (|dx| / |x|)^2): on C2-like RC ladders lanes leave
+    after one refinement pass where that suffices, every result stays within
+    1e-9 of FP64, and the reported estimate bounds the true error."""
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    b = W.c2_rc_ladder_mc(16, sections=50, seed=2)
+    c = kg.Circuit(b.netlists[0])
+    f = W.c2_frequencies(200)
+    _, src64 = source(c, 0)
+    x64, _, _ = host_run(src64, b.params, c.elements, f, c.n)
+    _, srcm = source(c, 1, -1)
+    xa, st, err = host_run(srcm, b.params, c.elements, f, c.n, refine=2, stop=1e-11)
+    x1, _, _ = host_run(srcm, b.params, c.elements, f, c.n, refine=1)
+    x2, _, _ = host_run(srcm, b.params, c.elements, f, c.n, refine=2)
+    assert (st == 0).all()
+    e = np.linalg.norm(xa - x64, axis=2) / np.linalg.norm(x64, axis=2)
+    assert e.max() < 1e-9, e.max()
+    stopped = np.all(xa == x1, axis=2)
+    assert 0.2 < stopped.mean() < 0.9, stopped.mean()  # some, not all, leave early
+    assert np.array_equal(xa[~stopped], x2[~stopped])
+    assert (err >= e.max(axis=1)).all() and (err <= 1e-9).all()

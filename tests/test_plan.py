@@ -184,7 +184,7 @@ LIN_CASES = ["low_pass_filter/transient.cir", "high_pass_filter/transient.cir", 
 
 def lin_bytes(h):
     al = lambda b: (b + 15) & ~15
-    return al(2 * h["words"]) + 8 * (h["nnz"] + 2 * h["n"] + 2 * h["ndyn"] + 5) + 8 * 10 * h["nsrc"] + 64
+    return al(2 * h["words_fwd"]) + 8 * (h["nnz"] + 2 * h["n"] + 2 * h["ndyn"] + 5) + 8 * 10 * h["nsrc"] + 64
 
 
 @pytest.mark.parametrize("method", [0, 1, 2])
@@ -230,3 +230,46 @@ def test_lin_plan_sizes():
     assert lin_bytes(lp.h) <= 160 * 1024, lin_bytes(lp.h)
     c = k.Circuit(read_circuit("rectifier/rectifier.cir"))
     assert pe.LinPlan(c.elements, c.n, c.params, 1e5).status == 6  # KG_UNEXPECTED: not linear
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("case", ["rlc4", "rc_ladder"])
+def test_lin_adjoint_vs_oracle(oracle, case, method):
+    """The LDS-resident adjoint's algorithm on the compact plan (transposed
+    level solves, element records, checkpoint segments), run on the CPU,
+    plus the operating-point term, against the oracle's discrete adjoint."""
+    import krets_amd as k
+    from krets_amd import workloads as W
+    if case == "rlc4":
+        text = W.c3_netlist([20.0, 35.0, 50.0, 65.0], [2e-9, 3e-9, 4e-9, 5e-9], [0.3e-12, 0.5e-12, 0.7e-12, 0.9e-12])
+        h, N, K, node = 1e-11, 40, 16, "V(n4)"
+    else:
+        b = W.c5_rc_ladder(1, sections=12, seed=5)
+        text, h, N, K, node = b.netlists[0], 1e-9, 30, 8, "V(n12)"
+    c = k.Circuit(text)
+    alpha = 2.0 if method == 2 else 1.0
+    P = pe.Plan(c.elements, c.n, pe.TRAN, pe.NESTED, c.params, alpha / h)
+    lp = pe.LinPlan(c.elements, c.n, c.params, alpha / h)
+    lp.check_indices(len(c.elements), P)
+    contrib, _ = values(c, pe.TRAN, h=h, alpha=alpha)
+    v, _ = P.gather(contrib)
+    vf = np.append(P.factor(v).real, 0.0)
+    oc = oracle.Circuit(text)
+    cvec = np.zeros(c.n)
+    cvec[c.names.index(node)] = 1.0
+    xr, _ = oc.tran(h, N * h, method=method)
+    J, g, xbar0 = pe.adj_lin(lp, vf, c.params, xr[0], h, N, method, cvec, K)
+    # operating-point term: lam0 = A_dc^-T xbar0, dJ/dR += (e.lam0)(e.x0) / R^2
+    A, _ = oc.assemble_dc()
+    lam0 = np.linalg.solve(A.T, xbar0)
+    x0 = xr[0]
+    for e in range(len(c.elements)):
+        el = c.elements[e]
+        if el["type"] == 0:
+            pl, mi = el["plus"], el["minus"]
+            d = (lam0[pl] if pl >= 0 else 0.0) - (lam0[mi] if mi >= 0 else 0.0)
+            dx = (x0[pl] if pl >= 0 else 0.0) - (x0[mi] if mi >= 0 else 0.0)
+            g[e] += d * dx / c.params[e, 0] ** 2
+    Jr, gr = oc.adjoint(h, N, cvec, method=method)
+    assert abs(J - Jr) <= 1e-12 * abs(Jr)
+    assert np.max(np.abs(g - gr)) <= 1e-9 * np.max(np.abs(gr)), np.max(np.abs(g - gr)) / np.max(np.abs(gr))
