@@ -2,17 +2,21 @@
 // element tables of the C ABI in krets' unknown numbering.
 //
 // Follows crates/krets-parser/src/parser.rs:27-154 (line handling, .control
-// and .subckt skipping, .model binding, index map at 105-125), the element
-// parsers in elements/{resistor,capacitor,inductor,current_source,diode,
-// voltage_source}.rs, utils.rs:32-78 (SI suffixes, value tokens) and
-// models.rs:40-95. Elements are matched in krets' `alt` order
-// (elements.rs:208-229) and, like krets, a line is not required to be
-// consumed entirely.
+// skipping, subcircuit expansion, .model binding, index map at 105-125), the
+// element parsers in elements/{resistor,capacitor,inductor,current_source,
+// diode,voltage_source}.rs, utils.rs:32-78 (SI suffixes, value tokens),
+// models.rs:40-95 and elements/subcircuit.rs:33-200 (definitions collected in
+// a first pass, instances expanded in place: pins map to the instance's
+// nodes, every other node, ground "0" included, and every element name get
+// the instance name and "_" as prefix, nested instances recursively).
+// Elements are matched in krets' `alt` order (elements.rs:44-65) and, like
+// krets, a line is not required to be consumed entirely.
 #include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <set>
 #include <string>
@@ -300,6 +304,52 @@ std::string low(const std::string& s) {
   return r;
 }
 
+// ---- subcircuits (elements/subcircuit.rs)
+struct XInst {
+  std::string iname, def;
+  std::vector<std::string> nodes;
+};
+struct SubItem {
+  bool is_x = false;
+  Rec r;
+  XInst x;
+};
+struct SubDef {
+  std::vector<std::string> pins;
+  std::vector<SubItem> items;
+};
+
+// space1, then an identifier ([A-Za-z0-9_]+); false (c untouched) if absent
+bool next_ident(Scan& c, std::string& out) {
+  Scan d = c;
+  if (!sp1(d) || !ident(d, out)) return false;
+  c = d;
+  return true;
+}
+
+// ".subckt NAME pin..." (subcircuit.rs:128-132)
+bool parse_subckt_header(const std::string& line, std::string& name, std::vector<std::string>& pins) {
+  Scan c{&line};
+  if (!word(c, ".subckt", true) || !next_ident(c, name)) return false;
+  std::string p;
+  while (next_ident(c, p)) pins.push_back(p);
+  return true;
+}
+
+// "X<name> node... DEF" (subcircuit.rs:134-157)
+bool parse_xinst(const std::string& line, XInst& x) {
+  Scan c{&line};
+  if (!word(c, "x", true) || !ident(c, x.iname)) return false;
+  std::vector<std::string> toks;
+  std::string t;
+  while (next_ident(c, t)) toks.push_back(t);
+  if (toks.empty()) return false;
+  x.def = toks.back();
+  toks.pop_back();
+  x.nodes = toks;
+  return true;
+}
+
 }  // namespace
 
 struct kg_circuit_impl {
@@ -317,6 +367,98 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
   if (!text || !out) return fail(KG_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   std::string src(text);
+  auto trimmed = [](std::string line) {
+    size_t a = line.find_first_not_of(" \t\r\n\v\f"), b = line.find_last_not_of(" \t\r\n\v\f");
+    return a == std::string::npos ? std::string() : line.substr(a, b - a + 1);
+  };
+  // one primitive element line, krets' alt order; 0 no match, 1 parsed,
+  // -1 a V line that is not a valid source
+  auto primitive = [](const std::string& line, Rec& r) {
+    int vs = 0;
+    bool ok = parse_two_terminal(line, 'R', r) || parse_two_terminal(line, 'C', r) ||
+              parse_two_terminal(line, 'L', r) || (vs = parse_vsrc(line, r)) == 1;
+    if (!ok && vs == 0) ok = parse_two_terminal(line, 'I', r) || parse_diode(line, r);
+    return ok ? 1 : 0;
+  };
+  // first pass (subcircuit.rs:159-200): definitions anywhere in the text
+  std::map<std::string, SubDef> defs;
+  {
+    std::string cur;
+    bool in_def = false;
+    size_t p0 = 0;
+    while (p0 < src.size()) {
+      size_t e = src.find('\n', p0);
+      if (e == std::string::npos) e = src.size();
+      std::string line = trimmed(src.substr(p0, e - p0));
+      p0 = e + 1;
+      if (line.empty() || line[0] == '%' || line[0] == '*') continue;
+      std::string lo = low(line);
+      if (lo.rfind(".subckt", 0) == 0) {
+        std::string nm;
+        std::vector<std::string> pins;
+        if (!parse_subckt_header(line, nm, pins)) return fail(KG_PARSE, "failed to parse subcircuit header: " + line);
+        defs[nm] = SubDef{pins, {}};
+        cur = nm;
+        in_def = true;
+        continue;
+      }
+      if (lo.rfind(".ends", 0) == 0) {
+        in_def = false;
+        cur.clear();
+        continue;
+      }
+      if (!in_def) continue;
+      SubItem it;
+      if (primitive(line, it.r) != 1) {
+        if (!parse_xinst(line, it.x)) {
+          char f = (char)std::toupper((unsigned char)line[0]);
+          return fail(f == 'Q' || f == 'M' ? KG_INVALID_ELEMENT_FORMAT : KG_PARSE,
+                      "failed to parse subcircuit element in '" + cur + "': " + line);
+        }
+        it.is_x = true;
+      }
+      defs[cur].items.push_back(it);
+    }
+  }
+  // instance expansion (subcircuit.rs:33-110), in place
+  std::function<bool(const XInst&, std::vector<Rec>&, int, std::string&)> expand =
+      [&](const XInst& x, std::vector<Rec>& out, int depth, std::string& why) -> bool {
+    auto d = defs.find(x.def);
+    if (d == defs.end()) {
+      why = "undefined subcircuit definition: " + x.def;
+      return false;
+    }
+    if (x.nodes.size() != d->second.pins.size()) {
+      why = "node/port mismatch for instance " + x.iname + ": expected " + std::to_string(d->second.pins.size()) +
+            ", got " + std::to_string(x.nodes.size());
+      return false;
+    }
+    if (depth > 64) {
+      why = "subcircuit nesting deeper than 64 levels (recursive definition?): " + x.def;
+      return false;
+    }
+    std::map<std::string, std::string> p2n;  // a repeated pin maps to its last node
+    for (size_t k = 0; k < x.nodes.size(); k++) p2n[d->second.pins[k]] = x.nodes[k];
+    auto map = [&](const std::string& nd) {
+      auto f = p2n.find(nd);
+      return f != p2n.end() ? f->second : x.iname + "_" + nd;
+    };
+    for (const SubItem& it : d->second.items) {
+      if (!it.is_x) {
+        Rec r = it.r;
+        r.plus = map(r.plus);
+        r.minus = map(r.minus);
+        r.name = x.iname + "_" + r.name;
+        out.push_back(r);
+      } else {
+        XInst y = it.x;
+        for (auto& nd : y.nodes) nd = map(nd);
+        y.iname = x.iname + "_" + y.iname;
+        if (!expand(y, out, depth + 1, why)) return false;
+      }
+    }
+    return true;
+  };
   auto* C = new kg_circuit_impl();
   std::map<std::string, Model> models;
   bool ctl = false, sub = false;
@@ -350,16 +492,21 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
       continue;
     }
     Rec r;
-    int vs = 0;
-    bool ok = parse_two_terminal(line, 'R', r) || parse_two_terminal(line, 'C', r) ||
-              parse_two_terminal(line, 'L', r) || (vs = parse_vsrc(line, r)) == 1;
-    if (!ok && vs == 0) ok = parse_two_terminal(line, 'I', r) || parse_diode(line, r);
-    if (!ok) {
+    if (primitive(line, r) != 1) {
+      XInst x;
+      if (parse_xinst(line, x)) {
+        std::string why;
+        if (!expand(x, C->recs, 0, why)) {
+          delete C;
+          return fail(KG_PARSE, "line " + std::to_string(lineno) + ": " + why);
+        }
+        continue;
+      }
       char f = (char)std::toupper((unsigned char)line[0]);
       delete C;
-      if (f == 'Q' || f == 'M' || f == 'X')
+      if (f == 'Q' || f == 'M')
         return fail(KG_INVALID_ELEMENT_FORMAT,
-                    "line " + std::to_string(lineno) + ": BJT/MOSFET/subcircuit elements are not supported");
+                    "line " + std::to_string(lineno) + ": BJT/MOSFET elements are not supported");
       return fail(KG_PARSE, "line " + std::to_string(lineno) + ": cannot parse '" + line + "'");
     }
     C->recs.push_back(r);

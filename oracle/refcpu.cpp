@@ -26,6 +26,7 @@
 #include "refcpu.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -431,7 +432,147 @@ std::string lower(const std::string& s) {
   return r;
 }
 
+// ---- subcircuits (elements/subcircuit.rs:33-200): definitions are collected
+// in a first pass over the whole text; an instance "X<name> node... DEF"
+// expands in place, every element of the definition mapped: pin names to the
+// instance's nodes, any other node (ground "0" too) to "<name>_<node>", the
+// element name to "<name>_<element name>"; nested instances the same way.
+struct XInst {
+  std::string iname, def;
+  std::vector<std::string> nodes;
+};
+struct SubItem {
+  bool is_x = false;
+  Elem el;
+  XInst x;
+};
+struct SubDef {
+  std::vector<std::string> pins;
+  std::vector<SubItem> items;
+};
+
+// space1 then [A-Za-z0-9_]+ starting at i (utils.rs:67-69); i advances on success
+bool take_ident(const std::string& s, size_t& i, std::string& out, bool need_space) {
+  size_t j = i;
+  if (need_space) {
+    size_t b = j;
+    while (j < s.size() && (s[j] == ' ' || s[j] == '\t')) j++;
+    if (j == b) return false;
+  }
+  size_t b = j;
+  while (j < s.size() && (std::isalnum((unsigned char)s[j]) || s[j] == '_')) j++;
+  if (j == b) return false;
+  out = s.substr(b, j - b);
+  i = j;
+  return true;
+}
+
+bool parse_subckt_header(const std::string& line, std::string& name, std::vector<std::string>& pins) {
+  if (line.size() < 7 || lower(line.substr(0, 7)) != ".subckt") return false;
+  size_t i = 7;
+  if (!take_ident(line, i, name, true)) return false;
+  std::string p;
+  while (take_ident(line, i, p, true)) pins.push_back(p);
+  return true;
+}
+
+bool parse_x_instance(const std::string& line, XInst& x) {
+  if (line.empty() || (line[0] != 'x' && line[0] != 'X')) return false;
+  size_t i = 1;
+  if (!take_ident(line, i, x.iname, false)) return false;
+  std::vector<std::string> toks;
+  std::string t;
+  while (take_ident(line, i, t, true)) toks.push_back(t);
+  if (toks.empty()) return false;
+  x.def = toks.back();
+  toks.pop_back();
+  x.nodes = toks;
+  return true;
+}
+
+bool parse_primitive(const std::string& line, Elem& el) {
+  bool hard = false;
+  if (parse_rlci(line, 'R', E_R, el)) return true;
+  if (parse_rlci(line, 'C', E_C, el)) return true;
+  if (parse_rlci(line, 'L', E_L, el)) return true;
+  if (parse_vsource(line, el, hard)) return true;
+  if (!hard && parse_rlci(line, 'I', E_I, el)) return true;
+  return !hard && parse_diode_line(line, el);
+}
+
+void expand_instance(const XInst& x, const std::map<std::string, SubDef>& defs, std::vector<Elem>& out, int depth) {
+  auto d = defs.find(x.def);
+  if (d == defs.end()) throw RefError{REF_PARSE, "undefined subcircuit definition: " + x.def};
+  if (x.nodes.size() != d->second.pins.size())
+    throw RefError{REF_PARSE, "node/port mismatch for instance " + x.iname};
+  if (depth > 64) throw RefError{REF_PARSE, "subcircuit nesting too deep: " + x.def};
+  std::map<std::string, std::string> p2n;
+  for (size_t k = 0; k < x.nodes.size(); k++) p2n[d->second.pins[k]] = x.nodes[k];
+  auto map = [&](const std::string& nd) {
+    auto f = p2n.find(nd);
+    return f != p2n.end() ? f->second : x.iname + "_" + nd;
+  };
+  for (const SubItem& it : d->second.items) {
+    if (!it.is_x) {
+      Elem e = it.el;
+      e.plus = map(e.plus);
+      e.minus = map(e.minus);
+      e.name = x.iname + "_" + e.name;
+      out.push_back(e);
+    } else {
+      XInst y = it.x;
+      for (auto& nd : y.nodes) nd = map(nd);
+      y.iname = x.iname + "_" + y.iname;
+      expand_instance(y, defs, out, depth + 1);
+    }
+  }
+}
+
+std::map<std::string, SubDef> collect_subcircuits(const std::string& text) {
+  std::map<std::string, SubDef> defs;
+  std::string cur;
+  bool in_def = false;
+  size_t pos = 0;
+  while (pos < text.size()) {
+    size_t e = text.find('\n', pos);
+    if (e == std::string::npos) e = text.size();
+    std::string line = text.substr(pos, e - pos);
+    pos = e + 1;
+    size_t a = line.find_first_not_of(" \t\r\n\v\f");
+    size_t b = line.find_last_not_of(" \t\r\n\v\f");
+    line = (a == std::string::npos) ? "" : line.substr(a, b - a + 1);
+    if (line.empty() || line[0] == '%' || line[0] == '*') continue;
+    std::string lo = lower(line);
+    if (lo.rfind(".subckt", 0) == 0) {
+      std::string nm;
+      std::vector<std::string> pins;
+      if (!parse_subckt_header(line, nm, pins)) throw RefError{REF_PARSE, "bad subcircuit header: " + line};
+      defs[nm] = SubDef{pins, {}};
+      cur = nm;
+      in_def = true;
+      continue;
+    }
+    if (lo.rfind(".ends", 0) == 0) {
+      in_def = false;
+      continue;
+    }
+    if (!in_def) continue;
+    SubItem it;
+    if (!parse_primitive(line, it.el)) {
+      if (!parse_x_instance(line, it.x)) {
+        char f = (char)std::toupper((unsigned char)line[0]);
+        throw RefError{f == 'Q' || f == 'M' ? REF_INVALID_ELEMENT_FORMAT : REF_PARSE,
+                       "bad element in subcircuit " + cur + ": " + line};
+      }
+      it.is_x = true;
+    }
+    defs[cur].items.push_back(it);
+  }
+  return defs;
+}
+
 Circuit* parse_netlist(const std::string& text) {
+  const std::map<std::string, SubDef> defs = collect_subcircuits(text);
   auto* C = new Circuit();
   std::map<std::string, DModel> models;
   bool in_ctl = false, in_sub = false;
@@ -467,17 +608,20 @@ Circuit* parse_netlist(const std::string& text) {
       continue;
     }
     Elem el;
-    bool ok = false, hard = false;
-    if (parse_rlci(line, 'R', E_R, el)) ok = true;
-    else if (parse_rlci(line, 'C', E_C, el)) ok = true;
-    else if (parse_rlci(line, 'L', E_L, el)) ok = true;
-    else if (parse_vsource(line, el, hard)) ok = true;
-    else if (!hard && parse_rlci(line, 'I', E_I, el)) ok = true;
-    else if (!hard && parse_diode_line(line, el)) ok = true;
-    if (!ok) {
+    if (!parse_primitive(line, el)) {
+      XInst x;
+      if (parse_x_instance(line, x)) {
+        try {
+          expand_instance(x, defs, C->el, 0);
+        } catch (...) {
+          delete C;
+          throw;
+        }
+        continue;
+      }
       char f = (char)std::toupper((unsigned char)line[0]);
       delete C;
-      if (f == 'Q' || f == 'M' || f == 'X')
+      if (f == 'Q' || f == 'M')
         throw RefError{REF_INVALID_ELEMENT_FORMAT,
                        "line " + std::to_string(lineno) + ": element outside the R/L/C/V/I/D subset"};
       throw RefError{REF_PARSE, "line " + std::to_string(lineno) + ": cannot parse '" + line + "'"};

@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 NETLISTS = [
     "circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir",
     "voltage_divider/voltage_divider.cir",
+    "voltage_divider/subckt/voltage_divider.cir",
     "low_pass_filter/low_pass_filter.cir",
     "low_pass_filter/transient.cir",
     "low_pass_filter/ac/ac.cir",

@@ -99,3 +99,30 @@ def test_spec_op_and_dc(kg, oracle, tmp_path):
     assert t["step"] == [float(i) for i in range(11)]
     x = np.array([t[k] for k in oc.names]).T
     assert max(nerr(x[i], xr[i]) for i in range(11)) < 1e-12
+
+
+SUBCKT_LADDER = ("V1 in 0 PULSE(0 1 0 1n 1n 10n 1) AC 1\nXa in m 0 sec2\nXb m out 0 sec2\nRL out 0 2k\n"
+                 ".subckt sec2 a b g\nXs1 a mid g sec\nXs2 mid b g sec\n.ends\n"
+                 ".subckt sec p q g\nR1 p q 1k\nC1 q g 1n\nL1 q x 1u\nR2 x g 10k\n.ends\n")
+
+
+def test_subcircuit_netlist_on_gpu(kg, oracle):
+    """A netlist built from nested subcircuit instances (ground passed as a
+    pin): OP, AC and transient on the GPU against the oracle."""
+    say("subcircuit netlist: OP, AC, transient")
+    c = kg.Circuit(SUBCKT_LADDER)
+    oc = oracle.Circuit(SUBCKT_LADDER)
+    assert c.names == oc.names and len(c.element_ids) == 18
+    s = kg.Solver(c)
+    r = s.solve(kg.Op())
+    xr, _ = oc.op()
+    assert nerr([r[k] for k in oc.names], xr) < 1e-12
+    ac = kg.Ac(kg.Linear(3), 10.0, 1e6)
+    f = ac.generate_frequencies()
+    res = s.solve(ac)
+    xa = oc.ac(f)
+    for j in range(3):
+        assert nerr([res[j][k] for k in oc.names], xa[j]) < 1e-12
+    res = s.solve(kg.Transient(1e-9, 100e-9))
+    xt, _ = oc.tran(1e-9, 100e-9)
+    assert max(nerr([res[i][k] for k in oc.names], xt[i]) for i in range(len(xt))) < 1e-9

@@ -237,3 +237,69 @@ def test_ac_writer_on_golden_columns(tmp_path):
     assert t["frequency"] == cols["frequency"]
     for k in t:
         np.testing.assert_allclose(t[k], cols[k], rtol=1e-14, atol=1e-12)
+
+
+# ------------------------------------------------------------- subcircuits
+NESTED = """
+xdiv1 10 7 0 vdivide
+
+.subckt vdivide 1 2 3
+xr1 1 2 myresistor
+xr2 2 3 myresistor
+.ends
+
+
+.subckt myresistor plus minus
+r0 plus minus 1k
+.ends
+"""
+
+
+def test_subcircuit_fixture_of_krets(kg, oracle):
+    """circuits/voltage_divider/subckt/voltage_divider.cir: the instance card
+    precedes the definition; pins 1 2 3 map to 10 7 0."""
+    text = read_circuit("voltage_divider/subckt/voltage_divider.cir")
+    c = same_circuit(kg, oracle, text)
+    assert c.names == ["V(10)", "V(7)", "I(V1)", "V(1)"]
+    assert c.element_ids == ["Rdiv1_1", "Rdiv1_2", "V1"]
+    assert list(c.params[:, 0]) == [10e3, 5e3, 10.0]
+    x, _ = oracle.Circuit(text).op()
+    assert list(x) == [0.0, 0.0, 0.0, 10.0]  # the divider has no source; V1 drives node 1 alone
+
+
+def test_subcircuit_nested_of_krets(kg, oracle):
+    # crates/krets-parser/tests/test_parse_netlist.rs:78-102: two elements
+    c = same_circuit(kg, oracle, NESTED)
+    assert c.element_ids == ["Rdiv1_r1_0", "Rdiv1_r2_0"] and c.names == ["V(10)", "V(7)"]
+
+
+def test_subcircuit_expansion_equals_flat_netlist(kg, oracle):
+    """An RC ladder built from nested instances is the netlist written out by
+    hand with krets' naming: pins map to the instance's nodes, other nodes
+    and element names take "<instance>_" as prefix, in definition order."""
+    sub = ("V1 in 0 DC 1 AC 1\nXa in m sec2\nXb m out sec2\nRL out 0 2k\n"
+           ".subckt sec2 a b\nXs1 a mid sec\nXs2 mid b sec\n.ends\n"
+           ".subckt sec p q\nR1 p q 1k\nC1 q 0 1n\n.ends\n")
+    flat = ("V1 in 0 DC 1 AC 1\n"
+            "Ra_s1_1 in a_mid 1k\nCa_s1_1 a_mid a_s1_0 1n\nRa_s2_1 a_mid m 1k\nCa_s2_1 m a_s2_0 1n\n"
+            "Rb_s1_1 m b_mid 1k\nCb_s1_1 b_mid b_s1_0 1n\nRb_s2_1 b_mid out 1k\nCb_s2_1 out b_s2_0 1n\n"
+            "RL out 0 2k\n")
+    cs, cf = same_circuit(kg, oracle, sub), same_circuit(kg, oracle, flat)
+    assert cs.names == cf.names and cs.element_ids == cf.element_ids
+    assert np.array_equal(cs.elements, cf.elements) and np.array_equal(cs.params, cf.params)
+    # ground inside a definition is a local node unless it is a pin: "a_s1_0"
+    assert "V(a_s1_0)" in cs.names and "V(0)" not in cs.names
+
+
+@pytest.mark.parametrize("text", [
+    "V1 a 0 1\nX1 a b nodef\n",                                    # undefined definition
+    "V1 a 0 1\nX1 a sec\n.subckt sec p q\nR1 p q 1\n.ends\n",        # pin count
+    "V1 a 0 1\nX1 a b loop\n.subckt loop p q\nX2 p q loop\n.ends\n",  # recursive
+    "V1 a 0 1\n.subckt bad p\n.model D1 D\n.ends\n",                 # not an element
+])
+def test_subcircuit_errors(kg, oracle, text):
+    with pytest.raises(kg.KretsGpuError) as e:
+        kg.Circuit(text)
+    assert e.value.code == 7  # ParseError
+    with pytest.raises(oracle.OracleError):
+        oracle.Circuit(text)
