@@ -39,7 +39,13 @@ typedef enum {
   KG_HIP_ERROR = 100
 } kg_status;
 
-typedef enum { KG_R = 0, KG_C = 1, KG_L = 2, KG_V = 3, KG_I = 4, KG_D = 5 } kg_element_type;
+/* KG_E .. KG_H are the linear controlled sources of krets' README to-do list
+ * (README.md:56-62), which krets does not implement yet: E (VCVS), G (VCCS),
+ * F (CCCS), H (CCVS), with SPICE syntax and sign conventions. */
+typedef enum {
+  KG_R = 0, KG_C = 1, KG_L = 2, KG_V = 3, KG_I = 4, KG_D = 5,
+  KG_E = 6, KG_G = 7, KG_F = 8, KG_H = 9
+} kg_element_type;
 typedef enum { KG_WAVE_NONE = 0, KG_WAVE_PULSE = 1, KG_WAVE_SIN = 2 } kg_wave_kind;
 typedef enum { KG_FP64 = 0, KG_MIXED = 1 } kg_precision;
 typedef enum { KG_BE_COMPAT = 0, KG_BE = 1, KG_TRAP = 2 } kg_method;
@@ -48,19 +54,27 @@ typedef enum { KG_MEM_HOST = 0, KG_MEM_DEVICE = 1 } kg_mem;
 typedef enum { KG_MAP_AUTO = 0 } kg_mapping;
 
 /* One element in krets' numbering: plus/minus are the indices of V(plus) and
- * V(minus) (-1 for ground "0"), branch the index of I(<id>) or -1. */
+ * V(minus) (-1 for ground "0"), branch the index of I(<id>) or -1. E and H
+ * own a branch current, like V. ctrl_plus / ctrl_minus are the controlling
+ * node indices of E and G (-1 for ground); ctrl_branch is the index of the
+ * controlling element's branch current for F and H. The ctrl fields are -1
+ * for every other type. */
 typedef struct {
   int32_t type;
   int32_t plus;
   int32_t minus;
   int32_t branch;
   int32_t g2; /* Group-2 flag of R/C (capacitor.rs:172) */
-  int32_t reserved;
+  int32_t ctrl_plus;
+  int32_t ctrl_minus;
+  int32_t ctrl_branch;
 } kg_element;
 
 /* Per (instance, element) parameter record of KG_NPARAM doubles:
  *   R: [0] ohms          C: [0] farads        L: [0] henries
  *   I: [0] amps          D: [0] Is, [1] N
+ *   E: [0] voltage gain  G: [0] transconductance (S)
+ *   F: [0] current gain  H: [0] transresistance (ohms)
  *   V: [0] dc value, [1] ac magnitude, [2] waveform kind (kg_wave_kind),
  *      [3..9] PULSE(v1 v2 td tr tf pw per) or SIN(vo va f td theta phase). */
 #define KG_NPARAM 10
@@ -105,7 +119,10 @@ typedef struct {
    * the whole step loop in LDS: compact 16-bit plan, factors, solutions and
    * companion states). */
   int32_t threads_per_system;
-  int32_t refine_iters;       /* KG_MIXED: refinement steps (0 = default 2) */
+  /* KG_MIXED: refinement steps. 0 = default: 2, except that the lane AC
+   * kernels (2002 / 2012) stop after the first pass where every lane of a
+   * wave predicts an error below 1e-11. */
+  int32_t refine_iters;
 } kg_solver_config;
 
 typedef struct kg_ctx kg_ctx;

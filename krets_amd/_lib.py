@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libkrets_gpu.so")
 
 OK, MATRIX_BUILD, DECOMPOSITION, MAX_ITERATIONS, ELEMENT_NOT_FOUND = 0, 1, 2, 3, 4
 INVALID_ELEMENT_FORMAT, UNEXPECTED, PARSE, EMPTY_NETLIST, INVALID_ARGUMENT, HIP_ERROR = 5, 6, 7, 8, 9, 100
-R, C, L, V, I, D = 0, 1, 2, 3, 4, 5
+R, C, L, V, I, D, E, G, F, H = range(10)  # kg_element_type
 WAVE_NONE, WAVE_PULSE, WAVE_SIN = 0, 1, 2
 FP64, MIXED = 0, 1
 BE_COMPAT, BE, TRAP = 0, 1, 2
@@ -30,11 +30,13 @@ STATUS_NAMES = {0: "Ok", 1: "MatrixBuild", 2: "DecompositionFailed", 3: "Maximum
 
 class KgElement(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("plus", ctypes.c_int32), ("minus", ctypes.c_int32),
-                ("branch", ctypes.c_int32), ("g2", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("branch", ctypes.c_int32), ("g2", ctypes.c_int32), ("ctrl_plus", ctypes.c_int32),
+                ("ctrl_minus", ctypes.c_int32), ("ctrl_branch", ctypes.c_int32)]
 
 
 ELEMENT_DTYPE = np.dtype([("type", np.int32), ("plus", np.int32), ("minus", np.int32), ("branch", np.int32),
-                          ("g2", np.int32), ("reserved", np.int32)])
+                          ("g2", np.int32), ("ctrl_plus", np.int32), ("ctrl_minus", np.int32),
+                          ("ctrl_branch", np.int32)])
 
 
 class KgBatch(ctypes.Structure):

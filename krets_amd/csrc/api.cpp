@@ -969,7 +969,7 @@ int32_t kg_load_batch(kg_ctx* c, const kg_batch_desc* b) {
       std::vector<int32_t> s{c->nunk[i]};
       for (int64_t r = c->eoff[i]; r < c->eoff[i + 1]; r++) {
         const kg_element& e = c->el[r];
-        s.insert(s.end(), {e.type, e.plus, e.minus, e.branch, e.g2});
+        s.insert(s.end(), {e.type, e.plus, e.minus, e.branch, e.g2, e.ctrl_plus, e.ctrl_minus, e.ctrl_branch});
       }
       auto it = sig.find(s);
       if (it == sig.end()) {
@@ -986,10 +986,14 @@ int32_t kg_load_batch(kg_ctx* c, const kg_batch_desc* b) {
     for (int64_t r = c->eoff[i]; r < c->eoff[i + 1]; r++) {
       const kg_element& e = c->el[r];
       int n = c->nunk[i];
-      if (e.type < KG_R || e.type > KG_D || e.plus >= n || e.minus >= n || e.branch >= n)
+      if (e.type < KG_R || e.type > KG_H || e.plus >= n || e.minus >= n || e.branch >= n)
         return set_err(c, KG_INVALID_ELEMENT_FORMAT, "element table out of range");
-      if ((e.type == KG_V || e.type == KG_L || e.type == KG_I) && e.branch < 0)
-        return set_err(c, KG_INVALID_ELEMENT_FORMAT, "V, L and I elements need a branch unknown");
+      if ((e.type == KG_V || e.type == KG_L || e.type == KG_I || e.type == KG_E || e.type == KG_H) && e.branch < 0)
+        return set_err(c, KG_INVALID_ELEMENT_FORMAT, "V, L, I, E and H elements need a branch unknown");
+      if ((e.type == KG_E || e.type == KG_G) && (e.ctrl_plus >= n || e.ctrl_minus >= n))
+        return set_err(c, KG_INVALID_ELEMENT_FORMAT, "controlling node out of range");
+      if ((e.type == KG_F || e.type == KG_H) && (e.ctrl_branch < 0 || e.ctrl_branch >= n))
+        return set_err(c, KG_INVALID_ELEMENT_FORMAT, "F and H elements need a controlling branch unknown");
     }
   HIPCHK(c, hipMalloc(&c->d_params, sizeof(double) * std::max<size_t>(c->params.size(), 1)));
   HIPCHK(c, hipMalloc(&c->d_ebase, sizeof(double) * std::max<size_t>(c->ebase.size(), 1)));

@@ -150,7 +150,8 @@ struct Gen {
   }
   // FP64 value of LU slot s as (re, im) expressions; both empty when the slot
   // receives no contribution. Follows the stamp kinds of the plan's gather
-  // lists (AC stamps, stampable.rs): +-y (1/R, j w C), +-1, branch diagonal.
+  // lists (AC stamps, stampable.rs): +-y (1/R, j w C), +-1, branch diagonal,
+  // +-k (controlled-source gains).
   bool slot(int s, std::string& re, std::string& im) {
     std::vector<std::string> R, I;
     for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) {
@@ -171,6 +172,8 @@ struct Gen {
           else if (ty == KG_R) R.push_back("-" + g(e));
           else R.push_back("1.0");
           break;
+        case CK_KP: R.push_back(g(e)); break;  // controlled-source gain
+        case CK_KN: R.push_back("-" + g(e)); break;
         default: why = "unknown stamp kind"; return false;
       }
     }

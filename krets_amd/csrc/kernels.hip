@@ -231,6 +231,8 @@ __device__ __forceinline__ A mat_contrib(int32_t code, const PatDev& P, const do
     case CK_YN: return -elem_y<A>(e, P, prm, eb, sa, ed);
     case CK_ONEP: return mk<A>(1.0, 0.0);
     case CK_ONEN: return mk<A>(-1.0, 0.0);
+    case CK_KP: return mk<A>(eb[e], 0.0);
+    case CK_KN: return mk<A>(-eb[e], 0.0);
     default: return elem_z<A>(e, P, prm, sa);
   }
 }

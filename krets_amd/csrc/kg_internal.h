@@ -23,7 +23,9 @@ enum : int32_t {
   CK_YN = 1,    // -y(e)
   CK_ONEP = 2,  // +1 (incidence, current-source diagonal, G2 capacitor AC diagonal)
   CK_ONEN = 3,  // -1
-  CK_BR = 4     // branch diagonal: -j*w*L, -alpha*L/h, -R (G2 resistor)
+  CK_BR = 4,    // branch diagonal: -j*w*L, -alpha*L/h, -R (G2 resistor)
+  CK_KP = 5,    // +k(e): controlled-source gain (E, G, F, H)
+  CK_KN = 6     // -k(e)
 };
 
 // Right-hand-side contribution kinds gathered per permuted row.

@@ -11,6 +11,15 @@
 // the instance name and "_" as prefix, nested instances recursively).
 // Elements are matched in krets' `alt` order (elements.rs:44-65) and, like
 // krets, a line is not required to be consumed entirely.
+//
+// Beyond krets' grammar: the linear controlled sources of its README to-do
+// list (README.md:56-62), in SPICE syntax. "E<name> n+ n- nc+ nc- gain"
+// (VCVS) and "H<name> n+ n- <source> r" (CCVS) own a branch current I(<id>)
+// like a voltage source; "G<name> n+ n- nc+ nc- gm" (VCCS) and
+// "F<name> n+ n- <source> gain" (CCCS) do not. Values take SI suffixes. In
+// the index map the controlling nodes of E and G follow n+ and n-. <source>
+// names an element with a branch current (V, L, I, E, H or a G2 resistor);
+// inside a subcircuit it names that instance's own element.
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +38,7 @@ namespace {
 struct Rec {
   char letter;
   std::string name, plus, minus, model;
+  std::string cplus, cminus, ctrl;  // E/G controlling nodes, F/H controlling element
   double p[KG_NPARAM] = {0};
   bool g2 = false;
 };
@@ -186,6 +196,18 @@ bool parse_two_terminal(const std::string& line, char letter, Rec& r) {
     if (sp1(d) && word(d, "G2", true)) r.g2 = true;
   }
   return true;
+}
+
+// E/G: "n+ n- nc+ nc- value"; F/H: "n+ n- <source> value"
+bool parse_controlled(const std::string& line, char letter, Rec& r) {
+  Scan c{&line};
+  if (!head(c, letter, r) || !sp1(c)) return false;
+  if (letter == 'E' || letter == 'G') {
+    if (!ident(c, r.cplus) || !sp1(c) || !ident(c, r.cminus) || !sp1(c)) return false;
+  } else if (!ident(c, r.ctrl) || !sp1(c)) {
+    return false;
+  }
+  return si_value(c, r.p[0]);
 }
 
 bool parse_diode(const std::string& line, Rec& r) {
@@ -378,6 +400,9 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
     bool ok = parse_two_terminal(line, 'R', r) || parse_two_terminal(line, 'C', r) ||
               parse_two_terminal(line, 'L', r) || (vs = parse_vsrc(line, r)) == 1;
     if (!ok && vs == 0) ok = parse_two_terminal(line, 'I', r) || parse_diode(line, r);
+    if (!ok && vs == 0)
+      for (char k : {'E', 'G', 'F', 'H'})
+        if (!ok) ok = parse_controlled(line, k, r);
     return ok ? 1 : 0;
   };
   // first pass (subcircuit.rs:159-200): definitions anywhere in the text
@@ -448,6 +473,11 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
         Rec r = it.r;
         r.plus = map(r.plus);
         r.minus = map(r.minus);
+        if (r.letter == 'E' || r.letter == 'G') {
+          r.cplus = map(r.cplus);
+          r.cminus = map(r.cminus);
+        }
+        if (!r.ctrl.empty()) r.ctrl = r.ctrl.substr(0, 1) + x.iname + "_" + r.ctrl.substr(1);
         r.name = x.iname + "_" + r.name;
         out.push_back(r);
       } else {
@@ -521,7 +551,8 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
   std::set<std::string> seen;
   int counter = 0;
   auto g2 = [](const Rec& r) {
-    return r.letter == 'V' || r.letter == 'L' || r.letter == 'I' || ((r.letter == 'C' || r.letter == 'R') && r.g2);
+    return r.letter == 'V' || r.letter == 'L' || r.letter == 'I' || r.letter == 'E' || r.letter == 'H' ||
+           ((r.letter == 'C' || r.letter == 'R') && r.g2);
   };
   for (auto& r : C->recs) {
     std::string id = std::string(1, r.letter) + r.name;
@@ -534,8 +565,11 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
       }
       idx[k] = counter++;
     }
-    for (const std::string* nd : {&r.plus, &r.minus})
+    const bool vc = r.letter == 'E' || r.letter == 'G';
+    for (const std::string* nd : {&r.plus, &r.minus, &r.cplus, &r.cminus}) {
+      if (!vc && (nd == &r.cplus || nd == &r.cminus)) continue;
       if (seen.insert(*nd).second && *nd != "0") idx["V(" + *nd + ")"] = counter++;
+    }
   }
   C->names.assign(counter, "");
   for (auto& kv : idx) C->names[kv.second] = kv.first;
@@ -544,15 +578,17 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
     if (r.letter == 'D') {  // parser.rs:131-140: a diode needs its .model
       auto it = models.find(r.model);
       if (it == models.end() || !it->second.diode) {
+        std::string msg = "diode " + C->ids[k] + " refers to missing .model '" + r.model + "'";
         delete C;
-        return fail(KG_UNEXPECTED, "diode " + C->ids[k] + " refers to missing .model '" + r.model + "'");
+        return fail(KG_UNEXPECTED, msg);
       }
       r.p[0] = it->second.is;
       r.p[1] = it->second.n;
     }
     kg_element e{};
-    static const char* L = "RCLVID";
+    static const char* L = "RCLVIDEGFH";
     e.type = (int32_t)(std::strchr(L, r.letter) - L);
+    e.ctrl_plus = e.ctrl_minus = e.ctrl_branch = -1;
     auto node = [&](const std::string& nd) {
       auto it = idx.find("V(" + nd + ")");
       return it == idx.end() ? -1 : it->second;
@@ -562,6 +598,21 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
     auto it = idx.find("I(" + C->ids[k] + ")");
     e.branch = it == idx.end() ? -1 : it->second;
     e.g2 = r.g2 ? 1 : 0;
+    if (r.letter == 'E' || r.letter == 'G') {
+      e.ctrl_plus = node(r.cplus);
+      e.ctrl_minus = node(r.cminus);
+    } else if (r.letter == 'F' || r.letter == 'H') {
+      std::string id = r.ctrl;
+      id[0] = (char)std::toupper((unsigned char)id[0]);
+      auto b = idx.find("I(" + id + ")");
+      if (b == idx.end()) {
+        std::string msg = "controlled source " + C->ids[k] + ": controlling element '" + id +
+                          "' not found or has no branch current";
+        delete C;
+        return fail(KG_ELEMENT_NOT_FOUND, msg);
+      }
+      e.ctrl_branch = b->second;
+    }
     C->el.push_back(e);
   }
   *out = reinterpret_cast<kg_circuit*>(C);

@@ -226,7 +226,9 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   for (int e = 0; e < ne; e++) {
     const kg_element& x = el[e];
     int p = x.plus, m = x.minus, c = x.branch;
-    if (p >= n || m >= n || c >= n) {
+    const bool ctl = x.type >= KG_E && x.type <= KG_H;
+    const int cp = ctl ? x.ctrl_plus : -1, cm = ctl ? x.ctrl_minus : -1, cb = ctl ? x.ctrl_branch : -1;
+    if (p >= n || m >= n || c >= n || cp >= n || cm >= n || cb >= n) {
       err = "element index out of range";
       L.status = KG_INVALID_ARGUMENT;
       return {};
@@ -280,6 +282,27 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
           rhs(m, e, RK_DM);
         }
         has_diode = true;
+        break;
+      // Controlled sources (SPICE conventions; no krets counterpart, README.md:56-62).
+      // The same stamps in every analysis: they are linear and memoryless.
+      case KG_E:  // V(p) - V(m) - k (V(cp) - V(cm)) = 0 on the branch row
+        incidence(p, m, c, e);
+        add(c, cp, e, CK_KN);
+        add(c, cm, e, CK_KP);
+        break;
+      case KG_G:  // k (V(cp) - V(cm)) flows from p through the source to m
+        add(p, cp, e, CK_KP);
+        add(p, cm, e, CK_KN);
+        add(m, cp, e, CK_KN);
+        add(m, cm, e, CK_KP);
+        break;
+      case KG_F:  // k I(cb) flows from p through the source to m
+        add(p, cb, e, CK_KP);
+        add(m, cb, e, CK_KN);
+        break;
+      case KG_H:  // V(p) - V(m) - k I(cb) = 0 on the branch row
+        incidence(p, m, c, e);
+        add(c, cb, e, CK_KN);
         break;
       default:
         err = "unknown element type";
@@ -385,6 +408,8 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
         case CK_YN: return -y();
         case CK_ONEP: return cd(1.0, 0.0);
         case CK_ONEN: return cd(-1.0, 0.0);
+        case CK_KP: return cd(v, 0.0);
+        case CK_KN: return cd(-v, 0.0);
         default:
           if (x.type == KG_L) return ak == AK_AC ? cd(0.0, -s_rep * v) : cd(-s_rep * v, 0.0);
           if (x.type == KG_R) return cd(-v, 0.0);
@@ -828,7 +853,7 @@ bool validate_plan(const PatLayout& L, const std::vector<int32_t>& b, std::strin
   for (int s = 0; s <= nnz; s++)
     if (s > 0 && scp[s] < scp[s - 1]) return fail("scon_ptr");
   for (int q = 0; q < scp[nnz]; q++)
-    if (!in(sc[q] >> 3, 0, ne) || (sc[q] & 7) > CK_BR) return fail("scon code");
+    if (!in(sc[q] >> 3, 0, ne) || (sc[q] & 7) > CK_KN) return fail("scon code");
   const int32_t* rcp = b.data() + L.o_rcon_ptr;
   const int32_t* rc = b.data() + L.o_rcon;
   for (int q = 0; q < rcp[n]; q++)
@@ -838,11 +863,12 @@ bool validate_plan(const PatLayout& L, const std::vector<int32_t>& b, std::strin
         !in(b[L.o_rpos + j], 0, n))
       return fail("permutation");
   for (int e = 0; e < ne; e++) {
-    if (!in(b[L.o_etype + e], KG_R, KG_D + 1)) return fail("etype");
+    if (!in(b[L.o_etype + e], KG_R, KG_H + 1)) return fail("etype");
     for (int o : {L.o_ep, L.o_em, L.o_ec, L.o_erp, L.o_erm, L.o_erc})
       if (!in(b[o + e], -1, n)) return fail("element index");
     int ty = b[L.o_etype + e];
-    if ((ty == KG_V || ty == KG_L || ty == KG_I) && b[L.o_ec + e] < 0) return fail("branch index");
+    if ((ty == KG_V || ty == KG_L || ty == KG_I || ty == KG_E || ty == KG_H) && b[L.o_ec + e] < 0)
+      return fail("branch index");
   }
   for (int o_ptr : {L.o_ut_ptr, L.o_lt_ptr}) {
     const int32_t* pp = b.data() + o_ptr;

@@ -22,6 +22,11 @@
 // trapezoidal companion models, SPICE3 pnjlim junction limiting (Nagel,
 // SPICE2 thesis 1975), divided-difference LTE step control for backward Euler,
 // and the discrete adjoint of the fixed-step recurrence.
+//
+// Beyond krets' element set: the linear controlled sources of krets' README
+// to-do list (README.md:56-62), restated from the SPICE element definitions
+// (SPICE3 user's guide, section on linear dependent sources): E (VCVS),
+// G (VCCS), F (CCCS), H (CCVS). Pinned by closed-form circuits in the tests.
 
 #include "refcpu.h"
 
@@ -57,7 +62,8 @@ struct RefError {
 
 thread_local std::string g_last_error;
 
-enum ET { E_R, E_C, E_L, E_V, E_I, E_D };
+enum ET { E_R, E_C, E_L, E_V, E_I, E_D, E_E, E_G, E_F, E_H };
+const char ET_LETTER[] = {'R', 'C', 'L', 'V', 'I', 'D', 'E', 'G', 'F', 'H'};
 
 struct Wave {
   int kind = 0;  // 0 none, 1 pulse, 2 sin
@@ -72,14 +78,13 @@ struct Elem {
   double ac = 0.0;
   Wave w;
   double n_emis = 1.0;
-  int ip = -1, im = -1, ic = -1;  // lean indices
-  std::string ident() const {
-    static const char L[] = {'R', 'C', 'L', 'V', 'I', 'D'};
-    return std::string(1, L[t]) + name;
-  }
-  bool is_g2() const {  // elements.rs:124-140
+  std::string cplus, cminus, ctrl;  // E/G controlling nodes, F/H controlling element id
+  int ip = -1, im = -1, ic = -1;    // lean indices
+  int icp = -1, icm = -1, icb = -1;  // controlling node / branch indices
+  std::string ident() const { return std::string(1, ET_LETTER[t]) + name; }
+  bool is_g2() const {  // elements.rs:124-140 (E and H own a branch current, like V)
     switch (t) {
-      case E_V: case E_L: case E_I: return true;
+      case E_V: case E_L: case E_I: case E_E: case E_H: return true;
       case E_R: case E_C: return g2;
       default: return false;
     }
@@ -273,6 +278,22 @@ bool parse_rlci(const std::string& line, char letter, ET t, Elem& e) {
     if (space1(d) && tag_nc(d, "G2")) e.g2 = true;
   }
   return true;
+}
+
+// "E<name> n+ n- nc+ nc- gain", "G<name> n+ n- nc+ nc- gm",
+// "F<name> n+ n- <source> gain", "H<name> n+ n- <source> r" (SPICE syntax)
+bool parse_controlled_line(const std::string& line, char letter, ET t, Elem& e) {
+  Cur c{line, 0};
+  char tg[2] = {letter, 0};
+  e = Elem();
+  e.t = t;
+  if (!tag_nc(c, tg) || !two_nodes(c, e) || !space1(c)) return false;
+  if (t == E_E || t == E_G) {
+    if (!ident1(c, e.cplus) || !space1(c) || !ident1(c, e.cminus) || !space1(c)) return false;
+  } else if (!ident1(c, e.ctrl) || !space1(c)) {
+    return false;
+  }
+  return value_tok(c, e.value);
 }
 
 bool parse_diode_line(const std::string& line, Elem& e) {  // diode.rs:75-93
@@ -497,7 +518,10 @@ bool parse_primitive(const std::string& line, Elem& el) {
   if (parse_rlci(line, 'L', E_L, el)) return true;
   if (parse_vsource(line, el, hard)) return true;
   if (!hard && parse_rlci(line, 'I', E_I, el)) return true;
-  return !hard && parse_diode_line(line, el);
+  if (hard) return false;
+  if (parse_diode_line(line, el)) return true;
+  return parse_controlled_line(line, 'E', E_E, el) || parse_controlled_line(line, 'G', E_G, el) ||
+         parse_controlled_line(line, 'F', E_F, el) || parse_controlled_line(line, 'H', E_H, el);
 }
 
 void expand_instance(const XInst& x, const std::map<std::string, SubDef>& defs, std::vector<Elem>& out, int depth) {
@@ -517,6 +541,12 @@ void expand_instance(const XInst& x, const std::map<std::string, SubDef>& defs, 
       Elem e = it.el;
       e.plus = map(e.plus);
       e.minus = map(e.minus);
+      if (e.t == E_E || e.t == E_G) {
+        e.cplus = map(e.cplus);
+        e.cminus = map(e.cminus);
+      }
+      // a controlling element named inside a definition is the instance's own
+      if (!e.ctrl.empty()) e.ctrl = e.ctrl.substr(0, 1) + x.iname + "_" + e.ctrl.substr(1);
       e.name = x.iname + "_" + e.name;
       out.push_back(e);
     } else {
@@ -636,7 +666,9 @@ Circuit* parse_netlist(const std::string& text) {
       std::string k = "I(" + el.ident() + ")";
       C->index[k] = counter++;
     }
-    for (const std::string* nd : {&el.plus, &el.minus}) {
+    std::vector<const std::string*> nodes{&el.plus, &el.minus};
+    if (el.t == E_E || el.t == E_G) nodes.insert(nodes.end(), {&el.cplus, &el.cminus});
+    for (const std::string* nd : nodes) {
       if (seen.insert(*nd).second) {
         if (*nd == "0") continue;
         C->index["V(" + *nd + ")"] = counter++;
@@ -658,8 +690,9 @@ Circuit* parse_netlist(const std::string& text) {
     if (el.t == E_D) {
       auto it = models.find(el.model);
       if (it == models.end() || !it->second.diode) {
+        std::string msg = "diode " + el.ident() + " has no .model '" + el.model + "'";
         delete C;
-        throw RefError{REF_UNEXPECTED, "diode " + el.ident() + " has no .model '" + el.model + "'"};
+        throw RefError{REF_UNEXPECTED, msg};
       }
       el.value = it->second.is;
       el.n_emis = it->second.n;
@@ -673,6 +706,22 @@ Circuit* parse_netlist(const std::string& text) {
     el.im = f(el.minus);
     auto it = C->index.find("I(" + el.ident() + ")");
     el.ic = it == C->index.end() ? -1 : it->second;
+    if (el.t == E_E || el.t == E_G) {
+      el.icp = f(el.cplus);
+      el.icm = f(el.cminus);
+    }
+    if (el.t == E_F || el.t == E_H) {
+      std::string id = el.ctrl;
+      id[0] = (char)std::toupper((unsigned char)id[0]);
+      auto b = C->index.find("I(" + id + ")");
+      if (b == C->index.end()) {
+        std::string msg = "controlling element " + id + " of " + el.ident() + " not found";
+        delete C;
+        throw RefError{REF_ELEMENT_NOT_FOUND, msg};
+      }
+      el.ctrl = id;
+      el.icb = b->second;
+    }
   }
   return C;
 }
@@ -1005,13 +1054,55 @@ bool converged(const SolMap& prev, const SolMap& cur, const ref_config& cfg) {
 struct Idx {
   const Circuit* C;
   bool compat;
-  int get(const Elem& e, int which) const {  // 0 plus, 1 minus, 2 current
-    if (!compat) return which == 0 ? e.ip : which == 1 ? e.im : e.ic;
-    std::string k = which == 2 ? "I(" + e.ident() + ")" : "V(" + (which == 0 ? e.plus : e.minus) + ")";
+  // 0 plus, 1 minus, 2 current, 3 / 4 controlling plus / minus, 5 controlling current
+  int get(const Elem& e, int which) const {
+    if (!compat) {
+      const int v[] = {e.ip, e.im, e.ic, e.icp, e.icm, e.icb};
+      return v[which];
+    }
+    if ((which == 3 || which == 4) && e.t != E_E && e.t != E_G) return -1;
+    if (which == 5 && e.t != E_F && e.t != E_H) return -1;
+    std::string k;
+    switch (which) {
+      case 0: k = "V(" + e.plus + ")"; break;
+      case 1: k = "V(" + e.minus + ")"; break;
+      case 2: k = "I(" + e.ident() + ")"; break;
+      case 3: k = "V(" + e.cplus + ")"; break;
+      case 4: k = "V(" + e.cminus + ")"; break;
+      default: k = "I(" + e.ctrl + ")"; break;
+    }
     auto it = C->index.find(k);
     return it == C->index.end() ? -1 : it->second;
   }
 };
+
+// Linear controlled sources, the same in every analysis (SPICE conventions):
+//   E: branch row V(p) - V(m) - k (V(cp) - V(cm)) = 0, branch current at p, m
+//   G: k (V(cp) - V(cm)) flows from p through the source to m
+//   F: k I(cb) flows from p through the source to m
+//   H: branch row V(p) - V(m) - k I(cb) = 0, branch current at p, m
+template <class T>
+void stamp_controlled(const Elem& e, const Idx& ix, std::vector<Trip<T>>& A) {
+  int p = ix.get(e, 0), m = ix.get(e, 1), c = ix.get(e, 2);
+  int cp = ix.get(e, 3), cm = ix.get(e, 4), cb = ix.get(e, 5);
+  const T k(e.value), one(1.0);
+  auto put = [&](int r, int col, T v) {
+    if (r >= 0 && col >= 0) A.push_back({r, col, v});
+  };
+  if (e.t == E_E || e.t == E_H) {
+    put(p, c, one);
+    put(c, p, one);
+    put(m, c, -one);
+    put(c, m, -one);
+  }
+  switch (e.t) {
+    case E_E: put(c, cp, -k); put(c, cm, k); break;
+    case E_G: put(p, cp, k); put(p, cm, -k); put(m, cp, -k); put(m, cm, k); break;
+    case E_F: put(p, cb, k); put(m, cb, -k); break;
+    case E_H: put(c, cb, -k); break;
+    default: break;
+  }
+}
 
 template <class T>
 void four(std::vector<Trip<T>>& A, int p, int m, T y) {
@@ -1104,6 +1195,7 @@ void stamp_dc(const Circuit& C, const Idx& ix, const SolMap& sol, std::vector<Tr
         if (m >= 0) b.push_back({m, 0, d.ieq});
         break;
       }
+      default: stamp_controlled(e, ix, A); break;
     }
   }
 }
@@ -1167,6 +1259,7 @@ void stamp_ac(const Circuit& C, const Idx& ix, const SolMap& op, double f, std::
         four(A, p, m, cd(d.g, 0.0));
         break;
       }
+      default: stamp_controlled(e, ix, A); break;
     }
   }
 }
@@ -1234,6 +1327,7 @@ int clean_newton(const Circuit& C, const ref_config& cfg, int kind, double t, do
   for (size_t k = 0; k < C.el.size(); k++)
     if (C.el[k].t == E_D) vdold[k] = vdiff(C.el[k], guess);
   double alpha = kind == 2 ? 2.0 : 1.0;
+  const Idx lean{&C, false};
   for (int it = 0; it < cfg.maxiter; it++) {
     std::vector<Trip<double>> A, b;
     bool any_lim = false;
@@ -1296,6 +1390,7 @@ int clean_newton(const Circuit& C, const ref_config& cfg, int kind, double t, do
           if (m >= 0) b.push_back({m, 0, ieq});
           break;
         }
+        default: stamp_controlled(e, lean, A); break;
       }
     }
     if (!ls.solve(n, A, b, x)) throw RefError{REF_DECOMPOSITION, "clean Newton: LU failed"};
@@ -1398,12 +1493,21 @@ int32_t ref_element_info(void* c, int32_t e, char* type, int32_t* plus, int32_t*
   const Circuit& C = *as_c(c);
   if (e < 0 || e >= (int)C.el.size()) return REF_ELEMENT_NOT_FOUND;
   const Elem& el = C.el[e];
-  static const char L[] = {'R', 'C', 'L', 'V', 'I', 'D'};
-  *type = L[el.t];
+  *type = ET_LETTER[el.t];
   *plus = el.ip;
   *minus = el.im;
   *branch = el.ic;
   *value = el.value;
+  return REF_OK;
+}
+
+int32_t ref_element_ctrl(void* c, int32_t e, int32_t* ctrl_plus, int32_t* ctrl_minus, int32_t* ctrl_branch) {
+  const Circuit& C = *as_c(c);
+  if (e < 0 || e >= (int)C.el.size()) return REF_ELEMENT_NOT_FOUND;
+  const Elem& el = C.el[e];
+  *ctrl_plus = el.icp;
+  *ctrl_minus = el.icm;
+  *ctrl_branch = el.icb;
   return REF_OK;
 }
 
@@ -1718,6 +1822,10 @@ int32_t ref_tran_adjoint(void* c, const ref_config* cfg, double h, int32_t N, in
           if (p >= 0) T->push_back({p, cc, 1.0});
           if (m >= 0) T->push_back({m, cc, -1.0});
         }
+      }
+      if (e.t >= E_E) {  // controlled sources: constant; their gains are not differentiated
+        stamp_controlled(e, Idx{&C, false}, A);
+        stamp_controlled(e, Idx{&C, false}, Adc);
       }
     }
     auto factor = [&](std::vector<Trip<double>>& T, SpLU<double>& lu) {

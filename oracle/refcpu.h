@@ -62,7 +62,11 @@ const char* ref_unknown_name(void* circuit, int32_t idx);
  * primary value. Returns 0 on success. */
 int32_t ref_element_info(void* circuit, int32_t e, char* type, int32_t* plus, int32_t* minus,
                          int32_t* branch, double* value);
-/* Overwrite an element's primary value (R, C, L, V dc, I, D Is). */
+/* Controlling node indices of E / G and controlling branch index of F / H
+ * (-1 where not applicable). Returns 0 on success. */
+int32_t ref_element_ctrl(void* circuit, int32_t e, int32_t* ctrl_plus, int32_t* ctrl_minus,
+                         int32_t* ctrl_branch);
+/* Overwrite an element's primary value (R, C, L, V dc, I, D Is, E/G/F/H gain). */
 int32_t ref_set_value(void* circuit, int32_t e, double value);
 const char* ref_last_error(void);
 

@@ -48,6 +48,7 @@ def lib():
         L.ref_unknown_name.restype = C
         L.ref_unknown_name.argtypes = [P, I]
         L.ref_element_info.argtypes = [P, I, ctypes.c_char_p, IP, IP, IP, D]
+        L.ref_element_ctrl.argtypes = [P, I, IP, IP, IP]
         L.ref_set_value.argtypes = [P, I, ctypes.c_double]
         L.ref_last_error.restype = C
         L.ref_op.argtypes = [P, CFG, D, IP]
@@ -133,6 +134,13 @@ class Circuit:
         if r:
             raise OracleError(r, "no element")
         return t.value.decode(), p.value, m.value, b.value, v.value
+
+    def element_ctrl(self, e):
+        """(ctrl_plus, ctrl_minus, ctrl_branch) of element e; -1 where not applicable."""
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        if lib().ref_element_ctrl(self._h, e, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)):
+            raise OracleError(4, "no element")
+        return a.value, b.value, c.value
 
     def set_value(self, e, v):
         lib().ref_set_value(self._h, e, float(v))

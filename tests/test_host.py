@@ -55,7 +55,7 @@ def test_library_exports_every_header_function(kg):
 
 
 # ------------------------------------------------------------- netlists
-TYPES = {"R": 0, "C": 1, "L": 2, "V": 3, "I": 4, "D": 5}
+TYPES = {"R": 0, "C": 1, "L": 2, "V": 3, "I": 4, "D": 5, "E": 6, "G": 7, "F": 8, "H": 9}
 
 
 def same_circuit(kg, oracle, text):
@@ -66,6 +66,7 @@ def same_circuit(kg, oracle, text):
         t, p, m, b, v = oc.element(e)
         row = c.elements[e]
         assert (int(row["type"]), int(row["plus"]), int(row["minus"]), int(row["branch"])) == (TYPES[t], p, m, b)
+        assert (int(row["ctrl_plus"]), int(row["ctrl_minus"]), int(row["ctrl_branch"])) == oc.element_ctrl(e)
         assert c.params[e, 0] == v
     return c
 
@@ -87,6 +88,37 @@ def test_netlist_reader_quirks(kg, oracle):
     assert e.value.code == 8  # EmptyNetlist
     with pytest.raises(kg.KretsGpuError):
         kg.Circuit("Q1 1 2 3 mod\n")
+
+
+def test_netlist_reader_controlled_sources(kg, oracle):
+    """E, G, F, H lines (SPICE syntax): types, branch currents of E and H,
+    controlling nodes and controlling branches, SI-suffixed gains, and the
+    index order (branch, n+, n-, nc+, nc-), the same as the oracle's."""
+    from conftest import CS_NETS, active_netlist
+    for text in list(CS_NETS.values()) + [active_netlist()]:
+        same_circuit(kg, oracle, text)
+    c = same_circuit(kg, oracle, "E1 o 0 a b 2k\nR1 a 0 1\nR2 b 0 1\nR3 o 0 1\nV1 a b 1\n")
+    assert c.names == ["I(E1)", "V(o)", "V(a)", "V(b)", "I(V1)"] and c.params[0, 0] == 2000.0
+    el = c.elements[0]
+    assert (el["type"], el["branch"], el["ctrl_plus"], el["ctrl_minus"], el["ctrl_branch"]) == (6, 0, 2, 3, -1)
+    # controlling element declared after its user; lower-case reference
+    c = same_circuit(kg, oracle, "h1 o 0 vsense 10\nR1 o 0 1k\nI1 0 n 1m\nvsense n 0 0\n")
+    assert c.element_ids == ["H1", "R1", "I1", "Vsense"]
+    assert c.elements[0]["ctrl_branch"] == c.names.index("I(Vsense)")
+    # inside subcircuits: the instance's own controlling element
+    net = (".subckt mirror a b gnd\nVs a gnd 0\nF1 gnd b Vs 2\nG1 gnd b a gnd 1m\n.ends\n"
+           "V2 n1 0 1\nR3 n1 ma 1k\nX2 ma mb 0 mirror\nX3 ma mc 0 mirror\nR4 mb 0 1k\nR5 mc 0 1k\n")
+    c = same_circuit(kg, oracle, net)
+    f = c.element_ids.index("F2_1")
+    assert c.elements[f]["ctrl_branch"] == c.names.index("I(V2_s)")
+    assert c.elements[c.element_ids.index("F3_1")]["ctrl_branch"] == c.names.index("I(V3_s)")
+    for bad in ("V1 1 0 1\nR1 1 0 1k\nF1 0 2 Vx 2\nR2 2 0 1k\n", "V1 1 0 1\nR1 1 0 1k\nH1 2 0 R1 2\n"):
+        with pytest.raises(kg.KretsGpuError) as e:
+            kg.Circuit(bad)
+        assert e.value.code == 4  # ElementNotFound
+    with pytest.raises(kg.KretsGpuError) as e:
+        kg.Circuit("E1 o 0 a 2\nR1 o 0 1\n")  # a VCVS needs two controlling nodes
+    assert e.value.code == 7
 
 
 # ------------------------------------------------------------- krets.toml

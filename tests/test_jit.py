@@ -222,3 +222,26 @@ def test_lane_source_adaptive_stop_on_host(oracle):
     assert 0.2 < stopped.mean() < 0.9, stopped.mean()  # some, not all, leave early
     assert np.array_equal(xa[~stopped], x2[~stopped])
     assert (err >= e.max(axis=1)).all() and (err <= 1e-9).all()
+
+
+@pytest.mark.parametrize("precision,refine", [(0, 2), (1, 2), (1, -1)])
+def test_lane_source_controlled_sources_on_host(oracle, precision, refine):
+    """The generated kernel stamps E, G, F and H gains (the active filter of
+    every controlled source, three instances with different values) and
+    matches the oracle's AC solves."""
+    import krets_amd as kg
+    from conftest import active_netlist
+    nets = [active_netlist(), active_netlist(15e3, 18e3, 2.2e-9, 0.8e-9, 1.5, 0.5, 3e-3, 2e3),
+            active_netlist(8e3, 30e3, 6.8e-9, 1.2e-9, 0.9, 4.0, 0.5e-3, 100.0)]
+    cs = [kg.Circuit(t) for t in nets]
+    c = cs[0]
+    params = np.stack([x.params for x in cs])
+    r, src = source(c, precision, refine)
+    assert r > 0, src
+    f = np.array([10.0, 3e3, 2e4, 1e5, 1e6])
+    x, st, _ = host_run(src, params, c.elements, f, c.n, refine=max(refine, 2), stop=0.0)
+    assert (st == 0).all()
+    for i, t in enumerate(nets):
+        xr = oracle.Circuit(t).ac(f)
+        e = max(nerr(x[i, j], xr[j]) for j in range(len(f)))
+        assert e < (1e-12 if precision == 0 else 1e-9), (i, e)

@@ -252,3 +252,117 @@ def test_lte_rectifier_accuracy(oracle):
     tf = np.arange(xf.shape[0]) * 2e-7
     vo = np.interp(t, tf, xf[:, c.index["V(out)"]])
     assert np.max(np.abs(x[:, c.index["V(out)"]] - vo)) < 2e-2
+
+
+# ----------------------------------------------- controlled sources (E, G, F, H)
+def test_controlled_sources_operating_points(oracle):
+    """SPICE definitions of the four linear dependent sources (krets' README
+    to-do list, README.md:56-62) on hand-solved circuits."""
+    from conftest import CS_NETS
+    exp = {"vcvs": {"V(out)": 10.0, "I(E1)": -0.01, "I(V1)": -1e-3},
+           "vccs": {"V(out)": -2.0, "I(V1)": -2e-3},
+           "cccs": {"I(Vs)": 1e-3, "V(out)": 5.0, "V(mid)": 0.0},
+           "ccvs": {"I(Vs)": 1e-3, "V(out)": 2.0, "I(H1)": -2e-3}}
+    for name, vals in exp.items():
+        c = oracle.Circuit(CS_NETS[name])
+        for cost in (0, 1):
+            s = sol(c, c.op(oracle.config(cost=cost))[0])
+            for k, v in vals.items():
+                assert abs(s[k] - v) <= 1e-15 * max(abs(v), 1.0), (name, k, s[k])
+    c = oracle.Circuit(CS_NETS["vcvs"])
+    assert c.names == ["I(V1)", "V(in)", "I(E1)", "V(out)"]  # branch first, then n+ n- nc+ nc-
+    t, p, m, b, v = c.element(2)
+    assert (t, p, m, b, v) == ("E", 3, -1, 2, 10.0) and c.element_ctrl(2) == (1, -1, -1)
+    c = oracle.Circuit(CS_NETS["cccs"])
+    assert c.element(3)[0] == "F" and c.element_ctrl(3) == (-1, -1, c.index["I(Vs)"])
+
+
+def test_controlled_sources_ac_closed_form(oracle):
+    """The active filter of every controlled source against its transfer
+    functions: Sallen-Key H(s) = 1 / (s^2 R1 R2 C1 C2 + s C2 (R1 + R2) + 1),
+    then I(Vsense) = V(out) / 1k, V(f) = kf I / (1/2k + s Cf),
+    V(g) = -gm Rg V(f), V(h) = rh I."""
+    from conftest import active_netlist
+    r1, r2, c1, c2, kf, gm, rh = 12e3, 27e3, 3.3e-9, 1.5e-9, 3.0, 2e-3, 750.0
+    c = oracle.Circuit(active_netlist(r1, r2, c1, c2, 1.0, kf, gm, rh))
+    f = np.array([10.0, 1e3, 5e3, 2e4, 1e5, 1e6])
+    x = c.ac(f)
+    s = 2j * np.pi * f
+    vout = 1.0 / (s * s * r1 * r2 * c1 * c2 + s * c2 * (r1 + r2) + 1.0)
+    isense = vout / 1e3
+    vf = kf * isense / (1.0 / 2e3 + s * 1e-9)
+    ref = {"V(out)": vout, "I(Vsense)": isense, "V(f)": vf, "V(g)": -gm * 1e3 * vf, "V(h)": rh * isense,
+           "V(in)": np.ones_like(s)}
+    for k, v in ref.items():
+        got = x[:, c.index[k]]
+        assert np.max(np.abs(got - v) / np.abs(v)) < 1e-12, k
+
+
+def test_controlled_sources_transient_closed_form(oracle):
+    """Backward Euler through an E buffer and an H sensing an inductor
+    current: v_n = 1 - (1 + h/RC)^-n, i_n = (1/R)(1 - (1 + hR/L)^-n)."""
+    net = ("V1 in 0 PULSE(0 1 0 0 0 1 1)\nR1 in a 1k\nC1 a 0 1u\nE1 out 0 a 0 3\nRL out 0 100\n"
+           "R2 in b 10\nL1 b 0 1m\nH1 o2 0 L1 5\nR3 o2 0 1k\n")
+    c = oracle.Circuit(net)
+    h = 1e-5
+    x, _ = c.tran(h, 1e-3, method=oracle.BE)
+    n = np.arange(x.shape[0])
+    v = 1 - (1 + h / 1e-3) ** -n
+    i = (1 - (1 + h * 10 / 1e-3) ** -n) / 10
+    assert np.max(np.abs(x[:, c.index["V(out)"]] - 3 * v)) < 1e-13
+    assert np.max(np.abs(x[:, c.index["I(L1)"]] - i)) < 1e-15
+    assert np.max(np.abs(x[:, c.index["V(o2)"]] - 5 * i)) < 1e-14
+
+
+def test_controlled_sources_in_subcircuits(oracle):
+    """An op-amp macro (E) with a finite gain A in a non-inverting amplifier,
+    V(out) = A / (1 + A/10), and a current mirror (F) whose sense source is
+    the instance's own: controlling nodes map through the pins and a
+    controlling element name gets the instance prefix. Ground is passed as a
+    pin: krets makes "0" inside a definition a local node."""
+    A = 1e5
+    net = (".subckt opamp inp inn out gnd\nE1 out gnd inp inn 1e5\n.ends\n"
+           ".subckt mirror a b gnd\nVs a gnd 0\nF1 gnd b Vs 2\n.ends\n"
+           "V1 in 0 1\nX1 in fb out 0 opamp\nR1 out fb 9k\nR2 fb 0 1k\n"
+           "V2 n1 0 1\nR3 n1 ma 1k\nX2 ma mb 0 mirror\nR4 mb 0 1k\n")
+    c = oracle.Circuit(net)
+    s = sol(c, c.op()[0])
+    assert abs(s["V(out)"] - A / (1 + A / 10)) < 1e-12
+    assert abs(s["I(V2_s)"] - 1e-3) < 1e-17 and abs(s["V(mb)"] - 2.0) < 1e-13
+    assert "I(E1_1)" in c.index
+
+
+def test_controlled_source_errors(oracle):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.Circuit("V1 1 0 1\nR1 1 0 1k\nF1 0 2 Vx 2\nR2 2 0 1k\n")
+    assert e.value.code == oracle.ELEMENT_NOT_FOUND
+    with pytest.raises(oracle.OracleError) as e:  # R1 has no branch current
+        oracle.Circuit("V1 1 0 1\nR1 1 0 1k\nH1 2 0 R1 2\nR2 2 0 1k\n")
+    assert e.value.code == oracle.ELEMENT_NOT_FOUND
+
+
+def test_adjoint_with_controlled_sources_vs_fd(oracle):
+    """Discrete adjoint of the active filter: R and C gradients against
+    central finite differences (controlled-source gains are constants of the
+    step matrix and get no gradient)."""
+    from conftest import active_netlist
+    c = oracle.Circuit(active_netlist())
+    cvec = np.zeros(c.n)
+    cvec[c.index["V(g)"]] = 1.0
+    cvec[c.index["V(out)"]] = 0.5
+    h, N = 1e-6, 120
+    for method in (oracle.TRAP, oracle.BE):
+        J, g = c.adjoint(h, N, cvec, method=method)
+        for e in range(c.n_elements):
+            t, _, _, _, v = c.element(e)
+            if t not in "RC":
+                assert t not in "EGFH" or g[e] == 0.0
+                continue
+            d = 1e-6 * v
+            c.set_value(e, v + d)
+            Jp, _ = c.adjoint(h, N, cvec, method=method)
+            c.set_value(e, v - d)
+            Jm, _ = c.adjoint(h, N, cvec, method=method)
+            c.set_value(e, v)
+            fd = (Jp - Jm) / (2 * d)
+            assert abs(g[e] - fd) <= 1e-6 * abs(fd) + 1e-9 * abs(J) / v, (method, e, g[e], fd)

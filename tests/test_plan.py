@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import plan_emulator as pe
-from conftest import read_circuit
+from conftest import CS_NETS, active_netlist, read_circuit
 
 CIRCUITS = [
     "circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir",
@@ -27,7 +27,13 @@ CIRCUITS = [
     "dual_rc_ladder/dual_rc_ladder.cir",
     "rectifier/rectifier.cir",
     "resistor_ladder_500/resistor_ladder_500.cir",
-]
+] + sorted(CS_NETS) + ["active"]
+
+
+def netlist(case):
+    if case == "active":
+        return active_netlist()
+    return CS_NETS[case] if case in CS_NETS else read_circuit(case)
 VT = 1.380649e-23 * 300.0 / 1.602176634e-19
 
 
@@ -68,8 +74,8 @@ def values(c, kind, xop=None, f=0.0, h=1e-9, alpha=1.0):
             return -pr[e, 0]
         return 1.0
 
-    def contrib(e, k):
-        return [y(e), -y(e), 1.0, -1.0, z(e)][k]
+    def contrib(e, k):  # k 5 / 6: +- controlled-source gain
+        return [y(e), -y(e), 1.0, -1.0, z(e), pr[e, 0], -pr[e, 0]][k]
 
     def rhs(e, k):
         t = el[e]["type"]
@@ -87,7 +93,7 @@ def values(c, kind, xop=None, f=0.0, h=1e-9, alpha=1.0):
 @pytest.mark.parametrize("path", CIRCUITS)
 def test_plan_dc_ac_matches_oracle(oracle, path, ordering):
     import krets_amd as k
-    text = read_circuit(path)
+    text = netlist(path)
     c = k.Circuit(text)
     oc = oracle.Circuit(text)
     assert c.names == oc.names
@@ -188,7 +194,7 @@ def lin_bytes(h):
 
 
 @pytest.mark.parametrize("method", [0, 1, 2])
-@pytest.mark.parametrize("case", LIN_CASES + ["ladder5", "ladder23", "c3"])
+@pytest.mark.parametrize("case", LIN_CASES + ["ladder5", "ladder23", "c3", "active"])
 def test_lin_plan_step_loop_vs_oracle(oracle, case, method):
     """The compact plan of the LDS-resident transient kernel and its step
     loop, run on the CPU, against the oracle's fixed-step transient."""
@@ -199,6 +205,8 @@ def test_lin_plan_step_loop_vs_oracle(oracle, case, method):
     elif case == "c3":
         b = W.c3_rlc_uneven(3, smin=8, smax=40, seed=11)
         text, h, N = b.netlists[2], 10e-12, 25
+    elif case == "active":
+        text, h, N = active_netlist(), 2e-6, 40
     else:
         text, h, N = read_circuit(case), 50e-6, 25
     c = k.Circuit(text)
@@ -233,7 +241,7 @@ def test_lin_plan_sizes():
 
 
 @pytest.mark.parametrize("method", [1, 2])
-@pytest.mark.parametrize("case", ["rlc4", "rc_ladder"])
+@pytest.mark.parametrize("case", ["rlc4", "rc_ladder", "active"])
 def test_lin_adjoint_vs_oracle(oracle, case, method):
     """The LDS-resident adjoint's algorithm on the compact plan (transposed
     level solves, element records, checkpoint segments), run on the CPU,
@@ -243,6 +251,8 @@ def test_lin_adjoint_vs_oracle(oracle, case, method):
     if case == "rlc4":
         text = W.c3_netlist([20.0, 35.0, 50.0, 65.0], [2e-9, 3e-9, 4e-9, 5e-9], [0.3e-12, 0.5e-12, 0.7e-12, 0.9e-12])
         h, N, K, node = 1e-11, 40, 16, "V(n4)"
+    elif case == "active":
+        text, h, N, K, node = active_netlist(), 2e-6, 40, 8, "V(g)"
     else:
         b = W.c5_rc_ladder(1, sections=12, seed=5)
         text, h, N, K, node = b.netlists[0], 1e-9, 30, 8, "V(n12)"
