@@ -251,6 +251,14 @@ int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n
 int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
                          double s_rep, int32_t precision, int32_t refine_iters, char* buf, int32_t cap);
 
+/* Diagnostic: the HIP source of the run-time specialised lane-per-instance
+ * kernel of kg_transient_lte for one topology (minimum-degree plan of the
+ * transient stamps, static pivots from params at step h0). No counterpart in
+ * krets. Returns the source length (copying at most cap bytes,
+ * NUL-terminated, to buf) or -status, with the reason in buf. */
+int32_t kg_jit_lte_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
+                          double h0, char* buf, int32_t cap);
+
 /* Netlist reader for the R/L/C/V/I/D subset of krets' grammar
  * (crates/krets-parser/src/parser.rs:27-154). */
 int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* err, int32_t errlen);

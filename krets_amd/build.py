@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libkrets_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["kernels.hip", "api.cpp", "symbolic.cpp", "netlist.cpp", "jit.cpp", "spec.cpp"]
+SOURCES = ["kernels.hip", "api.cpp", "symbolic.cpp", "netlist.cpp", "jit.cpp", "jit_tran.cpp", "spec.cpp"]
 # Packed FP32 instructions are disabled for all device code.
 DEVICE_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 

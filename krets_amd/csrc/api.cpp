@@ -95,6 +95,11 @@ struct kg_ctx {
   // captured launch sequences of repeated calls (AC sweeps): key -> graph
   bool graphs = true;
   std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> gexec;
+  // pinned host staging for the small per-call outputs (statuses, error
+  // estimates): a DMA each instead of a pageable copy; used only inside
+  // finish_status, which leaves no copy in flight into it
+  char* pin = nullptr;
+  size_t pin_cap = 0;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
   std::map<std::string, std::pair<double, int64_t>> prof_k;  // kernel -> (ms, launches)
@@ -275,6 +280,8 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
   if (added || !c->d_pats) {
     std::vector<PatDev> hv;
     for (auto& p : c->pats) hv.push_back(p.dev);
+    // kernels of this call already queued on the stream may still read the old table
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->d_pats) hipFree(c->d_pats);
     c->d_pats = nullptr;
     HIPCHK(c, hipMalloc(&c->d_pats, sizeof(PatDev) * hv.size()));
@@ -344,6 +351,22 @@ int32_t copy_out(kg_ctx* c, void* dst, const void* src, size_t bytes, int mem) {
   if (!dst || dst == src || bytes == 0) return KG_OK;
   HIPCHK(c, hipMemcpyAsync(dst, src, bytes, mem == KG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                            c->stream));
+  return KG_OK;
+}
+
+// Writes the status of each structurally failed topology group into its
+// instances' device statuses, in stream order after the call's kernels.
+int32_t mark_group_failures(kg_ctx* c, const Plan& P, int32_t* d_st) {
+  bool any = false;
+  for (int32_t g : P.group_status) any |= g != KG_OK;
+  if (!any) return KG_OK;
+  std::vector<int32_t> st(c->n_inst);
+  HIPCHK(c, hipMemcpyAsync(st.data(), d_st, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->n_inst; i++)
+    if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
+  HIPCHK(c, hipMemcpyAsync(d_st, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return KG_OK;
 }
 
@@ -565,6 +588,7 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
     for (int i = 0; i < c->n_inst; i++)
       if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
     HIPCHK(c, hipMemcpyAsync(*d_status, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // st is a local host buffer
   }
   return KG_OK;
 }
@@ -798,7 +822,8 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   });
 }
 
-int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem);
+int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem, void* xdst = nullptr,
+                      const void* xsrc = nullptr, size_t xbytes = 0);
 int32_t run_adj_op_term(kg_ctx* c, const Plan& P, double* d_xb, double* d_g, int ne, int32_t* d_st);
 
 // Backward pass with the LDS-resident adjoint kernel (FP64), then the
@@ -860,14 +885,37 @@ int32_t run_adj_op_term(kg_ctx* c, const Plan& P, double* d_xb, double* d_g, int
   });
 }
 
-int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem) {
-  std::vector<int32_t> st(c->n_inst);
-  HIPCHK(c, hipMemcpyAsync(st.data(), d_status, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost, c->stream));
-  if (status_out) {
-    int32_t r = copy_out(c, status_out, d_status, sizeof(int32_t) * c->n_inst, mem);
-    if (r) return r;
+// The end of every analysis call: copies the statuses (and one optional small
+// per-instance output, xdst <- xsrc) to the caller, synchronises the stream
+// and returns the first failure. Host copies go through the pinned staging
+// area; every copy enqueued here has completed when it returns.
+int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem, void* xdst,
+                      const void* xsrc, size_t xbytes) {
+  const size_t sb = sizeof(int32_t) * c->n_inst;
+  const size_t xo = (sb + 255) & ~(size_t)255;
+  const bool xcopy = xdst && xsrc && xbytes, xhost = xcopy && mem == KG_MEM_HOST;
+  const size_t need = xo + (xhost ? xbytes : 0);
+  if (need > c->pin_cap) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->pin) (void)hipHostFree(c->pin);
+    c->pin = nullptr;
+    c->pin_cap = 0;
+    const size_t cap = std::max<size_t>(2 * need, 1 << 16);
+    HIPCHK(c, hipHostMalloc((void**)&c->pin, cap, hipHostMallocDefault));
+    c->pin_cap = cap;
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  hipError_t e = hipMemcpyAsync(c->pin, d_status, sb, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && xcopy)
+    e = hipMemcpyAsync(xhost ? (void*)(c->pin + xo) : xdst, xsrc, xbytes,
+                       xhost ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess && status_out && mem != KG_MEM_HOST)
+    e = hipMemcpyAsync(status_out, d_status, sb, hipMemcpyDeviceToDevice, c->stream);
+  const hipError_t es = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess || es != hipSuccess)
+    return set_err(c, KG_HIP_ERROR, std::string("status copy: ") + hipGetErrorString(e != hipSuccess ? e : es));
+  std::vector<int32_t> st((const int32_t*)c->pin, (const int32_t*)c->pin + c->n_inst);
+  if (status_out && mem == KG_MEM_HOST) std::memcpy(status_out, c->pin, sb);
+  if (xhost) std::memcpy(xdst, c->pin + xo, xbytes);
   int32_t f = first_failure(st);
   if (f != KG_OK && c->err.empty()) c->err = "at least one instance failed";
   return f;
@@ -913,6 +961,7 @@ int32_t kg_destroy(kg_ctx* c) {
     if (kv.second.p) hipFree(kv.second.p);
   for (hipStream_t s : c->aux) hipStreamDestroy(s);
   hipStreamDestroy(c->stream);
+  if (c->pin) hipHostFree(c->pin);
   delete c;
   return KG_OK;
 }
@@ -1247,17 +1296,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
     }
   }
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
-  if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;
-  for (int g = 0; g < (int)P.group_status.size(); g++)
-    if (P.group_status[g] != KG_OK) {
-      std::vector<int32_t> st(c->n_inst);
-      HIPCHK(c, hipMemcpy(st.data(), d_st, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost));
-      for (int i = 0; i < c->n_inst; i++)
-        if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
-      HIPCHK(c, hipMemcpy(d_st, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice));
-      break;
-    }
-  return finish_status(c, d_st, status, mem);
+  if ((r = mark_group_failures(c, P, d_st))) return r;
+  return finish_status(c, d_st, status, mem, err_est, d_err, sizeof(double) * c->n_inst);
 }
 
 int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
@@ -1382,17 +1422,8 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   }
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if (iters && (r = copy_out(c, iters, d_iters, sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), mem))) return r;
-  if (err_est && (r = copy_out(c, err_est, d_err, sizeof(double) * c->n_inst, mem))) return r;
-  for (int g = 0; g < (int)P.group_status.size(); g++)
-    if (P.group_status[g] != KG_OK) {
-      std::vector<int32_t> st(c->n_inst);
-      HIPCHK(c, hipMemcpy(st.data(), d_st, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToHost));
-      for (int i = 0; i < c->n_inst; i++)
-        if (P.group_status[c->group_of[i]] != KG_OK) st[i] = P.group_status[c->group_of[i]];
-      HIPCHK(c, hipMemcpy(d_st, st.data(), sizeof(int32_t) * c->n_inst, hipMemcpyHostToDevice));
-      break;
-    }
-  return finish_status(c, d_st, status, mem);
+  if ((r = mark_group_failures(c, P, d_st))) return r;
+  return finish_status(c, d_st, status, mem, err_est, d_err, sizeof(double) * c->n_inst);
 }
 
 int32_t kg_last_autotune(const kg_ctx* c, int32_t* tps, float* ms, int32_t cap) {
@@ -1473,6 +1504,21 @@ int32_t kg_jit_ac_source(const kg_element* el, int32_t ne, int32_t n, const doub
   return (int32_t)src.size();
 }
 
+int32_t kg_jit_lte_source(const kg_element* el, int32_t ne, int32_t n, const double* params, double h0, char* buf,
+                          int32_t cap) {
+  if (!el || ne <= 0 || n <= 0 || !(h0 > 0)) return -KG_INVALID_ARGUMENT;
+  PatLayout L;
+  std::string err, src, why;
+  std::vector<int32_t> b = build_pattern(el, ne, n, AK_TRAN, ORD_MINDEG, params, 1.0 / h0, L, err);
+  if (L.status != KG_OK) return -L.status;
+  if (!jit_lte_source(L, b.data(), src, why)) {
+    if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
+    return -KG_UNEXPECTED;
+  }
+  if (buf && cap > 0) std::snprintf(buf, cap, "%s", src.c_str());
+  return (int32_t)src.size();
+}
+
 int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, double h0, double hmin, double hmax,
                          double lte_reltol, double lte_abstol, int32_t max_points, double* t, double* x,
                          int32_t* n_points, int32_t* newton_total, int32_t* rejected, int32_t* status, int32_t mem) {
@@ -1489,21 +1535,9 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   int32_t *d_st, *d_it;
   int32_t r = run_dc(c, &ocfg, -1, 0.0, 0.0, 1, &d_xop, &d_xoff, &d_st, &d_it);
   if (r) return r;
-  Plan P0, P;
+  Plan P0;
   double s_tr = 1.0 / h0;
   if ((r = prepare(c, AK_TRAN, ORD_NESTED, s_tr, P0))) return r;
-  int tps = cfg->threads_per_system > 0 ? cfg->threads_per_system : default_tps(c, P0.n_max);
-  if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
-  if ((r = prepare(c, AK_TRAN, order_for(tps), s_tr, P))) return r;
-  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 2, 1, 8, 8, 0, 0};
-  Launch L;
-  // stage the plan in LDS when it fits (the whole step loop then reads it there)
-  int pi = P.total_max;
-  if (configure(c, tps, lay, c->n_inst, L, lay_al(4 * (size_t)pi)) != KG_OK) {
-    pi = 0;
-    if ((r = configure(c, tps, lay, c->n_inst, L))) return r;
-  }
-  c->err.clear();
   auto off = out_offsets(c, max_points);
   int64_t* d_off;
   if ((r = upload_offsets(c, "lte_off", off, &d_off))) return r;
@@ -1514,17 +1548,103 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   int32_t* d_np = (int32_t*)ws_get(c, "lte_np", sizeof(int32_t) * ni, &e);
   int32_t* d_nt = (int32_t*)ws_get(c, "lte_nt", sizeof(int32_t) * ni, &e);
   int32_t* d_rj = (int32_t*)ws_get(c, "lte_rj", sizeof(int32_t) * ni, &e);
+  int32_t* d_st0 = (int32_t*)ws_get(c, "lte_st0", sizeof(int32_t) * ni, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
-  HIPCHK(c, hipMemcpyAsync(d_nt, d_it, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipMemsetAsync(d_t, 0, sizeof(double) * ni * max_points, c->stream));
+  // every run (autotuner trials included) starts from the operating point's statuses
+  HIPCHK(c, hipMemcpyAsync(d_st0, d_st, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
   NewtonArgs na = newton_args(cfg);
   na.compat = 0;
   LteArgs la{tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points};
-  BatchDev B = batch_dev(c, P);
-  r = timed(c, "k_tran_lte", [&] {
-    return run_lte(tps, L.grid, L.block, L.lds, c->stream, B, 0, na, la, d_xop, d_xoff, d_t, d_x, d_off, d_np, d_nt,
-                   d_rj, d_st, lay, pi);
-  });
+  // A mapping candidate: 1..256 threads per system (the generic kernel, plan
+  // interpreted from LDS) or 2001 (one lane per instance, code specialised
+  // to the shared topology at run time).
+  struct LteLaunch {
+    bool lane = false;
+    int tps = 0, pi = 0;
+    Plan P;
+    Lay lay{};
+    Launch L;
+    JitFn fn;
+  };
+  auto setup = [&](int cand, LteLaunch& A) -> int32_t {
+    A.lane = cand == 2001;
+    if (A.lane) {
+      if (!c->shared) return set_err(c, KG_UNEXPECTED, "lane kernel needs a shared topology");
+      int32_t rr = prepare(c, AK_TRAN, ORD_MINDEG, s_tr, A.P);
+      if (rr) return rr;
+      if (A.P.group_status[0] != KG_OK) return set_err(c, KG_UNEXPECTED, "lane kernel: no valid plan");
+      const int pidx = c->pat_index[std::make_tuple(0, (int)AK_TRAN, (int)ORD_MINDEG, s_tr)];
+      const auto key = std::make_tuple(pidx, 100, 0);
+      auto it = c->lane_fn.find(key);
+      if (it == c->lane_fn.end()) {
+        const PatEntry& pe = c->pats[pidx];
+        std::string src, why, log;
+        if (!jit_lte_source(pe.lay, pe.host.data(), src, why)) return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
+        if (jit_load(c->device, src, jit_lte_name(), A.fn, log) != hipSuccess)
+          return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
+        c->lane_fn[key] = A.fn;
+      } else {
+        A.fn = it->second;
+      }
+      A.L.grid = dim3((unsigned)((ni + 63) / 64));
+      A.L.block = dim3(64);
+      A.L.lds = 0;
+      return KG_OK;
+    }
+    A.tps = cand;
+    if (!valid_tps(A.tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64, 256 or 2001");
+    int32_t rr = prepare(c, AK_TRAN, order_for(A.tps), s_tr, A.P);
+    if (rr) return rr;
+    A.lay = Lay{A.P.nnz_max, A.P.n_max, A.P.ne_max, 1, 2, 1, 8, 8, 0, 1};
+    // stage the plan and each system's element records in LDS when they fit
+    // (the whole step loop then reads them there); else the plan alone; else neither
+    A.pi = A.P.total_max;
+    if (configure(c, A.tps, A.lay, c->n_inst, A.L, lay_al(4 * (size_t)A.pi)) != KG_OK) {
+      A.lay.stage_prm = 0;
+      if (configure(c, A.tps, A.lay, c->n_inst, A.L, lay_al(4 * (size_t)A.pi)) != KG_OK) {
+        A.pi = 0;
+        if ((rr = configure(c, A.tps, A.lay, c->n_inst, A.L))) return rr;
+      }
+    }
+    c->err.clear();
+    return KG_OK;
+  };
+  auto run = [&](LteLaunch& A) -> hipError_t {
+    hipError_t he = hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(d_nt, d_it, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream);
+    if (he == hipSuccess) he = hipMemsetAsync(d_t, 0, sizeof(double) * ni * max_points, c->stream);
+    if (he != hipSuccess) return he;
+    if (A.lane) {
+      BatchDev B = batch_dev(c, A.P);
+      const double* prm = B.params;
+      const double* eb = B.ebase;
+      int nia = (int)ni, mp = max_points, mi = na.maxiter;
+      double a_tstop = tstop, a_h0 = h0, a_hmin = hmin, a_hmax = hmax, a_rel = lte_reltol, a_abs = lte_abstol;
+      double a_reltol = na.reltol, a_itol = na.iabstol, a_vtol = na.vabstol;
+      void* args[] = {&prm,    &eb,   &d_xop, &d_xoff, &d_t,   &d_x,     &d_off,  &d_np,   &d_nt,
+                      &d_rj,   &d_st, &nia,   &a_tstop, &a_h0, &a_hmin, &a_hmax, &a_rel, &a_abs,
+                      &mp,     &a_reltol, &a_itol, &a_vtol, &mi};
+      return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr);
+    }
+    BatchDev B = batch_dev(c, A.P);
+    return run_lte(A.tps, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, na, la, d_xop, d_xoff, d_t, d_x, d_off, d_np,
+                   d_nt, d_rj, d_st, A.lay, A.pi);
+  };
+  int cand = cfg->threads_per_system;
+  if (cand == 0) {
+    std::vector<int> cands{default_tps(c, P0.n_max)};
+    if (c->shared) cands.push_back(2001);
+    r = autotune(c, tune_key(c, "lte", KG_FP64, P0, (int64_t)ni), cands, [&](int tc, float* ms) -> int32_t {
+      LteLaunch A;
+      int32_t rr = setup(tc, A);
+      if (rr) return rr;
+      return time_launch(c, [&] { return run(A); }, ms);
+    }, &cand);
+    if (r) return r;
+  }
+  LteLaunch A;
+  if ((r = setup(cand, A))) return r;
+  r = timed(c, A.lane ? jit_lte_name() : "k_tran_lte", [&] { return run(A); });
   if (r) return r;
   if ((r = copy_out(c, t, d_t, sizeof(double) * ni * max_points, mem))) return r;
   if ((r = copy_out(c, x, d_x, sizeof(double) * off.back(), mem))) return r;

@@ -1,0 +1,326 @@
+// Run-time specialisation of the LTE-controlled backward-Euler transient
+// (kg_transient_lte) to one circuit topology.
+//
+// The generic kernel interprets the plan (gather lists, factor program,
+// level lists) with one thread per small system, so every Newton iteration
+// is a chain of dependent plan and parameter reads. Here the plan is written
+// out as straight-line HIP: one lane integrates one instance, with its
+// element values, matrix slots, solution, previous states and diode states
+// in registers. The step controller, the Newton loop, the junction limiting
+// and the stamps follow the generic kernel operation for operation (the same
+// contributions summed in the same order, the same LU: multipliers
+// l_ik = a_ik * (1 / u_kk) stored in place, the upper factor unscaled, the
+// pivot reciprocal kept on the diagonal), so both produce the same accepted
+// time grid up to the compiler's contractions.
+#include <cstdio>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "kg_jit.h"
+
+namespace kg {
+namespace {
+
+struct TV {
+  int n = 0, nnz = 0, ne = 0;
+  const int32_t *row_ptr, *col, *diag, *fprog_ptr, *fprog, *scon_ptr, *scon, *rcon_ptr, *rcon, *col_of, *isbr,
+      *etype, *ep, *em, *ec;
+};
+
+TV tview(const PatLayout& L, const int32_t* b) {
+  TV v;
+  v.n = L.n;
+  v.nnz = L.nnz;
+  v.ne = L.ne;
+  v.row_ptr = b + L.o_row_ptr;
+  v.col = b + L.o_col;
+  v.diag = b + L.o_diag;
+  v.fprog_ptr = b + L.o_fprog_ptr;
+  v.fprog = b + L.o_fprog;
+  v.scon_ptr = b + L.o_scon_ptr;
+  v.scon = b + L.o_scon;
+  v.rcon_ptr = b + L.o_rcon_ptr;
+  v.rcon = b + L.o_rcon;
+  v.col_of = b + L.o_col_of;
+  v.isbr = b + L.o_isbr;
+  v.etype = b + L.o_etype;
+  v.ep = b + L.o_ep;
+  v.em = b + L.o_em;
+  v.ec = b + L.o_ec;
+  return v;
+}
+
+// Waveforms of the generic kernels (Pulse / Sinusoidal value_at,
+// voltage_source.rs:50-93), the thermal voltage (constants.rs:1-4).
+const char* kLtePrelude = R"(typedef long long i64;
+#define KG_VT (1.380649e-23 * 300.0 / 1.602176634e-19)
+__device__ __forceinline__ double wave(const double* p, double t) {
+  const int kind = (int)p[2];
+  const double* w = p + 3;
+  if (kind == 1) {
+    const double v1 = w[0], v2 = w[1], td = w[2], tr = w[3], tf = w[4], pw = w[5], per = w[6];
+    if (t < td) return v1;
+    const double tt = fmod(t - td, per);
+    if (tt < tr) return v1 + (v2 - v1) * (tt / tr);
+    if (tt < tr + pw) return v2;
+    if (tt < tr + pw + tf) return v2 + (v1 - v2) * ((tt - tr - pw) / tf);
+    return v1;
+  }
+  if (kind == 2) {
+    const double vo = w[0], va = w[1], f = w[2], td = w[3], th = w[4], ph = w[5];
+    if (t < td) return vo;
+    const double om = 2.0 * 3.14159265358979323846 * f;
+    const double phr = ph * (3.14159265358979323846 / 180.0);
+    const double tt = t - td;
+    return vo + va * exp(-th * tt) * sin(om * tt + phr);
+  }
+  return p[0];
+}
+)";
+
+std::string S(int i) { return std::to_string(i); }
+
+// Value of permuted unknown i in vector `pre` (ground: 0.0), as the generic
+// kernels write it: (p >= 0 ? x[p] : 0.0).
+std::string node(const std::string& pre, int i) { return i >= 0 ? pre + S(i) : std::string("0.0"); }
+
+}  // namespace
+
+const char* jit_lte_name() { return "kg_lte_lane"; }
+
+bool jit_lte_source(const PatLayout& L, const int32_t* b, std::string& src, std::string& why) {
+  if (L.status != KG_OK) {
+    why = "structurally singular pattern";
+    return false;
+  }
+  TV P = tview(L, b);
+  const int n = P.n, ne = P.ne;
+  // registers held across a Newton iteration (2 per double): matrix slots,
+  // right-hand side / solution, guess, two previous states, element values
+  int nvals = 0, nd = 0;
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    nvals += ty == KG_V ? 10 : ty == KG_D ? 4 : 1;
+    nd += ty == KG_D;
+  }
+  const int regs = 2 * (P.nnz + 5 * n + 3 * nd + nvals);
+  if (regs > 400) {
+    why = "per-lane state exceeds the register budget";
+    return false;
+  }
+  std::ostringstream o;
+  o << kLtePrelude;
+  o << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_lte_name()
+    << "(const double* __restrict__ prm,\n"
+       "    const double* __restrict__ ebase, const double* __restrict__ xop, const i64* __restrict__ xop_off,\n"
+       "    double* __restrict__ tout, double* __restrict__ xout, const i64* __restrict__ xout_off,\n"
+       "    int* __restrict__ npts, int* __restrict__ ntot, int* __restrict__ rejected, int* __restrict__ status,\n"
+       "    int ninst, double tstop, double h0, double hmin, double hmax, double lte_rel, double lte_abs,\n"
+       "    int max_points, double reltol, double iabstol, double vabstol, int maxiter) {\n"
+       "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
+       "  if (inst >= ninst) return;\n"
+       "  int st = status[inst], cnt = 0, nt = ntot[inst], rej = 0;\n"
+       "  if (st == 0) {\n";
+  // element values, loaded once
+  o << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
+    << "  const double* EB = ebase + (i64)inst * " << ne << ";\n";
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    const std::string pe = "PR[" + S(e * KG_NPARAM);
+    if (ty == KG_V) {
+      o << "  const double pw" << e << "[10] = {";
+      for (int k = 0; k < 10; k++) o << (k ? ", " : "") << "PR[" << e * KG_NPARAM + k << "]";
+      o << "};\n";
+    } else if (ty == KG_D) {
+      // diode.rs:24-71 with SPICE3 junction limiting (the generic clean mode)
+      o << "  const double is" << e << " = " << pe << "], nvt" << e << " = PR[" << e * KG_NPARAM + 1 << "] * KG_VT;\n"
+        << "  const double vcrit" << e << " = nvt" << e << " * log(nvt" << e << " / (1.4142135623730951 * is" << e
+        << "));\n";
+    } else if (ty == KG_R || ty == KG_E || ty == KG_G || ty == KG_F || ty == KG_H) {
+      o << "  const double g" << e << " = EB[" << e << "];\n";  // 1/R (value for G2 resistors), gains
+      if (ty == KG_R) o << "  const double r" << e << " = " << pe << "];\n";
+    } else {
+      o << "  const double p" << e << " = " << pe << "];\n";  // C, L, I values
+    }
+  }
+  // initial state: the operating point (permuted numbering), row 0 of the output
+  o << "  const double* x0 = xop + xop_off[inst];\n  double* o = xout + xout_off[inst];\n";
+  for (int j = 0; j < n; j++) o << "  o[" << j << "] = x0[" << j << "];\n";
+  for (int i = 0; i < n; i++) o << "  double xc" << i << " = x0[" << P.col_of[i] << "], xp" << i << " = xc" << i << ";\n";
+  o << "  tout[(i64)inst * max_points] = 0.0;\n  cnt = 1;\n"
+       "  double tcur = 0.0, h = h0, tprev = 0.0;\n  bool have_prev = false;\n"
+       "  const i64 guard_max = 64 * (i64)max_points + 1000;\n"
+       "  for (i64 guard = 0;; guard++) {\n"
+    << "    if (guard > guard_max) { st = " << (int)KG_MAX_ITERATIONS << "; break; }\n"
+    << "    h = fmin(h, hmax);\n    if (tcur + h > tstop) h = tstop - tcur;\n    const double tn = tcur + h;\n";
+  // per-step values: sources at tn, companion conductances and currents
+  // (stampable.rs:360-506, 676-817: i_eq = -(C/h) v_prev, -(L/h) i_prev)
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    if (ty == KG_V) o << "    const double vs" << e << " = wave(pw" << e << ", tn);\n";
+    if (ty == KG_C) {
+      o << "    const double ch" << e << " = 1.0 * (p" << e << " / h);\n"
+        << "    const double cq" << e << " = -(ch" << e << ") * (" << node("xc", P.ep[e]) << " - " << node("xc", P.em[e])
+        << ");\n";
+    }
+    if (ty == KG_L) {
+      o << "    const double lh" << e << " = 1.0 * (p" << e << " / h);\n"
+        << "    const double lq" << e << " = -(lh" << e << ") * xc" << P.ec[e] << ";\n";
+    }
+  }
+  // Newton from the previous state
+  for (int i = 0; i < n; i++) o << "    double g" << "u" << i << " = xc" << i << ", x" << i << " = 0.0;\n";
+  for (int e = 0; e < ne; e++)
+    if (P.etype[e] == KG_D)
+      o << "    double vo" << e << " = " << node("xc", P.ep[e]) << " - " << node("xc", P.em[e]) << ", gd" << e
+        << " = 0.0, id" << e << " = 0.0;\n";
+  o << "    bool done = false;\n    int its = 0;\n"
+       "    for (int it = 0; it < maxiter; it++) {\n      bool lim = false, bad = false;\n";
+  for (int e = 0; e < ne; e++) {
+    if (P.etype[e] != KG_D) continue;
+    const std::string E = S(e);
+    o << "      {\n        double v = " << node("gu", P.ep[e]) << " - " << node("gu", P.em[e]) << ";\n"
+      << "        if (v > vcrit" << E << " && fabs(v - vo" << E << ") > nvt" << E << " + nvt" << E << ") {\n"
+      << "          if (vo" << E << " > 0) {\n"
+      << "            const double arg = 1.0 + (v - vo" << E << ") / nvt" << E << ";\n"
+      << "            v = arg > 0 ? vo" << E << " + nvt" << E << " * log(arg) : vcrit" << E << ";\n"
+      << "          } else {\n            v = nvt" << E << " * log(v / nvt" << E << ");\n          }\n"
+      << "          lim = true;\n        }\n"
+      << "        const double ex = exp(v / nvt" << E << ");\n"
+      << "        gd" << E << " = is" << E << " / nvt" << E << " * ex;\n"
+      << "        id" << E << " = is" << E << " * (ex - 1.0) - gd" << E << " * v;\n"
+      << "        vo" << E << " = v;\n      }\n";
+  }
+  // matrix slots: contributions in the plan's order, summed from zero
+  for (int s = 0; s < P.nnz; s++) {
+    o << "      double v" << s << " = 0.0";
+    for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) {
+      const int code = P.scon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
+      std::string c;
+      switch (k) {
+        case CK_YP:
+        case CK_YN: {
+          if (ty == KG_R) c = "g" + S(e);
+          else if (ty == KG_C) c = "ch" + S(e);
+          else if (ty == KG_D) c = "gd" + S(e);
+          else c = "0.0";
+          if (k == CK_YN) c = "-" + c;
+          break;
+        }
+        case CK_ONEP: c = "1.0"; break;
+        case CK_ONEN: c = "-1.0"; break;
+        case CK_BR:
+          if (ty == KG_L) c = "-(lh" + S(e) + ")";
+          else if (ty == KG_R) c = "-r" + S(e);
+          else c = "1.0";
+          break;
+        case CK_KP: c = "g" + S(e); break;
+        case CK_KN: c = "-g" + S(e); break;
+        default: why = "unknown stamp kind"; return false;
+      }
+      o << " + (" << c << ")";
+    }
+    o << ";\n";
+  }
+  // right-hand side (permuted rows), then the solve in place (x = b)
+  for (int i = 0; i < n; i++) {
+    o << "      x" << i << " = 0.0";
+    for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) {
+      const int code = P.rcon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
+      std::string c;
+      switch (k) {
+        case RK_SRC: c = ty == KG_V ? "vs" + S(e) : "p" + S(e); break;
+        case RK_DP: c = "-id" + S(e); break;
+        case RK_DM: c = "id" + S(e); break;
+        case RK_CP: c = "-cq" + S(e); break;
+        case RK_CM: c = "cq" + S(e); break;
+        case RK_IND: c = "lq" + S(e); break;
+        default: why = "unknown excitation kind"; return false;
+      }
+      o << " + (" << c << ")";
+    }
+    o << ";\n";
+  }
+  // factorisation, row by row in pivot order (the plan's factor program)
+  for (int i = 0; i < n; i++) {
+    const int32_t* fp = P.fprog + P.fprog_ptr[i];
+    const int nk = *fp++;
+    for (int kk = 0; kk < nk; kk++) {
+      const int sik = fp[0], sdk = fp[1], nu = fp[2];
+      fp += 3;
+      o << "      v" << sik << " = v" << sik << " * v" << sdk << ";\n";
+      for (int u = 0; u < nu; u++, fp += 2) o << "      v" << fp[0] << " = v" << fp[0] << " - v" << sik << " * v" << fp[1] << ";\n";
+    }
+    const int d = P.diag[i];
+    o << "      bad |= !(fabs(v" << d << ") > 0.0) || !__builtin_isfinite(v" << d << ");\n"
+      << "      v" << d << " = 1.0 / v" << d << ";\n";
+  }
+  for (int i = 0; i < n; i++)
+    for (int p = P.row_ptr[i]; p < P.diag[i]; p++) o << "      x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
+  for (int i = n - 1; i >= 0; i--) {
+    const int d = P.diag[i];
+    for (int p = d + 1; p < P.row_ptr[i + 1]; p++) o << "      x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
+    o << "      x" << i << " = x" << i << " * v" << d << ";\n";
+  }
+  // convergence (the clean-mode test of the generic kernels)
+  o << "      bool cv = true;\n";
+  for (int i = 0; i < n; i++)
+    o << "      { const double df = fabs(x" << i << " - gu" << i << "), sc = fmax(fabs(x" << i << "), fabs(gu" << i
+      << ")); if (!(df <= reltol * sc + " << (P.isbr[i] ? "iabstol" : "vabstol") << ")) cv = false; }\n";
+  o << "      bool stop = " << (L.nonlinear ? "cv && !lim" : "true") << ";\n"
+    << "      if (bad) { st = " << (int)KG_DECOMPOSITION << "; stop = true; }\n"
+    << "      if (stop) { done = true; its = it + 1; break; }\n";
+  for (int i = 0; i < n; i++) o << "      gu" << i << " = x" << i << ";\n";
+  o << "    }\n"
+       "    if (st != 0) break;\n"
+       "    if (!done) {  // Newton failure: retry with h / 8\n"
+       "      rej++;\n      h /= 8.0;\n"
+    << "      if (h < hmin) { st = " << (int)KG_MAX_ITERATIONS << "; break; }\n"
+    << "      continue;\n    }\n"
+       "    nt += its;\n"
+       "    double ratio = 0.0;\n"
+       "    if (have_prev) {  // local truncation error of capacitor voltages and inductor currents\n"
+       "      const double hp = tcur - tprev;\n";
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    if (ty != KG_C && ty != KG_L) continue;
+    o << "      {\n";
+    if (ty == KG_C) {
+      o << "        const double a = " << node("x", P.ep[e]) << " - " << node("x", P.em[e]) << ";\n"
+        << "        const double b0 = " << node("xc", P.ep[e]) << " - " << node("xc", P.em[e]) << ";\n"
+        << "        const double bm = " << node("xp", P.ep[e]) << " - " << node("xp", P.em[e]) << ";\n";
+    } else {
+      o << "        const double a = x" << P.ec[e] << ", b0 = xc" << P.ec[e] << ", bm = xp" << P.ec[e] << ";\n";
+    }
+    o << "        const double dd2 = 2.0 * ((a - b0) / h - (b0 - bm) / hp) / (h + hp);\n"
+         "        const double lte = 0.5 * h * h * fabs(dd2);\n"
+         "        const double tol = lte_rel * fmax(fabs(a), fabs(b0)) + lte_abs;\n"
+         "        ratio = fmax(ratio, lte / tol);\n      }\n";
+  }
+  o << "    }\n"
+       "    double hnext = h;\n"
+       "    if (have_prev) {\n"
+       "      if (ratio > 1.0) {\n"
+       "        rej++;\n        h *= fmax(0.25, 0.9 / sqrt(ratio));\n"
+    << "        if (h < hmin) { st = " << (int)KG_MAX_ITERATIONS << "; break; }\n"
+    << "        continue;\n      }\n"
+       "      hnext = ratio > 0.0 ? h * fmin(2.0, fmax(0.25, 0.9 / sqrt(ratio))) : 2.0 * h;\n"
+       "    }\n"
+    << "    if (cnt >= max_points) { st = " << (int)KG_UNEXPECTED << "; break; }\n";
+  for (int i = 0; i < n; i++) o << "    xp" << i << " = xc" << i << ";\n    xc" << i << " = x" << i << ";\n";
+  o << "    {\n      double* orow = o + (i64)cnt * " << n << ";\n";
+  for (int i = 0; i < n; i++) o << "      orow[" << P.col_of[i] << "] = x" << i << ";\n";
+  o << "    }\n"
+       "    tout[(i64)inst * max_points + cnt] = tn;\n"
+       "    cnt++;\n    tprev = tcur;\n    tcur = tn;\n    h = hnext;\n    have_prev = true;\n"
+       "    if (!(tcur < tstop * (1.0 - 1e-12))) break;\n"
+       "  }\n"
+       "  }\n"
+       "  status[inst] = st;\n  npts[inst] = cnt;\n  ntot[inst] = nt;\n  rejected[inst] = rej;\n}\n";
+  src = o.str();
+  return true;
+}
+
+}  // namespace kg

@@ -497,6 +497,7 @@ struct SysMem {
   double* xp;  // previous solution(s) (transient); extra vectors follow at xp + k * xstride
   int xstride;
   double* ed;  // element state
+  double* prm;  // staged element records: params [ne][KG_NPARAM], then the value table [ne]
   int* flags;  // [0] status [1] active [2] iterations [4] any diode limited
 };
 
@@ -527,6 +528,8 @@ __device__ SysMem<F, A> carve(char* base, const Lay& L) {
   m.xstride = (int)(lay_al(8 * (size_t)L.n_max) / 8);
   p += (size_t)L.need_xp * lay_al(8 * (size_t)L.n_max);
   m.ed = L.need_ed ? (double*)p : nullptr;
+  if (L.need_ed) p += lay_al(8 * (size_t)ESTRIDE * L.ne_max);
+  m.prm = L.stage_prm ? (double*)p : nullptr;
   return m;
 }
 
@@ -1750,6 +1753,14 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
   const double* eb = B.ebase + B.eoff[ii];
   const double* x0 = xop + xop_off[ii];
   if (t == 0) M.flags[0] = valid ? status[inst] : KG_OK;
+  if (lay.stage_prm) {
+    // every Newton iteration reads the element records: keep the system's in LDS
+    const int np = P.ne * KG_NPARAM;
+    for (int k = t; k < np; k += TPS) M.prm[k] = prm[k];
+    for (int k = t; k < P.ne; k += TPS) M.prm[np + k] = eb[k];
+    prm = M.prm;
+    eb = M.prm + np;
+  }
   __syncthreads();
   bool active = valid && M.flags[0] == KG_OK;
   int cnt = 0, nt = valid ? newton_total[inst] : 0, rej = 0;

@@ -48,6 +48,15 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
 // (mixed, split solution).
 const char* jit_ac_name(bool mixed, bool xsplit);
 
+// Source of the lane-per-instance LTE-controlled backward-Euler transient
+// kernel (jit_lte_name) for one AK_TRAN pattern shared by every instance:
+// the step controller, Newton loop, junction limiting and stamps of the
+// generic k_tran_lte with the plan written out as straight-line code.
+// Returns false, with the reason in `why`, when the per-lane state would not
+// fit in registers.
+bool jit_lte_source(const PatLayout& L, const int32_t* buf, std::string& src, std::string& why);
+const char* jit_lte_name();
+
 // Compiles `src` for the device's architecture and loads kernel `name`;
 // cached per (device, source).
 hipError_t jit_load(int device, const std::string& src, const char* name, JitFn& out, std::string& log);

@@ -34,7 +34,8 @@ struct BatchDev {
 struct Lay {
   int32_t nnz_max, n_max, ne_max;
   int32_t need_g, need_xp, need_ed;  // need_xp: number of extra FP64 n-vectors
-  int32_t fsz, asz, mixed, pad;
+  int32_t fsz, asz, mixed;
+  int32_t stage_prm;  // the system's element records (params, value table) copied into LDS
 };
 
 __host__ __device__ inline size_t lay_al(size_t b) { return (b + 15) & ~size_t(15); }
@@ -45,6 +46,7 @@ __host__ __device__ inline size_t sys_bytes(const Lay& L) {
   if (L.need_g) s += lay_al(8 * (size_t)L.n_max);
   s += (size_t)L.need_xp * lay_al(8 * (size_t)L.n_max);
   if (L.need_ed) s += lay_al(8 * (size_t)ESTRIDE * L.ne_max);
+  if (L.stage_prm) s += lay_al(8 * (size_t)(KG_NPARAM + 1) * L.ne_max);
   return s;
 }
 

@@ -166,6 +166,7 @@ class Context:
         if st != _lib.OK:
             raise KretsGpuError(st, "kg_create failed (no usable HIP device?)")
         self._h = h
+        self.device = device
         self.n_instances = 0
 
     def close(self):
@@ -278,9 +279,28 @@ class Context:
             return self._raise_or(r, st), st, err
         return self._result(r, x, (nsteps + 1,), st, extra=(it.reshape(self.n_instances, nsteps + 1), err))
 
-    def transient_lte(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points, config=None):
+    def transient_lte(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points, config=None, device=False):
+        """LTE-controlled backward Euler. Returns (t [ni, max_points], x
+        [ni, max_points, n], n_points, newton_total, rejected). device=True
+        leaves every output in device memory (torch tensors on the context's
+        GPU) instead of copying them to the host."""
         cfg = (config or SolverConfig(compat=False)).to_c()
         ni = self.n_instances
+        if device:
+            import torch
+            if not self.shared:
+                raise KretsGpuError(_lib.INVALID_ARGUMENT, "device=True needs a shared topology")
+            dev = torch.device("cuda", self.device)
+            t = torch.empty(ni * max_points, dtype=torch.float64, device=dev)
+            x = torch.empty(int(self.n_unknowns.sum()) * max_points, dtype=torch.float64, device=dev)
+            npts, nt, rej, st = (torch.empty(ni, dtype=torch.int32, device=dev) for _ in range(4))
+            r = lib().kg_transient_lte(self._h, ctypes.byref(cfg), tstop, h0, hmin, hmax, lte_reltol, lte_abstol,
+                                       max_points, ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                       ctypes.c_void_p(npts.data_ptr()), ctypes.c_void_p(nt.data_ptr()),
+                                       ctypes.c_void_p(rej.data_ptr()), ctypes.c_void_p(st.data_ptr()),
+                                       _lib.MEM_DEVICE)
+            self._raise_or(r, st.cpu().numpy())
+            return (t.view(ni, max_points), x.view(ni, max_points, int(self.n_unknowns[0])), npts, nt, rej)
         t = np.zeros(ni * max_points)
         x = np.zeros(int(self.n_unknowns.sum()) * max_points)
         npts = np.zeros(ni, np.int32)

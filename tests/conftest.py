@@ -49,3 +49,15 @@ def active_netlist(r1=10e3, r2=22e3, c1=4.7e-9, c2=1e-9, k=1.0, kf=2.0, gm=1e-3,
             f"R1 in a {r1!r}\nR2 a b {r2!r}\nC1 a out {c1!r}\nC2 b 0 {c2!r}\nE1 out 0 b 0 {k!r}\n"
             f"Rs out s 1k\nVsense s 0 0\nF1 0 f Vsense {kf!r}\nRf f 0 2k\nCf f 0 1n\n"
             f"G1 g 0 f 0 {gm!r}\nRg g 0 1k\nH1 h 0 Vsense {rh!r}\nRh h 0 1k\n")
+
+
+def time_aligned(tr, xr, t):
+    """The oracle's states moved to the times t of another run of the same
+    step controller, to first order: x_j + (x_j - x_{j-1}) / (t_j - t_{j-1})
+    * (t - t_j). The accepted times of two runs differ by rounding (the
+    controller amplifies it to ~1e-13 s); at 1 kHz and up to 5 V the source
+    node then moves by ~1e-8 V, which is a time offset, not a state error."""
+    out = xr.copy()
+    for j in range(1, len(tr)):
+        out[j] = xr[j] + (xr[j] - xr[j - 1]) / (tr[j] - tr[j - 1]) * (t[j] - tr[j])
+    return out

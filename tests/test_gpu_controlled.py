@@ -142,3 +142,21 @@ def test_controlled_adjoint(kg, oracle, batch, method, tps):
         assert abs(J[i] - Jr) <= 1e-11 * abs(Jr)
         assert np.max(np.abs(g[i] - gr)) < 1e-9 * np.max(np.abs(gr)), i
         assert (g[i][gains] == 0).all()
+
+
+@pytest.mark.parametrize("tps", [1, 16, 2001])  # 2001: lane kernel specialised at run time
+def test_controlled_lte(kg, oracle, batch, tps):
+    from conftest import time_aligned
+    nets = batch[0]
+    say(f"controlled sources: LTE transient tps={tps}")
+    ctx = context(kg, batch)
+    cfg = kg.SolverConfig(compat=False, voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12,
+                          threads_per_system=tps)
+    args = (100e-6, 1e-8, 1e-16, 5e-6, 1e-3, 1e-6)
+    t, xs, npts, nt, rej = ctx.transient_lte(*args, 2000, cfg)
+    ocfg = oracle.config(vabstol=1e-9, iabstol=1e-12)
+    for i in range(NI):
+        tr, xr, ntr, rejr = oracle.Circuit(nets[i]).tran_lte(*args, cfg=ocfg)
+        assert npts[i] == len(tr) and rej[i] == rejr and nt[i] == ntr, (i, npts[i], len(tr), rej[i], rejr)
+        xa = time_aligned(tr, xr, t[i][:npts[i]])
+        assert max(nerr(xs[i][j], xa[j]) for j in range(len(tr))) < 1e-9

@@ -20,7 +20,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import read_circuit
+from conftest import read_circuit, time_aligned
 
 HIPCC = "/opt/rocm/bin/hipcc"
 
@@ -245,3 +245,121 @@ def test_lane_source_controlled_sources_on_host(oracle, precision, refine):
         xr = oracle.Circuit(t).ac(f)
         e = max(nerr(x[i, j], xr[j]) for j in range(len(f)))
         assert e < (1e-12 if precision == 0 else 1e-9), (i, e)
+
+
+# ------------------------------------------------- LTE transient lane kernel
+LTE_SHIM = r"""
+#include <cmath>
+#define __device__
+#define __forceinline__ inline
+#define __global__
+#define __launch_bounds__(x)
+struct kg_dim3 { unsigned x; };
+static kg_dim3 threadIdx, blockIdx, gridDim;
+using std::exp; using std::log; using std::sin; using std::fmod; using std::sqrt;
+using std::fmin; using std::fmax; using std::fabs;
+"""
+
+LTE_DRIVER = r"""
+extern "C" void run_host(const double* prm, const double* eb, const double* xop, const long long* xop_off,
+                         double* tout, double* xout, const long long* xout_off, int* npts, int* ntot, int* rej,
+                         int* status, int ninst, double tstop, double h0, double hmin, double hmax, double lrel,
+                         double labs, int maxp, double reltol, double itol, double vtol, int maxiter) {
+  gridDim.x = (unsigned)((ninst + 63) / 64);
+  for (unsigned b = 0; b < gridDim.x; b++)
+    for (unsigned t = 0; t < 64; t++) {
+      blockIdx.x = b;
+      threadIdx.x = t;
+      kg_lte_lane(prm, eb, xop, xop_off, tout, xout, xout_off, npts, ntot, rej, status, ninst, tstop, h0, hmin,
+                  hmax, lrel, labs, maxp, reltol, itol, vtol, maxiter);
+    }
+}
+"""
+
+
+def lte_source(c, params, h0):
+    from krets_amd._lib import lib
+    buf = ctypes.create_string_buffer(1 << 22)
+    el = np.ascontiguousarray(c.elements)
+    pr = np.ascontiguousarray(params, np.float64)
+    r = lib().kg_jit_lte_source(el.ctypes.data, len(el), c.n, pr.ctypes.data, h0, buf, len(buf))
+    return r, buf.value.decode()
+
+
+def lte_host_run(src, params, elements, x0, args, maxp, cfg):
+    d = tempfile.mkdtemp()
+    try:
+        cpp, so = os.path.join(d, "k.cpp"), os.path.join(d, "k.so")
+        open(cpp, "w").write(LTE_SHIM + src + LTE_DRIVER)
+        subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", cpp, "-o", so], check=True,
+                       capture_output=True)
+        L = ctypes.CDLL(so)
+        ni, n = x0.shape
+        prm = np.ascontiguousarray(params, np.float64)
+        eb = np.ascontiguousarray(ebase(elements, params), np.float64)
+        xop = np.ascontiguousarray(x0, np.float64)
+        xoff = np.arange(ni, dtype=np.int64) * n
+        tout = np.zeros(ni * maxp)
+        xout = np.zeros(ni * maxp * n)
+        ooff = np.arange(ni, dtype=np.int64) * maxp * n
+        npts, ntot, rej, st = (np.zeros(ni, np.int32) for _ in range(4))
+        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        L.run_host.argtypes = [P] * 11 + [I, D, D, D, D, D, D, I, D, D, D, I]
+        ptrs = [a.ctypes.data for a in (prm, eb, xop, xoff, tout, xout, ooff, npts, ntot, rej, st)]
+        L.run_host(*ptrs, ni, *args, maxp, *cfg)
+        return tout.reshape(ni, maxp), xout.reshape(ni, maxp, n), npts, ntot, rej, st
+    finally:
+        shutil.rmtree(d)
+
+
+def test_lte_lane_source_on_host_vs_oracle(oracle):
+    """The generated LTE lane kernel, run lane by lane on the host, takes the
+    oracle's accepted time grid, rejection count and Newton iterations on the
+    half-wave rectifiers, and its states agree to 1e-9 once the oracle's are
+    moved to the same times (time_aligned)."""
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    b = W.c4_rectifier(6, seed=4)
+    c = kg.Circuit(b.netlists[0])
+    args = (5e-3, 1e-6, 1e-14, 5e-5, 1e-3, 1e-6)
+    r, src = lte_source(c, b.params[0], args[1])
+    assert r > 0, src
+    ocfg = oracle.config(vabstol=1e-9, iabstol=1e-12)
+    refs = [oracle.Circuit(b.netlists[i]).tran_lte(*args, cfg=ocfg) for i in range(6)]
+    x0 = np.stack([ref[1][0] for ref in refs])
+    t, x, npts, nt, rej, st = lte_host_run(src, b.params, c.elements, x0, args, 4000, (1e-3, 1e-12, 1e-9, 300))
+    assert (st == 0).all()
+    op_iters = set()
+    for i, (tr, xr, ntr, rejr) in enumerate(refs):
+        assert npts[i] == len(tr) and rej[i] == rejr, (i, npts[i], len(tr), rej[i], rejr)
+        op_iters.add(ntr - nt[i])  # the oracle's total includes its operating point
+        assert np.max(np.abs(t[i][:npts[i]] - tr)) <= 1e-9 * 5e-3
+        xa = time_aligned(tr, xr, t[i][:npts[i]])
+        assert np.array_equal(x[i][0], xr[0])  # the operating point is the input
+        assert max(nerr(x[i][j], xa[j]) for j in range(1, len(tr))) < 1e-9
+    assert len(op_iters) == 1 and 1 <= op_iters.pop() <= 5
+
+
+def test_lte_lane_source_builds_for_gfx950():
+    """The generated LTE kernel builds for gfx950 without scratch on the
+    rectifier and on the active filter of every controlled source."""
+    import krets_amd as kg
+    from conftest import active_netlist
+    from krets_amd import workloads as W
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    b = W.c4_rectifier(1, seed=4)
+    for c in (kg.Circuit(b.netlists[0]), kg.Circuit(active_netlist())):
+        r, src = lte_source(c, c.params, 1e-6)
+        assert r > 0, src
+        d = tempfile.mkdtemp()
+        try:
+            p = os.path.join(d, "k.hip")
+            open(p, "w").write("#include <hip/hip_runtime.h>\n" + src)
+            out = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                  "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", p, "-o", p + ".o",
+                                  "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
+            assert out.returncode == 0, out.stderr[-2000:]
+            assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
+        finally:
+            shutil.rmtree(d)
