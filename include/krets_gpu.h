@@ -252,6 +252,14 @@ int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t
                          double s_rep, int32_t precision, int32_t refine_iters, char* buf, int32_t cap);
 
 /* Diagnostic: the HIP source of the run-time specialised lane-per-instance
+ * kernel of kg_op / kg_dc_sweep for linear circuits (minimum-degree plan of
+ * the DC stamps, static pivots from params). No counterpart in krets.
+ * Returns the source length (copying at most cap bytes, NUL-terminated, to
+ * buf) or -status, with the reason in buf. */
+int32_t kg_jit_dc_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
+                         char* buf, int32_t cap);
+
+/* Diagnostic: the HIP source of the run-time specialised lane-per-instance
  * kernel of kg_transient_lte for one topology (minimum-degree plan of the
  * transient stamps, static pivots from params at step h0). No counterpart in
  * krets. Returns the source length (copying at most cap bytes,

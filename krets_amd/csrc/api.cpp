@@ -539,6 +539,61 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   Plan P0;
   int32_t r = prepare(c, AK_DC, ORD_NESTED, 0.0, P0);
   if (r) return r;
+  // linear circuits of a shared topology: one solve per point, in the lane
+  // kernel specialised to the topology at run time (2001), unless another
+  // mapping is asked for or the topology does not fit it
+  if (c->shared && !P0.any_nonlinear && P0.group_status[0] == KG_OK && cfg->maxiter >= 1 &&
+      (tps0 == 0 || tps0 == 2001)) {
+    Plan PL;
+    if ((r = prepare(c, AK_DC, ORD_MINDEG, 0.0, PL))) return r;
+    const int pidx = c->pat_index[std::make_tuple(0, (int)AK_DC, (int)ORD_MINDEG, 0.0)];
+    const auto key = std::make_tuple(pidx, 200, 0);
+    auto it = c->lane_fn.find(key);
+    JitFn fn;
+    std::string why;
+    if (it != c->lane_fn.end()) {
+      fn = it->second;
+    } else {
+      const PatEntry& pe = c->pats[pidx];
+      std::string src, log;
+      if (jit_dc_source(pe.lay, pe.host.data(), src, why)) {
+        if (jit_load(c->device, src, jit_dc_name(), fn, log) != hipSuccess)
+          return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
+      }
+      c->lane_fn[key] = fn;  // an empty entry records a topology the generator declines
+    }
+    if (fn.fn) {
+      auto off = out_offsets(c, npts);
+      if ((r = upload_offsets(c, "newton_off", off, d_off))) return r;
+      hipError_t e;
+      *d_out = (double*)ws_get(c, "newton_out", sizeof(double) * std::max<int64_t>(off.back(), 1), &e);
+      if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+      *d_status = (int32_t*)ws_get(c, "status", sizeof(int32_t) * c->n_inst, &e);
+      *d_iters = (int32_t*)ws_get(c, "newton_iters", sizeof(int32_t) * (size_t)c->n_inst * npts, &e);
+      if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+      const double* prm = c->d_params;
+      const double* eb = c->d_ebase;
+      int32_t *st = *d_status, *its = *d_iters;
+      double* o = *d_out;
+      int64_t* oo = *d_off;
+      hipStream_t strm = c->stream;
+      const int ni = c->n_inst;
+      const hipFunction_t f = fn.fn;
+      std::vector<Deferred> steps{{jit_dc_name(), [=]() mutable {
+                                     int a_ni = ni, a_se = sweep_e, a_np = npts;
+                                     double a_st = start, a_sp = step;
+                                     void* args[] = {&prm, &eb, &a_ni, &a_se, &a_st, &a_sp, &a_np, &o, &oo, &st, &its};
+                                     return hipModuleLaunchKernel(f, (unsigned)((ni + 63) / 64), 1, 1, 64, 1, 1, 0, strm,
+                                                                  args, nullptr);
+                                   }}};
+      if (deferred) {
+        deferred->insert(deferred->end(), steps.begin(), steps.end());
+        return KG_OK;
+      }
+      return timed(c, steps[0].name, steps[0].launch);
+    }
+    if (tps0 == 2001) return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
+  }
   int tps = tps0 ? tps0 : default_tps(c, P0.n_max);
   if (!valid_tps(tps)) return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64 or 256");
   r = prepare(c, AK_DC, order_for(tps), 0.0, P);
@@ -1497,6 +1552,20 @@ int32_t kg_jit_ac_source(const kg_element* el, int32_t ne, int32_t n, const doub
   if (L.status != KG_OK) return -L.status;
   // refine_iters < 0 selects the mixed precision form with the split solution
   if (!jit_ac_source(L, b.data(), precision == KG_MIXED, refine_iters < 0, src, why)) {
+    if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
+    return -KG_UNEXPECTED;
+  }
+  if (buf && cap > 0) std::snprintf(buf, cap, "%s", src.c_str());
+  return (int32_t)src.size();
+}
+
+int32_t kg_jit_dc_source(const kg_element* el, int32_t ne, int32_t n, const double* params, char* buf, int32_t cap) {
+  if (!el || ne <= 0 || n <= 0) return -KG_INVALID_ARGUMENT;
+  PatLayout L;
+  std::string err, src, why;
+  std::vector<int32_t> b = build_pattern(el, ne, n, AK_DC, ORD_MINDEG, params, 0.0, L, err);
+  if (L.status != KG_OK) return -L.status;
+  if (!jit_dc_source(L, b.data(), src, why)) {
     if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
     return -KG_UNEXPECTED;
   }

@@ -13,6 +13,7 @@
 // pivot reciprocal kept on the diagonal), so both produce the same accepted
 // time grid up to the compiler's contractions.
 #include <cstdio>
+#include <functional>
 #include <set>
 #include <sstream>
 #include <string>
@@ -85,6 +86,127 @@ std::string S(int i) { return std::to_string(i); }
 // Value of permuted unknown i in vector `pre` (ground: 0.0), as the generic
 // kernels write it: (p >= 0 ? x[p] : 0.0).
 std::string node(const std::string& pre, int i) { return i >= 0 ? pre + S(i) : std::string("0.0"); }
+
+// Value of a stamp contribution (the plan's gather codes) and of a
+// right-hand-side contribution, as expressions over the kernel's names.
+using CodeFn = std::string (*)(const TV& P, int e, int k);
+
+// Emits, row by row in pivot order: the row's matrix slots v<s>
+// (contributions in the plan's order, summed from zero as the generic
+// kernels do), its right-hand side x<i>, its factorisation (the plan's factor
+// program: multipliers l_ik = a_ik * (1 / u_kk) stored in place, the pivot
+// reciprocal on the diagonal, `bad` set for a zero or non-finite pivot) and
+// its forward substitution; then the back substitution. Each row only reads
+// the upper factor of earlier rows, so a row's multipliers die with it and
+// the arithmetic is that of factorising everything first (the same
+// operations in the same order). before_row(i) may emit loads first.
+bool emit_solve(std::ostringstream& o, const TV& P, const char* ind, CodeFn slot, CodeFn exc, std::string& why,
+                const std::function<void(int)>& before_row = nullptr) {
+  const int n = P.n;
+  for (int i = 0; i < n; i++) {
+    if (before_row) before_row(i);
+    for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
+      o << ind << "double v" << s << " = 0.0";
+      for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) {
+        const std::string c = slot(P, P.scon[q] >> 3, P.scon[q] & 7);
+        if (c.empty()) {
+          why = "unknown stamp kind";
+          return false;
+        }
+        o << " + (" << c << ")";
+      }
+      o << ";\n";
+    }
+    o << ind << "x" << i << " = 0.0";
+    for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) {
+      const std::string c = exc(P, P.rcon[q] >> 3, P.rcon[q] & 7);
+      if (c.empty()) {
+        why = "unknown excitation kind";
+        return false;
+      }
+      o << " + (" << c << ")";
+    }
+    o << ";\n";
+    const int32_t* fp = P.fprog + P.fprog_ptr[i];
+    const int nk = *fp++;
+    for (int kk = 0; kk < nk; kk++) {
+      const int sik = fp[0], sdk = fp[1], nu = fp[2];
+      fp += 3;
+      if (sik < P.row_ptr[i] || sik >= P.diag[i]) {
+        why = "factor program leaves the row";
+        return false;
+      }
+      o << ind << "v" << sik << " = v" << sik << " * v" << sdk << ";\n";
+      for (int u = 0; u < nu; u++, fp += 2) {
+        if (fp[0] < P.row_ptr[i] || fp[0] >= P.row_ptr[i + 1]) {
+          why = "factor program leaves the row";
+          return false;
+        }
+        o << ind << "v" << fp[0] << " = v" << fp[0] << " - v" << sik << " * v" << fp[1] << ";\n";
+      }
+    }
+    const int d = P.diag[i];
+    o << ind << "bad |= !(fabs(v" << d << ") > 0.0) || !__builtin_isfinite(v" << d << ");\n"
+      << ind << "v" << d << " = 1.0 / v" << d << ";\n";
+    for (int p = P.row_ptr[i]; p < d; p++) o << ind << "x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    const int d = P.diag[i];
+    for (int p = d + 1; p < P.row_ptr[i + 1]; p++) o << ind << "x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
+    o << ind << "x" << i << " = x" << i << " * v" << d << ";\n";
+  }
+  return true;
+}
+
+// Backward-Euler transient stamps (stampable.rs, the generic kernels' elem_y
+// / elem_z / rhs_contrib at alpha = 1): names of the LTE kernel.
+std::string lte_slot(const TV& P, int e, int k) {
+  const int ty = P.etype[e];
+  switch (k) {
+    case CK_YP:
+    case CK_YN: {
+      std::string c = ty == KG_R ? "g" + S(e) : ty == KG_C ? "ch" + S(e) : ty == KG_D ? "gd" + S(e) : "0.0";
+      return k == CK_YN ? "-" + c : c;
+    }
+    case CK_ONEP: return "1.0";
+    case CK_ONEN: return "-1.0";
+    case CK_BR: return ty == KG_L ? "-(lh" + S(e) + ")" : ty == KG_R ? "-r" + S(e) : "1.0";
+    case CK_KP: return "g" + S(e);
+    case CK_KN: return "-g" + S(e);
+    default: return "";
+  }
+}
+std::string lte_exc(const TV& P, int e, int k) {
+  switch (k) {
+    case RK_SRC: return P.etype[e] == KG_V ? "vs" + S(e) : "p" + S(e);
+    case RK_DP: return "-id" + S(e);
+    case RK_DM: return "id" + S(e);
+    case RK_CP: return "-cq" + S(e);
+    case RK_CM: return "cq" + S(e);
+    case RK_IND: return "lq" + S(e);
+    default: return "";
+  }
+}
+
+// DC stamps (op.rs, dc.rs): capacitors are open, inductors shorts; a swept
+// source takes the sweep value.
+std::string dc_slot(const TV& P, int e, int k) {
+  const int ty = P.etype[e];
+  switch (k) {
+    case CK_YP: return ty == KG_R ? "g" + S(e) : "";
+    case CK_YN: return ty == KG_R ? "-g" + S(e) : "";
+    case CK_ONEP: return "1.0";
+    case CK_ONEN: return "-1.0";
+    case CK_BR: return ty == KG_R ? "-r" + S(e) : "";
+    case CK_KP: return "g" + S(e);
+    case CK_KN: return "-g" + S(e);
+    default: return "";
+  }
+}
+std::string dc_exc(const TV& P, int e, int k) {
+  if (k != RK_SRC) return "";
+  return "(sweep_e == " + S(e) + " ? sv : p" + S(e) + ")";
+}
 
 }  // namespace
 
@@ -193,77 +315,7 @@ bool jit_lte_source(const PatLayout& L, const int32_t* b, std::string& src, std:
       << "        id" << E << " = is" << E << " * (ex - 1.0) - gd" << E << " * v;\n"
       << "        vo" << E << " = v;\n      }\n";
   }
-  // matrix slots: contributions in the plan's order, summed from zero
-  for (int s = 0; s < P.nnz; s++) {
-    o << "      double v" << s << " = 0.0";
-    for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) {
-      const int code = P.scon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
-      std::string c;
-      switch (k) {
-        case CK_YP:
-        case CK_YN: {
-          if (ty == KG_R) c = "g" + S(e);
-          else if (ty == KG_C) c = "ch" + S(e);
-          else if (ty == KG_D) c = "gd" + S(e);
-          else c = "0.0";
-          if (k == CK_YN) c = "-" + c;
-          break;
-        }
-        case CK_ONEP: c = "1.0"; break;
-        case CK_ONEN: c = "-1.0"; break;
-        case CK_BR:
-          if (ty == KG_L) c = "-(lh" + S(e) + ")";
-          else if (ty == KG_R) c = "-r" + S(e);
-          else c = "1.0";
-          break;
-        case CK_KP: c = "g" + S(e); break;
-        case CK_KN: c = "-g" + S(e); break;
-        default: why = "unknown stamp kind"; return false;
-      }
-      o << " + (" << c << ")";
-    }
-    o << ";\n";
-  }
-  // right-hand side (permuted rows), then the solve in place (x = b)
-  for (int i = 0; i < n; i++) {
-    o << "      x" << i << " = 0.0";
-    for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) {
-      const int code = P.rcon[q], e = code >> 3, k = code & 7, ty = P.etype[e];
-      std::string c;
-      switch (k) {
-        case RK_SRC: c = ty == KG_V ? "vs" + S(e) : "p" + S(e); break;
-        case RK_DP: c = "-id" + S(e); break;
-        case RK_DM: c = "id" + S(e); break;
-        case RK_CP: c = "-cq" + S(e); break;
-        case RK_CM: c = "cq" + S(e); break;
-        case RK_IND: c = "lq" + S(e); break;
-        default: why = "unknown excitation kind"; return false;
-      }
-      o << " + (" << c << ")";
-    }
-    o << ";\n";
-  }
-  // factorisation, row by row in pivot order (the plan's factor program)
-  for (int i = 0; i < n; i++) {
-    const int32_t* fp = P.fprog + P.fprog_ptr[i];
-    const int nk = *fp++;
-    for (int kk = 0; kk < nk; kk++) {
-      const int sik = fp[0], sdk = fp[1], nu = fp[2];
-      fp += 3;
-      o << "      v" << sik << " = v" << sik << " * v" << sdk << ";\n";
-      for (int u = 0; u < nu; u++, fp += 2) o << "      v" << fp[0] << " = v" << fp[0] << " - v" << sik << " * v" << fp[1] << ";\n";
-    }
-    const int d = P.diag[i];
-    o << "      bad |= !(fabs(v" << d << ") > 0.0) || !__builtin_isfinite(v" << d << ");\n"
-      << "      v" << d << " = 1.0 / v" << d << ";\n";
-  }
-  for (int i = 0; i < n; i++)
-    for (int p = P.row_ptr[i]; p < P.diag[i]; p++) o << "      x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
-  for (int i = n - 1; i >= 0; i--) {
-    const int d = P.diag[i];
-    for (int p = d + 1; p < P.row_ptr[i + 1]; p++) o << "      x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
-    o << "      x" << i << " = x" << i << " * v" << d << ";\n";
-  }
+  if (!emit_solve(o, P, "      ", lte_slot, lte_exc, why)) return false;
   // convergence (the clean-mode test of the generic kernels)
   o << "      bool cv = true;\n";
   for (int i = 0; i < n; i++)
@@ -319,6 +371,91 @@ bool jit_lte_source(const PatLayout& L, const int32_t* b, std::string& src, std:
        "  }\n"
        "  }\n"
        "  status[inst] = st;\n  npts[inst] = cnt;\n  ntot[inst] = nt;\n  rejected[inst] = rej;\n}\n";
+  src = o.str();
+  return true;
+}
+
+}  // namespace kg
+
+namespace kg {
+
+const char* jit_dc_name() { return "kg_dc_lane"; }
+
+// Linear operating points and DC sweeps (op.rs:15-86, dc.rs:14-124 for
+// circuits without diodes: one solve per point, which krets' Newton loop
+// accepts after its first pass): one lane per instance, its points in order.
+bool jit_dc_source(const PatLayout& L, const int32_t* b, std::string& src, std::string& why) {
+  if (L.status != KG_OK) {
+    why = "structurally singular pattern";
+    return false;
+  }
+  if (L.nonlinear) {
+    why = "nonlinear circuits take the Newton kernels";
+    return false;
+  }
+  TV P = tview(L, b);
+  const int n = P.n, ne = P.ne;
+  // registers live across the solve: the upper factor and pivots, x, and the
+  // element values loaded ahead of their rows
+  int nu = 0;
+  for (int i = 0; i < n; i++) nu += P.row_ptr[i + 1] - P.diag[i];
+  if (2 * (nu + n) > 440) {
+    why = "per-lane state exceeds the register budget";
+    return false;
+  }
+  // element values each row reads, loaded `kAhead` rows before it (an empty
+  // asm statement with a memory clobber keeps the compiler from hoisting
+  // every load to the start)
+  constexpr int kAhead = 8;
+  std::vector<std::vector<std::pair<std::string, std::string>>> need(n);
+  for (int i = 0; i < n; i++) {
+    auto add = [&](const std::string& nm, const std::string& ld) { need[i].push_back({nm, ld}); };
+    for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++)
+      for (int q = P.scon_ptr[s]; q < P.scon_ptr[s + 1]; q++) {
+        const int e = P.scon[q] >> 3, k = P.scon[q] & 7;
+        if (k == CK_YP || k == CK_YN || k == CK_KP || k == CK_KN) add("g" + S(e), "EB[" + S(e) + "]");
+        if (k == CK_BR && P.etype[e] == KG_R) add("r" + S(e), "PR[" + S(e * KG_NPARAM) + "]");
+      }
+    for (int q = P.rcon_ptr[i]; q < P.rcon_ptr[i + 1]; q++) {
+      const int e = P.rcon[q] >> 3;
+      add("p" + S(e), "PR[" + S(e * KG_NPARAM) + "]");
+    }
+  }
+  std::ostringstream o;
+  std::set<std::string> have;
+  auto issue = [&](int r) {
+    if (r >= n) return;
+    bool any = false;
+    for (auto& v : need[r])
+      if (have.insert(v.first).second) {
+        o << "    const double " << v.first << " = " << v.second << ";\n";
+        any = true;
+      }
+    if (any) o << "    asm volatile(\"\" ::: \"memory\");\n";
+  };
+  o << "typedef long long i64;\n"
+    << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_dc_name()
+    << "(const double* __restrict__ prm,\n"
+       "    const double* __restrict__ ebase, int ninst, int sweep_e, double start, double step, int npts,\n"
+       "    double* __restrict__ out, const i64* __restrict__ out_off, int* __restrict__ status,\n"
+       "    int* __restrict__ iters) {\n"
+       "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
+       "  if (inst >= ninst) return;\n"
+    << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
+    << "  const double* EB = ebase + (i64)inst * " << ne << ";\n"
+    << "  int st = 0;\n";
+  for (int i = 0; i < n; i++) o << "  double x" << i << ";\n";
+  o << "  for (int pt = 0; pt < npts && st == 0; pt++) {\n"
+       "    const double sv = start + (double)pt * step;\n"
+       "    bool bad = false;\n";
+  for (int r = 0; r < kAhead; r++) issue(r);
+  if (!emit_solve(o, P, "    ", dc_slot, dc_exc, why, [&](int i) { issue(i + kAhead); })) return false;
+  o << "    if (bad) st = " << (int)KG_DECOMPOSITION << ";\n"
+    << "    double* o = out + out_off[inst] + (i64)pt * " << n << ";\n";
+  for (int i = 0; i < n; i++) o << "    o[" << P.col_of[i] << "] = x" << i << ";\n";
+  o << "    if (iters) iters[(i64)inst * npts + pt] = 1;\n"
+       "  }\n"
+       "  status[inst] = st;\n}\n";
   src = o.str();
   return true;
 }

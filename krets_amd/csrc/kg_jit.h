@@ -57,6 +57,12 @@ const char* jit_ac_name(bool mixed, bool xsplit);
 bool jit_lte_source(const PatLayout& L, const int32_t* buf, std::string& src, std::string& why);
 const char* jit_lte_name();
 
+// Source of the lane-per-instance kernel for linear operating points and DC
+// sweeps (jit_dc_name) for one AK_DC pattern shared by every instance: one
+// straight-line solve per point. Declines nonlinear circuits.
+bool jit_dc_source(const PatLayout& L, const int32_t* buf, std::string& src, std::string& why);
+const char* jit_dc_name();
+
 // Compiles `src` for the device's architecture and loads kernel `name`;
 // cached per (device, source).
 hipError_t jit_load(int device, const std::string& src, const char* name, JitFn& out, std::string& log);

@@ -110,7 +110,8 @@ def host_run(src, params, elements, freqs, n, refine=2, stop=0.0):
 
 
 def nerr(x, r):
-    return np.linalg.norm(x - r) / np.linalg.norm(r)
+    d = np.linalg.norm(r)
+    return np.linalg.norm(x - r) / (d if d > 0 else 1.0)
 
 
 @pytest.mark.parametrize("xsplit", [False, True])
@@ -363,3 +364,115 @@ def test_lte_lane_source_builds_for_gfx950():
             assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
         finally:
             shutil.rmtree(d)
+
+
+# ------------------------------------------------- linear OP / DC lane kernel
+DC_DRIVER = r"""
+extern "C" void run_host(const double* prm, const double* eb, int ninst, int sweep_e, double start, double step,
+                         int npts, double* out, const long long* off, int* status, int* iters) {
+  gridDim.x = (unsigned)((ninst + 63) / 64);
+  for (unsigned b = 0; b < gridDim.x; b++)
+    for (unsigned t = 0; t < 64; t++) {
+      blockIdx.x = b;
+      threadIdx.x = t;
+      kg_dc_lane(prm, eb, ninst, sweep_e, start, step, npts, out, off, status, iters);
+    }
+}
+"""
+
+
+def dc_source(c, params):
+    from krets_amd._lib import lib
+    buf = ctypes.create_string_buffer(1 << 22)
+    el = np.ascontiguousarray(c.elements)
+    pr = np.ascontiguousarray(params, np.float64)
+    r = lib().kg_jit_dc_source(el.ctypes.data, len(el), c.n, pr.ctypes.data, buf, len(buf))
+    return r, buf.value.decode()
+
+
+def dc_host_run(src, params, elements, n, sweep_e, start, step, npts):
+    d = tempfile.mkdtemp()
+    try:
+        cpp, so = os.path.join(d, "k.cpp"), os.path.join(d, "k.so")
+        shim = LTE_SHIM.replace("#include <cmath>", "#include <cmath>\n#define asm(...)\n#define volatile")
+        open(cpp, "w").write(shim + src + DC_DRIVER)
+        subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", cpp, "-o", so], check=True,
+                       capture_output=True)
+        L = ctypes.CDLL(so)
+        ni = params.shape[0]
+        prm = np.ascontiguousarray(params, np.float64)
+        eb = np.ascontiguousarray(ebase(elements, params), np.float64)
+        out = np.zeros(ni * npts * n)
+        off = np.arange(ni, dtype=np.int64) * npts * n
+        st, it = np.zeros(ni, np.int32), np.zeros(ni * npts, np.int32)
+        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        L.run_host.argtypes = [P, P, I, I, D, D, I, P, P, P, P]
+        L.run_host(prm.ctypes.data, eb.ctypes.data, ni, sweep_e, start, step, npts, out.ctypes.data, off.ctypes.data,
+                   st.ctypes.data, it.ctypes.data)
+        return out.reshape(ni, npts, n), st, it.reshape(ni, npts)
+    finally:
+        shutil.rmtree(d)
+
+
+def test_dc_lane_source_on_host_vs_oracle(oracle):
+    """The generated linear OP / DC-sweep kernel, run on the host: the C2 RC
+    ladders' operating points, a DC sweep of the voltage divider and of the
+    active filter of every controlled source, against the oracle (1e-12)."""
+    import krets_amd as kg
+    from conftest import active_netlist
+    from krets_amd import workloads as W
+    b = W.c2_rc_ladder_mc(4, sections=50, seed=3)
+    c = kg.Circuit(b.netlists[0])
+    r, src = dc_source(c, b.params[0])
+    assert r > 0, src
+    params = b.params.copy()
+    params[:, 0, 0] = 1.0 + np.arange(4)  # V1's DC value (the sweep's is 0)
+    x, st, it = dc_host_run(src, params, c.elements, c.n, -1, 0.0, 0.0, 1)
+    assert (st == 0).all() and (it == 1).all()
+    for i in range(4):
+        oc = oracle.Circuit(b.netlists[i])
+        oc.set_value(0, 1.0 + i)
+        assert nerr(x[i][0], oc.op()[0]) < 1e-12
+    for text in (read_circuit("voltage_divider/voltage_divider.cir"), active_netlist()):
+        c = kg.Circuit(text)
+        r, src = dc_source(c, c.params)
+        assert r > 0, src
+        x, st, _ = dc_host_run(src, c.params[None], c.elements, c.n, 0, -1.0, 0.25, 9)
+        xr = oracle.Circuit(text).dc("V1", -1.0, 1.0, 0.25)
+        assert st[0] == 0 and max(nerr(x[0][p], xr[p]) for p in range(9)) < 1e-12
+
+
+def test_dc_lane_source_declines_and_flags():
+    import krets_amd as kg
+    c = kg.Circuit(read_circuit("rectifier/rectifier.cir"))
+    r, why = dc_source(c, c.params)
+    assert r < 0 and "nonlinear" in why
+    # a zero resistor's conductance is infinite: a non-finite pivot (Decomposition)
+    c = kg.Circuit("V1 1 0 1\nR1 1 2 0\nR2 2 0 1k\n")
+    r, src = dc_source(c, c.params)
+    assert r > 0, src
+    _, st, _ = dc_host_run(src, c.params[None], c.elements, c.n, -1, 0.0, 0.0, 1)
+    assert st[0] == 2
+
+
+def test_dc_lane_source_builds_for_gfx950():
+    """The 50-section ladder's operating-point kernel builds for gfx950 without scratch."""
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    b = W.c2_rc_ladder_mc(1, sections=50, seed=2)
+    c = kg.Circuit(b.netlists[0])
+    r, src = dc_source(c, c.params)
+    assert r > 0, src
+    d = tempfile.mkdtemp()
+    try:
+        p = os.path.join(d, "k.hip")
+        open(p, "w").write("#include <hip/hip_runtime.h>\n" + src)
+        out = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                              "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", p, "-o", p + ".o",
+                              "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
+        assert out.returncode == 0, out.stderr[-2000:]
+        assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
+    finally:
+        shutil.rmtree(d)
