@@ -237,12 +237,17 @@ def c5(args):
                       "backward_over_forward_wall": dtb / dtf,
                       "J_mean": float(np.mean(J)), "J_zero_count": int((J == 0).sum())}
             if R and key == "fp64":
-                errs = []
+                # relative error over the gradient entries in the normal FP64 range: where J
+                # underflows, the gradient lives in subnormals (a few significant bits each)
+                errs, sub = [], 0
                 for i in (0, 255):
                     Jr, gr = R.Circuit(b.netlists[i]).adjoint(h, N, c, method=R.TRAP)
-                    if np.max(np.abs(gr)) > 0:
-                        errs.append(float(np.max(np.abs(g[i] - gr)) / np.max(np.abs(gr))))
+                    norm = np.abs(gr) >= np.finfo(np.float64).tiny
+                    sub += int(((gr != 0) & ~norm).sum())
+                    if norm.any():
+                        errs.append(float(np.max(np.abs(g[i] - gr)[norm]) / np.max(np.abs(gr[norm]))))
                 r[key]["grad_err_vs_oracle_fp64"] = max(errs) if errs else None
+                r[key]["grad_entries_subnormal_in_oracle"] = sub
             if key == "mixed":
                 r[key]["grad_rel_diff_vs_fp64_gpu"] = float(np.max(np.abs(g - r["_g"])) / max(np.max(np.abs(r["_g"])), 1e-300))
             else:

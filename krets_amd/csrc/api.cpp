@@ -1780,12 +1780,16 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
                            srcv, nsrc_max);
   };
   int tps = cfg->threads_per_system;
-  if (tps == 0 && ladj > 0) {  // autotune the forward pass over the feasible mappings
+  if (tps == 0) {  // autotune the forward pass over the feasible mappings, in the requested precision
     d_st0 = (int32_t*)ws_get(c, "adj_status0", sizeof(int32_t) * ni, &e);
     if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint)");
     HIPCHK(c, hipMemcpyAsync(d_st0, d_st, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
     const int tdef = default_tps(c, P0.n_max);
-    int32_t rr = autotune(c, tune_key(c, "adjf", precision, P0, c->n_inst), {tdef, 3064, 3256, 3512},
+    std::vector<int> cands{tdef};
+    for (int t : {16, 64, 256})
+      if (t != tdef) cands.push_back(t);
+    if (!mixed && ladj > 0) cands.insert(cands.end(), {3064, 3256, 3512});
+    int32_t rr = autotune(c, tune_key(c, "adjf", precision, P0, c->n_inst), cands,
                           [&](int cand, float* ms) -> int32_t {
       if (is_lin_cand(cand) && !lin_feasible(cand)) return set_err(c, KG_UNEXPECTED, "infeasible");
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * ni, hipMemcpyDeviceToDevice, c->stream));
@@ -1793,15 +1797,15 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
       Plan PP;
       int32_t r2 = prepare(c, AK_TRAN, order_for(cand), s_tr, PP);
       if (r2) return r2;
-      Lay ly{PP.nnz_max, PP.n_max, PP.ne_max, 0, 5, 1, 8, 8, 0, 0};
+      Lay ly{PP.nnz_max, PP.n_max, PP.ne_max, 0, 5, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
       Launch LL;
       int pi2 = (cand == 64 || cand == 256) ? PP.total_max : 0;
       if (pi2 && configure(c, cand, ly, c->n_inst, LL, lay_al(4 * (size_t)pi2)) != KG_OK) pi2 = 0;
       if (!pi2 && (r2 = configure(c, cand, ly, c->n_inst, LL))) return r2;
       BatchDev BB = batch_dev(c, PP);
       return time_launch(c, [&] {
-        return run_adj_fwd(cand, 0, LL.grid, LL.block, LL.lds, c->stream, BB, 0, aa, d_c, d_xop, d_xoff, d_ck,
-                           (int64_t)ncp * cpw, cpw, d_J, d_st, ly, pi2);
+        return run_adj_fwd(cand, mixed ? 1 : 0, LL.grid, LL.block, LL.lds, c->stream, BB, 0, aa, d_c, d_xop, d_xoff,
+                           d_ck, (int64_t)ncp * cpw, cpw, d_J, d_st, ly, pi2);
       }, ms);
     }, &tps);
     if (rr) return rr;
