@@ -209,10 +209,13 @@ def c4(args):
         res["worst_points"] = where
     # fixed-step BE-compat parity run at h = 1 us
     ccfg = kg.SolverConfig(voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12)
+    ctx.transient(1e-6, 5000, method=kg.BE_COMPAT, config=ccfg)  # autotune + warm
     dt2, kms2, kname2, (xs2, st2, (it2, _)) = _timed(ctx, lambda: ctx.transient(1e-6, 5000, method=kg.BE_COMPAT,
                                                                                 config=ccfg), 1)
-    res["be_compat_h1us"] = {"s": dt2, "newton_iterations_total": int(it2.sum()),
-                             "solves_per_s": float(it2.sum()) / dt2, "all_converged": bool((st2 == 0).all())}
+    res["be_compat_h1us"] = {"s": dt2, "kernel": kname2, "avg_kernel_ms": kms2,
+                             "newton_iterations_total": int(it2.sum()),
+                             "solves_per_s": float(it2.sum()) / dt2, "all_converged": bool((st2 == 0).all()),
+                             "autotune_ms_by_mapping": ctx.last_autotune()}
     if R:
         xr, itr = R.Circuit(b.netlists[3]).tran(1e-6, 5e-3, cfg=R.config(vabstol=1e-9, iabstol=1e-12))
         e2, w2 = _werr(xs2[3], xr)

@@ -117,7 +117,10 @@ typedef struct {
    * into LDS once per launch) and, for FP64 on circuits without diodes,
    * 3000 + t, t = 64, 128, 256, 512, 1024 (one system per block of t threads with
    * the whole step loop in LDS: compact 16-bit plan, factors, solutions and
-   * companion states). */
+   * companion states). For FP64 on a shared topology, fixed-step and LTE
+   * transients, operating points and DC sweeps (linear circuits) also take
+   * 2001: one lane per instance, code specialised to the topology at run
+   * time (registers only; small circuits). */
   int32_t threads_per_system;
   /* KG_MIXED: refinement steps. 0 = default: 2, except that the lane AC
    * kernels (2002 / 2012) stop after the first pass where every lane of a
@@ -258,6 +261,15 @@ int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t
  * buf) or -status, with the reason in buf. */
 int32_t kg_jit_dc_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
                          char* buf, int32_t cap);
+
+/* Diagnostic: the HIP source of the run-time specialised lane-per-instance
+ * kernel of kg_transient (FP64; minimum-degree plan of the transient stamps,
+ * static pivots from params at step h with the method's companion scale; the
+ * method itself is a launch argument). No counterpart in krets. Returns the
+ * source length (copying at most cap bytes, NUL-terminated, to buf) or
+ * -status, with the reason in buf. */
+int32_t kg_jit_tran_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
+                           double h, int32_t method, char* buf, int32_t cap);
 
 /* Diagnostic: the HIP source of the run-time specialised lane-per-instance
  * kernel of kg_transient_lte for one topology (minimum-degree plan of the

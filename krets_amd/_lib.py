@@ -118,6 +118,7 @@ def lib():
         L.kg_jit_ac_source.argtypes = [P, I32, I32, P, D, I32, I32, ctypes.c_char_p, I32]
         L.kg_jit_lte_source.argtypes = [P, I32, I32, P, D, ctypes.c_char_p, I32]
         L.kg_jit_dc_source.argtypes = [P, I32, I32, P, ctypes.c_char_p, I32]
+        L.kg_jit_tran_source.argtypes = [P, I32, I32, P, D, I32, ctypes.c_char_p, I32]
         L.kg_parse_netlist.argtypes = [C, ctypes.POINTER(P), ctypes.c_char_p, I32]
         L.kg_circuit_free.argtypes = [P]
         L.kg_circuit_size.argtypes = [P, PI, PI]

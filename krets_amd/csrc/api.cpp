@@ -1425,11 +1425,54 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
     return run_tran_lin(cand - 3000, rows, dim3((unsigned)c->n_inst), lin, st, batch_dev(c, PP), 0, ta, d_xop,
                         d_xoff, d_out, d_off, d_st, d_iters, d_err, srcv, nsrc_max);
   };
+  // 2001: one lane per instance, code specialised to the shared topology at
+  // run time (FP64; diodes included)
+  JitFn tran_fn;
+  auto lane_setup = [&](Plan& PP) -> int32_t {
+    if (!c->shared || mixed) return set_err(c, KG_UNEXPECTED, "the lane kernel needs FP64 and a shared topology");
+    int32_t rr = prepare(c, AK_TRAN, ORD_MINDEG, s_tr, PP);
+    if (rr) return rr;
+    if (PP.group_status[0] != KG_OK) return set_err(c, KG_UNEXPECTED, "lane kernel: no valid plan");
+    const int pidx = c->pat_index[std::make_tuple(0, (int)AK_TRAN, (int)ORD_MINDEG, s_tr)];
+    const auto key = std::make_tuple(pidx, 300, 0);
+    auto it = c->lane_fn.find(key);
+    if (it != c->lane_fn.end()) {
+      tran_fn = it->second;
+      return tran_fn.fn ? KG_OK : set_err(c, KG_UNEXPECTED, "lane kernel: topology not specialised");
+    }
+    const PatEntry& pe = c->pats[pidx];
+    std::string src, why, log;
+    if (!jit_tran_source(pe.lay, pe.host.data(), src, why)) {
+      c->lane_fn[key] = JitFn{};
+      return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
+    }
+    if (jit_load(c->device, src, jit_tran_name(), tran_fn, log) != hipSuccess)
+      return set_err(c, KG_UNEXPECTED, "lane kernel compilation failed: " + log);
+    c->lane_fn[key] = tran_fn;
+    return KG_OK;
+  };
+  auto lane_launch = [&]() -> hipError_t {
+    const double* prm = c->d_params;
+    const double* eb = c->d_ebase;
+    int a_ni = c->n_inst, a_ns = nsteps, a_m = method, a_mi = na.maxiter;
+    double a_h = h, a_al = ta.alpha, a_rt = na.reltol, a_it = na.iabstol, a_vt = na.vabstol;
+    void* args[] = {&prm, &eb, &d_xop, &d_xoff, &d_out, &d_off, &d_st, &d_iters, &d_err,
+                    &a_ni, &a_h, &a_al, &a_ns, &a_m, &a_rt, &a_it, &a_vt, &a_mi};
+    return hipModuleLaunchKernel(tran_fn.fn, (unsigned)((c->n_inst + 63) / 64), 1, 1, 64, 1, 1, 0, c->stream, args,
+                                 nullptr);
+  };
   if (cand == 0) {
     std::vector<int> cands = {1, 4, 16, 64, 256, 1064, 1256};
     cands.insert(cands.end(), LIN_CANDS.begin(), LIN_CANDS.end());
+    if (c->shared && !mixed) cands.push_back(2001);
     r = autotune(c, tune_key(c, "tran", precision, P0, c->n_inst), cands, [&](int t, float* ms) -> int32_t {
       Plan PP;
+      if (t == 2001) {
+        int32_t rr = lane_setup(PP);
+        if (rr) return rr;
+        HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
+        return time_launch(c, [&] { return lane_launch(); }, ms);
+      }
       if (is_lin_cand(t)) {
         int32_t rr = lin_setup(PP);
         if (rr) return rr;
@@ -1453,7 +1496,11 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
   }
-  if (is_lin_cand(cand)) {
+  if (cand == 2001) {
+    if ((r = lane_setup(P))) return r == KG_UNEXPECTED ? set_err(c, KG_INVALID_ARGUMENT, c->err) : r;
+    r = timed(c, jit_tran_name(), [&] { return lane_launch(); });
+    if (r) return r;
+  } else if (is_lin_cand(cand)) {
     if ((r = lin_setup(P))) return r == KG_UNEXPECTED ? set_err(c, KG_INVALID_ARGUMENT, c->err) : r;
     if (lin_rows(cand - 3000, P.n_max) == 0)
       return set_err(c, KG_INVALID_ARGUMENT, "system too large for this block size (more than 4 rows per thread)");
@@ -1462,7 +1509,7 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   } else {
   if (!valid_tps(cand % 1000) || (cand >= 1000 && cand != 1064 && cand != 1256))
     return set_err(c, KG_INVALID_ARGUMENT,
-                   "threads_per_system must be 1, 4, 16, 64, 256, 1064, 1256, 3064, 3128, 3256, 3512 or 4024");
+                   "threads_per_system must be 1, 4, 16, 64, 256, 1064, 1256, 2001, 3064, 3128, 3256, 3512 or 4024");
   Lay lay;
   Launch L;
   int pi = 0;
@@ -1566,6 +1613,22 @@ int32_t kg_jit_dc_source(const kg_element* el, int32_t ne, int32_t n, const doub
   std::vector<int32_t> b = build_pattern(el, ne, n, AK_DC, ORD_MINDEG, params, 0.0, L, err);
   if (L.status != KG_OK) return -L.status;
   if (!jit_dc_source(L, b.data(), src, why)) {
+    if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
+    return -KG_UNEXPECTED;
+  }
+  if (buf && cap > 0) std::snprintf(buf, cap, "%s", src.c_str());
+  return (int32_t)src.size();
+}
+
+int32_t kg_jit_tran_source(const kg_element* el, int32_t ne, int32_t n, const double* params, double h,
+                           int32_t method, char* buf, int32_t cap) {
+  if (!el || ne <= 0 || n <= 0 || !(h > 0)) return -KG_INVALID_ARGUMENT;
+  PatLayout L;
+  std::string err, src, why;
+  std::vector<int32_t> b =
+      build_pattern(el, ne, n, AK_TRAN, ORD_MINDEG, params, (method == KG_TRAP ? 2.0 : 1.0) / h, L, err);
+  if (L.status != KG_OK) return -L.status;
+  if (!jit_tran_source(L, b.data(), src, why)) {
     if (buf && cap > 0) std::snprintf(buf, cap, "%s", why.c_str());
     return -KG_UNEXPECTED;
   }

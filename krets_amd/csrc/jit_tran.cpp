@@ -268,9 +268,9 @@ bool jit_lte_source(const PatLayout& L, const int32_t* b, std::string& src, std:
     }
   }
   // initial state: the operating point (permuted numbering), row 0 of the output
-  o << "  const double* x0 = xop + xop_off[inst];\n  double* o = xout + xout_off[inst];\n";
-  for (int j = 0; j < n; j++) o << "  o[" << j << "] = x0[" << j << "];\n";
-  for (int i = 0; i < n; i++) o << "  double xc" << i << " = x0[" << P.col_of[i] << "], xp" << i << " = xc" << i << ";\n";
+  o << "  const double* xin = xop + xop_off[inst];\n  double* o = xout + xout_off[inst];\n";
+  for (int j = 0; j < n; j++) o << "  o[" << j << "] = xin[" << j << "];\n";
+  for (int i = 0; i < n; i++) o << "  double xc" << i << " = xin[" << P.col_of[i] << "], xp" << i << " = xc" << i << ";\n";
   o << "  tout[(i64)inst * max_points] = 0.0;\n  cnt = 1;\n"
        "  double tcur = 0.0, h = h0, tprev = 0.0;\n  bool have_prev = false;\n"
        "  const i64 guard_max = 64 * (i64)max_points + 1000;\n"
@@ -456,6 +456,166 @@ bool jit_dc_source(const PatLayout& L, const int32_t* b, std::string& src, std::
   o << "    if (iters) iters[(i64)inst * npts + pt] = 1;\n"
        "  }\n"
        "  status[inst] = st;\n}\n";
+  src = o.str();
+  return true;
+}
+
+}  // namespace kg
+
+namespace kg {
+
+const char* jit_tran_name() { return "kg_tran_lane"; }
+
+// Fixed-step transient (transient.rs:14-114; the generic k_tran): backward
+// Euler with krets' quirks (method 0: sources at the previous step's time on
+// the first Newton iteration, krets' diode companion with its 0.5 V default
+// for an absent plus node, the bookkeeping time key in the first
+// convergence test), clean backward Euler (1) and trapezoidal (2) with
+// SPICE3 junction limiting. One lane integrates one instance; the Newton
+// loop, companions and stamps are those of k_tran, operation for operation.
+bool jit_tran_source(const PatLayout& L, const int32_t* b, std::string& src, std::string& why) {
+  if (L.status != KG_OK) {
+    why = "structurally singular pattern";
+    return false;
+  }
+  TV P = tview(L, b);
+  const int n = P.n, ne = P.ne;
+  int nvals = 0, nd = 0, nu = 0;
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    nvals += ty == KG_V ? 12 : ty == KG_D ? 5 : ty == KG_C || ty == KG_L ? 4 : 1;
+    nd += ty == KG_D;
+  }
+  for (int i = 0; i < n; i++) nu += P.row_ptr[i + 1] - P.diag[i];
+  if (2 * (nu + 4 * n + 3 * nd + nvals) > 400) {
+    why = "per-lane state exceeds the register budget";
+    return false;
+  }
+  std::ostringstream o;
+  o << kLtePrelude;
+  o << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_tran_name()
+    << "(const double* __restrict__ prm,\n"
+       "    const double* __restrict__ ebase, const double* __restrict__ xop, const i64* __restrict__ xop_off,\n"
+       "    double* __restrict__ out, const i64* __restrict__ out_off, int* __restrict__ status,\n"
+       "    int* __restrict__ iters, double* __restrict__ err, int ninst, double h, double alpha, int nsteps,\n"
+       "    int method, double reltol, double iabstol, double vabstol, int maxiter) {\n"
+       "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
+       "  if (inst >= ninst) return;\n"
+       "  int st = status[inst];\n"
+       "  const bool compat = method == 0, trap = method == 2;\n"
+       "  if (st == 0) {\n"
+    << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
+    << "  const double* EB = ebase + (i64)inst * " << ne << ";\n";
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    const std::string pe = "PR[" + S(e * KG_NPARAM);
+    if (ty == KG_V) {
+      o << "  const double pw" << e << "[10] = {";
+      for (int k = 0; k < 10; k++) o << (k ? ", " : "") << "PR[" << e * KG_NPARAM + k << "]";
+      o << "};\n";
+    } else if (ty == KG_D) {
+      // diode.rs:24-71: krets' limit (effectively none) and the clean SPICE3 limit
+      o << "  const double is" << e << " = " << pe << "], nd" << e << " = PR[" << e * KG_NPARAM + 1 << "];\n"
+        << "  const double nvt" << e << " = nd" << e << " * KG_VT;\n"
+        << "  const double vcrit" << e << " = nvt" << e << " * log(nvt" << e << " / (1.4142135623730951 * is" << e
+        << "));\n"
+        << "  const double vcritc" << e << " = nvt" << e << " * log(1.7976931348623157e308 * nd" << e
+        << " * KG_VT / is" << e << ");\n";
+    } else if (ty == KG_R || ty == KG_E || ty == KG_G || ty == KG_F || ty == KG_H) {
+      o << "  const double g" << e << " = EB[" << e << "];\n";
+      if (ty == KG_R) o << "  const double r" << e << " = " << pe << "];\n";
+    } else {
+      o << "  const double p" << e << " = " << pe << "];\n";
+    }
+  }
+  o << "  const double* xin = xop + xop_off[inst];\n  double* o = out + out_off[inst];\n";
+  for (int j = 0; j < n; j++) o << "  o[" << j << "] = xin[" << j << "];\n";
+  for (int i = 0; i < n; i++) o << "  double xp" << i << " = xin[" << P.col_of[i] << "], x" << i << " = 0.0;\n";
+  o << "  if (iters) iters[(i64)inst * (nsteps + 1)] = 0;\n";
+  // trapezoidal companion states: capacitor currents 0, inductor voltages v(x0)
+  for (int e = 0; e < ne; e++) {
+    if (P.etype[e] == KG_C) o << "  double edc" << e << " = 0.0;\n";
+    if (P.etype[e] == KG_L) o << "  double edl" << e << " = " << node("xp", P.ep[e]) << " - " << node("xp", P.em[e]) << ";\n";
+  }
+  o << "  for (int s = 1; s <= nsteps; s++) {\n"
+       "    const double tn = (double)s * h, tprev = (double)(s - 1) * h;\n";
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    if (ty == KG_V) o << "    const double vsn" << e << " = wave(pw" << e << ", tn), vsp" << e << " = wave(pw" << e << ", tprev);\n";
+    if (ty == KG_C)
+      o << "    const double ch" << e << " = alpha * (p" << e << " / h);\n"
+        << "    double cq" << e << " = -(ch" << e << ") * (" << node("xp", P.ep[e]) << " - " << node("xp", P.em[e]) << ");\n"
+        << "    if (alpha != 1.0) cq" << e << " -= edc" << e << ";\n";
+    if (ty == KG_L)
+      o << "    const double lh" << e << " = alpha * (p" << e << " / h);\n"
+        << "    double lq" << e << " = -(lh" << e << ") * xp" << P.ec[e] << ";\n"
+        << "    if (alpha != 1.0) lq" << e << " -= edl" << e << ";\n";
+  }
+  for (int i = 0; i < n; i++) o << "    double gu" << i << " = xp" << i << ";\n";
+  for (int e = 0; e < ne; e++)
+    if (P.etype[e] == KG_D)
+      o << "    double vo" << e << " = " << node("xp", P.ep[e]) << " - " << node("xp", P.em[e]) << ", gd" << e
+        << " = 0.0, id" << e << " = 0.0;\n";
+  o << "    bool done = false;\n    int used = 1;\n"
+       "    for (int it = 0; it < maxiter; it++) {\n      bool lim = false, bad = false;\n";
+  for (int e = 0; e < ne; e++)
+    if (P.etype[e] == KG_V) o << "      const double vs" << e << " = (compat && it == 0) ? vsp" << e << " : vsn" << e << ";\n";
+  for (int e = 0; e < ne; e++) {
+    if (P.etype[e] != KG_D) continue;
+    const std::string E = S(e);
+    const int p = P.ep[e], m = P.em[e];
+    o << "      if (compat) {  // an absent plus node reads 0.5 V, an absent minus node 0 V (diode.rs:26-36)\n"
+      << "        const double vd = " << (p >= 0 ? "gu" + S(p) : std::string("0.5")) << " - "
+      << (m >= 0 ? "gu" + S(m) : std::string("0.0")) << ";\n"
+      << "        const double vl = fmin(fmax(vd, -vcritc" << E << "), vcritc" << E << ");\n"
+      << "        const double ex = exp(vl / nvt" << E << ");\n"
+      << "        gd" << E << " = (is" << E << " / nvt" << E << ") * ex;\n"
+      << "        id" << E << " = is" << E << " * (ex - 1.0) - gd" << E << " * vd;\n"
+      << "      } else {\n"
+      << "        double v = " << node("gu", p) << " - " << node("gu", m) << ";\n"
+      << "        if (v > vcrit" << E << " && fabs(v - vo" << E << ") > nvt" << E << " + nvt" << E << ") {\n"
+      << "          if (vo" << E << " > 0) {\n"
+      << "            const double arg = 1.0 + (v - vo" << E << ") / nvt" << E << ";\n"
+      << "            v = arg > 0 ? vo" << E << " + nvt" << E << " * log(arg) : vcrit" << E << ";\n"
+      << "          } else {\n            v = nvt" << E << " * log(v / nvt" << E << ");\n          }\n"
+      << "          lim = true;\n        }\n"
+      << "        const double ex = exp(v / nvt" << E << ");\n"
+      << "        gd" << E << " = is" << E << " / nvt" << E << " * ex;\n"
+      << "        id" << E << " = is" << E << " * (ex - 1.0) - gd" << E << " * v;\n"
+      << "        vo" << E << " = v;\n      }\n";
+  }
+  if (!emit_solve(o, P, "      ", lte_slot, lte_exc, why)) return false;
+  o << "      if (bad) { st = " << (int)KG_DECOMPOSITION << "; break; }\n";
+  if (L.nonlinear) {
+    o << "      bool cv = true;\n";
+    for (int i = 0; i < n; i++)
+      o << "      { const double df = fabs(x" << i << " - gu" << i << "), sc = fmax(fabs(x" << i << "), fabs(gu" << i
+        << ")); if (!(df <= reltol * sc + " << (P.isbr[i] ? "iabstol" : "vabstol") << ")) cv = false; }\n";
+    o << "      if (compat && it == 0)  // transient.rs:204, the time key t_n against t_{n-1}\n"
+         "        cv = cv && fabs(tn - tprev) <= reltol * fmax(fabs(tn), fabs(tprev)) + vabstol;\n"
+         "      if (!compat && lim) cv = false;\n"
+         "      if (cv) { done = true; used = it + 1; break; }\n";
+    for (int i = 0; i < n; i++) o << "      gu" << i << " = x" << i << ";\n";
+  } else {
+    o << "      done = true;\n      break;\n";
+  }
+  o << "    }\n"
+       "    if (st != 0) break;\n"
+    << "    if (!done) { st = " << (int)KG_MAX_ITERATIONS << "; break; }\n"
+    << "    if (trap) {\n";
+  for (int e = 0; e < ne; e++) {
+    if (P.etype[e] == KG_C)
+      o << "      edc" << e << " = 2.0 * p" << e << " / h * ((" << node("x", P.ep[e]) << " - " << node("x", P.em[e])
+        << ") - (" << node("xp", P.ep[e]) << " - " << node("xp", P.em[e]) << ")) - edc" << e << ";\n";
+    if (P.etype[e] == KG_L) o << "      edl" << e << " = " << node("x", P.ep[e]) << " - " << node("x", P.em[e]) << ";\n";
+  }
+  o << "    }\n    double* orow = o + (i64)s * " << n << ";\n";
+  for (int i = 0; i < n; i++) o << "    orow[" << P.col_of[i] << "] = x" << i << ";\n    xp" << i << " = x" << i << ";\n";
+  o << "    if (iters) iters[(i64)inst * (nsteps + 1) + s] = used;\n"
+       "  }\n"
+       "  }\n"
+       "  status[inst] = st;\n"
+       "  if (err) err[inst] = 0.0;\n}\n";
   src = o.str();
   return true;
 }

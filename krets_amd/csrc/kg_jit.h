@@ -57,6 +57,12 @@ const char* jit_ac_name(bool mixed, bool xsplit);
 bool jit_lte_source(const PatLayout& L, const int32_t* buf, std::string& src, std::string& why);
 const char* jit_lte_name();
 
+// Source of the lane-per-instance fixed-step transient kernel (jit_tran_name)
+// for one AK_TRAN pattern shared by every instance: BE-compat, BE and TRAP
+// (launch argument), the Newton loop, companions and stamps of k_tran. FP64.
+bool jit_tran_source(const PatLayout& L, const int32_t* buf, std::string& src, std::string& why);
+const char* jit_tran_name();
+
 // Source of the lane-per-instance kernel for linear operating points and DC
 // sweeps (jit_dc_name) for one AK_DC pattern shared by every instance: one
 // straight-line solve per point. Declines nonlinear circuits.

@@ -95,7 +95,7 @@ def test_controlled_ac(kg, oracle, batch, precision, tps):
         assert (err <= 1e-9).all()
 
 
-@pytest.mark.parametrize("tps", [0, 16, 1064, 3064, 3256])
+@pytest.mark.parametrize("tps", [0, 16, 1064, 2001, 3064, 3256])
 @pytest.mark.parametrize("method", [0, 1, 2])
 def test_controlled_transient(kg, oracle, batch, method, tps):
     nets = batch[0]

@@ -396,13 +396,14 @@ def test_lte_rectifier_vs_oracle(kg, oracle, tps):
         assert e < 1e-9, (i, e)
 
 
-def test_fixed_step_compat_rectifier_instances(kg, oracle):
+@pytest.mark.parametrize("tps", [0, 1, 64, 2001])  # 2001: lane kernel specialised at run time
+def test_fixed_step_compat_rectifier_instances(kg, oracle, tps):
     from krets_amd import workloads as W
-    say("fixed-step compat rectifier")
+    say(f"fixed-step compat rectifier tps={tps}")
     b = W.c4_rectifier(8, seed=4)
     ctx = kg.Context()
     b.load(ctx)
-    cfg = kg.SolverConfig(voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12)
+    cfg = kg.SolverConfig(voltage_absolute_tolerance=1e-9, current_absolute_tolerance=1e-12, threads_per_system=tps)
     xs, st, (it, _) = ctx.transient(1e-6, 2000, method=kg.BE_COMPAT, config=cfg)
     assert (st == 0).all()
     for i in (0, 3, 7):

@@ -75,7 +75,8 @@ def source(c, precision, refine=2, s_rep=2 * np.pi * 1e3):
 def ebase(elements, params):
     e = params[..., 0].copy()
     isr = (elements["type"] == 0) & (elements["g2"] == 0)
-    e[..., isr] = 1.0 / e[..., isr]
+    with np.errstate(divide="ignore"):  # a zero resistor's conductance is infinite, as in the library
+        e[..., isr] = 1.0 / e[..., isr]
     return e
 
 
@@ -476,3 +477,121 @@ def test_dc_lane_source_builds_for_gfx950():
         assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
     finally:
         shutil.rmtree(d)
+
+
+# ------------------------------------------------- fixed-step transient lane kernel
+TRAN_DRIVER = r"""
+extern "C" void run_host(const double* prm, const double* eb, const double* xop, const long long* xop_off,
+                         double* out, const long long* out_off, int* status, int* iters, double* err, int ninst,
+                         double h, double alpha, int nsteps, int method, double reltol, double itol, double vtol,
+                         int maxiter) {
+  gridDim.x = (unsigned)((ninst + 63) / 64);
+  for (unsigned b = 0; b < gridDim.x; b++)
+    for (unsigned t = 0; t < 64; t++) {
+      blockIdx.x = b;
+      threadIdx.x = t;
+      kg_tran_lane(prm, eb, xop, xop_off, out, out_off, status, iters, err, ninst, h, alpha, nsteps, method,
+                   reltol, itol, vtol, maxiter);
+    }
+}
+"""
+
+
+def tran_source(c, params, h, method):
+    from krets_amd._lib import lib
+    buf = ctypes.create_string_buffer(1 << 22)
+    el = np.ascontiguousarray(c.elements)
+    pr = np.ascontiguousarray(params, np.float64)
+    r = lib().kg_jit_tran_source(el.ctypes.data, len(el), c.n, pr.ctypes.data, h, method, buf, len(buf))
+    return r, buf.value.decode()
+
+
+def tran_host_run(src, params, elements, x0, h, nsteps, method, tol):
+    d = tempfile.mkdtemp()
+    try:
+        cpp, so = os.path.join(d, "k.cpp"), os.path.join(d, "k.so")
+        open(cpp, "w").write(LTE_SHIM + src + TRAN_DRIVER)
+        subprocess.run(["g++", "-O1", "-std=c++17", "-shared", "-fPIC", cpp, "-o", so], check=True,
+                       capture_output=True)
+        L = ctypes.CDLL(so)
+        ni, n = x0.shape
+        prm = np.ascontiguousarray(params, np.float64)
+        eb = np.ascontiguousarray(ebase(elements, params), np.float64)
+        xop = np.ascontiguousarray(x0, np.float64)
+        xoff = np.arange(ni, dtype=np.int64) * n
+        out = np.zeros(ni * (nsteps + 1) * n)
+        ooff = np.arange(ni, dtype=np.int64) * (nsteps + 1) * n
+        st, it, err = np.zeros(ni, np.int32), np.zeros(ni * (nsteps + 1), np.int32), np.zeros(ni)
+        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        L.run_host.argtypes = [P] * 9 + [I, D, D, I, I, D, D, D, I]
+        L.run_host(*[a.ctypes.data for a in (prm, eb, xop, xoff, out, ooff, st, it, err)], ni, h,
+                   2.0 if method == 2 else 1.0, nsteps, method, *tol)
+        return out.reshape(ni, nsteps + 1, n), st, it.reshape(ni, nsteps + 1)
+    finally:
+        shutil.rmtree(d)
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_tran_lane_source_rectifier_on_host_vs_oracle(oracle, method):
+    """The generated fixed-step kernel on the half-wave rectifiers at h = 1 us:
+    krets' integrator (method 0, with its lagged sources, 0.5 V default and
+    time key) and the clean BE / trapezoidal ones, against the oracle: the
+    same Newton iterations at every step, states to 1e-9."""
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    b = W.c4_rectifier(4, seed=4)
+    c = kg.Circuit(b.netlists[0])
+    h, N = 1e-6, 600
+    r, src = tran_source(c, b.params[0], h, method)
+    assert r > 0, src
+    ocfg = oracle.config(vabstol=1e-9, iabstol=1e-12)
+    refs = [oracle.Circuit(b.netlists[i]).tran(h, N * h, method=method, cfg=ocfg) for i in range(4)]
+    x0 = np.stack([xr[0] for xr, _ in refs])
+    x, st, it = tran_host_run(src, b.params, c.elements, x0, h, N, method, (1e-3, 1e-12, 1e-9, 300))
+    assert (st == 0).all()
+    for i, (xr, itr) in enumerate(refs):
+        assert np.array_equal(it[i][1:], itr[1:]), i
+        assert max(nerr(x[i][k], xr[k]) for k in range(1, N + 1)) < 1e-9
+
+
+@pytest.mark.parametrize("method", [0, 2])
+def test_tran_lane_source_linear_on_host_vs_oracle(oracle, method):
+    """Linear circuits through the same kernel: the active filter of every
+    controlled source and an RLC ladder (inductor companions)."""
+    import krets_amd as kg
+    from conftest import active_netlist
+    from krets_amd import workloads as W
+    for text, h, N in ((active_netlist(), 2e-6, 100),
+                       (W.c3_netlist([20.0, 35.0, 50.0], [2e-9, 3e-9, 4e-9], [0.3e-12, 0.5e-12, 0.7e-12]), 1e-11, 200)):
+        c = kg.Circuit(text)
+        r, src = tran_source(c, c.params, h, method)
+        assert r > 0, src
+        xr, _ = oracle.Circuit(text).tran(h, N * h, method=method)
+        x, st, it = tran_host_run(src, c.params[None], c.elements, xr[:1], h, N, method, (1e-3, 1e-12, 1e-6, 300))
+        assert st[0] == 0 and (it[0][1:] == 1).all()
+        assert max(nerr(x[0][k], xr[k]) for k in range(1, N + 1)) < 1e-12
+
+
+def test_tran_lane_source_builds_for_gfx950():
+    """The fixed-step kernel builds for gfx950 without scratch on the rectifier
+    and on the active filter of every controlled source."""
+    import krets_amd as kg
+    from conftest import active_netlist
+    from krets_amd import workloads as W
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
+    b = W.c4_rectifier(1, seed=4)
+    for c in (kg.Circuit(b.netlists[0]), kg.Circuit(active_netlist())):
+        r, src = tran_source(c, c.params, 1e-6, 2)
+        assert r > 0, src
+        d = tempfile.mkdtemp()
+        try:
+            p = os.path.join(d, "k.hip")
+            open(p, "w").write("#include <hip/hip_runtime.h>\n" + src)
+            out = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                  "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", p, "-o", p + ".o",
+                                  "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
+            assert out.returncode == 0, out.stderr[-2000:]
+            assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
+        finally:
+            shutil.rmtree(d)
