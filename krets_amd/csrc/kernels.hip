@@ -952,6 +952,24 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
 // arithmetic is k_tran's on a linear circuit, operation for operation
 // (solver/transient.rs:14-114 semantics, trapezoidal additions).
 constexpr int LIN_KE = 4;  // cached strictly-triangular entries per row
+
+// The entries of a long row beyond the cached ones, sum_p v[p] * x[col[p]]
+// over [p, end): four independent partial sums, so the column and value
+// loads of consecutive entries overlap instead of forming one dependent
+// chain (the top separator rows of a nested-dissection order on a ladder
+// hold up to ~2 log2 n entries, on the critical path of every solve).
+__device__ __forceinline__ double row_tail(const double* v, const double* x, const uint16_t* col, int p, int end) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (; p + 4 <= end; p += 4) {
+    const int c0 = col[p], c1 = col[p + 1], c2 = col[p + 2], c3 = col[p + 3];
+    s0 += v[p] * x[c0];
+    s1 += v[p + 1] * x[c1];
+    s2 += v[p + 2] * x[c2];
+    s3 += v[p + 3] * x[c3];
+  }
+  for (; p < end; p++) s0 += v[p] * x[col[p]];
+  return (s0 + s1) + (s2 + s3);
+}
 constexpr int LIN_KC = 2;  // cached right-hand-side contributions per row
 
 template <int TPS, int R>
@@ -1197,7 +1215,7 @@ struct LinSys {
         }
 #pragma unroll
         for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];  // padded entries subtract 0 * 0
-        for (int p = p0 + LIN_KE; p < p0 + cnt; p++) sum -= v[p] * xc[col[p]];
+        if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
         xc[i] = sum;
       }
       __syncthreads();
@@ -1218,7 +1236,7 @@ struct LinSys {
         }
 #pragma unroll
         for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-        for (int p = p0 + LIN_KE; p < p0 + cnt; p++) sum -= v[p] * xc[col[p]];
+        if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
         xc[i] = sum * vd;  // diagonal stored as its reciprocal
       }
       __syncthreads();
