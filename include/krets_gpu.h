@@ -11,7 +11,11 @@
  * A context owns its device memory and one HIP stream; it is not re-entrant
  * (one context per host thread), like krets' `Solver::solve(&mut self, ..)`.
  * Output buffers are caller-owned, either host memory (KG_MEM_HOST) or device
- * memory on the context's GPU (KG_MEM_DEVICE).
+ * memory on the context's GPU (KG_MEM_DEVICE). The per-instance status array
+ * and error estimates of kg_op / kg_dc_sweep / kg_ac_sweep / kg_transient /
+ * kg_transient_lte may be either, whatever `mem` says of the other outputs
+ * (the library checks the pointer): a device solution with host statuses is
+ * the common case.
  */
 #ifndef KRETS_GPU_H
 #define KRETS_GPU_H

@@ -99,6 +99,7 @@ struct kg_ctx {
   // estimates): a DMA each instead of a pageable copy; used only inside
   // finish_status, which leaves no copy in flight into it
   char* pin = nullptr;
+  char* pin_dev = nullptr;  // its device address (kernels write it directly), or null
   size_t pin_cap = 0;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
@@ -940,36 +941,64 @@ int32_t run_adj_op_term(kg_ctx* c, const Plan& P, double* d_xb, double* d_g, int
   });
 }
 
+// True when p is device memory of the context's GPU (host memory, pageable
+// or pinned, is not).
+bool on_device(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable host memory is unknown to the runtime
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
 // The end of every analysis call: copies the statuses (and one optional small
 // per-instance output, xdst <- xsrc) to the caller, synchronises the stream
-// and returns the first failure. Host copies go through the pinned staging
-// area; every copy enqueued here has completed when it returns.
+// and returns the first failure. These small outputs may be host or device
+// memory whatever `mem` says of the large ones (the pointer decides). Host
+// copies go through the pinned staging area; every copy enqueued here has
+// completed when it returns.
 int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem, void* xdst,
                       const void* xsrc, size_t xbytes) {
+  (void)mem;
   const size_t sb = sizeof(int32_t) * c->n_inst;
   const size_t xo = (sb + 255) & ~(size_t)255;
-  const bool xcopy = xdst && xsrc && xbytes, xhost = xcopy && mem == KG_MEM_HOST;
+  const bool xcopy = xdst && xsrc && xbytes, xhost = xcopy && !on_device(xdst);
+  const bool shost = status_out && !on_device(status_out);
   const size_t need = xo + (xhost ? xbytes : 0);
   if (need > c->pin_cap) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->pin) (void)hipHostFree(c->pin);
-    c->pin = nullptr;
+    c->pin = c->pin_dev = nullptr;
     c->pin_cap = 0;
     const size_t cap = std::max<size_t>(2 * need, 1 << 16);
     HIPCHK(c, hipHostMalloc((void**)&c->pin, cap, hipHostMallocDefault));
     c->pin_cap = cap;
+    // with a device address the statuses are published by one small kernel
+    // (two DMA copies cost several times its launch)
+    if (hipHostGetDevicePointer((void**)&c->pin_dev, c->pin, 0) != hipSuccess) {
+      c->pin_dev = nullptr;
+      (void)hipGetLastError();
+    }
   }
-  hipError_t e = hipMemcpyAsync(c->pin, d_status, sb, hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess && xcopy)
-    e = hipMemcpyAsync(xhost ? (void*)(c->pin + xo) : xdst, xsrc, xbytes,
-                       xhost ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, c->stream);
-  if (e == hipSuccess && status_out && mem != KG_MEM_HOST)
+  hipError_t e = hipSuccess;
+  if (c->pin_dev) {
+    e = run_copy_words((const uint32_t*)d_status, (uint32_t*)c->pin_dev, (int64_t)(sb / 4),
+                       xhost ? (const uint32_t*)xsrc : nullptr, (uint32_t*)(c->pin_dev + xo),
+                       xhost ? (int64_t)(xbytes / 4) : 0, c->stream);
+  } else {
+    e = hipMemcpyAsync(c->pin, d_status, sb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && xhost) e = hipMemcpyAsync(c->pin + xo, xsrc, xbytes, hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess && xcopy && !xhost)
+    e = hipMemcpyAsync(xdst, xsrc, xbytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess && status_out && !shost)
     e = hipMemcpyAsync(status_out, d_status, sb, hipMemcpyDeviceToDevice, c->stream);
   const hipError_t es = hipStreamSynchronize(c->stream);
   if (e != hipSuccess || es != hipSuccess)
     return set_err(c, KG_HIP_ERROR, std::string("status copy: ") + hipGetErrorString(e != hipSuccess ? e : es));
   std::vector<int32_t> st((const int32_t*)c->pin, (const int32_t*)c->pin + c->n_inst);
-  if (status_out && mem == KG_MEM_HOST) std::memcpy(status_out, c->pin, sb);
+  if (shost) std::memcpy(status_out, c->pin, sb);
   if (xhost) std::memcpy(xdst, c->pin + xo, xbytes);
   int32_t f = first_failure(st);
   if (f != KG_OK && c->err.empty()) c->err = "at least one instance failed";

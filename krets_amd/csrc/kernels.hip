@@ -2468,6 +2468,21 @@ __global__ void k_fill_f64(double* p, double v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+// Copies two word ranges (a -> da, b -> db) in one launch; used to publish a
+// call's statuses and per-instance estimates into pinned host memory.
+__global__ void k_copy_words(const uint32_t* a, uint32_t* da, int64_t na, const uint32_t* b, uint32_t* db,
+                             int64_t nb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) da[i] = a[i];
+  else if (i < na + nb) db[i - na] = b[i - na];
+}
+hipError_t run_copy_words(const uint32_t* a, uint32_t* da, int64_t na, const uint32_t* b, uint32_t* db, int64_t nb,
+                          hipStream_t st) {
+  if (na + nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, st, a, da, na, b, db, nb);
+  return hipGetLastError();
+}
+
 hipError_t run_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
