@@ -102,9 +102,13 @@ def main():
     # > 0 a fixed number of passes
     refine = {"iters": 0}
 
+    cfgs = {}
+
     def step(prec):
-        cfg = kg.SolverConfig(refine_iters=refine["iters"]) if prec == kg.MIXED else None
-        r, st, err = ctx.ac_sweep(f, precision=prec, out=out[prec], config=cfg)
+        key = (prec, refine["iters"])
+        if key not in cfgs:
+            cfgs[key] = kg.SolverConfig(refine_iters=refine["iters"]) if prec == kg.MIXED else None
+        r, st, err = ctx.ac_sweep(f, precision=prec, out=out[prec], config=cfgs[key])
         if (st != 0).any():
             raise RuntimeError(f"solve failed: statuses {np.unique(st)}")
         return err

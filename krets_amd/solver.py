@@ -210,6 +210,7 @@ class Context:
         b.elements = ctypes.c_void_p(el.ctypes.data)
         b.params = pr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         self._check(lib().kg_load_batch(self._h, ctypes.byref(b)))
+        self._n_total = int(self.n_unknowns.sum())
         self.n_instances = ni
         self.shared = elem_offset is None
         self.n_elements = el.shape[0] if elem_offset is None else np.diff(elem_offset)
@@ -252,7 +253,7 @@ class Context:
         cfg = (config or SolverConfig()).to_c()
         f = np.ascontiguousarray(freqs, np.float64)
         nf = len(f)
-        tot = int(self.n_unknowns.sum()) * nf * 2
+        tot = self._n_total * nf * 2
         x = np.zeros(tot) if out is None else out
         st = np.zeros(self.n_instances, np.int32)
         err = np.zeros(self.n_instances)

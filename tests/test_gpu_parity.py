@@ -66,6 +66,17 @@ def test_op_vs_oracle(kg, oracle, path, tps):
     assert nerr(x, xr) < tol
 
 
+@pytest.mark.parametrize("path", [p for p in OP_CASES if "diode" not in p])
+def test_op_lane_kernel_vs_oracle(kg, oracle, path):
+    """Linear operating points through the run-time specialised lane kernel
+    (2001), which the default mapping also takes for these circuits."""
+    say(f"op lane {path}")
+    text = read_circuit(path)
+    c, r = gpu_op(kg, text, 2001)
+    xr, _ = oracle.Circuit(text).op()
+    assert nerr(np.array([r[k] for k in oracle.Circuit(text).names]), xr) < 1e-12
+
+
 def test_op_krets_known_answers(kg):
     say("op known answers")
     _, s = gpu_op(kg, read_circuit("circuit_simulation_farid_n_najm/circuit_simulation_farid_n_najm.cir"))
@@ -371,6 +382,32 @@ def test_error_statuses(kg):
     with pytest.raises(kg.KretsGpuError) as e:
         s.solve(kg.Dc("V9", 0, 1, 1))
     assert e.value.code == 4
+
+
+def test_lane_mapping_errors_and_singular_lanes(kg):
+    """The run-time specialised lane mappings (2001) refuse mixed precision and
+    per-instance topologies with InvalidArgument, and a lane whose pivot is
+    zero reports Decomposition for its own instance only."""
+    from krets_amd import workloads as W
+    say("lane mapping errors")
+    b = W.c4_rectifier(4, seed=4)
+    ctx = kg.Context()
+    b.load(ctx)
+    with pytest.raises(kg.KretsGpuError) as e:
+        ctx.transient(1e-6, 10, method=kg.BE, precision=kg.MIXED, config=kg.SolverConfig(threads_per_system=2001))
+    assert e.value.code == 9
+    t = W.c3_rlc_uneven(3, smin=8, smax=16, seed=7)
+    ctx.load(t.elements, t.params, t.n_unknowns, t.elem_offset)
+    with pytest.raises(kg.KretsGpuError) as e:
+        ctx.transient(1e-11, 10, method=kg.TRAP, config=kg.SolverConfig(threads_per_system=2001))
+    assert e.value.code == 9
+    # instance 1 has a zero resistor (infinite conductance): only its status fails
+    c = kg.Circuit("V1 1 0 1\nR1 1 2 1k\nR2 2 0 1k\n")
+    p = np.repeat(c.params[None], 3, axis=0)
+    p[1, 1, 0] = 0.0
+    ctx.load(c.elements, p, c.n)
+    x, st, _ = ctx.op(config=kg.SolverConfig(threads_per_system=0))
+    assert list(st) == [0, 2, 0] and abs(x[0][c.names.index("V(2)")] - 0.5) < 1e-15
 
 
 @pytest.mark.parametrize("tps", [0, 1, 4, 2001])  # 2001: lane kernel specialised at run time
