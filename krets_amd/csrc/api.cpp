@@ -543,8 +543,7 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   // linear circuits of a shared topology: one solve per point, in the lane
   // kernel specialised to the topology at run time (2001), unless another
   // mapping is asked for or the topology does not fit it
-  if (c->shared && !P0.any_nonlinear && P0.group_status[0] == KG_OK && cfg->maxiter >= 1 &&
-      (tps0 == 0 || tps0 == 2001)) {
+  if (c->shared && P0.group_status[0] == KG_OK && cfg->maxiter >= 1 && (tps0 == 0 || tps0 == 2001)) {
     Plan PL;
     if ((r = prepare(c, AK_DC, ORD_MINDEG, 0.0, PL))) return r;
     const int pidx = c->pat_index[std::make_tuple(0, (int)AK_DC, (int)ORD_MINDEG, 0.0)];
@@ -580,10 +579,13 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
       hipStream_t strm = c->stream;
       const int ni = c->n_inst;
       const hipFunction_t f = fn.fn;
+      const NewtonArgs na = newton_args(cfg);
       std::vector<Deferred> steps{{jit_dc_name(), [=]() mutable {
-                                     int a_ni = ni, a_se = sweep_e, a_np = npts;
-                                     double a_st = start, a_sp = step;
-                                     void* args[] = {&prm, &eb, &a_ni, &a_se, &a_st, &a_sp, &a_np, &o, &oo, &st, &its};
+                                     int a_ni = ni, a_se = sweep_e, a_np = npts, a_cp = na.compat, a_mi = na.maxiter;
+                                     double a_st = start, a_sp = step, a_rt = na.reltol, a_it = na.iabstol,
+                                            a_vt = na.vabstol;
+                                     void* args[] = {&prm, &eb,  &a_ni, &a_se, &a_st, &a_sp, &a_np, &o,   &oo,
+                                                     &st,  &its, &a_cp, &a_rt, &a_it, &a_vt, &a_mi};
                                      return hipModuleLaunchKernel(f, (unsigned)((ni + 63) / 64), 1, 1, 64, 1, 1, 0, strm,
                                                                   args, nullptr);
                                    }}};

@@ -193,8 +193,8 @@ std::string lte_exc(const TV& P, int e, int k) {
 std::string dc_slot(const TV& P, int e, int k) {
   const int ty = P.etype[e];
   switch (k) {
-    case CK_YP: return ty == KG_R ? "g" + S(e) : "";
-    case CK_YN: return ty == KG_R ? "-g" + S(e) : "";
+    case CK_YP: return ty == KG_R ? "g" + S(e) : ty == KG_D ? "gd" + S(e) : "";
+    case CK_YN: return ty == KG_R ? "-g" + S(e) : ty == KG_D ? "-gd" + S(e) : "";
     case CK_ONEP: return "1.0";
     case CK_ONEN: return "-1.0";
     case CK_BR: return ty == KG_R ? "-r" + S(e) : "";
@@ -204,6 +204,8 @@ std::string dc_slot(const TV& P, int e, int k) {
   }
 }
 std::string dc_exc(const TV& P, int e, int k) {
+  if (k == RK_DP) return "-id" + S(e);
+  if (k == RK_DM) return "id" + S(e);
   if (k != RK_SRC) return "";
   return "(sweep_e == " + S(e) + " ? sv : p" + S(e) + ")";
 }
@@ -384,15 +386,14 @@ const char* jit_dc_name() { return "kg_dc_lane"; }
 // Linear operating points and DC sweeps (op.rs:15-86, dc.rs:14-124 for
 // circuits without diodes: one solve per point, which krets' Newton loop
 // accepts after its first pass): one lane per instance, its points in order.
+bool jit_dc_newton_source(const PatLayout& L, const TV& P, std::string& src, std::string& why);
+
 bool jit_dc_source(const PatLayout& L, const int32_t* b, std::string& src, std::string& why) {
   if (L.status != KG_OK) {
     why = "structurally singular pattern";
     return false;
   }
-  if (L.nonlinear) {
-    why = "nonlinear circuits take the Newton kernels";
-    return false;
-  }
+  if (L.nonlinear) return jit_dc_newton_source(L, tview(L, b), src, why);
   TV P = tview(L, b);
   const int n = P.n, ne = P.ne;
   // registers live across the solve: the upper factor and pivots, x, and the
@@ -438,7 +439,7 @@ bool jit_dc_source(const PatLayout& L, const int32_t* b, std::string& src, std::
     << "(const double* __restrict__ prm,\n"
        "    const double* __restrict__ ebase, int ninst, int sweep_e, double start, double step, int npts,\n"
        "    double* __restrict__ out, const i64* __restrict__ out_off, int* __restrict__ status,\n"
-       "    int* __restrict__ iters) {\n"
+       "    int* __restrict__ iters, int compat, double reltol, double iabstol, double vabstol, int maxiter) {\n"
        "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
        "  if (inst >= ninst) return;\n"
     << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
@@ -616,6 +617,119 @@ bool jit_tran_source(const PatLayout& L, const int32_t* b, std::string& src, std
        "  }\n"
        "  status[inst] = st;\n"
        "  if (err) err[inst] = 0.0;\n}\n";
+  src = o.str();
+  return true;
+}
+
+}  // namespace kg
+
+namespace kg {
+
+// Operating points and DC sweeps of circuits with diodes (op.rs:15-86,
+// dc.rs:14-124; the generic k_newton_dc): one lane per instance walks its
+// points, each a Newton loop warm-started from the previous point. compat
+// (krets): the first iteration of the first point sees an empty solution map
+// (diodes at 0.5 V / 0 V, no convergence), and the first iteration of every
+// later sweep point compares the bookkeeping step key; clean: zero start and
+// SPICE3 junction limiting.
+bool jit_dc_newton_source(const PatLayout& L, const TV& P, std::string& src, std::string& why) {
+  const int n = P.n, ne = P.ne;
+  int nvals = 0, nd = 0, nu = 0;
+  for (int e = 0; e < ne; e++) {
+    nvals += P.etype[e] == KG_D ? 5 : 2;
+    nd += P.etype[e] == KG_D;
+  }
+  for (int i = 0; i < n; i++) nu += P.row_ptr[i + 1] - P.diag[i];
+  if (2 * (nu + 3 * n + 3 * nd + nvals) > 400) {
+    why = "per-lane state exceeds the register budget";
+    return false;
+  }
+  std::ostringstream o;
+  o << "typedef long long i64;\n#define KG_VT (1.380649e-23 * 300.0 / 1.602176634e-19)\n"
+    << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_dc_name()
+    << "(const double* __restrict__ prm,\n"
+       "    const double* __restrict__ ebase, int ninst, int sweep_e, double start, double step, int npts,\n"
+       "    double* __restrict__ out, const i64* __restrict__ out_off, int* __restrict__ status,\n"
+       "    int* __restrict__ iters, int compat, double reltol, double iabstol, double vabstol, int maxiter) {\n"
+       "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
+       "  if (inst >= ninst) return;\n"
+    << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
+    << "  const double* EB = ebase + (i64)inst * " << ne << ";\n";
+  for (int e = 0; e < ne; e++) {
+    const int ty = P.etype[e];
+    if (ty == KG_V || ty == KG_I) o << "  const double p" << e << " = PR[" << e * KG_NPARAM << "];\n";
+    if (ty == KG_R || ty == KG_E || ty == KG_G || ty == KG_F || ty == KG_H)
+      o << "  const double g" << e << " = EB[" << e << "];\n";
+    if (ty == KG_R) o << "  const double r" << e << " = PR[" << e * KG_NPARAM << "];\n";
+    if (ty == KG_D)
+      o << "  const double is" << e << " = PR[" << e * KG_NPARAM << "], nd" << e << " = PR[" << e * KG_NPARAM + 1
+        << "];\n"
+        << "  const double nvt" << e << " = nd" << e << " * KG_VT;\n"
+        << "  const double vcrit" << e << " = nvt" << e << " * log(nvt" << e << " / (1.4142135623730951 * is" << e
+        << "));\n"
+        << "  const double vcritc" << e << " = nvt" << e << " * log(1.7976931348623157e308 * nd" << e
+        << " * KG_VT / is" << e << ");\n";
+  }
+  o << "  int st = 0;\n";
+  for (int i = 0; i < n; i++) o << "  double gu" << i << " = 0.0, x" << i << " = 0.0;\n";
+  o << "  for (int pt = 0; pt < npts && st == 0; pt++) {\n"
+       "    const double sv = start + (double)pt * step;\n"
+       "    bool prev_empty = compat && pt == 0;\n";
+  for (int e = 0; e < ne; e++)
+    if (P.etype[e] == KG_D)
+      o << "    double vo" << e << " = compat ? 0.0 : " << node("gu", P.ep[e]) << " - " << node("gu", P.em[e]) << ", gd"
+        << e << " = 0.0, id" << e << " = 0.0;\n";
+  o << "    bool done = false;\n    int used = maxiter;\n"
+       "    for (int it = 0; it < maxiter; it++) {\n      bool lim = false, bad = false;\n";
+  for (int e = 0; e < ne; e++) {
+    if (P.etype[e] != KG_D) continue;
+    const std::string E = S(e);
+    const int p = P.ep[e], m = P.em[e];
+    o << "      if (compat) {  // an absent key reads 0.5 V (plus) / 0 V (minus) (diode.rs:26-36)\n"
+      << "        const double vd = (prev_empty" << (p < 0 ? " || true" : "") << " ? 0.5 : "
+      << (p >= 0 ? "gu" + S(p) : std::string("0.5")) << ") - (prev_empty" << (m < 0 ? " || true" : "")
+      << " ? 0.0 : " << (m >= 0 ? "gu" + S(m) : std::string("0.0")) << ");\n"
+      << "        const double vl = fmin(fmax(vd, -vcritc" << E << "), vcritc" << E << ");\n"
+      << "        const double ex = exp(vl / nvt" << E << ");\n"
+      << "        gd" << E << " = (is" << E << " / nvt" << E << ") * ex;\n"
+      << "        id" << E << " = is" << E << " * (ex - 1.0) - gd" << E << " * vd;\n"
+      << "      } else {\n"
+      << "        double v = " << node("gu", p) << " - " << node("gu", m) << ";\n"
+      << "        if (v > vcrit" << E << " && fabs(v - vo" << E << ") > nvt" << E << " + nvt" << E << ") {\n"
+      << "          if (vo" << E << " > 0) {\n"
+      << "            const double arg = 1.0 + (v - vo" << E << ") / nvt" << E << ";\n"
+      << "            v = arg > 0 ? vo" << E << " + nvt" << E << " * log(arg) : vcrit" << E << ";\n"
+      << "          } else {\n            v = nvt" << E << " * log(v / nvt" << E << ");\n          }\n"
+      << "          lim = true;\n        }\n"
+      << "        const double ex = exp(v / nvt" << E << ");\n"
+      << "        gd" << E << " = is" << E << " / nvt" << E << " * ex;\n"
+      << "        id" << E << " = is" << E << " * (ex - 1.0) - gd" << E << " * v;\n"
+      << "        vo" << E << " = v;\n      }\n";
+  }
+  if (!emit_solve(o, P, "      ", dc_slot, dc_exc, why)) return false;
+  o << "      bool stop = bad;\n"
+       "      if (!prev_empty) {\n        bool cv = true;\n";
+  for (int i = 0; i < n; i++)
+    o << "        { const double df = fabs(x" << i << " - gu" << i << "), sc = fmax(fabs(x" << i << "), fabs(gu" << i
+      << ")); if (!(df <= reltol * sc + " << (P.isbr[i] ? "iabstol" : "vabstol") << ")) cv = false; }\n";
+  o << "        if (compat && it == 0 && pt > 0 && sweep_e >= 0) {  // the step key: pt against pt - 1\n"
+       "          const double v1 = (double)pt, v0 = (double)(pt - 1);\n"
+       "          cv = cv && fabs(v1 - v0) <= reltol * fmax(fabs(v1), fabs(v0)) + vabstol;\n"
+       "        }\n"
+       "        if (!compat && lim) cv = false;\n"
+       "        stop = stop || cv;\n"
+       "      }\n"
+       "      prev_empty = false;\n"
+    << "      if (bad) st = " << (int)KG_DECOMPOSITION << ";\n"
+    << "      if (stop) { done = true; used = it + 1; break; }\n";
+  for (int i = 0; i < n; i++) o << "      gu" << i << " = x" << i << ";\n";
+  o << "    }\n"
+    << "    if (!done) st = " << (int)KG_MAX_ITERATIONS << ";\n"
+    << "    double* o = out + out_off[inst] + (i64)pt * " << n << ";\n";
+  for (int i = 0; i < n; i++) o << "    o[" << P.col_of[i] << "] = x" << i << ";\n    gu" << i << " = x" << i << ";\n";
+  o << "    if (iters) iters[(i64)inst * npts + pt] = used;\n"
+       "  }\n"
+       "  status[inst] = st;\n}\n";
   src = o.str();
   return true;
 }

@@ -370,13 +370,15 @@ def test_lte_lane_source_builds_for_gfx950():
 # ------------------------------------------------- linear OP / DC lane kernel
 DC_DRIVER = r"""
 extern "C" void run_host(const double* prm, const double* eb, int ninst, int sweep_e, double start, double step,
-                         int npts, double* out, const long long* off, int* status, int* iters) {
+                         int npts, double* out, const long long* off, int* status, int* iters, int compat,
+                         double reltol, double itol, double vtol, int maxiter) {
   gridDim.x = (unsigned)((ninst + 63) / 64);
   for (unsigned b = 0; b < gridDim.x; b++)
     for (unsigned t = 0; t < 64; t++) {
       blockIdx.x = b;
       threadIdx.x = t;
-      kg_dc_lane(prm, eb, ninst, sweep_e, start, step, npts, out, off, status, iters);
+      kg_dc_lane(prm, eb, ninst, sweep_e, start, step, npts, out, off, status, iters, compat, reltol, itol, vtol,
+                 maxiter);
     }
 }
 """
@@ -391,7 +393,7 @@ def dc_source(c, params):
     return r, buf.value.decode()
 
 
-def dc_host_run(src, params, elements, n, sweep_e, start, step, npts):
+def dc_host_run(src, params, elements, n, sweep_e, start, step, npts, cfg=(1, 1e-3, 1e-12, 1e-6, 300)):
     d = tempfile.mkdtemp()
     try:
         cpp, so = os.path.join(d, "k.cpp"), os.path.join(d, "k.so")
@@ -407,9 +409,9 @@ def dc_host_run(src, params, elements, n, sweep_e, start, step, npts):
         off = np.arange(ni, dtype=np.int64) * npts * n
         st, it = np.zeros(ni, np.int32), np.zeros(ni * npts, np.int32)
         P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
-        L.run_host.argtypes = [P, P, I, I, D, D, I, P, P, P, P]
+        L.run_host.argtypes = [P, P, I, I, D, D, I, P, P, P, P, I, D, D, D, I]
         L.run_host(prm.ctypes.data, eb.ctypes.data, ni, sweep_e, start, step, npts, out.ctypes.data, off.ctypes.data,
-                   st.ctypes.data, it.ctypes.data)
+                   st.ctypes.data, it.ctypes.data, *cfg)
         return out.reshape(ni, npts, n), st, it.reshape(ni, npts)
     finally:
         shutil.rmtree(d)
@@ -443,11 +445,32 @@ def test_dc_lane_source_on_host_vs_oracle(oracle):
         assert st[0] == 0 and max(nerr(x[0][p], xr[p]) for p in range(9)) < 1e-12
 
 
-def test_dc_lane_source_declines_and_flags():
+def test_dc_lane_source_diodes_on_host_vs_oracle(oracle):
+    """Circuits with diodes through krets' Newton semantics (empty first map,
+    0.5 V default, the step key of sweeps): the diode I-V sweep and the
+    rectifier's operating point, with the oracle's iteration counts."""
     import krets_amd as kg
+    text = read_circuit("diode_iv_curve/diode_iv_curve.cir")
+    c = kg.Circuit(text)
+    r, src = dc_source(c, c.params)
+    assert r > 0, src
+    x, st, it = dc_host_run(src, c.params[None], c.elements, c.n, 0, 0.0, 0.25, 9)
+    oc = oracle.Circuit(text)
+    xr = oc.dc("V1", 0.0, 2.0, 0.25)
+    assert st[0] == 0 and max(nerr(x[0][p], xr[p]) for p in range(9)) < 1e-9
+    x, st, it = dc_host_run(src, c.params[None], c.elements, c.n, -1, 0.0, 0.0, 1)
+    xo, ito = oc.op()
+    assert st[0] == 0 and nerr(x[0][0], xo) < 1e-9 and it[0][0] == ito
     c = kg.Circuit(read_circuit("rectifier/rectifier.cir"))
-    r, why = dc_source(c, c.params)
-    assert r < 0 and "nonlinear" in why
+    r, src = dc_source(c, c.params)
+    assert r > 0, src
+    x, st, it = dc_host_run(src, c.params[None], c.elements, c.n, -1, 0.0, 0.0, 1)
+    xo, ito = oracle.Circuit(read_circuit("rectifier/rectifier.cir")).op()
+    assert st[0] == 0 and nerr(x[0][0], xo) < 1e-9 and it[0][0] == ito
+
+
+def test_dc_lane_source_flags_singular():
+    import krets_amd as kg
     # a zero resistor's conductance is infinite: a non-finite pivot (Decomposition)
     c = kg.Circuit("V1 1 0 1\nR1 1 2 0\nR2 2 0 1k\n")
     r, src = dc_source(c, c.params)
@@ -457,26 +480,27 @@ def test_dc_lane_source_declines_and_flags():
 
 
 def test_dc_lane_source_builds_for_gfx950():
-    """The 50-section ladder's operating-point kernel builds for gfx950 without scratch."""
+    """The operating-point kernels of the 50-section ladder (linear) and of the
+    rectifier (Newton) build for gfx950 without scratch."""
     import krets_amd as kg
     from krets_amd import workloads as W
     if not os.path.exists(HIPCC):
         pytest.skip("hipcc not installed")
     b = W.c2_rc_ladder_mc(1, sections=50, seed=2)
-    c = kg.Circuit(b.netlists[0])
-    r, src = dc_source(c, c.params)
-    assert r > 0, src
-    d = tempfile.mkdtemp()
-    try:
-        p = os.path.join(d, "k.hip")
-        open(p, "w").write("#include <hip/hip_runtime.h>\n" + src)
-        out = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
-                              "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", p, "-o", p + ".o",
-                              "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
-        assert out.returncode == 0, out.stderr[-2000:]
-        assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
-    finally:
-        shutil.rmtree(d)
+    for c in (kg.Circuit(b.netlists[0]), kg.Circuit(read_circuit("rectifier/rectifier.cir"))):
+        r, src = dc_source(c, c.params)
+        assert r > 0, src
+        d = tempfile.mkdtemp()
+        try:
+            p = os.path.join(d, "k.hip")
+            open(p, "w").write("#include <hip/hip_runtime.h>\n" + src)
+            out = subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                  "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", p, "-o", p + ".o",
+                                  "-Rpass-analysis=kernel-resource-usage"], capture_output=True, text=True)
+            assert out.returncode == 0, out.stderr[-2000:]
+            assert int(re.search(r"ScratchSize \[bytes/lane\]: (\d+)", out.stderr).group(1)) == 0
+        finally:
+            shutil.rmtree(d)
 
 
 # ------------------------------------------------- fixed-step transient lane kernel
