@@ -116,7 +116,13 @@ __device__ __forceinline__ i64 xcd_block() {
 
 // refinement passes load x as well as element values: a shorter distance
 // keeps the values in flight within the register budget
-int LOOKAHEAD_REFINE = 4;
+#ifndef KG_LANE_AHEAD_REFINE
+#define KG_LANE_AHEAD_REFINE 4
+#endif
+#ifndef KG_LANE_AHEAD
+#define KG_LANE_AHEAD 8
+#endif
+int LOOKAHEAD_REFINE = KG_LANE_AHEAD_REFINE;
 
 struct Gen {
   const View& P;
@@ -129,7 +135,7 @@ struct Gen {
   // mixed precision with the solution split as x = x0 + dx: x0 (the complex64
   // first solution) in LDS, dx (complex64 sum of the corrections) in registers
   bool xsplit = false;
-  int lookahead = 8;              // rows between a load and the row that first uses it
+  int lookahead = KG_LANE_AHEAD;  // rows between a load and the row that first uses it
   std::function<void(std::ostringstream&, int)> on_result;
   explicit Gen(const View& v) : P(v) {}
 
