@@ -74,6 +74,12 @@ __device__ __forceinline__ Z cz(Z a) { return a; }
 __device__ __forceinline__ Z cz(W a) { return {(double)a.re, (double)a.im}; }
 __device__ __forceinline__ W cw(Z a) { return {(float)a.re, (float)a.im}; }
 __device__ __forceinline__ Z scal(double s, Z a) { return {s * a.re, s * a.im}; }
+// product with a factor known to be real (a stamp with no imaginary part that
+// no elimination step has changed): two multiplications where the complex
+// product spends four plus two additions (IEEE arithmetic keeps its
+// 0 * x.im terms)
+__device__ __forceinline__ Z rmul(double a, Z x) { return {a * x.re, a * x.im}; }
+__device__ __forceinline__ W rmul(float a, W x) { return {a * x.re, a * x.im}; }
 __device__ __forceinline__ double m2(Z a) { return a.re * a.re + a.im * a.im; }
 __device__ __forceinline__ bool fin(Z a) { return __builtin_isfinite(a.re) && __builtin_isfinite(a.im); }
 // reciprocal of a pivot, conj(b) / |b|^2, without branches: an FP64 pivot is
@@ -91,12 +97,11 @@ __device__ __forceinline__ Z rcpF(Z b) {
   return {__builtin_amdgcn_ldexp(re * q, -k), __builtin_amdgcn_ldexp(-im * q, -k)};
 }
 // complex64 pivots: |b|^2 in FP64 (cannot leave its range), reciprocal from
-// the hardware estimate refined by two Newton steps (error far below the
-// FP32 rounding that follows)
+// the hardware estimate (relative error below 2^-22) refined by one Newton
+// step (error below 2^-44, far below the FP32 rounding that follows)
 __device__ __forceinline__ W rcpF(W b) {
   const double re = b.re, im = b.im, m = re * re + im * im;
   double q = __builtin_amdgcn_rcp(m);
-  q = q * (2.0 - m * q);
   q = q * (2.0 - m * q);
   return {(float)(re * q), (float)(-im * q)};
 }
@@ -213,6 +218,11 @@ struct Gen {
     return F == "Z" ? zval : "cw(" + zval + ")";
   }
 
+  // a * b, with a's real part only when a is known to be real
+  static std::string mul(bool a_real, const std::string& a, const std::string& b) {
+    return a_real ? "rmul(" + a + ".re, " + b + ")" : a + " * " + b;
+  }
+
   // FP64 x_j for a residual: a register, or (xsplit) x0_j read from LDS ahead
   // of its row plus dx_j
   std::string xref(int j) {
@@ -234,6 +244,9 @@ struct Gen {
     std::vector<std::string> code(n);
     std::vector<std::vector<std::string>> rload(n);
     std::vector<char> yz(n, 1);  // y'_i structurally zero
+    // slot value has no imaginary part: its stamp is real and no elimination
+    // step has updated it yet (products with it take the real form)
+    std::vector<char> real(P.nnz, 0);
     for (int i = 0; i < n; i++) {
       std::ostringstream o;
       loads.clear();
@@ -241,6 +254,7 @@ struct Gen {
       for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
         std::string re, im;
         if (!slot(s, re, im)) return false;
+        real[s] = im.empty();
         if (mode == 1) {
           if (has_slot(s)) o << "    const Z a" << s << " = " << zexpr(re, im) << ";\n";
           o << "    " << F << " v" << s << " = "
@@ -262,7 +276,8 @@ struct Gen {
       } else {
         o << "    Z r" << i << " = " << (b.empty() ? std::string("Z{0.0, 0.0}") : "Z{" + b + ", 0.0}") << ";\n";
         for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++)
-          if (has_slot(s)) o << "    r" << i << " = r" << i << " - a" << s << " * " << xref(P.col[s]) << ";\n";
+          if (has_slot(s))
+            o << "    r" << i << " = r" << i << " - " << mul(real[s], "a" + std::to_string(s), xref(P.col[s])) << ";\n";
         o << "    " << F << " y" << i << " = " << conv(F, "scal(inv_s, r" + std::to_string(i) + ")") << ";\n";
         ynz = true;
       }
@@ -275,22 +290,30 @@ struct Gen {
         for (int u = 0; u < nu; u++, fp += 2) {
           int dst = fp[0], src = fp[1];
           if (!(src > P.diag[k] && src < P.row_ptr[k + 1])) { why = "factor program reads outside U"; return false; }
-          o << "    v" << dst << " = v" << dst << " - v" << sik << " * u" << src << ";\n";
+          o << "    v" << dst << " = v" << dst << " - " << mul(real[sik], "v" + std::to_string(sik), "u" + std::to_string(src))
+            << ";\n";
+          real[dst] = 0;
         }
         if (!yz[k]) {
-          if (!ynz) o << "    " << F << " y" << i << " = " << F << "{0, 0} - v" << sik << " * y" << k << "s;\n";
-          else o << "    y" << i << " = y" << i << " - v" << sik << " * y" << k << "s;\n";
+          const std::string pr = mul(real[sik], "v" + std::to_string(sik), "y" + std::to_string(k) + "s");
+          if (!ynz) o << "    " << F << " y" << i << " = " << F << "{0, 0} - " << pr << ";\n";
+          else o << "    y" << i << " = y" << i << " - " << pr << ";\n";
           ynz = true;
         }
       }
       // pivot: decomposition check as the generic kernels (|piv| > 0, finite);
       // refinement passes repeat the first pass' factorisation exactly, so
-      // only the first pass checks
+      // only the first pass checks every pivot; they repeat the check of
+      // their last pivot, whose only effect is on register allocation (with
+      // no check at all in the refinement passes, the mixed kernel of the
+      // 50-section RC ladder takes 512 registers and 596 B of scratch per
+      // lane instead of 360 registers)
       int d = P.diag[i];
-      o << "    { const Z pz = cz(v" << d << "); bad |= !(m2(pz) > 0.0) || !fin(pz); }\n";
+      if (mode == 0 || i == n - 1) o <<"    { const Z pz = cz(v" << d << "); bad |= !(m2(pz) > 0.0) || !fin(pz); }\n";
       o << "    const " << F << " rd" << i << " = rcpF(v" << d << ");\n";
       for (int s = d + 1; s < P.row_ptr[i + 1]; s++)
-        o << "    const " << F << " u" << s << " = v" << s << " * rd" << i << ";\n";
+        o << "    const " << F << " u" << s << " = " << mul(real[s], "v" + std::to_string(s), "rd" + std::to_string(i))
+          << ";\n";
       if (ynz) {
         o << "    const " << F << " y" << i << "s = y" << i << " * rd" << i << ";\n";
         yz[i] = 0;
