@@ -67,6 +67,10 @@ struct kg_ctx {
   std::vector<int> group_of, group_first;
   double* d_params = nullptr;
   double* d_ebase = nullptr;
+  // shared topologies: the same values in blocks of 64 instances, element
+  // by element (entry (i / 64 * ne + e) * 64 + i % 64), read by the mixed
+  // precision lane AC kernels, whose waves hold 64 consecutive instances
+  double* d_ebase_t = nullptr;
   int64_t* d_eoff = nullptr;
   int32_t* d_nunk = nullptr;
   // plans
@@ -137,9 +141,10 @@ void* ws_get(kg_ctx* c, const std::string& name, size_t bytes, hipError_t* e) {
 }
 
 void free_batch(kg_ctx* c) {
-  for (auto* p : {(void*)c->d_params, (void*)c->d_ebase, (void*)c->d_eoff, (void*)c->d_nunk, (void*)c->d_pats})
+  for (auto* p : {(void*)c->d_params, (void*)c->d_ebase, (void*)c->d_ebase_t, (void*)c->d_eoff, (void*)c->d_nunk,
+                  (void*)c->d_pats})
     if (p) hipFree(p);
-  c->d_params = c->d_ebase = nullptr;
+  c->d_params = c->d_ebase = c->d_ebase_t = nullptr;
   c->d_eoff = nullptr;
   c->d_nunk = nullptr;
   c->d_pats = nullptr;
@@ -1136,6 +1141,14 @@ int32_t kg_load_batch(kg_ctx* c, const kg_batch_desc* b) {
   HIPCHK(c, hipMalloc(&c->d_nunk, sizeof(int32_t) * c->nunk.size()));
   HIPCHK(c, hipMemcpy(c->d_params, c->params.data(), sizeof(double) * c->params.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_ebase, c->ebase.data(), sizeof(double) * c->ebase.size(), hipMemcpyHostToDevice));
+  if (c->shared) {
+    const int64_t ne = b->n_elements;
+    std::vector<double> t((size_t)((ni + 63) / 64) * 64 * ne, 0.0);
+    for (int64_t i = 0; i < ni; i++)
+      for (int64_t e = 0; e < ne; e++) t[(i / 64 * ne + e) * 64 + i % 64] = c->ebase[i * ne + e];
+    HIPCHK(c, hipMalloc(&c->d_ebase_t, sizeof(double) * std::max<size_t>(t.size(), 1)));
+    HIPCHK(c, hipMemcpy(c->d_ebase_t, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(c, hipMemcpy(c->d_eoff, c->eoff.data(), sizeof(int64_t) * c->eoff.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_nunk, c->nunk.data(), sizeof(int32_t) * c->nunk.size(), hipMemcpyHostToDevice));
   return KG_OK;
@@ -1305,7 +1318,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   auto run = [&](const AcLaunch& A) -> hipError_t {
     if (A.lane) {
       const double* prm = c->d_params;
-      const double* eb = c->d_ebase;
+      const double* eb = mixed ? c->d_ebase_t : c->d_ebase;  // mixed: blocks of 64 instances
       const double* fr = d_f;
       int nfa = nf;
       int64_t ns = nsys;

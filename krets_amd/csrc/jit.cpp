@@ -127,6 +127,14 @@ __device__ __forceinline__ i64 xcd_block() {
 #ifndef KG_LANE_AHEAD
 #define KG_LANE_AHEAD 8
 #endif
+// rows between scheduling barriers in the forward (factor and forward
+// substitution) and backward sweeps; 0: none
+#ifndef KG_LANE_FWD_BARRIER
+#define KG_LANE_FWD_BARRIER 1
+#endif
+#ifndef KG_LANE_BWD_BARRIER
+#define KG_LANE_BWD_BARRIER 4
+#endif
 int LOOKAHEAD_REFINE = KG_LANE_AHEAD_REFINE;
 
 struct Gen {
@@ -140,6 +148,10 @@ struct Gen {
   // mixed precision with the solution split as x = x0 + dx: x0 (the complex64
   // first solution) in LDS, dx (complex64 sum of the corrections) in registers
   bool xsplit = false;
+  // element values from the table blocked by 64 instances (entry
+  // (inst / 64 * ne + e) * 64 + inst % 64), so that a wave of consecutive
+  // instances reads one contiguous run per value
+  bool emajor = false;
   int lookahead = KG_LANE_AHEAD;  // rows between a load and the row that first uses it
   std::function<void(std::ostringstream&, int)> on_result;
   explicit Gen(const View& v) : P(v) {}
@@ -150,7 +162,13 @@ struct Gen {
   // AC magnitude of voltage sources is read from the parameter records.
   std::string g(int e) {
     std::string nm = "g" + std::to_string(e);
-    if (have_g.insert(e).second) loads.push_back("    const double " + nm + " = EB[" + std::to_string(e) + "];\n");
+    if (have_g.insert(e).second) {
+      if (emajor)  // buffer load: the wave's block (scalar), the lane's offset, a constant per element
+        loads.push_back("    const double " + nm + " = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(EBR, jl8, " +
+                        std::to_string(512 * e) + ", 0));\n");
+      else
+        loads.push_back("    const double " + nm + " = EB[" + std::to_string(e) + "];\n");
+    }
     return nm;
   }
   std::string p(int e, int k) {  // params[e][k]
@@ -332,7 +350,7 @@ struct Gen {
     for (int i = 0; i < n; i++) {
       issue(i + lookahead);
       O << code[i];
-      O << "    __builtin_amdgcn_sched_barrier(0);\n";
+      if (KG_LANE_FWD_BARRIER > 0 && (i + 1) % KG_LANE_FWD_BARRIER == 0) O << "    __builtin_amdgcn_sched_barrier(0);\n";
     }
     // back substitution
     for (int i = n - 1; i >= 0; i--) {
@@ -341,7 +359,7 @@ struct Gen {
       for (int s = P.diag[i] + 1; s < P.row_ptr[i + 1]; s++)
         O << "    t" << i << " = t" << i << " - u" << s << " * t" << P.col[s] << ";\n";
       on_result(O, i);
-      if (i % 4 == 0) O << "    __builtin_amdgcn_sched_barrier(0);\n";
+      if (KG_LANE_BWD_BARRIER > 0 && i % KG_LANE_BWD_BARRIER == 0) O << "    __builtin_amdgcn_sched_barrier(0);\n";
     }
     return true;
   }
@@ -370,6 +388,7 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
   if (regs > 440) { why = "per-lane state exceeds the register budget"; return false; }
   Gen G(P);
   G.xsplit = mixed && xsplit;
+  G.emajor = mixed;
   std::ostringstream& o = G.out;
   o << kPrelude;
   o << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_ac_name(mixed, xsplit)
@@ -390,8 +409,17 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
          "  const int nbi = (int)(ni - ib * 64 < 64 ? ni - ib * 64 : 64);\n"
          "  const int fi = (int)(rem / nbi), inst = (int)(ib * 64 + rem % nbi);\n";
   }
-  o << "  const double* EB = ebase + (i64)inst * " << P.ne << ";\n"
-    << "  const double* PR = prm + (i64)inst * " << (int64_t)P.ne * KG_NPARAM << ";\n"
+  if (mixed) {
+    // ib is uniform across the wave (64 * nf consecutive systems per group of
+    // 64 instances): the group's block of values as a buffer resource
+    // (descriptor word 3 of gfx9 raw buffers; reads past the block return 0)
+    o << "  const __amdgpu_buffer_rsrc_t EBR = __builtin_amdgcn_make_buffer_rsrc((void*)(ebase + "
+         "(i64)__builtin_amdgcn_readfirstlane((int)ib) * "
+      << 64 * P.ne << "), (short)0, " << 512 * P.ne << ", 0x00020000);\n"
+      << "  const int jl8 = (inst - (int)ib * 64) * 8;\n";
+  }
+  else o << "  const double* EB = ebase + (i64)inst * " << P.ne << ";\n";
+  o << "  const double* PR = prm + (i64)inst * " << (int64_t)P.ne * KG_NPARAM << ";\n"
     << "  const double w = (2.0 * 3.14159265358979323846) * freqs[fi];\n"
     << "  Z* o = reinterpret_cast<Z*>(out) + ((i64)inst * nf + fi) * " << n << ";\n"
     << "  bool bad = false;\n";

@@ -46,11 +46,21 @@ static inline unsigned long long atomicMax(unsigned long long* p, unsigned long 
   unsigned long long o = *p; if (v > o) *p = v; return o; }
 static inline long long __double_as_longlong(double d) { long long r; std::memcpy(&r, &d, 8); return r; }
 using std::fmax; using std::sqrt;
+// raw buffer loads of the mixed precision kernels, bounds-checked on the host
+struct kg_rsrc { const char* p; long long n; };
+typedef kg_rsrc __amdgpu_buffer_rsrc_t;
+static int kg_oob = 0;
+static inline unsigned long long kg_bload(kg_rsrc r, int v, int s) {
+  if (v < 0 || s < 0 || (long long)v + s + 8 > r.n) { kg_oob++; return 0; }
+  unsigned long long x; std::memcpy(&x, r.p + v + s, 8); return x; }
+#define __builtin_amdgcn_make_buffer_rsrc(p, st, n, f) (kg_rsrc{(const char*)(p), (long long)(n)})
+#define __builtin_amdgcn_raw_buffer_load_b64(r, v, s, a) kg_bload((r), (v), (s))
+#define __builtin_amdgcn_readfirstlane(x) (x)
 """
 
 HOST_DRIVER = r"""
 extern "C" void run_host(const double* prm, const double* eb, const double* fr, int nf, long long nsys,
-                         double* out, int* st, double* err, int refine, double stop) {
+                         double* out, int* st, double* err, int refine, double stop, int* oob) {
   gridDim.x = (unsigned)((nsys + 63) / 64);
   for (unsigned b = 0; b < gridDim.x; b++)
     for (unsigned t = 0; t < 64; t++) {
@@ -58,6 +68,7 @@ extern "C" void run_host(const double* prm, const double* eb, const double* fr, 
       threadIdx.x = t;
       KERNEL(prm, eb, fr, nf, nsys, out, st, err, refine, stop);
     }
+  *oob = kg_oob;
 }
 """
 
@@ -95,15 +106,23 @@ def host_run(src, params, elements, freqs, n, refine=2, stop=0.0):
         L = ctypes.CDLL(so)
         ni, nf = params.shape[0], len(freqs)
         prm = np.ascontiguousarray(params, np.float64)
-        eb = np.ascontiguousarray(ebase(elements, params), np.float64)
+        eb = ebase(elements, params)
+        if "c64" in name:  # the mixed precision kernels read the table in blocks of 64 instances
+            nb = (ni + 63) // 64
+            t = np.zeros((nb * 64, eb.shape[1]))
+            t[:ni] = eb
+            eb = t.reshape(nb, 64, -1).transpose(0, 2, 1)
+        eb = np.ascontiguousarray(eb, np.float64)
         fr = np.ascontiguousarray(freqs, np.float64)
         out = np.zeros(ni * nf * n * 2)
         st = np.zeros(ni, np.int32)
         err = np.zeros(ni)
         P = ctypes.c_void_p
-        L.run_host.argtypes = [P, P, P, ctypes.c_int, ctypes.c_longlong, P, P, P, ctypes.c_int, ctypes.c_double]
+        L.run_host.argtypes = [P, P, P, ctypes.c_int, ctypes.c_longlong, P, P, P, ctypes.c_int, ctypes.c_double, P]
+        oob = np.zeros(1, np.int32)
         L.run_host(prm.ctypes.data, eb.ctypes.data, fr.ctypes.data, nf, ni * nf, out.ctypes.data, st.ctypes.data,
-                   err.ctypes.data, refine, stop)
+                   err.ctypes.data, refine, stop, oob.ctypes.data)
+        assert oob[0] == 0, "buffer load outside the instance group's block"
         x = out.reshape(ni, nf, n, 2)
         return x[..., 0] + 1j * x[..., 1], st, err
     finally:
