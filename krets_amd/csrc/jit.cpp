@@ -81,6 +81,11 @@ __device__ __forceinline__ Z scal(double s, Z a) { return {s * a.re, s * a.im}; 
 __device__ __forceinline__ Z rmul(double a, Z x) { return {a * x.re, a * x.im}; }
 __device__ __forceinline__ W rmul(float a, W x) { return {a * x.re, a * x.im}; }
 __device__ __forceinline__ double m2(Z a) { return a.re * a.re + a.im * a.im; }
+// running max norm over components, max(|re|, |im|), of complex64 values: one
+// instruction per value, and no squares that could leave the FP32 range
+__device__ __forceinline__ float cmax(float m, W a) {
+  return __builtin_fmaxf(m, __builtin_fmaxf(__builtin_fabsf(a.re), __builtin_fabsf(a.im)));
+}
 __device__ __forceinline__ bool fin(Z a) { return __builtin_isfinite(a.re) && __builtin_isfinite(a.im); }
 // reciprocal of a pivot, conj(b) / |b|^2, without branches: an FP64 pivot is
 // first scaled by the power of two of its larger component (exact), so
@@ -96,13 +101,14 @@ __device__ __forceinline__ Z rcpF(Z b) {
   q = q * (2.0 - m * q);
   return {__builtin_amdgcn_ldexp(re * q, -k), __builtin_amdgcn_ldexp(-im * q, -k)};
 }
-// complex64 pivots: |b|^2 in FP64 (cannot leave its range), reciprocal from
-// the hardware estimate (relative error below 2^-22) refined by one Newton
-// step (error below 2^-44, far below the FP32 rounding that follows)
+// complex64 pivots: |b|^2 in FP64 (cannot leave its range) and the hardware
+// reciprocal estimate as it is: its relative error (measured on gfx950: at
+// most 4.6e-8, 2^-24.4) is that of the FP32 rounding that follows, and the
+// refinement corrects the complex64 factorisation anyway (a Newton step
+// costs two more dependent instructions per pivot and 1.5% of the kernel)
 __device__ __forceinline__ W rcpF(W b) {
   const double re = b.re, im = b.im, m = re * re + im * im;
-  double q = __builtin_amdgcn_rcp(m);
-  q = q * (2.0 - m * q);
+  const double q = __builtin_amdgcn_rcp(m);
   return {(float)(re * q), (float)(-im * q)};
 }
 // block order that keeps consecutive systems (one instance) on one XCD
@@ -435,14 +441,14 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     } else {
       for (int i = 0; i < n; i++) o << "  Z x" << i << ";\n";
     }
-    o << "  double est = 0.0, d2prev = 0.0, d2last = 0.0, bm2 = 0.0, xm2 = 0.0;\n  int npass = 0;\n";
+    o << "  double est = 0.0, d2prev = 0.0, d2last = 0.0, bm2 = 0.0;\n  float xmf = 0.0f;\n  int npass = 0;\n";
     // pass 0: complex64 solve of b; x = its FP64 copy (xsplit: x0 = it, dx = 0)
     // |x|_inf is taken from this first solution (the refinement changes it by
     // the FP32 solve error only); tracking it through the refinement passes
     // makes the compiler keep far more values live
     G.on_result = [&](std::ostringstream& O, int i) {
-      if (G.xsplit) O << "    xs[" << 64 * i << " + lane] = t" << i << "; xm2 = fmax(xm2, m2(cz(t" << i << ")));\n";
-      else O << "    x" << i << " = cz(t" << i << "); xm2 = fmax(xm2, m2(x" << i << "));\n";
+      if (G.xsplit) O << "    xs[" << 64 * i << " + lane] = t" << i << "; xmf = cmax(xmf, t" << i << ");\n";
+      else O << "    x" << i << " = cz(t" << i << "); xmf = cmax(xmf, t" << i << ");\n";
     };
     G.track_bm = true;
     o << "  {\n";
@@ -450,7 +456,8 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     o << "  }\n";
     G.track_bm = false;
     // refinement: corrections scaled by |b|_inf stay in FP32 range
-    o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0;\n  const float sf = (float)s;\n";
+    o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0, xm2 = (double)xmf * (double)xmf;\n"
+         "  const float sf = (float)s;\n";
     // refinement passes after the first run inside a run-time loop: the pass
     // body is compiled once, so its registers are allocated once (an unrolled
     // sequence of passes lets live ranges of neighbouring passes overlap)
@@ -459,20 +466,20 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
       // precision dx is kept in; FP64 here costs registers, not accuracy)
       if (G.xsplit)
         O << "    { const W d = W{sf * t" << i << ".re, sf * t" << i << ".im}; dx" << i << " = W{dx" << i << ".re + d.re, dx"
-          << i << ".im + d.im}; dm2 = fmax(dm2, m2(cz(d))); }\n";
+          << i << ".im + d.im}; dmf = cmax(dmf, t" << i << "); }\n";
       else
-        O << "    { const Z d = scal(s, cz(t" << i << ")); x" << i << " = x" << i << " + d; dm2 = fmax(dm2, m2(d)); }\n";
+        O << "    x" << i << " = x" << i << " + scal(s, cz(t" << i << ")); dmf = cmax(dmf, t" << i << ");\n";
     };
     // the first pass is peeled off the loop: a wave-uniform branch around the
     // remaining passes (adaptive stop) leaves the loop itself unchanged,
     // while an exit inside it makes the compiler spill
-    o << "  if (s > 0.0 && refine > 0) {\n  {\n    double dm2 = 0.0;\n";
+    o << "  if (s > 0.0 && refine > 0) {\n  {\n    float dmf = 0.0f;\n";
     G.lookahead = LOOKAHEAD_REFINE;
     if (!G.pass("W", 1)) { why = G.why; return false; }
     // only the squared correction sizes of the last two passes are carried;
     // the estimate is formed after the loop (square roots and divisions in
     // the loop body make the compiler keep far more values live)
-    o << "    d2prev = d2last;\n    d2last = dm2;\n    npass = 1;\n  }\n";
+    o << "    d2prev = d2last;\n    d2last = s * (double)dmf * (s * (double)dmf);\n    npass = 1;\n  }\n";
     // adaptive stop (stop > 0): a wave leaves after the first pass once every
     // lane's predicted remaining error, the squared relative correction times
     // KG_REFINE_SAFETY (refinement contracts by about the relative size of
@@ -480,9 +487,9 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     o << "  if (refine > 1 && !(stop > 0.0 && __builtin_amdgcn_ballot_w64(" << KG_REFINE_SAFETY
       << " * d2last <= stop * xm2) == __builtin_amdgcn_read_exec()))\n"
          "#pragma clang loop unroll(disable)\n"
-         "  for (int it = 1; it < refine; it++) {\n    double dm2 = 0.0;\n";
+         "  for (int it = 1; it < refine; it++) {\n    float dmf = 0.0f;\n";
     if (!G.pass("W", 1)) { why = G.why; return false; }
-    o << "    d2prev = d2last;\n    d2last = dm2;\n    npass = it + 1;\n  }\n  }\n"
+    o << "    d2prev = d2last;\n    d2last = s * (double)dmf * (s * (double)dmf);\n    npass = it + 1;\n  }\n  }\n"
       << "  {\n    const double dm = sqrt(d2last), dp = sqrt(d2prev), xm = sqrt(xm2);\n"
       << "    const double rho = (npass > 1 && dp > 0.0) ? fmin(dm / dp, 1.0) : 1.0;\n"
       << "    est = xm > 0.0 ? (npass == 1 && stop > 0.0 ? " << KG_REFINE_SAFETY

@@ -23,8 +23,9 @@ struct JitFn {
 
 // Safety factor of the mixed precision kernel's adaptive stop: the error left
 // after a refinement pass is predicted as KG_REFINE_SAFETY * (|dx| / |x|)^2
-// (max norms). On the batched RC ladders the observed ratio of the true
-// error to (|dx| / |x|)^2 stays below 150 (median 2).
+// (max norms over the components, max_i max(|re_i|, |im_i|)). On the batched
+// RC ladders (2 x 16 instances x 200 frequencies) the observed ratio of the
+// true normwise error to that squared ratio stays below 130 (median 3).
 #define KG_REFINE_SAFETY 200.0
 // Default target of that prediction (the kernel's `stop` argument; 0 runs
 // every pass): two orders below the 1e-9 accuracy the mixed path promises.
