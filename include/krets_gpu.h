@@ -252,7 +252,10 @@ int32_t kg_plan_export(const kg_element* elements, int32_t n_elements, int32_t n
  * reactive scale s_rep; refine_iters < 0 selects the mixed precision form
  * with the solution split into a complex64 first solution in LDS and a
  * complex64 sum of corrections in registers; the refinement count is a
- * launch argument). No counterpart in krets. Returns the source length
+ * launch argument). The mixed precision forms read element values from the
+ * value table blocked by 64 instances (entry (i / 64 * ne + e) * 64 + i % 64)
+ * and run 64 consecutive instances at one frequency per wave. No
+ * counterpart in krets. Returns the source length
  * (copying at most cap bytes, NUL-terminated, to buf) or -status, with the
  * reason the topology is not specialised in buf. */
 int32_t kg_jit_ac_source(const kg_element* elements, int32_t n_elements, int32_t n_unknowns, const double* params,
