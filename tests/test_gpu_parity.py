@@ -210,6 +210,29 @@ def test_batched_ac_rc_ladder(kg, oracle, precision, tps):
         assert (err < 1e-8).all()  # a-posteriori estimate of the remaining refinement error
 
 
+@pytest.mark.parametrize("lane", [2001, 2002])
+def test_lane_ac_mixed_across_instance_blocks(kg, oracle, lane):
+    """The mixed lane kernels read element values from the table blocked by 64
+    instances: 130 instances are two full blocks and a partial one of 2, with
+    an odd frequency count. Every result is checked against the FP64 lane
+    kernel, and instances at the block edges against the oracle."""
+    from krets_amd import workloads as W
+    say(f"mixed lane AC across instance blocks tps={lane}")
+    b = W.c2_rc_ladder_mc(130, sections=50, seed=29)
+    f = W.c2_frequencies(200)[::12]
+    ctx = kg.Context()
+    b.load(ctx)
+    x64, st64, _ = ctx.ac_sweep(f, precision=0, config=kg.SolverConfig(threads_per_system=2001))
+    xm, st, err = ctx.ac_sweep(f, precision=1, config=kg.SolverConfig(threads_per_system=lane))
+    assert (st == 0).all() and (st64 == 0).all()
+    e = np.linalg.norm(xm - x64, axis=2) / np.linalg.norm(x64, axis=2)
+    assert e.max() < 1e-9, e.max()
+    assert (err <= 1e-9).all()
+    for i in (0, 63, 64, 127, 128, 129):
+        xr = oracle.Circuit(b.netlists[i]).ac(f)
+        assert max(nerr(xm[i][j], xr[j]) for j in range(len(f))) < 1e-9, i
+
+
 @pytest.mark.parametrize("refine", [1, 2, 3])
 @pytest.mark.parametrize("precision,lane", [(0, 2001), (1, 2001), (1, 2002), (0, 2011), (1, 2012)])
 def test_lane_ac_golden_and_rlc(kg, oracle, precision, lane, refine):

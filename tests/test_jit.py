@@ -156,6 +156,25 @@ def test_lane_source_on_host_vs_oracle(oracle, precision, refine, xsplit):
         assert (err > 0).all() and (err < (1e-4 if refine == 1 else 1e-8)).all()
 
 
+@pytest.mark.parametrize("xsplit", [False, True])
+def test_lane_source_mixed_across_instance_blocks_on_host(xsplit):
+    """70 instances: a full block of 64 in the blocked value table and a
+    partial one of 6. The host run bounds-checks every buffer load against
+    its block; the mixed results agree with the FP64 kernel's."""
+    import krets_amd as kg
+    from krets_amd import workloads as W
+    b = W.c2_rc_ladder_mc(70, sections=50, seed=31)
+    c = kg.Circuit(b.netlists[0])
+    f = W.c2_frequencies(200)[::40]
+    _, src64 = source(c, 0)
+    x64, _, _ = host_run(src64, b.params, c.elements, f, c.n)
+    _, srcm = source(c, 1, -1 if xsplit else 2)
+    xm, st, err = host_run(srcm, b.params, c.elements, f, c.n, refine=2)
+    assert (st == 0).all()
+    e = np.linalg.norm(xm - x64, axis=2) / np.linalg.norm(x64, axis=2)
+    assert e.max() < 1e-9, e.max()
+
+
 def test_lane_source_golden_ac_on_host():
     import json
     import krets_amd as kg
