@@ -31,6 +31,7 @@
 #include "refcpu.h"
 
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <complex>
@@ -1704,77 +1705,133 @@ int32_t ref_tran(void* c, const ref_config* cfg, double h, double stop, int32_t 
   });
 }
 
+// Backward Euler with divided-difference LTE step control. trace (may be
+// null) receives one row per attempted step: t, h, ratio (-1 without an
+// estimate), outcome (1 accepted, 0 LTE rejection, -1 Newton failure), next h.
+static void tran_lte_run(const Circuit& C, const ref_config& cfg, double tstop, double h0, double hmin, double hmax,
+                  double lte_reltol, double lte_abstol, double* times, double* x, int32_t maxpts, int32_t* npts,
+                  int32_t* newton_total, int32_t* rejected, std::vector<std::array<double, 5>>* trace) {
+  int n = N_of(C);
+  CleanState st;
+  st.ic.assign(C.el.size(), 0.0);
+  st.vl.assign(C.el.size(), 0.0);
+  std::vector<double> x0;
+  int nt = clean_newton(C, cfg, 0, 0.0, 1.0, std::vector<double>(n, 0.0), st, x0);
+  int rej = 0;
+  int cnt = 0;
+  auto push = [&](double t, const std::vector<double>& v) {
+    if (!times) {
+      cnt++;
+      return;
+    }
+    if (cnt >= maxpts) throw RefError{REF_UNEXPECTED, "output buffer too small"};
+    times[cnt] = t;
+    std::copy(v.begin(), v.end(), x + (size_t)cnt * n);
+    cnt++;
+  };
+  auto note = [&](double t, double h, double ratio, double outcome, double hn) {
+    if (trace) trace->push_back({t, h, ratio, outcome, hn});
+  };
+  push(0.0, x0);
+  double t = 0.0, h = h0, tprev = 0.0;
+  std::vector<double> xc = x0, xprev = x0, xn;
+  bool have_prev = false;
+  while (t < tstop * (1.0 - 1e-12)) {
+    h = std::min(h, hmax);
+    if (t + h > tstop) h = tstop - t;
+    int its;
+    try {
+      its = clean_newton(C, cfg, 1, t + h, h, xc, st, xn);
+    } catch (const RefError& e) {
+      if (e.code != REF_MAX_ITERATIONS) throw;
+      rej++;
+      note(t, h, -1.0, -1.0, h / 8.0);
+      h /= 8.0;
+      if (h < hmin) throw RefError{REF_MAX_ITERATIONS, "LTE: step below hmin after Newton failure"};
+      continue;
+    }
+    nt += its;
+    double hnext = h;
+    double ratio = -1.0;
+    if (have_prev) {
+      double hp = t - tprev;
+      ratio = 0.0;
+      for (const Elem& e : C.el) {
+        double a, b0, bm;
+        if (e.t == E_C) { a = vdiff(e, xn); b0 = vdiff(e, xc); bm = vdiff(e, xprev); }
+        else if (e.t == E_L) { a = xn[e.ic]; b0 = xc[e.ic]; bm = xprev[e.ic]; }
+        else continue;
+        double dd2 = 2.0 * ((a - b0) / h - (b0 - bm) / hp) / (h + hp);
+        double lte = 0.5 * h * h * std::fabs(dd2);
+        double tol = lte_reltol * std::max(std::fabs(a), std::fabs(b0)) + lte_abstol;
+        ratio = std::max(ratio, lte / tol);
+      }
+      if (ratio > 1.0) {
+        rej++;
+        double hr = h * std::max(0.25, 0.9 / std::sqrt(ratio));
+        note(t, h, ratio, 0.0, hr);
+        h = hr;
+        if (h < hmin) throw RefError{REF_MAX_ITERATIONS, "LTE: step below hmin"};
+        continue;
+      }
+      hnext = ratio > 0 ? h * std::min(2.0, std::max(0.25, 0.9 / std::sqrt(ratio))) : 2.0 * h;
+    }
+    note(t, h, ratio, 1.0, hnext);
+    tprev = t;
+    xprev = xc;
+    t += h;
+    xc = xn;
+    have_prev = true;
+    push(t, xc);
+    h = hnext;
+  }
+  *npts = cnt;
+  if (newton_total) *newton_total = nt;
+  if (rejected) *rejected = rej;
+}
+
 int32_t ref_tran_lte(void* c, const ref_config* cfg, double tstop, double h0, double hmin, double hmax,
                      double lte_reltol, double lte_abstol, double* times, double* x, int32_t maxpts,
                      int32_t* npts, int32_t* newton_total, int32_t* rejected) {
   return guarded([&] {
-    const Circuit& C = *as_c(c);
-    int n = N_of(C);
-    CleanState st;
-    st.ic.assign(C.el.size(), 0.0);
-    st.vl.assign(C.el.size(), 0.0);
-    std::vector<double> x0;
-    int nt = clean_newton(C, *cfg, 0, 0.0, 1.0, std::vector<double>(n, 0.0), st, x0);
-    int rej = 0;
-    int cnt = 0;
-    auto push = [&](double t, const std::vector<double>& v) {
-      if (cnt >= maxpts) throw RefError{REF_UNEXPECTED, "output buffer too small"};
-      times[cnt] = t;
-      std::copy(v.begin(), v.end(), x + (size_t)cnt * n);
-      cnt++;
-    };
-    push(0.0, x0);
-    double t = 0.0, h = h0, tprev = 0.0;
-    std::vector<double> xc = x0, xprev = x0, xn;
-    bool have_prev = false;
-    while (t < tstop * (1.0 - 1e-12)) {
-      h = std::min(h, hmax);
-      if (t + h > tstop) h = tstop - t;
-      int its;
-      try {
-        its = clean_newton(C, *cfg, 1, t + h, h, xc, st, xn);
-      } catch (const RefError& e) {
-        if (e.code != REF_MAX_ITERATIONS) throw;
-        rej++;
-        h /= 8.0;
-        if (h < hmin) throw RefError{REF_MAX_ITERATIONS, "LTE: step below hmin after Newton failure"};
-        continue;
-      }
-      nt += its;
-      double hnext = h;
-      if (have_prev) {
-        double hp = t - tprev;
-        double ratio = 0.0;
-        for (const Elem& e : C.el) {
-          double a, b0, bm;
-          if (e.t == E_C) { a = vdiff(e, xn); b0 = vdiff(e, xc); bm = vdiff(e, xprev); }
-          else if (e.t == E_L) { a = xn[e.ic]; b0 = xc[e.ic]; bm = xprev[e.ic]; }
-          else continue;
-          double dd2 = 2.0 * ((a - b0) / h - (b0 - bm) / hp) / (h + hp);
-          double lte = 0.5 * h * h * std::fabs(dd2);
-          double tol = lte_reltol * std::max(std::fabs(a), std::fabs(b0)) + lte_abstol;
-          ratio = std::max(ratio, lte / tol);
-        }
-        if (ratio > 1.0) {
-          rej++;
-          h *= std::max(0.25, 0.9 / std::sqrt(ratio));
-          if (h < hmin) throw RefError{REF_MAX_ITERATIONS, "LTE: step below hmin"};
-          continue;
-        }
-        hnext = ratio > 0 ? h * std::min(2.0, std::max(0.25, 0.9 / std::sqrt(ratio))) : 2.0 * h;
-      }
-      tprev = t;
-      xprev = xc;
-      t += h;
-      xc = xn;
-      have_prev = true;
-      push(t, xc);
-      h = hnext;
-    }
-    *npts = cnt;
-    if (newton_total) *newton_total = nt;
-    if (rejected) *rejected = rej;
+    tran_lte_run(*as_c(c), *cfg, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, times, x, maxpts, npts,
+                 newton_total, rejected, nullptr);
   });
+}
+
+int32_t ref_tran_lte_trace(void* c, const ref_config* cfg, double tstop, double h0, double hmin, double hmax,
+                           double lte_reltol, double lte_abstol, double* trace, int32_t maxtr, int32_t* ntr) {
+  return guarded([&] {
+    std::vector<std::array<double, 5>> tr;
+    int32_t npts = 0;
+    tran_lte_run(*as_c(c), *cfg, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, nullptr, nullptr, 0, &npts,
+                 nullptr, nullptr, &tr);
+    if ((int64_t)tr.size() > maxtr) throw RefError{REF_UNEXPECTED, "trace buffer too small"};
+    for (size_t k = 0; k < tr.size(); k++) std::copy(tr[k].begin(), tr[k].end(), trace + 5 * k);
+    *ntr = (int32_t)tr.size();
+  });
+}
+
+double ref_wave_value(int32_t kind, const double* p, double t) {
+  Elem e{};
+  e.t = E_V;
+  e.w.kind = kind;
+  std::copy(p, p + 7, e.w.p);
+  e.value = p[0];
+  return wave_value(e, t);
+}
+
+int32_t ref_parse_value(const char* tok, double* v) {
+  double out = 0.0;
+  if (!tok || !parse_value(tok, out)) return REF_PARSE;
+  *v = out;
+  return REF_OK;
+}
+
+int32_t ref_pnjlim(double vnew, double vold, double vt, double vcrit, double* out) {
+  bool lim = false;
+  *out = pnjlim(vnew, vold, vt, vcrit, lim);
+  return lim ? 1 : 0;
 }
 
 int32_t ref_tran_adjoint(void* c, const ref_config* cfg, double h, int32_t N, int32_t method,

@@ -95,6 +95,29 @@ int32_t ref_tran_lte(void* c, const ref_config* cfg, double tstop, double h0, do
 int32_t ref_tran_adjoint(void* c, const ref_config* cfg, double h, int32_t nsteps,
                          int32_t method, const double* cvec, double* J, double* grad);
 
+/* The LTE controller of ref_tran_lte with a record of every attempted step:
+ * trace is [maxtr][5] = (t at the step's start, h, ratio = the largest
+ * estimate / tolerance over the capacitors and inductors (-1 when the step has
+ * no estimate: the first step, or a Newton failure), outcome (1 accepted,
+ * 0 rejected by the estimate, -1 rejected after a Newton failure), the step
+ * size the controller tries next). *ntr receives the number of rows. */
+int32_t ref_tran_lte_trace(void* c, const ref_config* cfg, double tstop, double h0, double hmin,
+                           double hmax, double lte_reltol, double lte_abstol, double* trace,
+                           int32_t maxtr, int32_t* ntr);
+
+/* Single functions of the restatement, exposed so that tests can pin them on
+ * their own (no krets entry point corresponds):
+ * ref_wave_value: Pulse::value_at (kind 1, p = v1 v2 td tr tf pw per) and
+ *   Sinusoidal::value_at (kind 2, p = vo va f td theta phase_deg)
+ *   (voltage_source.rs:50-93); kind 0 returns p[0].
+ * ref_parse_value: utils.rs:32-64 parse_value on one token; REF_OK with *v,
+ *   or REF_PARSE.
+ * ref_pnjlim: SPICE3 junction-voltage limiting of the clean Newton loops;
+ *   *out receives the limited voltage; returns 1 if it limited, else 0. */
+double ref_wave_value(int32_t kind, const double* p, double t);
+int32_t ref_parse_value(const char* tok, double* v);
+int32_t ref_pnjlim(double vnew, double vold, double vt, double vcrit, double* out);
+
 /* Dense long-double LU with partial pivoting (n <= 256), real or complex.
  * A is row-major; complex arrays are interleaved. Returns 0 or 2 (singular). */
 int32_t ref_dense_solve(int32_t n, const double* A, const double* b, double* x);

@@ -59,6 +59,12 @@ def lib():
         L.ref_tran_lte.argtypes = [P, CFG, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D, I, IP,
                                    IP, IP]
+        L.ref_tran_lte_trace.argtypes = [P, CFG, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, D, I, IP]
+        L.ref_wave_value.restype = ctypes.c_double
+        L.ref_wave_value.argtypes = [I, D, ctypes.c_double]
+        L.ref_parse_value.argtypes = [C, D]
+        L.ref_pnjlim.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, D]
         L.ref_tran_adjoint.argtypes = [P, CFG, ctypes.c_double, I, I, D, D, D]
         L.ref_dense_solve.argtypes = [I, D, D, D]
         L.ref_dense_solve_c.argtypes = [I, D, D, D]
@@ -84,6 +90,30 @@ def ac_frequencies(variation, points, fstart, fstop):
     buf = np.zeros(100000)
     n = lib().ref_ac_frequencies(variation, points, fstart, fstop, _dp(buf), len(buf))
     return buf[:n].copy()
+
+
+def wave_value(kind, params, t):
+    """Pulse::value_at (kind 1: v1 v2 td tr tf pw per) or Sinusoidal::value_at
+    (kind 2: vo va f td theta phase_deg) of the oracle at time t."""
+    p = np.zeros(7)
+    p[:len(params)] = params
+    return lib().ref_wave_value(kind, _dp(p), float(t))
+
+
+def parse_value(tok):
+    """utils.rs parse_value of the oracle; raises OracleError(PARSE) on failure."""
+    v = ctypes.c_double()
+    r = lib().ref_parse_value(tok.encode(), ctypes.byref(v))
+    if r != OK:
+        raise OracleError(r, f"invalid value {tok!r}")
+    return v.value
+
+
+def pnjlim(vnew, vold, vt, vcrit):
+    """(limited voltage, whether it limited) of the oracle's SPICE3 pnjlim."""
+    out = ctypes.c_double()
+    lim = lib().ref_pnjlim(vnew, vold, vt, vcrit, ctypes.byref(out))
+    return out.value, bool(lim)
 
 
 def dense_solve(A, b):
@@ -192,6 +222,17 @@ class Circuit:
                                        ctypes.byref(nt), ctypes.byref(rej)))
         k = npts.value
         return t[:k].copy(), x[:k].copy(), nt.value, rej.value
+
+    def tran_lte_trace(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, cfg=None, maxtr=400000):
+        """Every attempted step of the LTE controller: rows (t, h, ratio,
+        outcome, next h); ratio -1 without an estimate, outcome 1 accepted,
+        0 rejected by the estimate, -1 rejected after a Newton failure."""
+        cfg = cfg or config()
+        tr = np.zeros((maxtr, 5))
+        ntr = ctypes.c_int32()
+        self._check(lib().ref_tran_lte_trace(self._h, ctypes.byref(cfg), tstop, h0, hmin, hmax, lte_reltol,
+                                             lte_abstol, _dp(tr), maxtr, ctypes.byref(ntr)))
+        return tr[:ntr.value].copy()
 
     def adjoint(self, h, nsteps, cvec, method=TRAP, cfg=None):
         cfg = cfg or config()

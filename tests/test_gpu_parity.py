@@ -478,8 +478,10 @@ def test_fixed_step_compat_rectifier_instances(kg, oracle, tps):
     for i in (0, 3, 7):
         xr, itr = oracle.Circuit(b.netlists[i]).tran(1e-6, 2e-3, cfg=oracle.config(vabstol=1e-9, iabstol=1e-12))
         assert np.array_equal(it[i], itr)
-        errs = [nerr(xs[i][k], xr[k]) for k in range(xr.shape[0])]
+        # normwise relative error with the norm floored at the voltage tolerance (1e-9 V): the operating
+        # point of a 0 V source is exactly 0, and both codes return rounding noise of ~1e-34 there
+        errs = [np.linalg.norm(xs[i][k] - xr[k]) / max(np.linalg.norm(xr[k]), 1e-9) for k in range(xr.shape[0])]
         k = int(np.argmax(errs))
         print(f"instance {i}: max normwise {errs[k]:.3e} at step {k}: gpu {xs[i][k]} oracle {xr[k]} "
               f"iters {it[i][max(k-2,0):k+2]}", flush=True)
-        assert errs[k] < 3e-9, (i, errs[k])
+        assert errs[k] < 1e-9, (i, errs[k])

@@ -90,6 +90,29 @@ def test_netlist_reader_quirks(kg, oracle):
         kg.Circuit("Q1 1 2 3 mod\n")
 
 
+# parse_value known answers of krets (crates/krets-parser/src/utils.rs:94-110), read through a
+# resistor line of the library's netlist reader and the oracle's
+SUFFIX_KAT = [("1.5k", 1500.0), ("10u", 10e-6), ("22n", 22e-9), ("1.2p", 1.2e-12), ("3MEG", 3e6), ("100", 100.0),
+              ("1e-6", 1e-6), ("1.23E-9", 1.23e-9), ("3meg", 3e6), ("4.7K", 4700.0), ("2f", 2e-15), ("1g", 1e9),
+              ("2T", 2e12), ("5m", 5e-3)]
+
+
+@pytest.mark.parametrize("tok,v", SUFFIX_KAT)
+def test_netlist_reader_value_suffixes(kg, oracle, tok, v):
+    c = same_circuit(kg, oracle, f"V1 1 0 1\nR1 1 0 {tok}\n")
+    assert abs(c.params[1, 0] - v) <= 1e-15 * max(1.0, abs(v))
+
+
+@pytest.mark.parametrize("tok", ["1.5x", "garbage", "1.5kk"])
+def test_netlist_reader_value_suffix_errors(kg, oracle, tok):
+    with pytest.raises(kg.KretsGpuError) as e:
+        kg.Circuit(f"V1 1 0 1\nR1 1 0 {tok}\n")
+    assert e.value.code == 7  # ParseError
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.Circuit(f"V1 1 0 1\nR1 1 0 {tok}\n")
+    assert e.value.code == oracle.PARSE
+
+
 def test_netlist_reader_controlled_sources(kg, oracle):
     """E, G, F, H lines (SPICE syntax): types, branch currents of E and H,
     controlling nodes and controlling branches, SI-suffixed gains, and the

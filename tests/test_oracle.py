@@ -366,3 +366,151 @@ def test_adjoint_with_controlled_sources_vs_fd(oracle):
             c.set_value(e, v)
             fd = (Jp - Jm) / (2 * d)
             assert abs(g[e] - fd) <= 1e-6 * abs(fd) + 1e-9 * abs(J) / v, (method, e, g[e], fd)
+
+
+# ------------------------------------------- single functions of the oracle
+# Pulse::value_at known answers: crates/krets-parser/src/elements/voltage_source.rs:369-423
+# (krets' tolerance 1e-9 absolute; kept).
+PULSE_KAT = [0.0, 5.0, 1e-6, 10e-9, 10e-9, 5e-6, 10e-6]
+
+
+@pytest.mark.parametrize("t,v", [(0.5e-6, 0.0), (1.005e-6, 2.5), (3.01e-6, 5.0), (6.015e-6, 2.5), (8.0e-6, 0.0),
+                                 (11.005e-6, 2.5)])
+def test_pulse_known_answers_of_krets(oracle, t, v):
+    assert abs(oracle.wave_value(1, PULSE_KAT, t) - v) < 1e-9
+
+
+# Sinusoidal::value_at known answers: voltage_source.rs:444-496 (offset 1, amplitude 5,
+# 1 kHz, delay 1 ms; the last with a 90 degree phase).
+@pytest.mark.parametrize("t,phase,v", [(0.5e-3, 0.0, 1.0), (1.25e-3, 0.0, 6.0), (1.5e-3, 0.0, 1.0),
+                                       (1.75e-3, 0.0, -4.0), (1e-3, 90.0, 6.0)])
+def test_sin_known_answers_of_krets(oracle, t, phase, v):
+    assert abs(oracle.wave_value(2, [1.0, 5.0, 1000.0, 1e-3, 0.0, phase], t) - v) < 1e-9
+
+
+def test_waveform_branches_beyond_krets_tests(oracle):
+    """Branches krets' own tests leave open, against hand-computed values."""
+    # damping: SIN(0 1 1k 1m 1000 90) one period after the delay: e^-1 * sin(2 pi + pi/2) = e^-1
+    assert abs(oracle.wave_value(2, [0.0, 1.0, 1000.0, 1e-3, 1000.0, 90.0], 2e-3) - 0.36787944117144233) < 1e-12
+    # a negative phase flips it; the offset adds
+    assert abs(oracle.wave_value(2, [0.25, 2.0, 1000.0, 1e-3, 1000.0, -90.0], 2e-3)
+               - (0.25 - 2 * 0.36787944117144233)) < 1e-12
+    # before the delay the offset holds even with damping and phase
+    assert oracle.wave_value(2, [0.25, 2.0, 1000.0, 1e-3, 1000.0, 90.0], 0.999e-3) == 0.25
+    # PULSE(1 3 2u 1u 4u 2u 20u): rise, hold, a quarter into the fall, low, and the same points
+    # three periods later (fmod wrap); fall from 3 to 1 over 4 us
+    p = [1.0, 3.0, 2e-6, 1e-6, 4e-6, 2e-6, 20e-6]
+    for k in range(4):
+        s = k * 20e-6
+        assert abs(oracle.wave_value(1, p, s + 2.5e-6) - 2.0) < 1e-9
+        assert oracle.wave_value(1, p, s + 4e-6) == 3.0
+        assert abs(oracle.wave_value(1, p, s + 6e-6) - 2.5) < 1e-9
+        assert oracle.wave_value(1, p, s + 15e-6) == 1.0
+    assert oracle.wave_value(1, p, 1e-6) == 1.0  # before the delay
+    # period 0: fmod gives NaN, every comparison fails and the pulse returns v1 (finding of the survey)
+    assert oracle.wave_value(1, [1.0, 3.0, 0.0, 1e-6, 1e-6, 1e-6, 0.0], 0.5e-6) == 1.0
+
+
+# parse_value known answers: crates/krets-parser/src/utils.rs:94-110 (tolerance 1e-15 there)
+@pytest.mark.parametrize("tok,v", [("1.5k", 1500.0), ("10u", 10e-6), ("22n", 22e-9), ("1.2p", 1.2e-12),
+                                   ("3MEG", 3e6), ("100", 100.0), ("1e-6", 1e-6), ("1.23E-9", 1.23e-9),
+                                   ("3meg", 3e6), ("4.7K", 4700.0), ("2f", 2e-15), ("1g", 1e9), ("2T", 2e12),
+                                   ("5m", 5e-3)])
+def test_parse_value_known_answers_of_krets(oracle, tok, v):
+    assert abs(oracle.parse_value(tok) - v) <= 1e-15 * max(1.0, abs(v))
+
+
+@pytest.mark.parametrize("tok", ["1.5x", "garbage", "", "k", "1.5kk"])
+def test_parse_value_errors(oracle, tok):
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.parse_value(tok)
+    assert e.value.code == oracle.PARSE
+
+
+def test_pnjlim_hand_computed(oracle):
+    """SPICE3 pnjlim (Nagel 1975; SPICE3 devsup.c DEVpnjlim), vt = 25 mV, vcrit = 0.6 V."""
+    vt, vc = 0.025, 0.6
+    v, lim = oracle.pnjlim(1.0, 0.7, vt, vc)  # vold > 0: vold + vt ln(1 + (vnew - vold)/vt) = 0.7 + vt ln 13
+    assert lim and abs(v - 0.7641237339365383) < 1e-15
+    v, lim = oracle.pnjlim(1.0, -0.1, vt, vc)  # vold <= 0: vt ln(vnew / vt) = vt ln 40
+    assert lim and abs(v - 0.09222198635284841) < 1e-15
+    v, lim = oracle.pnjlim(0.7, 0.9, vt, vc)  # arg = 1 - 8 <= 0: vcrit
+    assert lim and v == 0.6
+    assert oracle.pnjlim(0.55, 0.1, vt, vc) == (0.55, False)  # below vcrit
+    assert oracle.pnjlim(0.74, 0.7, vt, vc) == (0.74, False)  # within 2 vt of the old value
+    assert oracle.pnjlim(-3.0, 0.7, vt, vc) == (-3.0, False)  # reverse bias is not limited
+
+
+def rc_sine_response(t, tau, w):
+    """Closed form of tau x' + x = sin(w t), x(0) = 0 (the RC low-pass driven
+    by a sine): x = A sin(w t - phi) + A sin(phi) e^{-t/tau}, A = 1/sqrt(1+(w tau)^2),
+    phi = atan(w tau). Returns x, x' and a bound on |x''| near t."""
+    A, phi = 1.0 / math.sqrt(1.0 + (w * tau) ** 2), math.atan(w * tau)
+    x = A * np.sin(w * t - phi) + A * math.sin(phi) * np.exp(-t / tau)
+    x1 = w * A * np.cos(w * t - phi) - A * math.sin(phi) / tau * np.exp(-t / tau)
+    x2max = w * w * A + A * math.sin(phi) / tau ** 2 * np.exp(-t / tau)
+    return x, x1, x2max
+
+
+def test_lte_controller_against_closed_form(oracle):
+    """Every attempted step of the LTE-controlled backward Euler on an RC
+    low-pass driven by a sine, against the closed-form solution.
+
+    The controller estimates the local truncation error h^2/2 |x''| by the
+    second divided difference of the last three states. Backward Euler's
+    difference quotients (x_n - x_{n-1})/h_n equal the ODE's derivative at
+    the step's end, x'(t_n) (up to the global error), so the estimate on this
+    circuit is h^2/2 * |2 (x'(t+h) - x'(t)) / (h + h_prev)|, computed here
+    from the closed form's x'. It must agree to 5% of the estimate that the
+    local amplitude of x'' gives; a factor 2 anywhere in the estimate or the
+    tolerance fails that. A step is accepted exactly when its ratio
+    (estimate / tolerance) is at most 1; a rejected step retries with
+    h * max(1/4, 0.9/sqrt(ratio)), an accepted one proposes
+    h * clamp(0.9/sqrt(ratio), 1/4, 2), both capped by hmax and the stop time."""
+    tau, f = 1e-4, 1000.0
+    w = 2 * math.pi * f
+    c = oracle.Circuit("V1 in 0 SIN(0 1 1k 0 0 0)\nR1 in out 1k\nC1 out 0 100n\n")
+    reltol, abstol, hmax, tstop = 1e-4, 1e-7, 2e-5, 2e-3
+    tr = c.tran_lte_trace(tstop, 1e-8, 1e-14, hmax, reltol, abstol)
+    assert (tr[:, 3] == 0).sum() >= 3  # the sine makes the controller reject steps
+    t_acc = [0.0]
+    worst = 0.0
+    for k, (t, h, ratio, outcome, hn) in enumerate(tr):
+        assert t == t_acc[-1]
+        if ratio >= 0:
+            hp = t - t_acc[-2]
+            x, x1, x2max = rc_sine_response(np.array([t, t + h]), tau, w)
+            tol = reltol * max(abs(x[0]), abs(x[1])) + abstol
+            model = 0.5 * h * h * abs(2.0 * (x1[1] - x1[0]) / (h + hp)) / tol
+            scale = 0.5 * h * h * x2max[0] / tol
+            worst = max(worst, abs(ratio - model) / scale)
+            assert (outcome == 1) == (ratio <= 1.0)
+            g = 0.9 / math.sqrt(ratio) if ratio > 0 else 2.0
+            want = h * max(0.25, g) if outcome == 0 else h * min(2.0, max(0.25, g))
+            assert abs(hn - want) <= 1e-15 * want
+        else:
+            assert outcome == 1 and len(t_acc) == 1  # only the first step has no estimate
+            assert hn == h
+        if k + 1 < len(tr):
+            tn = t + h if outcome == 1 else t
+            assert abs(tr[k + 1, 1] - min(hn, hmax, tstop - tn)) <= 1e-12 * hn
+        if outcome == 1:
+            t_acc.append(t + h)
+    assert abs(t_acc[-1] - tstop) < 1e-15
+    assert worst < 0.05, worst
+
+
+def test_transient_source_time_on_a_divider(oracle):
+    """krets' backward Euler evaluates the source at the previous step's time
+    (transient.rs: the guess map's `time`, quirk 1): a resistor divider's node
+    at step n is 3/4 value_at(t_{n-1}); the clean integrators use t_n."""
+    h = 0.1e-6
+    for wave, kind in (("PULSE(0.5 2.5 1u 1u 2u 3u 10u)", 1), ("SIN(0.5 2 100k 5u 2e4 90)", 2)):
+        # waveform parameters take SI suffixes (parse_value: "5u" is 5 * 1e-6, one ulp below 5e-6)
+        p = [oracle.parse_value(tok) for tok in wave[wave.index("(") + 1:-1].split()]
+        c = oracle.Circuit(f"V1 in 0 0.25 {wave}\nC1 in 0 1n\nR1 in out 1k\nR2 out 0 3k\n")
+        k = c.index["V(out)"]
+        for method, lag in ((oracle.BE_COMPAT, 1), (oracle.BE, 0), (oracle.TRAP, 0)):
+            x, _ = c.tran(h, 400 * h, method=method)
+            want = [0.75 * 0.25] + [0.75 * oracle.wave_value(kind, p, (n - lag) * h) for n in range(1, 401)]
+            assert np.max(np.abs(x[:, k] - want)) < 1e-14, (wave, method)
