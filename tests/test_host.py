@@ -358,3 +358,40 @@ def test_subcircuit_errors(kg, oracle, text):
     assert e.value.code == 7  # ParseError
     with pytest.raises(oracle.OracleError):
         oracle.Circuit(text)
+
+
+def test_header_is_c99_and_host_entry_points_work_from_c(kg, tmp_path):
+    """include/krets_gpu.h compiles as strict C99 (gcc -std=c99 -pedantic
+    -Wall -Wextra -Werror), and a C caller of the host-only entry points
+    (netlist reader, element tables, plan export, default config, spec
+    reader) gets what the Python binding gets."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    exe = str(tmp_path / "abi_check")
+    lib_dir = os.path.join(ROOT, "krets_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "host_c99", "abi_check.c"), "-o",
+                        exe, "-L" + lib_dir, "-Wl,-rpath," + lib_dir, "-l:libkrets_gpu.so"], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    cir = os.path.join(GOLDEN, "circuits", "low_pass_filter", "ac", "ac.cir")
+    toml = os.path.join(GOLDEN, "circuits", "low_pass_filter", "ac", "krets.toml")
+    r = subprocess.run([exe, cir, toml], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    c = kg.Circuit(open(cir).read())
+    assert lines[0] == "parse 0" and lines[1] == f"size {c.n} {c.n_elements}"
+    assert [l.split()[2] for l in lines if l.startswith("unknown ")] == c.names
+    assert [l.split()[2] for l in lines if l.startswith("element ")] == c.element_ids
+    rows = [l.split()[2:] for l in lines if l.startswith("row ")]
+    for e, row in enumerate(rows):
+        el = c.elements[e]
+        assert [int(v) for v in row[:4]] == [int(el["type"]), int(el["plus"]), int(el["minus"]), int(el["branch"])]
+        assert float(row[4]) == c.params[e, 0]
+    plan = next(l for l in lines if l.startswith("plan ")).split()
+    assert plan[1] == "1" and int(plan[2]) == c.n
+    assert next(l for l in lines if l.startswith("config ")) == "config 0.001 9.9999999999999998e-13 9.9999999999999995e-07 300 1"
+    spec = next(l for l in lines if l.startswith("spec ")).split()
+    assert spec[1] == "0" and spec[2] == "2" and spec[3].endswith(".cir")
