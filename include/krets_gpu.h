@@ -76,7 +76,8 @@ typedef struct {
 
 /* Per (instance, element) parameter record of KG_NPARAM doubles:
  *   R: [0] ohms          C: [0] farads        L: [0] henries
- *   I: [0] amps          D: [0] Is, [1] N
+ *   I: [0] amps          D: [0] Is, [1] N, [2] CJO, [3] VJ, [4] M, [5] FC,
+ *                           [6] TT (junction charge; CJO = TT = 0: none)
  *   E: [0] voltage gain  G: [0] transconductance (S)
  *   F: [0] current gain  H: [0] transresistance (ohms)
  *   V: [0] dc value, [1] ac magnitude, [2] waveform kind (kg_wave_kind),
@@ -167,7 +168,13 @@ int32_t kg_dc_sweep(kg_ctx* ctx, const kg_solver_config* cfg, int32_t element, d
  * interleaved (re, im). With KG_MIXED, the factorisation runs in complex64
  * and FP64 iterative refinement restores accuracy; err_est[instance] (may be
  * NULL) receives the largest normwise relative size of the last refinement
- * correction. */
+ * correction. With KG_FP64, err_est receives a bound on the normwise
+ * backward error |b - A x| / (|A| |x| + |b|) of every solve of the instance.
+ * Pivots are fixed per topology; every solve checks its componentwise
+ * backward error in FP64 (mixed: its refinement estimate), and solves that
+ * fail are solved again with partial pivoting at their own frequency, as
+ * krets' LU does (systems of up to 90 unknowns; larger ones, and systems
+ * singular under partial pivoting, report KG_DECOMPOSITION). */
 int32_t kg_ac_sweep(kg_ctx* ctx, const kg_solver_config* cfg, const double* freqs, int32_t nf,
                     int32_t precision, double* x, int32_t* status, double* err_est, int32_t mem);
 
@@ -289,6 +296,21 @@ int32_t kg_jit_lte_source(const kg_element* elements, int32_t n_elements, int32_
 /* Netlist reader for the R/L/C/V/I/D subset of krets' grammar
  * (crates/krets-parser/src/parser.rs:27-154). */
 int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* err, int32_t errlen);
+
+/* kg_parse_netlist with options (no krets counterpart). flags = 0 reads
+ * exactly as kg_parse_netlist. KG_NETLIST_DIODE_PARASITICS models what krets
+ * parses and ignores (models/diode.rs:14-37) or does not parse at all:
+ *  - rs > 0: an internal anode node V(<id>:a) and a resistor <id>:rs from
+ *    the anode to it, both numbered after every krets unknown and element,
+ *    so krets' own numbering is unchanged;
+ *  - cjo (or cj0), vj, m, fc, tt: SPICE junction charge, into the diode's
+ *    parameter record [2..6] (defaults cjo 0, vj 1, m 0.5, fc 0.5, tt 0),
+ *    used by AC sweeps (small-signal capacitance at the operating point) and
+ *    by the clean transients KG_BE / KG_TRAP and kg_transient_lte (charge
+ *    companion); operating points, DC sweeps and KG_BE_COMPAT ignore it, as
+ *    krets' integrator has no junction charge. */
+#define KG_NETLIST_DIODE_PARASITICS 1
+int32_t kg_parse_netlist_ext(const char* text, int32_t flags, kg_circuit** out, char* err, int32_t errlen);
 void kg_circuit_free(kg_circuit* c);
 int32_t kg_circuit_size(const kg_circuit* c, int32_t* n_unknowns, int32_t* n_elements);
 /* Name of unknown idx, e.g. "V(out)" or "I(V1)"; NULL if out of range. */

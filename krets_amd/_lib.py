@@ -20,6 +20,7 @@ FP64, MIXED = 0, 1
 BE_COMPAT, BE, TRAP = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 NPARAM = 10
+NETLIST_DIODE_PARASITICS = 1  # kg_parse_netlist_ext flag
 SPEC_OP, SPEC_DC, SPEC_AC, SPEC_TRANSIENT = 0, 1, 2, 3
 SWEEP_DECADE, SWEEP_OCTAVE, SWEEP_LINEAR = 0, 1, 2
 
@@ -120,6 +121,7 @@ def lib():
         L.kg_jit_dc_source.argtypes = [P, I32, I32, P, ctypes.c_char_p, I32]
         L.kg_jit_tran_source.argtypes = [P, I32, I32, P, D, I32, ctypes.c_char_p, I32]
         L.kg_parse_netlist.argtypes = [C, ctypes.POINTER(P), ctypes.c_char_p, I32]
+        L.kg_parse_netlist_ext.argtypes = [C, I32, ctypes.POINTER(P), ctypes.c_char_p, I32]
         L.kg_circuit_free.argtypes = [P]
         L.kg_circuit_size.argtypes = [P, PI, PI]
         L.kg_circuit_unknown_name.restype = C

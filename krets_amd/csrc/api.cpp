@@ -327,6 +327,16 @@ struct Launch {
 
 bool valid_tps(int t) { return t == 1 || t == 4 || t == 16 || t == 64 || t == 256; }
 
+// True when some diode of the batch carries junction charge (CJO or TT of
+// the opt-in diode parasitics): the run-time specialised transient kernels do
+// not generate it.
+bool any_junction_charge(const kg_ctx* c) {
+  for (size_t r = 0; r < c->el.size(); r++)
+    if (c->el[r].type == KG_D && (c->params[r * KG_NPARAM + 2] != 0.0 || c->params[r * KG_NPARAM + 6] != 0.0))
+      return true;
+  return false;
+}
+
 int default_tps(const kg_ctx* c, int n_max) {
   if (c->shared && n_max <= 8) return 1;
   if (n_max <= 96) return 16;
@@ -903,7 +913,7 @@ int32_t adjoint_backward_lin(kg_ctx* c, double* grad, int32_t* status, int32_t m
   hipError_t e;
   size_t ni = c->n_inst;
   int ne = (int)(c->eoff[1] - c->eoff[0]);
-  int64_t seg_stride = (int64_t)(aa.K + 1) * P.n_max;
+  int64_t seg_stride = 2 * (int64_t)(aa.K + 1) * P.n_max;  // two segment buffers (pipelined recomputation)
   double* d_ck = (double*)ws_get(c, "adj_ckpt", 8, &e);
   double* d_c = (double*)ws_get(c, "adj_cvec", 8, &e);
   double* d_seg = (double*)ws_get(c, "adj_seg", sizeof(double) * ni * seg_stride, &e);
@@ -1241,6 +1251,10 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  // the accuracy guard's queue: a count, then up to one entry per system
+  const int64_t fixcap = (int64_t)c->n_inst * nf;
+  auto* d_fix = (unsigned long long*)ws_get(c, "fixq", sizeof(unsigned long long) * (1 + (size_t)fixcap), &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   // One AC launch for a mapping candidate: TPS threads per system, plus 1000
   // for the persistent variant (shared topology only) that stages the plan
   // and each instance's element records in LDS.
@@ -1324,18 +1338,27 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       int64_t ns = nsys;
       double* outp = d_out;
       int32_t* stp = d_st;
-      double* errp = mixed ? d_err : nullptr;
+      double* errp = d_err;
       int rf = refine;
       double stop = cfg->refine_iters > 0 ? 0.0 : KG_REFINE_STOP;  // explicit counts run every pass
-      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp, &rf, &stop};
+      unsigned long long* fq = d_fix;
+      int64_t fc = fixcap;
+      void* args[] = {&prm, &eb, &fr, &nfa, &ns, &outp, &stp, &errp, &rf, &stop, &fq, &fc};
       return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, (unsigned)A.L.lds, c->stream, args, nullptr);
     }
     BatchDev B = batch_dev(c, A.P);
     if (A.persistent)
       return run_ac_persistent(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop,
-                               d_xoff, d_out, d_off, d_st, d_err, refine, A.lay, A.plan_ints, A.chunk);
+                               d_xoff, d_out, d_off, d_st, d_err, refine, A.lay, A.plan_ints, A.chunk, d_fix, fixcap);
     return run_ac(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out,
-                  d_off, d_st, d_err, refine, A.lay);
+                  d_off, d_st, d_err, refine, A.lay, d_fix, fixcap);
+  };
+  // the guard's fixup over whatever the sweep kernel queued (persistent grid)
+  auto fixup = [&](const AcLaunch& A) -> hipError_t {
+    int nmax = std::min(A.P.n_max, KG_FIX_NMAX);
+    while (nmax > 1 && ac_fix_lds(nmax, A.P.ne_max) > LDS_MAX) nmax--;
+    return run_ac_fix(256, c->stream, batch_dev(c, A.P), 0, d_f, nf, d_xop, d_xoff, d_out, d_off, d_st, d_err,
+                      mixed ? 1 : 0, d_fix, fixcap, nmax, A.P.ne_max);
   };
   int cand = cfg->threads_per_system;
   if (cand == 0) {
@@ -1348,6 +1371,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       if (rr) return rr;
       if (!A.lane && A.tps * A.L.spb < 64 && A.tps < 64)
         return set_err(c, KG_UNEXPECTED, "mapping leaves most of a wave idle");
+      HIPCHK(c, run_fill_f64((double*)d_fix, 0.0, 1, c->stream));
       return time_launch(c, [&] { return run(A); }, ms);
     }, &cand);
     if (r) return r;
@@ -1357,17 +1381,19 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   P = A.P;
   int ni = c->n_inst;
   steps.push_back({nullptr, [&] { return run_fill_f64(d_err, 0.0, ni, c->stream); }});
+  steps.push_back({nullptr, [&] { return run_fill_f64((double*)d_fix, 0.0, 1, c->stream); }});
   steps.push_back({A.lane ? jit_ac_name(mixed, cand % 10 == 2)
                    : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
                                   : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),
                    [&] { return run(A); }});
+  steps.push_back({"k_ac_fix", [&] { return fixup(A); }});
   if (graph) {
     std::ostringstream k;
     k << "ac|" << cand << "|" << refine << "|" << (cfg->refine_iters > 0 ? "fixed" : "adaptive") << "|" << precision
       << "|" << nf << "|" << ocfg.reltol << "|" << ocfg.iabstol
       << "|" << ocfg.vabstol << "|" << ocfg.maxiter << "|" << ocfg.compat << "|" << (void*)d_out << "|" << (void*)d_f
       << "|" << (void*)d_xop << "|" << (void*)d_st << "|" << (void*)d_err << "|" << (void*)c->d_pats << "|"
-      << (void*)d_off << "|" << (void*)d_xoff << "|" << (void*)A.fn.fn;
+      << (void*)d_off << "|" << (void*)d_xoff << "|" << (void*)A.fn.fn << "|" << (void*)d_fix;
     auto it = c->gexec.find(k.str());
     if (it == c->gexec.end()) {
       hipGraph_t g = nullptr;
@@ -1474,6 +1500,8 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   JitFn tran_fn;
   auto lane_setup = [&](Plan& PP) -> int32_t {
     if (!c->shared || mixed) return set_err(c, KG_UNEXPECTED, "the lane kernel needs FP64 and a shared topology");
+    if (method != KG_BE_COMPAT && any_junction_charge(c))
+      return set_err(c, KG_UNEXPECTED, "lane kernel: junction charge is not generated");
     int32_t rr = prepare(c, AK_TRAN, ORD_MINDEG, s_tr, PP);
     if (rr) return rr;
     if (PP.group_status[0] != KG_OK) return set_err(c, KG_UNEXPECTED, "lane kernel: no valid plan");
@@ -1746,6 +1774,7 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
     A.lane = cand == 2001;
     if (A.lane) {
       if (!c->shared) return set_err(c, KG_UNEXPECTED, "lane kernel needs a shared topology");
+      if (any_junction_charge(c)) return set_err(c, KG_UNEXPECTED, "lane kernel: junction charge is not generated");
       int32_t rr = prepare(c, AK_TRAN, ORD_MINDEG, s_tr, A.P);
       if (rr) return rr;
       if (A.P.group_status[0] != KG_OK) return set_err(c, KG_UNEXPECTED, "lane kernel: no valid plan");

@@ -174,6 +174,43 @@ __device__ DiodeLin diode_eval(double vd, double is, double n, bool compat, doub
   return r;
 }
 
+// Junction charge of a diode at junction voltage v (opt-in diode parasitics;
+// SPICE2 model, Nagel 1975): depletion charge with parameters CJO, VJ, M, FC
+// from the record pe ([2..5]),
+//   q = CJO VJ / (1 - M) (1 - (1 - v/VJ)^(1-M)),  c = CJO (1 - v/VJ)^-M
+// below FC VJ, continued linearly in c above it, plus the diffusion charge
+// TT i (pe[6]) with i and its conductance g the diode's current at v.
+struct JQ {
+  double q, c;
+};
+__device__ __forceinline__ bool has_jq(const double* pe) { return pe[2] != 0.0 || pe[6] != 0.0; }
+__device__ JQ junction_charge(double v, const double* pe, double i, double g) {
+  const double cjo = pe[2], vj = pe[3], m = pe[4], fc = pe[5], tt = pe[6];
+  JQ r{tt * i, tt * g};
+  if (cjo != 0.0) {
+    const double fcv = fc * vj;
+    if (v < fcv) {
+      const double a = 1.0 - v / vj, s = pow(a, -m);
+      r.q += cjo * vj * (1.0 - a * s) / (1.0 - m);
+      r.c += cjo * s;
+    } else {
+      const double f1 = vj * (1.0 - pow(1.0 - fc, 1.0 - m)) / (1.0 - m);
+      const double f2 = pow(1.0 - fc, 1.0 + m), f3 = 1.0 - fc * (1.0 + m);
+      r.q += cjo * (f1 + (f3 * (v - fcv) + m / (2.0 * vj) * (v * v - fcv * fcv)) / f2);
+      r.c += cjo / f2 * (f3 + m * v / vj);
+    }
+  }
+  return r;
+}
+
+// Diode current and conductance at v without limiting (junction charge of
+// the previous step).
+__device__ __forceinline__ void diode_iv(double v, double is, double n, double& i, double& g) {
+  const double nvt = n * KG_VT, ex = exp(v / nvt);
+  i = is * (ex - 1.0);
+  g = is / nvt * ex;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
 // with its own L2. Remapping gives every XCD a contiguous range of logical
 // blocks, so systems that share an instance's element data (consecutive
@@ -206,7 +243,9 @@ __device__ __forceinline__ A elem_y(int e, const PatDev& P, const double* prm, c
     if (sa.ak == AK_AC) return mk<A>(0.0, 2.0 * KG_PI * sa.freq * c);
     return mk<A>(sa.alpha * (c / sa.h), 0.0);
   }
-  if (ty == KG_D) return mk<A>(ed[e * ESTRIDE], 0.0);
+  if (ty == KG_D)  // AC: conductance and junction capacitance at the operating point
+    return sa.ak == AK_AC ? mk<A>(ed[e * ESTRIDE], 2.0 * KG_PI * sa.freq * ed[e * ESTRIDE + 1])
+                          : mk<A>(ed[e * ESTRIDE], 0.0);
   return mk<A>(0.0, 0.0);
 }
 
@@ -486,6 +525,94 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
+// Entry norm of the backward-error guard: |re| + |im|.
+__device__ __forceinline__ double c1(double a) { return fabs(a); }
+__device__ __forceinline__ double c1(c128 a) { return fabs(a.re) + fabs(a.im); }
+
+// Componentwise backward error of an FP64 solve (Oettli-Prager), omega =
+// max_i |b - A x|_i / (|A| |x| + |b|)_i with |z| = |re| + |im| and A
+// re-gathered in FP64 from the stamps (the system's v holds its factors); it
+// bounds the normwise backward error. Reached by every thread.
+template <class A, int TPS>
+__device__ double guard_omega(const PatDev& P, const A* b, const A* x, const double* prm, const double* eb,
+                              const StampArgs& sa, const double* ed, double* red, int ls, int t, bool act) {
+  double om = 0.0;
+  if (act)
+    for (int i = t; i < P.n; i += TPS) {
+      A r = b[i];
+      double dr = c1(b[i]);
+      for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) {
+        int q0 = P.scon_ptr[p], q1 = P.scon_ptr[p + 1];
+        if (q0 == q1) continue;
+        A a = mk<A>(0.0, 0.0);
+        for (int q = q0; q < q1; q++) a += mat_contrib<A>(P.scon[q], P, prm, eb, sa, ed);
+        const A xj = x[P.col[p]];
+        r -= a * xj;
+        dr += c1(a) * c1(xj);
+      }
+      const double ri = c1(r);
+      if (ri > 0.0) om = fmax(om, dr > 0.0 ? ri / dr : INFINITY);
+    }
+  return sys_max<TPS>(om, red, ls, t);
+}
+
+// Normwise backward error |b - A x| / (|A| |x| + |b|) of a solve returned by
+// mixed precision refinement (infinity norms over |re| + |im|, A re-gathered
+// in FP64 from the stamps). Reached by every thread.
+template <class A, int TPS>
+__device__ double guard_eta(const PatDev& P, const A* b, const A* x, const double* prm, const double* eb,
+                            const StampArgs& sa, const double* ed, double* red, int ls, int t, bool act) {
+  double rm = 0.0, am = 0.0, bm = 0.0, xm = 0.0;
+  if (act)
+    for (int i = t; i < P.n; i += TPS) {
+      A r = b[i];
+      double ar = 0.0;
+      for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) {
+        int q0 = P.scon_ptr[p], q1 = P.scon_ptr[p + 1];
+        if (q0 == q1) continue;
+        A a = mk<A>(0.0, 0.0);
+        for (int q = q0; q < q1; q++) a += mat_contrib<A>(P.scon[q], P, prm, eb, sa, ed);
+        r -= a * x[P.col[p]];
+        ar += c1(a);
+      }
+      rm = fmax(rm, c1(r));
+      am = fmax(am, ar);
+      bm = fmax(bm, c1(b[i]));
+      xm = fmax(xm, c1(x[i]));
+    }
+  rm = sys_max<TPS>(rm, red, ls, t);
+  am = sys_max<TPS>(am, red, ls, t);
+  bm = sys_max<TPS>(bm, red, ls, t);
+  xm = sys_max<TPS>(xm, red, ls, t);
+  const double den = am * xm + bm;
+  return den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
+}
+
+// Queues system `sys` for k_ac_fix (queue words: [0] count, [1 + q] systems);
+// a full queue marks the instance as failed instead.
+__device__ __forceinline__ void fix_push(unsigned long long* fixq, int64_t cap, int64_t sys, int32_t* status,
+                                         int inst) {
+  const unsigned long long q = atomicAdd(fixq, 1ull);
+  if (q < (unsigned long long)cap) fixq[1 + q] = (unsigned long long)sys;
+  else atomicMax(&status[inst], KG_DECOMPOSITION);
+}
+
+// End of one AC solve (thread 0 of the system): a failed pivot, an FP64
+// backward error above KG_GUARD_OMEGA or a mixed error estimate above
+// KG_GUARD_MIXED queues the solve for k_ac_fix (when a queue is given);
+// otherwise the status and the instance's error figure are published.
+__device__ __forceinline__ void ac_finish(int32_t flag, double q, bool mixed, unsigned long long* fixq,
+                                          int64_t fixcap, int64_t sys, int32_t* status, double* err, int inst) {
+  const bool bad = flag != KG_OK || !(q <= (mixed ? KG_GUARD_MIXED : KG_GUARD_OMEGA));
+  if (fixq && bad) {
+    fix_push(fixq, fixcap, sys, status, inst);
+    return;
+  }
+  if (flag != KG_OK) atomicMax(&status[inst], flag);
+  if (err && q > 0.0) atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]),
+                                (unsigned long long)__double_as_longlong(q));
+}
+
 template <class F, class A>
 struct SysMem {
   F* v;
@@ -536,8 +663,13 @@ __device__ SysMem<F, A> carve(char* base, const Lay& L) {
 // Evaluates all diodes of a system at guess g (permuted numbering).
 // compat_first: krets' empty solution map on the first OP iteration.
 template <int TPS>
+// jq (clean transients only; null elsewhere): the step (h, alpha) whose
+// junction-charge companion is added to every diode that has one, with xp
+// the previous step's solution: i_q = alpha (q(v) - q(v_prev)) / h -
+// (alpha - 1) i_q,prev, linearised at the limited junction voltage.
 __device__ void eval_diodes(const PatDev& P, const double* prm, const double* g, double* ed, bool compat,
-                            bool compat_first, int* any_limited, int t, bool act) {
+                            bool compat_first, int* any_limited, int t, bool act, const StampArgs* jq = nullptr,
+                            const double* xp = nullptr) {
   if (!act || !P.has_diode) return;
   for (int e = t; e < P.ne; e += TPS) {
     if (P.etype[e] != KG_D) continue;
@@ -552,10 +684,42 @@ __device__ void eval_diodes(const PatDev& P, const double* prm, const double* g,
     }
     const double* pe = prm + e * KG_NPARAM;
     DiodeLin d = diode_eval(vp - vm, pe[0], pe[1], compat, ed[e * ESTRIDE + 2]);
+    if (jq && has_jq(pe)) {
+      const JQ q = junction_charge(d.vd, pe, d.ieq + d.g * d.vd, d.g);
+      const double vo = (p >= 0 ? xp[p] : 0.0) - (m >= 0 ? xp[m] : 0.0);
+      double io, go;
+      diode_iv(vo, pe[0], pe[1], io, go);
+      const JQ qo = junction_charge(vo, pe, io, go);
+      const double gq = jq->alpha * q.c / jq->h;
+      const double iq = jq->alpha * (q.q - qo.q) / jq->h - (jq->alpha - 1.0) * ed[e * ESTRIDE + 3];
+      d.g += gq;
+      d.ieq += iq - gq * d.vd;
+    }
     ed[e * ESTRIDE] = d.g;
     ed[e * ESTRIDE + 1] = d.ieq;
     ed[e * ESTRIDE + 2] = d.vd;
     if (d.limited) *any_limited = 1;
+  }
+}
+
+// Trapezoidal junction-charge currents after an accepted step from xo to xn
+// (clean transients): i_q = 2 (q(v_n) - q(v_o)) / h - i_q,prev.
+template <int TPS>
+__device__ void update_junction_charge(const PatDev& P, const double* prm, const double* xo, const double* xn,
+                                       double* ed, double h, int t, bool act) {
+  if (!act || !P.has_diode) return;
+  for (int e = t; e < P.ne; e += TPS) {
+    if (P.etype[e] != KG_D) continue;
+    const double* pe = prm + e * KG_NPARAM;
+    if (!has_jq(pe)) continue;
+    const int p = P.ep[e], m = P.em[e];
+    const double vn = (p >= 0 ? xn[p] : 0.0) - (m >= 0 ? xn[m] : 0.0);
+    const double vo = (p >= 0 ? xo[p] : 0.0) - (m >= 0 ? xo[m] : 0.0);
+    double in, gn, io, go;
+    diode_iv(vn, pe[0], pe[1], in, gn);
+    diode_iv(vo, pe[0], pe[1], io, go);
+    ed[e * ESTRIDE + 3] =
+        2.0 * (junction_charge(vn, pe, in, gn).q - junction_charge(vo, pe, io, go).q) / h - ed[e * ESTRIDE + 3];
   }
 }
 
@@ -687,7 +851,7 @@ __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_
 template <class F, class A, int TPS>
 __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
                      const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status, double* err,
-                     int refine, Lay lay) {
+                     int refine, Lay lay, unsigned long long* fixq, int64_t fixcap) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
@@ -710,8 +874,10 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
         const double* x0 = xop + xop_off[inst];
         int p = P.ep[e], m = P.em[e];
         double vp = p >= 0 ? x0[P.col_of[p]] : 0.5, vm = m >= 0 ? x0[P.col_of[m]] : 0.0;
-        DiodeLin d = diode_eval(vp - vm, prm[e * KG_NPARAM], prm[e * KG_NPARAM + 1], true, 0.0);
+        const double* pe = prm + e * KG_NPARAM;
+        DiodeLin d = diode_eval(vp - vm, pe[0], pe[1], true, 0.0);
         M.ed[e * ESTRIDE] = d.g;
+        M.ed[e * ESTRIDE + 1] = has_jq(pe) ? junction_charge(vp - vm, pe, d.ieq + d.g * (vp - vm), d.g).c : 0.0;
       }
   __syncthreads();
   StampArgs sa{AK_AC, valid ? freqs[fi] : 1.0, 1.0, 1.0, 0.0, -1, 0.0};
@@ -721,6 +887,12 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
   factor<F, TPS>(P, M.v, t, valid, &M.flags[0]);
   double est = solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, refine, red, ls, t, valid, M.r);
   __syncthreads();
+  constexpr bool mixed = !std::is_same<F, A>::value;
+  if constexpr (!mixed) {
+    est = guard_omega<A, TPS>(P, M.b, M.x, prm, eb, sa, M.ed, red, ls, t, valid);
+  } else if (!(guard_eta<A, TPS>(P, M.b, M.x, prm, eb, sa, M.ed, red, ls, t, valid) <= KG_GUARD_MIXED_ETA)) {
+    est = INFINITY;  // the returned solution misses the normwise guard: to the fixup
+  }
   if (valid) {
     double* o = out + out_off[inst] + (int64_t)fi * P.n * 2;
     for (int j = t; j < P.n; j += TPS) {
@@ -728,10 +900,7 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
       o[2 * c] = re_of(M.x[j]);
       o[2 * c + 1] = im_of(M.x[j]);
     }
-    if (t == 0) {
-      if (M.flags[0] != KG_OK) atomicMax(&status[inst], M.flags[0]);
-      if (err) atomic_max_nonneg(&err[inst], est);
-    }
+    if (t == 0) ac_finish(M.flags[0], est, mixed, fixq, fixcap, sys, status, err, inst);
   }
 }
 
@@ -817,6 +986,7 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
       double vd = (p >= 0 ? M.xp[p] : 0.0) - (m >= 0 ? M.xp[m] : 0.0);
       M.ed[e * ESTRIDE] = P.etype[e] == KG_L ? vd : 0.0;
       M.ed[e * ESTRIDE + 2] = P.etype[e] == KG_D ? vd : 0.0;
+      M.ed[e * ESTRIDE + 3] = 0.0;  // junction-charge current at the operating point
     }
   __syncthreads();
   StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
@@ -857,7 +1027,8 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
         sa.tsrc = (compat && it == 0) ? tprev : tn;
         if (t == 0) M.flags[4] = 0;
         __syncthreads();
-        eval_diodes<TPS>(P, prm, M.g, M.ed, compat, false, &M.flags[4], t, a2);
+        // junction charge in the clean integrators (krets' has none)
+        eval_diodes<TPS>(P, prm, M.g, M.ed, compat, false, &M.flags[4], t, a2, compat ? nullptr : &sa, M.xp);
         __syncthreads();
         stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
         stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, a2);
@@ -911,6 +1082,7 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
           M.ed[e * ESTRIDE] = vn;
         }
       }
+    if (trap) update_junction_charge<TPS>(P, prm, M.xp, M.x, M.ed, ta.h, t, act);  // x: FP64 in both precisions
     __syncthreads();
     if (act) {
       double* o = out + out_off[ii] + (int64_t)s * P.n;
@@ -1198,47 +1370,61 @@ struct LinSys {
     }
   }
 
-  // L y = b, U x = y in place in xc; ends with a barrier. Level 0 rows, when
-  // free of lower entries, are the right-hand side itself.
-  __device__ void solve() {
-    for (int l = lev0; l < nlev; l++) {
+  // One forward-substitution level l of L y = b in place in xc (no barrier).
+  __device__ void fwd_level(int l) {
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        if (f_lev[r] != l) continue;
-        const int i = (int)(f_ip[r] & 0xFFFF), p0 = (int)(f_ip[r] >> 16), cnt = (int)(f_nd[r] & 0xFFFF);
-        double sum = xc[i];
-        double vv[LIN_KE], xx[LIN_KE];
+    for (int r = 0; r < R; r++) {
+      if (f_lev[r] != l) continue;
+      const int i = (int)(f_ip[r] & 0xFFFF), p0 = (int)(f_ip[r] >> 16), cnt = (int)(f_nd[r] & 0xFFFF);
+      double sum = xc[i];
+      double vv[LIN_KE], xx[LIN_KE];
 #pragma unroll
-        for (int k = 0; k < LIN_KE; k++) {
-          vv[k] = v[k < cnt ? p0 + k : nnz];
-          xx[k] = xc[k & 1 ? f_col[r][k / 2] >> 16 : f_col[r][k / 2] & 0xFFFF];
-        }
-#pragma unroll
-        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];  // padded entries subtract 0 * 0
-        if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
-        xc[i] = sum;
+      for (int k = 0; k < LIN_KE; k++) {
+        vv[k] = v[k < cnt ? p0 + k : nnz];
+        xx[k] = xc[k & 1 ? f_col[r][k / 2] >> 16 : f_col[r][k / 2] & 0xFFFF];
       }
-      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];  // padded entries subtract 0 * 0
+      if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
+      xc[i] = sum;
     }
-    for (int l = 0; l < nblev; l++) {
+  }
+
+  // One back-substitution level l of U x = y in place in xc (no barrier).
+  __device__ void bwd_level(int l) {
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        if (b_lev[r] != l) continue;
-        const int i = (int)(b_ip[r] & 0xFFFF), p0 = (int)(b_ip[r] >> 16), cnt = (int)(b_nd[r] & 0xFFFF);
-        const int d = (int)(b_nd[r] >> 16);
-        double sum = xc[i];
-        const double vd = v[d];
-        double vv[LIN_KE], xx[LIN_KE];
+    for (int r = 0; r < R; r++) {
+      if (b_lev[r] != l) continue;
+      const int i = (int)(b_ip[r] & 0xFFFF), p0 = (int)(b_ip[r] >> 16), cnt = (int)(b_nd[r] & 0xFFFF);
+      const int d = (int)(b_nd[r] >> 16);
+      double sum = xc[i];
+      const double vd = v[d];
+      double vv[LIN_KE], xx[LIN_KE];
 #pragma unroll
-        for (int k = 0; k < LIN_KE; k++) {
-          vv[k] = v[k < cnt ? p0 + k : nnz];
-          xx[k] = xc[k & 1 ? b_col[r][k / 2] >> 16 : b_col[r][k / 2] & 0xFFFF];
-        }
-#pragma unroll
-        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-        if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
-        xc[i] = sum * vd;  // diagonal stored as its reciprocal
+      for (int k = 0; k < LIN_KE; k++) {
+        vv[k] = v[k < cnt ? p0 + k : nnz];
+        xx[k] = xc[k & 1 ? b_col[r][k / 2] >> 16 : b_col[r][k / 2] & 0xFFFF];
       }
+#pragma unroll
+      for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+      if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
+      xc[i] = sum * vd;  // diagonal stored as its reciprocal
+    }
+  }
+
+  // Phases of one step after the right-hand side: the forward levels (level
+  // 0 rows, when free of lower entries, are the right-hand side itself), then
+  // the backward levels. solve_phase(p), p < solve_phases(), does phase p.
+  __device__ int solve_phases() const { return (nlev - lev0) + nblev; }
+  __device__ void solve_phase(int p) {
+    if (p < nlev - lev0) fwd_level(lev0 + p);
+    else bwd_level(p - (nlev - lev0));
+  }
+
+  // L y = b, U x = y in place in xc; ends with a barrier.
+  __device__ void solve() {
+    for (int p = 0; p < solve_phases(); p++) {
+      solve_phase(p);
       __syncthreads();
     }
   }
@@ -1379,7 +1565,7 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_fwd(BatchDev B, int kind_slot, 
   LinSys<TPS, R> S;
   const LinHdr* Hg = reinterpret_cast<const LinHdr*>(P.lin);
   char* q = S.setup(P, Hg->words, smem, ta, srcv, nsrc_max, inst);
-  q += 8 * (4 * ((size_t)S.n + 1) + (size_t)S.ndyn + 1 + (size_t)Hg->ne + 1);  // backward-only arrays
+  q += 8 * (5 * ((size_t)S.n + 1) + (size_t)S.ndyn + 1 + (size_t)Hg->ne + 1);  // backward-only arrays
   double* red = reinterpret_cast<double*>(q);
   int* flags = reinterpret_cast<int*>(q + 8 * 32);
   if (t == 0) flags[0] = KG_OK;
@@ -1428,13 +1614,20 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_fwd(BatchDev B, int kind_slot, 
 }
 
 // Backward pass of the discrete adjoint from LDS (k_adj_bwd semantics): per
-// checkpoint segment, last to first, recompute the segment's solutions into
-// seg (permuted order), then run each step's adjoint backwards: the
+// checkpoint segment, last to first, each step's adjoint runs backwards: the
 // companions' adjoint loads, lambda = A^-T xbar by the transposed level
 // solves, the gradient terms of every R, C and L and the companions' adjoint
-// states. The recomputed state x_{s-1} is loaded into registers one step
-// ahead and staged into LDS at the end of step s. xbar0 receives dJ/dx_0
-// (krets numbering) for the operating-point term; grad: [instance][ne].
+// states. The states a segment's adjoint needs are recomputed from its
+// checkpoint, and that recomputation is software-pipelined with the adjoint
+// of the segment after it: while the adjoint walks segment k backwards, the
+// same threads recompute segment k - 1 forwards, one step of each per
+// iteration, with the phases of the two (the adjoint's transposed levels,
+// the recomputation's forward and backward levels) sharing barriers. A step
+// of the backward pass thus costs about one solve's worth of barrier phases
+// instead of two. Recomputed states go to a double-buffered segment store
+// (seg: two rows of K + 1 states per instance, permuted order); the adjoint
+// loads x_{s-1} into registers one step ahead. xbar0 receives dJ/dx_0 (krets
+// numbering) for the operating-point term; grad: [instance][ne].
 template <int TPS, int R>
 __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, TranArgs ta, int K,
                                                      const double* cvec, const double* ckpt, int64_t ck_stride,
@@ -1471,6 +1664,8 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   q += 8 * ((size_t)n + 1);
   double* XB = reinterpret_cast<double*>(q);
   q += 8 * ((size_t)n + 1);
+  double* lam = reinterpret_cast<double*>(q);  // lambda = A^-T xbar
+  q += 8 * ((size_t)n + 1);
   double* sbar = reinterpret_cast<double*>(q);  // companion adjoint states
   q += 8 * ((size_t)ndyn + 1);
   double* gr = reinterpret_cast<double*>(q);  // gradient per element
@@ -1478,7 +1673,7 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   int* flags = reinterpret_cast<int*>(q + 8 * 32);
   if (t == 0) {
     flags[0] = KG_OK;
-    XA[n] = XB[n] = 0.0;
+    XA[n] = XB[n] = lam[n] = 0.0;
   }
   __syncthreads();
   const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
@@ -1541,56 +1736,26 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   for (int d = t; d <= ndyn; d += TPS) sbar[d] = 0.0;
   for (int e = t; e <= ne; e += TPS) gr[e] = 0.0;
   if (t == 0) xbar[n] = xb0[n] = 0.0;
-  __syncthreads();
   const double* ck = ckpt + (int64_t)inst * ck_stride;
-  double* sg = seg + (int64_t)inst * seg_stride;
+  const int64_t half = (int64_t)(K + 1) * n;  // one segment buffer: states x_{s0} .. x_{s0+K}
+  double* sgbase = seg + (int64_t)inst * seg_stride;
   const int nseg = (ta.nsteps + K - 1) / K;
-  for (int sgi = nseg - 1; sgi >= 0; sgi--) {
-    const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
-    // restore the checkpoint and recompute the segment (solutions of steps
-    // s0..s1 into seg, permuted order)
-    {
-      const double* c = ck + (int64_t)sgi * cpw;
-      for (int j = t; j < n; j += TPS) {
-        const double xv = c[j];
-        S.xq[j] = xv;
-        sg[j] = xv;
-      }
-      for (int d = t; d < ndyn; d += TPS) S.ed[d] = c[n + d];
-    }
-    S.prefetch(s0 + 1);
-    __syncthreads();
-    for (int s = s0 + 1; s <= s1; s++) {
-      S.rhs(s);
-      __syncthreads();
-      S.solve();
-      S.update();
-      if (s < s1) {  // x_{s1} stays in LDS
-        double* so = sg + (int64_t)(s - s0) * n;
-        for (int j = t; j < n; j += TPS) so[j] = S.xc[j];
-      }
-      __syncthreads();
-      S.swap();
-    }
-    // stage x_{s1} (LDS) and x_{s1-1} (the block's own stores: row 0 is the
-    // checkpoint); lambda lives in the recompute's free solution buffer
-    double* X1 = XA;
-    double* Xs = XB;
-    double* lam = S.xc;
-    for (int j = t; j < n; j += TPS) {
-      X1[j] = S.xq[j];
-      Xs[j] = sg[(int64_t)(s1 - 1 - s0) * n + j];
-    }
-    __syncthreads();
-    for (int s = s1 - 1; s >= s0; s--) {
-      // x_{s-1} for the next step, into registers now
-      double nx[R];
+
+  // ---- phases of one adjoint step s (segment states in sa: row j = x_{sa0+j})
+  // 0 companions' loads; 1 lambda copy (fused U^T level 0); U^T levels; L^T
+  // levels; element terms; staging of the next step
+  const int nUT = ntlev - tl0, nLT = ntblev - tl0;
+  const int a_phases = 2 + nUT + nLT + 2;
+  double* X1 = XA;  // x_{s+1}
+  double* Xs = XB;  // x_s
+  double nx[R];     // x_{s-1}, loaded one step ahead
+  auto adj_phase = [&](int ph, int s, int s0, const double* sa) {
+    if (ph == 0) {
 #pragma unroll
-      for (int r = 0; r < R; r++)
-        nx[r] = (s > s0 && t + r * TPS < n) ? sg[(int64_t)(s - 1 - s0) * n + t + r * TPS] : 0.0;
+      for (int r = 0; r < R; r++) nx[r] = (s > s0 && t + r * TPS < n) ? sa[(int64_t)(s - 1 - s0) * n + t + r * TPS] : 0.0;
       // companions' adjoint loads of step s+1 (ic_{s+1} and vl_{s+1} depend
       // on x_{s+1}, x_s and ic_s)
-      if (trap) {
+      if (trap)
         for (int d = t; d < ndyn; d += TPS) {
           const uint16_t* rd = S.dyn + d * LIN_DYN;
           const double sb1 = sbar[d];
@@ -1607,57 +1772,59 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
             if (m < n) atomicAdd(&xbar[m], -sb1);
           }
         }
-        __syncthreads();
-      }
-      // lambda = A^-T xbar: U^T forward (level 0 fused with the copy), then
-      // L^T backward (level 0 rows, when free of entries, are done)
+      return;
+    }
+    if (ph == 1) {  // lambda = A^-T xbar: U^T forward (level 0 fused with the copy)
 #pragma unroll
       for (int r = 0; r < R; r++) {
         if (tf_lev[r] < 0) continue;
         const int i = (int)(tf_ip[r] & 0xFFFF);
         lam[i] = tl0 && tf_lev[r] == 0 ? xbar[i] * S.v[tf_nd[r] >> 16] : xbar[i];
       }
-      __syncthreads();
-      for (int l = tl0; l < ntlev; l++) {
+      return;
+    }
+    if (ph < 2 + nUT) {
+      const int l = tl0 + (ph - 2);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          if (tf_lev[r] != l) continue;
-          const int i = (int)(tf_ip[r] & 0xFFFF), q0 = (int)(tf_ip[r] >> 16), cnt = (int)(tf_nd[r] & 0xFFFF);
-          double sum = lam[i];
-          const double vd = S.v[tf_nd[r] >> 16];
-          double vv[LIN_KE], xx[LIN_KE];
+      for (int r = 0; r < R; r++) {
+        if (tf_lev[r] != l) continue;
+        const int i = (int)(tf_ip[r] & 0xFFFF), q0 = (int)(tf_ip[r] >> 16), cnt = (int)(tf_nd[r] & 0xFFFF);
+        double sum = lam[i];
+        const double vd = S.v[tf_nd[r] >> 16];
+        double vv[LIN_KE], xx[LIN_KE];
 #pragma unroll
-          for (int k = 0; k < LIN_KE; k++) {
-            vv[k] = S.v[tf_e[r][k] >> 16];
-            xx[k] = lam[tf_e[r][k] & 0xFFFF];
-          }
-#pragma unroll
-          for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-          for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
-          lam[i] = sum * vd;
+        for (int k = 0; k < LIN_KE; k++) {
+          vv[k] = S.v[tf_e[r][k] >> 16];
+          xx[k] = lam[tf_e[r][k] & 0xFFFF];
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+        for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
+        lam[i] = sum * vd;
       }
-      for (int l = tl0; l < ntblev; l++) {
+      return;
+    }
+    if (ph < 2 + nUT + nLT) {  // L^T backward (level 0 rows, when free of entries, are done)
+      const int l = tl0 + (ph - 2 - nUT);
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          if (tb_lev[r] != l) continue;
-          const int i = (int)(tb_ip[r] & 0xFFFF), q0 = (int)(tb_ip[r] >> 16), cnt = (int)(tb_nd[r] & 0xFFFF);
-          double sum = lam[i];
-          double vv[LIN_KE], xx[LIN_KE];
+      for (int r = 0; r < R; r++) {
+        if (tb_lev[r] != l) continue;
+        const int i = (int)(tb_ip[r] & 0xFFFF), q0 = (int)(tb_ip[r] >> 16), cnt = (int)(tb_nd[r] & 0xFFFF);
+        double sum = lam[i];
+        double vv[LIN_KE], xx[LIN_KE];
 #pragma unroll
-          for (int k = 0; k < LIN_KE; k++) {
-            vv[k] = S.v[tb_e[r][k] >> 16];
-            xx[k] = lam[tb_e[r][k] & 0xFFFF];
-          }
-#pragma unroll
-          for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-          for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
-          lam[i] = sum;
+        for (int k = 0; k < LIN_KE; k++) {
+          vv[k] = S.v[tb_e[r][k] >> 16];
+          xx[k] = lam[tb_e[r][k] & 0xFFFF];
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+        for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
+        lam[i] = sum;
       }
-      // element terms of step s -> s+1
+      return;
+    }
+    if (ph == 2 + nUT + nLT) {  // element terms of step s -> s+1
       for (int a = t; a < nadj; a += TPS) {
         const uint16_t* ra = adj + a * LIN_ADJ;
         const int kind = ra[0] >> 14, e = ra[0] & 0x3FFF;
@@ -1685,21 +1852,93 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
           gr[e] += el * al / h * d0 - al / h * el * d1;
         }
       }
-      __syncthreads();
-      // next step's adjoint load; stage x_{s-1}
-      const double w = (s == 0) ? 0.5 * h : h;
+      return;
+    }
+    // next step's adjoint load; stage x_{s-1} into X1 (free now; the
+    // pointers swap after the barrier)
+    const double w = (s == 0) ? 0.5 * h : h;
 #pragma unroll
-      for (int r = 0; r < R; r++)
-        if (t + r * TPS < n) {
-          const int j = S.o_slot[r];
-          xbar[j] = xb0[j] + w * cv[r];
-          xb0[j] = 0.0;
-          X1[t + r * TPS] = nx[r];  // X1 is free now: it becomes x_{s-1}
-        }
-      __syncthreads();
-      double* sw = X1;
-      X1 = Xs;
-      Xs = sw;
+    for (int r = 0; r < R; r++)
+      if (t + r * TPS < n) {
+        const int j = S.o_slot[r];
+        xbar[j] = xb0[j] + w * cv[r];
+        xb0[j] = 0.0;
+        X1[t + r * TPS] = nx[r];
+      }
+  };
+
+  // ---- phases of one recomputation step s (segment states into sr, row
+  // s - sr0): right-hand side; the solve's levels; companion update and store
+  const int r_phases = 1 + S.solve_phases() + 1;
+  auto rec_phase = [&](int ph, int s, int sr0, double* sr) {
+    if (ph == 0) {
+      S.rhs(s);
+    } else if (ph < 1 + S.solve_phases()) {
+      S.solve_phase(ph - 1);
+    } else {
+      S.update();
+      double* so = sr + (int64_t)(s - sr0) * n;
+      for (int j = t; j < n; j += TPS) so[j] = S.xc[j];
+    }
+  };
+  // restores checkpoint k into the recomputation's state and row 0 of sr
+  auto restore = [&](int k, double* sr) {
+    const double* c = ck + (int64_t)k * cpw;
+    for (int j = t; j < n; j += TPS) {
+      const double xv = c[j];
+      S.xq[j] = xv;
+      sr[j] = xv;
+    }
+    for (int d = t; d < ndyn; d += TPS) S.ed[d] = c[n + d];
+    S.prefetch(k * K + 1);
+  };
+
+  // prologue: recompute the last segment alone
+  {
+    const int sgi = nseg - 1, s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
+    double* sr = sgbase + (int64_t)(sgi & 1) * half;
+    restore(sgi, sr);
+    __syncthreads();
+    for (int s = s0 + 1; s <= s1; s++) {
+      for (int ph = 0; ph < r_phases; ph++) {
+        rec_phase(ph, s, s0, sr);
+        __syncthreads();
+      }
+      S.swap();
+    }
+  }
+  for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+    const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
+    const double* sa = sgbase + (int64_t)(sgi & 1) * half;
+    // the segment before, recomputed meanwhile into the other buffer
+    const bool rec = sgi > 0;
+    const int r0 = (sgi - 1) * K;
+    double* sr = sgbase + (int64_t)((sgi - 1) & 1) * half;
+    // stage x_{s1} and x_{s1-1}
+    X1 = XA;
+    Xs = XB;
+    for (int j = t; j < n; j += TPS) {
+      X1[j] = sa[(int64_t)(s1 - s0) * n + j];
+      Xs[j] = sa[(int64_t)(s1 - 1 - s0) * n + j];
+    }
+    if (rec) restore(sgi - 1, sr);
+    __syncthreads();
+    const int na = s1 - s0, nr = rec ? K : 0;
+    for (int it = 0; it < max(na, nr); it++) {
+      const bool a_on = it < na, r_on = it < nr;
+      const int sa_s = s1 - 1 - it, sr_s = r0 + 1 + it;
+      const int nph = max(a_on ? a_phases : 0, r_on ? r_phases : 0);
+      for (int ph = 0; ph < nph; ph++) {
+        if (a_on && ph < a_phases) adj_phase(ph, sa_s, s0, sa);
+        if (r_on && ph < r_phases) rec_phase(ph, sr_s, r0, sr);
+        __syncthreads();
+      }
+      if (a_on) {
+        double* sw = X1;
+        X1 = Xs;
+        Xs = sw;
+      }
+      if (r_on) S.swap();
     }
   }
   // vl_0 = v(x_0): its adjoint flows into x_0
@@ -1828,7 +2067,7 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
       bool a2 = !done;
       if (t == 0) M.flags[4] = 0;
       __syncthreads();
-      eval_diodes<TPS>(P, prm, M.g, M.ed, false, false, &M.flags[4], t, a2);
+      eval_diodes<TPS>(P, prm, M.g, M.ed, false, false, &M.flags[4], t, a2, &sa, xc);
       __syncthreads();
       stamp_matrix<double, double, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
       stamp_rhs<double, TPS>(P, M.b, prm, sa, M.ed, xc, t, a2);
@@ -1872,7 +2111,7 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
       for (int e = t; e < P.ne; e += TPS) {
         int ty = P.etype[e];
         double a, b0, bm;
-        if (ty == KG_C) {
+        if (ty == KG_C || (ty == KG_D && has_jq(prm + e * KG_NPARAM))) {  // junction charge: its voltage
           int p = P.ep[e], m = P.em[e];
           a = (p >= 0 ? M.x[p] : 0.0) - (m >= 0 ? M.x[m] : 0.0);
           b0 = (p >= 0 ? xc[p] : 0.0) - (m >= 0 ? xc[m] : 0.0);
@@ -1956,7 +2195,8 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
 template <class F, class A, int TPS>
 __global__ void k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
                        const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status, double* err,
-                       int refine, Lay lay, int plan_ints, int64_t chunk) {
+                       int refine, Lay lay, int plan_ints, int64_t chunk, unsigned long long* fixq,
+                       int64_t fixcap) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
@@ -1998,8 +2238,10 @@ __global__ void k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, c
           const double* x0 = xop + xop_off[inst];
           int p = P.ep[e], m = P.em[e];
           double vp = p >= 0 ? x0[P.col_of[p]] : 0.5, vm = m >= 0 ? x0[P.col_of[m]] : 0.0;
-          DiodeLin d = diode_eval(vp - vm, iprm[e * KG_NPARAM], iprm[e * KG_NPARAM + 1], true, 0.0);
+          const double* pe = iprm + e * KG_NPARAM;
+          DiodeLin d = diode_eval(vp - vm, pe[0], pe[1], true, 0.0);
           M.ed[e * ESTRIDE] = d.g;
+          M.ed[e * ESTRIDE + 1] = has_jq(pe) ? junction_charge(vp - vm, pe, d.ieq + d.g * (vp - vm), d.g).c : 0.0;
         }
     __syncthreads();
     StampArgs sa{AK_AC, valid ? freqs[fi] : 1.0, 1.0, 1.0, 0.0, -1, 0.0};
@@ -2010,6 +2252,12 @@ __global__ void k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, c
     double est =
         solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, iprm, ieb, sa, M.ed, refine, red, ls, t, valid, M.r);
     __syncthreads();
+    constexpr bool mixed = !std::is_same<F, A>::value;
+    if constexpr (!mixed) {
+      est = guard_omega<A, TPS>(P, M.b, M.x, iprm, ieb, sa, M.ed, red, ls, t, valid);
+    } else if (!(guard_eta<A, TPS>(P, M.b, M.x, iprm, ieb, sa, M.ed, red, ls, t, valid) <= KG_GUARD_MIXED_ETA)) {
+      est = INFINITY;  // the returned solution misses the normwise guard: to the fixup
+    }
     if (valid) {
       double* o = out + out_off[inst] + (int64_t)fi * P.n * 2;
       for (int j = t; j < P.n; j += TPS) {
@@ -2017,12 +2265,246 @@ __global__ void k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, c
         o[2 * c] = re_of(M.x[j]);
         o[2 * c + 1] = im_of(M.x[j]);
       }
-      if (t == 0) {
-        if (M.flags[0] != KG_OK) atomicMax(&status[inst], M.flags[0]);
-        if (err && est > 0.0) atomic_max_nonneg(&err[inst], est);
-      }
+      if (t == 0) ac_finish(M.flags[0], est, mixed, fixq, fixcap, (int64_t)inst * nf + fi, status, err, inst);
     }
   }
+}
+
+// ------------------------------------------------------------- AC fixup
+// Solves the AC systems queued by the sweep kernels' accuracy guard again,
+// the way krets' sparse LU does it: with row partial pivoting chosen at the
+// point itself (crates/krets-solver/src/solver/ac.rs:67-95 calls faer's
+// sp_lu per frequency). The system (at most KG_FIX_NMAX unknowns) is stamped
+// into a dense complex128 matrix in LDS in the plan's permuted numbering and
+// factored by one block of KG_FIX_NT threads: per column a one-wave pivot
+// search, a row swap, the multipliers and the rank-1 update of the trailing
+// matrix. Up to two FP64 refinement steps follow, with residuals from the
+// sparse stamps. Persistent over the queue: block k takes queued systems k,
+// k + gridDim.x, ..., so the grid never depends on the count.
+// err_mode 0 (FP64 calls) publishes the final backward error, 1 (mixed) the
+// size of the last refinement correction relative to x.
+constexpr int KG_FIX_NT = 256;
+
+__device__ __forceinline__ double wave_max_idx(double v, int& idx) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const int oi = __shfl_xor(idx, off);
+    if (ov > v || (ov == v && oi < idx)) {
+      v = ov;
+      idx = oi;
+    }
+  }
+  return v;
+}
+
+__device__ __forceinline__ c128 wave_sum_c(c128 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.re += __shfl_xor(v.re, off);
+    v.im += __shfl_xor(v.im, off);
+  }
+  return v;
+}
+
+__device__ double block_max(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = KG_FIX_NT / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmax(red[t], red[t + s]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Forward and back substitution with the pivoted dense LU (unit lower L
+// below the diagonal, U on and above it) on w, in place; one wave works, the
+// block synchronises around it.
+__device__ void fix_lu_solve(const c128* LU, int ld, int n, const int32_t* piv, c128* w) {
+  const int t = threadIdx.x;
+  if (t < 64) {
+    for (int k = 0; k < n; k++) {  // the row swaps of the factorisation, in order
+      const int p = piv[k];
+      if (t == 0 && p != k) {
+        const c128 tmp = w[k];
+        w[k] = w[p];
+        w[p] = tmp;
+      }
+    }
+    for (int i = 0; i < n; i++) {
+      c128 acc{0.0, 0.0};
+      for (int j = t; j < i; j += 64) acc += LU[i * ld + j] * w[j];
+      acc = wave_sum_c(acc);
+      if (t == 0) w[i] = w[i] - acc;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      c128 acc{0.0, 0.0};
+      for (int j = i + 1 + t; j < n; j += 64) acc += LU[i * ld + j] * w[j];
+      acc = wave_sum_c(acc);
+      if (t == 0) w[i] = (w[i] - acc) * rcp(LU[i * ld + i]);
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(KG_FIX_NT) k_ac_fix(BatchDev B, int kind_slot, const double* freqs, int nf,
+                                                      const double* xop, const int64_t* xop_off, double* out,
+                                                      const int64_t* out_off, int32_t* status, double* err,
+                                                      int err_mode, const unsigned long long* fixq, int64_t fixcap,
+                                                      int nmax, int ne_max) {
+  extern __shared__ __align__(16) char smem[];
+  const int t = threadIdx.x;
+  c128* LU = (c128*)smem;                      // [nmax][nmax]
+  c128* bb = LU + (size_t)nmax * nmax;         // right-hand side (plan row order)
+  c128* x = bb + nmax;                         // solution (plan column order)
+  c128* w = x + nmax;                          // work vector
+  double* red = (double*)(w + nmax);           // [KG_FIX_NT]
+  double* ed = red + KG_FIX_NT;                // [ESTRIDE * ne_max] diode small-signal state
+  int32_t* piv = (int32_t*)(ed + (size_t)ESTRIDE * ne_max);  // [nmax]
+  int32_t* sh = piv + nmax;                    // [0] pivot row, [1] status
+  const int64_t cnt = min((int64_t)fixq[0], fixcap);
+  for (int64_t q = blockIdx.x; q < cnt; q += gridDim.x) {
+    const int64_t sys = (int64_t)fixq[1 + q];
+    const int inst = (int)(sys / nf), fi = (int)(sys % nf);
+    const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+    const int n = P.n;
+    if (n > nmax || P.ne > ne_max) {  // block-uniform
+      if (t == 0) atomicMax(&status[inst], KG_DECOMPOSITION);
+      continue;
+    }
+    const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
+    const double* eb = B.ebase + B.eoff[inst];
+    for (int e = t; e < P.ne; e += KG_FIX_NT)
+      if (P.etype[e] == KG_D) {  // small-signal conductance at the operating point (stampable.rs:614-641)
+        const double* x0 = xop + xop_off[inst];
+        const int p = P.ep[e], m = P.em[e];
+        const double vp = p >= 0 ? x0[P.col_of[p]] : 0.5, vm = m >= 0 ? x0[P.col_of[m]] : 0.0;
+        const double* pe = prm + e * KG_NPARAM;
+        const DiodeLin d = diode_eval(vp - vm, pe[0], pe[1], true, 0.0);
+        ed[e * ESTRIDE] = d.g;
+        ed[e * ESTRIDE + 1] = has_jq(pe) ? junction_charge(vp - vm, pe, d.ieq + d.g * (vp - vm), d.g).c : 0.0;
+      }
+    for (int k = t; k < n * nmax; k += KG_FIX_NT) LU[k] = c128{0.0, 0.0};
+    if (t == 0) sh[1] = KG_OK;
+    __syncthreads();
+    const StampArgs sa{AK_AC, freqs[fi], 1.0, 1.0, 0.0, -1, 0.0};
+    for (int i = t; i < n; i += KG_FIX_NT) {
+      for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
+        c128 a{0.0, 0.0};
+        for (int c = P.scon_ptr[s]; c < P.scon_ptr[s + 1]; c++) a += mat_contrib<c128>(P.scon[c], P, prm, eb, sa, ed);
+        LU[i * nmax + P.col[s]] = a;
+      }
+      c128 r{0.0, 0.0};
+      for (int c = P.rcon_ptr[i]; c < P.rcon_ptr[i + 1]; c++) r += rhs_contrib<c128>(P.rcon[c], P, prm, sa, ed, nullptr);
+      bb[i] = r;
+    }
+    __syncthreads();
+    // LU with row partial pivoting
+    for (int k = 0; k < n; k++) {
+      if (t < 64) {
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + t; i < n; i += 64) {
+          const double m = c1(LU[i * nmax + k]);
+          if (m > best) {
+            best = m;
+            bi = i;
+          }
+        }
+        best = wave_max_idx(best, bi);
+        if (t == 0) {
+          piv[k] = bi;
+          sh[0] = bi;
+          if (!(best > 0.0) || !isfinite(best)) sh[1] = KG_DECOMPOSITION;
+        }
+      }
+      __syncthreads();
+      const int p = sh[0];
+      if (p != k)
+        for (int j = t; j < n; j += KG_FIX_NT) {
+          const c128 tmp = LU[k * nmax + j];
+          LU[k * nmax + j] = LU[p * nmax + j];
+          LU[p * nmax + j] = tmp;
+        }
+      __syncthreads();
+      const c128 rk = rcp(LU[k * nmax + k]);
+      for (int i = k + 1 + t; i < n; i += KG_FIX_NT) LU[i * nmax + k] = LU[i * nmax + k] * rk;
+      __syncthreads();
+      const int m = n - k - 1;
+      for (int idx = t; idx < m * m; idx += KG_FIX_NT) {
+        const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+        LU[i * nmax + j] -= LU[i * nmax + k] * LU[k * nmax + j];
+      }
+      __syncthreads();
+    }
+    const bool ok = sh[1] == KG_OK;
+    for (int i = t; i < n; i += KG_FIX_NT) {
+      w[i] = bb[i];
+      x[i] = c128{0.0, 0.0};
+    }
+    __syncthreads();
+    if (ok) fix_lu_solve(LU, nmax, n, piv, w);
+    // x = w; then refinement steps with FP64 residuals from the sparse stamps
+    double eta = 0.0, corr = 0.0;
+    for (int it = 0; it < 3; it++) {
+      if (ok)
+        for (int i = t; i < n; i += KG_FIX_NT) x[i] += w[i];
+      __syncthreads();
+      double rm = 0.0, am = 0.0, bm = 0.0, xm = 0.0, dm = 0.0;
+      for (int i = t; i < n; i += KG_FIX_NT) {
+        c128 r = bb[i];
+        double ar = 0.0;
+        for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
+          const int q0 = P.scon_ptr[s], q1 = P.scon_ptr[s + 1];
+          if (q0 == q1) continue;
+          c128 a{0.0, 0.0};
+          for (int c = q0; c < q1; c++) a += mat_contrib<c128>(P.scon[c], P, prm, eb, sa, ed);
+          r -= a * x[P.col[s]];
+          ar += c1(a);
+        }
+        rm = fmax(rm, c1(r));
+        am = fmax(am, ar);
+        bm = fmax(bm, c1(bb[i]));
+        xm = fmax(xm, c1(x[i]));
+        dm = fmax(dm, c1(w[i]));
+        w[i] = r;  // next correction's right-hand side (its own row: no other thread reads it)
+      }
+      rm = block_max(rm, red);
+      am = block_max(am, red);
+      bm = block_max(bm, red);
+      xm = block_max(xm, red);
+      dm = block_max(dm, red);
+      const double den = am * xm + bm;
+      eta = den > 0.0 ? rm / den : (rm > 0.0 ? 1.0 : 0.0);
+      corr = xm > 0.0 ? dm / xm : 0.0;
+      // stop once eta is at rounding level; the mixed figure needs one
+      // refinement step (its correction estimates the first solve's error)
+      if (!ok || it == 2 || (eta <= 4.0 * 2.220446049250313e-16 && (err_mode == 0 || it >= 1))) break;
+      fix_lu_solve(LU, nmax, n, piv, w);
+    }
+    if (t == 0) {
+      int32_t st = sh[1];
+      if (st == KG_OK && !(eta <= KG_FIX_FAIL)) st = KG_DECOMPOSITION;
+      if (st != KG_OK) atomicMax(&status[inst], st);
+      const double q2 = err_mode ? corr : eta;
+      if (err && q2 > 0.0)
+        atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(q2));
+    }
+    double* o = out + out_off[inst] + (int64_t)fi * n * 2;
+    for (int j = t; j < n; j += KG_FIX_NT) {
+      const int c = P.col_of[j];
+      o[2 * c] = x[j].re;
+      o[2 * c + 1] = x[j].im;
+    }
+    __syncthreads();
+  }
+}
+
+// LDS bytes of k_ac_fix for systems of up to nmax unknowns and ne_max elements.
+size_t ac_fix_lds(int nmax, int ne_max) {
+  return sizeof(c128) * ((size_t)nmax * nmax + 3 * (size_t)nmax) + 8 * (size_t)KG_FIX_NT +
+         8 * (size_t)ESTRIDE * ne_max + 4 * ((size_t)nmax + 2);
 }
 
 // ------------------------------------------------------------- adjoint
@@ -2437,13 +2919,22 @@ hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, Batch
 
 hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                   const double* freqs, int nf, const double* xop, const int64_t* xop_off, double* out,
-                  const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay) {
+                  const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
+                  unsigned long long* fixq, int64_t fixcap) {
   if (mixed) {
     KG_TPS_SWITCH(tps, launch(k_ac<c64, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out,
-                              out_off, status, err, refine, lay))
+                              out_off, status, err, refine, lay, fixq, fixcap))
   }
   KG_TPS_SWITCH(tps, launch(k_ac<c128, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out, out_off,
-                            status, err, refine, lay))
+                            status, err, refine, lay, fixq, fixcap))
+}
+
+hipError_t run_ac_fix(int grid, hipStream_t st, BatchDev B, int kind_slot, const double* freqs, int nf,
+                      const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
+                      int32_t* status, double* err, int err_mode, const unsigned long long* fixq, int64_t fixcap,
+                      int nmax, int ne_max) {
+  return launch(k_ac_fix, dim3(grid), dim3(KG_FIX_NT), ac_fix_lds(nmax, ne_max), st, B, kind_slot, freqs, nf, xop,
+                xop_off, out, out_off, status, err, err_mode, fixq, fixcap, nmax, ne_max);
 }
 
 hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
@@ -2553,13 +3044,13 @@ hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, Batch
 hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B,
                              int kind_slot, const double* freqs, int nf, const double* xop, const int64_t* xop_off,
                              double* out, const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
-                             int plan_ints, int64_t chunk) {
+                             int plan_ints, int64_t chunk, unsigned long long* fixq, int64_t fixcap) {
   if (mixed) {
     KG_TPS_SWITCH(tps, launch(k_ac_p<c64, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out,
-                              out_off, status, err, refine, lay, plan_ints, chunk))
+                              out_off, status, err, refine, lay, plan_ints, chunk, fixq, fixcap))
   }
   KG_TPS_SWITCH(tps, launch(k_ac_p<c128, c128, T>, g, b, lds, st, B, kind_slot, freqs, nf, xop, xop_off, out,
-                            out_off, status, err, refine, lay, plan_ints, chunk))
+                            out_off, status, err, refine, lay, plan_ints, chunk, fixq, fixcap))
 }
 
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

@@ -38,6 +38,33 @@ enum : int32_t {
   RK_IND = 5   // inductor companion on its branch row
 };
 
+// Accuracy guard of the AC sweeps. Pivots are fixed once per topology by the
+// symbolic pass, at one representative frequency and from one instance, so a
+// solve far from that point can lose backward stability (an inductor branch
+// pivot of magnitude w*L chosen at the sweep's centre is tiny at its low end).
+// Every FP64 solve therefore measures in FP64 its componentwise backward
+// error (Oettli-Prager),
+//   omega = max_i |b - A x|_i / (|A| |x| + |b|)_i     (|z| = |re| + |im|),
+// which also bounds the normwise one, |b - A x| / (|A| |x| + |b|) in infinity
+// norms; a mixed precision solve uses its refinement's error estimate and
+// the normwise backward error of the solution it returns. A solve with omega
+// above KG_GUARD_OMEGA, a failed static pivot, or a mixed solve whose estimate
+// stays above KG_GUARD_MIXED or whose returned solution's normwise backward
+// error is above KG_GUARD_MIXED_ETA, is queued and solved again
+// by k_ac_fix: a dense LU with row partial pivoting in FP64 (systems of at
+// most KG_FIX_NMAX unknowns), then up to two FP64 refinement steps, after
+// which its normwise backward error eta is measured; eta above KG_FIX_FAIL
+// reports KG_DECOMPOSITION. The FP64 error figure published per instance is
+// the largest omega (accepted solves) or eta (fixed solves): a bound on the
+// normwise backward error of every solve of the instance.
+constexpr double KG_GUARD_OMEGA = 1.4210854715202004e-14;  // 2^-46: 64 units in the last place
+constexpr double KG_GUARD_MIXED = 1e-10;                  // a tenth of the 1e-9 the mixed path promises
+constexpr double KG_GUARD_MIXED_ETA = 1.4210854715202004e-14;  // 2^-46, normwise, on the returned solution
+constexpr double KG_FIX_FAIL = 9.094947017729282e-13;     // 2^-40
+constexpr int KG_FIX_NMAX = 90;                           // 90 x 90 complex128 in LDS: 127 KB
+// Fixup queue layout (unsigned 64-bit words): [0] count, [1 + q] queued system
+// (instance * number of frequencies + frequency index), q < capacity.
+
 // Layout of one pattern inside a flat int32 buffer. Offsets are in int32 units.
 struct PatLayout {
   int32_t n, nnz, ne, nlev, nblev, ntlev, ntblev, nonlinear;

@@ -10,9 +10,11 @@
 namespace kg {
 
 // Element state stride in LDS (doubles per element): [0] diode G or
-// companion state (capacitor current / inductor voltage), [1] diode Ieq,
-// [2] diode junction voltage of the previous iterate.
-constexpr int ESTRIDE = 3;
+// companion state (capacitor current / inductor voltage), [1] diode Ieq (AC:
+// the diode's small-signal capacitance), [2] diode junction voltage of the
+// previous iterate, [3] trapezoidal junction-charge current of the previous
+// step (diodes with junction charge).
+constexpr int ESTRIDE = 4;
 
 // Per-instance views.
 struct BatchDev {
@@ -59,12 +61,13 @@ __host__ __device__ inline size_t lin_bytes(const LinHdr& h) {
 }
 
 // LDS bytes of one system of the linear adjoint kernels (k_adj_lin_fwd /
-// k_adj_lin_bwd): the whole plan, the transient's arrays, four more solution
-// vectors (adjoint load and its next-step part, two recomputed states), the
-// companion adjoint states, per-element gradients and a reduction buffer.
+// k_adj_lin_bwd): the whole plan, the transient's arrays, five more solution
+// vectors (adjoint load and its next-step part, two recomputed states,
+// lambda), the companion adjoint states, per-element gradients and a
+// reduction buffer.
 __host__ __device__ inline size_t lin_adj_bytes(const LinHdr& h) {
   return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 2 * (size_t)h.ndyn + 5) +
-         8 * (size_t)KG_NPARAM * h.nsrc + 8 * (4 * ((size_t)h.n + 1) + (size_t)h.ndyn + 1 + (size_t)h.ne + 1 + 32) +
+         8 * (size_t)KG_NPARAM * h.nsrc + 8 * (5 * ((size_t)h.n + 1) + (size_t)h.ndyn + 1 + (size_t)h.ne + 1 + 32) +
          64;
 }
 
@@ -95,9 +98,19 @@ struct LteArgs {
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
                       int32_t* status, int32_t* iters, Lay lay);
+// AC sweep kernels. fixq / fixcap: the accuracy guard's queue (see
+// KG_GUARD_OMEGA); null: no guard (statuses and error figures as computed).
 hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                   const double* freqs, int nf, const double* xop, const int64_t* xop_off, double* out,
-                  const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay);
+                  const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
+                  unsigned long long* fixq, int64_t fixcap);
+// Re-solves the systems in the guard's queue (k_ac_fix), persistent over the
+// queue with `grid` blocks; err_mode 0: backward error, 1: correction size.
+size_t ac_fix_lds(int nmax, int ne_max);
+hipError_t run_ac_fix(int grid, hipStream_t st, BatchDev B, int kind_slot, const double* freqs, int nf,
+                      const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
+                      int32_t* status, double* err, int err_mode, const unsigned long long* fixq, int64_t fixcap,
+                      int nmax, int ne_max);
 hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                    LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
@@ -119,7 +132,7 @@ hipError_t run_adj_op(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, Batch
 hipError_t run_ac_persistent(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B,
                              int kind_slot, const double* freqs, int nf, const double* xop, const int64_t* xop_off,
                              double* out, const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
-                             int plan_ints, int64_t chunk);
+                             int plan_ints, int64_t chunk, unsigned long long* fixq, int64_t fixcap);
 // Linear fixed-step transient from LDS (k_tran_lin), one system per block of
 // tps threads (64, 128, 256, 512, 1024), each owning up to r rows (1, 2, 4;
 // 1024 threads: 1, 2); every launched system needs n <= r * tps and
@@ -131,7 +144,8 @@ hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, Batc
 // Discrete adjoint from LDS (k_adj_lin_fwd / k_adj_lin_bwd), shared topology,
 // one system per block of tps threads (64, 256, 512) owning up to r rows
 // (1, 2, 4); lds >= lin_adj_bytes; checkpoints hold the permuted solution
-// then the companion states (cpw >= n + ndyn).
+// then the companion states (cpw >= n + ndyn); the backward pass takes a
+// double-buffered segment store, seg_stride >= 2 * (K + 1) * n.
 hipError_t run_adj_lin_fwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                            int K, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
                            int64_t ck_stride, int cpw, double* J, int32_t* status, double* srcv, int nsrc_max);

@@ -286,9 +286,13 @@ int parse_vsrc(const std::string& line, Rec& r) {
   return 1;
 }
 
+// Diode model (models/diode.rs:14-37: is, rs, n; krets ignores other keys).
+// cjo (cj0), vj, m, fc and tt are the SPICE junction-charge parameters,
+// read for the opt-in diode parasitics of kg_parse_netlist_ext only.
 struct Model {
   bool diode;
-  double is = 1e-12, n = 1.0;
+  double is = 1e-12, n = 1.0, rs = 0.0;
+  double cjo = 0.0, vj = 1.0, m = 0.5, fc = 0.5, tt = 0.0;
 };
 
 bool parse_model(std::string line, std::string& name, Model& m) {
@@ -315,6 +319,12 @@ bool parse_model(std::string line, std::string& name, Model& m) {
     for (auto& ch : k) ch = (char)std::tolower((unsigned char)ch);
     if (k == "is") m.is = v;
     else if (k == "n") m.n = v;
+    else if (k == "rs") m.rs = v;
+    else if (k == "cjo" || k == "cj0") m.cjo = v;
+    else if (k == "vj") m.vj = v;
+    else if (k == "m") m.m = v;
+    else if (k == "fc") m.fc = v;
+    else if (k == "tt") m.tt = v;
     c = d;
   }
   return true;
@@ -382,6 +392,11 @@ struct kg_circuit_impl {
 };
 
 extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* err, int32_t errlen) {
+  return kg_parse_netlist_ext(text, 0, out, err, errlen);
+}
+
+extern "C" int32_t kg_parse_netlist_ext(const char* text, int32_t flags, kg_circuit** out, char* err,
+                                        int32_t errlen) {
   auto fail = [&](int32_t code, const std::string& msg) {
     if (err && errlen > 0) std::snprintf(err, errlen, "%s", msg.c_str());
     return code;
@@ -584,6 +599,15 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
       }
       r.p[0] = it->second.is;
       r.p[1] = it->second.n;
+      if (flags & KG_NETLIST_DIODE_PARASITICS) {
+        const Model& md = it->second;
+        r.p[2] = md.cjo;
+        r.p[3] = md.vj;
+        r.p[4] = md.m;
+        r.p[5] = md.fc;
+        r.p[6] = md.tt;
+        r.p[7] = md.rs;  // consumed below: the series resistor element
+      }
     }
     kg_element e{};
     static const char* L = "RCLVIDEGFH";
@@ -614,6 +638,37 @@ extern "C" int32_t kg_parse_netlist(const char* text, kg_circuit** out, char* er
       e.ctrl_branch = b->second;
     }
     C->el.push_back(e);
+  }
+  // Opt-in series resistance: after krets' unknowns and elements, so their
+  // numbering is unchanged, each diode with rs > 0 gets an internal anode node
+  // V(<id>:a) and a resistor <id>:rs from its anode to that node; the diode
+  // then sits between the internal node and its cathode (SPICE's placement).
+  if (flags & KG_NETLIST_DIODE_PARASITICS) {
+    const size_t ne0 = C->el.size();
+    for (size_t k = 0; k < ne0; k++) {
+      if (C->recs[k].letter != 'D' || !(C->recs[k].p[7] > 0.0)) {
+        if (C->recs[k].letter == 'D') C->recs[k].p[7] = 0.0;
+        continue;
+      }
+      const double rs = C->recs[k].p[7];
+      C->recs[k].p[7] = 0.0;
+      const int32_t ia = (int32_t)C->names.size();
+      C->names.push_back("V(" + C->ids[k] + ":a)");
+      Rec rr;
+      rr.letter = 'R';
+      rr.name = C->recs[k].name + ":rs";
+      rr.p[0] = rs;
+      kg_element e{};
+      e.type = KG_R;
+      e.plus = C->el[k].plus;
+      e.minus = ia;
+      e.branch = -1;
+      e.ctrl_plus = e.ctrl_minus = e.ctrl_branch = -1;
+      C->el[k].plus = ia;
+      C->recs.push_back(rr);
+      C->ids.push_back(C->ids[k] + ":rs");
+      C->el.push_back(e);
+    }
   }
   *out = reinterpret_cast<kg_circuit*>(C);
   return KG_OK;

@@ -20,12 +20,18 @@ from ._lib import KretsGpuError, lib, ptr
 
 # ------------------------------------------------------------- netlists
 class Circuit:
-    """A parsed netlist in krets' numbering (parser.rs:27-154)."""
+    """A parsed netlist in krets' numbering (parser.rs:27-154).
 
-    def __init__(self, text: str):
+    diode_parasitics=True reads the diode models' series resistance (an
+    internal node and resistor after krets' own unknowns and elements) and
+    junction charge (cjo, vj, m, fc, tt), which krets parses or ignores
+    without using (models/diode.rs:14-37)."""
+
+    def __init__(self, text: str, diode_parasitics: bool = False):
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
-        st = lib().kg_parse_netlist(text.encode(), ctypes.byref(h), err, 512)
+        flags = _lib.NETLIST_DIODE_PARASITICS if diode_parasitics else 0
+        st = lib().kg_parse_netlist_ext(text.encode(), flags, ctypes.byref(h), err, 512)
         if st != _lib.OK:
             raise KretsGpuError(st, err.value.decode())
         try:

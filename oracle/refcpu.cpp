@@ -79,6 +79,9 @@ struct Elem {
   double ac = 0.0;
   Wave w;
   double n_emis = 1.0;
+  // diode junction charge (opt-in parasitics): cjo, vj, m, fc, tt
+  double jq[5] = {0.0, 1.0, 0.5, 0.5, 0.0};
+  bool has_jq() const { return t == E_D && (jq[0] != 0.0 || jq[4] != 0.0); }
   std::string cplus, cminus, ctrl;  // E/G controlling nodes, F/H controlling element id
   int ip = -1, im = -1, ic = -1;    // lean indices
   int icp = -1, icm = -1, icb = -1;  // controlling node / branch indices
@@ -406,6 +409,7 @@ double wave_value(const Elem& e, double t) {
 
 struct DModel {
   double is = 1e-12, rs = 0.0, n = 1.0;
+  double cjo = 0.0, vj = 1.0, m = 0.5, fc = 0.5, tt = 0.0;  // junction charge (SPICE), opt-in only
   bool diode = false;
 };
 
@@ -444,6 +448,11 @@ bool parse_model_line(const std::string& raw, std::string& name, DModel& m) {
     if (k == "is") m.is = p.second;
     else if (k == "rs") m.rs = p.second;
     else if (k == "n") m.n = p.second;
+    else if (k == "cjo" || k == "cj0") m.cjo = p.second;
+    else if (k == "vj") m.vj = p.second;
+    else if (k == "m") m.m = p.second;
+    else if (k == "fc") m.fc = p.second;
+    else if (k == "tt") m.tt = p.second;
   }
   return true;
 }
@@ -602,7 +611,9 @@ std::map<std::string, SubDef> collect_subcircuits(const std::string& text) {
   return defs;
 }
 
-Circuit* parse_netlist(const std::string& text) {
+// flags & 1: the opt-in diode parasitics (series resistance as an internal
+// node and resistor numbered after krets' own, junction charge parameters).
+Circuit* parse_netlist(const std::string& text, int flags = 0) {
   const std::map<std::string, SubDef> defs = collect_subcircuits(text);
   auto* C = new Circuit();
   std::map<std::string, DModel> models;
@@ -697,6 +708,14 @@ Circuit* parse_netlist(const std::string& text) {
       }
       el.value = it->second.is;
       el.n_emis = it->second.n;
+      if (flags & 1) {
+        const DModel& md = it->second;
+        el.jq[0] = md.cjo;
+        el.jq[1] = md.vj;
+        el.jq[2] = md.m;
+        el.jq[3] = md.fc;
+        el.jq[4] = md.tt;
+      }
       C->nonlinear = true;
     }
     auto f = [&](const std::string& nd) {
@@ -722,6 +741,32 @@ Circuit* parse_netlist(const std::string& text) {
       }
       el.ctrl = id;
       el.icb = b->second;
+    }
+  }
+  if (flags & 1) {
+    // series resistance: an internal anode node V(<id>:a) and a resistor
+    // <id>:rs from the anode to it, after every krets unknown and element
+    const size_t ne0 = C->el.size();
+    for (size_t k = 0; k < ne0; k++) {
+      Elem& d = C->el[k];
+      if (d.t != E_D) continue;
+      const double rs = models.at(d.model).rs;
+      if (!(rs > 0.0)) continue;
+      const std::string inode = d.ident() + ":a";
+      const int ia = (int)C->names.size();
+      C->names.push_back("V(" + inode + ")");
+      C->index["V(" + inode + ")"] = ia;
+      Elem r;
+      r.t = E_R;
+      r.name = d.name + ":rs";
+      r.plus = d.plus;
+      r.minus = inode;
+      r.value = rs;
+      r.ip = d.ip;
+      r.im = ia;
+      d.plus = inode;
+      d.ip = ia;
+      C->el.push_back(r);
     }
   }
   return C;
@@ -1201,6 +1246,8 @@ void stamp_dc(const Circuit& C, const Idx& ix, const SolMap& sol, std::vector<Tr
   }
 }
 
+std::pair<double, double> junction_charge(const Elem& e, double v);
+
 void stamp_ac(const Circuit& C, const Idx& ix, const SolMap& op, double f, std::vector<Trip<cd>>& A,
               std::vector<Trip<cd>>& b) {
   for (const Elem& e : C.el) {
@@ -1257,7 +1304,14 @@ void stamp_ac(const Circuit& C, const Idx& ix, const SolMap& op, double f, std::
       }
       case E_D: {
         DiodeLin d = diode_compat(e, op);
-        four(A, p, m, cd(d.g, 0.0));
+        double c = 0.0;
+        if (e.has_jq()) {  // junction capacitance at the operating point (opt-in parasitics)
+          double vp = 0.5, vm = 0.0, tv;
+          if (op.getV(e, true, tv)) vp = tv;
+          if (op.getV(e, false, tv)) vm = tv;
+          c = junction_charge(e, vp - vm).second;
+        }
+        four(A, p, m, cd(d.g, 2.0 * PI_ * f * c));
         break;
       }
       default: stamp_controlled(e, ix, A); break;
@@ -1291,6 +1345,29 @@ std::vector<double> op_solve(const Circuit& C, const ref_config& cfg, int* iters
 }
 
 // --------------------------------------------------- clean-mode helpers
+// Diode junction charge (opt-in parasitics; SPICE2 diode model, Nagel 1975):
+// depletion charge CJO VJ / (1 - M) (1 - (1 - v/VJ)^(1-M)) and capacitance
+// CJO (1 - v/VJ)^-M below FC VJ, continued linearly in the capacitance above
+// it, plus the diffusion charge TT * I(v). Returns (q, c) at v.
+std::pair<double, double> junction_charge(const Elem& e, double v) {
+  const double cjo = e.jq[0], vj = e.jq[1], m = e.jq[2], fc = e.jq[3], tt = e.jq[4];
+  const double nvt = e.n_emis * VT, ex = std::exp(v / nvt);
+  double q = tt * e.value * (ex - 1.0), c = tt * e.value / nvt * ex;
+  if (cjo != 0.0) {
+    if (v < fc * vj) {
+      q += cjo * vj / (1.0 - m) * (1.0 - std::pow(1.0 - v / vj, 1.0 - m));
+      c += cjo * std::pow(1.0 - v / vj, -m);
+    } else {
+      const double f1 = vj / (1.0 - m) * (1.0 - std::pow(1.0 - fc, 1.0 - m));
+      const double f2 = std::pow(1.0 - fc, 1.0 + m), f3 = 1.0 - fc * (1.0 + m);
+      const double v0 = fc * vj;
+      q += cjo * (f1 + (f3 * (v - v0) + m / (2.0 * vj) * (v * v - v0 * v0)) / f2);
+      c += cjo / f2 * (f3 + m * v / vj);
+    }
+  }
+  return {q, c};
+}
+
 // SPICE3 pnjlim junction-voltage limiting.
 double pnjlim(double vnew, double vold, double vt, double vcrit, bool& limited) {
   limited = false;
@@ -1309,6 +1386,7 @@ double pnjlim(double vnew, double vold, double vt, double vcrit, bool& limited) 
 struct CleanState {
   std::vector<double> ic;  // capacitor currents (TRAP)
   std::vector<double> vl;  // inductor voltages (TRAP)
+  std::vector<double> iq;  // diode junction-charge currents (TRAP)
 };
 
 double vdiff(const Elem& e, const std::vector<double>& x) {
@@ -1385,6 +1463,12 @@ int clean_newton(const Circuit& C, const ref_config& cfg, int kind, double t, do
           double ex = std::exp(vd / nvt);
           double g = e.value / nvt * ex;
           double id = e.value * (ex - 1.0);
+          if (kind != 0 && e.has_jq()) {  // junction-charge companion of the BE / TRAP step
+            const auto q = junction_charge(e, vd), qo = junction_charge(e, vdiff(e, xprev));
+            const double gq = alpha * q.second / h;
+            id += alpha * (q.first - qo.first) / h - (alpha - 1.0) * (st.iq.empty() ? 0.0 : st.iq[k]);
+            g += gq;
+          }
           double ieq = id - g * vd;
           four(A, p, m, g);
           if (p >= 0) b.push_back({p, 0, -ieq});
@@ -1415,6 +1499,9 @@ void trap_update(const Circuit& C, double h, const std::vector<double>& xold, co
     const Elem& e = C.el[k];
     if (e.t == E_C) st.ic[k] = 2.0 * e.value / h * (vdiff(e, xnew) - vdiff(e, xold)) - st.ic[k];
     if (e.t == E_L) st.vl[k] = vdiff(e, xnew);
+    if (e.has_jq() && !st.iq.empty())
+      st.iq[k] = 2.0 * (junction_charge(e, vdiff(e, xnew)).first - junction_charge(e, vdiff(e, xold)).first) / h -
+                 st.iq[k];
   }
 }
 
@@ -1477,6 +1564,16 @@ const char* ref_last_error(void) { return g_last_error.c_str(); }
 void* ref_parse(const char* netlist, int32_t* status, char* err, int32_t errlen) {
   Circuit* C = nullptr;
   int32_t st = guarded([&] { C = parse_netlist(netlist); });
+  if (status) *status = st;
+  if (st != REF_OK && err && errlen > 0) {
+    std::snprintf(err, errlen, "%s", g_last_error.c_str());
+  }
+  return C;
+}
+
+void* ref_parse_ext(const char* netlist, int32_t flags, int32_t* status, char* err, int32_t errlen) {
+  Circuit* C = nullptr;
+  int32_t st = guarded([&] { C = parse_netlist(netlist, flags); });
   if (status) *status = st;
   if (st != REF_OK && err && errlen > 0) {
     std::snprintf(err, errlen, "%s", g_last_error.c_str());
@@ -1688,8 +1785,11 @@ int32_t ref_tran(void* c, const ref_config* cfg, double h, double stop, int32_t 
     CleanState st;
     st.ic.assign(C.el.size(), 0.0);
     st.vl.assign(C.el.size(), 0.0);
-    int it0 = clean_newton(C, *cfg, 0, 0.0, 1.0, std::vector<double>(n, 0.0), st, x0);
-    if (iters) iters[0] = it0;
+    st.iq.assign(C.el.size(), 0.0);
+    clean_newton(C, *cfg, 0, 0.0, 1.0, std::vector<double>(n, 0.0), st, x0);
+    // step 0 is the operating point, whose Newton count the OP entry point
+    // reports; as for krets' integrator, the step record starts at 0
+    if (iters) iters[0] = 0;
     for (size_t k = 0; k < C.el.size(); k++)
       if (C.el[k].t == E_L) st.vl[k] = vdiff(C.el[k], x0);
     std::copy(x0.begin(), x0.end(), x);
@@ -1715,6 +1815,7 @@ static void tran_lte_run(const Circuit& C, const ref_config& cfg, double tstop, 
   CleanState st;
   st.ic.assign(C.el.size(), 0.0);
   st.vl.assign(C.el.size(), 0.0);
+  st.iq.assign(C.el.size(), 0.0);
   std::vector<double> x0;
   int nt = clean_newton(C, cfg, 0, 0.0, 1.0, std::vector<double>(n, 0.0), st, x0);
   int rej = 0;
@@ -1758,7 +1859,7 @@ static void tran_lte_run(const Circuit& C, const ref_config& cfg, double tstop, 
       ratio = 0.0;
       for (const Elem& e : C.el) {
         double a, b0, bm;
-        if (e.t == E_C) { a = vdiff(e, xn); b0 = vdiff(e, xc); bm = vdiff(e, xprev); }
+        if (e.t == E_C || e.has_jq()) { a = vdiff(e, xn); b0 = vdiff(e, xc); bm = vdiff(e, xprev); }
         else if (e.t == E_L) { a = xn[e.ic]; b0 = xc[e.ic]; bm = xprev[e.ic]; }
         else continue;
         double dd2 = 2.0 * ((a - b0) / h - (b0 - bm) / hp) / (h + hp);

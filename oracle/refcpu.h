@@ -53,6 +53,11 @@ void ref_default_config(ref_config* cfg);
 /* Parse a netlist (R/C/L/V/I/D subset). Returns NULL on error, with the
  * status in *status and a message in err. */
 void* ref_parse(const char* netlist, int32_t* status, char* err, int32_t errlen);
+/* ref_parse with options: flags & 1 reads the opt-in diode parasitics (series
+ * resistance as an internal node V(<id>:a) plus a resistor <id>:rs numbered
+ * after krets' own unknowns and elements; junction charge cjo vj m fc tt),
+ * as the library's kg_parse_netlist_ext does. */
+void* ref_parse_ext(const char* netlist, int32_t flags, int32_t* status, char* err, int32_t errlen);
 void ref_free(void* circuit);
 int32_t ref_num_unknowns(void* circuit);
 int32_t ref_num_elements(void* circuit);

@@ -42,6 +42,8 @@ def lib():
         CFG = ctypes.POINTER(RefConfig)
         L.ref_parse.restype = P
         L.ref_parse.argtypes = [C, IP, ctypes.c_char_p, I]
+        L.ref_parse_ext.restype = P
+        L.ref_parse_ext.argtypes = [C, I, IP, ctypes.c_char_p, I]
         L.ref_free.argtypes = [P]
         L.ref_num_unknowns.argtypes = [P]
         L.ref_num_elements.argtypes = [P]
@@ -139,10 +141,10 @@ def dense_solve(A, b):
 class Circuit:
     """A parsed netlist in the oracle, numbered like krets."""
 
-    def __init__(self, netlist):
+    def __init__(self, netlist, diode_parasitics=False):
         st = ctypes.c_int32(0)
         err = ctypes.create_string_buffer(512)
-        self._h = lib().ref_parse(netlist.encode(), ctypes.byref(st), err, 512)
+        self._h = lib().ref_parse_ext(netlist.encode(), 1 if diode_parasitics else 0, ctypes.byref(st), err, 512)
         if not self._h:
             raise OracleError(st.value, err.value.decode())
         self.n = lib().ref_num_unknowns(self._h)

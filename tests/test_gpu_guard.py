@@ -1,0 +1,123 @@
+"""GPU: the AC sweeps' accuracy guard (static pivots checked in FP64).
+
+Pivots are fixed once per topology at one representative frequency and from
+one instance; krets instead pivots every solve (faer's sp_lu per frequency,
+crates/krets-solver/src/solver/ac.rs:67-95). Here every FP64 solve measures
+its backward error and every mixed solve its refinement estimate; solves
+that miss the guard are re-solved by the dense partial-pivoting fixup.
+
+Workload: a 4-section RLC ladder (uH, nF) whose second inductor spreads over
+four decades across 16 Monte Carlo instances, swept from 1 mHz to 1 GHz.
+At the top of the sweep the static inductor-branch pivots chosen at the
+sweep's centre lose backward stability (eta up to ~1e-4 without the guard,
+tests/test_jit.py). Every mapping must return, for every solve: status OK and
+the oracle's result to 1e-12 (FP64, with a backward error <= 2.5e-14). Mixed
+precision (complex64 factors, FP64-residual refinement) checks the normwise
+backward error of the solution it returns against the same 2^-46, so its
+promise on these solves (condition numbers up to 1e8) is the
+condition-number bound max(1e-9, 2.5e-14 * cond) of the oracle's solution.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def wide_rlc_batch(ni=16, seed=3):
+    rng = np.random.default_rng(seed)
+    nets = []
+    for i in range(ni):
+        R = rng.uniform(10, 100, 4)
+        L = rng.uniform(1, 10, 4) * 1e-6
+        C = rng.uniform(0.1, 1, 4) * 1e-9
+        L[1] *= 10.0 ** rng.uniform(-2, 2)  # a Monte Carlo spread of 1e4 in one element
+        s = "V1 n0 0 AC 1\n"
+        for k in range(1, 5):
+            s += f"R{k} n{k-1} m{k} {float(R[k-1])!r}\nL{k} m{k} n{k} {float(L[k-1])!r}\nC{k} n{k} 0 {float(C[k-1])!r}\n"
+        nets.append(s)
+    return nets
+
+
+FREQS = np.logspace(-3, 9, 25)
+
+
+@pytest.fixture(scope="module")
+def setup(oracle):
+    import krets_amd as kg
+    nets = wide_rlc_batch()
+    cs = [kg.Circuit(t) for t in nets]
+    ctx = kg.Context()
+    ctx.load(cs[0].elements, np.stack([c.params for c in cs]), cs[0].n)
+    ref, AB, kap = [], [], []
+    for t in nets:
+        oc = oracle.Circuit(t)
+        ref.append(oc.ac(FREQS))
+        AB.append([oc.assemble_ac(np.zeros(oc.n), f) for f in FREQS])
+        kap.append([np.linalg.norm(A, np.inf) * np.linalg.norm(np.linalg.inv(A), np.inf) for A, _ in AB[-1]])
+    return kg, ctx, ref, AB, kap
+
+
+def eta(A, b, x):
+    return np.max(np.abs(b - A @ x)) / (np.max(np.abs(A).sum(1)) * np.max(np.abs(x)) + np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("prec,tps", [(0, 2001), (1, 2002), (1, 2001), (0, 16), (1, 16), (0, 64), (0, 1016)])
+def test_guard_keeps_every_solve_backward_stable(setup, prec, tps):
+    kg, ctx, ref, AB, kap = setup
+    say(f"guard wide RLC prec={prec} tps={tps}")
+    xs, st, err = ctx.ac_sweep(FREQS, precision=prec, config=kg.SolverConfig(threads_per_system=tps))
+    assert (st == 0).all(), st
+    worst_eta, worst_fwd, worst_rel = 0.0, 0.0, 0.0
+    for i in range(len(ref)):
+        for k in range(len(FREQS)):
+            A, b = AB[i][k]
+            worst_eta = max(worst_eta, eta(A, b, xs[i][k]))
+            r = ref[i][k]
+            fwd = np.linalg.norm(xs[i][k] - r) / np.linalg.norm(r)
+            worst_fwd = max(worst_fwd, fwd)
+            # mixed precision: every returned solve has a normwise backward error within the guard (2^-46 in
+            # |re| + |im| norms, 2.5e-14 here), so its forward error is at most the condition number (up to
+            # 1e8 here) times that
+            worst_rel = max(worst_rel, fwd / max(1e-9, 2.5e-14 * kap[i][k]))
+    if prec == 0:
+        # the guard's threshold, 2^-46 on |re| + |im| entry norms, is at most 2e-14 in the modulus norms used here
+        assert worst_eta <= 2.5e-14, worst_eta
+        assert worst_fwd <= 1e-12, worst_fwd
+    else:
+        assert worst_eta <= 2.5e-14, worst_eta
+        assert worst_rel <= 1.0, (worst_rel, worst_fwd)
+    if prec == 0:  # FP64: the largest backward error per instance
+        assert (err > 0).all() and (err <= 1.5e-14).all(), err
+    else:  # mixed: the refinement's estimate, or the fixup's last correction
+        assert (err <= 1e-9).all(), err
+
+
+def test_guard_conditioning_of_the_workload(setup):
+    """The batch is hard on purpose: the condition numbers reach 1e7-1e8 at
+    the top of the sweep (1e8), where the static order also loses stability."""
+    kap = np.array(setup[4])
+    assert kap.max() > 1e7 and np.median(kap) < 1e5
+
+
+def test_guard_singular_system_reports_decomposition(setup):
+    """A structurally singular AC system (a current source's branch row has no
+    diagonal in AC, quirk 5) reports Decomposition, as krets' sp_lu does,
+    through the default and the level-scheduled mappings. The lane kernel
+    needs a valid static pivot order and refuses to be forced onto it."""
+    import krets_amd as kg
+    say("guard singular AC system")
+    c = kg.Circuit("I1 0 1 1m\nR1 1 0 1k\nC1 1 0 1n\n")
+    ctx = kg.Context()
+    ctx.load(c.elements, c.params[None], c.n)
+    for tps in (0, 16):
+        _, st, _ = ctx.ac_sweep(np.array([1e3, 1e6]), config=kg.SolverConfig(threads_per_system=tps))
+        assert st[0] == 2, (tps, st)
+    with pytest.raises(kg.KretsGpuError):
+        ctx.ac_sweep(np.array([1e3, 1e6]), config=kg.SolverConfig(threads_per_system=2001))
