@@ -44,6 +44,73 @@ def ac_bytes_per_solve(n, n_elements, nf):
     return 16.0 * n + 8.0 * n_elements / nf
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Threads for the multi-core CPU baseline: the cores this process may
+    run on, at most 16 (a one-GPU box's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_rate(refcpu, netlists, f, cost, threads, seconds):
+    """Oracle AC solves/s over instances taken in order until `seconds` pass:
+    each instance is parsed and swept over f, as krets does per analysis.
+    Threads run independent instances (ctypes releases the GIL in the
+    oracle's calls). Returns (solves/s, instances done)."""
+    import threading
+    cfg = refcpu.config(cost=cost)
+    nxt = [0]
+    lock = threading.Lock()
+    t0 = time.perf_counter()
+
+    def work():
+        while time.perf_counter() - t0 < seconds:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= len(netlists):
+                return
+            refcpu.Circuit(netlists[i]).ac(f, cfg)
+
+    ts = [threading.Thread(target=work) for _ in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    done = min(nxt[0], len(netlists))
+    return done * len(f) / (time.perf_counter() - t0), done
+
+
+def copy_bandwidth_gbs(torch, dev):
+    """HBM bandwidth of a 1 GiB device-to-device copy (bytes read + written)."""
+    src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(5):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 5 * 2 * src.numel() * 8 / (a.elapsed_time(b) / 1e3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,7 +118,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0,
+                    help="seconds of CPU work per baseline mode (compat and lean, 1 thread and all threads)")
+    ap.add_argument("--no-others", action="store_true",
+                    help="leave out the measurements of the other workloads (C1, C3, C4, C5) from the line")
     ap.add_argument("--plain-instances", type=int, default=8)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default) prints the benchmark line; the others print measurement lines of the "
@@ -188,22 +258,27 @@ def main():
                 e = np.max(np.linalg.norm(xi - xr, axis=1) / np.linalg.norm(xr, axis=1))
                 err_oracle[key] = max(err_oracle[key], float(e))
         if not args.no_cpu:
-            cfgc = refcpu.config(cost=refcpu.COST_COMPAT)
-            t0 = time.perf_counter()
-            done = 0
-            while time.perf_counter() - t0 < args.cpu_seconds and done < ni:
-                refcpu.Circuit(b.netlists[done]).ac(f, cfgc)
-                done += 1
-            dt = time.perf_counter() - t0
-            cpu = {"value": done * nf / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+            nt = cpu_threads()
+            rate, done = cpu_rate(refcpu, b.netlists, f, refcpu.COST_COMPAT, 1, args.cpu_seconds)
+            lean, done_l = cpu_rate(refcpu, b.netlists, f, refcpu.COST_LEAN, 1, args.cpu_seconds)
+            rate_t, done_t = cpu_rate(refcpu, b.netlists, f, refcpu.COST_COMPAT, nt, args.cpu_seconds)
+            lean_t, done_lt = cpu_rate(refcpu, b.netlists, f, refcpu.COST_LEAN, nt, args.cpu_seconds)
+            cpu = {"value": rate, "unit": "solves/s", "cores": 1, "kind": "port",
                    "sample": f"{done} instances x {nf} frequencies, krets-cost restatement (string-keyed maps, "
-                             f"fresh ordering + partial-pivot sparse LU per solve), 1 thread"}
+                             f"fresh ordering + partial-pivot sparse LU per solve), 1 thread",
+                   "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                   "lean_1_thread": {"value": lean, "sample": f"{done_l} instances; the same results from integer "
+                                     "indices with the ordering reused"},
+                   "compat_threads": {"value": rate_t, "threads": nt, "sample": f"{done_t} instances, one per "
+                                      "thread at a time"},
+                   "lean_threads": {"value": lean_t, "threads": nt, "sample": f"{done_lt} instances"}}
     except Exception as e:  # the oracle is optional on the box; report why it is missing
         cpu = {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"unavailable: {e}"}
 
     bps = ac_bytes_per_solve(n, ne, nf)
     avg_kernel_s = (kms_m / 1000.0) / max(nl_m, 1)
     achieved = bps * nsolves / avg_kernel_s / 1e9
+    copy_gbs = copy_bandwidth_gbs(torch, dev)
     line = {
         "metric": "MNA solves/s: batched mixed-prec vs plain FP64 GPU at ≤1e-9 rel err; adj/fwd",
         "value": mixed_sps,
@@ -233,10 +308,28 @@ def main():
         "max_normwise_err_vs_oracle": err_oracle,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname_m), "kernel": kname_m,
-                     "avg_kernel_ms": 1000.0 * avg_kernel_s, "bytes_per_solve": bps},
-        "fp64_kernel": {"kernel": kname_d, "avg_kernel_ms": kms_d / max(nl_d, 1)},
+                     "avg_kernel_ms": 1000.0 * avg_kernel_s, "bytes_per_solve": bps,
+                     "measured_copy_gbs": copy_gbs, "frac_vs_measured_copy": achieved / copy_gbs},
+        "fp64_kernel": {"kernel": kname_d, "avg_kernel_ms": kms_d / max(nl_d, 1),
+                        "frac": bps * nsolves / (kms_d / max(nl_d, 1) / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "analyses_per_s": mixed_sps / nf,
+        "fp64_analyses_per_s": fp64_sps / nf,
         "cpu_baseline": cpu,
     }
+    if not args.no_others:
+        import bench_configs
+        others = {}
+        for name in ("c1", "c3", "c4", "c5"):
+            bench_configs._say(f"other workload {name}")
+            try:
+                others[name] = getattr(bench_configs, name)(args)
+            except Exception as e:  # the headline line is still printed; the failure is reported in it
+                others[name] = {"error": f"{type(e).__name__}: {e}"}
+        line["other_configs"] = others
+        c5 = others.get("c5", {}).get("companion_J_Vn1", {}).get("fp64", {})
+        if "adjoint_total_over_forward_wall" in c5:  # (checkpointing forward + backward) / forward, FP64, C5
+            line["adjoint_total_over_forward"] = c5["adjoint_total_over_forward_wall"]
+            line["adjoint_total_over_forward_kernel"] = c5["adjoint_total_over_forward_kernel"]
     print(json.dumps(line), flush=True)
 
 

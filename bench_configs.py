@@ -249,9 +249,13 @@ def c5(args):
             dta, ka, _, (J, st) = _timed(ctx, lambda: ctx.adjoint_forward(h, N, c, precision=prec), 1)
             dtb, kb, _, (g, st2) = _timed(ctx, lambda: ctx.adjoint_backward(), 1)
             # the backward timing covers k_adj_bwd and the operating-point kernel (kb is their mean)
+            # the whole gradient costs the checkpointing forward pass plus the backward pass
             r[key] = {"forward_transient_kernel_ms": kf, "adjoint_forward_kernel_ms": ka,
-                      "adjoint_backward_wall_s": dtb, "forward_transient_wall_s": dtf,
+                      "adjoint_backward_kernel_ms": kb,
+                      "adjoint_forward_wall_s": dta, "adjoint_backward_wall_s": dtb, "forward_transient_wall_s": dtf,
                       "backward_over_forward_wall": dtb / dtf,
+                      "adjoint_total_over_forward_wall": (dta + dtb) / dtf,
+                      "adjoint_total_over_forward_kernel": (ka + kb) / kf,
                       "J_mean": float(np.mean(J)), "J_zero_count": int((J == 0).sum())}
             if R and key == "fp64":
                 # relative error over the gradient entries in the normal FP64 range: where J
