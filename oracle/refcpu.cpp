@@ -1913,6 +1913,33 @@ int32_t ref_tran_lte_trace(void* c, const ref_config* cfg, double tstop, double 
   });
 }
 
+// Clean backward Euler on a given time grid: the step of ref_tran_lte
+// (clean_newton with the source at the step's end) without its controller.
+// times[0] must be 0; x[0] is the clean operating point.
+int32_t ref_tran_grid(void* c, const ref_config* cfg, const double* times, int32_t npts, double* x,
+                      int32_t* newton_total) {
+  return guarded([&] {
+    const Circuit& C = *as_c(c);
+    const int n = N_of(C);
+    if (npts < 1 || times[0] != 0.0) throw RefError{REF_UNEXPECTED, "the grid must start at t = 0"};
+    CleanState st;
+    st.ic.assign(C.el.size(), 0.0);
+    st.vl.assign(C.el.size(), 0.0);
+    st.iq.assign(C.el.size(), 0.0);
+    std::vector<double> xs;
+    int nt = clean_newton(C, *cfg, 0, 0.0, 1.0, std::vector<double>(n, 0.0), st, xs);
+    std::copy(xs.begin(), xs.end(), x);
+    for (int32_t s = 1; s < npts; s++) {
+      const double h = times[s] - times[s - 1];
+      if (!(h > 0.0)) throw RefError{REF_UNEXPECTED, "the grid must increase"};
+      std::vector<double> xp(x + (size_t)(s - 1) * n, x + (size_t)s * n), xn;
+      nt += clean_newton(C, *cfg, 1, times[s], h, xp, st, xn);
+      std::copy(xn.begin(), xn.end(), x + (size_t)s * n);
+    }
+    if (newton_total) *newton_total = nt;
+  });
+}
+
 double ref_wave_value(int32_t kind, const double* p, double t) {
   Elem e{};
   e.t = E_V;

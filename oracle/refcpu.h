@@ -110,6 +110,13 @@ int32_t ref_tran_lte_trace(void* c, const ref_config* cfg, double tstop, double 
                            double hmax, double lte_reltol, double lte_abstol, double* trace,
                            int32_t maxtr, int32_t* ntr);
 
+/* Clean backward Euler on a caller's time grid (times[0] = 0, increasing):
+ * the steps of ref_tran_lte without its controller, so that a run of the
+ * library's LTE transient can be checked on its own accepted grid. x is
+ * [npts][n]; x[0] is the clean operating point. */
+int32_t ref_tran_grid(void* c, const ref_config* cfg, const double* times, int32_t npts, double* x,
+                      int32_t* newton_total);
+
 /* Single functions of the restatement, exposed so that tests can pin them on
  * their own (no krets entry point corresponds):
  * ref_wave_value: Pulse::value_at (kind 1, p = v1 v2 td tr tf pw per) and

@@ -42,6 +42,8 @@ def lib():
         CFG = ctypes.POINTER(RefConfig)
         L.ref_parse.restype = P
         L.ref_parse.argtypes = [C, IP, ctypes.c_char_p, I]
+        L.ref_tran_grid.argtypes = [P, ctypes.POINTER(RefConfig), ctypes.POINTER(ctypes.c_double), I,
+                                    ctypes.POINTER(ctypes.c_double), IP]
         L.ref_parse_ext.restype = P
         L.ref_parse_ext.argtypes = [C, I, IP, ctypes.c_char_p, I]
         L.ref_free.argtypes = [P]
@@ -224,6 +226,15 @@ class Circuit:
                                        ctypes.byref(nt), ctypes.byref(rej)))
         k = npts.value
         return t[:k].copy(), x[:k].copy(), nt.value, rej.value
+
+    def tran_grid(self, times, cfg=None):
+        """Clean BE on the given grid (times[0] = 0): (x [npts, n], Newton total)."""
+        cfg = cfg or config()
+        t = np.ascontiguousarray(times, np.float64)
+        x = np.zeros((len(t), self.n))
+        nt = ctypes.c_int32()
+        self._check(lib().ref_tran_grid(self._h, ctypes.byref(cfg), _dp(t), len(t), _dp(x), ctypes.byref(nt)))
+        return x, nt.value
 
     def tran_lte_trace(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, cfg=None, maxtr=400000):
         """Every attempted step of the LTE controller: rows (t, h, ratio,

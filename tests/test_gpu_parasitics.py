@@ -120,7 +120,13 @@ def test_transient_lane_kernel_declines_junction_charge():
 
 @pytest.mark.parametrize("tps", [0, 1])
 def test_lte_with_junction_charge(oracle, tps):
-    from conftest import time_aligned
+    """The step-size controller runs the oracle's algorithm: the same numbers
+    of accepted points, Newton iterations and rejections, on grids that agree
+    to 1e-6 of the interval. Its decisions amplify rounding: on instance 7 a
+    1e-15 relative change of one resistor moves accepted times by 6e-16 s, and
+    an interpolated comparison then differs by 3e-9. So the solution is checked
+    on the library's own accepted grid, against the oracle's clean backward
+    Euler over exactly those times (ref_tran_grid), to 1e-9."""
     import krets_amd as kg
     say(f"parasitics LTE tps={tps}")
     nets = rectifier_batch(8, seed=9)
@@ -130,11 +136,14 @@ def test_lte_with_junction_charge(oracle, tps):
     args = (100e-9, 0.01e-9, 1e-16, 1e-9, 1e-3, 1e-6)
     t, xs, npts, nt, rej = ctx.transient_lte(*args, 20000, cfg)
     ocfg = oracle.config(vabstol=1e-9, iabstol=1e-12)
-    for i in (0, 3, 7):
-        tr, xr, ntr, rejr = oracle.Circuit(nets[i], diode_parasitics=True).tran_lte(*args, cfg=ocfg)
+    for i in range(8):
+        oc = oracle.Circuit(nets[i], diode_parasitics=True)
+        tr, xr, ntr, rejr = oc.tran_lte(*args, cfg=ocfg)
         assert npts[i] == len(tr) and nt[i] == ntr and rej[i] == rejr, (i, npts[i], len(tr), nt[i], ntr)
-        xa = time_aligned(tr, xr, t[i][:npts[i]])
-        e = max(nerr(xs[i][j], xa[j]) for j in range(len(tr)))
+        tg = np.asarray(t[i][:npts[i]])
+        assert np.max(np.abs(tg - tr)) <= 1e-6 * args[0], i
+        xg, _ = oc.tran_grid(tg, cfg=ocfg)
+        e = max(nerr(xs[i][j], xg[j]) for j in range(npts[i]))
         assert e < 1e-9, (i, e)
 
 

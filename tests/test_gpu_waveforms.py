@@ -23,8 +23,9 @@ pytestmark = pytest.mark.gpu
 WAVES = ("PULSE(0.5 2.5 1u 1u 2u 3u 10u)", "SIN(0.5 2 100k 5u 2e4 90)")
 NET = tuple(f"V1 in 0 0.25 {w}\nC1 in 0 1n\nR1 in out 1k\nR2 out 0 3k\n" for w in WAVES)
 # the parsed parameters (v1 v2 td tr tf pw per; vo va f td theta phase): SI suffixes multiply,
-# so td = 5u is 5 * 1e-6, one ulp below 5e-6, which decides the step at the sine's delay
-PULSE = (0.5, 2.5, 1e-6, 1e-6, 2e-6, 3e-6, 10e-6)
+# so td = 5u is 5 * 1e-6, one ulp below 5e-6, which decides the step at the sine's delay, and
+# per = 10u is 10 * 1e-6, one ulp below 1e-5
+PULSE = (0.5, 2.5, 1e-6, 1e-6, 2e-6, 3 * 1e-6, 10 * 1e-6)
 SIN = (0.5, 2.0, 100e3, 5 * 1e-6, 2e4, 90.0)
 H, NSTEPS = 0.1e-6, 400  # 40 us: four pulse periods, four sine periods after the delay
 

@@ -149,3 +149,19 @@ def test_junction_charge_transient_against_ode(oracle):
     xc, _ = oc.tran(0.1e-9, t_end, method=oracle.BE_COMPAT)
     xn, _ = oracle.Circuit(net).tran(0.1e-9, t_end, method=oracle.BE_COMPAT)
     assert np.array_equal(xc, xn)
+
+
+def test_grid_integrator_reproduces_the_lte_and_fixed_step_runs(oracle):
+    """ref_tran_grid (clean BE over a given grid, used to check the library's
+    LTE runs on their own accepted grids) reproduces the oracle's LTE run on
+    its grid and the fixed-step BE run on a uniform one."""
+    net = ("V1 in 0 SIN(0 2 20e6 0 0 0)\nR1 in a 100\nD1 a out dm\nRL out 0 5k\nCL out 0 5p\n"
+           ".model dm D(is=1e-14 n=1.5 rs=20 cjo=2p vj=0.7 m=0.4 fc=0.5 tt=5n)\n")
+    oc = oracle.Circuit(net, diode_parasitics=True)
+    cfg = oracle.config(vabstol=1e-9, iabstol=1e-12)
+    tr, xr, _, _ = oc.tran_lte(100e-9, 0.01e-9, 1e-16, 1e-9, 1e-3, 1e-6, cfg=cfg)
+    xg, _ = oc.tran_grid(tr, cfg=cfg)
+    assert np.max(np.abs(xg - xr)) <= 1e-13 * np.max(np.abs(xr))
+    xb, _ = oc.tran(0.1e-9, 50e-9, method=oracle.BE, cfg=cfg)
+    xu, _ = oc.tran_grid(np.arange(501) * 0.1e-9, cfg=cfg)
+    assert np.max(np.abs(xu - xb)) <= 1e-13 * np.max(np.abs(xb))
