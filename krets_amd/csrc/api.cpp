@@ -913,7 +913,7 @@ int32_t adjoint_backward_lin(kg_ctx* c, double* grad, int32_t* status, int32_t m
   hipError_t e;
   size_t ni = c->n_inst;
   int ne = (int)(c->eoff[1] - c->eoff[0]);
-  int64_t seg_stride = 2 * (int64_t)(aa.K + 1) * P.n_max;  // two segment buffers (pipelined recomputation)
+  int64_t seg_stride = (int64_t)(aa.K + 1) * P.n_max;
   double* d_ck = (double*)ws_get(c, "adj_ckpt", 8, &e);
   double* d_c = (double*)ws_get(c, "adj_cvec", 8, &e);
   double* d_seg = (double*)ws_get(c, "adj_seg", sizeof(double) * ni * seg_stride, &e);

@@ -89,8 +89,10 @@ __device__ __forceinline__ float cmax(float m, W a) {
   return __builtin_fmaxf(m, __builtin_fmaxf(__builtin_fabsf(a.re), __builtin_fabsf(a.im)));
 }
 __device__ __forceinline__ bool fin(Z a) { return __builtin_isfinite(a.re) && __builtin_isfinite(a.im); }
-// |re| + |im|: the entry norm of the backward-error guard
+// |re| + |im|: the entry norm of the growth guard (submultiplicative, within
+// a factor sqrt(2) of the modulus)
 __device__ __forceinline__ double c1(Z a) { return __builtin_fabs(a.re) + __builtin_fabs(a.im); }
+__device__ __forceinline__ double c1(W a) { return (double)__builtin_fabsf(a.re) + (double)__builtin_fabsf(a.im); }
 // queues system `sys` for the dense partial-pivoting fixup (queue words: [0]
 // count, [1 + q] systems); a full queue marks the instance instead
 __device__ __forceinline__ void fix_push(unsigned long long* fixq, i64 cap, i64 sys, int* status, int inst) {
@@ -149,11 +151,6 @@ __device__ __forceinline__ i64 xcd_block() {
 #ifndef KG_LANE_FWD_BARRIER
 #define KG_LANE_FWD_BARRIER 1
 #endif
-// rows of back substitution between the reloads of element values for the
-// FP64 guard's residual and the residual rows that use them
-#ifndef KG_LANE_AHEAD_RESID
-#define KG_LANE_AHEAD_RESID 4
-#endif
 #ifndef KG_LANE_BWD_BARRIER
 #define KG_LANE_BWD_BARRIER 4
 #endif
@@ -182,12 +179,12 @@ struct Gen {
   // instances reads one contiguous run per value
   bool emajor = false;
   int lookahead = KG_LANE_AHEAD;  // rows between a load and the row that first uses it
-  // names of element-value loads ("g", "p"; the residual of the FP64 guard
-  // loads its own copies, "h", "q", close to their use)
+  // names of element-value loads
   std::string gname = "g", pname = "p";
-  // normwise residual: x_j read from this array of the stored solution (in
-  // plan column order through col_of) when set, else from xref
-  std::string xo;
+  // growth guard (first pass): an = ||A||_inf and gm = || |L||U| ||_inf in
+  // |re| + |im| entry norms, the latter bounded from above through
+  // |l_ik| <= |v_ik| |1/u_kk| (c1 is submultiplicative)
+  bool growth = false;
   std::function<void(std::ostringstream&, int)> on_result;
   explicit Gen(const View& v) : P(v) {}
 
@@ -285,47 +282,6 @@ struct Gen {
     return "(cz(" + nm + ") + cz(dx" + std::to_string(j) + "))";
   }
 
-  // FP64 residual of row i for the backward-error guard, from the FP64
-  // stamps and the back-substitution results t<j>: the row's componentwise
-  // backward error |r_i| / (|A| |x| + |b|)_i (Oettli-Prager; |z| = |re| +
-  // |im|, the reciprocal from the hardware estimate) into the running maximum
-  // om. Element values are loaded into `loads` (prefixes h / q).
-  //
-  // normwise (mixed precision, x_j from xo or xref): accumulates instead
-  // |r_i| into rm, the row's sum of |a| into am and |b_i| into bm, for
-  // eta = rm / (am xm + bm).
-  bool residual_row(int i, std::ostringstream& o, bool normwise = false) {
-    std::string b;
-    if (!rhs(i, b)) return false;
-    const char* xp = "t";
-    o << "    {\n      Z rr = Z{" << (b.empty() ? std::string("0.0") : b) << ", 0.0};\n      double dr = "
-      << (b.empty() ? "0.0" : "fabs(rr.re)") << ";\n";
-    if (normwise && !b.empty()) o << "      bm = fmax(bm, dr);\n      dr = 0.0;\n";
-    for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
-      if (!has_slot(s)) continue;
-      std::string re, im;
-      if (!slot(s, re, im)) return false;
-      const std::string x = !normwise ? xp + std::to_string(P.col[s])
-                            : !xo.empty() ? xo + "[" + std::to_string(P.col_of[P.col[s]]) + "]"
-                                          : xref(P.col[s]);
-      if (normwise) {
-        if (im.empty()) o << "      { const double a = " << re << "; rr = rr - rmul(a, " << x << "); dr += fabs(a); }\n";
-        else o << "      { const Z a = " << zexpr(re, im) << "; rr = rr - a * " << x << "; dr += c1(a); }\n";
-      } else if (im.empty()) {
-        o << "      { const double a = " << re << "; rr = rr - rmul(a, " << x << "); dr += fabs(a) * c1(" << x << "); }\n";
-      } else {
-        o << "      { const Z a = " << zexpr(re, im) << "; rr = rr - a * " << x << "; dr += c1(a) * c1(" << x << "); }\n";
-      }
-    }
-    if (normwise) {
-      o << "      rm = fmax(rm, c1(rr));\n      am = fmax(am, dr);\n    }\n";
-      return true;
-    }
-    // 0 / 0 (an all-zero row with a zero residual) gives NaN, which fmax ignores
-    o << "      om = fmax(om, c1(rr) * __builtin_amdgcn_rcp(dr));\n    }\n";
-    return true;
-  }
-
   // One factorisation + forward + backward pass in type F ("Z" or "W"),
   // appended to `out`. mode 0: right-hand side b; mode 1: scaled FP64
   // residual b - A x. Row i's back-substitution result t<i> is handed to
@@ -357,6 +313,13 @@ struct Gen {
           o << "    " << F << " v" << s << " = " << conv(F, zexpr(re, im)) << ";\n";
         }
       }
+      const bool gg = growth && mode == 0;
+      if (gg) {  // ||A||_inf: this row's sum of |a|
+        o << "    an = fmax(an, 0.0";
+        for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++)
+          if (has_slot(s)) o << " + c1(v" << s << ")";
+        o << ");\n    gsr = 0.0;\n";
+      }
       // right-hand side of row i
       bool ynz = false;
       std::string b;
@@ -381,6 +344,8 @@ struct Gen {
       for (int kk = 0; kk < nk; kk++) {
         int sik = fp[0], k = P.col[sik], nu = fp[2];
         fp += 3;
+        // |l_ik| (row sum of |U| row k) <= |v_ik| w_k, v_ik final here
+        if (gg) o << "    gsr = gsr + c1(v" << sik << ") * gw" << k << ";\n";
         for (int u = 0; u < nu; u++, fp += 2) {
           int dst = fp[0], src = fp[1];
           if (!(src > P.diag[k] && src < P.row_ptr[k + 1])) { why = "factor program reads outside U"; return false; }
@@ -405,6 +370,12 @@ struct Gen {
       int d = P.diag[i];
       if (mode == 0 || i == n - 1) o <<"    { const Z pz = cz(v" << d << "); bad |= !(m2(pz) > 0.0) || !fin(pz); }\n";
       o << "    const " << F << " rd" << i << " = rcpF(v" << d << ");\n";
+      if (gg) {  // row i of U: its sum of |u|, the diagonal term of |L||U|, and w_i = |1/u_ii| sum
+        o << "    const double gr" << i << " = c1(v" << d << ")";
+        for (int s = d + 1; s < P.row_ptr[i + 1]; s++) o << " + c1(v" << s << ")";
+        o << ";\n    gm = fmax(gm, gsr + gr" << i << ");\n    const double gw" << i << " = c1(rd" << i << ") * gr" << i
+          << ";\n";
+      }
       for (int s = d + 1; s < P.row_ptr[i + 1]; s++)
         o << "    const " << F << " u" << s << " = " << mul(real[s], "v" + std::to_string(s), "rd" + std::to_string(i))
           << ";\n";
@@ -467,17 +438,20 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
   G.emajor = mixed;
   std::ostringstream& o = G.out;
   o << kPrelude;
-  std::string final_eta;  // mixed precision: the final check, placed before the kernel
   o << "extern \"C\" __global__ void __launch_bounds__(64) " << jit_ac_name(mixed, xsplit)
     << "(const double* __restrict__ prm,\n"
        "    const double* __restrict__ ebase, const double* __restrict__ freqs, int nf, i64 nsys,\n"
        "    double* __restrict__ out, int* __restrict__ status, double* __restrict__ err, int refine,\n"
        "    double stop, unsigned long long* __restrict__ fixq, i64 fixcap) {\n"
-       "  const i64 sys = xcd_block() * 64 + threadIdx.x;\n"
-       "  if (sys >= nsys) return;\n";
+       "  const i64 sys0 = xcd_block() * 64 + threadIdx.x;\n";
   if (!mixed) {  // consecutive frequencies of one instance per wave (element loads shared by the wave)
-    o << "  const int inst = (int)(sys / nf), fi = (int)(sys - (i64)inst * nf);\n";
+    // lanes past the last system repeat it (identical values to the same
+    // addresses; they queue nothing), so that the whole wave stays active for
+    // the error figure's wave reduction
+    o << "  const bool live = sys0 < nsys;\n  const i64 sys = live ? sys0 : nsys - 1;\n"
+         "  const int inst = (int)(sys / nf), fi = (int)(sys - (i64)inst * nf);\n";
   } else {
+    o << "  if (sys0 >= nsys) return;\n  const i64 sys = sys0;\n";
     // 64 instances at one frequency per wave: refinement converges alike
     // across the wave, so the adaptive stop applies to whole waves
     // (measured: 0.49 ms instead of 0.78 ms on the 4096 x 200 RC-ladder
@@ -500,62 +474,37 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     << "  const double w = (2.0 * 3.14159265358979323846) * freqs[fi];\n"
     << "  Z* o = reinterpret_cast<Z*>(out) + ((i64)inst * nf + fi) * " << n << ";\n"
     << "  bool bad = false;\n";
+  // growth guard: computed during the first pass' factorisation
+  G.growth = true;
+  int mrow = 1;
+  for (int i = 0; i < n; i++) mrow = std::max(mrow, (int)(P.row_ptr[i + 1] - P.row_ptr[i]));
+  o << "  double an = 0.0, gm = 0.0, gsr = 0.0;\n";
   if (!mixed) {
-    // Backward-error guard: residual row r is emitted right after the back
-    // substitution has produced the last x it reads (its smallest column in
-    // pivot order), so the solution stays in registers only while rows still
-    // need it; its element values are reloaded KG_LANE_AHEAD_RESID rows ahead.
-    std::vector<std::vector<int>> done_at(n);
-    for (int r = 0; r < n; r++) {
-      int mn = n - 1;
-      for (int s = P.row_ptr[r]; s < P.row_ptr[r + 1]; s++)
-        if (G.has_slot(s)) mn = std::min(mn, (int)P.col[s]);
-      done_at[mn].push_back(r);
-    }
-    std::vector<std::string> rcode(n);
-    std::vector<std::vector<std::string>> rloads(n);
-    for (int i = n - 1; i >= 0; i--) {
-      G.loads.clear();
-      std::ostringstream ro;
-      for (int r : done_at[i]) {
-        // each residual row loads its own copies: short live ranges
-        G.gname = "h" + std::to_string(r) + "_";
-        G.pname = "q" + std::to_string(r) + "_";
-        G.have_g.clear();
-        G.have_p.clear();
-        if (!G.residual_row(r, ro)) { why = G.why; return false; }
-      }
-      rcode[i] = ro.str();
-      rloads[i] = G.loads;
-    }
-    G.gname = "g";
-    G.pname = "p";
-    const int la = KG_LANE_AHEAD_RESID;
-    auto emit_loads = [&](std::ostringstream& O, int j) {
-      if (j < 0 || rloads[j].empty()) return;
-      for (auto& l : rloads[j]) O << l;
-      O << "    asm volatile(\"\" ::: \"memory\");\n";
-    };
-    // the residual rows completed at step j run right after it; their loads
-    // go out la steps earlier (the first steps' loads all at step n - 1)
-    std::vector<std::vector<int>> code_at(n), load_at(n);
-    for (int j = n - 1; j >= 0; j--) {
-      code_at[j].push_back(j);
-      load_at[std::min(j + la, n - 1)].push_back(j);
-    }
-    G.on_result = [&](std::ostringstream& O, int i) {
-      O << "    o[" << P.col_of[i] << "] = t" << i << ";\n";
-      for (int j : load_at[i]) emit_loads(O, j);
-      for (int j : code_at[i]) O << rcode[j];
-    };
-    o << "  double om = 0.0;\n  {\n";
+    G.on_result = [&](std::ostringstream& O, int i) { O << "    o[" << P.col_of[i] << "] = t" << i << ";\n"; };
+    o << "  {\n";
     if (!G.pass("Z", 0)) { why = G.why; return false; }
-    // the componentwise backward error bounds the normwise one: the figure
-    // published for the instance bounds every accepted solve's eta
+    // rho = gm / an above KG_GUARD_GROWTH (or a failed pivot): re-solved by
+    // the fixup; else the instance's figure is the backward-error bound
+    // c_m rho of the solve
+    // the figure is reduced over the wave first: its lanes hold one or two
+    // instances (consecutive frequencies), and 64 atomics on one address
+    // serialise (a segmented maximum over lanes of the same instance, then
+    // one atomic per instance)
     o << "  }\n"
-      << "  if (bad || !(om <= " << fmt_double(KG_GUARD_OMEGA) << ")) fix_push(fixq, fixcap, (i64)inst * nf + fi, status, inst);\n"
-      << "  else if (err != nullptr && om > 0.0)\n"
-      << "    atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(om));\n"
+      << "  const bool queue = bad || !(gm <= " << fmt_double(KG_GUARD_GROWTH) << " * an);\n"
+      << "  if (queue && live) fix_push(fixq, fixcap, (i64)inst * nf + fi, status, inst);\n"
+      << "  if (err != nullptr) {\n"
+      << "    const int lane = threadIdx.x;\n"
+      << "    double ev = queue || !(gm > 0.0) ? 0.0 : " << fmt_double(guard_bound_coeff(mrow)) << " * (gm / an);\n"
+      << "    for (int off = 1; off < 64; off <<= 1) {\n"
+      << "      const double ov = __shfl_down(ev, off, 64);\n"
+      << "      const int oi = __shfl_down(inst, off, 64);\n"
+      << "      if (lane + off < 64 && oi == inst) ev = fmax(ev, ov);\n"
+      << "    }\n"
+      << "    const int pi = __shfl_up(inst, 1, 64);\n"
+      << "    if ((lane == 0 || pi != inst) && ev > 0.0)\n"
+      << "      atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(ev));\n"
+      << "  }\n"
       << "}\n";
     src = o.str();
     return true;
@@ -578,7 +527,9 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
     G.track_bm = true;
     o << "  {\n";
     if (!G.pass("W", 0)) { why = G.why; return false; }
-    o << "  }\n";
+    // the growth verdict is taken here, so that only a lane mask stays live
+    // through the refinement
+    o << "  }\n  bad |= !(gm <= " << fmt_double(KG_GUARD_GROWTH) << " * an);\n";
     G.track_bm = false;
     // refinement: corrections scaled by |b|_inf stay in FP32 range
     o << "  const double s = sqrt(bm2), inv_s = s > 0.0 ? 1.0 / s : 0.0, xm2 = (double)xmf * (double)xmf;\n"
@@ -619,55 +570,20 @@ bool jit_ac_source(const PatLayout& L, const int32_t* buf, bool mixed, bool xspl
       << "    const double rho = (npass > 1 && dp > 0.0) ? fmin(dm / dp, 1.0) : 1.0;\n"
       << "    est = xm > 0.0 ? (npass == 1 && stop > 0.0 ? " << KG_REFINE_SAFETY
       << " * d2last / xm2 : rho * dm / xm) : 0.0;\n  }\n";
-    // the solution returned, and its normwise backward error in FP64 from the
-    // FP64 stamps: refinement can settle (or stall) short of FP64 accuracy on
-    // ill-conditioned points while its correction-based estimate stays small
-    // (the residual runs in a function of its own that reads the stored
-    // solution back: inlined after the refinement loop, it made the compiler
-    // spill inside the loop; a call leaves the loop's allocation as it is.
-    // A lane's loads after its own stores to the same addresses see them:
-    // one wave's vector memory operations complete in order.)
-    o << "  double xm = 0.0;\n";
     for (int i = 0; i < n; i++) {
-      if (G.xsplit) o << "  { const Z xf = cz(xs[" << 64 * i << " + lane]) + cz(dx" << i << ");\n";
-      else o << "  { const Z xf = x" << i << ";\n";
-      o << "  o[" << P.col_of[i] << "] = xf; xm = fmax(xm, c1(xf)); }\n";
+      if (G.xsplit) o << "  o[" << P.col_of[i] << "] = cz(xs[" << 64 * i << " + lane]) + cz(dx" << i << ");\n";
+      else o << "  o[" << P.col_of[i] << "] = x" << i << ";\n";
     }
-    o << "  const double eta_f = kg_final_eta(o, PR, EBR, jl8, w, xm);\n";
-    std::ostringstream fo;
-    fo << "// normwise backward error (FP64, |re| + |im|) of the solution stored in o\n"
-          "__device__ __attribute__((noinline)) double kg_final_eta(const Z* __restrict__ o,\n"
-          "    const double* __restrict__ PR, __amdgpu_buffer_rsrc_t EBR, int jl8, double w, double xm) {\n"
-          "  double rm = 0.0, am = 0.0, bm = 0.0;\n";
-    G.xo = "o";
-    for (int i = 0; i < n; i++) {
-      std::ostringstream ro;
-      G.loads.clear();
-      G.gname = "f" + std::to_string(i) + "_";
-      G.pname = "fq" + std::to_string(i) + "_";
-      G.have_g.clear();
-      G.have_p.clear();
-      if (!G.residual_row(i, ro, true)) { why = G.why; return false; }
-      for (auto& l : G.loads) fo << l;
-      fo << ro.str();
-    }
-    fo << "  const double den = am * xm + bm;\n"
-          "  return den > 0.0 ? rm / den : (rm > 0.0 ? __builtin_inf() : 0.0);\n}\n";
-    final_eta = fo.str();
-    // a solve whose estimate or final backward error misses the guard (or
-    // whose complex64 pivots failed) is solved again in FP64 with partial
-    // pivoting by the fixup
-    o << "  if (bad || !(est <= " << fmt_double(KG_GUARD_MIXED) << ") || !(eta_f <= " << fmt_double(KG_GUARD_MIXED_ETA)
-      << "))\n    fix_push(fixq, fixcap, (i64)inst * nf + fi, status, inst);\n"
+    // a solve whose complex64 factorisation grew (rho above
+    // KG_GUARD_GROWTH), whose estimate stays above KG_GUARD_MIXED or whose
+    // complex64 pivots failed is solved again in FP64 with partial pivoting
+    // by the fixup
+    o << "  if (bad || !(est <= " << fmt_double(KG_GUARD_MIXED) << "))\n    fix_push(fixq, fixcap, (i64)inst * nf + fi, status, inst);\n"
          "  else if (err != nullptr && est > 0.0)\n"
          "    atomicMax(reinterpret_cast<unsigned long long*>(&err[inst]), (unsigned long long)__double_as_longlong(est));\n";
   }
   o << "}\n";
   src = o.str();
-  if (!final_eta.empty()) {
-    const size_t k = src.find("extern \"C\" __global__");
-    src.insert(k, final_eta);
-  }
   return true;
 }
 

@@ -1370,61 +1370,47 @@ struct LinSys {
     }
   }
 
-  // One forward-substitution level l of L y = b in place in xc (no barrier).
-  __device__ void fwd_level(int l) {
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (f_lev[r] != l) continue;
-      const int i = (int)(f_ip[r] & 0xFFFF), p0 = (int)(f_ip[r] >> 16), cnt = (int)(f_nd[r] & 0xFFFF);
-      double sum = xc[i];
-      double vv[LIN_KE], xx[LIN_KE];
-#pragma unroll
-      for (int k = 0; k < LIN_KE; k++) {
-        vv[k] = v[k < cnt ? p0 + k : nnz];
-        xx[k] = xc[k & 1 ? f_col[r][k / 2] >> 16 : f_col[r][k / 2] & 0xFFFF];
-      }
-#pragma unroll
-      for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];  // padded entries subtract 0 * 0
-      if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
-      xc[i] = sum;
-    }
-  }
-
-  // One back-substitution level l of U x = y in place in xc (no barrier).
-  __device__ void bwd_level(int l) {
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      if (b_lev[r] != l) continue;
-      const int i = (int)(b_ip[r] & 0xFFFF), p0 = (int)(b_ip[r] >> 16), cnt = (int)(b_nd[r] & 0xFFFF);
-      const int d = (int)(b_nd[r] >> 16);
-      double sum = xc[i];
-      const double vd = v[d];
-      double vv[LIN_KE], xx[LIN_KE];
-#pragma unroll
-      for (int k = 0; k < LIN_KE; k++) {
-        vv[k] = v[k < cnt ? p0 + k : nnz];
-        xx[k] = xc[k & 1 ? b_col[r][k / 2] >> 16 : b_col[r][k / 2] & 0xFFFF];
-      }
-#pragma unroll
-      for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-      if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
-      xc[i] = sum * vd;  // diagonal stored as its reciprocal
-    }
-  }
-
-  // Phases of one step after the right-hand side: the forward levels (level
-  // 0 rows, when free of lower entries, are the right-hand side itself), then
-  // the backward levels. solve_phase(p), p < solve_phases(), does phase p.
-  __device__ int solve_phases() const { return (nlev - lev0) + nblev; }
-  __device__ void solve_phase(int p) {
-    if (p < nlev - lev0) fwd_level(lev0 + p);
-    else bwd_level(p - (nlev - lev0));
-  }
-
-  // L y = b, U x = y in place in xc; ends with a barrier.
+  // L y = b, U x = y in place in xc; ends with a barrier. Level 0 rows, when
+  // free of lower entries, are the right-hand side itself.
   __device__ void solve() {
-    for (int p = 0; p < solve_phases(); p++) {
-      solve_phase(p);
+    for (int l = lev0; l < nlev; l++) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (f_lev[r] != l) continue;
+        const int i = (int)(f_ip[r] & 0xFFFF), p0 = (int)(f_ip[r] >> 16), cnt = (int)(f_nd[r] & 0xFFFF);
+        double sum = xc[i];
+        double vv[LIN_KE], xx[LIN_KE];
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) {
+          vv[k] = v[k < cnt ? p0 + k : nnz];
+          xx[k] = xc[k & 1 ? f_col[r][k / 2] >> 16 : f_col[r][k / 2] & 0xFFFF];
+        }
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];  // padded entries subtract 0 * 0
+        if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
+        xc[i] = sum;
+      }
+      __syncthreads();
+    }
+    for (int l = 0; l < nblev; l++) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        if (b_lev[r] != l) continue;
+        const int i = (int)(b_ip[r] & 0xFFFF), p0 = (int)(b_ip[r] >> 16), cnt = (int)(b_nd[r] & 0xFFFF);
+        const int d = (int)(b_nd[r] >> 16);
+        double sum = xc[i];
+        const double vd = v[d];
+        double vv[LIN_KE], xx[LIN_KE];
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) {
+          vv[k] = v[k < cnt ? p0 + k : nnz];
+          xx[k] = xc[k & 1 ? b_col[r][k / 2] >> 16 : b_col[r][k / 2] & 0xFFFF];
+        }
+#pragma unroll
+        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+        if (cnt > LIN_KE) sum -= row_tail(v, xc, col, p0 + LIN_KE, p0 + cnt);
+        xc[i] = sum * vd;  // diagonal stored as its reciprocal
+      }
       __syncthreads();
     }
   }
@@ -1565,7 +1551,7 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_fwd(BatchDev B, int kind_slot, 
   LinSys<TPS, R> S;
   const LinHdr* Hg = reinterpret_cast<const LinHdr*>(P.lin);
   char* q = S.setup(P, Hg->words, smem, ta, srcv, nsrc_max, inst);
-  q += 8 * (5 * ((size_t)S.n + 1) + (size_t)S.ndyn + 1 + (size_t)Hg->ne + 1);  // backward-only arrays
+  q += 8 * (4 * ((size_t)S.n + 1) + (size_t)S.ndyn + 1 + (size_t)Hg->ne + 1);  // backward-only arrays
   double* red = reinterpret_cast<double*>(q);
   int* flags = reinterpret_cast<int*>(q + 8 * 32);
   if (t == 0) flags[0] = KG_OK;
@@ -1614,20 +1600,13 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_fwd(BatchDev B, int kind_slot, 
 }
 
 // Backward pass of the discrete adjoint from LDS (k_adj_bwd semantics): per
-// checkpoint segment, last to first, each step's adjoint runs backwards: the
+// checkpoint segment, last to first, recompute the segment's solutions into
+// seg (permuted order), then run each step's adjoint backwards: the
 // companions' adjoint loads, lambda = A^-T xbar by the transposed level
 // solves, the gradient terms of every R, C and L and the companions' adjoint
-// states. The states a segment's adjoint needs are recomputed from its
-// checkpoint, and that recomputation is software-pipelined with the adjoint
-// of the segment after it: while the adjoint walks segment k backwards, the
-// same threads recompute segment k - 1 forwards, one step of each per
-// iteration, with the phases of the two (the adjoint's transposed levels,
-// the recomputation's forward and backward levels) sharing barriers. A step
-// of the backward pass thus costs about one solve's worth of barrier phases
-// instead of two. Recomputed states go to a double-buffered segment store
-// (seg: two rows of K + 1 states per instance, permuted order); the adjoint
-// loads x_{s-1} into registers one step ahead. xbar0 receives dJ/dx_0 (krets
-// numbering) for the operating-point term; grad: [instance][ne].
+// states. The recomputed state x_{s-1} is loaded into registers one step
+// ahead and staged into LDS at the end of step s. xbar0 receives dJ/dx_0
+// (krets numbering) for the operating-point term; grad: [instance][ne].
 template <int TPS, int R>
 __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, TranArgs ta, int K,
                                                      const double* cvec, const double* ckpt, int64_t ck_stride,
@@ -1664,8 +1643,6 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   q += 8 * ((size_t)n + 1);
   double* XB = reinterpret_cast<double*>(q);
   q += 8 * ((size_t)n + 1);
-  double* lam = reinterpret_cast<double*>(q);  // lambda = A^-T xbar
-  q += 8 * ((size_t)n + 1);
   double* sbar = reinterpret_cast<double*>(q);  // companion adjoint states
   q += 8 * ((size_t)ndyn + 1);
   double* gr = reinterpret_cast<double*>(q);  // gradient per element
@@ -1673,7 +1650,7 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   int* flags = reinterpret_cast<int*>(q + 8 * 32);
   if (t == 0) {
     flags[0] = KG_OK;
-    XA[n] = XB[n] = lam[n] = 0.0;
+    XA[n] = XB[n] = 0.0;
   }
   __syncthreads();
   const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
@@ -1736,26 +1713,67 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   for (int d = t; d <= ndyn; d += TPS) sbar[d] = 0.0;
   for (int e = t; e <= ne; e += TPS) gr[e] = 0.0;
   if (t == 0) xbar[n] = xb0[n] = 0.0;
+  __syncthreads();
   const double* ck = ckpt + (int64_t)inst * ck_stride;
-  const int64_t half = (int64_t)(K + 1) * n;  // one segment buffer: states x_{s0} .. x_{s0+K}
-  double* sgbase = seg + (int64_t)inst * seg_stride;
+  double* sg = seg + (int64_t)inst * seg_stride;
   const int nseg = (ta.nsteps + K - 1) / K;
-
-  // ---- phases of one adjoint step s (segment states in sa: row j = x_{sa0+j})
-  // 0 companions' loads; 1 lambda copy (fused U^T level 0); U^T levels; L^T
-  // levels; element terms; staging of the next step
-  const int nUT = ntlev - tl0, nLT = ntblev - tl0;
-  const int a_phases = 2 + nUT + nLT + 2;
-  double* X1 = XA;  // x_{s+1}
-  double* Xs = XB;  // x_s
-  double nx[R];     // x_{s-1}, loaded one step ahead
-  auto adj_phase = [&](int ph, int s, int s0, const double* sa) {
-    if (ph == 0) {
+  for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+    const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
+    double* X1 = XA;
+    double* Xs = XB;
+    double* lam = S.xc;  // lambda lives in the recompute's free solution buffer
+    if (K == 1) {
+      // every step's state is a checkpoint (x_{s1} is the next one): the
+      // adjoint reads both states and nothing is recomputed
+      const double* c0 = ck + (int64_t)sgi * cpw;
+      for (int j = t; j < n; j += TPS) {
+        X1[j] = c0[cpw + j];
+        Xs[j] = c0[j];
+      }
+      __syncthreads();
+    } else {
+      // restore the checkpoint and recompute the segment (solutions of steps
+      // s0..s1 into seg, permuted order)
+      {
+        const double* c = ck + (int64_t)sgi * cpw;
+        for (int j = t; j < n; j += TPS) {
+          const double xv = c[j];
+          S.xq[j] = xv;
+          sg[j] = xv;
+        }
+        for (int d = t; d < ndyn; d += TPS) S.ed[d] = c[n + d];
+      }
+      S.prefetch(s0 + 1);
+      __syncthreads();
+      for (int s = s0 + 1; s <= s1; s++) {
+        S.rhs(s);
+        __syncthreads();
+        S.solve();
+        S.update();
+        if (s < s1) {  // x_{s1} stays in LDS
+          double* so = sg + (int64_t)(s - s0) * n;
+          for (int j = t; j < n; j += TPS) so[j] = S.xc[j];
+        }
+        __syncthreads();
+        S.swap();
+      }
+      // stage x_{s1} (LDS) and x_{s1-1} (the block's own stores: row 0 is the
+      // checkpoint)
+      for (int j = t; j < n; j += TPS) {
+        X1[j] = S.xq[j];
+        Xs[j] = sg[(int64_t)(s1 - 1 - s0) * n + j];
+      }
+      __syncthreads();
+    }
+    for (int s = s1 - 1; s >= s0; s--) {
+      // x_{s-1} for the next step, into registers now
+      double nx[R];
 #pragma unroll
-      for (int r = 0; r < R; r++) nx[r] = (s > s0 && t + r * TPS < n) ? sa[(int64_t)(s - 1 - s0) * n + t + r * TPS] : 0.0;
+      for (int r = 0; r < R; r++)
+        nx[r] = (s > s0 && t + r * TPS < n) ? sg[(int64_t)(s - 1 - s0) * n + t + r * TPS] : 0.0;
       // companions' adjoint loads of step s+1 (ic_{s+1} and vl_{s+1} depend
       // on x_{s+1}, x_s and ic_s)
-      if (trap)
+      if (trap) {
         for (int d = t; d < ndyn; d += TPS) {
           const uint16_t* rd = S.dyn + d * LIN_DYN;
           const double sb1 = sbar[d];
@@ -1772,59 +1790,57 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
             if (m < n) atomicAdd(&xbar[m], -sb1);
           }
         }
-      return;
-    }
-    if (ph == 1) {  // lambda = A^-T xbar: U^T forward (level 0 fused with the copy)
+        __syncthreads();
+      }
+      // lambda = A^-T xbar: U^T forward (level 0 fused with the copy), then
+      // L^T backward (level 0 rows, when free of entries, are done)
 #pragma unroll
       for (int r = 0; r < R; r++) {
         if (tf_lev[r] < 0) continue;
         const int i = (int)(tf_ip[r] & 0xFFFF);
         lam[i] = tl0 && tf_lev[r] == 0 ? xbar[i] * S.v[tf_nd[r] >> 16] : xbar[i];
       }
-      return;
-    }
-    if (ph < 2 + nUT) {
-      const int l = tl0 + (ph - 2);
+      __syncthreads();
+      for (int l = tl0; l < ntlev; l++) {
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        if (tf_lev[r] != l) continue;
-        const int i = (int)(tf_ip[r] & 0xFFFF), q0 = (int)(tf_ip[r] >> 16), cnt = (int)(tf_nd[r] & 0xFFFF);
-        double sum = lam[i];
-        const double vd = S.v[tf_nd[r] >> 16];
-        double vv[LIN_KE], xx[LIN_KE];
+        for (int r = 0; r < R; r++) {
+          if (tf_lev[r] != l) continue;
+          const int i = (int)(tf_ip[r] & 0xFFFF), q0 = (int)(tf_ip[r] >> 16), cnt = (int)(tf_nd[r] & 0xFFFF);
+          double sum = lam[i];
+          const double vd = S.v[tf_nd[r] >> 16];
+          double vv[LIN_KE], xx[LIN_KE];
 #pragma unroll
-        for (int k = 0; k < LIN_KE; k++) {
-          vv[k] = S.v[tf_e[r][k] >> 16];
-          xx[k] = lam[tf_e[r][k] & 0xFFFF];
+          for (int k = 0; k < LIN_KE; k++) {
+            vv[k] = S.v[tf_e[r][k] >> 16];
+            xx[k] = lam[tf_e[r][k] & 0xFFFF];
+          }
+#pragma unroll
+          for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+          for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
+          lam[i] = sum * vd;
         }
-#pragma unroll
-        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-        for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
-        lam[i] = sum * vd;
+        __syncthreads();
       }
-      return;
-    }
-    if (ph < 2 + nUT + nLT) {  // L^T backward (level 0 rows, when free of entries, are done)
-      const int l = tl0 + (ph - 2 - nUT);
+      for (int l = tl0; l < ntblev; l++) {
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        if (tb_lev[r] != l) continue;
-        const int i = (int)(tb_ip[r] & 0xFFFF), q0 = (int)(tb_ip[r] >> 16), cnt = (int)(tb_nd[r] & 0xFFFF);
-        double sum = lam[i];
-        double vv[LIN_KE], xx[LIN_KE];
+        for (int r = 0; r < R; r++) {
+          if (tb_lev[r] != l) continue;
+          const int i = (int)(tb_ip[r] & 0xFFFF), q0 = (int)(tb_ip[r] >> 16), cnt = (int)(tb_nd[r] & 0xFFFF);
+          double sum = lam[i];
+          double vv[LIN_KE], xx[LIN_KE];
 #pragma unroll
-        for (int k = 0; k < LIN_KE; k++) {
-          vv[k] = S.v[tb_e[r][k] >> 16];
-          xx[k] = lam[tb_e[r][k] & 0xFFFF];
+          for (int k = 0; k < LIN_KE; k++) {
+            vv[k] = S.v[tb_e[r][k] >> 16];
+            xx[k] = lam[tb_e[r][k] & 0xFFFF];
+          }
+#pragma unroll
+          for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
+          for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
+          lam[i] = sum;
         }
-#pragma unroll
-        for (int k = 0; k < LIN_KE; k++) sum -= vv[k] * xx[k];
-        for (int k = q0 + LIN_KE; k < q0 + cnt; k++) sum -= S.v[tent[2 * k + 1]] * lam[tent[2 * k]];
-        lam[i] = sum;
+        __syncthreads();
       }
-      return;
-    }
-    if (ph == 2 + nUT + nLT) {  // element terms of step s -> s+1
+      // element terms of step s -> s+1
       for (int a = t; a < nadj; a += TPS) {
         const uint16_t* ra = adj + a * LIN_ADJ;
         const int kind = ra[0] >> 14, e = ra[0] & 0x3FFF;
@@ -1852,93 +1868,21 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
           gr[e] += el * al / h * d0 - al / h * el * d1;
         }
       }
-      return;
-    }
-    // next step's adjoint load; stage x_{s-1} into X1 (free now; the
-    // pointers swap after the barrier)
-    const double w = (s == 0) ? 0.5 * h : h;
+      __syncthreads();
+      // next step's adjoint load; stage x_{s-1}
+      const double w = (s == 0) ? 0.5 * h : h;
 #pragma unroll
-    for (int r = 0; r < R; r++)
-      if (t + r * TPS < n) {
-        const int j = S.o_slot[r];
-        xbar[j] = xb0[j] + w * cv[r];
-        xb0[j] = 0.0;
-        X1[t + r * TPS] = nx[r];
-      }
-  };
-
-  // ---- phases of one recomputation step s (segment states into sr, row
-  // s - sr0): right-hand side; the solve's levels; companion update and store
-  const int r_phases = 1 + S.solve_phases() + 1;
-  auto rec_phase = [&](int ph, int s, int sr0, double* sr) {
-    if (ph == 0) {
-      S.rhs(s);
-    } else if (ph < 1 + S.solve_phases()) {
-      S.solve_phase(ph - 1);
-    } else {
-      S.update();
-      double* so = sr + (int64_t)(s - sr0) * n;
-      for (int j = t; j < n; j += TPS) so[j] = S.xc[j];
-    }
-  };
-  // restores checkpoint k into the recomputation's state and row 0 of sr
-  auto restore = [&](int k, double* sr) {
-    const double* c = ck + (int64_t)k * cpw;
-    for (int j = t; j < n; j += TPS) {
-      const double xv = c[j];
-      S.xq[j] = xv;
-      sr[j] = xv;
-    }
-    for (int d = t; d < ndyn; d += TPS) S.ed[d] = c[n + d];
-    S.prefetch(k * K + 1);
-  };
-
-  // prologue: recompute the last segment alone
-  {
-    const int sgi = nseg - 1, s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
-    double* sr = sgbase + (int64_t)(sgi & 1) * half;
-    restore(sgi, sr);
-    __syncthreads();
-    for (int s = s0 + 1; s <= s1; s++) {
-      for (int ph = 0; ph < r_phases; ph++) {
-        rec_phase(ph, s, s0, sr);
-        __syncthreads();
-      }
-      S.swap();
-    }
-  }
-  for (int sgi = nseg - 1; sgi >= 0; sgi--) {
-    const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
-    const double* sa = sgbase + (int64_t)(sgi & 1) * half;
-    // the segment before, recomputed meanwhile into the other buffer
-    const bool rec = sgi > 0;
-    const int r0 = (sgi - 1) * K;
-    double* sr = sgbase + (int64_t)((sgi - 1) & 1) * half;
-    // stage x_{s1} and x_{s1-1}
-    X1 = XA;
-    Xs = XB;
-    for (int j = t; j < n; j += TPS) {
-      X1[j] = sa[(int64_t)(s1 - s0) * n + j];
-      Xs[j] = sa[(int64_t)(s1 - 1 - s0) * n + j];
-    }
-    if (rec) restore(sgi - 1, sr);
-    __syncthreads();
-    const int na = s1 - s0, nr = rec ? K : 0;
-    for (int it = 0; it < max(na, nr); it++) {
-      const bool a_on = it < na, r_on = it < nr;
-      const int sa_s = s1 - 1 - it, sr_s = r0 + 1 + it;
-      const int nph = max(a_on ? a_phases : 0, r_on ? r_phases : 0);
-      for (int ph = 0; ph < nph; ph++) {
-        if (a_on && ph < a_phases) adj_phase(ph, sa_s, s0, sa);
-        if (r_on && ph < r_phases) rec_phase(ph, sr_s, r0, sr);
-        __syncthreads();
-      }
-      if (a_on) {
-        double* sw = X1;
-        X1 = Xs;
-        Xs = sw;
-      }
-      if (r_on) S.swap();
+      for (int r = 0; r < R; r++)
+        if (t + r * TPS < n) {
+          const int j = S.o_slot[r];
+          xbar[j] = xb0[j] + w * cv[r];
+          xb0[j] = 0.0;
+          X1[t + r * TPS] = nx[r];  // X1 is free now: it becomes x_{s-1}
+        }
+      __syncthreads();
+      double* sw = X1;
+      X1 = Xs;
+      Xs = sw;
     }
   }
   // vl_0 = v(x_0): its adjoint flows into x_0

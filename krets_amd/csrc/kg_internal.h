@@ -42,24 +42,41 @@ enum : int32_t {
 // symbolic pass, at one representative frequency and from one instance, so a
 // solve far from that point can lose backward stability (an inductor branch
 // pivot of magnitude w*L chosen at the sweep's centre is tiny at its low end).
-// Every FP64 solve therefore measures in FP64 its componentwise backward
-// error (Oettli-Prager),
-//   omega = max_i |b - A x|_i / (|A| |x| + |b|)_i     (|z| = |re| + |im|),
-// which also bounds the normwise one, |b - A x| / (|A| |x| + |b|) in infinity
-// norms; a mixed precision solve uses its refinement's error estimate and
-// the normwise backward error of the solution it returns. A solve with omega
-// above KG_GUARD_OMEGA, a failed static pivot, or a mixed solve whose estimate
-// stays above KG_GUARD_MIXED or whose returned solution's normwise backward
-// error is above KG_GUARD_MIXED_ETA, is queued and solved again
-// by k_ac_fix: a dense LU with row partial pivoting in FP64 (systems of at
-// most KG_FIX_NMAX unknowns), then up to two FP64 refinement steps, after
-// which its normwise backward error eta is measured; eta above KG_FIX_FAIL
-// reports KG_DECOMPOSITION. The FP64 error figure published per instance is
-// the largest omega (accepted solves) or eta (fixed solves): a bound on the
-// normwise backward error of every solve of the instance.
+//
+// Lane kernels (jit.cpp) bound it a priori: during the factorisation they
+// form rho = || |L||U| ||_inf / ||A||_inf (entry norm |re| + |im|, |L|
+// bounded through |l_ik| <= |v_ik| |1/u_kk|). LU without pivoting and the two
+// triangular solves give (A + dA) x = b with |dA| <= (3 g_m + g_m^2) |L||U|
+// (Higham, Accuracy and Stability of Numerical Algorithms, Thm 9.4, with the
+// complex g_m = m sqrt(2) gamma_2 / (1 - m sqrt(2) gamma_2), m the longest
+// row of the factor pattern), so the normwise backward error is at most
+// guard_bound_coeff(m) * rho. rho above KG_GUARD_GROWTH queues the solve; the
+// instance's FP64 figure is the largest bound. A mixed precision solve is
+// queued when the growth of its complex64 factorisation is above
+// KG_GUARD_GROWTH (refinement through factors that grew can settle short of
+// FP64 accuracy with small corrections) or its refinement estimate stays
+// above KG_GUARD_MIXED.
+// Level-scheduled kernels (k_ac, k_ac_p) measure instead, in FP64 from the
+// LDS-resident b and x, the componentwise backward error (Oettli-Prager)
+//   omega = max_i |b - A x|_i / (|A| |x| + |b|)_i,
+// which bounds the normwise one, against KG_GUARD_OMEGA, and for mixed
+// precision the returned solution's normwise backward error against
+// KG_GUARD_MIXED_ETA.
+// Queued solves (and failed static pivots) are solved again by k_ac_fix: a
+// dense LU with row partial pivoting in FP64 (systems of at most KG_FIX_NMAX
+// unknowns), then up to two FP64 refinement steps, after which its normwise
+// backward error eta is measured; eta above KG_FIX_FAIL reports
+// KG_DECOMPOSITION, and eta is the figure of a fixed solve.
+constexpr double KG_GUARD_GROWTH = 4.0;
 constexpr double KG_GUARD_OMEGA = 1.4210854715202004e-14;  // 2^-46: 64 units in the last place
 constexpr double KG_GUARD_MIXED = 1e-10;                  // a tenth of the 1e-9 the mixed path promises
 constexpr double KG_GUARD_MIXED_ETA = 1.4210854715202004e-14;  // 2^-46, normwise, on the returned solution
+// guard_bound_coeff(m) = 3.01 sqrt(2) g_m (the sqrt(2): rho is measured in
+// |re| + |im| norms, the bound holds in moduli)
+inline double guard_bound_coeff(int m) {
+  const double u = 0x1p-53, g2 = 2.0 * u / (1.0 - 2.0 * u), q = m * 1.4142135623730951 * g2;
+  return 3.01 * 1.4142135623730951 * q / (1.0 - q);
+}
 constexpr double KG_FIX_FAIL = 9.094947017729282e-13;     // 2^-40
 constexpr int KG_FIX_NMAX = 90;                           // 90 x 90 complex128 in LDS: 127 KB
 // Fixup queue layout (unsigned 64-bit words): [0] count, [1 + q] queued system

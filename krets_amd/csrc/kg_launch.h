@@ -61,13 +61,12 @@ __host__ __device__ inline size_t lin_bytes(const LinHdr& h) {
 }
 
 // LDS bytes of one system of the linear adjoint kernels (k_adj_lin_fwd /
-// k_adj_lin_bwd): the whole plan, the transient's arrays, five more solution
-// vectors (adjoint load and its next-step part, two recomputed states,
-// lambda), the companion adjoint states, per-element gradients and a
-// reduction buffer.
+// k_adj_lin_bwd): the whole plan, the transient's arrays, four more solution
+// vectors (adjoint load and its next-step part, two recomputed states), the
+// companion adjoint states, per-element gradients and a reduction buffer.
 __host__ __device__ inline size_t lin_adj_bytes(const LinHdr& h) {
   return lay_al(2 * (size_t)h.words) + 8 * ((size_t)h.nnz + 2 * (size_t)h.n + 2 * (size_t)h.ndyn + 5) +
-         8 * (size_t)KG_NPARAM * h.nsrc + 8 * (5 * ((size_t)h.n + 1) + (size_t)h.ndyn + 1 + (size_t)h.ne + 1 + 32) +
+         8 * (size_t)KG_NPARAM * h.nsrc + 8 * (4 * ((size_t)h.n + 1) + (size_t)h.ndyn + 1 + (size_t)h.ne + 1 + 32) +
          64;
 }
 
@@ -98,8 +97,9 @@ struct LteArgs {
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
                       int32_t* status, int32_t* iters, Lay lay);
-// AC sweep kernels. fixq / fixcap: the accuracy guard's queue (see
-// KG_GUARD_OMEGA); null: no guard (statuses and error figures as computed).
+// AC sweep kernels. fixq / fixcap: the accuracy guard's queue (see the
+// guard constants in kg_internal.h); null: no guard (statuses and error
+// figures as computed).
 hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                   const double* freqs, int nf, const double* xop, const int64_t* xop_off, double* out,
                   const int64_t* out_off, int32_t* status, double* err, int refine, Lay lay,
@@ -144,8 +144,7 @@ hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, Batc
 // Discrete adjoint from LDS (k_adj_lin_fwd / k_adj_lin_bwd), shared topology,
 // one system per block of tps threads (64, 256, 512) owning up to r rows
 // (1, 2, 4); lds >= lin_adj_bytes; checkpoints hold the permuted solution
-// then the companion states (cpw >= n + ndyn); the backward pass takes a
-// double-buffered segment store, seg_stride >= 2 * (K + 1) * n.
+// then the companion states (cpw >= n + ndyn).
 hipError_t run_adj_lin_fwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                            int K, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
                            int64_t ck_stride, int cpw, double* J, int32_t* status, double* srcv, int nsrc_max);

@@ -39,10 +39,13 @@ def rel(g, gr):
     return np.max(np.abs(g - gr)) / np.max(np.abs(gr))
 
 
-@pytest.mark.parametrize("tps", [1, 16, 64, 3064, 3256, 3512])  # 3000 + t: LDS-resident adjoint
+@pytest.mark.parametrize("tps,every", [(1, 64), (16, 64), (64, 64), (3064, 64), (3256, 64), (3512, 64),
+                                       (3256, 1), (3256, 7), (64, 1)])  # 3000 + t: LDS-resident adjoint
 @pytest.mark.parametrize("method", [1, 2])
-def test_adjoint_rlc_vs_oracle(kg, oracle, method, tps):
-    say(f"adjoint RLC method={method} tps={tps}")
+def test_adjoint_rlc_vs_oracle(kg, oracle, method, tps, every):
+    """every = 1 keeps every state as a checkpoint (the LDS-resident backward
+    pass then recomputes nothing); 7 leaves a shorter last segment."""
+    say(f"adjoint RLC method={method} tps={tps} checkpoint every {every}")
     nets, el, pr, n, names = rlc_batch(4, 6, seed=3)
     ctx = kg.Context()
     ctx.load(el, pr, n)
@@ -50,7 +53,7 @@ def test_adjoint_rlc_vs_oracle(kg, oracle, method, tps):
     c[names.index("V(n6)")] = 1.0
     c[names.index("I(L2)")] = 50.0
     h, N = 1e-11, 300
-    J, g, st = ctx.adjoint(h, N, c, method=method, checkpoint_every=64,
+    J, g, st = ctx.adjoint(h, N, c, method=method, checkpoint_every=every,
                            config=kg.SolverConfig(compat=False, threads_per_system=tps))
     assert (st == 0).all()
     for i in range(4):
