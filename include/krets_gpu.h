@@ -178,6 +178,29 @@ int32_t kg_dc_sweep(kg_ctx* ctx, const kg_solver_config* cfg, int32_t element, d
 int32_t kg_ac_sweep(kg_ctx* ctx, const kg_solver_config* cfg, const double* freqs, int32_t nf,
                     int32_t precision, double* x, int32_t* status, double* err_est, int32_t mem);
 
+/* Allocation-free AC sweep. No counterpart in krets: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation). It computes exactly what
+ * kg_ac_sweep computes with mem = KG_MEM_DEVICE, through the same kernels,
+ * but calls no hipMalloc, hipFree or other allocating routine: the
+ * intermediates (operating point and its offsets, output offsets, the copy
+ * of freqs, the accuracy guard's queue) live in a static device array of
+ * 8 MiB per GPU that the library holds until the process ends.
+ * x ([n_instances][nf][n] complex128), status [n_instances] and err_est
+ * [n_instances] (may be NULL) must be device memory of the context's GPU.
+ * Preconditions: the batch is loaded, and kg_ac_sweep has run on this
+ * context with the same precision, frequency range and solver settings, so
+ * that plans, compiled kernels and the work mapping are cached; otherwise
+ * KG_INVALID_ARGUMENT. A workload whose intermediates exceed the array is
+ * rejected with KG_INVALID_ARGUMENT and a message giving both sizes. Every
+ * call on one GPU, from any context, shares that GPU's array, which keeps the
+ * last call's contents: calls that overlap on one GPU race. Returns the first
+ * failing instance's status when the context's pinned staging area (set up
+ * by the earlier allocating call) can hold the statuses, else KG_OK once the
+ * kernels have completed; the per-instance statuses are in `status`. */
+int32_t kg_ac_sweep_noalloc(kg_ctx* ctx, const kg_solver_config* cfg, const double* freqs, int32_t nf,
+                            int32_t precision, double* x, int32_t* status, double* err_est);
+
 /* transient::solve (solver/transient.rs:14-114) with a fixed step h and
  * nsteps steps (krets: nsteps = round(stop/h)). method KG_BE_COMPAT is
  * krets' integrator with its quirks; KG_BE and KG_TRAP are clean.

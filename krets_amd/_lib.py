@@ -107,6 +107,7 @@ def lib():
         L.kg_op.argtypes = [P, CFG, P, P, P, I32]
         L.kg_dc_sweep.argtypes = [P, CFG, I32, D, D, D, P, I32, PI, P, P, I32]
         L.kg_ac_sweep.argtypes = [P, CFG, PD, I32, I32, P, P, P, I32]
+        L.kg_ac_sweep_noalloc.argtypes = [P, CFG, PD, I32, I32, P, P, P]
         L.kg_transient.argtypes = [P, CFG, D, I32, I32, I32, P, P, P, P, I32]
         L.kg_transient_lte.argtypes = [P, CFG, D, D, D, D, D, D, I32, P, P, P, P, P, P, I32]
         L.kg_adjoint_transient.argtypes = [P, CFG, D, I32, I32, I32, PD, I32, P, P, P, I32]

@@ -12,6 +12,7 @@
 #include <fstream>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <tuple>
@@ -48,6 +49,33 @@ struct DevBuf {
 };
 
 constexpr size_t LDS_MAX = 160 * 1024;
+
+// Device scratch of the allocation-free AC sweep (kg_ac_sweep_noalloc): one
+// static array per GPU, loaded with the library's code object and held until
+// the process ends. Host code never passes `scratch` itself to a kernel (on
+// the host the name is only a stand-in): kg_create resolves its address on
+// each device once, with that device current, and calls use that address.
+constexpr size_t KG_NOALLOC_SCRATCH_BYTES = size_t(8) << 20;
+constexpr int64_t KG_NOALLOC_FIXQ = 65536;  // accuracy-guard queue entries reserved in it
+__device__ __align__(256) char scratch[KG_NOALLOC_SCRATCH_BYTES];
+constexpr int KG_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[KG_MAX_DEVICES] = {};
+hipError_t g_scratch_err[KG_MAX_DEVICES] = {};
+bool g_scratch_done[KG_MAX_DEVICES] = {};
+
+// Resolves the scratch array's address on `device` (current), once.
+void scratch_init(int device) {
+  if (device < 0 || device >= KG_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? p : nullptr;
+  if (e != hipSuccess) (void)hipGetLastError();
+  g_scratch_done[device] = true;
+}
 constexpr size_t LDS_SHARED_BUDGET = 64 * 1024;
 
 }  // namespace
@@ -105,6 +133,11 @@ struct kg_ctx {
   char* pin = nullptr;
   char* pin_dev = nullptr;  // its device address (kernels write it directly), or null
   size_t pin_cap = 0;
+  // allocation-free call in progress (kg_ac_sweep_noalloc): every helper
+  // that would allocate fails instead, and workspace names resolve to the
+  // regions in `given` (the device scratch array or the caller's buffers)
+  bool noalloc = false;
+  std::map<std::string, std::pair<void*, size_t>> given;
   // kernel timing (HIP events on this context's stream)
   bool prof = false;
   std::map<std::string, std::pair<double, int64_t>> prof_k;  // kernel -> (ms, launches)
@@ -126,6 +159,15 @@ int32_t set_err(kg_ctx* c, int32_t code, const std::string& msg) {
   } while (0)
 
 void* ws_get(kg_ctx* c, const std::string& name, size_t bytes, hipError_t* e) {
+  if (c->noalloc) {
+    auto it = c->given.find(name);
+    if (it != c->given.end() && it->second.second >= bytes) {
+      *e = hipSuccess;
+      return it->second.first;
+    }
+    *e = hipErrorOutOfMemory;
+    return nullptr;
+  }
   DevBuf& b = c->ws[name];
   *e = hipSuccess;
   if (b.cap < bytes) {
@@ -242,6 +284,10 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
     auto it = c->pat_index.find(key);
     int idx;
     if (it == c->pat_index.end()) {
+      if (c->noalloc)
+        return set_err(c, KG_INVALID_ARGUMENT,
+                       "allocation-free call: no plan for this analysis yet (call the allocating entry point "
+                       "once with the same batch, precision and frequency range)");
       int i0 = c->group_first[g];
       int ne = (int)(c->eoff[i0 + 1] - c->eoff[i0]);
       PatEntry pe;
@@ -284,6 +330,7 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
     }
   }
   if (added || !c->d_pats) {
+    if (c->noalloc) return set_err(c, KG_INVALID_ARGUMENT, "allocation-free call: plans not uploaded");
     std::vector<PatDev> hv;
     for (auto& p : c->pats) hv.push_back(p.dev);
     // kernels of this call already queued on the stream may still read the old table
@@ -296,6 +343,7 @@ int32_t prepare(kg_ctx* c, AnalysisKind ak, Ordering ord, double s_rep, Plan& P)
   }
   auto k2 = std::make_tuple((int)ak, (int)ord, s_rep);
   if (!c->inst_pat.count(k2)) {
+    if (c->noalloc) return set_err(c, KG_INVALID_ARGUMENT, "allocation-free call: plans not uploaded");
     std::vector<int32_t> tab(c->n_inst);
     for (int i = 0; i < c->n_inst; i++) tab[i] = gpi[c->group_of[i]];
     int32_t* d = nullptr;
@@ -406,6 +454,10 @@ int32_t upload(kg_ctx* c, const std::string& name, const void* src, size_t bytes
   hipError_t e;
   *d = ws_get(c, name, bytes, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
+  if (c->noalloc) {  // the scratch array is shared by every call on the device: always copy
+    HIPCHK(c, hipMemcpyAsync(*d, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return KG_OK;
+  }
   auto& u = c->uploaded[name];
   if (u.first == *d && u.second.size() == bytes && std::memcmp(u.second.data(), src, bytes) == 0) return KG_OK;
   HIPCHK(c, hipMemcpyAsync(*d, src, bytes, hipMemcpyHostToDevice, c->stream));
@@ -423,7 +475,8 @@ int32_t upload_offsets(kg_ctx* c, const std::string& name, const std::vector<int
 template <class F>
 int32_t timed(kg_ctx* c, const char* name, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->prof) {
+  const bool prof = c->prof && !c->noalloc;
+  if (prof) {
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
     HIPCHK(c, hipEventRecord(a, c->stream));
@@ -434,7 +487,7 @@ int32_t timed(kg_ctx* c, const char* name, F&& launch) {
     if (b) hipEventDestroy(b);
     return set_err(c, KG_HIP_ERROR, std::string(name) + ": " + hipGetErrorString(e));
   }
-  if (c->prof) {
+  if (prof) {
     HIPCHK(c, hipEventRecord(b, c->stream));
     HIPCHK(c, hipEventSynchronize(b));
     float ms = 0.f;
@@ -569,6 +622,7 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
     if (it != c->lane_fn.end()) {
       fn = it->second;
     } else {
+      if (c->noalloc) return set_err(c, KG_INVALID_ARGUMENT, "allocation-free call: kernel not compiled yet");
       const PatEntry& pe = c->pats[pidx];
       std::string src, log;
       if (jit_dc_source(pe.lay, pe.host.data(), src, why)) {
@@ -980,9 +1034,15 @@ int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, i
   (void)mem;
   const size_t sb = sizeof(int32_t) * c->n_inst;
   const size_t xo = (sb + 255) & ~(size_t)255;
-  const bool xcopy = xdst && xsrc && xbytes, xhost = xcopy && !on_device(xdst);
+  const bool xcopy = xdst && xsrc && xbytes && xdst != xsrc, xhost = xcopy && !on_device(xdst);
   const bool shost = status_out && !on_device(status_out);
   const size_t need = xo + (xhost ? xbytes : 0);
+  if (need > c->pin_cap && c->noalloc) {
+    // no staging area to read the statuses through: they stay in the
+    // caller's device array, which the kernels wrote directly
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return KG_OK;
+  }
   if (need > c->pin_cap) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->pin) (void)hipHostFree(c->pin);
@@ -1009,7 +1069,7 @@ int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, i
   }
   if (e == hipSuccess && xcopy && !xhost)
     e = hipMemcpyAsync(xdst, xsrc, xbytes, hipMemcpyDeviceToDevice, c->stream);
-  if (e == hipSuccess && status_out && !shost)
+  if (e == hipSuccess && status_out && !shost && status_out != d_status)
     e = hipMemcpyAsync(status_out, d_status, sb, hipMemcpyDeviceToDevice, c->stream);
   const hipError_t es = hipStreamSynchronize(c->stream);
   if (e != hipSuccess || es != hipSuccess)
@@ -1049,6 +1109,7 @@ int32_t kg_create(kg_ctx** out, int32_t device) {
     delete c;
     return KG_HIP_ERROR;
   }
+  scratch_init(device);
   *out = c;
   return KG_OK;
 }
@@ -1228,7 +1289,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   const std::string tkey = tune_key(c, "ac", precision, P0, (int64_t)c->n_inst * nf);
   // A call whose work mapping is already known runs as one captured graph:
   // operating point (fill, Newton kernel), fill, sweep kernel.
-  bool graph = c->graphs && !c->prof && (cfg->threads_per_system != 0 || c->tune.count(tkey));
+  bool graph = c->graphs && !c->prof && !c->noalloc && (cfg->threads_per_system != 0 || c->tune.count(tkey));
   std::vector<Deferred> steps;
   // ac.rs:25: the operating point comes first
   kg_solver_config ocfg = *cfg;
@@ -1252,7 +1313,11 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   double* d_err = (double*)ws_get(c, "err", sizeof(double) * c->n_inst, &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   // the accuracy guard's queue: a count, then up to one entry per system
-  const int64_t fixcap = (int64_t)c->n_inst * nf;
+  int64_t fixcap = (int64_t)c->n_inst * nf;
+  if (c->noalloc) {  // the queue's region of the scratch array bounds it
+    auto it = c->given.find("fixq");
+    if (it != c->given.end()) fixcap = std::min<int64_t>(fixcap, (int64_t)(it->second.second / 8) - 1);
+  }
   auto* d_fix = (unsigned long long*)ws_get(c, "fixq", sizeof(unsigned long long) * (1 + (size_t)fixcap), &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   // One AC launch for a mapping candidate: TPS threads per system, plus 1000
@@ -1288,6 +1353,7 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       auto key = std::make_tuple(pidx, (int)mixed, (int)split);
       auto it = c->lane_fn.find(key);
       if (it == c->lane_fn.end()) {
+        if (c->noalloc) return set_err(c, KG_INVALID_ARGUMENT, "allocation-free call: kernel not compiled yet");
         const PatEntry& pe = c->pats[pidx];
         std::string src, why, log;
         if (!jit_ac_source(pe.lay, pe.host.data(), mixed, split, src, why))
@@ -1361,6 +1427,10 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
                       mixed ? 1 : 0, d_fix, fixcap, nmax, A.P.ne_max);
   };
   int cand = cfg->threads_per_system;
+  if (cand == 0 && c->noalloc && !c->tune.count(tkey))
+    return set_err(c, KG_INVALID_ARGUMENT,
+                   "allocation-free call: the work mapping is not known yet (give threads_per_system, or call "
+                   "kg_ac_sweep once on this workload first)");
   if (cand == 0) {
     std::vector<int> cands{1, 4, 16, 64, 256};
     if (c->shared) cands.insert(cands.end(), {1001, 1004, 1016, 1064, 2001, 2011});
@@ -1423,6 +1493,55 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if ((r = mark_group_failures(c, P, d_st))) return r;
   return finish_status(c, d_st, status, mem, err_est, d_err, sizeof(double) * c->n_inst);
+}
+
+// Allocation-free AC sweep: kg_ac_sweep with every intermediate in the
+// device's scratch array and the outputs in the caller's device buffers.
+int32_t kg_ac_sweep_noalloc(kg_ctx* c, const kg_solver_config* cfg, const double* freqs, int32_t nf,
+                            int32_t precision, double* x, int32_t* status, double* err_est) {
+  if (!c || !cfg || c->n_inst == 0 || !freqs || nf <= 0 || !x || !status)
+    return set_err(c, KG_INVALID_ARGUMENT, "bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  char* base = c->device < KG_MAX_DEVICES ? (char*)g_scratch_ptr[c->device] : nullptr;
+  if (!base)
+    return set_err(c, KG_HIP_ERROR,
+                   std::string("scratch array unavailable on this device: ") +
+                       hipGetErrorString(c->device < KG_MAX_DEVICES ? g_scratch_err[c->device] : hipErrorInvalidDevice));
+  if (!on_device(x) || !on_device(status) || (err_est && !on_device(err_est)))
+    return set_err(c, KG_INVALID_ARGUMENT, "x, status and err_est must be device memory of the context's GPU");
+  const int64_t ni = c->n_inst;
+  int64_t ntot = 0;
+  for (int32_t v : c->nunk) ntot += v;
+  size_t used = 0;
+  std::map<std::string, std::pair<void*, size_t>> reg;
+  auto carve = [&](const char* name, size_t bytes) {
+    used = (used + 255) & ~(size_t)255;
+    reg[name] = {base + used, bytes};
+    used += bytes;
+  };
+  carve("newton_out", 8 * (size_t)std::max<int64_t>(ntot, 1));
+  carve("newton_off", 8 * (size_t)(ni + 1));
+  carve("newton_iters", 4 * (size_t)ni);
+  carve("ac_off", 8 * (size_t)(ni + 1));
+  carve("freqs", 8 * (size_t)nf);
+  if (!err_est) carve("err", 8 * (size_t)ni);
+  carve("fixq", 8 * (size_t)(1 + std::min<int64_t>(ni * nf, KG_NOALLOC_FIXQ)));
+  if (used > KG_NOALLOC_SCRATCH_BYTES)
+    return set_err(c, KG_INVALID_ARGUMENT,
+                   "allocation-free call needs " + std::to_string(used) + " bytes of scratch; the maximum is " +
+                       std::to_string(KG_NOALLOC_SCRATCH_BYTES));
+  reg["status"] = {status, 4 * (size_t)ni};
+  if (err_est) reg["err"] = {err_est, 8 * (size_t)ni};
+  struct Reset {
+    kg_ctx* c;
+    ~Reset() {
+      c->noalloc = false;
+      c->given.clear();
+    }
+  } reset{c};
+  c->given = std::move(reg);
+  c->noalloc = true;
+  return kg_ac_sweep(c, cfg, freqs, nf, precision, x, status, err_est, KG_MEM_DEVICE);
 }
 
 int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
