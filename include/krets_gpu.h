@@ -255,6 +255,10 @@ int32_t kg_use_graphs(kg_ctx* ctx, int32_t enable);
  * accumulated time): its name, accumulated ms and launch count; then resets
  * and turns timing on (enable = 1) or off. */
 int32_t kg_profile(kg_ctx* ctx, int32_t enable, double* total_ms, int64_t* launches, const char** kernel);
+/* Every kernel timed since the previous kg_profile call, one line each,
+ * "<name> <accumulated ms> <launches>\n", into buf (NUL-terminated,
+ * truncated to cap); no reset. Returns the length it needed. */
+int32_t kg_profile_summary(const kg_ctx* ctx, char* buf, int32_t cap);
 
 /* Autotuner report (no krets counterpart): the candidate mappings (threads
  * per system) timed by the most recent autotuning run and their kernel times

@@ -114,6 +114,7 @@ def lib():
         L.kg_adjoint_forward.argtypes = [P, CFG, D, I32, I32, I32, PD, I32, P, P, I32]
         L.kg_adjoint_backward.argtypes = [P, P, P, I32]
         L.kg_last_autotune.argtypes = [P, PI, ctypes.POINTER(ctypes.c_float), I32]
+        L.kg_profile_summary.argtypes = [P, ctypes.c_char_p, I32]
         L.kg_profile.argtypes = [P, I32, PD, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_char_p)]
         L.kg_plan_export.argtypes = [P, I32, I32, I32, I32, P, D, P, I32, P, I32]
         L.kg_use_graphs.argtypes = [P, I32]

@@ -982,10 +982,43 @@ int32_t adjoint_backward_lin(kg_ctx* c, double* grad, int32_t* status, int32_t m
   HIPCHK(c, run_fill_i32(d_st, KG_OK, (int64_t)ni, c->stream));
   TranArgs ta{aa.h, aa.alpha, aa.nsteps, aa.method, 0};
   BatchDev B = batch_dev(c, P);
+  const int64_t ck_stride = (int64_t)c->adj.ncp * c->adj.cpw;
+  // Every state for the backward pass, so that it recomputes nothing: with a
+  // checkpoint per step they are the checkpoints; otherwise, when the whole
+  // trajectory fits in a quarter of the free device memory, all segments
+  // are recomputed in parallel from their checkpoints first (k_adj_lin_rec),
+  // instead of one after another inside the backward sweep.
+  const double* xs = nullptr;
+  int64_t xs_inst = 0, xs_step = 0;
+  if (aa.K == 1) {
+    xs = d_ck;
+    xs_inst = ck_stride;
+    xs_step = c->adj.cpw;
+  } else {
+    const int64_t traj_stride = (int64_t)(aa.nsteps + 1) * P.n_max;
+    const size_t traj_bytes = sizeof(double) * ni * (size_t)traj_stride;
+    size_t free_b = 0, total_b = 0;
+    const size_t lfwd = lin_need_one(c, s_tr, 0);
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && traj_bytes <= free_b / 4 && lfwd > 1 && lfwd <= LDS_MAX) {
+      double* d_traj = (double*)ws_get(c, "adj_traj", traj_bytes, &e);
+      if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (adjoint trajectory)");
+      const int64_t nseg = (aa.nsteps + aa.K - 1) / aa.K;
+      r = timed(c, "k_adj_lin_rec", [&] {
+        return run_adj_lin_rec(tps, rows, dim3((unsigned)(ni * nseg)), lfwd, c->stream, B, 0, ta, aa.K, d_ck,
+                               ck_stride, c->adj.cpw, d_traj, traj_stride, d_st, srcv, nsrc_max);
+      });
+      if (r) return r;
+      xs = d_traj;
+      xs_inst = traj_stride;
+      xs_step = P.n_max;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
   r = timed(c, "k_adj_lin_bwd", [&] {
-    return run_adj_lin_bwd(tps, rows, dim3((unsigned)ni), ladj, c->stream, B, 0, ta, aa.K, d_c, d_ck,
-                           (int64_t)c->adj.ncp * c->adj.cpw, c->adj.cpw, d_seg, seg_stride, d_g, ne, d_xb, d_xoff,
-                           d_st, srcv, nsrc_max);
+    return run_adj_lin_bwd(tps, rows, dim3((unsigned)ni), ladj, c->stream, B, 0, ta, aa.K, d_c, d_ck, ck_stride,
+                           c->adj.cpw, d_seg, seg_stride, d_g, ne, d_xb, d_xoff, d_st, srcv, nsrc_max, xs, xs_inst,
+                           xs_step);
   });
   if (r) return r;
   if ((r = run_adj_op_term(c, P, d_xb, d_g, ne, d_st))) return r;
@@ -1753,6 +1786,22 @@ int32_t kg_profile(kg_ctx* c, int32_t enable, double* total_ms, int64_t* launche
   c->prof = enable != 0;
   c->prof_k.clear();
   return KG_OK;
+}
+
+int32_t kg_profile_summary(const kg_ctx* c, char* buf, int32_t cap) {
+  if (!c) return -KG_INVALID_ARGUMENT;
+  std::string out;
+  char line[64];
+  for (auto& kv : c->prof_k) {
+    std::snprintf(line, sizeof(line), " %.6f %lld\n", kv.second.first, (long long)kv.second.second);
+    out += kv.first + line;
+  }
+  if (buf && cap > 0) {
+    const size_t k = std::min<size_t>(out.size(), (size_t)cap - 1);
+    std::memcpy(buf, out.data(), k);
+    buf[k] = 0;
+  }
+  return (int32_t)out.size();
 }
 
 int32_t kg_plan_export(const kg_element* el, int32_t ne, int32_t n, int32_t kind, int32_t ordering,

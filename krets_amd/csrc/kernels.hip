@@ -1599,6 +1599,60 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_fwd(BatchDev B, int kind_slot, 
   }
 }
 
+// Recomputation of every checkpoint segment at once, ahead of the backward
+// pass: block (instance, segment) restores the segment's checkpoint and runs
+// its K steps (the transient's LDS-resident step), writing each state
+// (permuted order) into traj: [instance][nsteps + 1][n]. Segments are
+// independent, so the whole trajectory costs about one segment's latency per
+// wave of blocks instead of the transient's length; the backward pass then
+// reads the states. Segment 0 also writes the state at step 0.
+template <int TPS, int R>
+__global__ void __launch_bounds__(TPS) k_adj_lin_rec(BatchDev B, int kind_slot, TranArgs ta, int K,
+                                                     const double* ckpt, int64_t ck_stride, int cpw, double* traj,
+                                                     int64_t traj_stride, const int32_t* status, double* srcv,
+                                                     int nsrc_max) {
+  extern __shared__ __align__(16) char smem[];
+  const int t = threadIdx.x;
+  const int nseg = (ta.nsteps + K - 1) / K;
+  const int64_t slot = xcd_block();
+  if (slot >= (int64_t)B.n_instances * nseg) return;
+  const int inst = (int)(slot / nseg), sgi = (int)(slot % nseg);
+  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
+  if (status[inst] != KG_OK) return;
+  if (P.lin == nullptr || P.n > R * TPS || (srcv == nullptr && nsrc_max > 0)) return;  // the backward pass reports
+  LinSys<TPS, R> S;
+  const int words = reinterpret_cast<const LinHdr*>(P.lin)->words_fwd;
+  int* flags = reinterpret_cast<int*>(S.setup(P, words, smem, ta, srcv, nsrc_max, inst));
+  if (t == 0) flags[0] = KG_OK;
+  __syncthreads();
+  S.factorise(P, B.params + B.eoff[inst] * KG_NPARAM, B.ebase + B.eoff[inst], ta, &flags[0]);
+  __syncthreads();
+  if (flags[0] != KG_OK) return;
+  S.cache();
+  const int n = S.n;
+  const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
+  const double* c = ckpt + (int64_t)inst * ck_stride + (int64_t)sgi * cpw;
+  double* tr = traj + (int64_t)inst * traj_stride;
+  for (int j = t; j < n; j += TPS) {
+    const double xv = c[j];
+    S.xq[j] = xv;
+    if (sgi == 0) tr[j] = xv;
+  }
+  for (int d = t; d < S.ndyn; d += TPS) S.ed[d] = c[n + d];
+  S.prefetch(s0 + 1);
+  __syncthreads();  // also publishes the source table within the block
+  for (int s = s0 + 1; s <= s1; s++) {
+    S.rhs(s);
+    __syncthreads();
+    S.solve();
+    S.update();
+    double* so = tr + (int64_t)s * n;
+    for (int j = t; j < n; j += TPS) so[j] = S.xc[j];
+    __syncthreads();
+    S.swap();
+  }
+}
+
 // Backward pass of the discrete adjoint from LDS (k_adj_bwd semantics): per
 // checkpoint segment, last to first, recompute the segment's solutions into
 // seg (permuted order), then run each step's adjoint backwards: the
@@ -1612,7 +1666,9 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
                                                      const double* cvec, const double* ckpt, int64_t ck_stride,
                                                      int cpw, double* seg, int64_t seg_stride, double* grad,
                                                      int64_t grad_stride, double* xbar0, const int64_t* xop_off,
-                                                     int32_t* status, double* srcv, int nsrc_max) {
+                                                     int32_t* status, double* srcv, int nsrc_max,
+                                                     const double* xs_all, int64_t xs_inst_stride,
+                                                     int64_t xs_step_stride) {
   extern __shared__ __align__(16) char smem[];
   const int t = threadIdx.x;
   const int inst = (int)xcd_block();
@@ -1716,19 +1772,23 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
   __syncthreads();
   const double* ck = ckpt + (int64_t)inst * ck_stride;
   double* sg = seg + (int64_t)inst * seg_stride;
-  const int nseg = (ta.nsteps + K - 1) / K;
+  // every state given (xs_all: the checkpoints when K = 1, or the trajectory
+  // k_adj_lin_rec recomputed): one segment over the whole run, whose states
+  // are read where they lie, and nothing is recomputed
+  const bool direct = xs_all != nullptr;
+  const int KS = direct ? ta.nsteps : K;
+  const double* sgp = direct ? xs_all + (int64_t)inst * xs_inst_stride : sg;  // the segment's states
+  const int64_t sst = direct ? xs_step_stride : n;                            // and their stride
+  const int nseg = (ta.nsteps + KS - 1) / KS;
   for (int sgi = nseg - 1; sgi >= 0; sgi--) {
-    const int s0 = sgi * K, s1 = min(s0 + K, ta.nsteps);
+    const int s0 = sgi * KS, s1 = min(s0 + KS, ta.nsteps);
     double* X1 = XA;
     double* Xs = XB;
     double* lam = S.xc;  // lambda lives in the recompute's free solution buffer
-    if (K == 1) {
-      // every step's state is a checkpoint (x_{s1} is the next one): the
-      // adjoint reads both states and nothing is recomputed
-      const double* c0 = ck + (int64_t)sgi * cpw;
+    if (direct) {
       for (int j = t; j < n; j += TPS) {
-        X1[j] = c0[cpw + j];
-        Xs[j] = c0[j];
+        X1[j] = sgp[(int64_t)s1 * sst + j];
+        Xs[j] = sgp[(int64_t)(s1 - 1) * sst + j];
       }
       __syncthreads();
     } else {
@@ -1770,7 +1830,7 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
       double nx[R];
 #pragma unroll
       for (int r = 0; r < R; r++)
-        nx[r] = (s > s0 && t + r * TPS < n) ? sg[(int64_t)(s - 1 - s0) * n + t + r * TPS] : 0.0;
+        nx[r] = (s > s0 && t + r * TPS < n) ? sgp[(int64_t)(s - 1 - s0) * sst + t + r * TPS] : 0.0;
       // companions' adjoint loads of step s+1 (ic_{s+1} and vl_{s+1} depend
       // on x_{s+1}, x_s and ic_s)
       if (trap) {
@@ -3055,15 +3115,32 @@ hipError_t run_adj_lin_fwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, B
   return hipErrorInvalidValue;
 }
 
+hipError_t run_adj_lin_rec(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                           int K, const double* ckpt, int64_t ck_stride, int cpw, double* traj, int64_t traj_stride,
+                           const int32_t* status, double* srcv, int nsrc_max) {
+  const dim3 b(tps);
+#define KG_LIN(T, RR)                                                                                             \
+  if (tps == T && r == RR)                                                                                      \
+    return launch(k_adj_lin_rec<T, RR>, g, b, lds, st, B, kind_slot, ta, K, ckpt, ck_stride, cpw, traj, traj_stride, \
+                  status, srcv, nsrc_max);
+  KG_LIN(64, 1) KG_LIN(64, 2) KG_LIN(64, 4)
+  KG_LIN(256, 1) KG_LIN(256, 2) KG_LIN(256, 4)
+  KG_LIN(512, 1) KG_LIN(512, 2) KG_LIN(512, 4)
+#undef KG_LIN
+  return hipErrorInvalidValue;
+}
+
 hipError_t run_adj_lin_bwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                            int K, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
                            int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
-                           const int64_t* xop_off, int32_t* status, double* srcv, int nsrc_max) {
+                           const int64_t* xop_off, int32_t* status, double* srcv, int nsrc_max,
+                           const double* xs_all, int64_t xs_inst_stride, int64_t xs_step_stride) {
   const dim3 b(tps);
 #define KG_LIN(T, RR)                                                                                        \
   if (tps == T && r == RR)                                                                                 \
     return launch(k_adj_lin_bwd<T, RR>, g, b, lds, st, B, kind_slot, ta, K, cvec, ckpt, ck_stride, cpw, seg, \
-                  seg_stride, grad, grad_stride, xbar0, xop_off, status, srcv, nsrc_max);
+                  seg_stride, grad, grad_stride, xbar0, xop_off, status, srcv, nsrc_max, xs_all, xs_inst_stride,  \
+                  xs_step_stride);
   KG_LIN(64, 1) KG_LIN(64, 2) KG_LIN(64, 4)
   KG_LIN(256, 1) KG_LIN(256, 2) KG_LIN(256, 4)
   KG_LIN(512, 1) KG_LIN(512, 2) KG_LIN(512, 4)

@@ -148,10 +148,20 @@ hipError_t run_tran_lin(int tps, int r, dim3 g, size_t lds, hipStream_t st, Batc
 hipError_t run_adj_lin_fwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                            int K, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
                            int64_t ck_stride, int cpw, double* J, int32_t* status, double* srcv, int nsrc_max);
+// Backward pass. xs_all (optional): every state, [instance] (stride
+// xs_inst_stride) [step] (stride xs_step_stride), permuted order; the pass
+// then recomputes nothing.
 hipError_t run_adj_lin_bwd(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
                            int K, const double* cvec, const double* ckpt, int64_t ck_stride, int cpw, double* seg,
                            int64_t seg_stride, double* grad, int64_t grad_stride, double* xbar0,
-                           const int64_t* xop_off, int32_t* status, double* srcv, int nsrc_max);
+                           const int64_t* xop_off, int32_t* status, double* srcv, int nsrc_max,
+                           const double* xs_all = nullptr, int64_t xs_inst_stride = 0, int64_t xs_step_stride = 0);
+// Recomputes every checkpoint segment in parallel (grid: instances x
+// segments) into traj ([instance][nsteps + 1][n], permuted order, stride
+// traj_stride per instance); lds >= lin_bytes.
+hipError_t run_adj_lin_rec(int tps, int r, dim3 g, size_t lds, hipStream_t st, BatchDev B, int kind_slot, TranArgs ta,
+                           int K, const double* ckpt, int64_t ck_stride, int cpw, double* traj, int64_t traj_stride,
+                           const int32_t* status, double* srcv, int nsrc_max);
 hipError_t run_tran(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                     NewtonArgs na, TranArgs ta, const double* xop, const int64_t* xop_off, double* out,
                     const int64_t* out_off, int32_t* status, int32_t* iters, double* err, Lay lay,

@@ -381,6 +381,18 @@ class Context:
         lib().kg_profile(self._h, 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(name))
         return ms.value, n.value, (name.value or b"").decode()
 
+    def profile_summary(self):
+        """{kernel: (accumulated ms, launches)} of every kernel timed since
+        the previous profile() call (no reset)."""
+        need = lib().kg_profile_summary(self._h, None, 0)
+        buf = ctypes.create_string_buffer(max(need, 0) + 1)
+        lib().kg_profile_summary(self._h, buf, len(buf))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, ms, n = line.rsplit(" ", 2)
+            out[name] = (float(ms), int(n))
+        return out
+
     def _raise_or(self, r, st):
         if r != _lib.OK and (st == _lib.OK).all():
             self._check(r)
