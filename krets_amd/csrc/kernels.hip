@@ -2917,6 +2917,9 @@ hipError_t launch(K fn, dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
                       int32_t* status, int32_t* iters, Lay lay) {
+  if (tps == 1024)  // one large system per CU: four times the threads per level of 256
+    return launch(k_newton_dc<1024>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out, out_off,
+                  status, iters, lay);
   KG_TPS_SWITCH(tps, launch(k_newton_dc<T>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out,
                             out_off, status, iters, lay))
 }

@@ -100,7 +100,7 @@ def test_op_krets_known_answers(kg):
     assert abs(s["V(out)"] - 0.517) < 1e-3
 
 
-@pytest.mark.parametrize("tps", [1, 64, 256])
+@pytest.mark.parametrize("tps", [0, 1, 64, 256, 1024])  # 0: the default, 1024 threads for this single system
 def test_c1_series_shunt_ladder(kg, oracle, tps):
     from krets_amd import workloads as W
     say(f"C1 ladder tps={tps}")
