@@ -37,7 +37,17 @@ static inline int kg_frexp_exp(double a) { int e = 0; std::frexp(a, &e); return 
 #define __builtin_amdgcn_frexp_exp(a) kg_frexp_exp(a)
 #define __builtin_amdgcn_ldexp(a, e) std::ldexp((a), (e))
 #define __builtin_amdgcn_sched_barrier(x) ((void)0)
-#define __builtin_amdgcn_rcp(a) (1.0 / (a))
+// the hardware reciprocal estimate: off by up to 2^-24 relative (measured on
+// gfx950: at most 4.6e-8 = 2^-24.4); the host takes the full 2^-24, with a
+// sign that varies with the operand, so the host runs see the estimate's
+// worst case
+static inline double kg_rcp(double a) {
+  const double r = 1.0 / a;
+  unsigned long long b;
+  std::memcpy(&b, &a, 8);
+  return r * (1.0 + (((b >> 29) & 1) ? 0x1p-24 : -0x1p-24));
+}
+#define __builtin_amdgcn_rcp(a) kg_rcp(a)
 // one lane at a time: a wave vote is the lane's own predicate
 #define __builtin_amdgcn_ballot_w64(p) ((p) ? ~0ull : 0ull)
 #define __builtin_amdgcn_read_exec() (~0ull)
