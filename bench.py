@@ -123,7 +123,7 @@ def main():
     ap.add_argument("--no-others", action="store_true",
                     help="leave out the measurements of the other workloads (C1, C3, C4, C5) from the line")
     ap.add_argument("--plain-instances", type=int, default=8)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "criterion"],
                     help="c2 (default) prints the benchmark line; the others print measurement lines of the "
                          "other workloads")
     ap.add_argument("--c3-instances", type=int, default=2048)
@@ -319,7 +319,7 @@ def main():
     if not args.no_others:
         import bench_configs
         others = {}
-        for name in ("c1", "c3", "c4", "c5"):
+        for name in ("c1", "c3", "c4", "c5", "criterion"):
             bench_configs._say(f"other workload {name}")
             try:
                 others[name] = getattr(bench_configs, name)(args)

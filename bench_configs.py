@@ -322,3 +322,64 @@ def c5(args):
             r["fd_check"] = [{"param": e, "fd": fd, "adjoint": float(g[i, e])} for e, fd in zip((1, 2, 200, 399), fds)]
         res[name] = r
     return res
+
+
+def criterion(args):
+    """krets' own criterion workloads (crates/krets-solver/benches/benchmark.rs:7-111), one analysis per call
+    through the krets-mirror Solver (its context persists between calls; krets builds a Solver per iteration, a
+    CPU-only cost), beside the oracle in krets-cost and lean modes on one host thread."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "circuits")
+
+    def circ(rel):
+        with open(os.path.join(gold, rel)) as f:
+            return f.read()
+
+    work = [
+        ("resistor_ladder_500", circ("resistor_ladder_500/resistor_ladder_500.cir"), kg.Op()),
+        ("resistor_ladder_1000", circ("resistor_ladder_1000/resistor_ladder_1000.cir"), kg.Op()),
+        ("resistor_ladder_5000", W.series_ladder(5000), kg.Op()),
+        ("dc_voltage_divider_100_steps", circ("voltage_divider/voltage_divider.cir"), kg.Dc("V1", 0.0, 1.0, 0.01)),
+        ("ac_low_pass_filter_100_points", circ("low_pass_filter/low_pass_filter.cir"),
+         kg.Ac(kg.Linear(100), 1.0, 1000.0)),
+        ("tran_dual_rc_ladder_1000_steps", circ("dual_rc_ladder/dual_rc_ladder.cir"), kg.Transient(50e-6, 50e-3)),
+    ]
+    R = _oracle()
+    out = {}
+    for name, text, an in work:
+        _say(f"criterion {name}")
+        r = {}
+        try:
+            s = kg.Solver(kg.Circuit(text))
+            s.solve(an)
+            torch.cuda.synchronize()
+            t0, k = time.perf_counter(), 0
+            while time.perf_counter() - t0 < 1.0:
+                s.solve(an)
+                k += 1
+            r["gpu_ms_per_analysis"] = 1e3 * (time.perf_counter() - t0) / k
+        except kg.KretsGpuError as e:
+            r["gpu_error"] = str(e)
+        if R:
+            oc = R.Circuit(text)
+            for mode, cost in (("cpu_krets_cost_ms", R.COST_COMPAT), ("cpu_lean_ms", R.COST_LEAN)):
+                cfg = R.config(cost=cost)
+
+                def run():
+                    if isinstance(an, kg.Op):
+                        oc.op(cfg)
+                    elif isinstance(an, kg.Dc):
+                        oc.dc(an.element, an.start, an.stop, an.step_size, cfg)
+                    elif isinstance(an, kg.Ac):
+                        oc.ac(np.array(an.generate_frequencies()), cfg)
+                    else:
+                        oc.tran(an.time_step, an.stop_time, cfg=cfg)
+
+                run()
+                t0, k = time.perf_counter(), 0
+                while time.perf_counter() - t0 < 0.5:
+                    run()
+                    k += 1
+                r[mode] = 1e3 * (time.perf_counter() - t0) / k
+        out[name] = r
+    return out

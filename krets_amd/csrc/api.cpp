@@ -675,12 +675,26 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   if (r) return r;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, 1, 8, 8, 0, 0};
   Launch L;
+  hipError_t e;
+  char* gws = nullptr;
+  int64_t gstride = 0;
   r = configure(c, tps, lay, c->n_inst, L);
-  if (r) return r;
+  if (r) {
+    // too large for LDS: one system per block, its values in a global
+    // workspace region of the block (they stay in the L2 between levels)
+    c->err.clear();
+    gstride = (int64_t)lay_al(sys_bytes(lay) + 8 * (size_t)tps);
+    L.tps = tps;
+    L.spb = 1;
+    L.lds = 0;
+    L.block = dim3(tps);
+    L.grid = dim3((unsigned)c->n_inst);
+    gws = (char*)ws_get(c, "newton_gws", (size_t)gstride * c->n_inst, &e);
+    if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (large systems)");
+  }
   auto off = out_offsets(c, npts);
   r = upload_offsets(c, "newton_off", off, d_off);
   if (r) return r;
-  hipError_t e;
   *d_out = (double*)ws_get(c, "newton_out", sizeof(double) * std::max<int64_t>(off.back(), 1), &e);
   if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
   *d_status = (int32_t*)ws_get(c, "status", sizeof(int32_t) * c->n_inst, &e);
@@ -697,7 +711,7 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
       {nullptr, [=] { return run_fill_i32(st, KG_OK, ni, strm); }},
       {"k_newton_dc", [=] {
          return run_newton(tps, L.grid, L.block, L.lds, strm, B, 0, na, sweep_e, start, step, npts, o, oo, st, its,
-                           lay);
+                           lay, gws, gstride);
        }}};
   // instances whose topology has no plan fail as krets' sp_lu would
   bool any_bad = false;

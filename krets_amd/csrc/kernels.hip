@@ -747,7 +747,7 @@ __device__ bool newton_converged(const PatDev& P, const double* xnew, const doub
 template <int TPS>
 __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_local_e, double start, double step,
                             int npts, double* out, const int64_t* out_off, int32_t* status, int32_t* iters,
-                            Lay lay) {
+                            Lay lay, char* gws, int64_t gstride) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
@@ -755,8 +755,11 @@ __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_
   const bool valid = inst < B.n_instances;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + (valid ? inst : 0)]];
   size_t sb = sys_bytes(lay);
-  double* red = (double*)(smem + spb * sb);
-  SysMem<double, double> M = carve<double, double>(smem + ls * sb, lay);
+  // systems too large for LDS keep their values in a global workspace, one
+  // region per block (the block's barriers order its accesses there too)
+  char* base = gws ? gws + (int64_t)blockIdx.x * gstride : smem;
+  double* red = (double*)(base + spb * sb);
+  SysMem<double, double> M = carve<double, double>(base + ls * sb, lay);
   const double* prm = B.params + (valid ? B.eoff[inst] : 0) * KG_NPARAM;
   const double* eb = B.ebase + (valid ? B.eoff[inst] : 0);
   if (t == 0) {
@@ -2916,12 +2919,12 @@ hipError_t launch(K fn, dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args
 
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
-                      int32_t* status, int32_t* iters, Lay lay) {
+                      int32_t* status, int32_t* iters, Lay lay, char* gws, int64_t gstride) {
   if (tps == 1024)  // one large system per CU: four times the threads per level of 256
     return launch(k_newton_dc<1024>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out, out_off,
-                  status, iters, lay);
+                  status, iters, lay, gws, gstride);
   KG_TPS_SWITCH(tps, launch(k_newton_dc<T>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out,
-                            out_off, status, iters, lay))
+                            out_off, status, iters, lay, gws, gstride))
 }
 
 hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

@@ -96,7 +96,7 @@ struct LteArgs {
 
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
-                      int32_t* status, int32_t* iters, Lay lay);
+                      int32_t* status, int32_t* iters, Lay lay, char* gws = nullptr, int64_t gstride = 0);
 // AC sweep kernels. fixq / fixcap: the accuracy guard's queue (see the
 // guard constants in kg_internal.h); null: no guard (statuses and error
 // figures as computed).

@@ -44,6 +44,14 @@ def c1_ladder(sections=1000):
     return "\n".join(lines) + "\n"
 
 
+def series_ladder(n):
+    """The N-sized series resistor ladder of krets' circuits/resistor_ladder_N (V1 at node 1, R_k from node k
+    to k + 1, the last to ground)."""
+    lines = ["% N-sized resistor ladder", "V1 1 0 1"]
+    lines += [f"R{k} {k} {k + 1} 1000" for k in range(1, n)] + [f"R{n} {n} 0 1000"]
+    return "\n".join(lines) + "\n"
+
+
 def c2_netlist(r, c):
     lines = ["* RC ladder low-pass", "V1 in 0 0 AC 1"]
     prev = "in"

@@ -66,6 +66,26 @@ def test_op_vs_oracle(kg, oracle, path, tps):
     assert nerr(x, xr) < tol
 
 
+def test_op_larger_than_lds(kg, oracle):
+    """krets' resistor_ladder_5000 (a series chain of 5000 resistors from V1 to
+    ground, benches/benchmark.rs:38-51): 5001 unknowns need 480 KB per system,
+    more than a CU's LDS, so the operating point runs with its values in a
+    global workspace. V(k) = 1 - (k - 1) / 5000 and I(V1) = -1 / 5 MOhm.
+    The chain's conductance matrix (a path Laplacian) has condition number
+    about 4 n^2 / pi^2 = 1e7, so rounding alone allows errors near
+    kappa u = 1e-9 of the 1 V scale; the bound is that (measured: 3.5e-11 on
+    the GPU, 1.6e-12 for the oracle's different elimination order)."""
+    from krets_amd import workloads as W
+    say("OP larger than LDS (5000-resistor chain)")
+    text = W.series_ladder(5000)
+    c, r = gpu_op(kg, text, 0)
+    v = np.array([r[f"V({k})"] for k in range(1, 5001)])
+    assert np.max(np.abs(v - (1 - np.arange(5000) / 5000.0))) < 1e-9
+    assert abs(r["I(V1)"] / (-1.0 / 5e6) - 1) < 1e-9
+    xr, _ = oracle.Circuit(text).op()
+    assert nerr(np.array([r[k] for k in c.names]), xr) < 1e-9
+
+
 @pytest.mark.parametrize("path", [p for p in OP_CASES if "diode" not in p and "ladder" not in p])
 def test_op_lane_kernel_vs_oracle(kg, oracle, path):
     """Linear operating points through the run-time specialised lane kernel
