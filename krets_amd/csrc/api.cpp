@@ -411,6 +411,45 @@ int32_t configure(kg_ctx* c, int tps, Lay lay, int64_t nsys, Launch& L, size_t e
   return KG_OK;
 }
 
+// Systems too large for LDS: one system per block of tps threads, its values
+// and the block's reduction buffer in a region of a global workspace (`key`),
+// launched in chunks of at most *chunk systems (lay.sys_off the first of a
+// chunk) so that the workspace stays within a quarter of free device memory
+// and 2048 regions (eight blocks per CU already fill the chip). The regions
+// are rewritten from scratch by every system, so chunks reuse them.
+constexpr int64_t KG_GWS_MAX_REGIONS = 2048;
+int32_t configure_global(kg_ctx* c, int tps, Lay& lay, int64_t nsys, Launch& L, const char* key, int64_t* chunk) {
+  lay.gstride = (int64_t)lay_al(sys_bytes(lay) + 8 * (size_t)tps);
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+  const int64_t fit = (int64_t)(free_b / 4) / lay.gstride;
+  if (fit < 1) return set_err(c, KG_UNEXPECTED, "system too large for device memory");
+  *chunk = std::max<int64_t>(1, std::min({nsys, KG_GWS_MAX_REGIONS, fit}));
+  hipError_t e;
+  lay.gws = (char*)ws_get(c, key, (size_t)lay.gstride * *chunk, &e);
+  if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (systems larger than LDS)");
+  lay.sys_off = 0;
+  L.tps = tps;
+  L.spb = 1;
+  L.lds = 0;
+  L.block = dim3(tps);
+  L.grid = dim3((unsigned)*chunk);
+  return KG_OK;
+}
+
+// Runs launch(lay, grid) over nsys systems in chunks of `chunk` (one launch
+// when the systems sit in LDS).
+template <class F>
+hipError_t chunked(Lay lay, int64_t nsys, int64_t chunk, const Launch& L, F&& launch) {
+  if (!lay.gws) return launch(lay, L.grid);
+  for (int64_t o = 0; o < nsys; o += chunk) {
+    lay.sys_off = o;
+    hipError_t e = launch(lay, dim3((unsigned)std::min(chunk, nsys - o)));
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 int32_t copy_out(kg_ctx* c, void* dst, const void* src, size_t bytes, int mem) {
   if (!dst || dst == src || bytes == 0) return KG_OK;
   HIPCHK(c, hipMemcpyAsync(dst, src, bytes, mem == KG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
@@ -676,21 +715,11 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, 1, 8, 8, 0, 0};
   Launch L;
   hipError_t e;
-  char* gws = nullptr;
-  int64_t gstride = 0;
-  r = configure(c, tps, lay, c->n_inst, L);
-  if (r) {
-    // too large for LDS: one system per block, its values in a global
-    // workspace region of the block (they stay in the L2 between levels)
-    c->err.clear();
-    gstride = (int64_t)lay_al(sys_bytes(lay) + 8 * (size_t)tps);
-    L.tps = tps;
-    L.spb = 1;
-    L.lds = 0;
-    L.block = dim3(tps);
-    L.grid = dim3((unsigned)c->n_inst);
-    gws = (char*)ws_get(c, "newton_gws", (size_t)gstride * c->n_inst, &e);
-    if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (large systems)");
+  int64_t chunk = c->n_inst;
+  if ((r = configure(c, tps, lay, c->n_inst, L))) {
+    if (tps < 64) return r;
+    c->err.clear();  // too large for LDS: the values go to a global workspace
+    if ((r = configure_global(c, tps, lay, c->n_inst, L, "newton_gws", &chunk))) return r;
   }
   auto off = out_offsets(c, npts);
   r = upload_offsets(c, "newton_off", off, d_off);
@@ -710,8 +739,9 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
   std::vector<Deferred> steps{
       {nullptr, [=] { return run_fill_i32(st, KG_OK, ni, strm); }},
       {"k_newton_dc", [=] {
-         return run_newton(tps, L.grid, L.block, L.lds, strm, B, 0, na, sweep_e, start, step, npts, o, oo, st, its,
-                           lay, gws, gstride);
+         return chunked(lay, ni, chunk, L, [&](Lay ly, dim3 g) {
+           return run_newton(tps, g, L.block, L.lds, strm, B, 0, na, sweep_e, start, step, npts, o, oo, st, its, ly);
+         });
        }}};
   // instances whose topology has no plan fail as krets' sp_lu would
   bool any_bad = false;
@@ -847,9 +877,10 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(c->group_of[i], (int)AK_TRAN, (int)ORD_NESTED, s_tr)]];
     if (pe.lay.status != KG_OK) continue;  // structurally singular: status set by the caller
     Lay l{pe.lay.nnz, pe.lay.n, pe.lay.ne, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
-    if (sys_bytes(l) + red > LDS_MAX) return set_err(c, KG_UNEXPECTED, "system too large for on-chip memory");
-    const size_t lb = lin_ok ? lin_need_one(c, s_tr, i) : 0;
-    int per_cu = (int)std::min<size_t>(8, LDS_MAX / (lin_ok ? lb : sys_bytes(l) + red));
+    // class -1: systems larger than LDS, their values in a global workspace
+    const bool big = sys_bytes(l) + red > LDS_MAX;
+    const size_t lb = lin_ok && !big ? lin_need_one(c, s_tr, i) : 0;
+    int per_cu = big ? -1 : (int)std::min<size_t>(8, LDS_MAX / (lin_ok ? lb : sys_bytes(l) + red));
     cls[per_cu].push_back(i);
     Lay& b = clay.emplace(per_cu, l).first->second;
     b.nnz_max = std::max(b.nnz_max, l.nnz_max);
@@ -866,6 +897,8 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     int32_t n = 0;
     int plan_ints = 0;
     size_t lin = 0;
+    Launch gL;           // class -1: launch shape for 256 threads (workspace regions sized for it)
+    int64_t gchunk = 0;  // class -1: systems per launch
   };
   std::vector<Bucket> bk;
   for (auto& kv : cls) {
@@ -879,6 +912,7 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     if (e != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed");
     HIPCHK(c, hipMemcpyAsync(b.list, kv.second.data(), sizeof(int32_t) * kv.second.size(), hipMemcpyHostToDevice,
                              c->stream));
+    if (kv.first < 0 && (r = configure_global(c, 256, b.lay, b.n, b.gL, "tran_gws", &b.gchunk))) return r;
     bk.push_back(b);
   }
   std::sort(bk.begin(), bk.end(), [](const Bucket& a, const Bucket& b) { return a.lay.nnz_max > b.lay.nnz_max; });
@@ -901,13 +935,21 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     }
     Launch L;
     int tps = cand % 1000, pi = cand >= 1000 ? b.plan_ints : 0;
-    if (configure(c, tps, b.lay, b.n, L, pi ? lay_al(4 * (size_t)pi) : 0) != KG_OK)
+    if (b.lay.gws) {
+      if (pi || tps > 256) return hipErrorInvalidConfiguration;
+      L = b.gL;
+      L.tps = tps;
+      L.block = dim3(tps);
+    } else if (configure(c, tps, b.lay, b.n, L, pi ? lay_al(4 * (size_t)pi) : 0) != KG_OK) {
       return hipErrorInvalidConfiguration;
+    }
     BatchDev B = B0;
     B.inst_list = b.list;
     B.n_list = b.n;
-    return run_tran(tps, mixed ? 1 : 0, L.grid, L.block, L.lds, st, B, 0, na, ta, d_xop, d_xoff, d_out, d_off, d_st,
-                    d_iters, d_err, b.lay, pi);
+    return chunked(b.lay, b.n, b.gchunk, L, [&](Lay ly, dim3 g) {
+      return run_tran(tps, mixed ? 1 : 0, g, L.block, L.lds, st, B, 0, na, ta, d_xop, d_xoff, d_out, d_off, d_st,
+                      d_iters, d_err, ly, pi);
+    });
   };
   // mapping per size class, timed on that class alone
   for (auto& b : bk) {
@@ -915,11 +957,14 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
     Pb.n_max = b.lay.n_max;
     Pb.nnz_max = b.lay.nnz_max;
     Pb.ne_max = b.lay.ne_max;
-    std::vector<int> cands = {64, 256, 1064, 1256};
-    if (lin_ok && b.lin <= LDS_MAX) cands.insert(cands.end(), LIN_CANDS.begin(), LIN_CANDS.end());
+    std::vector<int> cands = {64, 256};
+    if (!b.lay.gws) cands.insert(cands.end(), {1064, 1256});
+    if (lin_ok && !b.lay.gws && b.lin <= LDS_MAX) cands.insert(cands.end(), LIN_CANDS.begin(), LIN_CANDS.end());
     r = autotune(c, tune_key(c, lin_ok ? "tranl" : "tran", precision, Pb, b.n), cands,
                  [&](int t, float* ms) -> int32_t {
-      if (!is_lin_cand(t)) {
+      if (b.lay.gws) {
+        if (t >= 1000) return set_err(c, KG_UNEXPECTED, "systems larger than LDS: plan not staged");
+      } else if (!is_lin_cand(t)) {
         Launch L;
         int32_t rr = configure(c, t % 1000, b.lay, b.n, L, t >= 1000 ? lay_al(4 * (size_t)b.plan_ints) : 0);
         if (rr) return rr;
@@ -1384,7 +1429,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
     Launch L;
     bool persistent = false, lane = false;
     int tps = 0, plan_ints = 0;
-    int64_t chunk = 0;
+    int64_t chunk = 0;   // persistent: systems per block
+    int64_t gchunk = 0;  // systems larger than LDS: systems per launch
     JitFn fn;
   };
   const int64_t nsys = (int64_t)c->n_inst * nf;
@@ -1423,7 +1469,12 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
     int32_t rr = prepare(c, AK_AC, order_for(A.tps), s_ac, A.P);
     if (rr) return rr;
     A.lay = Lay{A.P.nnz_max, A.P.n_max, A.P.ne_max, 0, 0, A.P.any_diode ? 1 : 0, mixed ? 8 : 16, 16, mixed ? 1 : 0, 0};
-    if (!A.persistent) return configure(c, A.tps, A.lay, (int64_t)c->n_inst * nf, A.L);
+    if (!A.persistent) {
+      const int32_t rc = configure(c, A.tps, A.lay, nsys, A.L);
+      if (rc == KG_OK || A.tps < 64) return rc;
+      c->err.clear();  // too large for LDS: the values go to a global workspace
+      return configure_global(c, A.tps, A.lay, nsys, A.L, "ac_gws", &A.gchunk);
+    }
     if (!c->shared) return set_err(c, KG_UNEXPECTED, "persistent AC mapping needs a shared topology");
     const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(0, (int)AK_AC, (int)order_for(A.tps), s_ac)]];
     A.plan_ints = pe.lay.total;
@@ -1466,8 +1517,10 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
     if (A.persistent)
       return run_ac_persistent(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop,
                                d_xoff, d_out, d_off, d_st, d_err, refine, A.lay, A.plan_ints, A.chunk, d_fix, fixcap);
-    return run_ac(A.tps, mixed ? 1 : 0, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out,
-                  d_off, d_st, d_err, refine, A.lay, d_fix, fixcap);
+    return chunked(A.lay, nsys, A.gchunk, A.L, [&](Lay ly, dim3 g) {
+      return run_ac(A.tps, mixed ? 1 : 0, g, A.L.block, A.L.lds, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out, d_off,
+                    d_st, d_err, refine, ly, d_fix, fixcap);
+    });
   };
   // the guard's fixup over whatever the sweep kernel queued (persistent grid)
   auto fixup = [&](const AcLaunch& A) -> hipError_t {
@@ -1628,15 +1681,21 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   TranArgs ta{h, method == KG_TRAP ? 2.0 : 1.0, nsteps, method, cfg->refine_iters > 0 ? cfg->refine_iters : 2};
   NewtonArgs na = newton_args(cfg);
   na.compat = method == KG_BE_COMPAT ? 1 : 0;
-  // cand: threads per system, plus 1000 for the plan staged in LDS (64, 256)
-  auto setup = [&](int cand, Plan& PP, Lay& lay, Launch& L, int& pi) -> int32_t {
+  // cand: threads per system, plus 1000 for the plan staged in LDS (64, 256).
+  // chunk: systems per launch (systems larger than LDS, whose values go to a
+  // global workspace with 64 or 256 threads each).
+  auto setup = [&](int cand, Plan& PP, Lay& lay, Launch& L, int& pi, int64_t& chunk) -> int32_t {
     int tps = cand % 1000;
     int32_t rr = prepare(c, AK_TRAN, order_for(tps), s_tr, PP);
     if (rr) return rr;
     lay = Lay{PP.nnz_max, PP.n_max, PP.ne_max, 1, 1, 1, mixed ? 4 : 8, 8, mixed ? 1 : 0, 0};
     pi = cand >= 1000 ? PP.total_max : 0;
     if (pi && tps != 64 && tps != 256) return set_err(c, KG_INVALID_ARGUMENT, "staged plans need 64 or 256 threads");
-    return configure(c, tps, lay, c->n_inst, L, pi ? lay_al(4 * (size_t)pi) : 0);
+    chunk = c->n_inst;
+    rr = configure(c, tps, lay, c->n_inst, L, pi ? lay_al(4 * (size_t)pi) : 0);
+    if (rr == KG_OK || pi || tps < 64) return rr;
+    c->err.clear();
+    return configure_global(c, tps, lay, c->n_inst, L, "tran_gws", &chunk);
   };
   if (!c->shared && cfg->threads_per_system == 0 && c->n_inst >= 64) {
     r = run_tran_buckets(c, na, ta, mixed, s_tr, precision, d_xop, d_xoff, d_out, d_off, d_st, d_st0, d_iters, d_err);
@@ -1724,14 +1783,17 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
       Lay lay;
       Launch L;
       int pi = 0;
-      int32_t rr = setup(t, PP, lay, L, pi);
+      int64_t chunk = 0;
+      int32_t rr = setup(t, PP, lay, L, pi, chunk);
       if (rr) return rr;
       if ((t % 1000) * L.spb < 64 && (t % 1000) < 64) return set_err(c, KG_UNEXPECTED, "sub-wave mapping");
       BatchDev B = batch_dev(c, PP);
       HIPCHK(c, hipMemcpyAsync(d_st, d_st0, sizeof(int32_t) * c->n_inst, hipMemcpyDeviceToDevice, c->stream));
       return time_launch(c, [&] {
-        return run_tran(t % 1000, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff,
-                        d_out, d_off, d_st, d_iters, d_err, lay, pi);
+        return chunked(lay, c->n_inst, chunk, L, [&](Lay ly, dim3 g) {
+          return run_tran(t % 1000, mixed ? 1 : 0, g, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out,
+                          d_off, d_st, d_iters, d_err, ly, pi);
+        });
       }, ms);
     }, &cand);
     if (r) return r;
@@ -1754,11 +1816,14 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   Lay lay;
   Launch L;
   int pi = 0;
-  if ((r = setup(cand, P, lay, L, pi))) return r;
+  int64_t chunk = 0;
+  if ((r = setup(cand, P, lay, L, pi, chunk))) return r;
   BatchDev B = batch_dev(c, P);
   r = timed(c, mixed ? "k_tran<float,double>" : "k_tran<double,double>", [&] {
-    return run_tran(cand % 1000, mixed ? 1 : 0, L.grid, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out,
-                    d_off, d_st, d_iters, d_err, lay, pi);
+    return chunked(lay, c->n_inst, chunk, L, [&](Lay ly, dim3 g) {
+      return run_tran(cand % 1000, mixed ? 1 : 0, g, L.block, L.lds, c->stream, B, 0, na, ta, d_xop, d_xoff, d_out,
+                      d_off, d_st, d_iters, d_err, ly, pi);
+    });
   });
   if (r) return r;
   }
@@ -1950,6 +2015,7 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   struct LteLaunch {
     bool lane = false;
     int tps = 0, pi = 0;
+    int64_t gchunk = 0;  // systems larger than LDS: systems per launch
     Plan P;
     Lay lay{};
     Launch L;
@@ -1987,13 +2053,18 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
     if (rr) return rr;
     A.lay = Lay{A.P.nnz_max, A.P.n_max, A.P.ne_max, 1, 2, 1, 8, 8, 0, 1};
     // stage the plan and each system's element records in LDS when they fit
-    // (the whole step loop then reads them there); else the plan alone; else neither
+    // (the whole step loop then reads them there); else the plan alone; else
+    // neither; else (systems larger than LDS) the values in a global workspace
     A.pi = A.P.total_max;
     if (configure(c, A.tps, A.lay, c->n_inst, A.L, lay_al(4 * (size_t)A.pi)) != KG_OK) {
       A.lay.stage_prm = 0;
       if (configure(c, A.tps, A.lay, c->n_inst, A.L, lay_al(4 * (size_t)A.pi)) != KG_OK) {
         A.pi = 0;
-        if ((rr = configure(c, A.tps, A.lay, c->n_inst, A.L))) return rr;
+        if ((rr = configure(c, A.tps, A.lay, c->n_inst, A.L))) {
+          if (A.tps < 64) return rr;
+          c->err.clear();
+          if ((rr = configure_global(c, A.tps, A.lay, (int64_t)ni, A.L, "lte_gws", &A.gchunk))) return rr;
+        }
       }
     }
     c->err.clear();
@@ -2017,8 +2088,10 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
       return hipModuleLaunchKernel(A.fn.fn, A.L.grid.x, 1, 1, 64, 1, 1, 0, c->stream, args, nullptr);
     }
     BatchDev B = batch_dev(c, A.P);
-    return run_lte(A.tps, A.L.grid, A.L.block, A.L.lds, c->stream, B, 0, na, la, d_xop, d_xoff, d_t, d_x, d_off, d_np,
-                   d_nt, d_rj, d_st, A.lay, A.pi);
+    return chunked(A.lay, (int64_t)ni, A.gchunk, A.L, [&](Lay ly, dim3 g) {
+      return run_lte(A.tps, g, A.L.block, A.L.lds, c->stream, B, 0, na, la, d_xop, d_xoff, d_t, d_x, d_off, d_np,
+                     d_nt, d_rj, d_st, ly, A.pi);
+    });
   };
   int cand = cfg->threads_per_system;
   if (cand == 0) {

@@ -530,12 +530,21 @@ __device__ __forceinline__ double c1(double a) { return fabs(a); }
 __device__ __forceinline__ double c1(c128 a) { return fabs(a.re) + fabs(a.im); }
 
 // Componentwise backward error of an FP64 solve (Oettli-Prager), omega =
-// max_i |b - A x|_i / (|A| |x| + |b|)_i with |z| = |re| + |im| and A
-// re-gathered in FP64 from the stamps (the system's v holds its factors); it
-// bounds the normwise backward error. Reached by every thread.
+// max_i |b - A x|_i / ((|A| |x| + |b|)_i + u ||b||_inf) with |z| = |re| + |im|
+// and A re-gathered in FP64 from the stamps (the system's v holds its
+// factors); it bounds the normwise backward error. The floor u ||b||_inf
+// (perturbations of b allowed down to FP64's resolution of its largest
+// entry) keeps rows whose solution has decayed towards underflow measurable:
+// a long RC ladder at high frequency attenuates its node voltages into the
+// subnormal range, where the plain componentwise ratio is rounding noise of
+// order 1 even for a partial-pivoting solve. Reached by every thread.
 template <class A, int TPS>
 __device__ double guard_omega(const PatDev& P, const A* b, const A* x, const double* prm, const double* eb,
                               const StampArgs& sa, const double* ed, double* red, int ls, int t, bool act) {
+  double bm = 0.0;
+  if (act)
+    for (int i = t; i < P.n; i += TPS) bm = fmax(bm, c1(b[i]));
+  const double floor_b = 0x1p-53 * sys_max<TPS>(bm, red, ls, t);
   double om = 0.0;
   if (act)
     for (int i = t; i < P.n; i += TPS) {
@@ -551,7 +560,8 @@ __device__ double guard_omega(const PatDev& P, const A* b, const A* x, const dou
         dr += c1(a) * c1(xj);
       }
       const double ri = c1(r);
-      if (ri > 0.0) om = fmax(om, dr > 0.0 ? ri / dr : INFINITY);
+      const double den = dr + floor_b;
+      if (ri > 0.0) om = fmax(om, den > 0.0 ? ri / den : INFINITY);
     }
   return sys_max<TPS>(om, red, ls, t);
 }
@@ -747,17 +757,15 @@ __device__ bool newton_converged(const PatDev& P, const double* xnew, const doub
 template <int TPS>
 __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_local_e, double start, double step,
                             int npts, double* out, const int64_t* out_off, int32_t* status, int32_t* iters,
-                            Lay lay, char* gws, int64_t gstride) {
+                            Lay lay) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int inst = (int)(xcd_block() * spb + ls);
+  const int inst = (int)(lay.sys_off + xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + (valid ? inst : 0)]];
   size_t sb = sys_bytes(lay);
-  // systems too large for LDS keep their values in a global workspace, one
-  // region per block (the block's barriers order its accesses there too)
-  char* base = gws ? gws + (int64_t)blockIdx.x * gstride : smem;
+  char* base = sys_region(smem, lay);
   double* red = (double*)(base + spb * sb);
   SysMem<double, double> M = carve<double, double>(base + ls * sb, lay);
   const double* prm = B.params + (valid ? B.eoff[inst] : 0) * KG_NPARAM;
@@ -858,15 +866,16 @@ __global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, con
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int64_t sys = xcd_block() * spb + ls;
+  const int64_t sys = lay.sys_off + xcd_block() * spb + ls;
   const int64_t nsys = (int64_t)B.n_instances * nf;
   const bool valid = sys < nsys;
   const int inst = valid ? (int)(sys / nf) : 0;
   const int fi = valid ? (int)(sys % nf) : 0;
   const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
   size_t sb = sys_bytes(lay);
-  double* red = (double*)(smem + spb * sb);
-  SysMem<F, A> M = carve<F, A>(smem + ls * sb, lay);
+  char* base = sys_region(smem, lay);
+  double* red = (double*)(base + spb * sb);
+  SysMem<F, A> M = carve<F, A>(base + ls * sb, lay);
   const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
   const double* eb = B.ebase + B.eoff[inst];
   if (t == 0) M.flags[0] = KG_OK;
@@ -939,13 +948,13 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int64_t slot0 = xcd_block() * spb;
+  const int64_t slot0 = lay.sys_off + xcd_block() * spb;
   const int slot = (int)(slot0 + ls);
   const int nslot = B.inst_list ? B.n_list : B.n_instances;
   const bool valid = slot < nslot;
   const int inst = valid ? (B.inst_list ? B.inst_list[slot] : slot) : 0;
   const int ii = inst;
-  char* sys_lds = smem;
+  char* sys_lds = sys_region(smem, lay);
   const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
   PatDev Ps;
   if constexpr (STAGED) {
@@ -955,7 +964,7 @@ __global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, co
     for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
     __syncthreads();
     Ps = rebase(G, pl);
-    sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
+    if (!lay.gws) sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
   }
   const PatDev& P = [&]() -> const PatDev& {
     if constexpr (STAGED) return Ps;
@@ -1988,11 +1997,11 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
-  const int64_t slot0 = xcd_block() * spb;
+  const int64_t slot0 = lay.sys_off + xcd_block() * spb;
   const int inst = (int)(slot0 + ls);
   const bool valid = inst < B.n_instances;
   const int ii = valid ? inst : 0;
-  char* sys_lds = smem;
+  char* sys_lds = sys_region(smem, lay);
   const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + ii]];
   PatDev Ps;
   if constexpr (STAGED) {
@@ -2002,7 +2011,7 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
     for (int i = threadIdx.x; i < G.total; i += blockDim.x) pl[i] = G.base[i];
     __syncthreads();
     Ps = rebase(G, pl);
-    sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
+    if (!lay.gws) sys_lds = smem + lay_al(sizeof(int32_t) * (size_t)plan_ints);
   }
   const PatDev& P = [&]() -> const PatDev& {
     if constexpr (STAGED) return Ps;
@@ -2919,12 +2928,12 @@ hipError_t launch(K fn, dim3 g, dim3 b, size_t lds, hipStream_t st, Args... args
 
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
-                      int32_t* status, int32_t* iters, Lay lay, char* gws, int64_t gstride) {
+                      int32_t* status, int32_t* iters, Lay lay) {
   if (tps == 1024)  // one large system per CU: four times the threads per level of 256
     return launch(k_newton_dc<1024>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out, out_off,
-                  status, iters, lay, gws, gstride);
+                  status, iters, lay);
   KG_TPS_SWITCH(tps, launch(k_newton_dc<T>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out,
-                            out_off, status, iters, lay, gws, gstride))
+                            out_off, status, iters, lay))
 }
 
 hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,

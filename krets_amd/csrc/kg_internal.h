@@ -58,10 +58,11 @@ enum : int32_t {
 // above KG_GUARD_MIXED.
 // Level-scheduled kernels (k_ac, k_ac_p) measure instead, in FP64 from the
 // LDS-resident b and x, the componentwise backward error (Oettli-Prager)
-//   omega = max_i |b - A x|_i / (|A| |x| + |b|)_i,
-// which bounds the normwise one, against KG_GUARD_OMEGA, and for mixed
-// precision the returned solution's normwise backward error against
-// KG_GUARD_MIXED_ETA.
+//   omega = max_i |b - A x|_i / ((|A| |x| + |b|)_i + u ||b||_inf),
+// which bounds the normwise one, against KG_GUARD_OMEGA (the floor
+// u ||b||_inf keeps rows whose solution decays into the subnormal range
+// measurable), and for mixed precision the returned solution's normwise
+// backward error against KG_GUARD_MIXED_ETA.
 // Queued solves (and failed static pivots) are solved again by k_ac_fix: a
 // dense LU with row partial pivoting in FP64 (systems of at most KG_FIX_NMAX
 // unknowns), then up to two FP64 refinement steps, after which its normwise

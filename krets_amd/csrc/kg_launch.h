@@ -38,9 +38,24 @@ struct Lay {
   int32_t need_g, need_xp, need_ed;  // need_xp: number of extra FP64 n-vectors
   int32_t fsz, asz, mixed;
   int32_t stage_prm;  // the system's element records (params, value table) copied into LDS
+  // Systems too large for LDS: with gws set, the launch runs one system per
+  // block, block b's system (values and reduction buffer) lives at
+  // gws + b * gstride in global memory, and the launch covers systems
+  // sys_off .. sys_off + gridDim.x - 1 (the kernels that take it: k_newton_dc,
+  // k_ac, k_tran, k_tran_lte).
+  char* gws = nullptr;
+  int64_t gstride = 0, sys_off = 0;
 };
 
 __host__ __device__ inline size_t lay_al(size_t b) { return (b + 15) & ~size_t(15); }
+
+#ifdef __HIPCC__
+// Where a block's systems live: its LDS, or its region of the global workspace.
+// The block's barriers order its accesses there as in LDS (one CU, one L1).
+__device__ inline char* sys_region(char* smem, const Lay& lay) {
+  return lay.gws ? lay.gws + (int64_t)blockIdx.x * lay.gstride : smem;
+}
+#endif
 
 __host__ __device__ inline size_t sys_bytes(const Lay& L) {
   size_t s = 64 + lay_al((size_t)L.fsz * L.nnz_max) + 2 * lay_al((size_t)L.asz * L.n_max);
@@ -96,7 +111,7 @@ struct LteArgs {
 
 hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                       int sweep_e, double start, double step, int npts, double* out, const int64_t* out_off,
-                      int32_t* status, int32_t* iters, Lay lay, char* gws = nullptr, int64_t gstride = 0);
+                      int32_t* status, int32_t* iters, Lay lay);
 // AC sweep kernels. fixq / fixcap: the accuracy guard's queue (see the
 // guard constants in kg_internal.h); null: no guard (statuses and error
 // figures as computed).
