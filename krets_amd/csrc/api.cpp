@@ -712,7 +712,7 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
     return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64, 256 or 1024");
   r = prepare(c, AK_DC, order_for(tps), 0.0, P);
   if (r) return r;
-  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, 1, 8, 8, 0, 0};
+  Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, P.any_diode ? 1 : 0, 8, 8, 0, 0};
   Launch L;
   hipError_t e;
   int64_t chunk = c->n_inst;
@@ -720,6 +720,14 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
     if (tps < 64) return r;
     c->err.clear();  // too large for LDS: the values go to a global workspace
     if ((r = configure_global(c, tps, lay, c->n_inst, L, "newton_gws", &chunk))) return r;
+  } else if (c->shared && tps >= 256) {
+    // one large system per block: its factor and solve structure goes to LDS too when it fits
+    const PatEntry& pe = c->pats[c->pat_index[std::make_tuple(0, (int)AK_DC, (int)order_for(tps), 0.0)]];
+    const size_t add = sizeof(int32_t) * (size_t)pe.lay.o_col;
+    if (pe.lay.status == KG_OK && L.lds + add <= LDS_MAX) {
+      lay.stage_plan = pe.lay.o_col;
+      L.lds += add;
+    }
   }
   auto off = out_offsets(c, npts);
   r = upload_offsets(c, "newton_off", off, d_off);

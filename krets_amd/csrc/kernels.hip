@@ -221,6 +221,13 @@ __device__ __forceinline__ int64_t xcd_block() {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// Largest block of a level-scheduled kernel with TPS threads per system:
+// one wave (64 / TPS systems) below 64 threads per system, else one system.
+// Declared as the kernels' launch bound, so that the register allocator
+// budgets for it instead of for a 1024-thread block (128 VGPRs, which made
+// the transient kernels spill about 300 bytes per lane).
+#define KG_LB(TPS) __launch_bounds__((TPS) < 64 ? 64 : (TPS))
+
 // Per-point stamp parameters.
 struct StampArgs {
   int ak;            // AnalysisKind
@@ -755,7 +762,7 @@ __device__ bool newton_converged(const PatDev& P, const double* xnew, const doub
 // instance, sweep points and Newton iterations inside the kernel.
 // out: [instance][npts][n] (krets numbering). FP64.
 template <int TPS>
-__global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_local_e, double start, double step,
+__global__ void KG_LB(TPS) k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_local_e, double start, double step,
                             int npts, double* out, const int64_t* out_off, int32_t* status, int32_t* iters,
                             Lay lay) {
   extern __shared__ __align__(16) char smem[];
@@ -763,11 +770,28 @@ __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_
   const int ls = threadIdx.x / TPS, t = threadIdx.x % TPS;
   const int inst = (int)(lay.sys_off + xcd_block() * spb + ls);
   const bool valid = inst < B.n_instances;
-  const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + (valid ? inst : 0)]];
+  const PatDev& Pg = B.pats[B.inst_pat[kind_slot * B.n_instances + (valid ? inst : 0)]];
   size_t sb = sys_bytes(lay);
   char* base = sys_region(smem, lay);
   double* red = (double*)(base + spb * sb);
   SysMem<double, double> M = carve<double, double>(base + ls * sb, lay);
+  // One large system per block (shared topology): the factor and solve
+  // structure is copied into LDS behind the reduction buffer, so that the
+  // dependent index loads of every level phase are LDS round trips rather
+  // than L2 / MALL ones (the factorisation of a 1002-unknown ladder is a
+  // chain of about 630 dependent updates).
+  PatDev Ps = Pg;
+  if (lay.stage_plan) {
+    int4* dst = (int4*)(smem + spb * sb + 8 * (size_t)spb * TPS);
+    const int4* src = (const int4*)Pg.base;
+    for (int i = threadIdx.x; i < lay.stage_plan / 4; i += blockDim.x) dst[i] = src[i];
+    const int32_t* pl = (const int32_t*)dst;
+    auto rb = [&](const int32_t* p) { return p - Pg.base < lay.stage_plan ? pl + (p - Pg.base) : p; };
+    Ps.row_ptr = rb(Pg.row_ptr); Ps.diag = rb(Pg.diag); Ps.fprog_ptr = rb(Pg.fprog_ptr); Ps.fprog = rb(Pg.fprog);
+    Ps.lvl_ptr = rb(Pg.lvl_ptr); Ps.lvl_rows = rb(Pg.lvl_rows); Ps.blvl_ptr = rb(Pg.blvl_ptr);
+    Ps.blvl_rows = rb(Pg.blvl_rows);
+  }
+  const PatDev& P = Ps;
   const double* prm = B.params + (valid ? B.eoff[inst] : 0) * KG_NPARAM;
   const double* eb = B.ebase + (valid ? B.eoff[inst] : 0);
   if (t == 0) {
@@ -784,10 +808,10 @@ __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_
     bool prev_empty = na.compat ? !have_last : false;
     if (act && !na.compat && !have_last)
       for (int i = t; i < P.n; i += TPS) M.g[i] = 0.0;
-    if (act)
+    if (act && M.ed)  // element state exists when the batch has diodes
       for (int e = t; e < P.ne; e += TPS) M.ed[e * ESTRIDE + 2] = 0.0;
     __syncthreads();
-    if (act && !na.compat)
+    if (act && !na.compat && M.ed)
       for (int e = t; e < P.ne; e += TPS)
         if (P.etype[e] == KG_D) {
           int p = P.ep[e], m = P.em[e];
@@ -860,7 +884,7 @@ __global__ void k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na, int sweep_
 // xop: operating points [instance][n] (krets numbering) for diode small-signal
 // conductances. out: complex128 [instance][nf][n].
 template <class F, class A, int TPS>
-__global__ void k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
+__global__ void KG_LB(TPS) k_ac(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
                      const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status, double* err,
                      int refine, Lay lay, unsigned long long* fixq, int64_t fixcap) {
   extern __shared__ __align__(16) char smem[];
@@ -942,7 +966,7 @@ __device__ PatDev rebase(const PatDev& G, const int32_t* lds) {
 // thousands of steps read their row structure and level lists from LDS
 // instead of issuing dependent L2 loads every step.
 template <class F, class A, int TPS, bool STAGED>
-__global__ void k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, const double* xop,
+__global__ void KG_LB(TPS) k_tran(BatchDev B, int kind_slot, NewtonArgs na, TranArgs ta, const double* xop,
                        const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status,
                        int32_t* iters, double* err, Lay lay, int plan_ints) {
   extern __shared__ __align__(16) char smem[];
@@ -1991,7 +2015,7 @@ __global__ void __launch_bounds__(TPS) k_adj_lin_bwd(BatchDev B, int kind_slot, 
 // system per block); it is copied into the first lay_al(4 * plan_ints) bytes
 // of LDS once.
 template <int TPS, bool STAGED>
-__global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la, const double* xop,
+__global__ void KG_LB(TPS) k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la, const double* xop,
                            const int64_t* xop_off, double* tout, double* xout, const int64_t* xout_off, int32_t* npts,
                            int32_t* newton_total, int32_t* rejected, int32_t* status, Lay lay, int plan_ints) {
   extern __shared__ __align__(16) char smem[];
@@ -2209,7 +2233,7 @@ __global__ void k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, LteArgs la,
 // LDS: [plan (plan_ints, 16-aligned)] [element records ne*KG_NPARAM + ebase ne]
 //      [SPB systems] [reduction scratch].
 template <class F, class A, int TPS>
-__global__ void k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
+__global__ void KG_LB(TPS) k_ac_p(BatchDev B, int kind_slot, const double* freqs, int nf, const double* xop,
                        const int64_t* xop_off, double* out, const int64_t* out_off, int32_t* status, double* err,
                        int refine, Lay lay, int plan_ints, int64_t chunk, unsigned long long* fixq,
                        int64_t fixcap) {
@@ -2547,7 +2571,7 @@ __device__ double sys_sum(double v, double* red, int ls, int t) {
 // (solution and companion states) every K steps.
 // ckpt: [instance][ncp][cpw] with cpw >= n + ne; cvec in krets numbering.
 template <class F, class A, int TPS, bool STAGED>
-__global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* xop,
+__global__ void KG_LB(TPS) k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* xop,
                           const int64_t* xop_off, double* ckpt, int64_t ck_stride, int cpw, double* Jout,
                           int32_t* status, Lay lay, int plan_ints) {
   extern __shared__ __align__(16) char smem[];
@@ -2645,7 +2669,7 @@ __global__ void k_adj_fwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
 // every R, C and L, and the adjoint states of the trapezoidal companions.
 // xbar0 receives dJ/dx_0 (krets numbering) for the operating-point adjoint.
 template <class F, class A, int TPS, bool STAGED>
-__global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* ckpt,
+__global__ void KG_LB(TPS) k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* cvec, const double* ckpt,
                           int64_t ck_stride, int cpw, double* seg, int64_t seg_stride, double* grad,
                           int64_t grad_stride, double* xbar0, const int64_t* xop_off, int32_t* status, Lay lay,
                           int plan_ints) {
@@ -2868,7 +2892,7 @@ __global__ void k_adj_bwd(BatchDev B, int kind_slot, AdjArgs aa, const double* c
 // b_dc, so dJ/dR gains (e^T lam0)(e^T x_0)/R^2 with lam0 = A_dc^-T xbar0.
 // FP64; DC plan; xbar0 and xop in krets numbering.
 template <int TPS>
-__global__ void k_adj_op(BatchDev B, int kind_slot, const double* xbar0, const double* xop, const int64_t* xop_off,
+__global__ void KG_LB(TPS) k_adj_op(BatchDev B, int kind_slot, const double* xbar0, const double* xop, const int64_t* xop_off,
                          double* grad, int64_t grad_stride, int32_t* status, Lay lay) {
   extern __shared__ __align__(16) char smem[];
   const int spb = blockDim.x / TPS;

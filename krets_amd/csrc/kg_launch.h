@@ -38,6 +38,10 @@ struct Lay {
   int32_t need_g, need_xp, need_ed;  // need_xp: number of extra FP64 n-vectors
   int32_t fsz, asz, mixed;
   int32_t stage_prm;  // the system's element records (params, value table) copied into LDS
+  // k_newton_dc: ints of the plan prefix (factor and solve structure, up to
+  // o_col) copied into LDS behind the block's systems and reduction buffer;
+  // 0: read from global memory
+  int32_t stage_plan = 0;
   // Systems too large for LDS: with gws set, the launch runs one system per
   // block, block b's system (values and reduction buffer) lives at
   // gws + b * gstride in global memory, and the launch covers systems

@@ -593,8 +593,10 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
     while (buf.size() % 4) buf.push_back(0);
     return off;
   };
+  // The factor and solve structure first (row pointers, diagonal slots, the
+  // factor program, both level lists): a kernel can copy the prefix up to
+  // o_col into LDS in one pass.
   L.o_row_ptr = put(row_ptr);
-  L.o_col = put(col);
   L.o_diag = put(diag);
   L.o_fprog_ptr = put(fptr);
   L.o_fprog = put(fprog);
@@ -602,6 +604,7 @@ std::vector<int32_t> build_pattern(const kg_element* el, int ne, int n, Analysis
   L.o_lvl_rows = put(lvl_rows);
   L.o_blvl_ptr = put(blvl_ptr);
   L.o_blvl_rows = put(blvl_rows);
+  L.o_col = put(col);
   L.o_scon_ptr = put(scon_ptr);
   L.o_scon = put(scon);
   L.o_rcon_ptr = put(rcon_ptr);

@@ -64,6 +64,20 @@ def test_op_series_ladders(oracle):
         assert abs(s["I(V1)"] * N * 1000.0 + 1.0) < 1e-11
 
 
+def test_series_ladder_generator_matches_krets_files():
+    """workloads.series_ladder(N) (used for krets' resistor_ladder_5000, whose
+    circuit file is not among the fixtures) writes the element lines of krets'
+    resistor_ladder_N files, which are."""
+    import re
+    from krets_amd import workloads as W
+
+    def elements(text):
+        return [ln.split() for ln in text.splitlines() if re.match(r"^[RV]\d", ln)]
+
+    for N in (500, 1000):
+        assert elements(W.series_ladder(N)) == elements(read_circuit(f"resistor_ladder_{N}/resistor_ladder_{N}.cir"))
+
+
 def test_op_diode(oracle):
     # test_op.rs:131-140; closed form: scalar Newton on 1 = 1000*I + V, I = Is(e^{V/Vt}-1)
     c = oracle.Circuit(read_circuit("diode_iv_curve/diode_iv_curve.cir"))
