@@ -147,7 +147,13 @@ const char* kg_last_error(const kg_ctx* ctx);
  * dc.rs:81-84, ac.rs:67-70, transient.rs:74-77) with one host symbolic
  * pass per distinct topology: pattern, static pivot order (structural
  * matching plus fill-reducing order), fill, level sets; then uploads the
- * batch. */
+ * batch. The pivots come from the topology's first instance; an instance
+ * whose static-pivot factorisation is not backward stable is reported, not
+ * returned silently: operating points, DC sweeps and transients (FP64) bound
+ * each returned solution's normwise backward error by the LU growth of its
+ * factors and report KG_DECOMPOSITION above 2^-30 (krets' partial pivoting
+ * would have re-pivoted); AC sweeps re-solve such systems with partial
+ * pivoting (kg_ac_sweep). */
 int32_t kg_load_batch(kg_ctx* ctx, const kg_batch_desc* batch);
 
 /* op::solve (crates/krets-solver/src/solver/op.rs:15-86) for every instance.
@@ -206,7 +212,10 @@ int32_t kg_ac_sweep_noalloc(kg_ctx* ctx, const kg_solver_config* cfg, const doub
  * krets' integrator with its quirks; KG_BE and KG_TRAP are clean.
  * x: [n_instances][nsteps+1][n] (row 0 = operating point); iters:
  * [n_instances][nsteps+1] Newton iterations (may be NULL). One persistent
- * launch runs all steps of all instances. */
+ * launch runs all steps of all instances. err_est [n_instances] (may be
+ * NULL): with KG_MIXED the refinement's estimate of the relative error,
+ * with KG_FP64 the largest bound on the normwise backward error of the
+ * instance's solves (the static-pivot guard of kg_load_batch). */
 int32_t kg_transient(kg_ctx* ctx, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
                      int32_t precision, double* x, int32_t* status, int32_t* iters, double* err_est,
                      int32_t mem);

@@ -706,10 +706,11 @@ int32_t run_dc(kg_ctx* c, const kg_solver_config* cfg, int sweep_e, double start
     if (tps0 == 2001) return set_err(c, KG_UNEXPECTED, "lane kernel: " + why);
   }
   // a few large systems (a single operating point of a long ladder): a whole
-  // CU per system, 1024 threads over each elimination level
-  int tps = tps0 ? tps0 : (P0.n_max > 600 && c->n_inst <= 64 ? 1024 : default_tps(c, P0.n_max));
-  if (!valid_tps(tps) && tps != 1024)
-    return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64, 256 or 1024");
+  // CU per system, 512 threads over each elimination level (1024 halves the
+  // registers per thread to 128, and the kernel spills there)
+  int tps = tps0 ? tps0 : (P0.n_max > 600 && c->n_inst <= 64 ? 512 : default_tps(c, P0.n_max));
+  if (!valid_tps(tps) && tps != 512 && tps != 1024)
+    return set_err(c, KG_INVALID_ARGUMENT, "threads_per_system must be 1, 4, 16, 64, 256, 512 or 1024");
   r = prepare(c, AK_DC, order_for(tps), 0.0, P);
   if (r) return r;
   Lay lay{P.nnz_max, P.n_max, P.ne_max, 1, 0, P.any_diode ? 1 : 0, 8, 8, 0, 0};

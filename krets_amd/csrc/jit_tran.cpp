@@ -12,6 +12,7 @@
 // l_ik = a_ik * (1 / u_kk) stored in place, the upper factor unscaled, the
 // pivot reciprocal kept on the diagonal), so both produce the same accepted
 // time grid up to the compiler's contractions.
+#include <algorithm>
 #include <cstdio>
 #include <functional>
 #include <set>
@@ -83,6 +84,13 @@ __device__ __forceinline__ double wave(const double* p, double t) {
 
 std::string S(int i) { return std::to_string(i); }
 
+// KG_GUARD_STATIC as a literal of the generated code.
+std::string fmt_static() {
+  char b[32];
+  std::snprintf(b, sizeof b, "%.17g", KG_GUARD_STATIC);
+  return b;
+}
+
 // Value of permuted unknown i in vector `pre` (ground: 0.0), as the generic
 // kernels write it: (p >= 0 ? x[p] : 0.0).
 std::string node(const std::string& pre, int i) { return i >= 0 ? pre + S(i) : std::string("0.0"); }
@@ -99,10 +107,24 @@ using CodeFn = std::string (*)(const TV& P, int e, int k);
 // its forward substitution; then the back substitution. Each row only reads
 // the upper factor of earlier rows, so a row's multipliers die with it and
 // the arithmetic is that of factorising everything first (the same
-// operations in the same order). before_row(i) may emit loads first.
+// operations in the same order). before_row(i) may emit loads first;
+// on_result(i) runs once x<i> is final (back substitution order).
+//
+// It also assigns the caller's `gb` the static-pivot guard's bound on the
+// solve's normwise backward error (KG_GUARD_STATIC; the level-scheduled
+// kernels' pivot_guard): guard_bound_coeff(m) * rho with
+// rho = || |L||U| ||_inf / ||A||_inf. an_ accumulates ||A||_inf over the
+// stamped rows, gr<i> is row i's |U| sum (its pivot and upper entries once
+// the row is factored), gm_ the rows of |L||U|.
 bool emit_solve(std::ostringstream& o, const TV& P, const char* ind, CodeFn slot, CodeFn exc, std::string& why,
-                const std::function<void(int)>& before_row = nullptr) {
+                const std::function<void(int)>& before_row = nullptr,
+                const std::function<void(int)>& on_result = nullptr) {
   const int n = P.n;
+  int m = 0;
+  for (int i = 0; i < n; i++) m = std::max(m, P.row_ptr[i + 1] - P.row_ptr[i]);
+  char coeff[32];
+  std::snprintf(coeff, sizeof coeff, "%.17g", guard_bound_coeff(m));
+  o << ind << "double an_ = 0.0, gm_ = 0.0;\n";
   for (int i = 0; i < n; i++) {
     if (before_row) before_row(i);
     for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) {
@@ -126,7 +148,9 @@ bool emit_solve(std::ostringstream& o, const TV& P, const char* ind, CodeFn slot
       }
       o << " + (" << c << ")";
     }
-    o << ";\n";
+    o << ";\n" << ind << "an_ = fmax(an_, 0.0";
+    for (int s = P.row_ptr[i]; s < P.row_ptr[i + 1]; s++) o << " + fabs(v" << s << ")";
+    o << ");\n";
     const int32_t* fp = P.fprog + P.fprog_ptr[i];
     const int nk = *fp++;
     for (int kk = 0; kk < nk; kk++) {
@@ -146,6 +170,14 @@ bool emit_solve(std::ostringstream& o, const TV& P, const char* ind, CodeFn slot
       }
     }
     const int d = P.diag[i];
+    o << ind << "const double gr" << i << " = fabs(v" << d << ")";
+    for (int p = d + 1; p < P.row_ptr[i + 1]; p++) o << " + fabs(v" << p << ")";
+    o << ";\n" << ind << "gm_ = fmax(gm_, gr" << i;
+    for (int p = P.row_ptr[i]; p < d; p++) o << " + fabs(v" << p << ") * gr" << P.col[p];
+    o << ");\n";
+    // the running maxima are taken here, not sunk to their use at the end
+    // (which would keep every row's pivot and multipliers live)
+    o << ind << "asm volatile(\"\" :: \"v\"(an_), \"v\"(gm_));\n";
     o << ind << "bad |= !(fabs(v" << d << ") > 0.0) || !__builtin_isfinite(v" << d << ");\n"
       << ind << "v" << d << " = 1.0 / v" << d << ";\n";
     for (int p = P.row_ptr[i]; p < d; p++) o << ind << "x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
@@ -154,7 +186,9 @@ bool emit_solve(std::ostringstream& o, const TV& P, const char* ind, CodeFn slot
     const int d = P.diag[i];
     for (int p = d + 1; p < P.row_ptr[i + 1]; p++) o << ind << "x" << i << " = x" << i << " - v" << p << " * x" << P.col[p] << ";\n";
     o << ind << "x" << i << " = x" << i << " * v" << d << ";\n";
+    if (on_result) on_result(i);
   }
+  o << ind << "gb = gm_ > 0.0 ? " << coeff << " * (gm_ / an_) : 0.0;\n";
   return true;
 }
 
@@ -245,7 +279,7 @@ bool jit_lte_source(const PatLayout& L, const int32_t* b, std::string& src, std:
        "    int max_points, double reltol, double iabstol, double vabstol, int maxiter) {\n"
        "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
        "  if (inst >= ninst) return;\n"
-       "  int st = status[inst], cnt = 0, nt = ntot[inst], rej = 0;\n"
+       "  int st = status[inst], cnt = 0, nt = ntot[inst], rej = 0;\n  double gb = 0.0;\n"
        "  if (st == 0) {\n";
   // element values, loaded once
   o << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
@@ -328,6 +362,7 @@ bool jit_lte_source(const PatLayout& L, const int32_t* b, std::string& src, std:
     << "      if (stop) { done = true; its = it + 1; break; }\n";
   for (int i = 0; i < n; i++) o << "      gu" << i << " = x" << i << ";\n";
   o << "    }\n"
+    << "    if (done && !(gb <= " << fmt_static() << ")) st = " << (int)KG_DECOMPOSITION << ";  // static-pivot guard\n"
        "    if (st != 0) break;\n"
        "    if (!done) {  // Newton failure: retry with h / 8\n"
        "      rej++;\n      h /= 8.0;\n"
@@ -444,16 +479,18 @@ bool jit_dc_source(const PatLayout& L, const int32_t* b, std::string& src, std::
        "  if (inst >= ninst) return;\n"
     << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
     << "  const double* EB = ebase + (i64)inst * " << ne << ";\n"
-    << "  int st = 0;\n";
+    << "  int st = 0;\n  double gb = 0.0;\n";
   for (int i = 0; i < n; i++) o << "  double x" << i << ";\n";
   o << "  for (int pt = 0; pt < npts && st == 0; pt++) {\n"
        "    const double sv = start + (double)pt * step;\n"
        "    bool bad = false;\n";
   for (int r = 0; r < kAhead; r++) issue(r);
-  if (!emit_solve(o, P, "    ", dc_slot, dc_exc, why, [&](int i) { issue(i + kAhead); })) return false;
-  o << "    if (bad) st = " << (int)KG_DECOMPOSITION << ";\n"
-    << "    double* o = out + out_off[inst] + (i64)pt * " << n << ";\n";
-  for (int i = 0; i < n; i++) o << "    o[" << P.col_of[i] << "] = x" << i << ";\n";
+  // each unknown is written as soon as the back substitution has it (no solution vector held to the end)
+  o << "    double* o = out + out_off[inst] + (i64)pt * " << n << ";\n";
+  if (!emit_solve(o, P, "    ", dc_slot, dc_exc, why, [&](int i) { issue(i + kAhead); },
+                  [&](int i) { o << "    o[" << P.col_of[i] << "] = x" << i << ";\n"; }))
+    return false;
+  o << "    if (bad || !(gb <= " << fmt_static() << ")) st = " << (int)KG_DECOMPOSITION << ";\n";
   o << "    if (iters) iters[(i64)inst * npts + pt] = 1;\n"
        "  }\n"
        "  status[inst] = st;\n}\n";
@@ -502,7 +539,7 @@ bool jit_tran_source(const PatLayout& L, const int32_t* b, std::string& src, std
        "    int method, double reltol, double iabstol, double vabstol, int maxiter) {\n"
        "  const int inst = (int)blockIdx.x * 64 + (int)threadIdx.x;\n"
        "  if (inst >= ninst) return;\n"
-       "  int st = status[inst];\n"
+       "  int st = status[inst];\n  double gb = 0.0, gbmax = 0.0;\n"
        "  const bool compat = method == 0, trap = method == 2;\n"
        "  if (st == 0) {\n"
     << "  const double* PR = prm + (i64)inst * " << (int64_t)ne * KG_NPARAM << ";\n"
@@ -601,6 +638,8 @@ bool jit_tran_source(const PatLayout& L, const int32_t* b, std::string& src, std
     o << "      done = true;\n      break;\n";
   }
   o << "    }\n"
+    << "    if (done) {  // static-pivot guard\n      gbmax = fmax(gbmax, gb);\n"
+    << "      if (!(gb <= " << fmt_static() << ")) st = " << (int)KG_DECOMPOSITION << ";\n    }\n"
        "    if (st != 0) break;\n"
     << "    if (!done) { st = " << (int)KG_MAX_ITERATIONS << "; break; }\n"
     << "    if (trap) {\n";
@@ -616,7 +655,7 @@ bool jit_tran_source(const PatLayout& L, const int32_t* b, std::string& src, std
        "  }\n"
        "  }\n"
        "  status[inst] = st;\n"
-       "  if (err) err[inst] = 0.0;\n}\n";
+       "  if (err) err[inst] = gbmax;\n}\n";
   src = o.str();
   return true;
 }
@@ -670,7 +709,7 @@ bool jit_dc_newton_source(const PatLayout& L, const TV& P, std::string& src, std
         << "  const double vcritc" << e << " = nvt" << e << " * log(1.7976931348623157e308 * nd" << e
         << " * KG_VT / is" << e << ");\n";
   }
-  o << "  int st = 0;\n";
+  o << "  int st = 0;\n  double gb = 0.0;\n";
   for (int i = 0; i < n; i++) o << "  double gu" << i << " = 0.0, x" << i << " = 0.0;\n";
   o << "  for (int pt = 0; pt < npts && st == 0; pt++) {\n"
        "    const double sv = start + (double)pt * step;\n"
@@ -725,6 +764,7 @@ bool jit_dc_newton_source(const PatLayout& L, const TV& P, std::string& src, std
   for (int i = 0; i < n; i++) o << "      gu" << i << " = x" << i << ";\n";
   o << "    }\n"
     << "    if (!done) st = " << (int)KG_MAX_ITERATIONS << ";\n"
+    << "    else if (st == 0 && !(gb <= " << fmt_static() << ")) st = " << (int)KG_DECOMPOSITION << ";  // static-pivot guard\n"
     << "    double* o = out + out_off[inst] + (i64)pt * " << n << ";\n";
   for (int i = 0; i < n; i++) o << "    o[" << P.col_of[i] << "] = x" << i << ";\n    gu" << i << " = x" << i << ";\n";
   o << "    if (iters) iters[(i64)inst * npts + pt] = used;\n"

@@ -345,9 +345,17 @@ __device__ void stamp_rhs(const PatDev& P, A* b, const double* prm, const StampA
     }
 }
 
-template <class F, int TPS>
-__device__ void factor(const PatDev& P, F* v, int t, bool act, int* stat) {
+// anorm (NORM, FP64): the largest row sum of |A| among the rows this thread
+// factors, taken from each row just before its elimination (the static-pivot
+// guard's ||A||_inf without re-gathering the stamps).
+template <class F, int TPS, bool NORM = false>
+__device__ void factor(const PatDev& P, F* v, int t, bool act, int* stat, double* anorm = nullptr) {
   auto row = [&](int i) {
+    if constexpr (NORM) {
+      double s = 0.0;
+      for (int p = P.row_ptr[i]; p < P.row_ptr[i + 1]; p++) s += fabs(v[p]);
+      *anorm = fmax(*anorm, s);
+    }
     const int32_t* fp = P.fprog + P.fprog_ptr[i];
     int nk = *fp++;
     for (int kk = 0; kk < nk; kk++) {
@@ -605,6 +613,92 @@ __device__ double guard_eta(const PatDev& P, const A* b, const A* x, const doubl
   return den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
 }
 
+// Static-pivot guard of the FP64 OP, DC sweep and transient factorisations
+// (KG_GUARD_STATIC): the lane AC kernels' a-priori bound guard_bound_coeff(m)
+// * rho, rho = || |L||U| ||_inf / ||A||_inf, formed after the fact from the
+// factors in v (multipliers below the diagonal, the pivot's reciprocal on it,
+// the upper factor unscaled), with ||A||_inf taken by factor<..., NORM> as
+// it eliminates. This thread's parts over the rows t, t + TPS, ...:
+// pivot_guard_rows returns the longest row m and writes the row sums of |U|
+// into us; after a barrier, pivot_guard_lu returns the rows of |L||U|.
+template <int TPS>
+__device__ double pivot_guard_rows(const PatDev& P, const double* v, double* us, int t, bool act) {
+  double m = 0.0;
+  if (act)
+    for (int i = t; i < P.n; i += TPS) {
+      const int d = P.diag[i];
+      double u = 1.0 / fabs(v[d]);
+      for (int p = d + 1; p < P.row_ptr[i + 1]; p++) u += fabs(v[p]);
+      us[i] = u;
+      m = fmax(m, (double)(P.row_ptr[i + 1] - P.row_ptr[i]));
+    }
+  return m;
+}
+
+template <int TPS>
+__device__ double pivot_guard_lu(const PatDev& P, const double* v, const double* us, int t, bool act) {
+  double gm = 0.0;
+  if (act)
+    for (int i = t; i < P.n; i += TPS) {
+      double g = us[i];
+      for (int p = P.row_ptr[i]; p < P.diag[i]; p++) g += fabs(v[p]) * us[P.col[p]];
+      gm = fmax(gm, g);
+    }
+  return gm;
+}
+
+__device__ __forceinline__ double pivot_guard_bound(double gm, double an, double m) {
+  return gm > 0.0 ? (an > 0.0 ? guard_bound_coeff((int)m) * (gm / an) : INFINITY) : 0.0;
+}
+
+// Maxima of three values over the TPS threads of a system: shuffles inside
+// the wave (a system's threads are consecutive lanes), then, for systems of
+// several waves, one exchange through red (TPS doubles of the system) between
+// two barriers. Reached by every thread of the block.
+template <int TPS>
+__device__ void sys_max3(double& a, double& b, double& c, double* red, int ls, int t) {
+  constexpr int W = TPS < 64 ? TPS : 64;
+  for (int o = W / 2; o > 0; o >>= 1) {
+    a = fmax(a, __shfl_xor(a, o, W));
+    b = fmax(b, __shfl_xor(b, o, W));
+    c = fmax(c, __shfl_xor(c, o, W));
+  }
+  if constexpr (TPS > 64) {
+    double* r = red + ls * TPS;
+    if ((t & 63) == 0) {
+      r[3 * (t / 64)] = a;
+      r[3 * (t / 64) + 1] = b;
+      r[3 * (t / 64) + 2] = c;
+    }
+    __syncthreads();
+    for (int k = 0; k < TPS / 64; k++) {
+      a = fmax(a, r[3 * k]);
+      b = fmax(b, r[3 * k + 1]);
+      c = fmax(c, r[3 * k + 2]);
+    }
+    __syncthreads();
+  }
+}
+
+// Level-scheduled form, reached by every thread of the system; an: this
+// thread's part of ||A||_inf from factor<..., NORM>; us: n doubles of scratch.
+template <int TPS>
+__device__ double pivot_guard(const PatDev& P, const double* v, double an, double* us, double* red, int ls, int t,
+                              bool act) {
+  double m = pivot_guard_rows<TPS>(P, v, us, t, act);
+  __syncthreads();
+  double gm = pivot_guard_lu<TPS>(P, v, us, t, act);
+  sys_max3<TPS>(an, m, gm, red, ls, t);
+  return pivot_guard_bound(gm, an, m);
+}
+
+// Block-wide maximum of non-negative values into a 64-bit LDS word that was
+// zeroed before a barrier: a wave maximum, then one atomic per wave.
+__device__ __forceinline__ void block_max_to(double v, unsigned long long* w) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(w, (unsigned long long)__double_as_longlong(v));
+}
+
 // Queues system `sys` for k_ac_fix (queue words: [0] count, [1 + q] systems);
 // a full queue marks the instance as failed instead.
 __device__ __forceinline__ void fix_push(unsigned long long* fixq, int64_t cap, int64_t sys, int32_t* status,
@@ -800,7 +894,7 @@ __global__ void KG_LB(TPS) k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na,
     M.flags[2] = 0;
   }
   __syncthreads();
-  StampArgs sa{AK_DC, 0.0, 1.0, 1.0, 0.0, sweep_local_e, 0.0};
+    StampArgs sa{AK_DC, 0.0, 1.0, 1.0, 0.0, sweep_local_e, 0.0};
   bool have_last = false;  // dc.rs: warm start from the previous point
   for (int pt = 0; pt < npts; pt++) {
     sa.sweep_val = start + (double)pt * step;
@@ -820,6 +914,7 @@ __global__ void KG_LB(TPS) k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na,
     __syncthreads();
     int it = 0;
     bool done = !act;
+    double anp = 0.0;  // this thread's part of ||A||_inf of the last factorisation
     for (it = 0; it < na.maxiter; it++) {
       bool a2 = !done;
       if (t == 0) M.flags[4] = 0;
@@ -829,7 +924,8 @@ __global__ void KG_LB(TPS) k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na,
       stamp_matrix<double, double, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
       stamp_rhs<double, TPS>(P, M.b, prm, sa, M.ed, nullptr, t, a2);
       __syncthreads();
-      factor<double, TPS>(P, M.v, t, a2, &M.flags[0]);
+      if (a2) anp = 0.0;  // a system that converged earlier keeps its last factorisation's
+      factor<double, TPS, true>(P, M.v, t, a2, &M.flags[0], &anp);
       solve_refine<double, double, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, 0, red, ls, t, a2);
       __syncthreads();
       bool stop;
@@ -866,6 +962,12 @@ __global__ void KG_LB(TPS) k_newton_dc(BatchDev B, int kind_slot, NewtonArgs na,
       M.flags[2] = na.maxiter;
     }
     __syncthreads();
+    {  // static-pivot guard on the factors of the point's solution (b is free: scratch)
+      const bool g2 = act && M.flags[0] == KG_OK;
+      const double bnd = pivot_guard<TPS>(P, M.v, anp, M.b, red, ls, t, g2);
+      if (g2 && t == 0 && !(bnd <= KG_GUARD_STATIC)) M.flags[0] = KG_DECOMPOSITION;
+      __syncthreads();
+    }
     if (act) {
       double* o = out + out_off[inst] + (int64_t)pt * P.n;
       for (int j = t; j < P.n; j += TPS) o[P.col_of[j]] = M.x[j];
@@ -1027,13 +1129,23 @@ __global__ void KG_LB(TPS) k_tran(BatchDev B, int kind_slot, NewtonArgs na, Tran
   __syncthreads();
   StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
   const bool linear = !P.nonlinear;
+  // FP64: the static-pivot guard's bound is the error figure (mixed: the refinement estimate)
+  constexpr bool fp64 = std::is_same<F, double>::value;
+  double est_max = 0.0;
+  double anp = 0.0;  // FP64: this thread's part of ||A||_inf of the last factorisation
   if (linear) {
     stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, act);
     __syncthreads();
-    factor<F, TPS>(P, M.v, t, act, &M.flags[0]);
+    factor<F, TPS, fp64>(P, M.v, t, act, &M.flags[0], &anp);
     __syncthreads();
+    if constexpr (fp64) {  // static-pivot guard: one factorisation serves every step (b: scratch)
+      const bool g2 = act && M.flags[0] == KG_OK;
+      const double bnd = pivot_guard<TPS>(P, M.v, anp, M.b, red, ls, t, g2);
+      if (g2) est_max = bnd;
+      if (g2 && t == 0 && !(bnd <= KG_GUARD_STATIC)) M.flags[0] = KG_DECOMPOSITION;
+      __syncthreads();
+    }
   }
-  double est_max = 0.0;
   for (int s = 1; s <= ta.nsteps; s++) {
     act = M.flags[1] != 0 && M.flags[0] == KG_OK;
     const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
@@ -1069,7 +1181,8 @@ __global__ void KG_LB(TPS) k_tran(BatchDev B, int kind_slot, NewtonArgs na, Tran
         stamp_matrix<F, A, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
         stamp_rhs<A, TPS>(P, M.b, prm, sa, M.ed, M.xp, t, a2);
         __syncthreads();
-        factor<F, TPS>(P, M.v, t, a2, &M.flags[0]);
+        if (a2) anp = 0.0;  // a system that converged earlier keeps its last factorisation's
+        factor<F, TPS, fp64>(P, M.v, t, a2, &M.flags[0], &anp);
         double est =
             solve_refine<F, A, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, ta.refine, red, ls, t, a2, M.r);
         est_max = fmax(est_max, est);
@@ -1102,6 +1215,13 @@ __global__ void KG_LB(TPS) k_tran(BatchDev B, int kind_slot, NewtonArgs na, Tran
         used = na.maxiter;
       }
       __syncthreads();
+      if constexpr (fp64) {  // static-pivot guard on the factors of the step's solution (b: scratch)
+        const bool g2 = act && M.flags[0] == KG_OK;
+        const double bnd = pivot_guard<TPS>(P, M.v, anp, M.b, red, ls, t, g2);
+        if (g2) est_max = fmax(est_max, bnd);
+        if (g2 && t == 0 && !(bnd <= KG_GUARD_STATIC)) M.flags[0] = KG_DECOMPOSITION;
+        __syncthreads();
+      }
       act = act && M.flags[0] == KG_OK;
     }
     // trapezoidal state update, store, advance
@@ -1260,12 +1380,17 @@ struct LinSys {
   }
 
   // Stamps and factors the constant matrix (full plan P), fills the companion
-  // coefficients, source records and the source table. flag: LDS status word.
-  __device__ void factorise(const PatDev& P, const double* prm, const double* eb, const TranArgs& ta, int* flag) {
+  // coefficients, source records and the source table. flag: LDS status
+  // word, followed by at least 28 free bytes. Returns the static-pivot
+  // guard's bound (KG_GUARD_STATIC; a bound above it sets *flag).
+  __device__ double factorise(const PatDev& P, const double* prm, const double* eb, const TranArgs& ta, int* flag) {
     StampArgs sa{AK_TRAN, 0.0, ta.h, ta.alpha, 0.0, -1, 0.0};
+    unsigned long long* gw = reinterpret_cast<unsigned long long*>(flag + 2);  // ||A||, m, |||L||U|||
+    if (t == 0) gw[0] = gw[1] = gw[2] = 0ull;
     stamp_matrix<double, double, TPS>(P, v, prm, eb, sa, ed, t, true);
     __syncthreads();
-    factor<double, TPS>(P, v, t, true, flag);
+    double anp = 0.0;
+    factor<double, TPS, true>(P, v, t, true, flag, &anp);
     for (int d = t; d < ndyn; d += TPS) {
       const double val = prm[(int)(dyn[d * LIN_DYN] & 0x3FFF) * KG_NPARAM];
       cf[d] = ta.alpha * (val / ta.h);
@@ -1273,6 +1398,15 @@ struct LinSys {
     for (int i = t; i < nsrc * KG_NPARAM; i += TPS)
       sp[i] = prm[(int)src[(i / KG_NPARAM) * LIN_SRC + 1] * KG_NPARAM + i % KG_NPARAM];
     __syncthreads();
+    // static-pivot guard (xc is free until the first step: scratch for the |U| row sums)
+    block_max_to(anp, &gw[0]);
+    block_max_to(pivot_guard_rows<TPS>(P, v, xc, t, true), &gw[1]);
+    __syncthreads();
+    block_max_to(pivot_guard_lu<TPS>(P, v, xc, t, true), &gw[2]);
+    __syncthreads();
+    const double bnd = pivot_guard_bound(__longlong_as_double((long long)gw[2]), __longlong_as_double((long long)gw[0]),
+                                         __longlong_as_double((long long)gw[1]));
+    if (t == 0 && *flag == KG_OK && !(bnd <= KG_GUARD_STATIC)) *flag = KG_DECOMPOSITION;
     for (int i = t; i < nsrc * nst1; i += TPS) {
       const int sidx = i / nst1, s = i - sidx * nst1;
       const double tn = (double)s * ta.h, tprev = (double)(s - 1) * ta.h;
@@ -1280,6 +1414,7 @@ struct LinSys {
       const double* pe = sp + sidx * KG_NPARAM;
       tab[i] = s == 0 ? 0.0 : (src[sidx * LIN_SRC] == KG_V ? wave_value(pe, tsrc) : pe[0]);
     }
+    return bnd;
   }
 
   __device__ void contrib(int code, uint32_t& ab, uint32_t& df) const {
@@ -1511,7 +1646,7 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
   int* flags = reinterpret_cast<int*>(S.setup(P, words, smem, ta, srcv, nsrc_max, inst));
   if (t == 0) flags[0] = KG_OK;
   __syncthreads();
-  S.factorise(P, B.params + B.eoff[inst] * KG_NPARAM, B.ebase + B.eoff[inst], ta, &flags[0]);
+  const double bnd = S.factorise(P, B.params + B.eoff[inst] * KG_NPARAM, B.ebase + B.eoff[inst], ta, &flags[0]);
   const double* x0 = xop + xop_off[inst];
   double* o = out + out_off[inst];
   for (int j = t; j < S.n; j += TPS) {
@@ -1524,7 +1659,7 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
   if (flags[0] != KG_OK) {
     if (t == 0) {
       status[inst] = flags[0];
-      if (err) err[inst] = 0.0;
+      if (err) err[inst] = bnd;
     }
     return;
   }
@@ -1547,7 +1682,7 @@ __global__ void __launch_bounds__(TPS) k_tran_lin(BatchDev B, int kind_slot, Tra
   }
   if (t == 0) {
     status[inst] = flags[0];
-    if (err) err[inst] = 0.0;
+    if (err) err[inst] = bnd;
   }
 }
 
@@ -2075,7 +2210,7 @@ __global__ void KG_LB(TPS) k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, 
     cnt = 1;
   }
   __syncthreads();
-  StampArgs sa{AK_TRAN, 0.0, 1.0, 1.0, 0.0, -1, 0.0};
+    StampArgs sa{AK_TRAN, 0.0, 1.0, 1.0, 0.0, -1, 0.0};
   const int64_t guard_max = 64 * (int64_t)la.max_points + 1000;
   for (int64_t guard = 0;; guard++) {
     if (!__syncthreads_or(active ? 1 : 0)) break;
@@ -2103,6 +2238,7 @@ __global__ void KG_LB(TPS) k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, 
     __syncthreads();
     bool done = !active;
     int its = 0;
+    double anp = 0.0;  // this thread's part of ||A||_inf of the last factorisation
     for (int it = 0; it < na.maxiter; it++) {
       bool a2 = !done;
       if (t == 0) M.flags[4] = 0;
@@ -2112,7 +2248,8 @@ __global__ void KG_LB(TPS) k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, 
       stamp_matrix<double, double, TPS>(P, M.v, prm, eb, sa, M.ed, t, a2);
       stamp_rhs<double, TPS>(P, M.b, prm, sa, M.ed, xc, t, a2);
       __syncthreads();
-      factor<double, TPS>(P, M.v, t, a2, &M.flags[0]);
+      if (a2) anp = 0.0;  // a system that converged earlier keeps its last factorisation's
+      factor<double, TPS, true>(P, M.v, t, a2, &M.flags[0], &anp);
       solve_refine<double, double, TPS>(P, M.v, M.b, M.x, M.w, prm, eb, sa, M.ed, 0, red, ls, t, a2);
       __syncthreads();
       bool stop;
@@ -2134,6 +2271,14 @@ __global__ void KG_LB(TPS) k_tran_lte(BatchDev B, int kind_slot, NewtonArgs na, 
       if (__syncthreads_and(done ? 1 : 0)) break;
     }
     bool ok = active && done && M.flags[0] == KG_OK;
+    {  // static-pivot guard on the factors of the step's solution (b: scratch)
+      const double bnd = pivot_guard<TPS>(P, M.v, anp, M.b, red, ls, t, ok);
+      if (ok && !(bnd <= KG_GUARD_STATIC)) {
+        if (t == 0) M.flags[0] = KG_DECOMPOSITION;
+        ok = false;
+      }
+      __syncthreads();
+    }
     if (active && M.flags[0] != KG_OK) active = false;
     if (active && !done) {  // Newton failure: retry with h / 8
       rej++;
@@ -2955,6 +3100,9 @@ hipError_t run_newton(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, Batch
                       int32_t* status, int32_t* iters, Lay lay) {
   if (tps == 1024)  // one large system per CU: four times the threads per level of 256
     return launch(k_newton_dc<1024>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out, out_off,
+                  status, iters, lay);
+  if (tps == 512)  // the same with twice the registers per thread (256 VGPRs)
+    return launch(k_newton_dc<512>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out, out_off,
                   status, iters, lay);
   KG_TPS_SWITCH(tps, launch(k_newton_dc<T>, g, b, lds, st, B, kind_slot, na, sweep_e, start, step, npts, out,
                             out_off, status, iters, lay))

@@ -74,10 +74,24 @@ constexpr double KG_GUARD_MIXED = 1e-10;                  // a tenth of the 1e-9
 constexpr double KG_GUARD_MIXED_ETA = 1.4210854715202004e-14;  // 2^-46, normwise, on the returned solution
 // guard_bound_coeff(m) = 3.01 sqrt(2) g_m (the sqrt(2): rho is measured in
 // |re| + |im| norms, the bound holds in moduli)
-inline double guard_bound_coeff(int m) {
+#ifdef __HIP__
+#define KG_HOST_DEVICE __host__ __device__
+#else  // the host-only sanitizer build of the symbolic pass (g++)
+#define KG_HOST_DEVICE
+#endif
+KG_HOST_DEVICE inline double guard_bound_coeff(int m) {
   const double u = 0x1p-53, g2 = 2.0 * u / (1.0 - 2.0 * u), q = m * 1.4142135623730951 * g2;
   return 3.01 * 1.4142135623730951 * q / (1.0 - q);
 }
+// Static-pivot guard of the FP64 OP, DC sweep and transient factorisations
+// (level-scheduled and lane kernels): the same a-priori bound,
+// guard_bound_coeff(m) * rho, formed from each returned solution's factors
+// (every Newton iterate of a nonlinear circuit that ends a point or a step;
+// once per instance for a linear transient's single factorisation). There
+// is no fixup: a bound above KG_GUARD_STATIC reports KG_DECOMPOSITION (krets'
+// partial pivoting would have re-pivoted), and a transient's FP64 err_est is
+// the largest bound of the instance.
+constexpr double KG_GUARD_STATIC = 9.313225746154785e-10;  // 2^-30
 constexpr double KG_FIX_FAIL = 9.094947017729282e-13;     // 2^-40
 constexpr int KG_FIX_NMAX = 90;                           // 90 x 90 complex128 in LDS: 127 KB
 // Fixup queue layout (unsigned 64-bit words): [0] count, [1 + q] queued system

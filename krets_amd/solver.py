@@ -286,9 +286,11 @@ class Context:
             return self._raise_or(r, st), st, err
         return self._result(r, x, (nsteps + 1,), st, extra=(it.reshape(self.n_instances, nsteps + 1), err))
 
-    def transient_lte(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points, config=None, device=False):
+    def transient_lte(self, tstop, h0, hmin, hmax, lte_reltol, lte_abstol, max_points, config=None, device=False,
+                      with_status=False):
         """LTE-controlled backward Euler. Returns (t [ni, max_points], x
-        [ni, max_points, n], n_points, newton_total, rejected). device=True
+        [ni, max_points, n], n_points, newton_total, rejected), and the
+        per-instance statuses after them with with_status=True. device=True
         leaves every output in device memory (torch tensors on the context's
         GPU) instead of copying them to the host."""
         cfg = (config or SolverConfig(compat=False)).to_c()
@@ -307,7 +309,8 @@ class Context:
                                        ctypes.c_void_p(rej.data_ptr()), ctypes.c_void_p(st.data_ptr()),
                                        _lib.MEM_DEVICE)
             self._raise_or(r, st.cpu().numpy())
-            return (t.view(ni, max_points), x.view(ni, max_points, int(self.n_unknowns[0])), npts, nt, rej)
+            out = (t.view(ni, max_points), x.view(ni, max_points, int(self.n_unknowns[0])), npts, nt, rej)
+            return out + (st,) if with_status else out
         t = np.zeros(ni * max_points)
         x = np.zeros(int(self.n_unknowns.sum()) * max_points)
         npts = np.zeros(ni, np.int32)
@@ -319,7 +322,8 @@ class Context:
                                    ctypes.c_void_p(npts.ctypes.data), ctypes.c_void_p(nt.ctypes.data),
                                    ctypes.c_void_p(rej.ctypes.data), ctypes.c_void_p(st.ctypes.data), _lib.MEM_HOST)
         xs, _, _ = self._result(r, x, (max_points,), st)
-        return t.reshape(ni, max_points), xs, npts, nt, rej
+        out = (t.reshape(ni, max_points), xs, npts, nt, rej)
+        return out + (st,) if with_status else out
 
     def adjoint_forward(self, h, nsteps, cvec, method=_lib.TRAP, precision=_lib.FP64, checkpoint_every=64,
                         config=None):

@@ -133,6 +133,35 @@ class Plan:
         return w
 
 
+def guard_bound_coeff(m):
+    """Backward-error bound per unit growth of an LU without pivoting over
+    rows of at most m entries (kg_internal.h guard_bound_coeff)."""
+    u = 2.0 ** -53
+    g2 = 2 * u / (1 - 2 * u)
+    q = m * np.sqrt(2) * g2
+    return 3.01 * np.sqrt(2) * q / (1 - q)
+
+
+def static_pivot_bound(plan, contrib):
+    """The static-pivot guard's bound guard_bound_coeff(m) * rho,
+    rho = || |L||U| ||_inf / ||A||_inf, for the plan's pivot order, from a
+    dense elimination without pivoting of the plan's permuted matrix in numpy
+    (m: the longest row of the factor pattern); entries |re| + |im|."""
+    _, M = plan.gather(contrib)
+    n = M.shape[0]
+    L, U = np.eye(n, dtype=complex), M.copy()
+    for j in range(n):
+        L[j + 1:, j] = U[j + 1:, j] / U[j, j]
+        U[j + 1:, :] -= np.outer(L[j + 1:, j], U[j, :])
+        U[j + 1:, j] = 0.0
+
+    def ab(A):
+        return np.abs(A.real) + np.abs(A.imag)
+
+    m = int(np.max(np.diff(plan.row_ptr)))
+    return guard_bound_coeff(m) * np.max(ab(L) @ ab(U).sum(1)) / np.max(ab(M).sum(1))
+
+
 # ------------------------------------------------------------- compact plan
 LIN_HDR = ["n", "nnz", "nlev", "nblev", "ndyn", "ncon", "nsrc", "words", "o_lvl_ptr", "o_fdesc", "o_blvl_ptr",
            "o_bdesc", "o_col", "o_cpos", "o_rcon_ptr", "o_rcon", "o_dyn", "o_src", "lev0_free", "words_fwd",

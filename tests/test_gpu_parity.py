@@ -120,7 +120,7 @@ def test_op_krets_known_answers(kg):
     assert abs(s["V(out)"] - 0.517) < 1e-3
 
 
-@pytest.mark.parametrize("tps", [0, 1, 64, 256, 1024])  # 0: the default, 1024 threads for this single system
+@pytest.mark.parametrize("tps", [0, 1, 64, 256, 512, 1024])  # 0: the default, 512 threads for this single system
 def test_c1_series_shunt_ladder(kg, oracle, tps):
     from krets_amd import workloads as W
     say(f"C1 ladder tps={tps}")
@@ -323,7 +323,8 @@ def test_lds_resident_transient_uneven_rlc(kg, oracle, method, tps):
     b.load(ctx)
     h, n = 10e-12, 300
     xs, st, (it, err) = ctx.transient(h, n, method=method, config=kg.SolverConfig(threads_per_system=tps))
-    assert (st == 0).all() and (it[:, 1:] == 1).all() and (it[:, 0] == 0).all() and (err == 0).all()
+    assert (st == 0).all() and (it[:, 1:] == 1).all() and (it[:, 0] == 0).all()
+    assert (err > 0).all() and (err < 1e-13).all(), err  # the static-pivot bound of these ladders
     for i in range(5):
         xr, _ = oracle.Circuit(b.netlists[i]).tran(h, n * h, method=method)
         e = max(nerr(xs[i][k], xr[k]) for k in range(n + 1))

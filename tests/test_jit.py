@@ -602,7 +602,7 @@ def tran_source(c, params, h, method):
     return r, buf.value.decode()
 
 
-def tran_host_run(src, params, elements, x0, h, nsteps, method, tol):
+def tran_host_run(src, params, elements, x0, h, nsteps, method, tol, with_err=False):
     d = tempfile.mkdtemp()
     try:
         cpp, so = os.path.join(d, "k.cpp"), os.path.join(d, "k.so")
@@ -622,7 +622,8 @@ def tran_host_run(src, params, elements, x0, h, nsteps, method, tol):
         L.run_host.argtypes = [P] * 9 + [I, D, D, I, I, D, D, D, I]
         L.run_host(*[a.ctypes.data for a in (prm, eb, xop, xoff, out, ooff, st, it, err)], ni, h,
                    2.0 if method == 2 else 1.0, nsteps, method, *tol)
-        return out.reshape(ni, nsteps + 1, n), st, it.reshape(ni, nsteps + 1)
+        res = (out.reshape(ni, nsteps + 1, n), st, it.reshape(ni, nsteps + 1))
+        return res + (err,) if with_err else res
     finally:
         shutil.rmtree(d)
 
@@ -666,6 +667,35 @@ def test_tran_lane_source_linear_on_host_vs_oracle(oracle, method):
         x, st, it = tran_host_run(src, c.params[None], c.elements, xr[:1], h, N, method, (1e-3, 1e-12, 1e-6, 300))
         assert st[0] == 0 and (it[0][1:] == 1).all()
         assert max(nerr(x[0][k], xr[k]) for k in range(1, N + 1)) < 1e-12
+
+
+def test_tran_lane_static_pivot_guard_on_host(oracle):
+    """The generated fixed-step kernel's static-pivot guard on three RLC
+    ladders of one topology whose pivots come from the first (L and C of the
+    others at 1e-2 and 1e-6 of it): the published error figure of the two
+    kept instances equals guard_bound_coeff(m) * || |L||U| ||_inf / ||A||_inf
+    of their static-pivot LU, recomputed by dense elimination in numpy (to
+    1e-9 relative), their states equal the oracle's, and the third (growth
+    above 1e7) reports Decomposition."""
+    import krets_amd as kg
+    import plan_emulator as pe
+    from test_plan import values
+    from test_gpu_static_guard import H, netlists
+    nets = netlists()
+    cs = [kg.Circuit(t) for t in nets]
+    N = 40
+    for method, alpha in ((1, 1.0), (2, 2.0)):
+        r, src = tran_source(cs[0], cs[0].params, H, method)
+        assert r > 0, src
+        x, st, it, err = tran_host_run(src, np.stack([c.params for c in cs]), cs[0].elements, np.zeros((3, cs[0].n)),
+                                       H, N, method, (1e-3, 1e-12, 1e-6, 300), with_err=True)
+        assert list(st) == [0, 0, 2], st
+        plan = pe.Plan(cs[0].elements, cs[0].n, pe.TRAN, pe.MINDEG, cs[0].params, s_rep=alpha / H)
+        for k in (0, 1):
+            ref = pe.static_pivot_bound(plan, values(cs[k], pe.TRAN, h=H, alpha=alpha)[0])
+            assert abs(err[k] / ref - 1.0) < 1e-9, (method, k, err[k], ref)
+            xr, _ = oracle.Circuit(nets[k]).tran(H, N * H, method=method)
+            assert max(nerr(x[k][j], xr[j]) for j in range(1, N + 1)) < 1e-9
 
 
 def test_tran_lane_source_builds_for_gfx950():
