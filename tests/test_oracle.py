@@ -421,7 +421,7 @@ def test_waveform_branches_beyond_krets_tests(oracle):
         assert abs(oracle.wave_value(1, p, s + 6e-6) - 2.5) < 1e-9
         assert oracle.wave_value(1, p, s + 15e-6) == 1.0
     assert oracle.wave_value(1, p, 1e-6) == 1.0  # before the delay
-    # period 0: fmod gives NaN, every comparison fails and the pulse returns v1 (finding of the survey)
+    # period 0: fmod gives NaN, every comparison fails and the pulse returns v1 (voltage_source.rs:68-93)
     assert oracle.wave_value(1, [1.0, 3.0, 0.0, 1e-6, 1e-6, 1e-6, 0.0], 0.5e-6) == 1.0
 
 
