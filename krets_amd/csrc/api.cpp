@@ -450,13 +450,6 @@ hipError_t chunked(Lay lay, int64_t nsys, int64_t chunk, const Launch& L, F&& la
   return hipSuccess;
 }
 
-int32_t copy_out(kg_ctx* c, void* dst, const void* src, size_t bytes, int mem) {
-  if (!dst || dst == src || bytes == 0) return KG_OK;
-  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, mem == KG_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                           c->stream));
-  return KG_OK;
-}
-
 // Writes the status of each structurally failed topology group into its
 // instances' device statuses, in stream order after the call's kernels.
 int32_t mark_group_failures(kg_ctx* c, const Plan& P, int32_t* d_st) {
@@ -1020,8 +1013,18 @@ int32_t run_tran_buckets(kg_ctx* c, const NewtonArgs& na, const TranArgs& ta, bo
   });
 }
 
-int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem, void* xdst = nullptr,
-                      const void* xsrc = nullptr, size_t xbytes = 0);
+// A result handed back to the caller: bytes from the device array src into
+// dst. big: one of the call's large outputs, whose memory `mem` names; the
+// small per-instance ones (iterations, error figures) may be either (the
+// pointer decides).
+struct Out {
+  void* dst;
+  const void* src;
+  size_t bytes;
+  bool big;
+};
+int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem,
+                      std::initializer_list<Out> outs = {});
 int32_t run_adj_op_term(kg_ctx* c, const Plan& P, double* d_xb, double* d_g, int ne, int32_t* d_st);
 
 // Backward pass with the LDS-resident adjoint kernel (FP64), then the
@@ -1093,8 +1096,7 @@ int32_t adjoint_backward_lin(kg_ctx* c, double* grad, int32_t* status, int32_t m
   });
   if (r) return r;
   if ((r = run_adj_op_term(c, P, d_xb, d_g, ne, d_st))) return r;
-  if (grad && (r = copy_out(c, grad, d_g, sizeof(double) * ni * ne, mem))) return r;
-  return finish_status(c, d_st, status, mem);
+  return finish_status(c, d_st, status, mem, {{grad, d_g, sizeof(double) * ni * ne, true}});
 }
 
 // Operating-point part of the adjoint (DC plan, FP64): grad += dJ/dx_0 terms.
@@ -1127,24 +1129,43 @@ bool on_device(const void* p) {
   return a.type == hipMemoryTypeDevice;
 }
 
-// The end of every analysis call: copies the statuses (and one optional small
-// per-instance output, xdst <- xsrc) to the caller, synchronises the stream
-// and returns the first failure. These small outputs may be host or device
-// memory whatever `mem` says of the large ones (the pointer decides). Host
-// copies go through the pinned staging area; every copy enqueued here has
-// completed when it returns.
-int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem, void* xdst,
-                      const void* xsrc, size_t xbytes) {
-  (void)mem;
+// The end of every analysis call: copies the statuses and the call's results
+// to the caller, synchronises the stream and returns the first failure.
+// Device destinations get device copies. Host destinations up to
+// KG_PIN_STAGE bytes in all are written by one copy kernel into the pinned
+// staging area (with a device address; else by DMA copies into it) and
+// copied out after the synchronisation: a copy into pageable memory costs
+// more than the kernel. Larger host results are copied directly. Every copy
+// enqueued here has completed when it returns.
+constexpr size_t KG_PIN_STAGE = 4u << 20;
+int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, int mem,
+                      std::initializer_list<Out> outs) {
   const size_t sb = sizeof(int32_t) * c->n_inst;
-  const size_t xo = (sb + 255) & ~(size_t)255;
-  const bool xcopy = xdst && xsrc && xbytes && xdst != xsrc, xhost = xcopy && !on_device(xdst);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  std::vector<Out> staged;
+  std::vector<size_t> at;
+  size_t need = al(sb);
+  hipError_t e = hipSuccess;
+  for (const Out& o : outs) {
+    if (!o.dst || o.dst == o.src || o.bytes == 0) continue;
+    const bool dev = o.big ? mem == KG_MEM_DEVICE : on_device(o.dst);
+    if (dev) {
+      if (e == hipSuccess) e = hipMemcpyAsync(o.dst, o.src, o.bytes, hipMemcpyDeviceToDevice, c->stream);
+    } else if (need + al(o.bytes) <= KG_PIN_STAGE && staged.size() + 1 < (size_t)KG_COPY_SEGS) {
+      staged.push_back(o);
+      at.push_back(need);
+      need += al(o.bytes);
+    } else if (e == hipSuccess) {
+      e = hipMemcpyAsync(o.dst, o.src, o.bytes, hipMemcpyDeviceToHost, c->stream);
+    }
+  }
   const bool shost = status_out && !on_device(status_out);
-  const size_t need = xo + (xhost ? xbytes : 0);
   if (need > c->pin_cap && c->noalloc) {
     // no staging area to read the statuses through: they stay in the
     // caller's device array, which the kernels wrote directly
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess || es != hipSuccess)
+      return set_err(c, KG_HIP_ERROR, std::string("result copy: ") + hipGetErrorString(e != hipSuccess ? e : es));
     return KG_OK;
   }
   if (need > c->pin_cap) {
@@ -1155,32 +1176,38 @@ int32_t finish_status(kg_ctx* c, const int32_t* d_status, int32_t* status_out, i
     const size_t cap = std::max<size_t>(2 * need, 1 << 16);
     HIPCHK(c, hipHostMalloc((void**)&c->pin, cap, hipHostMallocDefault));
     c->pin_cap = cap;
-    // with a device address the statuses are published by one small kernel
-    // (two DMA copies cost several times its launch)
+    // with a device address the results are published by one small kernel
+    // (DMA copies cost several times its launch)
     if (hipHostGetDevicePointer((void**)&c->pin_dev, c->pin, 0) != hipSuccess) {
       c->pin_dev = nullptr;
       (void)hipGetLastError();
     }
   }
-  hipError_t e = hipSuccess;
-  if (c->pin_dev) {
-    e = run_copy_words((const uint32_t*)d_status, (uint32_t*)c->pin_dev, (int64_t)(sb / 4),
-                       xhost ? (const uint32_t*)xsrc : nullptr, (uint32_t*)(c->pin_dev + xo),
-                       xhost ? (int64_t)(xbytes / 4) : 0, c->stream);
-  } else {
+  if (e == hipSuccess && c->pin_dev) {
+    CopySegs cs{};
+    cs.n = 0;
+    auto seg = [&](const void* src, size_t off, size_t bytes) {
+      cs.src[cs.n] = (const uint32_t*)src;
+      cs.dst[cs.n] = (uint32_t*)(c->pin_dev + off);
+      cs.start[cs.n + 1] = cs.start[cs.n] + (int64_t)(bytes / 4);
+      cs.n++;
+    };
+    seg(d_status, 0, sb);
+    for (size_t k = 0; k < staged.size(); k++) seg(staged[k].src, at[k], staged[k].bytes);
+    e = run_copy_segments(cs, c->stream);
+  } else if (e == hipSuccess) {
     e = hipMemcpyAsync(c->pin, d_status, sb, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && xhost) e = hipMemcpyAsync(c->pin + xo, xsrc, xbytes, hipMemcpyDeviceToHost, c->stream);
+    for (size_t k = 0; k < staged.size() && e == hipSuccess; k++)
+      e = hipMemcpyAsync(c->pin + at[k], staged[k].src, staged[k].bytes, hipMemcpyDeviceToHost, c->stream);
   }
-  if (e == hipSuccess && xcopy && !xhost)
-    e = hipMemcpyAsync(xdst, xsrc, xbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess && status_out && !shost && status_out != d_status)
     e = hipMemcpyAsync(status_out, d_status, sb, hipMemcpyDeviceToDevice, c->stream);
   const hipError_t es = hipStreamSynchronize(c->stream);
   if (e != hipSuccess || es != hipSuccess)
-    return set_err(c, KG_HIP_ERROR, std::string("status copy: ") + hipGetErrorString(e != hipSuccess ? e : es));
+    return set_err(c, KG_HIP_ERROR, std::string("result copy: ") + hipGetErrorString(e != hipSuccess ? e : es));
   std::vector<int32_t> st((const int32_t*)c->pin, (const int32_t*)c->pin + c->n_inst);
   if (shost) std::memcpy(status_out, c->pin, sb);
-  if (xhost) std::memcpy(xdst, c->pin + xo, xbytes);
+  for (size_t k = 0; k < staged.size(); k++) std::memcpy(staged[k].dst, c->pin + at[k], staged[k].bytes);
   int32_t f = first_failure(st);
   if (f != KG_OK && c->err.empty()) c->err = "at least one instance failed";
   return f;
@@ -1340,10 +1367,8 @@ int32_t kg_op(kg_ctx* c, const kg_solver_config* cfg, double* x, int32_t* status
   if (r) return r;
   int64_t tot = 0;
   for (int i = 0; i < c->n_inst; i++) tot += c->nunk[i];
-  r = copy_out(c, x, d_out, sizeof(double) * tot, mem);
-  if (r) return r;
-  if (iters && (r = copy_out(c, iters, d_it, sizeof(int32_t) * c->n_inst, mem))) return r;
-  return finish_status(c, d_st, status, mem);
+  return finish_status(c, d_st, status, mem,
+                       {{x, d_out, sizeof(double) * tot, true}, {iters, d_it, sizeof(int32_t) * c->n_inst, false}});
 }
 
 int32_t kg_dc_sweep(kg_ctx* c, const kg_solver_config* cfg, int32_t element, double start, double stop, double step,
@@ -1372,9 +1397,9 @@ int32_t kg_dc_sweep(kg_ctx* c, const kg_solver_config* cfg, int32_t element, dou
   if (r) return r;
   int64_t tot = 0;
   for (int i = 0; i < c->n_inst; i++) tot += (int64_t)c->nunk[i] * npts;
-  if ((r = copy_out(c, x, d_out, sizeof(double) * tot, mem))) return r;
-  if (iters && (r = copy_out(c, iters, d_it, sizeof(int32_t) * c->n_inst * (size_t)npts, mem))) return r;
-  return finish_status(c, d_st, status, mem);
+  return finish_status(c, d_st, status, mem,
+                       {{x, d_out, sizeof(double) * tot, true},
+                        {iters, d_it, sizeof(int32_t) * c->n_inst * (size_t)npts, false}});
 }
 
 int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs, int32_t nf, int32_t precision,
@@ -1602,9 +1627,9 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
       }
     }
   }
-  if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
   if ((r = mark_group_failures(c, P, d_st))) return r;
-  return finish_status(c, d_st, status, mem, err_est, d_err, sizeof(double) * c->n_inst);
+  return finish_status(c, d_st, status, mem,
+                       {{x, d_out, sizeof(double) * off.back(), true}, {err_est, d_err, sizeof(double) * c->n_inst, false}});
 }
 
 // Allocation-free AC sweep: kg_ac_sweep with every intermediate in the
@@ -1837,10 +1862,11 @@ int32_t kg_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t n
   if (r) return r;
   }
   }
-  if (d_out != x && (r = copy_out(c, x, d_out, sizeof(double) * off.back(), mem))) return r;
-  if (iters && (r = copy_out(c, iters, d_iters, sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), mem))) return r;
   if ((r = mark_group_failures(c, P, d_st))) return r;
-  return finish_status(c, d_st, status, mem, err_est, d_err, sizeof(double) * c->n_inst);
+  return finish_status(c, d_st, status, mem,
+                       {{x, d_out, sizeof(double) * off.back(), true},
+                        {iters, d_iters, sizeof(int32_t) * (size_t)c->n_inst * (nsteps + 1), false},
+                        {err_est, d_err, sizeof(double) * c->n_inst, false}});
 }
 
 int32_t kg_last_autotune(const kg_ctx* c, int32_t* tps, float* ms, int32_t cap) {
@@ -2118,12 +2144,10 @@ int32_t kg_transient_lte(kg_ctx* c, const kg_solver_config* cfg, double tstop, d
   if ((r = setup(cand, A))) return r;
   r = timed(c, A.lane ? jit_lte_name() : "k_tran_lte", [&] { return run(A); });
   if (r) return r;
-  if ((r = copy_out(c, t, d_t, sizeof(double) * ni * max_points, mem))) return r;
-  if ((r = copy_out(c, x, d_x, sizeof(double) * off.back(), mem))) return r;
-  if (n_points && (r = copy_out(c, n_points, d_np, sizeof(int32_t) * ni, mem))) return r;
-  if (newton_total && (r = copy_out(c, newton_total, d_nt, sizeof(int32_t) * ni, mem))) return r;
-  if (rejected && (r = copy_out(c, rejected, d_rj, sizeof(int32_t) * ni, mem))) return r;
-  return finish_status(c, d_st, status, mem);
+  return finish_status(c, d_st, status, mem,
+                       {{t, d_t, sizeof(double) * ni * max_points, true}, {x, d_x, sizeof(double) * off.back(), true},
+                        {n_points, d_np, sizeof(int32_t) * ni, true}, {newton_total, d_nt, sizeof(int32_t) * ni, true},
+                        {rejected, d_rj, sizeof(int32_t) * ni, true}});
 }
 
 int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,
@@ -2220,8 +2244,7 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
       return set_err(c, KG_INVALID_ARGUMENT, "adjoint: the LDS-resident mapping needs FP64 and a fitting system");
     r = timed(c, "k_adj_lin_fwd", [&] { return run_lin_fwd(tps); });
     if (r) return r;
-    if (J && (r = copy_out(c, J, d_J, sizeof(double) * ni, mem))) return r;
-    r = finish_status(c, d_st, status, mem);
+    r = finish_status(c, d_st, status, mem, {{J, d_J, sizeof(double) * ni, true}});
     c->adj.ready = true;
     c->adj.aa = aa;
     c->adj.tps = tps;
@@ -2245,8 +2268,7 @@ int32_t kg_adjoint_forward(kg_ctx* c, const kg_solver_config* cfg, double h, int
                        (int64_t)ncp * cpw, cpw, d_J, d_st, lay, pi);
   });
   if (r) return r;
-  if (J && (r = copy_out(c, J, d_J, sizeof(double) * ni, mem))) return r;
-  r = finish_status(c, d_st, status, mem);
+  r = finish_status(c, d_st, status, mem, {{J, d_J, sizeof(double) * ni, true}});
   c->adj.ready = true;
   c->adj.aa = aa;
   c->adj.tps = tps;
@@ -2306,8 +2328,7 @@ int32_t kg_adjoint_backward(kg_ctx* c, double* grad, int32_t* status, int32_t me
     return run_adj_op(tpd, LD.grid, LD.block, LD.lds, c->stream, BD, 0, d_xb, d_xop, d_xoff, d_g, ne, d_st, ld);
   });
   if (r) return r;
-  if (grad && (r = copy_out(c, grad, d_g, sizeof(double) * ni * ne, mem))) return r;
-  return finish_status(c, d_st, status, mem);
+  return finish_status(c, d_st, status, mem, {{grad, d_g, sizeof(double) * ni * ne, true}});
 }
 
 int32_t kg_adjoint_transient(kg_ctx* c, const kg_solver_config* cfg, double h, int32_t nsteps, int32_t method,

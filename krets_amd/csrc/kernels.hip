@@ -3152,16 +3152,17 @@ __global__ void k_fill_f64(double* p, double v, int64_t n) {
 }
 // Copies two word ranges (a -> da, b -> db) in one launch; used to publish a
 // call's statuses and per-instance estimates into pinned host memory.
-__global__ void k_copy_words(const uint32_t* a, uint32_t* da, int64_t na, const uint32_t* b, uint32_t* db,
-                             int64_t nb) {
+// Copies up to KG_COPY_SEGS word arrays in one launch (the results of a call
+// into the pinned staging area).
+__global__ void k_copy_segments(CopySegs c) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < na) da[i] = a[i];
-  else if (i < na + nb) db[i - na] = b[i - na];
+  for (int k = 0; k < c.n; k++)
+    if (i >= c.start[k] && i < c.start[k + 1]) c.dst[k][i - c.start[k]] = c.src[k][i - c.start[k]];
 }
-hipError_t run_copy_words(const uint32_t* a, uint32_t* da, int64_t na, const uint32_t* b, uint32_t* db, int64_t nb,
-                          hipStream_t st) {
-  if (na + nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_copy_words, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, st, a, da, na, b, db, nb);
+hipError_t run_copy_segments(const CopySegs& c, hipStream_t st) {
+  const int64_t n = c.start[c.n];
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_segments, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c);
   return hipGetLastError();
 }
 

@@ -135,8 +135,16 @@ hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,
                    Lay lay, int plan_ints = 0);
 hipError_t run_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
-hipError_t run_copy_words(const uint32_t* a, uint32_t* da, int64_t na, const uint32_t* b, uint32_t* db, int64_t nb,
-                          hipStream_t st);
+// Word-array copies of one launch: segment k copies start[k + 1] - start[k]
+// words from src[k] to dst[k].
+constexpr int KG_COPY_SEGS = 8;
+struct CopySegs {
+  const uint32_t* src[KG_COPY_SEGS];
+  uint32_t* dst[KG_COPY_SEGS];
+  int64_t start[KG_COPY_SEGS + 1];
+  int n;
+};
+hipError_t run_copy_segments(const CopySegs& c, hipStream_t st);
 hipError_t run_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
