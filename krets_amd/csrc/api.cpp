@@ -158,6 +158,16 @@ int32_t set_err(kg_ctx* c, int32_t code, const std::string& msg) {
       return set_err(ctx, KG_HIP_ERROR, std::string(#x ": ") + hipGetErrorString(e_));           \
   } while (0)
 
+// Destroys the captured call graphs (they hold device addresses of the
+// batch and of workspace buffers).
+void drop_graphs(kg_ctx* c) {
+  for (auto& kv : c->gexec) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
+  c->gexec.clear();
+}
+
 void* ws_get(kg_ctx* c, const std::string& name, size_t bytes, hipError_t* e) {
   if (c->noalloc) {
     auto it = c->given.find(name);
@@ -172,7 +182,12 @@ void* ws_get(kg_ctx* c, const std::string& name, size_t bytes, hipError_t* e) {
   *e = hipSuccess;
   if (b.cap < bytes) {
     c->uploaded.erase(name);  // a new allocation holds nothing uploaded
-    if (b.p) hipFree(b.p);
+    // a captured graph may hold the old address (not every buffer is part of
+    // a graph's key): replays must not outlive it
+    if (b.p) {
+      drop_graphs(c);
+      hipFree(b.p);  // waits for the device
+    }
     b.p = nullptr;
     b.cap = 0;
     *e = hipMalloc(&b.p, std::max<size_t>(bytes, 256));
@@ -201,11 +216,7 @@ void free_batch(kg_ctx* c) {
     if (kv.second) hipFree(kv.second);
   c->inst_pat.clear();
   c->lane_fn.clear();
-  for (auto& kv : c->gexec) {
-    hipGraphExecDestroy(kv.second.second);
-    hipGraphDestroy(kv.second.first);
-  }
-  c->gexec.clear();
+  drop_graphs(c);
   c->n_inst = 0;
 }
 
