@@ -1598,8 +1598,8 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   if ((r = setup(cand, A))) return r;
   P = A.P;
   int ni = c->n_inst;
-  steps.push_back({nullptr, [&] { return run_fill_f64(d_err, 0.0, ni, c->stream); }});
-  steps.push_back({nullptr, [&] { return run_fill_f64((double*)d_fix, 0.0, 1, c->stream); }});
+  // error figures and the guard's queue count, zeroed in one launch
+  steps.push_back({nullptr, [&] { return run_zero2_f64(d_err, ni, (double*)d_fix, 1, c->stream); }});
   steps.push_back({A.lane ? jit_ac_name(mixed, cand % 10 == 2)
                    : A.persistent ? (mixed ? "k_ac_p<complex64,complex128>" : "k_ac_p<complex128,complex128>")
                                   : (mixed ? "k_ac<complex64,complex128>" : "k_ac<complex128,complex128>"),

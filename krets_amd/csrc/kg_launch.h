@@ -146,6 +146,7 @@ struct CopySegs {
 };
 hipError_t run_copy_segments(const CopySegs& c, hipStream_t st);
 hipError_t run_fill_f64(double* p, double v, int64_t n, hipStream_t st);
+hipError_t run_zero2_f64(double* a, int64_t na, double* b, int64_t nb, hipStream_t st);
 hipError_t run_adj_fwd(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot,
                        AdjArgs aa, const double* cvec, const double* xop, const int64_t* xop_off, double* ckpt,
                        int64_t ck_stride, int cpw, double* J, int32_t* status, Lay lay, int plan_ints = 0);
