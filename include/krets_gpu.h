@@ -179,7 +179,7 @@ int32_t kg_dc_sweep(kg_ctx* ctx, const kg_solver_config* cfg, int32_t element, d
  * Pivots are fixed per topology; every solve checks its componentwise
  * backward error in FP64 (mixed: its refinement estimate), and solves that
  * fail are solved again with partial pivoting at their own frequency, as
- * krets' LU does (systems of up to 90 unknowns; larger ones, and systems
+ * krets' LU does (systems of up to 1024 unknowns; larger ones, and systems
  * singular under partial pivoting, report KG_DECOMPOSITION). */
 int32_t kg_ac_sweep(kg_ctx* ctx, const kg_solver_config* cfg, const double* freqs, int32_t nf,
                     int32_t precision, double* x, int32_t* status, double* err_est, int32_t mem);

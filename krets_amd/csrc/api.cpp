@@ -1567,12 +1567,27 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
                     d_st, d_err, refine, ly, d_fix, fixcap);
     });
   };
-  // the guard's fixup over whatever the sweep kernel queued (persistent grid)
-  auto fixup = [&](const AcLaunch& A) -> hipError_t {
+  // the guard's fixup over whatever the sweep kernel queued (persistent grid):
+  // systems of up to nmax unknowns in LDS, larger ones (up to KG_FIX_GMAX) in
+  // a second launch with a global workspace region per block (fix_gws, set
+  // up with the final mapping)
+  char* fix_gws = nullptr;
+  int64_t fix_gstride = 0;
+  int fix_gmax = 0;
+  auto fix_nmax = [&](const AcLaunch& A) {
     int nmax = std::min(A.P.n_max, KG_FIX_NMAX);
     while (nmax > 1 && ac_fix_lds(nmax, A.P.ne_max) > LDS_MAX) nmax--;
-    return run_ac_fix(256, c->stream, batch_dev(c, A.P), 0, d_f, nf, d_xop, d_xoff, d_out, d_off, d_st, d_err,
-                      mixed ? 1 : 0, d_fix, fixcap, nmax, A.P.ne_max);
+    return nmax;
+  };
+  auto fixup = [&](const AcLaunch& A) -> hipError_t {
+    const int nmax = fix_nmax(A);
+    const bool big = fix_gws && A.P.n_max > nmax;
+    const BatchDev B = batch_dev(c, A.P);
+    hipError_t he = run_ac_fix(256, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out, d_off, d_st, d_err,
+                               mixed ? 1 : 0, d_fix, fixcap, nmax, A.P.ne_max, 0, big ? 1 : 0, nullptr, 0);
+    if (he != hipSuccess || !big) return he;
+    return run_ac_fix(KG_FIX_GBLOCKS, c->stream, B, 0, d_f, nf, d_xop, d_xoff, d_out, d_off, d_st, d_err,
+                      mixed ? 1 : 0, d_fix, fixcap, fix_gmax, A.P.ne_max, nmax, 0, fix_gws, fix_gstride);
   };
   int cand = cfg->threads_per_system;
   if (cand == 0 && c->noalloc && !c->tune.count(tkey))
@@ -1597,6 +1612,13 @@ int32_t kg_ac_sweep(kg_ctx* c, const kg_solver_config* cfg, const double* freqs,
   AcLaunch A;
   if ((r = setup(cand, A))) return r;
   P = A.P;
+  if (A.P.n_max > fix_nmax(A) && !c->noalloc) {  // queued systems above the LDS fixup's size
+    fix_gmax = std::min(A.P.n_max, KG_FIX_GMAX);
+    fix_gstride = (int64_t)lay_al(ac_fix_lds(fix_gmax, A.P.ne_max));
+    hipError_t eg;
+    fix_gws = (char*)ws_get(c, "fix_gws", (size_t)fix_gstride * KG_FIX_GBLOCKS, &eg);
+    if (eg != hipSuccess) return set_err(c, KG_HIP_ERROR, "workspace allocation failed (fixup regions)");
+  }
   int ni = c->n_inst;
   // error figures and the guard's queue count, zeroed in one launch
   steps.push_back({nullptr, [&] { return run_zero2_f64(d_err, ni, (double*)d_fix, 1, c->stream); }});

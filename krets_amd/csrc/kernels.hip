@@ -2537,10 +2537,17 @@ __global__ void __launch_bounds__(KG_FIX_NT) k_ac_fix(BatchDev B, int kind_slot,
                                                       const double* xop, const int64_t* xop_off, double* out,
                                                       const int64_t* out_off, int32_t* status, double* err,
                                                       int err_mode, const unsigned long long* fixq, int64_t fixcap,
-                                                      int nmax, int ne_max) {
+                                                      int nmax, int ne_max, int nskip, int defer, char* gws,
+                                                      int64_t gstride) {
+  // Re-solves the queued systems of nskip < n <= nmax unknowns. The first
+  // launch works in LDS (nskip = 0); with defer, larger systems are left to a
+  // second launch that works from a global workspace region per block
+  // (gws + block * gstride, the same carve-up) with nskip the first one's
+  // nmax; a system beyond every launch's nmax reports KG_DECOMPOSITION.
   extern __shared__ __align__(16) char smem[];
   const int t = threadIdx.x;
-  c128* LU = (c128*)smem;                      // [nmax][nmax]
+  char* base = gws ? gws + (int64_t)blockIdx.x * gstride : smem;
+  c128* LU = (c128*)base;                      // [nmax][nmax]
   c128* bb = LU + (size_t)nmax * nmax;         // right-hand side (plan row order)
   c128* x = bb + nmax;                         // solution (plan column order)
   c128* w = x + nmax;                          // work vector
@@ -2554,8 +2561,9 @@ __global__ void __launch_bounds__(KG_FIX_NT) k_ac_fix(BatchDev B, int kind_slot,
     const int inst = (int)(sys / nf), fi = (int)(sys % nf);
     const PatDev& P = B.pats[B.inst_pat[kind_slot * B.n_instances + inst]];
     const int n = P.n;
+    if (n <= nskip) continue;  // re-solved by the earlier launch
     if (n > nmax || P.ne > ne_max) {  // block-uniform
-      if (t == 0) atomicMax(&status[inst], KG_DECOMPOSITION);
+      if (!defer && t == 0) atomicMax(&status[inst], KG_DECOMPOSITION);
       continue;
     }
     const double* prm = B.params + B.eoff[inst] * KG_NPARAM;
@@ -3123,9 +3131,10 @@ hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st
 hipError_t run_ac_fix(int grid, hipStream_t st, BatchDev B, int kind_slot, const double* freqs, int nf,
                       const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
                       int32_t* status, double* err, int err_mode, const unsigned long long* fixq, int64_t fixcap,
-                      int nmax, int ne_max) {
-  return launch(k_ac_fix, dim3(grid), dim3(KG_FIX_NT), ac_fix_lds(nmax, ne_max), st, B, kind_slot, freqs, nf, xop,
-                xop_off, out, out_off, status, err, err_mode, fixq, fixcap, nmax, ne_max);
+                      int nmax, int ne_max, int nskip, int defer, char* gws, int64_t gstride) {
+  return launch(k_ac_fix, dim3(grid), dim3(KG_FIX_NT), gws ? 0 : ac_fix_lds(nmax, ne_max), st, B, kind_slot, freqs,
+                nf, xop, xop_off, out, out_off, status, err, err_mode, fixq, fixcap, nmax, ne_max, nskip, defer, gws,
+                gstride);
 }
 
 hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,

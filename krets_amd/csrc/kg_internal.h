@@ -67,7 +67,9 @@ enum : int32_t {
 // dense LU with row partial pivoting in FP64 (systems of at most KG_FIX_NMAX
 // unknowns), then up to two FP64 refinement steps, after which its normwise
 // backward error eta is measured; eta above KG_FIX_FAIL reports
-// KG_DECOMPOSITION, and eta is the figure of a fixed solve.
+// KG_DECOMPOSITION, and eta is the figure of a fixed solve. Larger queued
+// systems (up to KG_FIX_GMAX unknowns) take the same dense LU in a global
+// workspace region per block, in a second launch.
 constexpr double KG_GUARD_GROWTH = 4.0;
 constexpr double KG_GUARD_OMEGA = 1.4210854715202004e-14;  // 2^-46: 64 units in the last place
 constexpr double KG_GUARD_MIXED = 1e-10;                  // a tenth of the 1e-9 the mixed path promises
@@ -94,6 +96,8 @@ KG_HOST_DEVICE inline double guard_bound_coeff(int m) {
 constexpr double KG_GUARD_STATIC = 9.313225746154785e-10;  // 2^-30
 constexpr double KG_FIX_FAIL = 9.094947017729282e-13;     // 2^-40
 constexpr int KG_FIX_NMAX = 90;                           // 90 x 90 complex128 in LDS: 127 KB
+constexpr int KG_FIX_GMAX = 1024;  // larger queued systems: a dense LU in a global region per block (16 MB)
+constexpr int KG_FIX_GBLOCKS = 16;  // blocks (regions) of that second launch
 // Fixup queue layout (unsigned 64-bit words): [0] count, [1 + q] queued system
 // (instance * number of frequencies + frequency index), q < capacity.
 

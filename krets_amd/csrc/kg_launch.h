@@ -125,11 +125,14 @@ hipError_t run_ac(int tps, int mixed, dim3 g, dim3 b, size_t lds, hipStream_t st
                   unsigned long long* fixq, int64_t fixcap);
 // Re-solves the systems in the guard's queue (k_ac_fix), persistent over the
 // queue with `grid` blocks; err_mode 0: backward error, 1: correction size.
+// Systems of nskip < n <= nmax unknowns, in LDS (gws null) or in a global
+// workspace region per block (gws + block * gstride, ac_fix_lds bytes each);
+// defer: larger systems are left to a later launch instead of failing.
 size_t ac_fix_lds(int nmax, int ne_max);
 hipError_t run_ac_fix(int grid, hipStream_t st, BatchDev B, int kind_slot, const double* freqs, int nf,
                       const double* xop, const int64_t* xop_off, double* out, const int64_t* out_off,
                       int32_t* status, double* err, int err_mode, const unsigned long long* fixq, int64_t fixcap,
-                      int nmax, int ne_max);
+                      int nmax, int ne_max, int nskip, int defer, char* gws, int64_t gstride);
 hipError_t run_lte(int tps, dim3 g, dim3 b, size_t lds, hipStream_t st, BatchDev B, int kind_slot, NewtonArgs na,
                    LteArgs la, const double* xop, const int64_t* xop_off, double* tout, double* xout,
                    const int64_t* xout_off, int32_t* npts, int32_t* newton_total, int32_t* rejected, int32_t* status,

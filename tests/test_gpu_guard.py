@@ -33,16 +33,16 @@ def say(msg):
     sys.stdout.flush()
 
 
-def wide_rlc_batch(ni=16, seed=3):
+def wide_rlc_batch(ni=16, seed=3, sections=4):
     rng = np.random.default_rng(seed)
     nets = []
     for i in range(ni):
-        R = rng.uniform(10, 100, 4)
-        L = rng.uniform(1, 10, 4) * 1e-6
-        C = rng.uniform(0.1, 1, 4) * 1e-9
+        R = rng.uniform(10, 100, sections)
+        L = rng.uniform(1, 10, sections) * 1e-6
+        C = rng.uniform(0.1, 1, sections) * 1e-9
         L[1] *= 10.0 ** rng.uniform(-2, 2)  # a Monte Carlo spread of 1e4 in one element
         s = "V1 n0 0 AC 1\n"
-        for k in range(1, 5):
+        for k in range(1, sections + 1):
             s += f"R{k} n{k-1} m{k} {float(R[k-1])!r}\nL{k} m{k} n{k} {float(L[k-1])!r}\nC{k} n{k} 0 {float(C[k-1])!r}\n"
         nets.append(s)
     return nets
@@ -131,3 +131,39 @@ def test_guard_singular_system_reports_decomposition(setup):
         assert st[0] == 2, (tps, st)
     with pytest.raises(kg.KretsGpuError):
         ctx.ac_sweep(np.array([1e3, 1e6]), config=kg.SolverConfig(threads_per_system=2001))
+
+
+def test_fixup_of_systems_larger_than_its_lds_form(oracle):
+    """The 40-section form of the batch (122 unknowns): the static order of
+    the sweep's centre grows by 1e34 and more at nearly every frequency (its
+    |L||U| growth recomputed in numpy below), so nearly every solve is queued,
+    and at 122 unknowns none fits the fixup's LDS form (90). The second fixup
+    launch re-solves them by partial pivoting from global memory: every status
+    OK, every returned solve backward stable (normwise eta <= 2.5e-14 against
+    the oracle's assembled system) and within the condition-number bound
+    max(1e-12, 1e-14 cond) of the oracle's solution."""
+    import krets_amd as kg
+    import plan_emulator as pe
+    from test_jit import static_growth
+    say("guard fixup from global memory, 122 unknowns")
+    nets = wide_rlc_batch(4, seed=5, sections=40)
+    cs = [kg.Circuit(t) for t in nets]
+    assert cs[0].n == 122
+    plan = pe.Plan(cs[0].elements, cs[0].n, pe.AC, pe.NESTED, cs[0].params,
+                   s_rep=2 * np.pi * np.sqrt(FREQS[0] * FREQS[-1]))
+    grown = sum(static_growth(plan, c, 2 * np.pi * f) > 4 for c in cs for f in FREQS)
+    assert grown > len(cs) * len(FREQS) // 2, grown
+    ctx = kg.Context()
+    ctx.load(cs[0].elements, np.stack([c.params for c in cs]), cs[0].n)
+    for tps in (64, 256):
+        xs, st, err = ctx.ac_sweep(FREQS, precision=0, config=kg.SolverConfig(threads_per_system=tps))
+        assert (st == 0).all(), (tps, st)
+        for i, t in enumerate(nets):
+            oc = oracle.Circuit(t)
+            ref = oc.ac(FREQS)
+            for k, f in enumerate(FREQS):
+                A, b = oc.assemble_ac(np.zeros(oc.n), f)
+                assert eta(A, b, xs[i][k]) <= 2.5e-14, (tps, i, k)
+                kap = np.linalg.norm(A, np.inf) * np.linalg.norm(np.linalg.inv(A), np.inf)
+                fwd = np.linalg.norm(xs[i][k] - ref[k]) / np.linalg.norm(ref[k])
+                assert fwd <= max(1e-12, 1e-14 * kap), (tps, i, k, fwd, kap)
