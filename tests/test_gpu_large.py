@@ -72,6 +72,30 @@ def test_dc_sweep_larger_than_lds(kg, oracle):
         assert nerr(np.array([r[k] for k in c.names]), xr[j]) < 1e-9
 
 
+def test_op_batch_in_chunks(kg):
+    """2100 instances of a 3500-resistor chain (3501 unknowns, over the LDS
+    limit), each with its own source voltage: two launches of the global
+    workspace regions (2048 systems, then 52); every instance against the
+    closed form V(k) = V1 (1 - (k - 1) / 3500)."""
+    from krets_amd import workloads as W
+    say("OP batch larger than LDS in two chunks")
+    c = kg.Circuit(W.series_ladder(3500))
+    ni = 2100
+    p = np.repeat(c.params[None], ni, axis=0)
+    v1 = 0.5 + np.arange(ni) / ni
+    p[:, 0, 0] = v1  # V1's dc value
+    ctx = kg.Context()
+    ctx.load(c.elements, p, c.n)
+    with pytest.raises(kg.KretsGpuError, match="too large"):
+        ctx.op(kg.SolverConfig(threads_per_system=16))
+    xs, st, _ = ctx.op()
+    assert (st == 0).all()
+    ks = np.array([c.index_map[f"V({k})"] for k in range(1, 3501)])
+    shape = 1 - np.arange(3500) / 3500.0
+    err = max(np.max(np.abs(xs[i][ks] - v1[i] * shape)) for i in range(ni))
+    assert err < 1e-9, err
+
+
 @pytest.mark.parametrize("precision,tps", [(0, 0), (0, 64), (0, 256), (1, 256)])
 def test_ac_larger_than_lds(kg, oracle, precision, tps):
     """A 2500-section RC low-pass (2502 unknowns, about 200 KB per complex
