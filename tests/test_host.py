@@ -395,3 +395,13 @@ def test_header_is_c99_and_host_entry_points_work_from_c(kg, tmp_path):
     assert next(l for l in lines if l.startswith("config ")) == "config 0.001 9.9999999999999998e-13 9.9999999999999995e-07 300 1"
     spec = next(l for l in lines if l.startswith("spec ")).split()
     assert spec[1] == "0" and spec[2] == "2" and spec[3].endswith(".cir")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N above 1 exits with status 2 and says why,
+    before it touches torch or a GPU."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
